@@ -1,0 +1,102 @@
+/*
+ * hs_internal.h -- device-side descriptors of the per-timestep engine and the
+ * launchers shared by the translation units of libhs.
+ *
+ * HBM layout of one network (all arrays structure-of-arrays, 256-B aligned):
+ *   group  : var[v][n] (R), last[n] (int32 step of the last spike),
+ *            acc[2][v][n] (R, fused form only, double-buffered by step parity),
+ *            hist[Q][n] (int32 spike lists, one per step of a ring of Q >= dmax+2
+ *            steps) with hist_cnt[Q]
+ *   pathway: bstart[n_src*D+1] (int32 offsets of the (pre, delay) bundles),
+ *            post[S] (int32 target index), w[S] or w[n_src] (R) by weight mode
+ *   control: HsCtl (step base, event counter, reset guard)
+ */
+#ifndef HS_INTERNAL_H
+#define HS_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/hs_models.h"
+
+#define HS_MAX_STATE_MONS 4
+#define HS_MAX_DELAY_SLOTS 4096 /* D per pathway (LDS prefix table) */
+
+struct HsCtl {
+    long long base;            /* simulation step of chunk offset 0 */
+    long long no_reset_before; /* resets of spikes at steps < this were already applied */
+    unsigned long long events; /* synaptic events delivered */
+    unsigned long long pad;
+};
+
+template <typename R>
+struct DGroup {
+    int model, n, nvars, Q;
+    int has_reset, ref_steps;
+    unsigned int stream;
+    int nst;
+    R *var[HS_MAX_VARS];
+    R *acc[2][HS_MAX_VARS];
+    int32_t *last;
+    int32_t *hist;
+    int32_t *hist_cnt;
+    R k[16];
+    R vth, vr;
+    unsigned long long seed;
+    /* spike monitor */
+    int32_t *smon_i;
+    int32_t *smon_t;
+    unsigned long long *smon_n;
+    long long smon_cap;
+    /* state monitors (when='start') */
+    int st_var[HS_MAX_STATE_MONS], st_start[HS_MAX_STATE_MONS], st_stop[HS_MAX_STATE_MONS];
+    long long st_t0[HS_MAX_STATE_MONS], st_rows[HS_MAX_STATE_MONS];
+    R *st_buf[HS_MAX_STATE_MONS];
+    int tile_begin; /* first tile index of this group in the concatenated tile space */
+};
+
+template <typename R>
+struct DPath {
+    int src, tgt, var;
+    int n_src, dmin, D, Q, wmode;
+    const int32_t *hist;     /* source group's spike ring */
+    const int32_t *hist_cnt;
+    const int32_t *bstart;
+    const int32_t *post;
+    const R *w;
+    R wscalar;
+    R *target;               /* plain: the state variable */
+    R *acc[2];               /* fused: the target's accumulators */
+};
+
+/* launchers (hs_kernels.hip) -- all on the runtime stream, return hipError_t */
+hipError_t hs_launch_threshold(int prec, const void *v, int n, double vth, int32_t *last,
+                               long long step, int ref_steps, int sorted, int32_t *spikes,
+                               int32_t *count, int32_t *scratch, hipStream_t s);
+hipError_t hs_launch_randn(unsigned long long seed, unsigned int stream, long long step, int n,
+                           int prec, void *out, int uniform, hipStream_t s);
+hipError_t hs_launch_stateupdate(int model, int prec, int n, void *const *vars, const int32_t *last,
+                                 const double *k, int nk, unsigned long long seed, unsigned int stream,
+                                 int ref_steps, long long step, hipStream_t s);
+hipError_t hs_launch_math(int fn, int n, const double *x, double *y, hipStream_t s);
+
+/* one pathway at step ctl->base (acc != 0: into p.acc[step & 1]) */
+template <typename R>
+hipError_t hs_launch_deliver_one_ctl(const DPath<R> &p, HsCtl *ctl, int acc, unsigned long long *events, int blocks,
+                                     hipStream_t s);
+
+/* engine kernels over device descriptor arrays */
+template <typename R>
+hipError_t hs_launch_update(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int koff,
+                            int fused, int block, int grid, hipStream_t s);
+template <typename R>
+hipError_t hs_launch_deliver(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int fused, int block,
+                             int grid, hipStream_t s);
+template <typename R>
+hipError_t hs_launch_fused(const DGroup<R> *groups, int ng, int total_tiles, const DPath<R> *paths, int np,
+                           HsCtl *ctl, int koff, int block, int grid, hipStream_t s);
+template <typename R>
+hipError_t hs_launch_finalize(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int fused,
+                              int block, hipStream_t s);
+hipError_t hs_launch_tick(HsCtl *ctl, long long k, hipStream_t s);
+
+#endif /* HS_INTERNAL_H */

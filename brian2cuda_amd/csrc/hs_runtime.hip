@@ -1,0 +1,1047 @@
+/*
+ * hs_runtime.hip -- the C ABI of libhs (include/hs.h): device and memory
+ * management, the bundle builder, network assembly and the graph-captured step
+ * loop that replaces the generated objects.cu / network.cu of the reference.
+ */
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/hs.h"
+#include "hs_internal.h"
+#include "hs_model_exprs.h"
+
+/* ------------------------------------------------------------------------ */
+/* errors                                                                    */
+/* ------------------------------------------------------------------------ */
+static thread_local std::string g_err;
+
+static int hs_fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HS_HIP(call)                                                                                      \
+    do {                                                                                                  \
+        hipError_t _e = (call);                                                                           \
+        if (_e != hipSuccess)                                                                             \
+            return hs_fail(HS_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(_e), __FILE__, \
+                           __LINE__);                                                                     \
+    } while (0)
+
+#define HS_TRY(expr)                   \
+    do {                               \
+        int _r = (expr);               \
+        if (_r != HS_OK) return _r;    \
+    } while (0)
+
+extern "C" const char *hs_last_error(void) { return g_err.c_str(); }
+extern "C" int hs_version(void) { return 1; }
+
+/* ------------------------------------------------------------------------ */
+/* runtime                                                                   */
+/* ------------------------------------------------------------------------ */
+struct HsRuntime {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    long long hbm = 0;
+    char name[256] = {0};
+    HsCtl *ctl = nullptr;          /* control block of the standalone kernels */
+    int32_t *scratch = nullptr;    /* scratch of the standalone kernels */
+    size_t scratch_n = 0;
+    double *kbuf = nullptr;        /* constants of hs_stateupdate */
+};
+static HsRuntime g_rt;
+
+static int hs_need_init()
+{
+    if (!g_rt.stream) return hs_fail(HS_ERR_STATE, "hs_init() has not been called");
+    return HS_OK;
+}
+
+extern "C" int hs_init(int device)
+{
+    if (g_rt.stream && g_rt.device == device) return HS_OK;
+    if (g_rt.stream) return hs_fail(HS_ERR_STATE, "already initialised on device %d", g_rt.device);
+    int n = 0;
+    HS_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return hs_fail(HS_ERR_ARG, "device %d not present (%d devices)", device, n);
+    HS_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HS_HIP(hipGetDeviceProperties(&prop, device));
+    g_rt.n_cu = prop.multiProcessorCount;
+    g_rt.hbm = (long long)prop.totalGlobalMem;
+    snprintf(g_rt.name, sizeof(g_rt.name), "%s", prop.gcnArchName);
+    HS_HIP(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
+    HS_HIP(hipMalloc((void **)&g_rt.ctl, sizeof(HsCtl)));
+    HS_HIP(hipMemset(g_rt.ctl, 0, sizeof(HsCtl)));
+    HS_HIP(hipMalloc((void **)&g_rt.kbuf, 16 * sizeof(double)));
+    g_rt.device = device;
+    return HS_OK;
+}
+
+extern "C" int hs_device_info(char *name, int name_cap, int *compute_units, long long *hbm_bytes)
+{
+    HS_TRY(hs_need_init());
+    if (name && name_cap > 0) snprintf(name, name_cap, "%s", g_rt.name);
+    if (compute_units) *compute_units = g_rt.n_cu;
+    if (hbm_bytes) *hbm_bytes = g_rt.hbm;
+    return HS_OK;
+}
+
+extern "C" int hs_alloc(size_t bytes, void **dev_ptr)
+{
+    HS_TRY(hs_need_init());
+    if (!dev_ptr) return hs_fail(HS_ERR_ARG, "null output pointer");
+    if (hipMalloc(dev_ptr, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return hs_fail(HS_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+    }
+    return HS_OK;
+}
+
+extern "C" int hs_free(void *p)
+{
+    HS_TRY(hs_need_init());
+    if (p) HS_HIP(hipFree(p));
+    return HS_OK;
+}
+
+extern "C" int hs_h2d(void *dst, const void *src, size_t bytes)
+{
+    HS_TRY(hs_need_init());
+    if (!bytes) return HS_OK;
+    HS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+extern "C" int hs_d2h(void *dst, const void *src, size_t bytes)
+{
+    HS_TRY(hs_need_init());
+    if (!bytes) return HS_OK;
+    HS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+extern "C" int hs_zero(void *p, size_t bytes)
+{
+    HS_TRY(hs_need_init());
+    if (!bytes) return HS_OK;
+    HS_HIP(hipMemsetAsync(p, 0, bytes, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+extern "C" int hs_sync(void)
+{
+    HS_TRY(hs_need_init());
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+static int hs_scratch(size_t n)
+{
+    if (g_rt.scratch_n >= n) return HS_OK;
+    if (g_rt.scratch) HS_HIP(hipFree(g_rt.scratch));
+    g_rt.scratch = nullptr;
+    g_rt.scratch_n = 0;
+    HS_HIP(hipMalloc((void **)&g_rt.scratch, n * sizeof(int32_t)));
+    g_rt.scratch_n = n;
+    return HS_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* model table                                                               */
+/* ------------------------------------------------------------------------ */
+struct HsModelInfo {
+    int nvars, nparams, has_reset;
+    int i_vth, i_vr, i_ref, i_stream; /* parameter positions (-1: none) */
+};
+
+static bool hs_model_info(int model, HsModelInfo *mi)
+{
+    if (model == HS_MODEL_LIF_CUBA) { *mi = {HS_LIF_CUBA_NVARS, HS_LIF_CUBA_NPARAMS, 1, 1, 2, 6, -1}; return true; }
+    if (model == HS_MODEL_LIF_NOISE) { *mi = {HS_LIF_NOISE_NVARS, HS_LIF_NOISE_NPARAMS, 1, 3, 4, 5, 6}; return true; }
+    if (model == HS_MODEL_HH_COBA) { *mi = {HS_HH_NVARS, HS_HH_NPARAMS, 0, 12, -1, 13, -1}; return true; }
+    return false;
+}
+
+/* ------------------------------------------------------------------------ */
+/* standalone kernels                                                        */
+/* ------------------------------------------------------------------------ */
+extern "C" int hs_threshold_compact(int precision, const void *v, int n, double vth, int32_t *last, int64_t step,
+                                    int ref_steps, int sorted, int32_t *spikes, int32_t *count)
+{
+    HS_TRY(hs_need_init());
+    if (n < 0 || (precision != HS_F64 && precision != HS_F32) || !v || !last || !spikes || !count)
+        return hs_fail(HS_ERR_ARG, "hs_threshold_compact: bad arguments");
+    if (n == 0) return hs_zero(count, sizeof(int32_t));
+    if (sorted) HS_TRY(hs_scratch((size_t)(n + 255) / 256 + 1));
+    HS_HIP(hs_launch_threshold(precision, v, n, vth, last, step, ref_steps, sorted, spikes, count, g_rt.scratch,
+                               g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+extern "C" int hs_randn(uint64_t seed, uint32_t stream, int64_t step, int n, int precision, void *out)
+{
+    HS_TRY(hs_need_init());
+    if (n < 0 || !out || (precision != HS_F64 && precision != HS_F32)) return hs_fail(HS_ERR_ARG, "hs_randn: bad arguments");
+    if (n == 0) return HS_OK;
+    HS_HIP(hs_launch_randn(seed, stream, step, n, precision, out, 0, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+extern "C" int hs_rand(uint64_t seed, uint32_t stream, int64_t step, int n, void *out)
+{
+    HS_TRY(hs_need_init());
+    if (n < 0 || !out) return hs_fail(HS_ERR_ARG, "hs_rand: bad arguments");
+    if (n == 0) return HS_OK;
+    HS_HIP(hs_launch_randn(seed, stream, step, n, HS_F64, out, 1, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+extern "C" int hs_stateupdate(int model, int precision, int n, void *const *vars, const int32_t *last,
+                              const double *params, int nparams, double dt, uint64_t seed, int64_t step)
+{
+    HS_TRY(hs_need_init());
+    HsModelInfo mi;
+    if (!hs_model_info(model, &mi)) return hs_fail(HS_ERR_UNSUPPORTED, "unknown model %d", model);
+    if (nparams != mi.nparams || !params || !vars || !last || n < 0 || (precision != HS_F64 && precision != HS_F32))
+        return hs_fail(HS_ERR_ARG, "hs_stateupdate: bad arguments (model %d expects %d params)", model, mi.nparams);
+    double k[16] = {0};
+    if (hs_model_constants(model, params, dt, k) < 0) return hs_fail(HS_ERR_ARG, "bad model");
+    int ref_steps = (int)hs_timestep(params[mi.i_ref], dt);
+    unsigned int stream = mi.i_stream >= 0 ? (unsigned int)params[mi.i_stream] : 0u;
+    if (precision == HS_F64) {
+        HS_HIP(hipMemcpyAsync(g_rt.kbuf, k, sizeof(k), hipMemcpyHostToDevice, g_rt.stream));
+    } else {
+        float kf[16];
+        for (int j = 0; j < 16; ++j) kf[j] = (float)k[j];
+        HS_HIP(hipMemcpyAsync(g_rt.kbuf, kf, sizeof(kf), hipMemcpyHostToDevice, g_rt.stream));
+    }
+    if (n > 0)
+        HS_HIP(hs_launch_stateupdate(model, precision, n, vars, last, g_rt.kbuf, 16, seed, stream, ref_steps, step,
+                                     g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+extern "C" int hs_math_eval(int fn, int n, const double *x, double *y)
+{
+    HS_TRY(hs_need_init());
+    if (fn < 0 || fn > 4 || n < 0 || !x || !y) return hs_fail(HS_ERR_ARG, "hs_math_eval: bad arguments");
+    if (n == 0) return HS_OK;
+    HS_HIP(hs_launch_math(fn, n, x, y, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+template <typename R>
+static int hs_deliver_t(int n_src, int dmin, int D, int Q, const int32_t *hist, const int32_t *hist_cnt, int64_t step,
+                        const int32_t *bstart, const int32_t *post, int wmode, const void *w, double wscalar,
+                        void *target, uint64_t *events)
+{
+    DPath<R> p;
+    memset(&p, 0, sizeof(p));
+    p.n_src = n_src;
+    p.dmin = dmin;
+    p.D = D;
+    p.Q = Q;
+    p.wmode = wmode;
+    p.hist = hist;
+    p.hist_cnt = hist_cnt;
+    p.bstart = bstart;
+    p.post = post;
+    p.w = (const R *)w;
+    p.wscalar = (R)wscalar;
+    p.target = (R *)target;
+    HsCtl c;
+    memset(&c, 0, sizeof(c));
+    c.base = step;
+    HS_HIP(hipMemcpyAsync(g_rt.ctl, &c, sizeof(c), hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hs_launch_deliver_one_ctl<R>(p, g_rt.ctl, 0, (unsigned long long *)events, g_rt.n_cu * 2, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+extern "C" int hs_deliver(int precision, int n_src, int dmin, int D, int Q, const int32_t *hist,
+                          const int32_t *hist_cnt, int64_t step, const int32_t *bstart, const int32_t *post, int wmode,
+                          const void *w, double wscalar, void *target, uint64_t *events)
+{
+    HS_TRY(hs_need_init());
+    if (n_src <= 0 || dmin < 0 || D <= 0 || D > HS_MAX_DELAY_SLOTS || Q < dmin + D || !hist || !hist_cnt || !bstart ||
+        !post || !target || !events || wmode < 0 || wmode > 2 || (wmode != HS_W_SCALAR && !w))
+        return hs_fail(HS_ERR_ARG, "hs_deliver: bad arguments");
+    if (precision == HS_F64)
+        return hs_deliver_t<double>(n_src, dmin, D, Q, hist, hist_cnt, step, bstart, post, wmode, w, wscalar, target,
+                                    events);
+    if (precision == HS_F32)
+        return hs_deliver_t<float>(n_src, dmin, D, Q, hist, hist_cnt, step, bstart, post, wmode, w, wscalar, target,
+                                   events);
+    return hs_fail(HS_ERR_ARG, "bad precision");
+}
+
+/* ------------------------------------------------------------------------ */
+/* networks                                                                  */
+/* ------------------------------------------------------------------------ */
+struct HGroup {
+    int model, n;
+    HsModelInfo mi;
+    double params[16];
+    double k[16];
+    int ref_steps;
+    unsigned int stream;
+    void *var[HS_MAX_VARS] = {nullptr};
+    void *acc[2][HS_MAX_VARS] = {{nullptr}};
+    int32_t *last = nullptr;
+    int32_t *hist = nullptr;
+    int32_t *hist_cnt = nullptr;
+    int Q = 0;
+    int smon = -1;
+    std::vector<int> stmons;
+};
+
+struct HPath {
+    int src, tgt, var;
+    int64_t nsyn;
+    int dmin, dmax, D;
+    int wmode;
+    double wscalar;
+    int32_t *bstart = nullptr;
+    int32_t *post = nullptr;
+    void *w = nullptr;
+};
+
+struct HMon {
+    int kind; /* 0 spikes, 1 state */
+    int gid, var, start, stop;
+    int64_t cap = 0;                   /* spike records */
+    int32_t *mi = nullptr, *mt = nullptr;
+    unsigned long long *mn = nullptr;
+    void *buf = nullptr;               /* state rows */
+    int64_t rows = 0;                  /* capacity in rows */
+    int64_t t0 = 0;
+};
+
+struct hs_net {
+    int prec, engine;
+    double dt;
+    uint64_t seed;
+    std::vector<HGroup> g;
+    std::vector<HPath> p;
+    std::vector<HMon> m;
+    bool built = false;     /* rings and accumulators exist: no more groups/pathways */
+    bool desc_dirty = true; /* descriptors must be re-uploaded */
+    HsCtl *ctl = nullptr;
+    void *dgroups = nullptr, *dpaths = nullptr;
+    int total_tiles = 0;
+    bool single_launch = false;
+    int64_t step = 0;
+    long long opt[4] = {256, 0, 256, 0};
+    std::map<int64_t, hipGraphExec_t> graphs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_wall = 0, last_gpu_ms = 0;
+    int64_t last_steps = 0;
+    uint64_t events = 0;
+    std::vector<void *> allocs;
+};
+
+static size_t hs_rsize(const hs_net *net) { return net->prec == HS_F64 ? sizeof(double) : sizeof(float); }
+
+static int hs_dalloc(hs_net *net, size_t bytes, void **p, bool zero)
+{
+    if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return hs_fail(HS_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+    }
+    net->allocs.push_back(*p);
+    if (zero) HS_HIP(hipMemsetAsync(*p, 0, bytes ? bytes : 1, g_rt.stream));
+    return HS_OK;
+}
+
+static void hs_dfree(hs_net *net, void *p)
+{
+    if (!p) return;
+    auto it = std::find(net->allocs.begin(), net->allocs.end(), p);
+    if (it != net->allocs.end()) net->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+static void hs_drop_graphs(hs_net *net)
+{
+    for (auto &kv : net->graphs) (void)hipGraphExecDestroy(kv.second);
+    net->graphs.clear();
+}
+
+extern "C" int hs_net_create(int precision, int engine, double dt, uint64_t seed, hs_net **out)
+{
+    HS_TRY(hs_need_init());
+    if (!out || (precision != HS_F64 && precision != HS_F32) ||
+        (engine != HS_ENGINE_PLAIN && engine != HS_ENGINE_FUSED) || !(dt > 0))
+        return hs_fail(HS_ERR_ARG, "hs_net_create: bad arguments");
+    hs_net *net = new hs_net();
+    net->prec = precision;
+    net->engine = engine;
+    net->dt = dt;
+    net->seed = seed;
+    if (hs_dalloc(net, sizeof(HsCtl), (void **)&net->ctl, true) != HS_OK) {
+        delete net;
+        return HS_ERR_NOMEM;
+    }
+    HS_HIP(hipEventCreate(&net->ev0));
+    HS_HIP(hipEventCreate(&net->ev1));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    *out = net;
+    return HS_OK;
+}
+
+extern "C" void hs_net_destroy(hs_net *net)
+{
+    if (!net) return;
+    (void)hipStreamSynchronize(g_rt.stream);
+    hs_drop_graphs(net);
+    for (void *p : net->allocs) (void)hipFree(p);
+    if (net->ev0) (void)hipEventDestroy(net->ev0);
+    if (net->ev1) (void)hipEventDestroy(net->ev1);
+    delete net;
+}
+
+extern "C" int hs_net_add_group(hs_net *net, int model, int n, const double *params, int nparams, int *gid)
+{
+    if (!net) return hs_fail(HS_ERR_ARG, "null net");
+    if (net->built) return hs_fail(HS_ERR_STATE, "groups must be added before the first run");
+    HsModelInfo mi;
+    if (!hs_model_info(model, &mi)) return hs_fail(HS_ERR_UNSUPPORTED, "unknown model %d", model);
+    if (n <= 0 || !params || nparams != mi.nparams)
+        return hs_fail(HS_ERR_ARG, "hs_net_add_group: model %d expects %d params, n > 0", model, mi.nparams);
+    HGroup g;
+    g.model = model;
+    g.n = n;
+    g.mi = mi;
+    memset(g.params, 0, sizeof(g.params));
+    memcpy(g.params, params, sizeof(double) * nparams);
+    memset(g.k, 0, sizeof(g.k));
+    hs_model_constants(model, params, net->dt, g.k);
+    g.ref_steps = (int)hs_timestep(params[mi.i_ref], net->dt);
+    g.stream = mi.i_stream >= 0 ? (unsigned int)params[mi.i_stream] : 0u;
+    for (int v = 0; v < mi.nvars; ++v) HS_TRY(hs_dalloc(net, hs_rsize(net) * n, &g.var[v], true));
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * n, (void **)&g.last, false));
+    std::vector<int32_t> never((size_t)n, HS_NEVER_SPIKED);
+    HS_HIP(hipMemcpyAsync(g.last, never.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    net->g.push_back(g);
+    net->desc_dirty = true;
+    if (gid) *gid = (int)net->g.size() - 1;
+    return HS_OK;
+}
+
+extern "C" int hs_net_set_state(hs_net *net, int gid, int var, const double *values)
+{
+    if (!net || gid < 0 || gid >= (int)net->g.size() || !values) return hs_fail(HS_ERR_ARG, "bad group");
+    HGroup &g = net->g[gid];
+    if (var < 0 || var >= g.mi.nvars) return hs_fail(HS_ERR_ARG, "bad variable %d", var);
+    if (net->prec == HS_F64) {
+        HS_HIP(hipMemcpyAsync(g.var[var], values, sizeof(double) * g.n, hipMemcpyHostToDevice, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    } else {
+        std::vector<float> f((size_t)g.n);
+        for (int i = 0; i < g.n; ++i) f[i] = (float)values[i];
+        HS_HIP(hipMemcpyAsync(g.var[var], f.data(), sizeof(float) * g.n, hipMemcpyHostToDevice, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    }
+    return HS_OK;
+}
+
+extern "C" int hs_net_get_state(hs_net *net, int gid, int var, double *values)
+{
+    if (!net || gid < 0 || gid >= (int)net->g.size() || !values) return hs_fail(HS_ERR_ARG, "bad group");
+    HGroup &g = net->g[gid];
+    if (var < 0 || var >= g.mi.nvars) return hs_fail(HS_ERR_ARG, "bad variable %d", var);
+    if (net->prec == HS_F64) {
+        HS_HIP(hipMemcpyAsync(values, g.var[var], sizeof(double) * g.n, hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    } else {
+        std::vector<float> f((size_t)g.n);
+        HS_HIP(hipMemcpyAsync(f.data(), g.var[var], sizeof(float) * g.n, hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+        for (int i = 0; i < g.n; ++i) values[i] = (double)f[i];
+    }
+    return HS_OK;
+}
+
+extern "C" int hs_net_get_last_spike(hs_net *net, int gid, int32_t *steps)
+{
+    if (!net || gid < 0 || gid >= (int)net->g.size() || !steps) return hs_fail(HS_ERR_ARG, "bad group");
+    HGroup &g = net->g[gid];
+    HS_HIP(hipMemcpyAsync(steps, g.last, sizeof(int32_t) * g.n, hipMemcpyDeviceToHost, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+/*
+ * Bundle builder: counting sort of the synapses by key (pre, delay - dmin), stable,
+ * so each (pre, delay) bundle is a contiguous run of post indices in synapse
+ * order.  O(S) host work and one copy per array, where the reference sorts per
+ * presynaptic neuron and copies once per bundle (synapses_push_spikes.cu:441-575).
+ */
+extern "C" int hs_net_add_pathway(hs_net *net, int src, int tgt, int var, int64_t nsyn, const int32_t *pre,
+                                  const int32_t *post, const int32_t *delay_steps, int scalar_delay, int wmode,
+                                  const double *weights, double wscalar, int *pid)
+{
+    if (!net) return hs_fail(HS_ERR_ARG, "null net");
+    if (net->built) return hs_fail(HS_ERR_STATE, "pathways must be added before the first run");
+    if (src < 0 || src >= (int)net->g.size() || tgt < 0 || tgt >= (int)net->g.size())
+        return hs_fail(HS_ERR_ARG, "bad source/target group");
+    HGroup &gs = net->g[src];
+    HGroup &gt = net->g[tgt];
+    if (var < 0 || var >= gt.mi.nvars) return hs_fail(HS_ERR_ARG, "bad target variable %d", var);
+    if (nsyn < 0 || nsyn >= 0x7fffffffLL) return hs_fail(HS_ERR_ARG, "synapse count %lld out of range", (long long)nsyn);
+    if (nsyn > 0 && (!pre || !post)) return hs_fail(HS_ERR_ARG, "null pre/post");
+    if (wmode < 0 || wmode > 2) return hs_fail(HS_ERR_ARG, "bad weight mode");
+    if (wmode != HS_W_SCALAR && !weights) return hs_fail(HS_ERR_ARG, "weights required for weight mode %d", wmode);
+    HPath P;
+    P.src = src;
+    P.tgt = tgt;
+    P.var = var;
+    P.nsyn = nsyn;
+    P.wmode = wmode;
+    P.wscalar = wscalar;
+    int dmin = scalar_delay, dmax = scalar_delay;
+    if (delay_steps && nsyn > 0) {
+        dmin = dmax = delay_steps[0];
+        for (int64_t s = 0; s < nsyn; ++s) {
+            dmin = std::min(dmin, delay_steps[s]);
+            dmax = std::max(dmax, delay_steps[s]);
+        }
+    }
+    if (dmin < 0) return hs_fail(HS_ERR_ARG, "negative delay");
+    const int D = dmax - dmin + 1;
+    if (D > HS_MAX_DELAY_SLOTS) return hs_fail(HS_ERR_UNSUPPORTED, "delay range %d steps > %d", D, HS_MAX_DELAY_SLOTS);
+    P.dmin = dmin;
+    P.dmax = dmax;
+    P.D = D;
+    const int64_t nb = (int64_t)gs.n * D;
+    std::vector<int32_t> start((size_t)nb + 1, 0);
+    for (int64_t s = 0; s < nsyn; ++s) {
+        if (pre[s] < 0 || pre[s] >= gs.n || post[s] < 0 || post[s] >= gt.n)
+            return hs_fail(HS_ERR_ARG, "synapse %lld: index out of range", (long long)s);
+        const int d = delay_steps ? delay_steps[s] : scalar_delay;
+        start[(size_t)((int64_t)pre[s] * D + (d - dmin)) + 1]++;
+    }
+    for (int64_t b = 0; b < nb; ++b) start[b + 1] += start[b];
+    std::vector<int32_t> pos(start.begin(), start.end() - 1);
+    std::vector<int32_t> post_sorted((size_t)(nsyn ? nsyn : 1));
+    const size_t rs = hs_rsize(net);
+    std::vector<unsigned char> w_sorted;
+    if (wmode == HS_W_SYNAPSE) w_sorted.resize(rs * (size_t)(nsyn ? nsyn : 1));
+    for (int64_t s = 0; s < nsyn; ++s) {
+        const int d = delay_steps ? delay_steps[s] : scalar_delay;
+        const int32_t j = pos[(size_t)((int64_t)pre[s] * D + (d - dmin))]++;
+        post_sorted[j] = post[s];
+        if (wmode == HS_W_SYNAPSE) {
+            if (net->prec == HS_F64) ((double *)w_sorted.data())[j] = weights[s];
+            else ((float *)w_sorted.data())[j] = (float)weights[s];
+        }
+    }
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)(nb + 1), (void **)&P.bstart, false));
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)(nsyn ? nsyn : 1), (void **)&P.post, false));
+    HS_HIP(hipMemcpyAsync(P.bstart, start.data(), sizeof(int32_t) * (size_t)(nb + 1), hipMemcpyHostToDevice,
+                          g_rt.stream));
+    if (nsyn)
+        HS_HIP(hipMemcpyAsync(P.post, post_sorted.data(), sizeof(int32_t) * (size_t)nsyn, hipMemcpyHostToDevice,
+                              g_rt.stream));
+    if (wmode == HS_W_SYNAPSE && nsyn) {
+        HS_TRY(hs_dalloc(net, rs * (size_t)nsyn, &P.w, false));
+        HS_HIP(hipMemcpyAsync(P.w, w_sorted.data(), rs * (size_t)nsyn, hipMemcpyHostToDevice, g_rt.stream));
+    } else if (wmode == HS_W_PRE) {
+        std::vector<unsigned char> wp(rs * (size_t)gs.n);
+        for (int i = 0; i < gs.n; ++i) {
+            if (net->prec == HS_F64) ((double *)wp.data())[i] = weights[i];
+            else ((float *)wp.data())[i] = (float)weights[i];
+        }
+        HS_TRY(hs_dalloc(net, rs * (size_t)gs.n, &P.w, false));
+        HS_HIP(hipMemcpyAsync(P.w, wp.data(), rs * (size_t)gs.n, hipMemcpyHostToDevice, g_rt.stream));
+    }
+    HS_HIP(hipStreamSynchronize(g_rt.stream)); /* host vectors die here */
+    net->p.push_back(P);
+    net->desc_dirty = true;
+    if (pid) *pid = (int)net->p.size() - 1;
+    return HS_OK;
+}
+
+extern "C" int hs_net_add_spike_monitor(hs_net *net, int gid, int64_t capacity, int *mid)
+{
+    if (!net || gid < 0 || gid >= (int)net->g.size()) return hs_fail(HS_ERR_ARG, "bad group");
+    if (net->g[gid].smon >= 0) return hs_fail(HS_ERR_UNSUPPORTED, "one spike monitor per group");
+    if (capacity <= 0) capacity = 1 << 20;
+    HMon M;
+    M.kind = 0;
+    M.gid = gid;
+    M.var = 0;
+    M.start = 0;
+    M.stop = net->g[gid].n;
+    M.cap = capacity;
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * capacity, (void **)&M.mi, false));
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * capacity, (void **)&M.mt, false));
+    HS_TRY(hs_dalloc(net, sizeof(unsigned long long), (void **)&M.mn, true));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    net->m.push_back(M);
+    net->g[gid].smon = (int)net->m.size() - 1;
+    net->desc_dirty = true;
+    hs_drop_graphs(net);
+    if (mid) *mid = (int)net->m.size() - 1;
+    return HS_OK;
+}
+
+extern "C" int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start, int stop, int *mid)
+{
+    if (!net || gid < 0 || gid >= (int)net->g.size()) return hs_fail(HS_ERR_ARG, "bad group");
+    HGroup &g = net->g[gid];
+    if (var < 0 || var >= g.mi.nvars || start < 0 || stop > g.n || stop <= start)
+        return hs_fail(HS_ERR_ARG, "bad state monitor range");
+    if ((int)g.stmons.size() >= HS_MAX_STATE_MONS) return hs_fail(HS_ERR_UNSUPPORTED, "at most %d state monitors per group", HS_MAX_STATE_MONS);
+    HMon M;
+    M.kind = 1;
+    M.gid = gid;
+    M.var = var;
+    M.start = start;
+    M.stop = stop;
+    M.t0 = net->step;
+    net->m.push_back(M);
+    g.stmons.push_back((int)net->m.size() - 1);
+    net->desc_dirty = true;
+    hs_drop_graphs(net);
+    if (mid) *mid = (int)net->m.size() - 1;
+    return HS_OK;
+}
+
+extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
+{
+    if (!net || key < 0 || key > 3) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (key == 0 && (value < 64 || value > 1024 || value % 64)) return hs_fail(HS_ERR_ARG, "block size must be a multiple of 64 in [64,1024]");
+    if ((key == 1 || key == 3) && (value < 0 || value > (1 << 20))) return hs_fail(HS_ERR_ARG, "bad grid size");
+    if (key == 2 && (value < 0 || value > 100000)) return hs_fail(HS_ERR_ARG, "bad graph chunk");
+    net->opt[key] = value;
+    net->desc_dirty = true;
+    hs_drop_graphs(net);
+    return HS_OK;
+}
+
+extern "C" int hs_net_get_option(hs_net *net, int key, long long *value)
+{
+    if (!net || key < 0 || key > 3 || !value) return hs_fail(HS_ERR_ARG, "bad option key");
+    *value = net->opt[key];
+    return HS_OK;
+}
+
+/* allocate rings/accumulators once (first run) */
+static int hs_build(hs_net *net)
+{
+    if (net->built) return HS_OK;
+    for (HGroup &g : net->g) {
+        int dmax = 0;
+        for (const HPath &P : net->p)
+            if (P.src == (int)(&g - &net->g[0])) dmax = std::max(dmax, P.dmax);
+        g.Q = dmax + 2;
+        HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.Q * g.n, (void **)&g.hist, false));
+        HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.Q, (void **)&g.hist_cnt, true));
+    }
+    if (net->engine == HS_ENGINE_FUSED) {
+        for (const HPath &P : net->p) {
+            HGroup &gt = net->g[P.tgt];
+            for (int b = 0; b < 2; ++b)
+                if (!gt.acc[b][P.var]) HS_TRY(hs_dalloc(net, hs_rsize(net) * gt.n, &gt.acc[b][P.var], true));
+        }
+    }
+    net->single_launch = net->engine == HS_ENGINE_FUSED;
+    for (const HPath &P : net->p)
+        if (P.dmin < 1) net->single_launch = false;
+    HsCtl c;
+    memset(&c, 0, sizeof(c));
+    c.base = net->step;
+    HS_HIP(hipMemcpyAsync(net->ctl, &c, sizeof(c), hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    net->built = true;
+    return HS_OK;
+}
+
+template <typename R>
+static int hs_upload_desc_t(hs_net *net)
+{
+    const int block = (int)net->opt[0];
+    std::vector<DGroup<R>> dg(net->g.size());
+    int tiles = 0;
+    for (size_t gi = 0; gi < net->g.size(); ++gi) {
+        const HGroup &g = net->g[gi];
+        DGroup<R> &d = dg[gi];
+        memset(&d, 0, sizeof(d));
+        d.model = g.model;
+        d.n = g.n;
+        d.nvars = g.mi.nvars;
+        d.Q = g.Q;
+        d.has_reset = g.mi.has_reset;
+        d.ref_steps = g.ref_steps;
+        d.stream = g.stream;
+        for (int v = 0; v < HS_MAX_VARS; ++v) {
+            d.var[v] = (R *)g.var[v];
+            d.acc[0][v] = (R *)g.acc[0][v];
+            d.acc[1][v] = (R *)g.acc[1][v];
+        }
+        d.last = g.last;
+        d.hist = g.hist;
+        d.hist_cnt = g.hist_cnt;
+        for (int j = 0; j < 16; ++j) d.k[j] = (R)g.k[j];
+        d.vth = (R)g.params[g.mi.i_vth];
+        d.vr = g.mi.i_vr >= 0 ? (R)g.params[g.mi.i_vr] : (R)0;
+        d.seed = net->seed;
+        if (g.smon >= 0) {
+            const HMon &M = net->m[g.smon];
+            d.smon_i = M.mi;
+            d.smon_t = M.mt;
+            d.smon_n = M.mn;
+            d.smon_cap = M.cap;
+        }
+        d.nst = (int)g.stmons.size();
+        for (int s = 0; s < d.nst; ++s) {
+            const HMon &M = net->m[g.stmons[s]];
+            d.st_var[s] = M.var;
+            d.st_start[s] = M.start;
+            d.st_stop[s] = M.stop;
+            d.st_t0[s] = M.t0;
+            d.st_rows[s] = M.rows;
+            d.st_buf[s] = (R *)M.buf;
+        }
+        d.tile_begin = tiles;
+        tiles += (g.n + block - 1) / block;
+    }
+    std::vector<DPath<R>> dp(net->p.size());
+    for (size_t pi = 0; pi < net->p.size(); ++pi) {
+        const HPath &P = net->p[pi];
+        const HGroup &gs = net->g[P.src];
+        const HGroup &gt = net->g[P.tgt];
+        DPath<R> &d = dp[pi];
+        memset(&d, 0, sizeof(d));
+        d.src = P.src;
+        d.tgt = P.tgt;
+        d.var = P.var;
+        d.n_src = gs.n;
+        d.dmin = P.dmin;
+        d.D = P.D;
+        d.Q = gs.Q;
+        d.wmode = P.wmode;
+        d.hist = gs.hist;
+        d.hist_cnt = gs.hist_cnt;
+        d.bstart = P.bstart;
+        d.post = P.post;
+        d.w = (const R *)P.w;
+        d.wscalar = (R)P.wscalar;
+        d.target = (R *)gt.var[P.var];
+        d.acc[0] = (R *)gt.acc[0][P.var];
+        d.acc[1] = (R *)gt.acc[1][P.var];
+    }
+    if (!net->dgroups) HS_TRY(hs_dalloc(net, sizeof(DGroup<R>) * std::max<size_t>(1, dg.size()), &net->dgroups, false));
+    if (!net->dpaths) HS_TRY(hs_dalloc(net, sizeof(DPath<R>) * std::max<size_t>(1, dp.size()), &net->dpaths, false));
+    HS_HIP(hipMemcpyAsync(net->dgroups, dg.data(), sizeof(DGroup<R>) * dg.size(), hipMemcpyHostToDevice, g_rt.stream));
+    if (!dp.empty())
+        HS_HIP(hipMemcpyAsync(net->dpaths, dp.data(), sizeof(DPath<R>) * dp.size(), hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    net->total_tiles = tiles;
+    net->desc_dirty = false;
+    return HS_OK;
+}
+
+static int hs_upload_desc(hs_net *net)
+{
+    return net->prec == HS_F64 ? hs_upload_desc_t<double>(net) : hs_upload_desc_t<float>(net);
+}
+
+/* grow state-monitor buffers to hold rows up to step `until` */
+static int hs_grow_state_monitors(hs_net *net, int64_t until)
+{
+    for (HMon &M : net->m) {
+        if (M.kind != 1) continue;
+        const int64_t need = until - M.t0;
+        if (need <= M.rows) continue;
+        int64_t rows = std::max<int64_t>(need, M.rows * 2);
+        const size_t w = (size_t)(M.stop - M.start);
+        void *nb = nullptr;
+        HS_TRY(hs_dalloc(net, hs_rsize(net) * w * (size_t)rows, &nb, false));
+        if (M.buf && M.rows)
+            HS_HIP(hipMemcpyAsync(nb, M.buf, hs_rsize(net) * w * (size_t)M.rows, hipMemcpyDeviceToDevice, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+        hs_dfree(net, M.buf);
+        M.buf = nb;
+        M.rows = rows;
+        net->desc_dirty = true;
+    }
+    return HS_OK;
+}
+
+/* grid of the delivery / fused kernels */
+static int hs_deliver_grid(const hs_net *net)
+{
+    if (net->opt[1] > 0) return (int)net->opt[1];
+    return g_rt.n_cu * 4;
+}
+
+static int hs_fused_grid(const hs_net *net)
+{
+    if (net->opt[3] > 0) return (int)net->opt[3];
+    int g = std::max(net->total_tiles, g_rt.n_cu * 2);
+    return std::min(g, g_rt.n_cu * 16);
+}
+
+static int hs_update_grid(const hs_net *net)
+{
+    return std::max(1, std::min(net->total_tiles, 1 << 20));
+}
+
+/* enqueue k steps (chunk offsets 0..k-1) and the tick */
+static int hs_enqueue_steps(hs_net *net, int64_t k)
+{
+    const int block = (int)net->opt[0];
+    const int ng = (int)net->g.size(), np = (int)net->p.size();
+    const bool fused = net->engine == HS_ENGINE_FUSED;
+    for (int64_t j = 0; j < k; ++j) {
+        const int koff = (int)j;
+        if (net->prec == HS_F64) {
+            const DGroup<double> *dg = (const DGroup<double> *)net->dgroups;
+            const DPath<double> *dp = (const DPath<double> *)net->dpaths;
+            if (net->single_launch && np > 0) {
+                HS_HIP(hs_launch_fused<double>(dg, ng, net->total_tiles, dp, np, net->ctl, koff, block,
+                                               hs_fused_grid(net), g_rt.stream));
+            } else {
+                HS_HIP(hs_launch_update<double>(dg, ng, net->total_tiles, net->ctl, koff, fused, block,
+                                                hs_update_grid(net), g_rt.stream));
+                if (np > 0)
+                    HS_HIP(hs_launch_deliver<double>(dp, np, net->ctl, koff, fused, 256, hs_deliver_grid(net),
+                                                     g_rt.stream));
+            }
+        } else {
+            const DGroup<float> *dg = (const DGroup<float> *)net->dgroups;
+            const DPath<float> *dp = (const DPath<float> *)net->dpaths;
+            if (net->single_launch && np > 0) {
+                HS_HIP(hs_launch_fused<float>(dg, ng, net->total_tiles, dp, np, net->ctl, koff, block,
+                                              hs_fused_grid(net), g_rt.stream));
+            } else {
+                HS_HIP(hs_launch_update<float>(dg, ng, net->total_tiles, net->ctl, koff, fused, block,
+                                               hs_update_grid(net), g_rt.stream));
+                if (np > 0)
+                    HS_HIP(hs_launch_deliver<float>(dp, np, net->ctl, koff, fused, 256, hs_deliver_grid(net),
+                                                    g_rt.stream));
+            }
+        }
+    }
+    HS_HIP(hs_launch_tick(net->ctl, k, g_rt.stream));
+    return HS_OK;
+}
+
+static int hs_chunk(hs_net *net, int64_t k)
+{
+    if (net->opt[2] <= 0) return hs_enqueue_steps(net, k);
+    auto it = net->graphs.find(k);
+    if (it == net->graphs.end()) {
+        hipGraph_t graph;
+        HS_HIP(hipStreamBeginCapture(g_rt.stream, hipStreamCaptureModeThreadLocal));
+        int r = hs_enqueue_steps(net, k);
+        hipError_t e = hipStreamEndCapture(g_rt.stream, &graph);
+        if (r != HS_OK) return r;
+        if (e != hipSuccess) return hs_fail(HS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        hipGraphExec_t exec;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hs_fail(HS_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+        it = net->graphs.emplace(k, exec).first;
+    }
+    HS_HIP(hipGraphLaunch(it->second, g_rt.stream));
+    return HS_OK;
+}
+
+/* before a chunk of k steps: make sure the spike monitors have room */
+static int hs_check_spike_monitors(hs_net *net, int64_t k, bool *resized)
+{
+    *resized = false;
+    for (HMon &M : net->m) {
+        if (M.kind != 0) continue;
+        unsigned long long n = 0;
+        HS_HIP(hipMemcpyAsync(&n, M.mn, sizeof(n), hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+        if ((int64_t)n > M.cap)
+            return hs_fail(HS_ERR_OVERFLOW, "spike monitor overflow: %llu records for capacity %lld", n,
+                           (long long)M.cap);
+        /* a chunk of k steps adds at most n_group * k records: keep that much room */
+        const int64_t headroom = (int64_t)net->g[M.gid].n * k;
+        if ((int64_t)n + headroom > M.cap) {
+            int64_t cap = std::max<int64_t>(2 * M.cap, (int64_t)n + headroom);
+            int32_t *ni = nullptr, *nt = nullptr;
+            HS_TRY(hs_dalloc(net, sizeof(int32_t) * cap, (void **)&ni, false));
+            HS_TRY(hs_dalloc(net, sizeof(int32_t) * cap, (void **)&nt, false));
+            if (n) {
+                HS_HIP(hipMemcpyAsync(ni, M.mi, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, g_rt.stream));
+                HS_HIP(hipMemcpyAsync(nt, M.mt, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, g_rt.stream));
+            }
+            HS_HIP(hipStreamSynchronize(g_rt.stream));
+            hs_dfree(net, M.mi);
+            hs_dfree(net, M.mt);
+            M.mi = ni;
+            M.mt = nt;
+            M.cap = cap;
+            *resized = true;
+        }
+    }
+    return HS_OK;
+}
+
+extern "C" int hs_net_run(hs_net *net, int64_t nsteps)
+{
+    if (!net) return hs_fail(HS_ERR_ARG, "null net");
+    if (nsteps < 0) return hs_fail(HS_ERR_ARG, "negative step count");
+    if (net->g.empty()) return hs_fail(HS_ERR_STATE, "network has no groups");
+    HS_TRY(hs_build(net));
+    HS_TRY(hs_grow_state_monitors(net, net->step + nsteps));
+    if (net->desc_dirty) {
+        hs_drop_graphs(net);
+        HS_TRY(hs_upload_desc(net));
+    }
+    bool have_smon = false;
+    for (const HMon &M : net->m) have_smon |= (M.kind == 0);
+    const int64_t K = net->opt[2] > 0 ? net->opt[2] : 256;
+    auto t0 = std::chrono::steady_clock::now();
+    HS_HIP(hipEventRecord(net->ev0, g_rt.stream));
+    int64_t done = 0;
+    while (done < nsteps) {
+        const int64_t k = std::min<int64_t>(K, nsteps - done);
+        if (have_smon) {
+            bool resized = false;
+            HS_TRY(hs_check_spike_monitors(net, k, &resized));
+            if (resized) {
+                hs_drop_graphs(net);
+                HS_TRY(hs_upload_desc(net));
+            }
+        }
+        HS_TRY(hs_chunk(net, k));
+        done += k;
+    }
+    HS_HIP(hipEventRecord(net->ev1, g_rt.stream));
+    const int block = (int)net->opt[0];
+    const bool fused = net->engine == HS_ENGINE_FUSED;
+    if (net->prec == HS_F64)
+        HS_HIP(hs_launch_finalize<double>((const DGroup<double> *)net->dgroups, (int)net->g.size(), net->total_tiles,
+                                          net->ctl, fused, block, g_rt.stream));
+    else
+        HS_HIP(hs_launch_finalize<float>((const DGroup<float> *)net->dgroups, (int)net->g.size(), net->total_tiles,
+                                         net->ctl, fused, block, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    auto t1 = std::chrono::steady_clock::now();
+    float ms = 0;
+    HS_HIP(hipEventElapsedTime(&ms, net->ev0, net->ev1));
+    net->last_gpu_ms = ms;
+    net->last_wall = std::chrono::duration<double>(t1 - t0).count();
+    net->last_steps = nsteps;
+    net->step += nsteps;
+    HsCtl c;
+    HS_HIP(hipMemcpyAsync(&c, net->ctl, sizeof(c), hipMemcpyDeviceToHost, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    net->events = c.events;
+    if (c.base != net->step) return hs_fail(HS_ERR_STATE, "device step %lld != host step %lld", c.base, (long long)net->step);
+    for (const HMon &M : net->m) {
+        if (M.kind != 0) continue;
+        unsigned long long n = 0;
+        HS_HIP(hipMemcpyAsync(&n, M.mn, sizeof(n), hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+        if ((int64_t)n > M.cap)
+            return hs_fail(HS_ERR_OVERFLOW, "spike monitor overflow: %llu records for capacity %lld", n,
+                           (long long)M.cap);
+    }
+    return HS_OK;
+}
+
+extern "C" int hs_net_stats(hs_net *net, hs_stats *out)
+{
+    if (!net || !out) return hs_fail(HS_ERR_ARG, "null argument");
+    out->steps = net->step;
+    out->events = net->events;
+    out->last_run_wall_s = net->last_wall;
+    out->last_run_gpu_ms = net->last_gpu_ms;
+    out->last_run_steps = net->last_steps;
+    out->launches_per_step = net->p.empty() ? 1 : (net->single_launch ? 1 : 2);
+    return HS_OK;
+}
+
+extern "C" int hs_net_mon_count(hs_net *net, int mid, int64_t *n)
+{
+    if (!net || mid < 0 || mid >= (int)net->m.size() || !n) return hs_fail(HS_ERR_ARG, "bad monitor");
+    const HMon &M = net->m[mid];
+    if (M.kind == 0) {
+        unsigned long long c = 0;
+        HS_HIP(hipMemcpyAsync(&c, M.mn, sizeof(c), hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+        *n = (int64_t)c;
+    } else {
+        *n = std::max<int64_t>(0, net->step - M.t0);
+    }
+    return HS_OK;
+}
+
+extern "C" int hs_net_get_spikes(hs_net *net, int mid, int32_t *i, int64_t *step, int64_t cap)
+{
+    if (!net || mid < 0 || mid >= (int)net->m.size() || net->m[mid].kind != 0) return hs_fail(HS_ERR_ARG, "bad monitor");
+    const HMon &M = net->m[mid];
+    int64_t n = 0;
+    HS_TRY(hs_net_mon_count(net, mid, &n));
+    if (n > M.cap) return hs_fail(HS_ERR_OVERFLOW, "spike monitor overflow: %lld records for capacity %lld", (long long)n, (long long)M.cap);
+    if (cap < n) return hs_fail(HS_ERR_ARG, "output arrays too small (%lld < %lld)", (long long)cap, (long long)n);
+    std::vector<int32_t> hi((size_t)n), ht((size_t)n);
+    if (n) {
+        HS_HIP(hipMemcpyAsync(hi.data(), M.mi, sizeof(int32_t) * n, hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipMemcpyAsync(ht.data(), M.mt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    }
+    /* records of one step arrive tile by tile; order them as cpp_standalone does */
+    std::vector<uint64_t> key((size_t)n);
+    for (int64_t r = 0; r < n; ++r) key[r] = ((uint64_t)(uint32_t)ht[r] << 32) | (uint32_t)hi[r];
+    std::sort(key.begin(), key.end());
+    for (int64_t r = 0; r < n; ++r) {
+        step[r] = (int64_t)(key[r] >> 32);
+        i[r] = (int32_t)(key[r] & 0xffffffffu);
+    }
+    return HS_OK;
+}
+
+extern "C" int hs_net_get_state_monitor(hs_net *net, int mid, double *out, int64_t cap)
+{
+    if (!net || mid < 0 || mid >= (int)net->m.size() || net->m[mid].kind != 1) return hs_fail(HS_ERR_ARG, "bad monitor");
+    const HMon &M = net->m[mid];
+    const int64_t rows = std::max<int64_t>(0, net->step - M.t0);
+    const int64_t w = M.stop - M.start;
+    if (cap < rows * w) return hs_fail(HS_ERR_ARG, "output too small");
+    if (!rows) return HS_OK;
+    if (net->prec == HS_F64) {
+        HS_HIP(hipMemcpyAsync(out, M.buf, sizeof(double) * rows * w, hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    } else {
+        std::vector<float> f((size_t)(rows * w));
+        HS_HIP(hipMemcpyAsync(f.data(), M.buf, sizeof(float) * rows * w, hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+        for (int64_t r = 0; r < rows * w; ++r) out[r] = f[r];
+    }
+    return HS_OK;
+}

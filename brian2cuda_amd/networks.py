@@ -1,0 +1,251 @@
+"""Synthetic workloads of the hot path, generated from a seed as plain arrays.
+
+Every generator returns a :class:`NetSpec` -- groups (model, size, parameters,
+initial state), pathways (pre, post, quantised delays, weights) and monitors --
+that both the device runtime (:func:`build_hs`) and the oracle consume, so "same
+seed and inputs" holds by construction.
+
+Models and parameters follow the reference's examples and benchmarks:
+  cuba      examples/cuba.py:118-143                      (CUBA LIF, exact)
+  cobahh    examples/cobahh.py:140-190, tests/features/speed.py:73-113
+  brunel    examples/brunelhakim.py:140-163                (Euler-Maruyama, noise)
+  batch     many CUBA networks concatenated into one index space
+"""
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from .runtime import (HS_MODEL_HH_COBA, HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_W_PRE, HS_W_SCALAR,
+                      HS_W_SYNAPSE)
+
+mV, ms, nS = 1e-3, 1e-3, 1e-9
+
+
+@dataclass
+class GroupSpec:
+    model: int
+    n: int
+    params: list
+    init: dict = field(default_factory=dict)  # var index -> array (or scalar)
+
+
+@dataclass
+class PathSpec:
+    src: int
+    tgt: int
+    var: int
+    pre: np.ndarray
+    post: np.ndarray
+    delay_steps: Optional[np.ndarray] = None
+    scalar_delay: int = 0
+    wmode: int = HS_W_SCALAR
+    weights: Optional[np.ndarray] = None
+    wscalar: float = 0.0
+
+    @property
+    def nsyn(self):
+        return int(self.pre.size)
+
+
+@dataclass
+class NetSpec:
+    name: str
+    dt: float
+    seed: int
+    groups: List[GroupSpec]
+    paths: List[PathSpec]
+    spike_monitors: List[int] = field(default_factory=list)         # group ids
+    state_monitors: List[tuple] = field(default_factory=list)       # (gid, var, start, stop)
+    info: dict = field(default_factory=dict)
+
+    @property
+    def n_neurons(self):
+        return sum(g.n for g in self.groups)
+
+    @property
+    def n_synapses(self):
+        return sum(p.nsyn for p in self.paths)
+
+
+def quantise_delay(d, dt):
+    """Delay in steps as the reference computes it: (int)(d/dt + 0.5)
+    (brian2cuda/templates/synapses_push_spikes.cu:375)."""
+    return (np.asarray(d, dtype=np.float64) / dt + 0.5).astype(np.int32)
+
+
+def bernoulli_connect(rng, n_pre, n_post, p, pre_offset=0, post_offset=0):
+    """Independent Bernoulli(p) synapses for every (pre, post) pair.
+
+    Gaps between successes of a Bernoulli process are geometric, so the
+    successes over the flattened n_pre*n_post pairs are a cumulative sum of
+    geometric draws (the skip method the reference's generator uses per row,
+    synapses_create_generator.cu:315-333).  Returns pre, post (int32), ordered
+    by (pre, post)."""
+    total = int(n_pre) * int(n_post)
+    if p <= 0 or total == 0:
+        return np.zeros(0, np.int32), np.zeros(0, np.int32)
+    if p >= 1:
+        flat = np.arange(total, dtype=np.int64)
+    else:
+        mean = total * p
+        m = int(mean + 8 * np.sqrt(mean) + 16)
+        pos = np.cumsum(rng.geometric(p, size=m).astype(np.int64)) - 1
+        while pos[-1] < total:
+            more = np.cumsum(rng.geometric(p, size=m).astype(np.int64)) + pos[-1]
+            pos = np.concatenate([pos, more])
+        flat = pos[pos < total]
+    pre = (flat // n_post).astype(np.int32) + np.int32(pre_offset)
+    post = (flat % n_post).astype(np.int32) + np.int32(post_offset)
+    return pre, post
+
+
+# ---- C1: CUBA --------------------------------------------------------------------
+
+def cuba_params(El=-49 * mV, Vt=-50 * mV, Vr=-60 * mV, taum=20 * ms, taue=5 * ms, taui=10 * ms, refractory=5 * ms):
+    return [El, Vt, Vr, taum, taue, taui, refractory]
+
+
+def cuba(N=4000, seed=1234, p=0.02, delay_steps=1, dt=0.1 * ms, record=100, we=1.62 * mV, wi=-9 * mV,
+         monitors=True):
+    """C1: 4000 LIF (3200 exc / 800 inh), p=0.02, delay one dt."""
+    rng = np.random.default_rng(seed)
+    Ne = int(0.8 * N)
+    Vt, Vr = -50 * mV, -60 * mV
+    v0 = Vr + rng.random(N) * (Vt - Vr)
+    g = GroupSpec(HS_MODEL_LIF_CUBA, N, cuba_params(), {0: v0, 1: 0.0, 2: 0.0})
+    pe, qe = bernoulli_connect(rng, Ne, N, p)
+    pi, qi = bernoulli_connect(rng, N - Ne, N, p, pre_offset=Ne)
+    paths = [
+        PathSpec(0, 0, 1, pe, qe, scalar_delay=delay_steps, wmode=HS_W_SCALAR, wscalar=we),
+        PathSpec(0, 0, 2, pi, qi, scalar_delay=delay_steps, wmode=HS_W_SCALAR, wscalar=wi),
+    ]
+    spec = NetSpec("cuba", dt, seed, [g], paths, info={"N": N, "p": p})
+    if monitors:
+        spec.spike_monitors = [0]
+        if record:
+            spec.state_monitors = [(0, 0, 0, min(record, N))]
+    return spec
+
+
+# ---- C2: COBAHH ------------------------------------------------------------------
+
+def cobahh_params():
+    area = 20000e-12  # m^2
+    Cm = 1e-2 * area   # 1 uF/cm^2
+    gl = 0.5 * area    # 5e-5 S/cm^2
+    gNa = 1000.0 * area
+    gK = 300.0 * area
+    return [Cm, gl, -60 * mV, -90 * mV, 50 * mV, gNa, gK, -63 * mV, 0 * mV, -80 * mV, 5 * ms, 10 * ms,
+            -20 * mV, 3 * ms]
+
+
+def cobahh(N=4000, seed=1234, p=None, dt=0.1 * ms, we=6 * nS, wi=67 * nS, delay_steps=0, monitors=True):
+    """C2: COBAHH, 80/20 exc/inh, conductance synapses; p=0.02 at N=4000, else mean in-degree 80."""
+    rng = np.random.default_rng(seed)
+    if p is None:
+        p = 0.02 if N <= 4000 else 80.0 / N
+    Ne = int(0.8 * N)
+    v0 = -60 * mV + 5 * mV * rng.standard_normal(N)
+    g = GroupSpec(HS_MODEL_HH_COBA, N, cobahh_params(), {0: v0, 1: 0.0, 2: 0.0, 3: 0.0, 4: 0.0, 5: 0.0})
+    pe, qe = bernoulli_connect(rng, Ne, N, p)
+    pi, qi = bernoulli_connect(rng, N - Ne, N, p, pre_offset=Ne)
+    paths = [
+        PathSpec(0, 0, 4, pe, qe, scalar_delay=delay_steps, wmode=HS_W_SCALAR, wscalar=we),
+        PathSpec(0, 0, 5, pi, qi, scalar_delay=delay_steps, wmode=HS_W_SCALAR, wscalar=wi),
+    ]
+    spec = NetSpec("cobahh", dt, seed, [g], paths, info={"N": N, "p": p})
+    if monitors:
+        spec.spike_monitors = [0]
+    return spec
+
+
+# ---- C3: Brunel-Hakim --------------------------------------------------------------
+
+def brunel_params(tau=20 * ms, mu=25 * mV, sigma=1 * mV, theta=20 * mV, Vr=10 * mV, refractory=2 * ms, stream=0):
+    return [tau, mu, sigma, theta, Vr, refractory, float(stream)]
+
+
+def brunel(N=100_000, K=1000, seed=1234, dt=0.1 * ms, J=-0.1 * mV, dlo=0.5 * ms, dhi=5.0 * ms,
+           per_synapse_weights=True, monitors=False, sigma=1 * mV, mu=25 * mV):
+    """C3: sparse inhibitory LIF network with noise and heterogeneous delays.
+
+    connect(p=K/N) (examples/brunelhakim.py:144-156), delays U[dlo, dhi] quantised
+    to dt, on_pre 'V += -|J|' with a per-synapse weight array."""
+    rng = np.random.default_rng(seed)
+    g = GroupSpec(HS_MODEL_LIF_NOISE, N, brunel_params(sigma=sigma, mu=mu), {0: 10 * mV})
+    pre, post = bernoulli_connect(rng, N, N, K / N)
+    d = quantise_delay(dlo + (dhi - dlo) * rng.random(pre.size), dt)
+    if per_synapse_weights:
+        ps = PathSpec(0, 0, 0, pre, post, delay_steps=d, wmode=HS_W_SYNAPSE,
+                      weights=np.full(pre.size, J, dtype=np.float32).astype(np.float64))
+    else:
+        ps = PathSpec(0, 0, 0, pre, post, delay_steps=d, wmode=HS_W_SCALAR, wscalar=J)
+    spec = NetSpec("brunel", dt, seed, [g], [ps], info={"N": N, "K": K})
+    if monitors:
+        spec.spike_monitors = [0]
+    return spec
+
+
+# ---- C4: batched parameter sweep ----------------------------------------------------
+
+def batch_cuba(n_nets=256, seed=1234, nmin=500, nmax=16000, p=0.02, dt=0.1 * ms, wi=-9 * mV, delay_steps=1,
+               monitors=True):
+    """C4: n_nets independent CUBA networks of log-uniform sizes in one index space.
+
+    Net b (seed+b) has its own w_e ~ U[0.5, 3] mV; its neurons occupy
+    [offset_b, offset_b + N_b) and its synapses stay inside that range, so the
+    concatenated group advances all networks in the same launches.  The
+    excitatory pathway carries w_e per presynaptic neuron (weight mode PRE)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=n_nets))).astype(np.int64)
+    offsets = np.concatenate([[0], np.cumsum(sizes)])
+    N = int(offsets[-1])
+    Vt, Vr = -50 * mV, -60 * mV
+    v0 = np.empty(N)
+    wpre = np.zeros(N)
+    we_net = np.empty(n_nets)
+    pre_e, post_e, pre_i, post_i = [], [], [], []
+    for b in range(n_nets):
+        rb = np.random.default_rng(seed + b)
+        nb, o = int(sizes[b]), int(offsets[b])
+        ne = int(0.8 * nb)
+        v0[o:o + nb] = Vr + rb.random(nb) * (Vt - Vr)
+        we_net[b] = rb.uniform(0.5, 3.0) * mV
+        wpre[o:o + nb] = we_net[b]
+        a, c = bernoulli_connect(rb, ne, nb, p, pre_offset=o, post_offset=o)
+        pre_e.append(a); post_e.append(c)
+        a, c = bernoulli_connect(rb, nb - ne, nb, p, pre_offset=o + ne, post_offset=o)
+        pre_i.append(a); post_i.append(c)
+    g = GroupSpec(HS_MODEL_LIF_CUBA, N, cuba_params(), {0: v0, 1: 0.0, 2: 0.0})
+    paths = [
+        PathSpec(0, 0, 1, np.concatenate(pre_e), np.concatenate(post_e), scalar_delay=delay_steps,
+                 wmode=HS_W_PRE, weights=wpre),
+        PathSpec(0, 0, 2, np.concatenate(pre_i), np.concatenate(post_i), scalar_delay=delay_steps,
+                 wmode=HS_W_SCALAR, wscalar=wi),
+    ]
+    spec = NetSpec("batch_cuba", dt, seed, [g], paths,
+                   info={"n_nets": n_nets, "sizes": sizes, "offsets": offsets, "we": we_net})
+    if monitors:
+        spec.spike_monitors = [0]
+    return spec
+
+
+# ---- instantiate on the device --------------------------------------------------------
+
+def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options=None):
+    """Create a runtime.Net from a spec.  Returns (net, spike_monitor_ids, state_monitor_ids)."""
+    from .runtime import Net
+    net = Net(precision, engine, spec.dt, spec.seed)
+    for g in spec.groups:
+        gid = net.add_group(g.model, g.n, g.params)
+        for var, val in g.init.items():
+            net.set_state(gid, var, val)
+    for p in spec.paths:
+        net.add_pathway(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode, p.weights,
+                        p.wscalar)
+    smons = [net.add_spike_monitor(g, spike_capacity) for g in spec.spike_monitors]
+    stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
+    for k, v in (options or {}).items():
+        net.set_option(k, v)
+    return net, smons, stmons

@@ -1,0 +1,255 @@
+"""Python host side of libhs: device buffers, the standalone kernels and networks.
+
+This mirrors what the reference's generated C++ does around the hot path
+(brian2cuda/templates/objects.cu: arrays; network.cu: the step loop;
+synapses_push_spikes.cu: connectivity preparation) with numpy arrays on the
+host and HBM buffers owned by libhs.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import HsStats, call, init
+
+HS_F64, HS_F32 = 64, 32
+HS_ENGINE_PLAIN, HS_ENGINE_FUSED = 0, 1
+HS_W_SCALAR, HS_W_SYNAPSE, HS_W_PRE = 0, 1, 2
+
+HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_HH_COBA = 1, 2, 3
+MODEL_NVARS = {HS_MODEL_LIF_CUBA: 3, HS_MODEL_LIF_NOISE: 1, HS_MODEL_HH_COBA: 6}
+MODEL_NPARAMS = {HS_MODEL_LIF_CUBA: 7, HS_MODEL_LIF_NOISE: 7, HS_MODEL_HH_COBA: 14}
+MODEL_VARS = {
+    HS_MODEL_LIF_CUBA: ("v", "ge", "gi"),
+    HS_MODEL_LIF_NOISE: ("V",),
+    HS_MODEL_HH_COBA: ("v", "m", "n", "h", "ge", "gi"),
+}
+NEVER_SPIKED = -1073741824
+
+OPT_UPDATE_BLOCK, OPT_DELIVER_GRID, OPT_GRAPH_CHUNK, OPT_FUSED_GRID = 0, 1, 2, 3
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _i32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _f64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class DeviceArray:
+    """An HBM buffer owned by libhs (hs_alloc) with numpy-typed copies."""
+
+    def __init__(self, n, dtype):
+        init()
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        call("hs_alloc", self.nbytes, C.byref(p))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.n:
+            raise ValueError(f"size {a.size} != {self.n}")
+        call("hs_h2d", self.ptr, _ptr(a), self.nbytes)
+
+    def numpy(self):
+        out = np.empty(self.n, self.dtype)
+        call("hs_d2h", _ptr(out), self.ptr, self.nbytes)
+        return out
+
+    def zero(self):
+        call("hs_zero", self.ptr, self.nbytes)
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            call("hs_free", self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr is not None and self.ptr.value and _lib._lib is not None:
+                _lib._lib.hs_free(self.ptr)
+        except Exception:
+            pass
+
+
+# ---- standalone kernels ---------------------------------------------------------
+
+def threshold_compact(v, vth, last, step, ref_steps, sorted_=True):
+    """hs_threshold_compact on device arrays v (f64/f32) and last (int32).
+
+    Returns the spike indices (numpy int32)."""
+    prec = HS_F64 if v.dtype == np.float64 else HS_F32
+    spikes = DeviceArray(max(v.n, 1), np.int32)
+    count = DeviceArray(1, np.int32)
+    call("hs_threshold_compact", prec, v.ptr, v.n, float(vth), last.ptr, int(step), int(ref_steps),
+         1 if sorted_ else 0, spikes.ptr, count.ptr)
+    n = int(count.numpy()[0])
+    return spikes.numpy()[:n]
+
+
+def randn(seed, stream, step, n, precision=HS_F64):
+    out = DeviceArray(n, np.float64 if precision == HS_F64 else np.float32)
+    call("hs_randn", int(seed), int(stream), int(step), int(n), precision, out.ptr)
+    return out.numpy()
+
+
+def rand(seed, stream, step, n):
+    out = DeviceArray(n, np.float64)
+    call("hs_rand", int(seed), int(stream), int(step), int(n), out.ptr)
+    return out.numpy()
+
+
+def math_eval(fn, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    dx = DeviceArray.from_numpy(x)
+    dy = DeviceArray(x.size, np.float64)
+    call("hs_math_eval", int(fn), x.size, dx.ptr, dy.ptr)
+    return dy.numpy()
+
+
+def stateupdate(model, precision, vars_dev, last_dev, params, dt, seed, step):
+    """One in-place state-update step of `model` on device arrays."""
+    arr = (C.c_void_p * len(vars_dev))(*[v.ptr.value for v in vars_dev])
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    call("hs_stateupdate", model, precision, vars_dev[0].n, arr, last_dev.ptr, _f64(p), p.size, float(dt),
+         int(seed), int(step))
+
+
+def deliver(precision, n_src, dmin, D, Q, hist, hist_cnt, step, bstart, post, wmode, w, wscalar, target, events):
+    call("hs_deliver", precision, n_src, dmin, D, Q, hist.ptr, hist_cnt.ptr, int(step), bstart.ptr, post.ptr,
+         wmode, (w.ptr if w is not None else None), float(wscalar), target.ptr, events.ptr)
+
+
+# ---- networks -------------------------------------------------------------------
+
+class Net:
+    """A network on the device: NeuronGroups, pathways, monitors and the step loop.
+
+    precision: 64 or 32.  engine: "plain" (Brian2's in-place order, two launches
+    per step) or "fused" (one launch per step; effects applied at the start of the
+    next step).
+    """
+
+    def __init__(self, precision=64, engine="plain", dt=1e-4, seed=1234):
+        init()
+        self.precision = int(precision)
+        self.engine = {"plain": HS_ENGINE_PLAIN, "fused": HS_ENGINE_FUSED}[engine]
+        self.dt = float(dt)
+        self.seed = int(seed)
+        h = C.c_void_p()
+        call("hs_net_create", self.precision, self.engine, self.dt, self.seed, C.byref(h))
+        self.h = h
+        self.groups = []
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            _lib.load_library().hs_net_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_group(self, model, n, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        gid = C.c_int()
+        call("hs_net_add_group", self.h, int(model), int(n), _f64(p), p.size, C.byref(gid))
+        self.groups.append((model, n))
+        return gid.value
+
+    def set_state(self, gid, var, values):
+        n = self.groups[gid][1]
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), (n,)))
+        call("hs_net_set_state", self.h, gid, var, _f64(v))
+
+    def get_state(self, gid, var):
+        out = np.empty(self.groups[gid][1], np.float64)
+        call("hs_net_get_state", self.h, gid, var, _f64(out))
+        return out
+
+    def get_last_spike(self, gid):
+        out = np.empty(self.groups[gid][1], np.int32)
+        call("hs_net_get_last_spike", self.h, gid, _i32(out))
+        return out
+
+    def add_pathway(self, src, tgt, var, pre, post, delay_steps=None, scalar_delay=0, wmode=HS_W_SCALAR,
+                    weights=None, wscalar=0.0):
+        pre = np.ascontiguousarray(pre, dtype=np.int32)
+        post = np.ascontiguousarray(post, dtype=np.int32)
+        if pre.size != post.size:
+            raise ValueError("pre/post size mismatch")
+        d = None
+        if delay_steps is not None:
+            d = np.ascontiguousarray(delay_steps, dtype=np.int32)
+            if d.size != pre.size:
+                raise ValueError("delay size mismatch")
+        w = None
+        if wmode != HS_W_SCALAR:
+            w = np.ascontiguousarray(weights, dtype=np.float64)
+        pid = C.c_int()
+        call("hs_net_add_pathway", self.h, src, tgt, var, pre.size, _i32(pre), _i32(post),
+             (_i32(d) if d is not None else None), int(scalar_delay), int(wmode),
+             (_f64(w) if w is not None else None), float(wscalar), C.byref(pid))
+        return pid.value
+
+    def add_spike_monitor(self, gid, capacity=1 << 20):
+        mid = C.c_int()
+        call("hs_net_add_spike_monitor", self.h, gid, int(capacity), C.byref(mid))
+        return mid.value
+
+    def add_state_monitor(self, gid, var, start, stop):
+        mid = C.c_int()
+        call("hs_net_add_state_monitor", self.h, gid, var, int(start), int(stop), C.byref(mid))
+        return mid.value
+
+    def set_option(self, key, value):
+        call("hs_net_set_option", self.h, int(key), int(value))
+
+    def get_option(self, key):
+        v = C.c_longlong()
+        call("hs_net_get_option", self.h, int(key), C.byref(v))
+        return v.value
+
+    def run(self, nsteps):
+        call("hs_net_run", self.h, int(nsteps))
+
+    def stats(self):
+        s = HsStats()
+        call("hs_net_stats", self.h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in HsStats._fields_}
+
+    def mon_count(self, mid):
+        n = C.c_int64()
+        call("hs_net_mon_count", self.h, mid, C.byref(n))
+        return n.value
+
+    def spikes(self, mid):
+        """(i, step) of every recorded spike, sorted by (step, i)."""
+        n = self.mon_count(mid)
+        i = np.empty(max(n, 1), np.int32)
+        t = np.empty(max(n, 1), np.int64)
+        call("hs_net_get_spikes", self.h, mid, _i32(i), t.ctypes.data_as(C.POINTER(C.c_int64)), n)
+        return i[:n], t[:n]
+
+    def state_monitor(self, mid, width):
+        rows = self.mon_count(mid)
+        out = np.empty(max(rows * width, 1), np.float64)
+        call("hs_net_get_state_monitor", self.h, mid, _f64(out), rows * width)
+        return out[: rows * width].reshape(rows, width)

@@ -1,0 +1,195 @@
+/*
+ * hs.h -- C ABI of libhs, the MI355X (gfx950) runtime behind the
+ * `hip_standalone` Brian2 device.  Plain C types only: pointers, sizes, ints.
+ *
+ * Conventions
+ *   - every entry point returns HS_OK (0) or a negative HS_ERR_* code; the text
+ *     of the last error of the calling thread is returned by hs_last_error().
+ *     Nothing calls exit() (the reference's CUDA_SAFE_CALL does:
+ *     brian2cuda/brianlib/cuda_utils.h:20,77-104).
+ *   - "device pointer" arguments are HBM addresses from hs_alloc (or any HIP
+ *     allocation on the current device); "host" arguments are ordinary memory
+ *     owned by the caller.  The runtime owns the memory behind an hs_net.
+ *   - calls on one hs_net must come from one thread at a time.
+ *   - all work runs on one HIP stream per process (default priority); hs_sync()
+ *     waits for it.
+ */
+#ifndef HS_H
+#define HS_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "hs_models.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HS_OK 0
+#define HS_ERR_ARG (-1)         /* bad argument (index out of range, size, mode) */
+#define HS_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define HS_ERR_NOMEM (-3)       /* device or host allocation failed */
+#define HS_ERR_STATE (-4)       /* call not valid in the object's current state */
+#define HS_ERR_OVERFLOW (-5)    /* a monitor buffer was too small; see hs_net_mon_count */
+#define HS_ERR_UNSUPPORTED (-6) /* model / form combination not provided */
+
+/* ---- runtime ------------------------------------------------------------ */
+
+/* Text of the last error on this thread ("" if none). */
+const char *hs_last_error(void);
+
+/* ABI version, bumped on incompatible change. */
+int hs_version(void);
+
+/* Select and initialise a device; creates the runtime's stream.
+ * Replaces cudaSetDevice + heap-limit setup (brian2cuda/templates/main.cu:71-85). */
+int hs_init(int device);
+
+/* Device name, compute units and HBM bytes of the initialised device. */
+int hs_device_info(char *name, int name_cap, int *compute_units, long long *hbm_bytes);
+
+/* Memory: replaces the cudaMalloc/cudaMemcpy/cudaFree calls of _init_arrays /
+ * _dealloc_arrays (brian2cuda/templates/objects.cu:340-489,654-726). */
+int hs_alloc(size_t bytes, void **dev_ptr);
+int hs_free(void *dev_ptr);
+int hs_h2d(void *dev_dst, const void *host_src, size_t bytes);
+int hs_d2h(void *host_dst, const void *dev_src, size_t bytes);
+int hs_zero(void *dev_ptr, size_t bytes);
+int hs_sync(void);
+
+/* ---- individually testable kernels (device pointers) ---------------------- */
+
+/*
+ * Threshold + spikespace compaction over n neurons of state v (double if
+ * precision == HS_F64, else float): neuron i spikes iff v[i] > vth and
+ * (step - last[i]) >= ref_steps.  Spiking indices are written to spikes[0..*count)
+ * and last[i] = step for them.  sorted != 0 gives ascending order (the order
+ * cpp_standalone produces); sorted == 0 gives ascending order within each
+ * workgroup tile, tiles in completion order.
+ * Replaces threshold.cu:19-45 (+ the eventspace reset kernel :49-75 and the
+ * host memset :99-105: the count is written, no O(N) -1 fill).
+ * count: device int32[1].
+ */
+int hs_threshold_compact(int precision, const void *v, int n, double vth, int32_t *last,
+                         int64_t step, int ref_steps, int sorted, int32_t *spikes, int32_t *count);
+
+/*
+ * n standard normals out[i] = N(seed, stream, i, step) (Philox4x32-10 +
+ * Box-Muller, the deterministic math of the numerical contract); out is double
+ * or float by precision (float values are the double draws rounded).
+ * Replaces the cuRAND buffer refill (rand.cu:395-463, curand_buffer.h:20-160).
+ */
+int hs_randn(uint64_t seed, uint32_t stream, int64_t step, int n, int precision, void *out);
+
+/* Uniform [0,1) draws with the same counter layout (first 53-bit uniform). */
+int hs_rand(uint64_t seed, uint32_t stream, int64_t step, int n, void *out_f64);
+
+/*
+ * One state-update step of `model` for n neurons, in place: vars[v] are device
+ * pointers of the model's state variables (host array of nvars pointers),
+ * last the int32 last-spike steps (refractoriness), params the model's
+ * parameter list (hs_models.h).  No threshold, no reset.
+ * Replaces the generated _run_kernel_<stateupdater> (common_group.cu:108-160).
+ */
+int hs_stateupdate(int model, int precision, int n, void *const *vars, const int32_t *last,
+                   const double *params, int nparams, double dt, uint64_t seed, int64_t step);
+
+/*
+ * Synaptic effect application for one pathway at step t: for every delay
+ * d in [dmin, dmin+D) and every spike k of the spike list emitted at step t-d
+ * (hist[((t-d) mod Q)*n_src + k], count hist_cnt[(t-d) mod Q]), every synapse of
+ * the bundle (pre, d) -- post[bstart[pre*D+d-dmin] .. bstart[pre*D+d-dmin+1]) --
+ * adds its weight to target[post] with a hardware atomic.  events (device
+ * uint64[1]) is incremented by the number of synapses delivered.
+ * wmode HS_W_SCALAR uses wscalar, HS_W_SYNAPSE w[synapse], HS_W_PRE w[pre].
+ * Replaces _run_kernel_<synapses_pre> (synapses.cu:12-164) together with the
+ * queue pop (spikequeue.h:504-516).
+ */
+int hs_deliver(int precision, int n_src, int dmin, int D, int Q, const int32_t *hist,
+               const int32_t *hist_cnt, int64_t step, const int32_t *bstart, const int32_t *post,
+               int wmode, const void *w, double wscalar, void *target, uint64_t *events);
+
+/*
+ * Device evaluation of the deterministic math (for bit-equality checks):
+ * fn 0 exp, 1 log, 2 sqrt, 3 cos(2 pi x), 4 expf (x and y read/written as double).
+ */
+int hs_math_eval(int fn, int n, const double *x, double *y);
+
+/* ---- networks: the per-timestep engine ------------------------------------ */
+
+typedef struct hs_net hs_net;
+
+/*
+ * Create an empty network.  precision HS_F64 / HS_F32; engine HS_ENGINE_PLAIN
+ * (state updated in place by the effects, Brian2's order, two launches per
+ * step) or HS_ENGINE_FUSED (effects accumulated and applied at the start of the
+ * next step; one launch per step when every delay is >= 1 step).
+ * Replaces the generated objects.cu/network.cu pair (objects.cu:340-489,
+ * network.cu:40-137).
+ */
+int hs_net_create(int precision, int engine, double dt, uint64_t seed, hs_net **out);
+void hs_net_destroy(hs_net *net);
+
+/* Add a NeuronGroup of n neurons with a model of hs_models.h. */
+int hs_net_add_group(hs_net *net, int model, int n, const double *params, int nparams, int *gid);
+
+/* Set / get a state variable (host arrays of n doubles). */
+int hs_net_set_state(hs_net *net, int gid, int var, const double *values);
+int hs_net_get_state(hs_net *net, int gid, int var, double *values);
+/* last spike step per neuron (HS_NEVER_SPIKED if none), host int32[n] */
+int hs_net_get_last_spike(hs_net *net, int gid, int32_t *steps);
+
+/*
+ * Add a pathway (Synapses ... on_pre='var += w') from group src to group tgt.
+ * pre/post: host int32[nsyn] (indices within the groups); delay_steps: host
+ * int32[nsyn] or NULL for the scalar delay scalar_delay (delays are already
+ * quantised: (int)(d/dt+0.5), synapses_push_spikes.cu:375).  The bundle table
+ * (synapses grouped by (pre, delay), synapse order kept within a bundle) is
+ * built here by a host counting sort and copied once.
+ * Replaces _before_run_<push_spikes> (synapses_push_spikes.cu:171-993).
+ */
+int hs_net_add_pathway(hs_net *net, int src, int tgt, int var, int64_t nsyn, const int32_t *pre,
+                       const int32_t *post, const int32_t *delay_steps, int scalar_delay, int wmode,
+                       const double *weights, double wscalar, int *pid);
+
+/* SpikeMonitor on a group: capacity = device records kept (i, step). */
+int hs_net_add_spike_monitor(hs_net *net, int gid, int64_t capacity, int *mid);
+/* StateMonitor (when='start') of var over neurons [start, stop). */
+int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start, int stop, int *mid);
+
+/* Tuning knobs (autotuner): key 0 update block size, 1 delivery blocks,
+ * 2 steps per captured graph (0: no graph), 3 fused-kernel grid size. */
+int hs_net_set_option(hs_net *net, int key, long long value);
+int hs_net_get_option(hs_net *net, int key, long long *value);
+
+/*
+ * Advance nsteps time steps.  Steps run in chunks captured as HIP graphs; no
+ * host synchronisation inside a chunk (counts stay on the device).  Pending
+ * effects and resets are applied at the end so that state reads see Brian2's
+ * post-step state.  Replaces Network::run (network.cu:40-137).
+ */
+int hs_net_run(hs_net *net, int64_t nsteps);
+
+/* Run statistics. */
+typedef struct {
+    int64_t steps;           /* steps simulated so far */
+    uint64_t events;         /* synaptic events delivered so far */
+    double last_run_wall_s;  /* wall time of the last hs_net_run (host clock) */
+    double last_run_gpu_ms;  /* device time of the last run's step chunks (HIP events) */
+    int64_t last_run_steps;
+    int64_t launches_per_step;
+} hs_stats;
+int hs_net_stats(hs_net *net, hs_stats *out);
+
+/* Number of records of a monitor (spikes, or recorded steps). */
+int hs_net_mon_count(hs_net *net, int mid, int64_t *n);
+/* Spike records sorted by (step, i); cap = room in the host arrays. */
+int hs_net_get_spikes(hs_net *net, int mid, int32_t *i, int64_t *step, int64_t cap);
+/* State samples, time-major [n_steps][stop-start], as double. */
+int hs_net_get_state_monitor(hs_net *net, int mid, double *out, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HS_H */

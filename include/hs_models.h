@@ -1,0 +1,74 @@
+/*
+ * hs_models.h -- model identifiers, state-variable slots and parameter layouts
+ * shared by the C-ABI (hs.h) and every caller.  Plain C, no device code.
+ *
+ * Each model is one Brian2 NeuronGroup equation set that the reference's code
+ * generator would turn into a state-update kernel (brian2cuda/templates/
+ * common_group.cu:108-160 via stateupdate.cu:1-3), a thresholder
+ * (threshold.cu:19-45) and a resetter (reset.cu:3-27).  Parameters are SI
+ * values as Brian2 stores them (volt, second, siemens, farad).
+ */
+#ifndef HS_MODELS_H
+#define HS_MODELS_H
+
+/* precision of state, weights and arithmetic */
+#define HS_F64 64
+#define HS_F32 32
+
+/* step-loop forms */
+#define HS_ENGINE_PLAIN 0 /* per-phase kernels, effects applied to the state in place */
+#define HS_ENGINE_FUSED 1 /* one launch per step, effects accumulated and applied next step */
+
+/* weight modes of a pathway */
+#define HS_W_SCALAR 0  /* one weight for all synapses (on_pre='v += w' with constant w) */
+#define HS_W_SYNAPSE 1 /* one weight per synapse */
+#define HS_W_PRE 2     /* one weight per presynaptic neuron (batched parameter sweeps) */
+
+/*
+ * HS_MODEL_LIF_CUBA -- examples/cuba.py:122-133, method='exact':
+ *   dv/dt  = (ge+gi-(v-El))/taum : volt (unless refractory)
+ *   dge/dt = -ge/taue ; dgi/dt = -gi/taui
+ *   threshold v>Vt, reset v=Vr, refractory
+ * params: El, Vt, Vr, taum, taue, taui, refractory
+ */
+#define HS_MODEL_LIF_CUBA 1
+#define HS_LIF_CUBA_NVARS 3
+#define HS_LIF_CUBA_NPARAMS 7
+#define HS_V 0
+#define HS_GE 1
+#define HS_GI 2
+
+/*
+ * HS_MODEL_LIF_NOISE -- examples/brunelhakim.py:155-160, method='euler':
+ *   dV/dt = (-V + mu + sigma*sqrt(tau)*xi)/tau : volt
+ *   threshold V>theta, reset V=Vr, refractory
+ * params: tau, mu, sigma, theta, Vr, refractory, noise_stream
+ */
+#define HS_MODEL_LIF_NOISE 2
+#define HS_LIF_NOISE_NVARS 1
+#define HS_LIF_NOISE_NPARAMS 7
+
+/*
+ * HS_MODEL_HH_COBA -- examples/cobahh.py:160-185 / brian2cuda/tests/features/
+ * speed.py:73-113, method='exponential_euler':
+ *   Hodgkin-Huxley (Traub-Miles rates) with conductance synapses ge, gi
+ *   threshold v>v_th, refractory (no reset)
+ * vars: v, m, n, h, ge, gi
+ * params: Cm, gl, El, EK, ENa, gNa, gK, VT, Ee, Ei, taue, taui, v_th, refractory
+ */
+#define HS_MODEL_HH_COBA 3
+#define HS_HH_NVARS 6
+#define HS_HH_NPARAMS 14
+#define HS_HH_V 0
+#define HS_HH_M 1
+#define HS_HH_N 2
+#define HS_HH_H 3
+#define HS_HH_GE 4
+#define HS_HH_GI 5
+
+#define HS_MAX_VARS 6
+
+/* "never spiked" marker for the integer last-spike step */
+#define HS_NEVER_SPIKED (-1073741824)
+
+#endif /* HS_MODELS_H */

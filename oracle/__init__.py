@@ -1,0 +1,219 @@
+"""The oracle: ctypes wrapper of libcpuref.so (oracle/cpu_ref.c).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke() and by
+bench.py's CPU-baseline leg, always as the checker; the product (brian2cuda_amd)
+never imports it.  Build with `make -C oracle` (done by __graft_entry__.build()).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcpuref.so")
+
+SEQ, ACC = 0, 1
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    U32P = C.POINTER(C.c_uint32)
+    I32P = C.POINTER(C.c_int32)
+    I64P = C.POINTER(C.c_int64)
+    DP = C.POINTER(C.c_double)
+    V = C.c_void_p
+    sig = {
+        "ref_philox4x32_10": (None, [U32P, U32P, U32P]),
+        "ref_randn": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64]),
+        "ref_rand": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64]),
+        "ref_quantize_delay": (C.c_int, [C.c_double, C.c_double]),
+        "ref_timestep": (C.c_longlong, [C.c_double, C.c_double]),
+        "ref_exp": (C.c_double, [C.c_double]),
+        "ref_expf": (C.c_float, [C.c_float]),
+        "ref_log": (C.c_double, [C.c_double]),
+        "ref_sqrt": (C.c_double, [C.c_double]),
+        "ref_cos2pi": (C.c_double, [C.c_double]),
+        "ref_model_constants": (C.c_int, [C.c_int, DP, C.c_double, DP]),
+        "ref_stateupdate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(DP), I32P, DP, C.c_double,
+                                      C.c_uint64, C.c_int64]),
+        "ref_net_create": (V, [C.c_int, C.c_int, C.c_double, C.c_uint64]),
+        "ref_net_add_group": (C.c_int, [V, C.c_int, C.c_int, DP, C.c_int]),
+        "ref_net_set_state": (C.c_int, [V, C.c_int, C.c_int, DP]),
+        "ref_net_get_state": (C.c_int, [V, C.c_int, C.c_int, DP]),
+        "ref_net_get_lastspike": (C.c_int, [V, C.c_int, DP]),
+        "ref_net_add_pathway": (C.c_int, [V, C.c_int, C.c_int, C.c_int, C.c_int64, I32P, I32P, I32P, C.c_int,
+                                          C.c_int, DP, C.c_double]),
+        "ref_net_add_spike_monitor": (C.c_int, [V, C.c_int]),
+        "ref_net_add_state_monitor": (C.c_int, [V, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "ref_net_run": (C.c_int, [V, C.c_int64]),
+        "ref_net_step": (C.c_int64, [V]),
+        "ref_net_events": (C.c_uint64, [V]),
+        "ref_mon_count": (C.c_int64, [V, C.c_int]),
+        "ref_mon_get_spikes": (C.c_int, [V, C.c_int, I32P, I64P]),
+        "ref_mon_get_state": (C.c_int, [V, C.c_int, DP]),
+        "ref_net_destroy": (None, [V]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().ref_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+def randn(seed, stream, index, step):
+    return lib().ref_randn(seed, stream, index, step)
+
+
+def model_constants(model, params, dt):
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    k = np.zeros(16)
+    n = lib().ref_model_constants(model, p.ctypes.data_as(C.POINTER(C.c_double)), dt,
+                                  k.ctypes.data_as(C.POINTER(C.c_double)))
+    return k[:n]
+
+
+def stateupdate(model, precision, arrays, last, params, dt, seed, step):
+    """One oracle state-update step; arrays (float64 numpy, modified in place)."""
+    arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    DP = C.POINTER(C.c_double)
+    ptrs = (DP * len(arrays))(*[a.ctypes.data_as(DP) for a in arrays])
+    last = np.ascontiguousarray(last, dtype=np.int32)
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    r = lib().ref_stateupdate(model, precision, arrays[0].size, ptrs, last.ctypes.data_as(C.POINTER(C.c_int32)),
+                              p.ctypes.data_as(DP), dt, seed, step)
+    if r < 0:
+        raise ValueError("bad model")
+    return arrays
+
+
+class RefNet:
+    """Serial cpp_standalone-semantics network (SEQ) or the fused form's semantics (ACC)."""
+
+    def __init__(self, precision=64, semantics=SEQ, dt=1e-4, seed=1234):
+        self.L = lib()
+        self.h = self.L.ref_net_create(precision, semantics, dt, seed)
+        if not self.h:
+            raise ValueError("bad precision")
+        self.sizes = []
+
+    def close(self):
+        if self.h:
+            self.L.ref_net_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_group(self, model, n, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        g = self.L.ref_net_add_group(self.h, model, n, p.ctypes.data_as(C.POINTER(C.c_double)), p.size)
+        if g < 0:
+            raise ValueError("bad group")
+        self.sizes.append(n)
+        return g
+
+    def set_state(self, gid, var, values):
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), (self.sizes[gid],)))
+        if self.L.ref_net_set_state(self.h, gid, var, v.ctypes.data_as(C.POINTER(C.c_double))) < 0:
+            raise ValueError("bad state")
+
+    def get_state(self, gid, var):
+        out = np.empty(self.sizes[gid])
+        if self.L.ref_net_get_state(self.h, gid, var, out.ctypes.data_as(C.POINTER(C.c_double))) < 0:
+            raise ValueError("bad state")
+        return out
+
+    def get_lastspike(self, gid):
+        out = np.empty(self.sizes[gid])
+        self.L.ref_net_get_lastspike(self.h, gid, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def add_pathway(self, src, tgt, var, pre, post, delay_steps=None, scalar_delay=0, wmode=0, weights=None,
+                    wscalar=0.0):
+        I32P = C.POINTER(C.c_int32)
+        pre = np.ascontiguousarray(pre, dtype=np.int32)
+        post = np.ascontiguousarray(post, dtype=np.int32)
+        d = None if delay_steps is None else np.ascontiguousarray(delay_steps, dtype=np.int32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        r = self.L.ref_net_add_pathway(self.h, src, tgt, var, pre.size, pre.ctypes.data_as(I32P),
+                                       post.ctypes.data_as(I32P), None if d is None else d.ctypes.data_as(I32P),
+                                       int(scalar_delay), int(wmode),
+                                       None if w is None else w.ctypes.data_as(C.POINTER(C.c_double)),
+                                       float(wscalar))
+        if r < 0:
+            raise ValueError(f"bad pathway ({r})")
+        return r
+
+    def add_spike_monitor(self, gid):
+        return self.L.ref_net_add_spike_monitor(self.h, gid)
+
+    def add_state_monitor(self, gid, var, start, stop):
+        m = self.L.ref_net_add_state_monitor(self.h, gid, var, start, stop)
+        if m < 0:
+            raise ValueError("bad state monitor")
+        return m
+
+    def run(self, nsteps):
+        self.L.ref_net_run(self.h, int(nsteps))
+
+    @property
+    def step(self):
+        return self.L.ref_net_step(self.h)
+
+    @property
+    def events(self):
+        return self.L.ref_net_events(self.h)
+
+    def spikes(self, mid):
+        n = self.L.ref_mon_count(self.h, mid)
+        i = np.empty(max(n, 1), np.int32)
+        t = np.empty(max(n, 1), np.int64)
+        self.L.ref_mon_get_spikes(self.h, mid, i.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  t.ctypes.data_as(C.POINTER(C.c_int64)))
+        return i[:n], t[:n]
+
+    def state_monitor(self, mid, width):
+        rows = self.L.ref_mon_count(self.h, mid)
+        out = np.empty(max(rows * width, 1))
+        self.L.ref_mon_get_state(self.h, mid, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[: rows * width].reshape(rows, width)
+
+
+def build_ref(spec, precision=64, semantics=SEQ):
+    """Instantiate a brian2cuda_amd.networks.NetSpec on the oracle."""
+    net = RefNet(precision, semantics, spec.dt, spec.seed)
+    for g in spec.groups:
+        gid = net.add_group(g.model, g.n, g.params)
+        for var, val in g.init.items():
+            net.set_state(gid, var, val)
+    for p in spec.paths:
+        net.add_pathway(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode, p.weights,
+                        p.wscalar)
+    smons = [net.add_spike_monitor(g) for g in spec.spike_monitors]
+    stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
+    return net, smons, stmons

@@ -1,0 +1,437 @@
+/*
+ * cpu_ref.c -- THE ORACLE.  Test infrastructure only: tests/, the smoke check
+ * and bench.py's CPU-baseline leg load it as the checker; the product never
+ * links, loads or calls it.
+ *
+ * A serial C restatement of what Brian2's cpp_standalone computes per time step
+ * for the networks of the hot path, following the reference's templates:
+ *   schedule      network.cu:40-137 (objects called in schedule order, t = i*dt,
+ *                 clocks.h:27-56); slots start, groups, thresholds, synapses, resets
+ *   threshold     threshold.cu:19-45 -- but the spike list is produced in index
+ *                 order, as cpp_standalone's serial loop produces it
+ *                 (docs_sphinx/introduction/known_issues.rst:106-113)
+ *   refractory    not_refractory = timestep(t-lastspike,dt) >= timestep(ref,dt)
+ *                 (cuda_generator.py:997-1006); lastspike = t on a spike
+ *                 (threshold.cu:34-41)
+ *   reset         reset.cu:3-27 (spiking neurons only, after the synapses slot)
+ *   delays        delay steps = (int)(d/dt + 0.5) (synapses_push_spikes.cu:375)
+ *   spike queue   spikequeue.h:432-516: Q = max_delay+1 slots, a spike pushes each
+ *                 of its synapses into slot (offset+delay)%Q, the current slot is
+ *                 delivered in push order, cleared, and offset advances
+ *   effects       synapses.cu:59-161: target[post] += w, serially, in queue order
+ *   monitors      spikemonitor.cu:51-196 (i, t of every spike after thresholds),
+ *                 statemonitor.cu:74-100 (when='start': before the state update)
+ *
+ * The per-neuron update expressions are the shared numerical contract
+ * (brian2cuda_amd/csrc/hs_model_body.inc); they are pinned independently of
+ * the kernels by closed forms in tests/test_oracle.py.
+ *
+ * Two semantics are restated:
+ *   SEQ (0): Brian2's -- effects are applied to the state during delivery.
+ *   ACC (1): the fused form's -- effects of step t are summed into an
+ *            accumulator and added to the state at the start of step t+1,
+ *            before the (deferred) reset of step t's spikers.
+ * Two precisions: all state and arithmetic in double (64) or float (32).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+#include "../include/hs_models.h"
+#include "../brian2cuda_amd/csrc/hs_model_exprs.h"
+#include "cpu_ref.h"
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10, written from Salmon et al. (SC'11), independent of the     */
+/* product's header; pinned by the Random123 known-answer vectors.           */
+/* ------------------------------------------------------------------------ */
+void ref_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out)
+{
+    uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    int round;
+    for (round = 0; round < 10; ++round) {
+        uint64_t a = (uint64_t)0xD2511F53u * x0;
+        uint64_t b = (uint64_t)0xCD9E8D57u * x2;
+        uint32_t y0 = (uint32_t)(b >> 32) ^ x1 ^ k0;
+        uint32_t y1 = (uint32_t)b;
+        uint32_t y2 = (uint32_t)(a >> 32) ^ x3 ^ k1;
+        uint32_t y3 = (uint32_t)a;
+        x0 = y0; x1 = y1; x2 = y2; x3 = y3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+static double ref_u53(uint32_t a, uint32_t b)
+{
+    uint64_t k = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+    return ldexp((double)k, -53);
+}
+
+double ref_randn(uint64_t seed, uint32_t stream, uint32_t index, int64_t step)
+{
+    uint32_t c[4] = {index, (uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32), stream};
+    uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t w[4];
+    ref_philox4x32_10(c, k, w);
+    double u1 = ref_u53(w[0], w[1]) + ldexp(1.0, -53);
+    double u2 = ref_u53(w[2], w[3]);
+    return hs_box_muller(u1, u2);
+}
+
+double ref_rand(uint64_t seed, uint32_t stream, uint32_t index, int64_t step)
+{
+    uint32_t c[4] = {index, (uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32), stream};
+    uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t w[4];
+    ref_philox4x32_10(c, k, w);
+    return ref_u53(w[0], w[1]);
+}
+
+int ref_quantize_delay(double delay, double dt) { return (int)(delay / dt + 0.5); }
+
+long long ref_timestep(double t, double dt) { return hs_timestep(t, dt); }
+
+double ref_exp(double x) { return hs_exp(x); }
+float ref_expf(float x) { return hs_expf(x); }
+double ref_log(double x) { return hs_log(x); }
+double ref_sqrt(double x) { return hs_sqrt(x); }
+double ref_cos2pi(double u) { return hs_cos2pi(u); }
+
+int ref_model_constants(int model, const double *params, double dt, double *k)
+{
+    return hs_model_constants(model, params, dt, k);
+}
+
+/* ------------------------------------------------------------------------ */
+/* network                                                                   */
+/* ------------------------------------------------------------------------ */
+#define REF_MAX_GROUPS 16
+#define REF_MAX_PATHS 32
+#define REF_MAX_MONS 32
+
+typedef struct {
+    int model, n, nvars;
+    double p[16];
+    double k[16];
+    int nk;
+    double vth, vreset, refractory;
+    int has_reset;
+    int64_t ref_steps;
+    uint32_t stream;
+    void *var[HS_MAX_VARS];
+    void *acc[HS_MAX_VARS];
+    double *lastspike;   /* seconds, Brian2's representation */
+    int32_t *spk;        /* this step's spikes, ascending */
+    int64_t nspk;
+    int32_t *pend;       /* spikes whose reset is pending (ACC) */
+    int64_t npend;
+} ref_group;
+
+typedef struct {
+    int src, tgt, var;
+    int64_t nsyn;
+    int32_t *pre, *post, *delay;
+    int64_t *pre_start; /* CSR by presynaptic neuron, synapse order kept */
+    int64_t *by_pre;
+    int wmode;
+    double wscalar;
+    void *w; /* REAL[nsyn] or REAL[n_src] */
+    int Q, offset;
+    int64_t **q;
+    int64_t *qlen, *qcap;
+} ref_path;
+
+typedef struct {
+    int kind; /* 0 spikes, 1 state */
+    int gid, var, start, stop;
+    int64_t n, cap;
+    int32_t *i;
+    int64_t *step;
+    double *data;
+} ref_mon;
+
+struct ref_net {
+    int prec, sem;
+    double dt;
+    uint64_t seed;
+    int64_t step;
+    uint64_t events;
+    int ng, np, nm;
+    ref_group g[REF_MAX_GROUPS];
+    ref_path pw[REF_MAX_PATHS];
+    ref_mon mon[REF_MAX_MONS];
+};
+
+static size_t rsz(const ref_net *net) { return net->prec == HS_F64 ? sizeof(double) : sizeof(float); }
+
+static double rget(const ref_net *net, const void *a, int64_t i)
+{
+    return net->prec == HS_F64 ? ((const double *)a)[i] : (double)((const float *)a)[i];
+}
+
+ref_net *ref_net_create(int precision, int semantics, double dt, uint64_t seed)
+{
+    if (precision != HS_F64 && precision != HS_F32) return NULL;
+    ref_net *net = (ref_net *)calloc(1, sizeof(ref_net));
+    net->prec = precision;
+    net->sem = semantics;
+    net->dt = dt;
+    net->seed = seed;
+    return net;
+}
+
+static int model_nvars(int model)
+{
+    if (model == HS_MODEL_LIF_CUBA) return HS_LIF_CUBA_NVARS;
+    if (model == HS_MODEL_LIF_NOISE) return HS_LIF_NOISE_NVARS;
+    if (model == HS_MODEL_HH_COBA) return HS_HH_NVARS;
+    return -1;
+}
+
+int ref_net_add_group(ref_net *net, int model, int n, const double *params, int nparams)
+{
+    if (net->ng >= REF_MAX_GROUPS || n <= 0) return -1;
+    int nv = model_nvars(model);
+    if (nv < 0 || nparams > 16) return -1;
+    ref_group *g = &net->g[net->ng];
+    memset(g, 0, sizeof(*g));
+    g->model = model;
+    g->n = n;
+    g->nvars = nv;
+    memcpy(g->p, params, sizeof(double) * nparams);
+    g->nk = hs_model_constants(model, params, net->dt, g->k);
+    if (model == HS_MODEL_LIF_CUBA) {
+        g->vth = params[1]; g->vreset = params[2]; g->refractory = params[6]; g->has_reset = 1;
+    } else if (model == HS_MODEL_LIF_NOISE) {
+        g->vth = params[3]; g->vreset = params[4]; g->refractory = params[5]; g->has_reset = 1;
+        g->stream = (uint32_t)params[6];
+    } else {
+        g->vth = params[12]; g->refractory = params[13]; g->has_reset = 0;
+    }
+    g->ref_steps = hs_timestep(g->refractory, net->dt);
+    for (int v = 0; v < nv; ++v) g->var[v] = calloc((size_t)n, rsz(net));
+    g->lastspike = (double *)malloc(sizeof(double) * n);
+    for (int i = 0; i < n; ++i) g->lastspike[i] = -1e4; /* never spiked */
+    g->spk = (int32_t *)malloc(sizeof(int32_t) * n);
+    g->pend = (int32_t *)malloc(sizeof(int32_t) * n);
+    return net->ng++;
+}
+
+int ref_net_set_state(ref_net *net, int gid, int var, const double *values)
+{
+    if (gid < 0 || gid >= net->ng || var < 0 || var >= net->g[gid].nvars) return -1;
+    ref_group *g = &net->g[gid];
+    for (int i = 0; i < g->n; ++i) {
+        if (net->prec == HS_F64) ((double *)g->var[var])[i] = values[i];
+        else ((float *)g->var[var])[i] = (float)values[i];
+    }
+    return 0;
+}
+
+int ref_net_get_state(ref_net *net, int gid, int var, double *out)
+{
+    if (gid < 0 || gid >= net->ng || var < 0 || var >= net->g[gid].nvars) return -1;
+    ref_group *g = &net->g[gid];
+    for (int i = 0; i < g->n; ++i) out[i] = rget(net, g->var[var], i);
+    return 0;
+}
+
+int ref_net_get_lastspike(ref_net *net, int gid, double *out)
+{
+    if (gid < 0 || gid >= net->ng) return -1;
+    memcpy(out, net->g[gid].lastspike, sizeof(double) * net->g[gid].n);
+    return 0;
+}
+
+int ref_net_add_pathway(ref_net *net, int src, int tgt, int var, int64_t nsyn, const int32_t *pre,
+                        const int32_t *post, const int32_t *delay_steps, int scalar_delay,
+                        int wmode, const double *weights, double wscalar)
+{
+    if (net->np >= REF_MAX_PATHS || src < 0 || src >= net->ng || tgt < 0 || tgt >= net->ng) return -1;
+    ref_group *gs = &net->g[src], *gt = &net->g[tgt];
+    if (var < 0 || var >= gt->nvars) return -1;
+    ref_path *p = &net->pw[net->np];
+    memset(p, 0, sizeof(*p));
+    p->src = src; p->tgt = tgt; p->var = var; p->nsyn = nsyn;
+    p->pre = (int32_t *)malloc(sizeof(int32_t) * (nsyn ? nsyn : 1));
+    p->post = (int32_t *)malloc(sizeof(int32_t) * (nsyn ? nsyn : 1));
+    p->delay = (int32_t *)malloc(sizeof(int32_t) * (nsyn ? nsyn : 1));
+    int maxd = 0;
+    for (int64_t s = 0; s < nsyn; ++s) {
+        if (pre[s] < 0 || pre[s] >= gs->n || post[s] < 0 || post[s] >= gt->n) return -2;
+        p->pre[s] = pre[s];
+        p->post[s] = post[s];
+        p->delay[s] = delay_steps ? delay_steps[s] : scalar_delay;
+        if (p->delay[s] < 0) return -3;
+        if (p->delay[s] > maxd) maxd = p->delay[s];
+    }
+    /* CSR by pre, keeping synapse order (counting sort, stable) */
+    p->pre_start = (int64_t *)calloc((size_t)gs->n + 1, sizeof(int64_t));
+    for (int64_t s = 0; s < nsyn; ++s) p->pre_start[pre[s] + 1]++;
+    for (int i = 0; i < gs->n; ++i) p->pre_start[i + 1] += p->pre_start[i];
+    p->by_pre = (int64_t *)malloc(sizeof(int64_t) * (nsyn ? nsyn : 1));
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * gs->n);
+    memcpy(fill, p->pre_start, sizeof(int64_t) * gs->n);
+    for (int64_t s = 0; s < nsyn; ++s) p->by_pre[fill[pre[s]]++] = s;
+    free(fill);
+    p->wmode = wmode;
+    p->wscalar = wscalar;
+    int64_t nw = wmode == HS_W_SYNAPSE ? nsyn : (wmode == HS_W_PRE ? gs->n : 0);
+    if (nw > 0) {
+        p->w = malloc(rsz(net) * nw);
+        for (int64_t s = 0; s < nw; ++s) {
+            if (net->prec == HS_F64) ((double *)p->w)[s] = weights[s];
+            else ((float *)p->w)[s] = (float)weights[s];
+        }
+    }
+    p->Q = maxd + 1;
+    p->offset = 0;
+    p->q = (int64_t **)calloc(p->Q, sizeof(int64_t *));
+    p->qlen = (int64_t *)calloc(p->Q, sizeof(int64_t));
+    p->qcap = (int64_t *)calloc(p->Q, sizeof(int64_t));
+    if (!gt->acc[var]) gt->acc[var] = calloc((size_t)gt->n, rsz(net));
+    return net->np++;
+}
+
+int ref_net_add_spike_monitor(ref_net *net, int gid)
+{
+    if (net->nm >= REF_MAX_MONS || gid < 0 || gid >= net->ng) return -1;
+    ref_mon *m = &net->mon[net->nm];
+    memset(m, 0, sizeof(*m));
+    m->kind = 0;
+    m->gid = gid;
+    return net->nm++;
+}
+
+int ref_net_add_state_monitor(ref_net *net, int gid, int var, int start, int stop)
+{
+    if (net->nm >= REF_MAX_MONS || gid < 0 || gid >= net->ng) return -1;
+    if (var < 0 || var >= net->g[gid].nvars || start < 0 || stop > net->g[gid].n || stop <= start) return -1;
+    ref_mon *m = &net->mon[net->nm];
+    memset(m, 0, sizeof(*m));
+    m->kind = 1;
+    m->gid = gid; m->var = var; m->start = start; m->stop = stop;
+    return net->nm++;
+}
+
+static void mon_spikes(ref_net *net, ref_mon *m, const ref_group *g)
+{
+    if (m->n + g->nspk > m->cap) {
+        int64_t c = m->cap ? m->cap : 1024;
+        while (c < m->n + g->nspk) c *= 2;
+        m->i = (int32_t *)realloc(m->i, sizeof(int32_t) * c);
+        m->step = (int64_t *)realloc(m->step, sizeof(int64_t) * c);
+        m->cap = c;
+    }
+    for (int64_t k = 0; k < g->nspk; ++k) {
+        m->i[m->n] = g->spk[k];
+        m->step[m->n] = net->step;
+        m->n++;
+    }
+}
+
+static void mon_state(ref_net *net, ref_mon *m, const ref_group *g)
+{
+    int64_t w = m->stop - m->start;
+    if ((m->n + 1) * w > m->cap) {
+        int64_t c = m->cap ? m->cap : 1024 * w;
+        while (c < (m->n + 1) * w) c *= 2;
+        m->data = (double *)realloc(m->data, sizeof(double) * c);
+        m->cap = c;
+    }
+    for (int64_t j = 0; j < w; ++j) m->data[m->n * w + j] = rget(net, g->var[m->var], m->start + j);
+    m->n++;
+}
+
+static void queue_push(ref_path *p, int slot, int64_t syn)
+{
+    if (p->qlen[slot] == p->qcap[slot]) {
+        int64_t c = p->qcap[slot] ? 2 * p->qcap[slot] : 256;
+        p->q[slot] = (int64_t *)realloc(p->q[slot], sizeof(int64_t) * c);
+        p->qcap[slot] = c;
+    }
+    p->q[slot][p->qlen[slot]++] = syn;
+}
+
+/* the per-precision body: update groups, push/deliver, reset */
+#define R double
+#define RSFX(name) name##_f64
+#include "cpu_ref_step.inc"
+#undef R
+#undef RSFX
+#define R float
+#define RSFX(name) name##_f32
+#include "cpu_ref_step.inc"
+#undef R
+#undef RSFX
+
+int ref_stateupdate(int model, int precision, int n, double **vars, const int32_t *last, const double *params,
+                    double dt, uint64_t seed, int64_t step)
+{
+    if (model_nvars(model) < 0) return -1;
+    if (precision == HS_F64) return ref_stateupdate_f64(model, n, vars, last, params, dt, seed, step);
+    if (precision == HS_F32) return ref_stateupdate_f32(model, n, vars, last, params, dt, seed, step);
+    return -1;
+}
+
+int ref_net_run(ref_net *net, int64_t nsteps)
+{
+    for (int64_t s = 0; s < nsteps; ++s) {
+        if (net->prec == HS_F64) ref_step_f64(net);
+        else ref_step_f32(net);
+    }
+    if (net->sem == 1) {
+        if (net->prec == HS_F64) ref_flush_f64(net);
+        else ref_flush_f32(net);
+    }
+    return 0;
+}
+
+int64_t ref_net_step(ref_net *net) { return net->step; }
+uint64_t ref_net_events(ref_net *net) { return net->events; }
+
+int64_t ref_mon_count(ref_net *net, int mid)
+{
+    if (mid < 0 || mid >= net->nm) return -1;
+    return net->mon[mid].n;
+}
+
+int ref_mon_get_spikes(ref_net *net, int mid, int32_t *i, int64_t *step)
+{
+    if (mid < 0 || mid >= net->nm || net->mon[mid].kind != 0) return -1;
+    ref_mon *m = &net->mon[mid];
+    if (m->n) {
+        memcpy(i, m->i, sizeof(int32_t) * m->n);
+        memcpy(step, m->step, sizeof(int64_t) * m->n);
+    }
+    return 0;
+}
+
+int ref_mon_get_state(ref_net *net, int mid, double *out)
+{
+    if (mid < 0 || mid >= net->nm || net->mon[mid].kind != 1) return -1;
+    ref_mon *m = &net->mon[mid];
+    if (m->n) memcpy(out, m->data, sizeof(double) * m->n * (m->stop - m->start));
+    return 0;
+}
+
+void ref_net_destroy(ref_net *net)
+{
+    if (!net) return;
+    for (int g = 0; g < net->ng; ++g) {
+        for (int v = 0; v < HS_MAX_VARS; ++v) { free(net->g[g].var[v]); free(net->g[g].acc[v]); }
+        free(net->g[g].lastspike); free(net->g[g].spk); free(net->g[g].pend);
+    }
+    for (int p = 0; p < net->np; ++p) {
+        ref_path *pw = &net->pw[p];
+        free(pw->pre); free(pw->post); free(pw->delay); free(pw->pre_start); free(pw->by_pre); free(pw->w);
+        for (int q = 0; q < pw->Q; ++q) free(pw->q[q]);
+        free(pw->q); free(pw->qlen); free(pw->qcap);
+    }
+    for (int m = 0; m < net->nm; ++m) { free(net->mon[m].i); free(net->mon[m].step); free(net->mon[m].data); }
+    free(net);
+}

@@ -1,0 +1,49 @@
+/*
+ * cpu_ref.h -- C interface of the oracle (test infrastructure only).
+ * See cpu_ref.c for what each part restates.
+ */
+#ifndef CPU_REF_H
+#define CPU_REF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void ref_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+double ref_randn(uint64_t seed, uint32_t stream, uint32_t index, int64_t step);
+double ref_rand(uint64_t seed, uint32_t stream, uint32_t index, int64_t step);
+int ref_quantize_delay(double delay, double dt);
+long long ref_timestep(double t, double dt);
+double ref_exp(double x);
+float ref_expf(float x);
+double ref_log(double x);
+double ref_sqrt(double x);
+double ref_cos2pi(double u);
+int ref_model_constants(int model, const double *params, double dt, double *k);
+int ref_stateupdate(int model, int precision, int n, double **vars, const int32_t *last, const double *params,
+                    double dt, uint64_t seed, int64_t step);
+
+typedef struct ref_net ref_net;
+ref_net *ref_net_create(int precision, int semantics, double dt, uint64_t seed);
+int ref_net_add_group(ref_net *net, int model, int n, const double *params, int nparams);
+int ref_net_set_state(ref_net *net, int gid, int var, const double *values);
+int ref_net_get_state(ref_net *net, int gid, int var, double *out);
+int ref_net_get_lastspike(ref_net *net, int gid, double *out);
+int ref_net_add_pathway(ref_net *net, int src, int tgt, int var, int64_t nsyn, const int32_t *pre,
+                        const int32_t *post, const int32_t *delay_steps, int scalar_delay,
+                        int wmode, const double *weights, double wscalar);
+int ref_net_add_spike_monitor(ref_net *net, int gid);
+int ref_net_add_state_monitor(ref_net *net, int gid, int var, int start, int stop);
+int ref_net_run(ref_net *net, int64_t nsteps);
+int64_t ref_net_step(ref_net *net);
+uint64_t ref_net_events(ref_net *net);
+int64_t ref_mon_count(ref_net *net, int mid);
+int ref_mon_get_spikes(ref_net *net, int mid, int32_t *i, int64_t *step);
+int ref_mon_get_state(ref_net *net, int mid, double *out);
+void ref_net_destroy(ref_net *net);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
