@@ -32,10 +32,12 @@ class HsStats(C.Structure):
     _fields_ = [
         ("steps", C.c_int64),
         ("events", C.c_uint64),
+        ("spikes", C.c_uint64),
         ("last_run_wall_s", C.c_double),
         ("last_run_gpu_ms", C.c_double),
         ("last_run_steps", C.c_int64),
-        ("launches_per_step", C.c_int64),
+        ("launches_per_step", C.c_double),
+        ("window", C.c_int64),
     ]
 
 

@@ -30,7 +30,7 @@
 #include <math.h>
 
 #ifdef __HIP__
-#define HS_HD __host__ __device__ static inline
+#define HS_HD __host__ __device__ static inline __attribute__((always_inline))
 #else
 #define HS_HD static inline
 #endif

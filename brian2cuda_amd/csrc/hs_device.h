@@ -1,0 +1,348 @@
+/*
+ * hs_device.h -- device building blocks shared by the engine kernels and the
+ * standalone kernels: wave64 helpers, the per-neuron step, block compaction of
+ * the spike list, and the delivery of (spike, delay) bundles.
+ */
+#ifndef HS_DEVICE_H
+#define HS_DEVICE_H
+
+#include "hs_internal.h"
+#include "hs_model_exprs.h"
+#include "hs_philox.h"
+
+#define HS_WAVE 64
+
+__device__ __forceinline__ int hs_lane() { return threadIdx.x & (HS_WAVE - 1); }
+
+__device__ __forceinline__ unsigned long long hs_lanemask_lt() { return (1ull << hs_lane()) - 1ull; }
+
+/* exclusive prefix sum across the 64 lanes; *total gets the wave sum */
+__device__ __forceinline__ int hs_wave_excl_scan(int x, int *total)
+{
+    int inc = x;
+#pragma unroll
+    for (int o = 1; o < HS_WAVE; o <<= 1) {
+        int y = __shfl_up(inc, o, HS_WAVE);
+        if (hs_lane() >= o) inc += y;
+    }
+    *total = __shfl(inc, HS_WAVE - 1, HS_WAVE);
+    return inc - x;
+}
+
+__device__ __forceinline__ long long hs_mod(long long a, long long m)
+{
+    long long r = a % m;
+    return r < 0 ? r + m : r;
+}
+
+/* x mod m for x in (-m, 2m): ring offsets relative to a base already reduced */
+__device__ __forceinline__ int hs_wrap(int x, int m)
+{
+    if (x < 0) x += m;
+    else if (x >= m) x -= m;
+    return x;
+}
+
+template <typename R> struct HsOps;
+template <> struct HsOps<double> {
+    static __device__ __forceinline__ void cuba(double *v, double *ge, double *gi, int nr, const double *k)
+    {
+        hs_lif_cuba_update_f64(v, ge, gi, nr, k);
+    }
+    static __device__ __forceinline__ double noise(double V, double xi, const double *k)
+    {
+        return hs_lif_noise_update_f64(V, xi, k);
+    }
+    static __device__ __forceinline__ void hh(double *s, const double *k) { hs_hh_coba_update_f64(s, k); }
+};
+template <> struct HsOps<float> {
+    static __device__ __forceinline__ void cuba(float *v, float *ge, float *gi, int nr, const float *k)
+    {
+        hs_lif_cuba_update_f32(v, ge, gi, nr, k);
+    }
+    static __device__ __forceinline__ float noise(float V, float xi, const float *k)
+    {
+        return hs_lif_noise_update_f32(V, xi, k);
+    }
+    static __device__ __forceinline__ void hh(float *s, const float *k) { hs_hh_coba_update_f32(s, k); }
+};
+
+template <int MODEL> struct HsNV;
+template <> struct HsNV<HS_MODEL_LIF_CUBA> { static constexpr int value = HS_LIF_CUBA_NVARS; };
+template <> struct HsNV<HS_MODEL_LIF_NOISE> { static constexpr int value = HS_LIF_NOISE_NVARS; };
+template <> struct HsNV<HS_MODEL_HH_COBA> { static constexpr int value = HS_HH_NVARS; };
+
+/* the model's noise draw for neuron i at step t (0 for noiseless models) */
+template <typename R, int MODEL>
+__device__ __forceinline__ R hs_noise(unsigned long long seed, unsigned int stream, int i, long long t)
+{
+    if (MODEL == HS_MODEL_LIF_NOISE) return (R)hs_randn(seed, stream, (uint32_t)i, (int64_t)t);
+    return (R)0;
+}
+
+/* one neuron, one step with its noise draw xi; returns the threshold condition */
+template <typename R, int MODEL>
+__device__ __forceinline__ bool hs_neuron_step(R *s, const R *k, int not_ref, R vth, R xi)
+{
+    if (MODEL == HS_MODEL_LIF_CUBA) {
+        HsOps<R>::cuba(&s[0], &s[1], &s[2], not_ref, k);
+    } else if (MODEL == HS_MODEL_LIF_NOISE) {
+        s[0] = HsOps<R>::noise(s[0], xi, k);
+    } else {
+        HsOps<R>::hh(s, k);
+    }
+    return (s[0] > vth) && not_ref;
+}
+
+/* ------------------------------------------------------------------------ */
+/* block compaction: wave ballot + popcount, prefix over the block's waves,  */
+/* one atomic per block on the list's count (indices ascending in a block)   */
+/* ------------------------------------------------------------------------ */
+struct HsCompactLds {
+    int wave_off[16];
+    int base;
+    unsigned long long mbase;
+};
+
+__device__ __forceinline__ void hs_block_compact(bool pred, int idx, int32_t *list, int32_t *cnt, HsCompactLds *L,
+                                                 int32_t *mon_i, int32_t *mon_t, unsigned long long *mon_n,
+                                                 long long mon_cap, long long t, unsigned long long *spk_total)
+{
+    const int wid = threadIdx.x / HS_WAVE;
+    const int nw = blockDim.x / HS_WAVE;
+    unsigned long long m = __ballot(pred);
+    if (hs_lane() == 0) L->wave_off[wid] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < nw; ++w) {
+            int c = L->wave_off[w];
+            L->wave_off[w] = tot;
+            tot += c;
+        }
+        L->base = tot ? atomicAdd(cnt, tot) : 0;
+        if (tot && spk_total) atomicAdd(spk_total, (unsigned long long)tot);
+        L->mbase = (tot && mon_n) ? atomicAdd(mon_n, (unsigned long long)tot) : 0ull;
+    }
+    __syncthreads();
+    if (pred) {
+        int pos = L->wave_off[wid] + __popcll(m & hs_lanemask_lt());
+        list[L->base + pos] = idx;
+        if (mon_n) {
+            unsigned long long r = L->mbase + (unsigned long long)pos;
+            if ((long long)r < mon_cap) {
+                mon_i[r] = idx;
+                mon_t[r] = (int32_t)t;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+/* ------------------------------------------------------------------------ */
+/* delivery of the bundles due in steps T .. T+w-1                           */
+/* ------------------------------------------------------------------------ */
+struct HsDeliverLds {
+    int pref[HS_MAX_DELAY_SLOTS + 1];
+};
+
+/*
+ * Work items are (step j of the window, delay slot di, spike k of the list
+ * emitted at step T+j-dmin-di); their count per (j, di) comes from the spike
+ * ring and is prefix-summed in LDS.  The items are dealt out CH per wavefront,
+ * CH = the power of two that spreads them over every wave of the grid (1..64),
+ * so that a few hundred bundles keep as many waves busy as 100k would.  Lane
+ * l < CH resolves its bundle (pre, delay) to [s0, s0+len); a wave scan of len
+ * flattens the CH bundles into one synapse index space that the lanes walk 64
+ * at a time -- consecutive lanes read consecutive synapses of a bundle
+ * (coalesced post/weight reads) and each finds its bundle by a 6-step binary
+ * search over the scanned offsets (cross-lane reads).  Effects are hardware
+ * atomics (global_atomic_add_f64 / _f32) into the state (plain) or the
+ * accumulator slot of step T+j (fused).
+ * wave_rank / n_waves: this wave's index among the waves doing delivery.
+ */
+template <typename R, bool ACC>
+__device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, int w, HsDeliverLds *L,
+                                                long long wave_rank,
+                                long long n_waves, unsigned long long *ev_lane)
+{
+    const int D = p.D;
+    const int npairs = w * D;
+    const int tq = (int)hs_mod(T, p.Q);                 /* ring slot of step T */
+    const int ta = ACC ? (int)hs_mod(T, p.nacc) : 0;    /* accumulator slot of step T */
+    if (threadIdx.x < HS_WAVE) {
+        int carry = 0;
+        for (int c = 0; c < npairs; c += HS_WAVE) {
+            const int q = c + hs_lane();
+            int cnt = 0;
+            if (q < npairs) {
+                const int j = q / D, di = q - j * D;
+                cnt = p.hist_cnt[hs_wrap(tq + j - (p.dmin + di), p.Q)];
+            }
+            int tot;
+            const int ex = hs_wave_excl_scan(cnt, &tot);
+            if (q < npairs) L->pref[q] = carry + ex;
+            carry += tot;
+        }
+        if (hs_lane() == 0) L->pref[npairs] = carry;
+    }
+    __syncthreads();
+    const int W = L->pref[npairs];
+    long long per = ((long long)W + n_waves - 1) / n_waves;
+    int CH = 1;
+    while (CH < per && CH < HS_WAVE) CH <<= 1;
+    for (long long c = wave_rank; c * CH < W; c += n_waves) {
+        int len = 0, s0 = 0, pre = 0, jw = 0;
+        if (hs_lane() < CH) {
+            const long long wi = c * CH + hs_lane();
+            if (wi < W) {
+                int lo = 0, hi = npairs; /* pref[lo] <= wi < pref[hi] */
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (L->pref[mid] <= wi) lo = mid;
+                    else hi = mid;
+                }
+                const int kk = (int)(wi - L->pref[lo]);
+                jw = lo / D;
+                const int di = lo - jw * D;
+                const long long slot = hs_wrap(tq + jw - (p.dmin + di), p.Q);
+                pre = p.hist[slot * p.n_src + kk];
+                const long long b = (long long)pre * D + di;
+                s0 = p.bstart[b];
+                len = p.bstart[b + 1] - s0;
+            }
+        }
+        int tot;
+        const int off = hs_wave_excl_scan(len, &tot);
+        for (int j0 = 0; j0 < tot; j0 += HS_WAVE) {
+            const int j = j0 + hs_lane();
+            int lo = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int cand = lo + step;
+                const int oc = __shfl(off, cand, HS_WAVE);
+                if (oc <= j) lo = cand;
+            }
+            const int sb = __shfl(s0, lo, HS_WAVE);
+            const int ob = __shfl(off, lo, HS_WAVE);
+            const int pb = __shfl(pre, lo, HS_WAVE);
+            const int jb = __shfl(jw, lo, HS_WAVE);
+            if (j < tot) {
+                const int syn = sb + (j - ob);
+                const int tg = p.post[syn];
+                R wv;
+                if (p.wmode == HS_W_SCALAR) wv = p.wscalar;
+                else if (p.wmode == HS_W_SYNAPSE) wv = p.w[syn];
+                else wv = p.w[pb];
+                R *target = ACC ? p.acc + (long long)hs_wrap(ta + jb, p.nacc) * p.n_tgt : p.target;
+                unsafeAtomicAdd(&target[tg], wv);
+            }
+        }
+        *ev_lane += (unsigned long long)tot;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void hs_flush_events(unsigned long long ev, unsigned long long *events)
+{
+    /* every lane of a wave holds the wave's count; lane 0 publishes it */
+    if (hs_lane() == 0 && ev) atomicAdd(events, ev);
+}
+
+__device__ __forceinline__ unsigned long long hs_wave_sum(unsigned long long x)
+{
+#pragma unroll
+    for (int o = HS_WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, HS_WAVE);
+    return x;
+}
+
+/* ------------------------------------------------------------------------ */
+/* gather delivery of one target tile                                        */
+/* ------------------------------------------------------------------------ */
+struct HsGatherLds {
+    int pref[HS_MAX_DELAY_SLOTS + HS_MAX_WINDOW + 1];
+};
+
+/*
+ * The block owns targets [tile*HS_GTILE, +HS_GTILE) and a private LDS copy of
+ * their accumulators for the w steps of the window (fused) or of their state
+ * (plain, w == 1).  Every spike emitted in [T-dmax, T+w-1-dmin] can have
+ * synapses arriving in the window; for each, the block reads its (pre, tile)
+ * segment (on average out-degree / tiles entries, one cache line), keeps the
+ * entries whose arrival step u + delay falls in the window and adds their
+ * weights with LDS atomics (ds_add_f32 / ds_add_f64).  The tile is then written
+ * back with ordinary coalesced stores: no global atomics, no target conflicts
+ * between blocks.  Plain form: the LDS holds the state itself, so each event is
+ * state += w exactly as the serial loop does it.
+ */
+template <typename R, bool ACC>
+__device__ __forceinline__ void hs_gather_tile(const DPath<R> &p, int tile, long long T, int w, HsGatherLds *L, R *lacc,
+                               unsigned long long *ev_lane)
+{
+    const int base = tile * HS_GTILE;
+    const int ntl = min(HS_GTILE, p.n_tgt - base);
+    if (ACC) {
+        for (int i = threadIdx.x; i < w * HS_GTILE; i += blockDim.x) lacc[i] = (R)0;
+    } else {
+        for (int i = threadIdx.x; i < HS_GTILE; i += blockDim.x) lacc[i] = i < ntl ? p.target[base + i] : (R)0;
+    }
+    const int dmax = p.dmin + p.D - 1;
+    const int nU = p.D - 1 + w; /* candidate emission steps T-dmax .. T+w-1-dmin */
+    const int tq = (int)hs_mod(T, p.Q);
+    const int ta = ACC ? (int)hs_mod(T, p.nacc) : 0;
+    if (threadIdx.x < HS_WAVE) {
+        int carry = 0;
+        for (int c = 0; c < nU; c += HS_WAVE) {
+            const int q = c + hs_lane();
+            const int cnt = q < nU ? p.hist_cnt[hs_wrap(tq - dmax + q, p.Q)] : 0;
+            int tot;
+            const int ex = hs_wave_excl_scan(cnt, &tot);
+            if (q < nU) L->pref[q] = carry + ex;
+            carry += tot;
+        }
+        if (hs_lane() == 0) L->pref[nU] = carry;
+    }
+    __syncthreads();
+    const int total = L->pref[nU];
+    unsigned long long ev = 0;
+    for (int c = threadIdx.x; c < total; c += blockDim.x) {
+        int lo = 0, hi = nU; /* pref[lo] <= c < pref[hi] */
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (L->pref[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        const long long u = T - dmax + lo;
+        const int pre = p.hist[(long long)hs_wrap(tq - dmax + lo, p.Q) * p.n_src + (c - L->pref[lo])];
+        const long long gi = (long long)tile * p.n_src + pre; /* tile-major: a block stays in its tile's rows */
+        const int a = p.goff[gi], b = p.goff[gi + 1];
+        for (int s = a; s < b; ++s) {
+            const uint32_t e = p.gsyn[s];
+            const long long arr = u + (long long)(p.dmin + (int)(e >> 16)) - T;
+            if (arr >= 0 && arr < w) {
+                R wv;
+                if (p.wmode == HS_W_SCALAR) wv = p.wscalar;
+                else if (p.wmode == HS_W_SYNAPSE) wv = p.w[s];
+                else wv = p.w[pre];
+                atomicAdd(&lacc[(ACC ? (int)arr * HS_GTILE : 0) + (int)(e & 0xffffu)], wv);
+                ++ev;
+            }
+        }
+    }
+    *ev_lane += hs_wave_sum(ev);
+    __syncthreads();
+    if (ACC) {
+        for (int i = threadIdx.x; i < w * HS_GTILE; i += blockDim.x) {
+            const int j = i / HS_GTILE, l = i - j * HS_GTILE;
+            if (l < ntl) {
+                R *a = p.acc + (long long)hs_wrap(ta + j, p.nacc) * p.n_tgt + base + l;
+                *a = *a + lacc[i];
+            }
+        }
+    } else {
+        for (int i = threadIdx.x; i < ntl; i += blockDim.x) p.target[base + i] = lacc[i];
+    }
+    __syncthreads();
+}
+
+#endif /* HS_DEVICE_H */

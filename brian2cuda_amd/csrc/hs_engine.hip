@@ -1,0 +1,402 @@
+/*
+ * hs_engine.hip -- gfx950 kernels of the per-timestep engine.
+ *
+ *   update  : for every neuron, steps t .. t+w-1 with the state held in
+ *             registers: pending effects (fused form) and the deferred reset,
+ *             StateMonitor sample (when='start'), state update, threshold; then
+ *             the w spike lists are compacted at once (wave64 ballot/popcount,
+ *             one prefix over the block's waves, w parallel count reservations)
+ *             into the spike ring (threshold.cu:19-45, reset.cu:3-27,
+ *             common_group.cu:108-160, statemonitor.cu:74-100,
+ *             spikemonitor.cu:165-196 in one pass)
+ *   deliver : scatter-mode pathways -- effects of every (spike, delay) bundle due
+ *             in steps t .. t+w-1, global atomics (synapses.cu:12-164 with the
+ *             queue pop, spikequeue.h:504-516)
+ *   gather  : gather-mode pathways -- one block per target tile, LDS accumulation
+ *   fused   : update of step t and scatter delivery of step t in one launch on
+ *             disjoint blocks (every delay >= 1 step: the delivery reads only
+ *             spikes of earlier steps and writes only the accumulator of step t,
+ *             which the update reads one step later)
+ *   finalize: applies what is pending when a run ends
+ *
+ * Ring invariants (w <= W = the engine's window <= minimum delay):
+ *   accumulators: nacc = W + 1 slots; deliver/gather of window [T, T+w) write
+ *     slots T..T+w-1 while slot T-1 (written by the previous window) is still
+ *     unread -- W + 1 slots keep them apart.
+ *   spike ring: Q = dmax + 2W slots; an update launch clears the counts of steps
+ *     T+w .. T+w+W-1 (the next window's), which alias steps older than any the
+ *     current or next delivery reads.
+ */
+#include "hs_device.h"
+
+struct HsCompactLdsW {
+    int wave_cnt[HS_MAX_WINDOW][16];
+    int base[HS_MAX_WINDOW];
+    unsigned long long mbase[HS_MAX_WINDOW];
+};
+
+/* one step of one neuron inside the update window; returns the spike */
+template <typename R, int MODEL, bool FUSED>
+__device__ __forceinline__ bool hs_step_one(const DGroup<R> &g, R *s, const R *k, int32_t &last, long long t, int i,
+                                            long long no_reset_before, R xi)
+{
+    constexpr int NV = HsNV<MODEL>::value;
+    if (g.has_reset && (long long)last == t - 1 && t - 1 >= no_reset_before) s[0] = g.vr;
+    for (int m = 0; m < g.nst; ++m) {
+        if (i >= g.st_start[m] && i < g.st_stop[m]) {
+            const long long row = t - g.st_t0[m];
+            if (row >= 0 && row < g.st_rows[m]) {
+                R val = s[0];
+#pragma unroll
+                for (int v = 1; v < NV; ++v)
+                    if (v == g.st_var[m]) val = s[v];
+                g.st_buf[m][row * (long long)(g.st_stop[m] - g.st_start[m]) + (i - g.st_start[m])] = val;
+            }
+        }
+    }
+    const int not_ref = (t - (long long)last) >= (long long)g.ref_steps;
+    const bool spk = hs_neuron_step<R, MODEL>(s, k, not_ref, g.vth, xi);
+    if (spk) last = (int32_t)t;
+    return spk;
+}
+
+template <typename R, int MODEL, bool FUSED>
+__device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int tile, long long T, int w, int wmax,
+                                               long long no_reset_before,
+                               HsCompactLdsW *L, unsigned long long *spk_total)
+{
+    constexpr int NV = HsNV<MODEL>::value;
+    const int i = tile * blockDim.x + threadIdx.x;
+    const bool valid = i < g.n;
+    const int tq = (int)hs_mod(T, g.Q);  /* ring slot of step T */
+    /* ring slots of the steps after this launch start from an empty list */
+    if (tile == 0 && threadIdx.x == 0)
+        for (int m = 0; m < wmax; ++m) g.hist_cnt[hs_wrap(tq + w + m, g.Q)] = 0;
+    R s[NV];
+    int32_t last = HS_NEVER_SPIKED;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s[v] = valid ? g.var[v][i] : (R)0;
+    if (valid) last = g.last[i];
+    R k[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) k[j] = g.k[j];
+    unsigned int bits = 0; /* bit j: spike at step T+j */
+    if (valid) {
+        /* the noise of the whole window does not depend on the state: its w
+           draws are independent chains, evaluated together (instruction-level
+           parallelism) and consumed one per step */
+        constexpr bool NOISY = MODEL == HS_MODEL_LIF_NOISE;
+        R xi[NOISY ? HS_MAX_WINDOW : 1];
+        if (NOISY) {
+#pragma unroll
+            for (int j = 0; j < HS_MAX_WINDOW; ++j)
+                xi[j] = j < w ? hs_noise<R, MODEL>(g.seed, g.stream, i, T + j) : (R)0;
+        }
+        /* the effects due at steps T-1 .. T+w-2 were all delivered before this
+           launch: small models load the whole window's values at once (one
+           memory latency per window) and consume them through a shift register;
+           larger ones load one step ahead */
+        constexpr int PF = (FUSED && NV <= 3) ? HS_MAX_WINDOW : 1;
+        R pf[PF][NV];
+        const int a0 = FUSED ? (int)hs_mod(T - 1, g.nacc) : 0; /* accumulator slot of step T-1 */
+        if (FUSED) {
+#pragma unroll
+            for (int j = 0; j < PF; ++j) {
+                const long long sj = (long long)hs_wrap(a0 + (j < w ? j : 0), g.nacc) * g.n + i;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) pf[j][v] = (j < w && g.acc[v]) ? g.acc[v][sj] : (R)0;
+            }
+        }
+#pragma unroll 1
+        for (int j = 0; j < w; ++j) {
+            if (FUSED) {
+                R cur[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) cur[v] = pf[0][v];
+                const long long slot = (long long)hs_wrap(a0 + j, g.nacc) * g.n + i;
+                if (PF > 1) {
+#pragma unroll
+                    for (int m = 0; m + 1 < PF; ++m)
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) pf[m][v] = pf[m + 1][v];
+                } else if (j + 1 < w) {
+                    const long long sn = (long long)hs_wrap(a0 + j + 1, g.nacc) * g.n + i;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) pf[0][v] = g.acc[v] ? g.acc[v][sn] : (R)0;
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    if (g.acc[v]) {
+                        s[v] = s[v] + cur[v];
+                        g.acc[v][slot] = (R)0;
+                    }
+                }
+            }
+            R xj = (R)0;
+            if (NOISY) {
+                xj = xi[0];
+#pragma unroll
+                for (int m = 0; m + 1 < (NOISY ? HS_MAX_WINDOW : 1); ++m) xi[m] = xi[m + 1];
+            }
+            if (hs_step_one<R, MODEL, FUSED>(g, s, k, last, T + j, i, no_reset_before, xj)) bits |= 1u << j;
+        }
+    }
+    if (valid) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) g.var[v][i] = s[v];
+        g.last[i] = last;
+    }
+    /* compaction of the w spike lists: counts per wave, then w reservations */
+    const int wid = threadIdx.x / HS_WAVE;
+    const int nw = blockDim.x / HS_WAVE;
+    for (int j = 0; j < w; ++j) {
+        const unsigned long long m = __ballot((bits >> j) & 1u);
+        if (hs_lane() == 0) L->wave_cnt[j][wid] = __popcll(m);
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < w) {
+        const int j = threadIdx.x;
+        int tot = 0;
+        for (int v = 0; v < nw; ++v) {
+            const int c = L->wave_cnt[j][v];
+            L->wave_cnt[j][v] = tot;
+            tot += c;
+        }
+        L->base[j] = tot ? atomicAdd(g.hist_cnt + hs_wrap(tq + j, g.Q), tot) : 0;
+        if (tot && spk_total) atomicAdd(spk_total, (unsigned long long)tot);
+        L->mbase[j] = (tot && g.smon_n) ? atomicAdd(g.smon_n, (unsigned long long)tot) : 0ull;
+    }
+    __syncthreads();
+    for (int j = 0; j < w; ++j) {
+        const bool spk = (bits >> j) & 1u;
+        const unsigned long long m = __ballot(spk);
+        if (spk) {
+            const int pos = L->wave_cnt[j][wid] + __popcll(m & hs_lanemask_lt());
+            g.hist[(long long)hs_wrap(tq + j, g.Q) * g.n + L->base[j] + pos] = i;
+            if (g.smon_n) {
+                const unsigned long long r = L->mbase[j] + (unsigned long long)pos;
+                if ((long long)r < g.smon_cap) {
+                    g.smon_i[r] = i;
+                    g.smon_t[r] = (int32_t)(T + j);
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+/* MS: the model of every group (kernel specialised for it, so the register
+   budget is that model's), or 0 for networks mixing models */
+template <typename R, bool FUSED, int MS>
+__device__ __forceinline__ void hs_update_phase(const DGroup<R> *groups, int ng, int total_tiles, long long T, int w,
+                                                int wmax,
+                                long long no_reset_before, HsCompactLdsW *L, unsigned long long *spk_total,
+                                int block_rank, int n_blocks)
+{
+    for (int tile = block_rank; tile < total_tiles; tile += n_blocks) {
+        int gi = 0;
+        while (gi + 1 < ng && tile >= groups[gi + 1].tile_begin) ++gi;
+        const DGroup<R> &g = groups[gi];
+        const int lt = tile - g.tile_begin;
+        if (MS != 0) {
+            hs_update_tile<R, (MS != 0 ? MS : 1), FUSED>(g, lt, T, w, wmax, no_reset_before, L, spk_total);
+        } else if (g.model == HS_MODEL_LIF_CUBA) {
+            hs_update_tile<R, HS_MODEL_LIF_CUBA, FUSED>(g, lt, T, w, wmax, no_reset_before, L, spk_total);
+        } else if (g.model == HS_MODEL_LIF_NOISE) {
+            hs_update_tile<R, HS_MODEL_LIF_NOISE, FUSED>(g, lt, T, w, wmax, no_reset_before, L, spk_total);
+        } else {
+            hs_update_tile<R, HS_MODEL_HH_COBA, FUSED>(g, lt, T, w, wmax, no_reset_before, L, spk_total);
+        }
+    }
+}
+
+template <typename R, bool FUSED, int MS>
+__global__ __launch_bounds__(256) void hs_k_update(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl,
+                                                    int koff, int w, int wmax)
+{
+    __shared__ HsCompactLdsW L;
+    const long long T = ctl->base + koff;
+    hs_update_phase<R, FUSED, MS>(groups, ng, total_tiles, T, w, wmax, ctl->no_reset_before, &L, &ctl->spikes,
+                                  blockIdx.x, gridDim.x);
+}
+
+template <typename R, bool ACC>
+__global__ __launch_bounds__(1024) void hs_k_deliver(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w)
+{
+    __shared__ HsDeliverLds L;
+    const long long T = ctl->base + koff;
+    const int nwb = blockDim.x / HS_WAVE;
+    const long long rank = (long long)blockIdx.x * nwb + threadIdx.x / HS_WAVE;
+    const long long nw = (long long)gridDim.x * nwb;
+    unsigned long long ev = 0;
+    for (int pi = 0; pi < np; ++pi)
+        if (paths[pi].dmode == HS_DELIVER_SCATTER) hs_deliver_path<R, ACC>(paths[pi], T, w, &L, rank, nw, &ev);
+    hs_flush_events(ev, &ctl->events);
+}
+
+template <typename R, bool ACC>
+__global__ __launch_bounds__(1024) void hs_k_gather(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w)
+{
+    __shared__ HsGatherLds L;
+    __shared__ R lacc[HS_MAX_WINDOW * HS_GTILE];
+    const long long T = ctl->base + koff;
+    unsigned long long ev = 0;
+    for (int pi = 0; pi < np; ++pi) {
+        const DPath<R> &p = paths[pi];
+        if (p.dmode != HS_DELIVER_GATHER) continue;
+        for (int tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x)
+            hs_gather_tile<R, ACC>(p, tile, T, w, &L, lacc, &ev);
+    }
+    hs_flush_events(ev, &ctl->events);
+}
+
+template <typename R, int MS>
+__global__ __launch_bounds__(256) void hs_k_fused(const DGroup<R> *groups, int ng, int total_tiles,
+                                                   const DPath<R> *paths, int np, HsCtl *ctl, int koff, int n_upd)
+{
+    __shared__ union {
+        HsCompactLdsW c;
+        HsDeliverLds d;
+    } L;
+    const long long T = ctl->base + koff;
+    if ((int)blockIdx.x < n_upd) {
+        hs_update_phase<R, true, MS>(groups, ng, total_tiles, T, 1, 1, ctl->no_reset_before, &L.c, &ctl->spikes,
+                                     blockIdx.x, n_upd);
+    } else {
+        const int nwb = blockDim.x / HS_WAVE;
+        const long long rank = (long long)(blockIdx.x - n_upd) * nwb + threadIdx.x / HS_WAVE;
+        const long long nw = (long long)(gridDim.x - n_upd) * nwb;
+        unsigned long long ev = 0;
+        for (int pi = 0; pi < np; ++pi) hs_deliver_path<R, true>(paths[pi], T, 1, &L.d, rank, nw, &ev);
+        hs_flush_events(ev, &ctl->events);
+    }
+}
+
+/* end of a run at step t_end = ctl->base: apply the last step's accumulated
+   effects (fused form) and the resets of its spikers */
+template <typename R, bool FUSED>
+__global__ __launch_bounds__(256) void hs_k_finalize(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl)
+{
+    const long long t = ctl->base;
+    for (int tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
+        int gi = 0;
+        while (gi + 1 < ng && tile >= groups[gi + 1].tile_begin) ++gi;
+        const DGroup<R> &g = groups[gi];
+        const int i = (tile - g.tile_begin) * blockDim.x + threadIdx.x;
+        if (i >= g.n) continue;
+        if (FUSED) {
+            const long long slot = hs_mod(t - 1, g.nacc) * (long long)g.n + i;
+            for (int v = 0; v < g.nvars; ++v) {
+                R *a = g.acc[v];
+                if (a) {
+                    g.var[v][i] = g.var[v][i] + a[slot];
+                    a[slot] = (R)0;
+                }
+            }
+        }
+        if (g.has_reset && (long long)g.last[i] == t - 1 && t - 1 >= ctl->no_reset_before) g.var[0][i] = g.vr;
+    }
+}
+
+__global__ void hs_k_tick(HsCtl *ctl, long long k) { ctl->base += k; }
+__global__ void hs_k_mark_reset(HsCtl *ctl) { ctl->no_reset_before = ctl->base; }
+
+/* ------------------------------------------------------------------------ */
+/* launchers                                                                 */
+/* ------------------------------------------------------------------------ */
+template <typename R, bool FUSED, int MS>
+static void hs_update_ms(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int koff, int w, int wmax,
+                         int block, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((hs_k_update<R, FUSED, MS>), dim3(grid), dim3(block), 0, s, groups, ng, total_tiles, ctl, koff,
+                       w, wmax);
+}
+
+template <typename R>
+hipError_t hs_launch_update(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int koff, int w, int wmax,
+                            int fused, int ms, int block, int grid, hipStream_t s)
+{
+    if (w < 1 || w > HS_MAX_WINDOW || wmax < w || wmax > HS_MAX_WINDOW || block > 256) return hipErrorInvalidValue;
+#define HS_UPD(F, M) hs_update_ms<R, F, M>(groups, ng, total_tiles, ctl, koff, w, wmax, block, grid, s)
+    if (fused) {
+        if (ms == HS_MODEL_LIF_CUBA) HS_UPD(true, HS_MODEL_LIF_CUBA);
+        else if (ms == HS_MODEL_LIF_NOISE) HS_UPD(true, HS_MODEL_LIF_NOISE);
+        else if (ms == HS_MODEL_HH_COBA) HS_UPD(true, HS_MODEL_HH_COBA);
+        else HS_UPD(true, 0);
+    } else {
+        if (ms == HS_MODEL_LIF_CUBA) HS_UPD(false, HS_MODEL_LIF_CUBA);
+        else if (ms == HS_MODEL_LIF_NOISE) HS_UPD(false, HS_MODEL_LIF_NOISE);
+        else if (ms == HS_MODEL_HH_COBA) HS_UPD(false, HS_MODEL_HH_COBA);
+        else HS_UPD(false, 0);
+    }
+#undef HS_UPD
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t hs_launch_deliver(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w, int fused, int block,
+                             int grid, hipStream_t s)
+{
+    if (fused) hipLaunchKernelGGL((hs_k_deliver<R, true>), dim3(grid), dim3(block), 0, s, paths, np, ctl, koff, w);
+    else hipLaunchKernelGGL((hs_k_deliver<R, false>), dim3(grid), dim3(block), 0, s, paths, np, ctl, koff, w);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t hs_launch_gather(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w, int fused, int block,
+                            int grid, hipStream_t s)
+{
+    if (w < 1 || w > HS_MAX_WINDOW || block < 64 || block > 1024 || block % 64) return hipErrorInvalidValue;
+    if (fused) hipLaunchKernelGGL((hs_k_gather<R, true>), dim3(grid), dim3(block), 0, s, paths, np, ctl, koff, w);
+    else hipLaunchKernelGGL((hs_k_gather<R, false>), dim3(grid), dim3(block), 0, s, paths, np, ctl, koff, w);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t hs_launch_fused(const DGroup<R> *groups, int ng, int total_tiles, const DPath<R> *paths, int np,
+                           HsCtl *ctl, int koff, int ms, int block, int n_upd, int n_del, hipStream_t s)
+{
+    if (block > 256) return hipErrorInvalidValue;
+#define HS_FUS(M)                                                                                                 \
+    hipLaunchKernelGGL((hs_k_fused<R, M>), dim3(n_upd + n_del), dim3(block), 0, s, groups, ng, total_tiles, paths, \
+                       np, ctl, koff, n_upd)
+    if (ms == HS_MODEL_LIF_CUBA) HS_FUS(HS_MODEL_LIF_CUBA);
+    else if (ms == HS_MODEL_LIF_NOISE) HS_FUS(HS_MODEL_LIF_NOISE);
+    else if (ms == HS_MODEL_HH_COBA) HS_FUS(HS_MODEL_HH_COBA);
+    else HS_FUS(0);
+#undef HS_FUS
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t hs_launch_finalize(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int fused, int block,
+                              hipStream_t s)
+{
+    int grid = total_tiles < 4096 ? total_tiles : 4096;
+    if (grid < 1) grid = 1;
+    if (fused)
+        hipLaunchKernelGGL((hs_k_finalize<R, true>), dim3(grid), dim3(block), 0, s, groups, ng, total_tiles, ctl);
+    else
+        hipLaunchKernelGGL((hs_k_finalize<R, false>), dim3(grid), dim3(block), 0, s, groups, ng, total_tiles, ctl);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(hs_k_mark_reset, dim3(1), dim3(1), 0, s, ctl);
+    return hipGetLastError();
+}
+
+hipError_t hs_launch_tick(HsCtl *ctl, long long k, hipStream_t s)
+{
+    hipLaunchKernelGGL(hs_k_tick, dim3(1), dim3(1), 0, s, ctl, k);
+    return hipGetLastError();
+}
+
+#define HS_INSTANTIATE(R)                                                                                         \
+    template hipError_t hs_launch_update<R>(const DGroup<R> *, int, int, HsCtl *, int, int, int, int, int, int, int, \
+                                            hipStream_t);                                                         \
+    template hipError_t hs_launch_deliver<R>(const DPath<R> *, int, HsCtl *, int, int, int, int, int, hipStream_t); \
+    template hipError_t hs_launch_gather<R>(const DPath<R> *, int, HsCtl *, int, int, int, int, int, hipStream_t);  \
+    template hipError_t hs_launch_fused<R>(const DGroup<R> *, int, int, const DPath<R> *, int, HsCtl *, int, int,   \
+                                           int, int, int, hipStream_t);                                           \
+    template hipError_t hs_launch_finalize<R>(const DGroup<R> *, int, int, HsCtl *, int, int, hipStream_t);
+HS_INSTANTIATE(double)
+HS_INSTANTIATE(float)

@@ -4,9 +4,10 @@
  *
  * HBM layout of one network (all arrays structure-of-arrays, 256-B aligned):
  *   group  : var[v][n] (R), last[n] (int32 step of the last spike),
- *            acc[2][v][n] (R, fused form only, double-buffered by step parity),
- *            hist[Q][n] (int32 spike lists, one per step of a ring of Q >= dmax+2
- *            steps) with hist_cnt[Q]
+ *            acc[v][nacc][n] (R, fused form only: the effects due at step t are
+ *            summed in slot t mod nacc, nacc = window + 1),
+ *            hist[Q][n] (int32 spike lists, one per step of a ring of
+ *            Q = dmax + 2*window steps) with hist_cnt[Q]
  *   pathway: bstart[n_src*D+1] (int32 offsets of the (pre, delay) bundles),
  *            post[S] (int32 target index), w[S] or w[n_src] (R) by weight mode
  *   control: HsCtl (step base, event counter, reset guard)
@@ -20,12 +21,18 @@
 
 #define HS_MAX_STATE_MONS 4
 #define HS_MAX_DELAY_SLOTS 4096 /* D per pathway (LDS prefix table) */
+#define HS_MAX_WINDOW 8         /* steps per update launch of the windowed form */
+#define HS_GTILE 256            /* targets per block of the gather delivery */
+
+/* delivery modes of a pathway */
+#define HS_DELIVER_SCATTER 1 /* per (pre, delay) bundles, global atomics on the targets */
+#define HS_DELIVER_GATHER 2  /* per (pre, target tile) segments, LDS accumulation per tile */
 
 struct HsCtl {
     long long base;            /* simulation step of chunk offset 0 */
     long long no_reset_before; /* resets of spikes at steps < this were already applied */
     unsigned long long events; /* synaptic events delivered */
-    unsigned long long pad;
+    unsigned long long spikes; /* spikes emitted */
 };
 
 template <typename R>
@@ -35,7 +42,8 @@ struct DGroup {
     unsigned int stream;
     int nst;
     R *var[HS_MAX_VARS];
-    R *acc[2][HS_MAX_VARS];
+    R *acc[HS_MAX_VARS]; /* fused form: ring of nacc step slots of n values (nullptr: not a target) */
+    int nacc;
     int32_t *last;
     int32_t *hist;
     int32_t *hist_cnt;
@@ -65,10 +73,18 @@ struct DPath {
     const R *w;
     R wscalar;
     R *target;               /* plain: the state variable */
-    R *acc[2];               /* fused: the target's accumulators */
+    R *acc;                  /* fused: the target variable's accumulator ring */
+    int nacc, n_tgt;
+    /* gather layout: synapses sorted by (target tile, pre), synapse order kept;
+       segment of (tile, pre) = gsyn[goff[tile*n_src+pre] .. goff[tile*n_src+pre+1]),
+       entry = (delay - dmin) << 16 | (post - tile*HS_GTILE); weights follow the
+       same order */
+    int dmode, n_tiles;
+    const int32_t *goff;
+    const uint32_t *gsyn;
 };
 
-/* launchers (hs_kernels.hip) -- all on the runtime stream, return hipError_t */
+/* launchers (hs_ops.hip, hs_engine.hip) -- all on the runtime stream, return hipError_t */
 hipError_t hs_launch_threshold(int prec, const void *v, int n, double vth, int32_t *last,
                                long long step, int ref_steps, int sorted, int32_t *spikes,
                                int32_t *count, int32_t *scratch, hipStream_t s);
@@ -79,21 +95,34 @@ hipError_t hs_launch_stateupdate(int model, int prec, int n, void *const *vars, 
                                  int ref_steps, long long step, hipStream_t s);
 hipError_t hs_launch_math(int fn, int n, const double *x, double *y, hipStream_t s);
 
-/* one pathway at step ctl->base (acc != 0: into p.acc[step & 1]) */
+/* one pathway at step ctl->base, effects in place */
 template <typename R>
-hipError_t hs_launch_deliver_one_ctl(const DPath<R> &p, HsCtl *ctl, int acc, unsigned long long *events, int blocks,
+hipError_t hs_launch_deliver_one_ctl(const DPath<R> &p, HsCtl *ctl, unsigned long long *events, int blocks,
                                      hipStream_t s);
 
-/* engine kernels over device descriptor arrays */
+/*
+ * Engine kernels over device descriptor arrays; every kernel reads the step base
+ * from ctl (t = ctl->base + koff) so a captured graph replays for any step.
+ *   update : steps t .. t+w-1 of every group (plain: w == 1); wmax = the
+ *            engine's window, for the ring slots it clears ahead
+ *   deliver: bundles due in steps t .. t+w-1 (plain: w == 1, in place)
+ *   fused  : update of step t on blocks [0, n_upd) and delivery of step t on
+ *            blocks [n_upd, n_upd + n_del), one launch (every delay >= 1 step)
+ */
+/* ms: the model shared by every group (specialised kernel), 0 for mixed networks */
 template <typename R>
-hipError_t hs_launch_update(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int koff,
-                            int fused, int block, int grid, hipStream_t s);
+hipError_t hs_launch_update(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int koff, int w,
+                            int wmax, int fused, int ms, int block, int grid, hipStream_t s);
 template <typename R>
-hipError_t hs_launch_deliver(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int fused, int block,
+hipError_t hs_launch_deliver(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w, int fused, int block,
                              int grid, hipStream_t s);
 template <typename R>
 hipError_t hs_launch_fused(const DGroup<R> *groups, int ng, int total_tiles, const DPath<R> *paths, int np,
-                           HsCtl *ctl, int koff, int block, int grid, hipStream_t s);
+                           HsCtl *ctl, int koff, int ms, int block, int n_upd, int n_del, hipStream_t s);
+/* gather-mode pathways (scatter-mode ones are skipped; hs_launch_deliver skips gather ones) */
+template <typename R>
+hipError_t hs_launch_gather(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w, int fused, int block,
+                            int grid, hipStream_t s);
 template <typename R>
 hipError_t hs_launch_finalize(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int fused,
                               int block, hipStream_t s);

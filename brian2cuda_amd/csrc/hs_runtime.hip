@@ -173,6 +173,16 @@ struct HsModelInfo {
     int i_vth, i_vr, i_ref, i_stream; /* parameter positions (-1: none) */
 };
 
+/* refractory period in steps (Brian2's timestep()), held in int32 next to the
+   int32 last-spike step; longer than 2^30 steps means "never again" */
+static int hs_ref_steps(double refractory, double dt)
+{
+    long long r = hs_timestep(refractory, dt);
+    if (r < 0) r = 0;
+    if (r > (1LL << 30) - 1) r = (1LL << 30) - 1;
+    return (int)r;
+}
+
 static bool hs_model_info(int model, HsModelInfo *mi)
 {
     if (model == HS_MODEL_LIF_CUBA) { *mi = {HS_LIF_CUBA_NVARS, HS_LIF_CUBA_NPARAMS, 1, 1, 2, 6, -1}; return true; }
@@ -228,7 +238,7 @@ extern "C" int hs_stateupdate(int model, int precision, int n, void *const *vars
         return hs_fail(HS_ERR_ARG, "hs_stateupdate: bad arguments (model %d expects %d params)", model, mi.nparams);
     double k[16] = {0};
     if (hs_model_constants(model, params, dt, k) < 0) return hs_fail(HS_ERR_ARG, "bad model");
-    int ref_steps = (int)hs_timestep(params[mi.i_ref], dt);
+    int ref_steps = hs_ref_steps(params[mi.i_ref], dt);
     unsigned int stream = mi.i_stream >= 0 ? (unsigned int)params[mi.i_stream] : 0u;
     if (precision == HS_F64) {
         HS_HIP(hipMemcpyAsync(g_rt.kbuf, k, sizeof(k), hipMemcpyHostToDevice, g_rt.stream));
@@ -277,7 +287,7 @@ static int hs_deliver_t(int n_src, int dmin, int D, int Q, const int32_t *hist, 
     memset(&c, 0, sizeof(c));
     c.base = step;
     HS_HIP(hipMemcpyAsync(g_rt.ctl, &c, sizeof(c), hipMemcpyHostToDevice, g_rt.stream));
-    HS_HIP(hs_launch_deliver_one_ctl<R>(p, g_rt.ctl, 0, (unsigned long long *)events, g_rt.n_cu * 2, g_rt.stream));
+    HS_HIP(hs_launch_deliver_one_ctl<R>(p, g_rt.ctl, (unsigned long long *)events, g_rt.n_cu * 2, g_rt.stream));
     HS_HIP(hipStreamSynchronize(g_rt.stream));
     return HS_OK;
 }
@@ -310,7 +320,8 @@ struct HGroup {
     int ref_steps;
     unsigned int stream;
     void *var[HS_MAX_VARS] = {nullptr};
-    void *acc[2][HS_MAX_VARS] = {{nullptr}};
+    void *acc[HS_MAX_VARS] = {nullptr}; /* fused: ring of nacc slots */
+    int nacc = 0;
     int32_t *last = nullptr;
     int32_t *hist = nullptr;
     int32_t *hist_cnt = nullptr;
@@ -328,6 +339,10 @@ struct HPath {
     int32_t *bstart = nullptr;
     int32_t *post = nullptr;
     void *w = nullptr;
+    int dmode = HS_DELIVER_SCATTER;
+    int n_tiles = 0;
+    int32_t *goff = nullptr;
+    uint32_t *gsyn = nullptr;
 };
 
 struct HMon {
@@ -353,14 +368,20 @@ struct hs_net {
     HsCtl *ctl = nullptr;
     void *dgroups = nullptr, *dpaths = nullptr;
     int total_tiles = 0;
-    bool single_launch = false;
+    int window = 1;        /* steps per update launch (fused form) */
+    int dmin_all = 0;      /* smallest delay over all pathways */
+    int mode = 0;          /* 0 plain, 1 fused per-step pair, 2 fused single launch, 3 fused windows */
     int64_t step = 0;
-    long long opt[4] = {256, 0, 256, 0};
+    /* 0 update block, 1 scatter grid, 2 graph chunk, 3 fused delivery blocks,
+       4 window, 5 delivery mode, 6 gather block */
+    long long opt[7] = {256, 0, 256, 0, 0, 0, 256};
+    int n_scatter = 0, n_gather = 0, gather_tiles = 0;
     std::map<int64_t, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_wall = 0, last_gpu_ms = 0;
     int64_t last_steps = 0;
     uint64_t events = 0;
+    uint64_t spikes = 0;
     std::vector<void *> allocs;
 };
 
@@ -440,7 +461,7 @@ extern "C" int hs_net_add_group(hs_net *net, int model, int n, const double *par
     memcpy(g.params, params, sizeof(double) * nparams);
     memset(g.k, 0, sizeof(g.k));
     hs_model_constants(model, params, net->dt, g.k);
-    g.ref_steps = (int)hs_timestep(params[mi.i_ref], net->dt);
+    g.ref_steps = hs_ref_steps(params[mi.i_ref], net->dt);
     g.stream = mi.i_stream >= 0 ? (unsigned int)params[mi.i_stream] : 0u;
     for (int v = 0; v < mi.nvars; ++v) HS_TRY(hs_dalloc(net, hs_rsize(net) * n, &g.var[v], true));
     HS_TRY(hs_dalloc(net, sizeof(int32_t) * n, (void **)&g.last, false));
@@ -538,36 +559,84 @@ extern "C" int hs_net_add_pathway(hs_net *net, int src, int tgt, int var, int64_
     P.dmin = dmin;
     P.dmax = dmax;
     P.D = D;
-    const int64_t nb = (int64_t)gs.n * D;
-    std::vector<int32_t> start((size_t)nb + 1, 0);
-    for (int64_t s = 0; s < nsyn; ++s) {
+    for (int64_t s = 0; s < nsyn; ++s)
         if (pre[s] < 0 || pre[s] >= gs.n || post[s] < 0 || post[s] >= gt.n)
             return hs_fail(HS_ERR_ARG, "synapse %lld: index out of range", (long long)s);
-        const int d = delay_steps ? delay_steps[s] : scalar_delay;
-        start[(size_t)((int64_t)pre[s] * D + (d - dmin)) + 1]++;
-    }
-    for (int64_t b = 0; b < nb; ++b) start[b + 1] += start[b];
-    std::vector<int32_t> pos(start.begin(), start.end() - 1);
-    std::vector<int32_t> post_sorted((size_t)(nsyn ? nsyn : 1));
+    /*
+     * Delivery mode.  Gather pays one (pre, tile) lookup per candidate spike and
+     * target tile and reads each synapse once per window its spike is in range
+     * of; scatter pays one global atomic per event.  Gather wins for dense,
+     * heterogeneous-delay connectivity (many synapses per (pre, tile), delay span
+     * >= 4 steps); scatter for sparse or block-diagonal connectivity.
+     */
+    const int n_tiles = (gt.n + HS_GTILE - 1) / HS_GTILE;
+    const double per_seg = nsyn > 0 ? (double)nsyn / ((double)gs.n * n_tiles) : 0.0;
+    int dmode = HS_DELIVER_SCATTER;
+    if (net->opt[5] == HS_DELIVER_GATHER) dmode = HS_DELIVER_GATHER;
+    else if (net->opt[5] == 0 && D >= 4 && per_seg >= 0.5) dmode = HS_DELIVER_GATHER;
+    if (dmode == HS_DELIVER_GATHER && (double)gs.n * n_tiles > 2.0e9)
+        return hs_fail(HS_ERR_UNSUPPORTED, "gather table of %d x %d segments too large", gs.n, n_tiles);
+    P.dmode = dmode;
     const size_t rs = hs_rsize(net);
     std::vector<unsigned char> w_sorted;
     if (wmode == HS_W_SYNAPSE) w_sorted.resize(rs * (size_t)(nsyn ? nsyn : 1));
-    for (int64_t s = 0; s < nsyn; ++s) {
-        const int d = delay_steps ? delay_steps[s] : scalar_delay;
-        const int32_t j = pos[(size_t)((int64_t)pre[s] * D + (d - dmin))]++;
-        post_sorted[j] = post[s];
-        if (wmode == HS_W_SYNAPSE) {
-            if (net->prec == HS_F64) ((double *)w_sorted.data())[j] = weights[s];
-            else ((float *)w_sorted.data())[j] = (float)weights[s];
+    auto put_w = [&](int32_t j, int64_t s) {
+        if (wmode != HS_W_SYNAPSE) return;
+        if (net->prec == HS_F64) ((double *)w_sorted.data())[j] = weights[s];
+        else ((float *)w_sorted.data())[j] = (float)weights[s];
+    };
+    if (dmode == HS_DELIVER_SCATTER) {
+        /* bundles: stable counting sort by (pre, delay) */
+        const int64_t nb = (int64_t)gs.n * D;
+        std::vector<int32_t> start((size_t)nb + 1, 0);
+        for (int64_t s = 0; s < nsyn; ++s) {
+            const int d = delay_steps ? delay_steps[s] : scalar_delay;
+            start[(size_t)((int64_t)pre[s] * D + (d - dmin)) + 1]++;
         }
-    }
-    HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)(nb + 1), (void **)&P.bstart, false));
-    HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)(nsyn ? nsyn : 1), (void **)&P.post, false));
-    HS_HIP(hipMemcpyAsync(P.bstart, start.data(), sizeof(int32_t) * (size_t)(nb + 1), hipMemcpyHostToDevice,
-                          g_rt.stream));
-    if (nsyn)
-        HS_HIP(hipMemcpyAsync(P.post, post_sorted.data(), sizeof(int32_t) * (size_t)nsyn, hipMemcpyHostToDevice,
+        for (int64_t b = 0; b < nb; ++b) start[b + 1] += start[b];
+        std::vector<int32_t> pos(start.begin(), start.end() - 1);
+        std::vector<int32_t> post_sorted((size_t)(nsyn ? nsyn : 1));
+        for (int64_t s = 0; s < nsyn; ++s) {
+            const int d = delay_steps ? delay_steps[s] : scalar_delay;
+            const int32_t j = pos[(size_t)((int64_t)pre[s] * D + (d - dmin))]++;
+            post_sorted[j] = post[s];
+            put_w(j, s);
+        }
+        HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)(nb + 1), (void **)&P.bstart, false));
+        HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)(nsyn ? nsyn : 1), (void **)&P.post, false));
+        HS_HIP(hipMemcpyAsync(P.bstart, start.data(), sizeof(int32_t) * (size_t)(nb + 1), hipMemcpyHostToDevice,
                               g_rt.stream));
+        if (nsyn)
+            HS_HIP(hipMemcpyAsync(P.post, post_sorted.data(), sizeof(int32_t) * (size_t)nsyn, hipMemcpyHostToDevice,
+                                  g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    } else {
+        /* segments: stable counting sort by (target tile, pre) -- tile-major, so the
+           block that owns a tile reads one contiguous region; entry packs the
+           delay slot and the target's index inside its tile */
+        P.n_tiles = n_tiles;
+        const int64_t nseg = (int64_t)gs.n * n_tiles;
+        std::vector<int32_t> start((size_t)nseg + 1, 0);
+        for (int64_t s = 0; s < nsyn; ++s) start[(size_t)((int64_t)(post[s] / HS_GTILE) * gs.n + pre[s]) + 1]++;
+        for (int64_t b = 0; b < nseg; ++b) start[b + 1] += start[b];
+        std::vector<int32_t> pos(start.begin(), start.end() - 1);
+        std::vector<uint32_t> ent((size_t)(nsyn ? nsyn : 1));
+        for (int64_t s = 0; s < nsyn; ++s) {
+            const int d = delay_steps ? delay_steps[s] : scalar_delay;
+            const int tl = post[s] / HS_GTILE;
+            const int32_t j = pos[(size_t)((int64_t)tl * gs.n + pre[s])]++;
+            ent[j] = ((uint32_t)(d - dmin) << 16) | (uint32_t)(post[s] - tl * HS_GTILE);
+            put_w(j, s);
+        }
+        HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)(nseg + 1), (void **)&P.goff, false));
+        HS_TRY(hs_dalloc(net, sizeof(uint32_t) * (size_t)(nsyn ? nsyn : 1), (void **)&P.gsyn, false));
+        HS_HIP(hipMemcpyAsync(P.goff, start.data(), sizeof(int32_t) * (size_t)(nseg + 1), hipMemcpyHostToDevice,
+                              g_rt.stream));
+        if (nsyn)
+            HS_HIP(hipMemcpyAsync(P.gsyn, ent.data(), sizeof(uint32_t) * (size_t)nsyn, hipMemcpyHostToDevice,
+                                  g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    }
     if (wmode == HS_W_SYNAPSE && nsyn) {
         HS_TRY(hs_dalloc(net, rs * (size_t)nsyn, &P.w, false));
         HS_HIP(hipMemcpyAsync(P.w, w_sorted.data(), rs * (size_t)nsyn, hipMemcpyHostToDevice, g_rt.stream));
@@ -581,6 +650,12 @@ extern "C" int hs_net_add_pathway(hs_net *net, int src, int tgt, int var, int64_
         HS_HIP(hipMemcpyAsync(P.w, wp.data(), rs * (size_t)gs.n, hipMemcpyHostToDevice, g_rt.stream));
     }
     HS_HIP(hipStreamSynchronize(g_rt.stream)); /* host vectors die here */
+    if (P.dmode == HS_DELIVER_GATHER) {
+        net->n_gather++;
+        net->gather_tiles = std::max(net->gather_tiles, P.n_tiles);
+    } else {
+        net->n_scatter++;
+    }
     net->p.push_back(P);
     net->desc_dirty = true;
     if (pid) *pid = (int)net->p.size() - 1;
@@ -635,10 +710,16 @@ extern "C" int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start
 
 extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
 {
-    if (!net || key < 0 || key > 3) return hs_fail(HS_ERR_ARG, "bad option key");
-    if (key == 0 && (value < 64 || value > 1024 || value % 64)) return hs_fail(HS_ERR_ARG, "block size must be a multiple of 64 in [64,1024]");
+    if (!net || key < 0 || key > 6) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (key == 6 && (value < 64 || value > 1024 || value % 64))
+        return hs_fail(HS_ERR_ARG, "gather block size must be a multiple of 64 in [64,1024]");
+    if (key == 5 && (value < 0 || value > 2)) return hs_fail(HS_ERR_ARG, "delivery mode: 0 auto, 1 scatter, 2 gather");
+    if (key == 0 && (value < 64 || value > 256 || value % 64))
+        return hs_fail(HS_ERR_ARG, "update block size must be 64, 128, 192 or 256");
     if ((key == 1 || key == 3) && (value < 0 || value > (1 << 20))) return hs_fail(HS_ERR_ARG, "bad grid size");
     if (key == 2 && (value < 0 || value > 100000)) return hs_fail(HS_ERR_ARG, "bad graph chunk");
+    if (key == 4 && (value < 0 || value > HS_MAX_DELAY_SLOTS)) return hs_fail(HS_ERR_ARG, "bad window");
+    if (key == 4 && net->built) return hs_fail(HS_ERR_STATE, "the window is fixed at the first run");
     net->opt[key] = value;
     net->desc_dirty = true;
     hs_drop_graphs(net);
@@ -647,7 +728,7 @@ extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
 
 extern "C" int hs_net_get_option(hs_net *net, int key, long long *value)
 {
-    if (!net || key < 0 || key > 3 || !value) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (!net || key < 0 || key > 6 || !value) return hs_fail(HS_ERR_ARG, "bad option key");
     *value = net->opt[key];
     return HS_OK;
 }
@@ -656,24 +737,48 @@ extern "C" int hs_net_get_option(hs_net *net, int key, long long *value)
 static int hs_build(hs_net *net)
 {
     if (net->built) return HS_OK;
+    /* form of the step loop */
+    int dmin_all = 1 << 30, dspan = 1;
+    for (const HPath &P : net->p) {
+        dmin_all = std::min(dmin_all, P.dmin);
+        dspan = std::max(dspan, P.D);
+    }
+    if (net->p.empty()) dmin_all = 1;
+    net->dmin_all = dmin_all;
+    net->window = 1;
+    if (net->engine == HS_ENGINE_PLAIN) {
+        net->mode = 0;
+    } else if (net->p.empty() || dmin_all < 1) {
+        net->mode = 1; /* effects due in the same step: update, then deliver */
+    } else {
+        /* the window cannot exceed the smallest delay, and its (step, delay) pairs
+           must fit the delivery kernel's prefix table */
+        int w = net->opt[4] > 0 ? (int)net->opt[4] : HS_MAX_WINDOW;
+        w = std::min(w, HS_MAX_WINDOW);
+        w = std::min(w, dmin_all);
+        w = std::min(w, HS_MAX_DELAY_SLOTS / dspan);
+        w = std::max(w, 1);
+        net->window = w;
+        /* the single-launch kernel carries scatter delivery only */
+        net->mode = (w == 1 && net->n_gather == 0) ? 2 : 3;
+    }
+    const int W = net->window;
     for (HGroup &g : net->g) {
         int dmax = 0;
         for (const HPath &P : net->p)
             if (P.src == (int)(&g - &net->g[0])) dmax = std::max(dmax, P.dmax);
-        g.Q = dmax + 2;
+        g.Q = dmax + 2 * W; /* see the ring invariants in hs_engine.hip */
         HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.Q * g.n, (void **)&g.hist, false));
         HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.Q, (void **)&g.hist_cnt, true));
     }
     if (net->engine == HS_ENGINE_FUSED) {
         for (const HPath &P : net->p) {
             HGroup &gt = net->g[P.tgt];
-            for (int b = 0; b < 2; ++b)
-                if (!gt.acc[b][P.var]) HS_TRY(hs_dalloc(net, hs_rsize(net) * gt.n, &gt.acc[b][P.var], true));
+            gt.nacc = W + 1;
+            if (!gt.acc[P.var])
+                HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)gt.n * gt.nacc, &gt.acc[P.var], true));
         }
     }
-    net->single_launch = net->engine == HS_ENGINE_FUSED;
-    for (const HPath &P : net->p)
-        if (P.dmin < 1) net->single_launch = false;
     HsCtl c;
     memset(&c, 0, sizeof(c));
     c.base = net->step;
@@ -702,9 +807,9 @@ static int hs_upload_desc_t(hs_net *net)
         d.stream = g.stream;
         for (int v = 0; v < HS_MAX_VARS; ++v) {
             d.var[v] = (R *)g.var[v];
-            d.acc[0][v] = (R *)g.acc[0][v];
-            d.acc[1][v] = (R *)g.acc[1][v];
+            d.acc[v] = (R *)g.acc[v];
         }
+        d.nacc = g.nacc > 0 ? g.nacc : 1;
         d.last = g.last;
         d.hist = g.hist;
         d.hist_cnt = g.hist_cnt;
@@ -754,8 +859,13 @@ static int hs_upload_desc_t(hs_net *net)
         d.w = (const R *)P.w;
         d.wscalar = (R)P.wscalar;
         d.target = (R *)gt.var[P.var];
-        d.acc[0] = (R *)gt.acc[0][P.var];
-        d.acc[1] = (R *)gt.acc[1][P.var];
+        d.acc = (R *)gt.acc[P.var];
+        d.nacc = gt.nacc;
+        d.n_tgt = gt.n;
+        d.dmode = P.dmode;
+        d.n_tiles = P.n_tiles;
+        d.goff = P.goff;
+        d.gsyn = P.gsyn;
     }
     if (!net->dgroups) HS_TRY(hs_dalloc(net, sizeof(DGroup<R>) * std::max<size_t>(1, dg.size()), &net->dgroups, false));
     if (!net->dpaths) HS_TRY(hs_dalloc(net, sizeof(DPath<R>) * std::max<size_t>(1, dp.size()), &net->dpaths, false));
@@ -802,11 +912,11 @@ static int hs_deliver_grid(const hs_net *net)
     return g_rt.n_cu * 4;
 }
 
-static int hs_fused_grid(const hs_net *net)
+/* delivery blocks of the single-launch fused kernel */
+static int hs_fused_deliver_blocks(const hs_net *net)
 {
     if (net->opt[3] > 0) return (int)net->opt[3];
-    int g = std::max(net->total_tiles, g_rt.n_cu * 2);
-    return std::min(g, g_rt.n_cu * 16);
+    return g_rt.n_cu * 2;
 }
 
 static int hs_update_grid(const hs_net *net)
@@ -814,44 +924,58 @@ static int hs_update_grid(const hs_net *net)
     return std::max(1, std::min(net->total_tiles, 1 << 20));
 }
 
-/* enqueue k steps (chunk offsets 0..k-1) and the tick */
-static int hs_enqueue_steps(hs_net *net, int64_t k)
+/*
+ * enqueue k steps (chunk offsets 0..k-1) and the tick.
+ *   mode 0 plain          : update(t), deliver(t) in place
+ *   mode 1 fused, delay 0 : update(t), deliver(t) into the accumulator
+ *   mode 2 fused, W = 1   : one kernel per step
+ *   mode 3 fused, W >= 2  : deliver(window), update(window) per W steps
+ */
+template <typename R>
+static int hs_enqueue_steps_t(hs_net *net, int64_t k)
 {
     const int block = (int)net->opt[0];
     const int ng = (int)net->g.size(), np = (int)net->p.size();
-    const bool fused = net->engine == HS_ENGINE_FUSED;
-    for (int64_t j = 0; j < k; ++j) {
-        const int koff = (int)j;
-        if (net->prec == HS_F64) {
-            const DGroup<double> *dg = (const DGroup<double> *)net->dgroups;
-            const DPath<double> *dp = (const DPath<double> *)net->dpaths;
-            if (net->single_launch && np > 0) {
-                HS_HIP(hs_launch_fused<double>(dg, ng, net->total_tiles, dp, np, net->ctl, koff, block,
-                                               hs_fused_grid(net), g_rt.stream));
+    const DGroup<R> *dg = (const DGroup<R> *)net->dgroups;
+    const DPath<R> *dp = (const DPath<R> *)net->dpaths;
+    const int fused = net->engine == HS_ENGINE_FUSED;
+    int ms = net->g.empty() ? 0 : net->g[0].model; /* one model for all groups: specialised kernels */
+    for (const HGroup &g : net->g)
+        if (g.model != ms) ms = 0;
+    const int ggrid = std::max(1, std::min(net->gather_tiles, 1 << 20));
+    auto deliver = [&](int koff, int w) -> int {
+        if (net->n_scatter)
+            HS_HIP(hs_launch_deliver<R>(dp, np, net->ctl, koff, w, fused, 256, hs_deliver_grid(net), g_rt.stream));
+        if (net->n_gather)
+            HS_HIP(hs_launch_gather<R>(dp, np, net->ctl, koff, w, fused, (int)net->opt[6], ggrid, g_rt.stream));
+        return HS_OK;
+    };
+    if (net->mode == 3) {
+        for (int64_t j = 0; j < k; j += net->window) {
+            const int w = (int)std::min<int64_t>(net->window, k - j);
+            HS_TRY(deliver((int)j, w));
+            HS_HIP(hs_launch_update<R>(dg, ng, net->total_tiles, net->ctl, (int)j, w, net->window, 1, ms, block,
+                                       hs_update_grid(net), g_rt.stream));
+        }
+    } else {
+        for (int64_t j = 0; j < k; ++j) {
+            if (net->mode == 2) {
+                HS_HIP(hs_launch_fused<R>(dg, ng, net->total_tiles, dp, np, net->ctl, (int)j, ms, block,
+                                          hs_update_grid(net), hs_fused_deliver_blocks(net), g_rt.stream));
             } else {
-                HS_HIP(hs_launch_update<double>(dg, ng, net->total_tiles, net->ctl, koff, fused, block,
-                                                hs_update_grid(net), g_rt.stream));
-                if (np > 0)
-                    HS_HIP(hs_launch_deliver<double>(dp, np, net->ctl, koff, fused, 256, hs_deliver_grid(net),
-                                                     g_rt.stream));
-            }
-        } else {
-            const DGroup<float> *dg = (const DGroup<float> *)net->dgroups;
-            const DPath<float> *dp = (const DPath<float> *)net->dpaths;
-            if (net->single_launch && np > 0) {
-                HS_HIP(hs_launch_fused<float>(dg, ng, net->total_tiles, dp, np, net->ctl, koff, block,
-                                              hs_fused_grid(net), g_rt.stream));
-            } else {
-                HS_HIP(hs_launch_update<float>(dg, ng, net->total_tiles, net->ctl, koff, fused, block,
-                                               hs_update_grid(net), g_rt.stream));
-                if (np > 0)
-                    HS_HIP(hs_launch_deliver<float>(dp, np, net->ctl, koff, fused, 256, hs_deliver_grid(net),
-                                                    g_rt.stream));
+                HS_HIP(hs_launch_update<R>(dg, ng, net->total_tiles, net->ctl, (int)j, 1, 1, fused, ms, block,
+                                           hs_update_grid(net), g_rt.stream));
+                if (np > 0) HS_TRY(deliver((int)j, 1));
             }
         }
     }
     HS_HIP(hs_launch_tick(net->ctl, k, g_rt.stream));
     return HS_OK;
+}
+
+static int hs_enqueue_steps(hs_net *net, int64_t k)
+{
+    return net->prec == HS_F64 ? hs_enqueue_steps_t<double>(net, k) : hs_enqueue_steps_t<float>(net, k);
 }
 
 static int hs_chunk(hs_net *net, int64_t k)
@@ -961,6 +1085,7 @@ extern "C" int hs_net_run(hs_net *net, int64_t nsteps)
     HS_HIP(hipMemcpyAsync(&c, net->ctl, sizeof(c), hipMemcpyDeviceToHost, g_rt.stream));
     HS_HIP(hipStreamSynchronize(g_rt.stream));
     net->events = c.events;
+    net->spikes = c.spikes;
     if (c.base != net->step) return hs_fail(HS_ERR_STATE, "device step %lld != host step %lld", c.base, (long long)net->step);
     for (const HMon &M : net->m) {
         if (M.kind != 0) continue;
@@ -979,10 +1104,16 @@ extern "C" int hs_net_stats(hs_net *net, hs_stats *out)
     if (!net || !out) return hs_fail(HS_ERR_ARG, "null argument");
     out->steps = net->step;
     out->events = net->events;
+    out->spikes = net->spikes;
     out->last_run_wall_s = net->last_wall;
     out->last_run_gpu_ms = net->last_gpu_ms;
     out->last_run_steps = net->last_steps;
-    out->launches_per_step = net->p.empty() ? 1 : (net->single_launch ? 1 : 2);
+    if (net->p.empty() || net->mode == 2) out->launches_per_step = 1.0;
+    else {
+        const double per = 1.0 + (net->n_scatter ? 1 : 0) + (net->n_gather ? 1 : 0);
+        out->launches_per_step = net->mode == 3 ? per / net->window : per;
+    }
+    out->window = net->window;
     return HS_OK;
 }
 

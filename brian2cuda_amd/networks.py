@@ -237,6 +237,8 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
     """Create a runtime.Net from a spec.  Returns (net, spike_monitor_ids, state_monitor_ids)."""
     from .runtime import Net
     net = Net(precision, engine, spec.dt, spec.seed)
+    for k, v in (options or {}).items():  # before the pathways: the delivery mode is chosen per pathway
+        net.set_option(k, v)
     for g in spec.groups:
         gid = net.add_group(g.model, g.n, g.params)
         for var, val in g.init.items():
@@ -246,6 +248,4 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
                         p.wscalar)
     smons = [net.add_spike_monitor(g, spike_capacity) for g in spec.spike_monitors]
     stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
-    for k, v in (options or {}).items():
-        net.set_option(k, v)
     return net, smons, stmons

@@ -1,0 +1,112 @@
+"""Small networks that restate the reference's spike-propagation tests.
+
+The reference drives them with a SpikeGeneratorGroup and a NeuronGroup 'v:1'
+(no dynamics), threshold v>1, reset v=0, on_pre 'v += 1.1'
+(brian2cuda/tests/test_synaptic_propagations.py).  Here the input is a driver
+neuron that starts above threshold (it spikes at step 0, once, or at every step
+when its rest and reset sit above threshold), and 'v:1' is a CUBA LIF with an
+infinite membrane time constant (v is constant between effects, exactly: Em
+rounds to 1).  A target that receives 1.1 in the synapses slot of step d crosses
+threshold at the update of step d+1, so it spikes at step delay+1, as the
+reference's expectations state.
+"""
+import numpy as np
+
+from .networks import GroupSpec, NetSpec, PathSpec
+from .runtime import HS_MODEL_LIF_CUBA, HS_W_SCALAR, HS_W_SYNAPSE
+
+INF_TAU = 1e30  # exp(-dt/tau) rounds to 1.0: v is exactly constant
+
+
+def static_params(refractory=0.0):
+    """'v:1' with threshold v>1, reset v=0 (El=0, Vt=1, Vr=0)."""
+    return [0.0, 1.0, 0.0, INF_TAU, 5e-3, 10e-3, refractory]
+
+
+def driver(n=1, every_step=False):
+    """Neurons that spike at step 0 only (every_step=False) or at every step."""
+    if every_step:  # rest and reset at 2 > threshold, no refractoriness
+        return GroupSpec(HS_MODEL_LIF_CUBA, n, [2.0, 1.0, 2.0, INF_TAU, 5e-3, 10e-3, 0.0], {0: 2.0, 1: 0.0, 2: 0.0})
+    # refractory 1 s: after its spike at step 0 it stays silent for any of these runs
+    return GroupSpec(HS_MODEL_LIF_CUBA, n, [0.0, 1.0, 0.0, INF_TAU, 5e-3, 10e-3, 1.0], {0: 2.0, 1: 0.0, 2: 0.0})
+
+
+def push_outer_loop(dt=1e-4):
+    """test_CudaSpikeQueue_push_outer_loop (test_synaptic_propagations.py:11-48):
+    15*1025 targets, delays alternating 0 / dt; even targets spike at step 1, odd at step 2."""
+    N = 15 * 1025
+    g_in = driver(1)
+    g = GroupSpec(HS_MODEL_LIF_CUBA, N, static_params(), {0: 0.0, 1: 0.0, 2: 0.0})
+    pre = np.zeros(N, np.int32)
+    post = np.arange(N, dtype=np.int32)
+    d = (np.arange(N) % 2).astype(np.int32)
+    p = PathSpec(0, 1, 0, pre, post, delay_steps=d, wmode=HS_W_SCALAR, wscalar=1.1)
+    spec = NetSpec("push_outer_loop", dt, 1, [g_in, g], [p], spike_monitors=[1])
+    expect = {1: np.arange(0, N, 2), 2: np.arange(1, N, 2)}
+    return spec, 3, expect
+
+
+def push_outer_loop2(dt=1e-4):
+    """test_CudaSpikeQueue_push_outer_loop2 (test_synaptic_propagations.py:51-150):
+    2048 targets; input 0 fires at every step with weight 0 and delays j; input 1
+    fires once with weight 1.1 and delays j except delay[1024] = 1023.  Target j
+    spikes at step delay_j + 1: 1023 and 1024 together, nothing at 1025."""
+    N = 2048
+    g_every = driver(1, every_step=True)
+    g_once = driver(1)
+    g = GroupSpec(HS_MODEL_LIF_CUBA, N, static_params(), {0: 0.0, 1: 0.0, 2: 0.0})
+    post = np.arange(N, dtype=np.int32)
+    d0 = np.arange(N, dtype=np.int32)
+    d1 = d0.copy()
+    d1[1024] = 1023
+    paths = [
+        PathSpec(0, 2, 0, np.zeros(N, np.int32), post, delay_steps=d0, wmode=HS_W_SYNAPSE, weights=np.zeros(N)),
+        PathSpec(1, 2, 0, np.zeros(N, np.int32), post, delay_steps=d1, wmode=HS_W_SYNAPSE,
+                 weights=np.full(N, 1.1)),
+    ]
+    spec = NetSpec("push_outer_loop2", dt, 1, [g_every, g_once, g], paths, spike_monitors=[2])
+    expect = {}
+    for j in range(N):
+        expect.setdefault(int(d1[j]) + 1, []).append(j)
+    expect = {k: np.array(v) for k, v in expect.items()}
+    return spec, N + 2, expect
+
+
+def circular_eventspaces(dt=1e-4):
+    """test_circular_eventspaces (test_synaptic_propagations.py:153-181): homogeneous
+    delays 0, 2dt, 5dt and heterogeneous j*dt for j=3,4; spike steps 1, 3, 6, 4, 5."""
+    g_in = driver(1)
+    g = GroupSpec(HS_MODEL_LIF_CUBA, 5, static_params(), {0: 0.0, 1: 0.0, 2: 0.0})
+    z = np.zeros(1, np.int32)
+    paths = [
+        PathSpec(0, 1, 0, z, np.array([0], np.int32), scalar_delay=0, wscalar=1.1),
+        PathSpec(0, 1, 0, z, np.array([1], np.int32), scalar_delay=2, wscalar=1.1),
+        PathSpec(0, 1, 0, z, np.array([2], np.int32), scalar_delay=5, wscalar=1.1),
+        PathSpec(0, 1, 0, np.zeros(2, np.int32), np.array([3, 4], np.int32),
+                 delay_steps=np.array([3, 4], np.int32), wscalar=1.1),
+    ]
+    spec = NetSpec("circular_eventspaces", dt, 1, [g_in, g], paths, spike_monitors=[1])
+    expect = {1: np.array([0]), 3: np.array([1]), 4: np.array([3]), 5: np.array([4]), 6: np.array([2])}
+    return spec, 7, expect
+
+
+def effects_cancel(n_pre=15360, n_post=64, dt=1e-4):
+    """test_synaptic_effect_modes (test_synaptic_propagations.py:316-340) restated
+    with two on_pre pathways: every pre neuron spikes at every step, one pathway
+    adds 1 and the other subtracts 1 from the same targets, so v stays 0."""
+    g_in = driver(n_pre, every_step=True)
+    g = GroupSpec(HS_MODEL_LIF_CUBA, n_post, static_params(), {0: 0.0, 1: 0.0, 2: 0.0})
+    rng = np.random.default_rng(3)
+    post = rng.integers(0, n_post, n_pre).astype(np.int32)
+    pre = np.arange(n_pre, dtype=np.int32)
+    paths = [PathSpec(0, 1, 0, pre, post, scalar_delay=1, wscalar=1.0),
+             PathSpec(0, 1, 0, pre, post, scalar_delay=1, wscalar=-1.0)]
+    spec = NetSpec("effects_cancel", dt, 1, [g_in, g], paths, spike_monitors=[1])
+    return spec, 20, {}
+
+
+def spikes_by_step(i, t):
+    out = {}
+    for step in np.unique(t):
+        out[int(step)] = np.sort(i[t == step])
+    return out

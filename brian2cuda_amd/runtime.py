@@ -26,7 +26,8 @@ MODEL_VARS = {
 }
 NEVER_SPIKED = -1073741824
 
-OPT_UPDATE_BLOCK, OPT_DELIVER_GRID, OPT_GRAPH_CHUNK, OPT_FUSED_GRID = 0, 1, 2, 3
+OPT_UPDATE_BLOCK, OPT_DELIVER_GRID, OPT_GRAPH_CHUNK, OPT_FUSED_GRID, OPT_WINDOW, OPT_DELIVERY = 0, 1, 2, 3, 4, 5
+DELIVERY_AUTO, DELIVERY_SCATTER, DELIVERY_GATHER = 0, 1, 2
 
 
 def _ptr(a):

@@ -174,10 +174,12 @@ int hs_net_run(hs_net *net, int64_t nsteps);
 typedef struct {
     int64_t steps;           /* steps simulated so far */
     uint64_t events;         /* synaptic events delivered so far */
+    uint64_t spikes;         /* spikes emitted so far */
     double last_run_wall_s;  /* wall time of the last hs_net_run (host clock) */
     double last_run_gpu_ms;  /* device time of the last run's step chunks (HIP events) */
     int64_t last_run_steps;
-    int64_t launches_per_step;
+    double launches_per_step; /* kernel launches per simulated step */
+    int64_t window;           /* steps per update launch (fused form with delays >= 2 steps) */
 } hs_stats;
 int hs_net_stats(hs_net *net, hs_stats *out);
 

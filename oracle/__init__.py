@@ -95,8 +95,8 @@ def model_constants(model, params, dt):
 
 
 def stateupdate(model, precision, arrays, last, params, dt, seed, step):
-    """One oracle state-update step; arrays (float64 numpy, modified in place)."""
-    arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    """One oracle state-update step on copies of `arrays`; returns the new arrays."""
+    arrays = [np.array(a, dtype=np.float64, copy=True) for a in arrays]
     DP = C.POINTER(C.c_double)
     ptrs = (DP * len(arrays))(*[a.ctypes.data_as(DP) for a in arrays])
     last = np.ascontiguousarray(last, dtype=np.int32)

@@ -70,6 +70,45 @@ def test_brunel_parity_small(hs, oracle_lib, prec, engine, sem):
     assert ref.events > 0
 
 
+@pytest.mark.parametrize("window", [1, 2, 3, 5])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_brunel_windows(hs, oracle_lib, window, prec):
+    """The windowed fused form (delivery of w steps, then w update steps per tile)
+    is bit-identical to the oracle's ACC semantics for every window <= the minimum
+    delay (5 steps), including runs whose length is not a multiple of the window."""
+    spec = nw.brunel(N=3000, K=300, seed=4, monitors=True)
+    net, ref = _compare(spec, prec, "fused", 1, None, oracle_lib, options={4: window, 2: 64},
+                        chunks=[7, 600, 393, 1000])
+    s = net.stats()
+    assert s["window"] == window
+    assert s["spikes"] == len(ref.spikes(0)[0])
+
+
+@pytest.mark.parametrize("delivery", [1, 2])
+@pytest.mark.parametrize("engine,sem", ENGINES)
+@pytest.mark.parametrize("prec", [64, 32])
+def test_brunel_scatter_and_gather(hs, oracle_lib, delivery, engine, sem, prec):
+    """Both delivery modes (per-(pre,delay) bundles with global atomics, and per
+    target-tile gather with LDS accumulation) give the oracle's bits; the gather
+    tile edge is exercised by N not a multiple of the 256-neuron tile."""
+    spec = nw.brunel(N=2900, K=290, seed=8, monitors=True)
+    _compare(spec, prec, engine, sem, None, oracle_lib, options={5: delivery}, chunks=[500, 700])
+
+
+def test_cuba_gather_forced(hs, oracle_lib):
+    """Gather on a homogeneous one-step delay (single-delay segments)."""
+    spec = nw.cuba(N=1300, seed=21)
+    _compare(spec, 64, "plain", 0, 1200, oracle_lib, options={5: 2})
+    _compare(spec, 32, "fused", 1, 1200, oracle_lib, options={5: 2})
+
+
+def test_window_is_capped_by_min_delay(hs):
+    spec = nw.brunel(N=2000, K=100, seed=1, dlo=0.3e-3, dhi=1e-3)  # delays 3..10 steps
+    net, _, _ = nw.build_hs(spec, 32, "fused", options={4: 16})
+    net.run(10)
+    assert net.stats()["window"] == 3
+
+
 @pytest.mark.parametrize("engine,sem", ENGINES)
 def test_batch_parity_small(hs, oracle_lib, engine, sem):
     """C4 scaled to 12 networks of 200..1500 neurons, FP32, 0.1 s."""
