@@ -83,6 +83,15 @@ SIGNATURES = {
     "hs_net_mon_count": (C.c_int, [P, C.c_int, I64P]),
     "hs_net_get_spikes": (C.c_int, [P, C.c_int, I32P, I64P, C.c_int64]),
     "hs_net_get_state_monitor": (C.c_int, [P, C.c_int, DP, C.c_int64]),
+    "hs_hines_create": (C.c_int, [C.c_int, C.c_double, C.c_int, I32P, I32P, I32P, I32P, I32P, DP, DP, DP, DP,
+                                  C.c_double, DP, C.c_double, C.POINTER(C.c_void_p)]),
+    "hs_hines_destroy": (None, [P]),
+    "hs_hines_set_state": (C.c_int, [P, C.c_int, DP]),
+    "hs_hines_get_state": (C.c_int, [P, C.c_int, DP]),
+    "hs_hines_set_option": (C.c_int, [P, C.c_int, C.c_longlong]),
+    "hs_hines_run": (C.c_int, [P, C.c_int64]),
+    "hs_hines_spike_counts": (C.c_int, [P, I32P]),
+    "hs_hines_stats": (C.c_int, [P, DP, DP, I64P]),
 }
 
 _lib = None

@@ -84,6 +84,12 @@ struct DPath {
     const uint32_t *gsyn;
 };
 
+/* runtime services for the translation units (hs_runtime.hip) */
+hipStream_t hs_rt_stream();
+int hs_rt_cus();
+int hs_rt_need_init();
+int hs_rt_fail(int code, const char *fmt, ...);
+
 /* launchers (hs_ops.hip, hs_engine.hip) -- all on the runtime stream, return hipError_t */
 hipError_t hs_launch_threshold(int prec, const void *v, int n, double vth, int32_t *last,
                                long long step, int ref_steps, int sorted, int32_t *spikes,

@@ -72,6 +72,20 @@ static int hs_need_init()
     return HS_OK;
 }
 
+hipStream_t hs_rt_stream() { return g_rt.stream; }
+int hs_rt_cus() { return g_rt.n_cu; }
+int hs_rt_need_init() { return hs_need_init(); }
+int hs_rt_fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
 extern "C" int hs_init(int device)
 {
     if (g_rt.stream && g_rt.device == device) return HS_OK;

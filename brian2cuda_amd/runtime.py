@@ -254,3 +254,61 @@ class Net:
         out = np.empty(max(rows * width, 1), np.float64)
         call("hs_net_get_state_monitor", self.h, mid, _f64(out), rows * width)
         return out[: rows * width].reshape(rows, width)
+
+
+class Hines:
+    """A batch of multicompartment trees on the device (hs_hines_*): one
+    workgroup per tree, the tree kept on chip for `steps per launch` steps.
+    precision: channel arithmetic (64 or 32); the cable solve is FP64."""
+
+    V, M, N, H, I = 0, 1, 2, 3, 4
+
+    def __init__(self, tb, precision=64):
+        init()
+        self.precision = int(precision)
+        self.n_comp = tb.n_comp
+        self.n_trees = tb.n_trees
+        ints = [np.ascontiguousarray(a, dtype=np.int32) for a in
+                (tb.tree_comp, tb.tree_sec, tb.sec_first, tb.sec_n, tb.sec_parent)]
+        flts = [np.ascontiguousarray(a, dtype=np.float64) for a in (tb.area, tb.rl1, tb.rl2, tb.Cm, tb.chan)]
+        h = C.c_void_p()
+        call("hs_hines_create", self.precision, float(tb.dt), tb.n_trees, *[_i32(a) for a in ints],
+             _f64(flts[0]), _f64(flts[1]), _f64(flts[2]), _f64(flts[3]), float(tb.Ri), _f64(flts[4]),
+             float(tb.vth), C.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            _lib.load_library().hs_hines_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, var, values):
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(values, np.float64), (self.n_comp,)))
+        call("hs_hines_set_state", self.h, int(var), _f64(v))
+
+    def get_state(self, var):
+        out = np.empty(self.n_comp)
+        call("hs_hines_get_state", self.h, int(var), _f64(out))
+        return out
+
+    def set_option(self, key, value):
+        call("hs_hines_set_option", self.h, int(key), int(value))
+
+    def run(self, nsteps):
+        call("hs_hines_run", self.h, int(nsteps))
+
+    def spike_counts(self):
+        out = np.empty(self.n_trees, np.int32)
+        call("hs_hines_spike_counts", self.h, _i32(out))
+        return out
+
+    def stats(self):
+        w, g, s = C.c_double(), C.c_double(), C.c_int64()
+        call("hs_hines_stats", self.h, C.byref(w), C.byref(g), C.byref(s))
+        return {"last_run_wall_s": w.value, "last_run_gpu_ms": g.value, "steps": s.value}

@@ -190,6 +190,42 @@ int hs_net_get_spikes(hs_net *net, int mid, int32_t *i, int64_t *step, int64_t c
 /* State samples, time-major [n_steps][stop-start], as double. */
 int hs_net_get_state_monitor(hs_net *net, int mid, double *out, int64_t cap);
 
+/* ---- multicompartment cells: the Hines engine ----------------------------- */
+
+typedef struct hs_hines hs_hines;
+
+/*
+ * A batch of independent SpatialNeuron trees advanced together.  Sections are
+ * numbered depth-first per tree (a parent before its children); compartments
+ * are contiguous per section.
+ *   tree_comp[n_trees+1], tree_sec[n_trees+1]: offsets of each tree's
+ *     compartments / sections in the flat arrays
+ *   sec_first, sec_n, sec_parent [n_sections]: tree-local first compartment,
+ *     compartment count, tree-local parent section (-1: the root)
+ *   area, r_length_1, r_length_2, Cm [n_compartments]: Brian2's morphology
+ *     variables (m^2, m, m, F/m^2); Ri the axial resistivity (ohm m)
+ *   chan[6] = gl, El, gNa, ENa, gK, EK (S/m^2, V): HH (1952) channels,
+ *     exponential Euler, in `precision`; the cable solve is always FP64
+ *   v_spike: somatic (compartment 0) upward crossings are counted per tree
+ * Limits: 1024 compartments and 255 sections per tree.
+ * Replaces the spatial state updater (spatialstateupdate.cu:18-93 prep,
+ * :108-113 channel kernel, :128-192 per-section Thomas, :202-307 coupling,
+ * :316-351 combine) -- one SpatialNeuron code object per tree in the reference.
+ */
+int hs_hines_create(int precision, double dt, int n_trees, const int32_t *tree_comp, const int32_t *tree_sec,
+                    const int32_t *sec_first, const int32_t *sec_n, const int32_t *sec_parent, const double *area,
+                    const double *r_length_1, const double *r_length_2, const double *Cm, double Ri,
+                    const double *chan, double v_spike, hs_hines **out);
+void hs_hines_destroy(hs_hines *h);
+/* var: 0 v, 1 m, 2 n, 3 h, 4 point current I (A) per compartment; host double[n_compartments] */
+int hs_hines_set_state(hs_hines *h, int var, const double *values);
+int hs_hines_get_state(hs_hines *h, int var, double *values);
+/* key 0: steps per launch (each workgroup keeps its tree on chip for that many steps) */
+int hs_hines_set_option(hs_hines *h, int key, long long value);
+int hs_hines_run(hs_hines *h, int64_t nsteps);
+int hs_hines_spike_counts(hs_hines *h, int32_t *counts);
+int hs_hines_stats(hs_hines *h, double *last_run_wall_s, double *last_run_gpu_ms, int64_t *steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,13 @@ def lib():
         "ref_mon_get_spikes": (C.c_int, [V, C.c_int, I32P, I64P]),
         "ref_mon_get_state": (C.c_int, [V, C.c_int, DP]),
         "ref_net_destroy": (None, [V]),
+        "ref_mc_create": (V, [C.c_int, C.c_double, C.c_int, I32P, I32P, I32P, I32P, I32P, DP, DP, DP, DP,
+                              C.c_double, DP, C.c_double]),
+        "ref_mc_set": (C.c_int, [V, C.c_int, DP]),
+        "ref_mc_get": (C.c_int, [V, C.c_int, DP]),
+        "ref_mc_run": (C.c_int, [V, C.c_int64]),
+        "ref_mc_spike_counts": (C.c_int, [V, I32P]),
+        "ref_mc_destroy": (None, [V]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -217,3 +224,56 @@ def build_ref(spec, precision=64, semantics=SEQ):
     smons = [net.add_spike_monitor(g) for g in spec.spike_monitors]
     stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
     return net, smons, stmons
+
+
+class RefMC:
+    """Batch of multicompartment trees, advanced serially as the reference's
+    spatial state updater does (cpu_ref_mc.c).  precision: channel arithmetic."""
+
+    def __init__(self, tb, precision=64):
+        self.L = lib()
+        I32P = C.POINTER(C.c_int32)
+        DP = C.POINTER(C.c_double)
+        self._keep = [np.ascontiguousarray(a, dtype=np.int32) for a in
+                      (tb.tree_comp, tb.tree_sec, tb.sec_first, tb.sec_n, tb.sec_parent)]
+        self._keepf = [np.ascontiguousarray(a, dtype=np.float64) for a in (tb.area, tb.rl1, tb.rl2, tb.Cm, tb.chan)]
+        i32 = [a.ctypes.data_as(I32P) for a in self._keep]
+        f64 = [a.ctypes.data_as(DP) for a in self._keepf]
+        self.h = self.L.ref_mc_create(precision, tb.dt, tb.n_trees, *i32, f64[0], f64[1], f64[2], f64[3], tb.Ri,
+                                      f64[4], tb.vth)
+        if not self.h:
+            raise ValueError("bad tree batch")
+        self.n_comp = tb.n_comp
+        self.n_trees = tb.n_trees
+        for var, val in tb.init.items():
+            self.set_state(var, val)
+
+    def set_state(self, var, values):
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(values, np.float64), (self.n_comp,)))
+        if self.L.ref_mc_set(self.h, var, v.ctypes.data_as(C.POINTER(C.c_double))) < 0:
+            raise ValueError("bad variable")
+
+    def get_state(self, var):
+        out = np.empty(self.n_comp)
+        if self.L.ref_mc_get(self.h, var, out.ctypes.data_as(C.POINTER(C.c_double))) < 0:
+            raise ValueError("bad variable")
+        return out
+
+    def run(self, nsteps):
+        self.L.ref_mc_run(self.h, int(nsteps))
+
+    def spike_counts(self):
+        out = np.empty(self.n_trees, np.int32)
+        self.L.ref_mc_spike_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
+
+    def close(self):
+        if self.h:
+            self.L.ref_mc_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -43,6 +43,18 @@ int ref_mon_get_spikes(ref_net *net, int mid, int32_t *i, int64_t *step);
 int ref_mon_get_state(ref_net *net, int mid, double *out);
 void ref_net_destroy(ref_net *net);
 
+/* multicompartment trees (cpu_ref_mc.c) */
+typedef struct ref_mc ref_mc;
+ref_mc *ref_mc_create(int precision, double dt, int ntrees, const int32_t *tree_comp, const int32_t *tree_sec,
+                      const int32_t *sec_first, const int32_t *sec_n, const int32_t *sec_parent, const double *area,
+                      const double *rl1, const double *rl2, const double *Cm, double Ri, const double *chan,
+                      double vth);
+int ref_mc_set(ref_mc *M, int var, const double *values);
+int ref_mc_get(ref_mc *M, int var, double *out);
+int ref_mc_run(ref_mc *M, int64_t nsteps);
+int ref_mc_spike_counts(ref_mc *M, int32_t *out);
+void ref_mc_destroy(ref_mc *M);
+
 #ifdef __cplusplus
 }
 #endif
