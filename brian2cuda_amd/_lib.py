@@ -80,6 +80,7 @@ SIGNATURES = {
     "hs_net_get_option": (C.c_int, [P, C.c_int, C.POINTER(C.c_longlong)]),
     "hs_net_run": (C.c_int, [P, C.c_int64]),
     "hs_net_stats": (C.c_int, [P, C.POINTER(HsStats)]),
+    "hs_net_kernel_time": (C.c_int, [P, C.c_int, DP, I64P]),
     "hs_net_mon_count": (C.c_int, [P, C.c_int, I64P]),
     "hs_net_get_spikes": (C.c_int, [P, C.c_int, I32P, I64P, C.c_int64]),
     "hs_net_get_state_monitor": (C.c_int, [P, C.c_int, DP, C.c_int64]),

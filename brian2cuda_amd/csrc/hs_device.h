@@ -144,6 +144,7 @@ __device__ __forceinline__ void hs_block_compact(bool pred, int idx, int32_t *li
 /* ------------------------------------------------------------------------ */
 struct HsDeliverLds {
     int pref[HS_MAX_DELAY_SLOTS + 1];
+    int wsum[16]; /* per-wave totals of the block scan (blocks of up to 1024 threads) */
 };
 
 /*
@@ -170,10 +171,13 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
     const int npairs = w * D;
     const int tq = (int)hs_mod(T, p.Q);                 /* ring slot of step T */
     const int ta = ACC ? (int)hs_mod(T, p.nacc) : 0;    /* accumulator slot of step T */
-    if (threadIdx.x < HS_WAVE) {
+    {
+        /* every thread of the block loads one count per pass: one memory
+           latency for up to blockDim.x pairs, then a block scan */
+        const int wid = threadIdx.x / HS_WAVE, nwv = blockDim.x / HS_WAVE;
         int carry = 0;
-        for (int c = 0; c < npairs; c += HS_WAVE) {
-            const int q = c + hs_lane();
+        for (int c = 0; c < npairs; c += blockDim.x) {
+            const int q = c + threadIdx.x;
             int cnt = 0;
             if (q < npairs) {
                 const int j = q / D, di = q - j * D;
@@ -181,10 +185,19 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
             }
             int tot;
             const int ex = hs_wave_excl_scan(cnt, &tot);
-            if (q < npairs) L->pref[q] = carry + ex;
-            carry += tot;
+            if (hs_lane() == 0) L->wsum[wid] = tot;
+            __syncthreads();
+            int before = carry, all = carry;
+            for (int v = 0; v < nwv; ++v) {
+                const int x = L->wsum[v];
+                before += v < wid ? x : 0;
+                all += x;
+            }
+            if (q < npairs) L->pref[q] = before + ex;
+            carry = all;
+            __syncthreads();
         }
-        if (hs_lane() == 0) L->pref[npairs] = carry;
+        if (threadIdx.x == 0) L->pref[npairs] = carry;
     }
     __syncthreads();
     const int W = L->pref[npairs];
@@ -247,6 +260,12 @@ __device__ __forceinline__ void hs_flush_events(unsigned long long ev, unsigned 
 {
     /* every lane of a wave holds the wave's count; lane 0 publishes it */
     if (hs_lane() == 0 && ev) atomicAdd(events, ev);
+}
+
+/* the same into the control block's spread counters, slot by wave rank */
+__device__ __forceinline__ void hs_flush_events(unsigned long long ev, HsCtl *ctl, long long rank)
+{
+    if (hs_lane() == 0 && ev) atomicAdd(&ctl->ev_slot[(int)(rank % HS_EV_SLOTS) * HS_EV_STRIDE], ev);
 }
 
 __device__ __forceinline__ unsigned long long hs_wave_sum(unsigned long long x)

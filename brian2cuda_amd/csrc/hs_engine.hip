@@ -231,7 +231,7 @@ __global__ __launch_bounds__(1024) void hs_k_deliver(const DPath<R> *paths, int 
     unsigned long long ev = 0;
     for (int pi = 0; pi < np; ++pi)
         if (paths[pi].dmode == HS_DELIVER_SCATTER) hs_deliver_path<R, ACC>(paths[pi], T, w, &L, rank, nw, &ev);
-    hs_flush_events(ev, &ctl->events);
+    hs_flush_events(ev, ctl, rank);
 }
 
 template <typename R, bool ACC>
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(1024) void hs_k_gather(const DPath<R> *paths, int n
         for (int tile = blockIdx.x; tile < p.n_tiles; tile += gridDim.x)
             hs_gather_tile<R, ACC>(p, tile, T, w, &L, lacc, &ev);
     }
-    hs_flush_events(ev, &ctl->events);
+    hs_flush_events(ev, ctl, (long long)blockIdx.x * (blockDim.x / HS_WAVE) + threadIdx.x / HS_WAVE);
 }
 
 template <typename R, int MS>
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void hs_k_fused(const DGroup<R> *groups, int n
         const long long nw = (long long)(gridDim.x - n_upd) * nwb;
         unsigned long long ev = 0;
         for (int pi = 0; pi < np; ++pi) hs_deliver_path<R, true>(paths[pi], T, 1, &L.d, rank, nw, &ev);
-        hs_flush_events(ev, &ctl->events);
+        hs_flush_events(ev, ctl, rank);
     }
 }
 

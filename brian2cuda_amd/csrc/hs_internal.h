@@ -28,12 +28,26 @@
 #define HS_DELIVER_SCATTER 1 /* per (pre, delay) bundles, global atomics on the targets */
 #define HS_DELIVER_GATHER 2  /* per (pre, target tile) segments, LDS accumulation per tile */
 
+/* The delivered-event count is split over HS_EV_SLOTS counters, one per 128-byte
+   line: thousands of waves add to it per launch, and same-address atomics
+   serialise in the L2 (one counter cost ~10 ns per wave). */
+#define HS_EV_SLOTS 256
+#define HS_EV_STRIDE 16
+
 struct HsCtl {
     long long base;            /* simulation step of chunk offset 0 */
     long long no_reset_before; /* resets of spikes at steps < this were already applied */
-    unsigned long long events; /* synaptic events delivered */
     unsigned long long spikes; /* spikes emitted */
+    unsigned long long pad_;
+    unsigned long long ev_slot[HS_EV_SLOTS * HS_EV_STRIDE]; /* synaptic events delivered: sum of ev_slot[k*STRIDE] */
 };
+
+static inline unsigned long long hs_ctl_events(const HsCtl &c)
+{
+    unsigned long long e = 0;
+    for (int k = 0; k < HS_EV_SLOTS; ++k) e += c.ev_slot[k * HS_EV_STRIDE];
+    return e;
+}
 
 template <typename R>
 struct DGroup {

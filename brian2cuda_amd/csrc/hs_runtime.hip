@@ -387,8 +387,13 @@ struct hs_net {
     int mode = 0;          /* 0 plain, 1 fused per-step pair, 2 fused single launch, 3 fused windows */
     int64_t step = 0;
     /* 0 update block, 1 scatter grid, 2 graph chunk, 3 fused delivery blocks,
-       4 window, 5 delivery mode, 6 gather block */
-    long long opt[7] = {256, 0, 256, 0, 0, 0, 256};
+       4 window, 5 delivery mode, 6 gather block, 7 kernel timing */
+    long long opt[8] = {256, 0, 256, 0, 0, 0, 256, 0};
+    /* kernel timing: event pairs of the current run, totals per kind */
+    std::vector<hipEvent_t> kt_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> kt_open;
+    double kt_ms[4] = {0, 0, 0, 0};
+    int64_t kt_n[4] = {0, 0, 0, 0};
     int n_scatter = 0, n_gather = 0, gather_tiles = 0;
     std::map<int64_t, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -456,6 +461,11 @@ extern "C" void hs_net_destroy(hs_net *net)
     for (void *p : net->allocs) (void)hipFree(p);
     if (net->ev0) (void)hipEventDestroy(net->ev0);
     if (net->ev1) (void)hipEventDestroy(net->ev1);
+    for (auto &o : net->kt_open) {
+        (void)hipEventDestroy(o.second.first);
+        (void)hipEventDestroy(o.second.second);
+    }
+    for (hipEvent_t e : net->kt_pool) (void)hipEventDestroy(e);
     delete net;
 }
 
@@ -578,16 +588,15 @@ extern "C" int hs_net_add_pathway(hs_net *net, int src, int tgt, int var, int64_
             return hs_fail(HS_ERR_ARG, "synapse %lld: index out of range", (long long)s);
     /*
      * Delivery mode.  Gather pays one (pre, tile) lookup per candidate spike and
-     * target tile and reads each synapse once per window its spike is in range
-     * of; scatter pays one global atomic per event.  Gather wins for dense,
-     * heterogeneous-delay connectivity (many synapses per (pre, tile), delay span
-     * >= 4 steps); scatter for sparse or block-diagonal connectivity.
+     * target tile and reads every synapse of a spike in each window the spike's
+     * delay span overlaps ((D + w - 1) / w times); scatter reads each delivered
+     * synapse once and pays one global atomic per event.  On the 100k-neuron,
+     * 46-delay network scatter is 1.3x (FP32, w = 5) to 1.9x (FP64, w = 1)
+     * faster, so scatter is the default and gather is chosen explicitly.
      */
     const int n_tiles = (gt.n + HS_GTILE - 1) / HS_GTILE;
-    const double per_seg = nsyn > 0 ? (double)nsyn / ((double)gs.n * n_tiles) : 0.0;
     int dmode = HS_DELIVER_SCATTER;
     if (net->opt[5] == HS_DELIVER_GATHER) dmode = HS_DELIVER_GATHER;
-    else if (net->opt[5] == 0 && D >= 4 && per_seg >= 0.5) dmode = HS_DELIVER_GATHER;
     if (dmode == HS_DELIVER_GATHER && (double)gs.n * n_tiles > 2.0e9)
         return hs_fail(HS_ERR_UNSUPPORTED, "gather table of %d x %d segments too large", gs.n, n_tiles);
     P.dmode = dmode;
@@ -724,7 +733,13 @@ extern "C" int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start
 
 extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
 {
-    if (!net || key < 0 || key > 6) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (!net || key < 0 || key > 7) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (key == 7 && (value < 0 || value > 1)) return hs_fail(HS_ERR_ARG, "kernel timing: 0 off, 1 on");
+    if (key == 7)
+        for (int q = 0; q < 4; ++q) {
+            net->kt_ms[q] = 0;
+            net->kt_n[q] = 0;
+        }
     if (key == 6 && (value < 64 || value > 1024 || value % 64))
         return hs_fail(HS_ERR_ARG, "gather block size must be a multiple of 64 in [64,1024]");
     if (key == 5 && (value < 0 || value > 2)) return hs_fail(HS_ERR_ARG, "delivery mode: 0 auto, 1 scatter, 2 gather");
@@ -742,7 +757,7 @@ extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
 
 extern "C" int hs_net_get_option(hs_net *net, int key, long long *value)
 {
-    if (!net || key < 0 || key > 6 || !value) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (!net || key < 0 || key > 7 || !value) return hs_fail(HS_ERR_ARG, "bad option key");
     *value = net->opt[key];
     return HS_OK;
 }
@@ -938,6 +953,55 @@ static int hs_update_grid(const hs_net *net)
     return std::max(1, std::min(net->total_tiles, 1 << 20));
 }
 
+/* kernel timing (option 7): an event pair around one launch of kind `kind` */
+static hipEvent_t hs_kt_event(hs_net *net)
+{
+    if (!net->kt_pool.empty()) {
+        hipEvent_t e = net->kt_pool.back();
+        net->kt_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+template <typename F>
+static int hs_kt_launch(hs_net *net, int kind, F launch)
+{
+    if (!net->opt[7]) return launch();
+    hipEvent_t a = hs_kt_event(net), b = hs_kt_event(net);
+    if (!a || !b) return hs_fail(HS_ERR_HIP, "hipEventCreate failed");
+    HS_HIP(hipEventRecord(a, g_rt.stream));
+    const int r = launch();
+    HS_HIP(hipEventRecord(b, g_rt.stream));
+    net->kt_open.push_back({kind, {a, b}});
+    return r;
+}
+
+/* after the stream has drained: add the open pairs to the totals */
+static int hs_kt_collect(hs_net *net)
+{
+    for (auto &o : net->kt_open) {
+        float ms = 0;
+        HS_HIP(hipEventElapsedTime(&ms, o.second.first, o.second.second));
+        net->kt_ms[o.first] += ms;
+        net->kt_n[o.first] += 1;
+        net->kt_pool.push_back(o.second.first);
+        net->kt_pool.push_back(o.second.second);
+    }
+    net->kt_open.clear();
+    return HS_OK;
+}
+
+extern "C" int hs_net_kernel_time(hs_net *net, int kind, double *total_ms, int64_t *launches)
+{
+    if (!net || kind < 0 || kind > 3) return hs_fail(HS_ERR_ARG, "kernel kind: 0 update, 1 scatter, 2 gather, 3 fused");
+    if (total_ms) *total_ms = net->kt_ms[kind];
+    if (launches) *launches = net->kt_n[kind];
+    return HS_OK;
+}
+
 /*
  * enqueue k steps (chunk offsets 0..k-1) and the tick.
  *   mode 0 plain          : update(t), deliver(t) in place
@@ -959,26 +1023,41 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
     const int ggrid = std::max(1, std::min(net->gather_tiles, 1 << 20));
     auto deliver = [&](int koff, int w) -> int {
         if (net->n_scatter)
-            HS_HIP(hs_launch_deliver<R>(dp, np, net->ctl, koff, w, fused, 256, hs_deliver_grid(net), g_rt.stream));
+            HS_TRY(hs_kt_launch(net, 1, [&]() -> int {
+                HS_HIP(hs_launch_deliver<R>(dp, np, net->ctl, koff, w, fused, 256, hs_deliver_grid(net),
+                                            g_rt.stream));
+                return HS_OK;
+            }));
         if (net->n_gather)
-            HS_HIP(hs_launch_gather<R>(dp, np, net->ctl, koff, w, fused, (int)net->opt[6], ggrid, g_rt.stream));
+            HS_TRY(hs_kt_launch(net, 2, [&]() -> int {
+                HS_HIP(hs_launch_gather<R>(dp, np, net->ctl, koff, w, fused, (int)net->opt[6], ggrid, g_rt.stream));
+                return HS_OK;
+            }));
         return HS_OK;
+    };
+    auto update = [&](int koff, int w, int wmax, int fu) -> int {
+        return hs_kt_launch(net, 0, [&]() -> int {
+            HS_HIP(hs_launch_update<R>(dg, ng, net->total_tiles, net->ctl, koff, w, wmax, fu, ms, block,
+                                       hs_update_grid(net), g_rt.stream));
+            return HS_OK;
+        });
     };
     if (net->mode == 3) {
         for (int64_t j = 0; j < k; j += net->window) {
             const int w = (int)std::min<int64_t>(net->window, k - j);
             HS_TRY(deliver((int)j, w));
-            HS_HIP(hs_launch_update<R>(dg, ng, net->total_tiles, net->ctl, (int)j, w, net->window, 1, ms, block,
-                                       hs_update_grid(net), g_rt.stream));
+            HS_TRY(update((int)j, w, net->window, 1));
         }
     } else {
         for (int64_t j = 0; j < k; ++j) {
             if (net->mode == 2) {
-                HS_HIP(hs_launch_fused<R>(dg, ng, net->total_tiles, dp, np, net->ctl, (int)j, ms, block,
-                                          hs_update_grid(net), hs_fused_deliver_blocks(net), g_rt.stream));
+                HS_TRY(hs_kt_launch(net, 3, [&]() -> int {
+                    HS_HIP(hs_launch_fused<R>(dg, ng, net->total_tiles, dp, np, net->ctl, (int)j, ms, block,
+                                              hs_update_grid(net), hs_fused_deliver_blocks(net), g_rt.stream));
+                    return HS_OK;
+                }));
             } else {
-                HS_HIP(hs_launch_update<R>(dg, ng, net->total_tiles, net->ctl, (int)j, 1, 1, fused, ms, block,
-                                           hs_update_grid(net), g_rt.stream));
+                HS_TRY(update((int)j, 1, 1, fused));
                 if (np > 0) HS_TRY(deliver((int)j, 1));
             }
         }
@@ -994,7 +1073,7 @@ static int hs_enqueue_steps(hs_net *net, int64_t k)
 
 static int hs_chunk(hs_net *net, int64_t k)
 {
-    if (net->opt[2] <= 0) return hs_enqueue_steps(net, k);
+    if (net->opt[2] <= 0 || net->opt[7]) return hs_enqueue_steps(net, k);
     auto it = net->graphs.find(k);
     if (it == net->graphs.end()) {
         hipGraph_t graph;
@@ -1089,16 +1168,18 @@ extern "C" int hs_net_run(hs_net *net, int64_t nsteps)
                                          net->ctl, fused, block, g_rt.stream));
     HS_HIP(hipStreamSynchronize(g_rt.stream));
     auto t1 = std::chrono::steady_clock::now();
+    HS_TRY(hs_kt_collect(net));
     float ms = 0;
     HS_HIP(hipEventElapsedTime(&ms, net->ev0, net->ev1));
     net->last_gpu_ms = ms;
     net->last_wall = std::chrono::duration<double>(t1 - t0).count();
     net->last_steps = nsteps;
     net->step += nsteps;
-    HsCtl c;
+    std::vector<HsCtl> cbuf(1);
+    HsCtl &c = cbuf[0];
     HS_HIP(hipMemcpyAsync(&c, net->ctl, sizeof(c), hipMemcpyDeviceToHost, g_rt.stream));
     HS_HIP(hipStreamSynchronize(g_rt.stream));
-    net->events = c.events;
+    net->events = hs_ctl_events(c);
     net->spikes = c.spikes;
     if (c.base != net->step) return hs_fail(HS_ERR_STATE, "device step %lld != host step %lld", c.base, (long long)net->step);
     for (const HMon &M : net->m) {

@@ -27,7 +27,10 @@ MODEL_VARS = {
 NEVER_SPIKED = -1073741824
 
 OPT_UPDATE_BLOCK, OPT_DELIVER_GRID, OPT_GRAPH_CHUNK, OPT_FUSED_GRID, OPT_WINDOW, OPT_DELIVERY = 0, 1, 2, 3, 4, 5
+OPT_GATHER_BLOCK, OPT_KERNEL_TIMING = 6, 7
 DELIVERY_AUTO, DELIVERY_SCATTER, DELIVERY_GATHER = 0, 1, 2
+KT_UPDATE, KT_SCATTER, KT_GATHER, KT_FUSED = 0, 1, 2, 3
+KERNEL_NAMES = {KT_UPDATE: "hs_k_update", KT_SCATTER: "hs_k_deliver", KT_GATHER: "hs_k_gather", KT_FUSED: "hs_k_fused"}
 
 
 def _ptr(a):
@@ -235,6 +238,12 @@ class Net:
         s = HsStats()
         call("hs_net_stats", self.h, C.byref(s))
         return {k: getattr(s, k) for k, _ in HsStats._fields_}
+
+    def kernel_time(self, kind):
+        """(total device ms, launches) of one kernel kind (KT_*) while OPT_KERNEL_TIMING is on."""
+        ms, n = C.c_double(), C.c_int64()
+        call("hs_net_kernel_time", self.h, int(kind), C.byref(ms), C.byref(n))
+        return ms.value, n.value
 
     def mon_count(self, mid):
         n = C.c_int64()

@@ -157,10 +157,18 @@ int hs_net_add_spike_monitor(hs_net *net, int gid, int64_t capacity, int *mid);
 /* StateMonitor (when='start') of var over neurons [start, stop). */
 int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start, int stop, int *mid);
 
-/* Tuning knobs (autotuner): key 0 update block size, 1 delivery blocks,
- * 2 steps per captured graph (0: no graph), 3 fused-kernel grid size. */
+/* Tuning knobs: key 0 update block size, 1 scatter-delivery grid, 2 steps per
+ * captured graph (0: no graph), 3 delivery blocks of the single-launch fused
+ * kernel, 4 window (steps per update launch, 0: largest allowed), 5 delivery
+ * layout (0 auto, 1 scatter, 2 gather), 6 gather block size, 7 kernel timing
+ * (1: no graphs, every launch bracketed by HIP events; totals reset when set). */
 int hs_net_set_option(hs_net *net, int key, long long value);
 int hs_net_get_option(hs_net *net, int key, long long *value);
+/* With option 7 on: device time and launch count, summed over the runs since
+ * the option was set, of one kernel kind: 0 update, 1 scatter delivery,
+ * 2 gather delivery, 3 single-launch fused step.  No reference counterpart
+ * (the reference is profiled with nvprof). */
+int hs_net_kernel_time(hs_net *net, int kind, double *total_ms, int64_t *launches);
 
 /*
  * Advance nsteps time steps.  Steps run in chunks captured as HIP graphs; no

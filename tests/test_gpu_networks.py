@@ -114,3 +114,23 @@ def test_batch_parity_small(hs, oracle_lib, engine, sem):
     """C4 scaled to 12 networks of 200..1500 neurons, FP32, 0.1 s."""
     spec = nw.batch_cuba(n_nets=12, nmin=200, nmax=1500, seed=9)
     _compare(spec, 32, engine, sem, 1000, oracle_lib)
+
+
+@pytest.mark.parametrize("engine,sem,delivery", [("plain", 0, 1), ("fused", 1, 2), ("fused", 1, 1)])
+def test_kernel_timing_keeps_results(hs, oracle_lib, engine, sem, delivery):
+    """Kernel timing (no graphs, an event pair per launch) changes no bits and
+    counts one update launch per window and one delivery launch per window."""
+    from brian2cuda_amd import runtime as rt
+    spec = nw.brunel(N=2500, K=250, seed=6, monitors=True)
+    net, _ = _compare(spec, 32, engine, sem, None, oracle_lib, options={5: delivery, 7: 1}, chunks=[300, 211])
+    w = net.stats()["window"]
+    chunk = net.get_option(rt.OPT_GRAPH_CHUNK)  # a run advances in chunks; a window never spans two
+    pieces = [min(chunk, k - j) for k in (300, 211) for j in range(0, k, chunk)]
+    launches = sum(-(-p // w) for p in pieces)
+    ms_u, n_u = net.kernel_time(rt.KT_UPDATE)
+    kind = rt.KT_SCATTER if delivery == 1 else rt.KT_GATHER
+    ms_d, n_d = net.kernel_time(kind)
+    assert n_u == launches and n_d == launches
+    assert ms_u > 0 and ms_d > 0
+    other = rt.KT_GATHER if delivery == 1 else rt.KT_SCATTER
+    assert net.kernel_time(other) == (0.0, 0)
