@@ -6,9 +6,9 @@ neurons, connect(p=1000/N) (about 1e8 synapses, float32 weights J=-0.1 mV),
 delays U[0.5, 5] ms quantised to dt=0.1 ms (5..50 steps), sigma=1 mV, mu=25 mV,
 Philox noise; 1 s of simulated time (10,000 steps) after a warm-up.
 
-  fp32 batched : FP32 state, weights and arithmetic; one launch per step (update
-                 of step t and delivery of step t in one kernel), K steps per
-                 captured HIP graph
+  fp32 batched : FP32 state, weights and arithmetic; effects accumulated per
+                 step, W = min(8, min delay) steps per update launch and per
+                 delivery launch, K steps per captured HIP graph
   fp64 plain   : FP64 state, weights and arithmetic; update kernel + delivery
                  kernel per step, effects applied in place (Brian2's order)
 
@@ -17,23 +17,32 @@ are built on the host from a seed and copied to HBM before timing; the timed
 region is the step loop only (like Network::_last_run_time).  `value` is the
 FP32 batched form's synaptic events per wall-clock second over the K timed steps.
 
-Roofline: the dominant kernel is the fused step kernel of the FP32 form.  Its
-algorithmic bytes per launch (one step) are
-   N*20 + n_spk*4 + n_spk*D*12 + E*8
-(state: V r/w 4+4, last-spike r 4, accumulator r/w 4+4; spike list 4 per spike;
-per (spike, delay) bundle: spike index 4 + two bundle offsets 8; per event: post
-index 4 + weight 4), with n_spk and E the measured per-step means.  The launch
-duration is the device time of the timed chunks (HIP events on the runtime's
-stream) divided by the number of launches.
+Roofline (dominant kernel of the FP32 form, the one with the largest device time
+in a kernel-timed run of the same network): algorithmic bytes per launch are the
+terms of the per-step byte model that the kernel touches, summed over the W steps
+of a launch, with n_spk and E the measured means of the timed run:
+  update  : N*(r_s + w_s) + n_spk*4          r_s = V + lastspike + not_ref =
+            (FP32) 4+4+1, w_s = V + not_ref = 4+1 (halved FP fields of 17/9)
+  delivery: n_spk*(8 + 12*b) + E*(4 + w_b)   b = delay slots per spike (46),
+            w_b = 4 (FP32 per-synapse weight)
+The average launch duration comes from HIP events around every launch on the
+runtime's stream (hs_net_set_option(net, 7, 1): no graphs in that run).
+
+Drift: both forms are re-run with spike monitors over `--drift-steps` steps;
+spikes of each neuron are matched greedily in time order (pairs within 5 ms) and
+the RMS of the matched time differences is reported with the unmatched fraction.
 
 CPU baseline: the oracle (serial C, cpp_standalone semantics, FP64) on the same
 network, timed over a bounded window of steps on one host core.
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -43,22 +52,6 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def bytes_per_step(workload, N, n_spk, E, D, prec, engine):
-    """Algorithmic HBM bytes of one step of the headline kernel(s)."""
-    r = 4 if prec == 32 else 8
-    if workload == "brunel":
-        state = N * (r + 4 + r)          # V read, last read, V write
-        if engine == "fused":
-            state += N * 2 * r           # accumulator read + zero
-        wbytes = r                       # per-synapse weight
-    else:  # CUBA-type (cuba, batch): v, ge, gi r/w
-        state = N * (3 * r + 4 + 3 * r)
-        if engine == "fused":
-            state += N * 2 * 2 * r       # two accumulators (ge, gi)
-        wbytes = 0
-    return state + n_spk * 4 + n_spk * D * 12 + E * (4 + wbytes)
 
 
 def make_spec(workload, args):
@@ -72,6 +65,21 @@ def make_spec(workload, args):
     if workload == "cobahh":
         return nw.cobahh(N=args.N, seed=args.seed, monitors=False)
     raise ValueError(workload)
+
+
+def state_bytes(workload, prec):
+    """(r_s, w_s): bytes of per-neuron state read and written per step (Brian2's arrays)."""
+    f = 8 if prec == 64 else 4
+    if workload == "brunel":       # V, lastspike, not_refractory
+        return 2 * f + 1, f + 1
+    if workload in ("cuba", "batch"):  # v, ge, gi, lastspike, not_refractory
+        return 4 * f + 1, 3 * f + 1
+    return 7 * f + 1, 6 * f + 1    # COBAHH: v, m, n, h, ge, gi, lastspike, not_refractory
+
+
+def weight_bytes(spec, prec):
+    from brian2cuda_amd.runtime import HS_W_SYNAPSE
+    return (8 if prec == 64 else 4) if any(p.wmode == HS_W_SYNAPSE for p in spec.paths) else 0
 
 
 def run_form(spec, prec, engine, steps, warmup, options=None):
@@ -105,6 +113,80 @@ def run_form(spec, prec, engine, steps, warmup, options=None):
     return res
 
 
+def kernel_times(spec, prec, engine, steps, warmup):
+    """Per-kernel device time with an event pair around every launch."""
+    from brian2cuda_amd import networks as nw
+    from brian2cuda_amd import runtime as rt
+    net, _, _ = nw.build_hs(spec, prec, engine)
+    net.run(warmup)
+    net.set_option(rt.OPT_KERNEL_TIMING, 1)
+    s0 = net.stats()
+    net.run(steps)
+    s1 = net.stats()
+    out = {}
+    for kind, name in rt.KERNEL_NAMES.items():
+        ms, n = net.kernel_time(kind)
+        if n:
+            out[name] = {"total_ms": ms, "launches": n, "avg_us": 1e3 * ms / n}
+    window = int(s1["window"])
+    spikes = s1["spikes"] - s0["spikes"]
+    events = s1["events"] - s0["events"]
+    net.close()
+    return out, window, spikes / steps, events / steps
+
+
+def spike_trains(spec, prec, engine, steps):
+    from brian2cuda_amd import networks as nw
+    sp = dataclasses.replace(spec, spike_monitors=[0])
+    net, smon, _ = nw.build_hs(sp, prec, engine, spike_capacity=1 << 24)
+    net.run(steps)
+    i, t = net.spikes(smon[0])
+    net.close()
+    return i, t
+
+
+def drift(spec, steps, tol_steps=50):
+    """Greedy time-ordered matching of FP32-batched to FP64-plain spikes per neuron."""
+    i64, t64 = spike_trains(spec, 64, "plain", steps)
+    i32, t32 = spike_trains(spec, 32, "fused", steps)
+    res = match_spikes(i64, t64, i32, t32, spec.n_neurons, tol_steps)
+    res["steps"] = steps
+    return res
+
+
+def match_spikes(i64, t64, i32, t32, n, tol_steps):
+    """Per neuron, walk both time-sorted trains: a pair within tol_steps matches,
+    otherwise the earlier spike is unmatched."""
+    o64 = np.lexsort((t64, i64))
+    o32 = np.lexsort((t32, i32))
+    i64, t64, i32, t32 = i64[o64], t64[o64], i32[o32], t32[o32]
+    b64 = np.searchsorted(i64, np.arange(n + 1))
+    b32 = np.searchsorted(i32, np.arange(n + 1))
+    d = []
+    unmatched = 0
+    for k in range(n):
+        a = t64[b64[k]:b64[k + 1]]
+        b = t32[b32[k]:b32[k + 1]]
+        p = q = 0
+        while p < len(a) and q < len(b):
+            if abs(int(a[p]) - int(b[q])) <= tol_steps:
+                d.append(int(b[q]) - int(a[p]))
+                p += 1
+                q += 1
+            elif a[p] < b[q]:
+                p += 1
+                unmatched += 1
+            else:
+                q += 1
+                unmatched += 1
+        unmatched += (len(a) - p) + (len(b) - q)
+    d = np.asarray(d, dtype=np.float64)
+    total = len(t64) + len(t32)
+    return {"spikes_fp64": int(len(t64)), "spikes_fp32": int(len(t32)),
+            "matched_pairs": int(d.size), "rms_dt_steps": float(np.sqrt(np.mean(d * d))) if d.size else None,
+            "unmatched_frac": unmatched / total if total else 0.0, "match_window_steps": tol_steps}
+
+
 def cpu_baseline(spec, skip_steps, steps):
     """Oracle (cpp_standalone semantics, FP64, 1 core) on a window of steps."""
     import oracle
@@ -134,6 +216,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--cpu-steps", type=int, default=300, help="oracle timed steps (0: skip)")
     ap.add_argument("--cpu-skip", type=int, default=300, help="oracle untimed steps before its window")
+    ap.add_argument("--kt-steps", type=int, default=2000, help="steps of the kernel-timed run (0: skip)")
+    ap.add_argument("--drift-steps", type=int, default=10000, help="steps of the drift comparison (0: skip)")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -160,16 +244,32 @@ def main():
         log("fp64 plain", fp64)
 
     N = spec.n_neurons
-    n_spk = fp32["spikes"] / args.steps
-    E = fp32["events"] / args.steps
-    bpl = bytes_per_step(args.workload, N, n_spk, E, D, 32, "fused")
-    # device time per simulated step over the timed region (all kernels of the step)
-    step_s = fp32["gpu_ms"] / 1e3 / args.steps
-    achieved = bpl / step_s / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                "kernel": "fused step kernels (FP32 form)", "bytes_per_step": int(bpl),
-                "step_us": round(step_s * 1e6, 3)}
+    roofline = None
+    if args.kt_steps > 0:
+        kt, W, n_spk, E = kernel_times(spec, 32, "fused", args.kt_steps, args.warmup)
+        log("kernel times (fp32 batched)", kt)
+        r_s, w_s = state_bytes(args.workload, 32)
+        w_b = weight_bytes(spec, 32)
+        per_step = {
+            "hs_k_update": N * (r_s + w_s) + n_spk * 4,
+            "hs_k_deliver": n_spk * (8 + 12 * D) + E * (4 + w_b),
+            "hs_k_gather": n_spk * (8 + 12 * D) + E * (4 + w_b),
+            "hs_k_fused": N * (r_s + w_s) + n_spk * (12 + 12 * D) + E * (4 + w_b),
+        }
+        name = max(kt, key=lambda k: kt[k]["total_ms"])
+        steps_per_launch = args.kt_steps / kt[name]["launches"]
+        bpl = per_step[name] * steps_per_launch
+        achieved = bpl / (kt[name]["avg_us"] * 1e-6) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                    "kernel": name, "bytes_per_launch": int(bpl), "steps_per_launch": round(steps_per_launch, 3),
+                    "avg_launch_us": round(kt[name]["avg_us"], 3),
+                    "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kt.items()}}
+
+    dr = None
+    if args.drift_steps > 0 and not args.no_fp64 and args.workload == "brunel":
+        dr = drift(spec, args.drift_steps)
+        log("drift", dr)
 
     cpu = None
     if args.cpu_steps > 0:
@@ -202,6 +302,7 @@ def main():
         "fp64_plain": ({k: (round(v, 6) if isinstance(v, float) else v) for k, v in fp64.items()} if fp64 else None),
         "speedup_fp32_batched_vs_fp64_plain": (round(fp32["events_per_s"] / fp64["events_per_s"], 4)
                                                if fp64 and fp64["events_per_s"] > 0 else None),
+        "drift_fp32_batched_vs_fp64_plain": dr,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "device": info,

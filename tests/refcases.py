@@ -12,8 +12,8 @@ reference's expectations state.
 """
 import numpy as np
 
-from .networks import GroupSpec, NetSpec, PathSpec
-from .runtime import HS_MODEL_LIF_CUBA, HS_W_SCALAR, HS_W_SYNAPSE
+from brian2cuda_amd.networks import GroupSpec, NetSpec, PathSpec
+from brian2cuda_amd.runtime import HS_MODEL_LIF_CUBA, HS_W_SCALAR, HS_W_SYNAPSE
 
 INF_TAU = 1e30  # exp(-dt/tau) rounds to 1.0: v is exactly constant
 

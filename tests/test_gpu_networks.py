@@ -134,3 +134,31 @@ def test_kernel_timing_keeps_results(hs, oracle_lib, engine, sem, delivery):
     assert ms_u > 0 and ms_d > 0
     other = rt.KT_GATHER if delivery == 1 else rt.KT_SCATTER
     assert net.kernel_time(other) == (0.0, 0)
+
+
+@pytest.mark.parametrize("case", ["push_outer_loop", "push_outer_loop2", "circular_eventspaces"])
+@pytest.mark.parametrize("engine", ["plain", "fused"])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_reference_propagation_cases_gpu(hs, case, engine, prec):
+    """The reference's spike-propagation expectations (test_synaptic_propagations.py),
+    checked on the GPU directly against the reference's expected index sets."""
+    import refcases as rc
+    spec, steps, expect = getattr(rc, case)()
+    net, sm, _ = nw.build_hs(spec, prec, engine)
+    net.run(steps)
+    got = rc.spikes_by_step(*net.spikes(sm[0]))
+    assert sorted(got) == sorted(expect)
+    for k in expect:
+        np.testing.assert_array_equal(got[k], expect[k])
+
+
+@pytest.mark.parametrize("engine", ["plain", "fused"])
+def test_reference_effects_cancel_gpu(hs, engine):
+    """test_synaptic_effect_modes: +1 and -1 onto every target each step cancel exactly."""
+    import refcases as rc
+    spec, steps, _ = rc.effects_cancel()
+    net, sm, _ = nw.build_hs(spec, 64, engine)
+    net.run(steps)
+    assert len(net.spikes(sm[0])[0]) == 0
+    assert np.all(net.get_state(1, 0) == 0.0)
+    assert net.stats()["events"] == 2 * 15360 * (steps - 1)

@@ -9,7 +9,7 @@ from scipy import linalg, stats
 
 import oracle
 from brian2cuda_amd import networks as nw
-from brian2cuda_amd import refcases as rc
+import refcases as rc
 
 # ---------------------------------------------------------------------------
 # Philox4x32-10: Random123 known-answer vectors (Salmon et al., SC'11)
