@@ -8,7 +8,7 @@
  * spiking network one ulp becomes a different spike train within a few hundred
  * milliseconds.  Every transcendental on the per-step path is therefore evaluated
  * here with IEEE-754 operations whose results are fixed by the standard
- * (+ - * /, floor, frexp, ldexp, int<->float conversion), in a fixed order and
+ * (+ - * /, sqrt, floor, frexp, ldexp, int<->float conversion), in a fixed order and
  * with no fused multiply-add.  The expression text is the contract; the host
  * oracle and the device kernels include this one header.
  *
@@ -18,7 +18,7 @@
  *          Taylor series to r^13 (truncation < 4e-18 relative); 2^k by ldexp.
  *   log  : x = m 2^e, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s=(m-1)/(m+1),
  *          |s| <= 0.1716, odd series to s^21 (truncation < 2e-17).
- *   sqrt : exponent halving + 6 Newton steps y <- (y + x/y)/2 from a linear seed.
+ *   sqrt : the IEEE operation (correctly rounded by the standard).
  *   sin/cos(2 pi u): quadrant split of u (exact: u*4 is a power-of-two scaling),
  *          reflection to [0, pi/4], Taylor series to a^17 / a^18.
  * Accuracy versus a correctly rounded libm is checked by the CPU tests
@@ -113,24 +113,12 @@ HS_HD double hs_log(double x)
     return ed * HS_LN2_HI + (ed * HS_LN2_LO + lm);
 }
 
-/* square root for finite x >= 0 */
+/* square root: IEEE-754 requires it correctly rounded, like + - * /, so the
+   hardware/library operation is already deterministic (x86 sqrtsd; the gfx950
+   f64 expansion rsq + Newton + fix-up is correctly rounded) */
 HS_HD double hs_sqrt(double x)
 {
-    if (!(x > 0.0)) return x * 0.0; /* 0 -> 0, negative/NaN -> NaN-ish, unused */
-    int e;
-    double m = frexp(x, &e); /* x = m 2^e, m in [0.5,1) */
-    if (e & 1) {             /* make the exponent even: m in [0.25, 1) */
-        m = m * 0.5;
-        e = e + 1;
-    }
-    double y = 0.41731 + 0.59016 * m; /* linear seed on [0.25,1): rel err < 2.5e-2 */
-    y = 0.5 * (y + m / y);
-    y = 0.5 * (y + m / y);
-    y = 0.5 * (y + m / y);
-    y = 0.5 * (y + m / y);
-    y = 0.5 * (y + m / y);
-    y = 0.5 * (y + m / y);
-    return ldexp(y, e / 2);
+    return sqrt(x);
 }
 
 /* sin(a), cos(a) for a in [0, pi/4] by Taylor series */
@@ -191,6 +179,86 @@ HS_HD double hs_cos2pi(double u)
 HS_HD double hs_box_muller(double u1, double u2)
 {
     return hs_sqrt(-2.0 * hs_log(u1)) * hs_cos2pi(u2);
+}
+
+/* ---- single precision (the noise of the FP32 form) ------------------------
+ * Same methods with float arithmetic throughout: the series stop where the
+ * truncation falls below 2^-26 relative.  IEEE float + - * / sqrt are correctly
+ * rounded on both sides (x86 SSE; gfx950 with correctly rounded float divide
+ * and sqrt, clang's default for HIP).
+ */
+HS_HD float hs_logf(float x)
+{
+    int e;
+    float m = frexpf(x, &e); /* m in [0.5, 1) */
+    if (m < 0.70710677f) {
+        m = m * 2.0f;
+        e = e - 1;
+    }
+    float s = (m - 1.0f) / (m + 1.0f);          /* |s| <= 0.1716 */
+    float s2 = s * s;
+    float q = 0.11111111f;                        /* 1/9; next term 2 s^11/11 < 2e-9 |lm| */
+    q = 0.14285715f + s2 * q;                     /* 1/7 */
+    q = 0.2f + s2 * q;                            /* 1/5 */
+    q = 0.33333334f + s2 * q;                     /* 1/3 */
+    float lm = 2.0f * s + (2.0f * s) * (s2 * q);
+    float ed = (float)e;                          /* |e| < 2^8: ed * HS_LN2F_HI exact */
+    return ed * HS_LN2F_HI + (ed * HS_LN2F_LO + lm);
+}
+
+HS_HD float hs_sqrtf(float x)
+{
+    return sqrtf(x);
+}
+
+/* sin(a), cos(a) for a in [0, pi/4]: to a^9 (trunc. 1.7e-9), to a^10 (1.1e-10) */
+HS_HD float hs_sin_redf(float a)
+{
+    float a2 = a * a;
+    float p = 2.7557319e-06f;                     /* 1/9! */
+    p = -0.0001984127f + a2 * p;                  /* -1/7! */
+    p = 0.008333334f + a2 * p;                    /* 1/5! */
+    p = -0.16666667f + a2 * p;                    /* -1/3! */
+    return a + a * (a2 * p);
+}
+
+HS_HD float hs_cos_redf(float a)
+{
+    float a2 = a * a;
+    float p = 2.7557319e-07f;                     /* 1/10! */
+    p = -2.4801588e-05f + a2 * p;                 /* -1/8! */
+    p = 0.0013888889f + a2 * p;                   /* 1/6! */
+    p = -0.041666668f + a2 * p;                   /* -1/4! */
+    p = 0.5f + a2 * p;
+    return 1.0f - a2 * p;
+}
+
+/* cos(2 pi u) for u in [0, 1) */
+HS_HD float hs_cos2pif(float u)
+{
+    float t = u * 4.0f;
+    float qf = floorf(t);
+    float f = t - qf; /* exact */
+    int q = (int)qf & 3;
+    float s, c;
+    if (f <= 0.5f) {
+        float a = f * 1.5707964f;
+        s = hs_sin_redf(a);
+        c = hs_cos_redf(a);
+    } else {
+        float a = (1.0f - f) * 1.5707964f;
+        s = hs_cos_redf(a);
+        c = hs_sin_redf(a);
+    }
+    if (q == 0) return c;
+    if (q == 1) return -s;
+    if (q == 2) return -c;
+    return s;
+}
+
+HS_HD float hs_box_mullerf(float u1, float u2)
+{
+    return hs_sqrtf(-2.0f * hs_logf(u1)) * hs_cos2pif(u2);
 }
 
 #endif /* HS_DETMATH_H */

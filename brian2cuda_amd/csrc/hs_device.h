@@ -76,7 +76,10 @@ template <> struct HsNV<HS_MODEL_HH_COBA> { static constexpr int value = HS_HH_N
 template <typename R, int MODEL>
 __device__ __forceinline__ R hs_noise(unsigned long long seed, unsigned int stream, int i, long long t)
 {
-    if (MODEL == HS_MODEL_LIF_NOISE) return (R)hs_randn(seed, stream, (uint32_t)i, (int64_t)t);
+    if (MODEL == HS_MODEL_LIF_NOISE) {
+        if (sizeof(R) == sizeof(float)) return (R)hs_randn_f32(seed, stream, (uint32_t)i, (int64_t)t);
+        return (R)hs_randn(seed, stream, (uint32_t)i, (int64_t)t);
+    }
     return (R)0;
 }
 

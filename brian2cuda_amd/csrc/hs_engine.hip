@@ -88,9 +88,14 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int tile, lon
         constexpr bool NOISY = MODEL == HS_MODEL_LIF_NOISE;
         R xi[NOISY ? HS_MAX_WINDOW : 1];
         if (NOISY) {
+            if (g.xi) {
 #pragma unroll
-            for (int j = 0; j < HS_MAX_WINDOW; ++j)
-                xi[j] = j < w ? hs_noise<R, MODEL>(g.seed, g.stream, i, T + j) : (R)0;
+                for (int j = 0; j < HS_MAX_WINDOW; ++j) xi[j] = j < w ? g.xi[(long long)j * g.n + i] : (R)0;
+            } else {
+#pragma unroll
+                for (int j = 0; j < HS_MAX_WINDOW; ++j)
+                    xi[j] = j < w ? hs_noise<R, MODEL>(g.seed, g.stream, i, T + j) : (R)0;
+            }
         }
         /* the effects due at steps T-1 .. T+w-2 were all delivered before this
            launch: small models load the whole window's values at once (one
@@ -221,7 +226,8 @@ __global__ __launch_bounds__(256) void hs_k_update(const DGroup<R> *groups, int 
 }
 
 template <typename R, bool ACC>
-__global__ __launch_bounds__(1024) void hs_k_deliver(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w)
+__global__ __launch_bounds__(1024) void hs_k_deliver(const DPath<R> *paths, int np, const DGroup<R> *groups, int ng,
+                                                     HsCtl *ctl, int koff, int w)
 {
     __shared__ HsDeliverLds L;
     const long long T = ctl->base + koff;
@@ -232,6 +238,21 @@ __global__ __launch_bounds__(1024) void hs_k_deliver(const DPath<R> *paths, int 
     for (int pi = 0; pi < np; ++pi)
         if (paths[pi].dmode == HS_DELIVER_SCATTER) hs_deliver_path<R, ACC>(paths[pi], T, w, &L, rank, nw, &ev);
     hs_flush_events(ev, ctl, rank);
+    /* the window's noise draws: independent of the state, so they can be made
+       here, by waves whose delivery chains are waiting on memory; the update of
+       the window reads them (xi[j*n + i] = draw of neuron i at step T+j) */
+    const long long nthr = (long long)gridDim.x * blockDim.x;
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int gi = 0; gi < ng; ++gi) {
+        const DGroup<R> &g = groups[gi];
+        if (!g.xi) continue;
+        const long long tot = (long long)g.n * w;
+        for (long long q = tid; q < tot; q += nthr) {
+            const int j = (int)(q / g.n);
+            const int i = (int)(q - (long long)j * g.n);
+            g.xi[q] = hs_noise<R, HS_MODEL_LIF_NOISE>(g.seed, g.stream, i, T + j);
+        }
+    }
 }
 
 template <typename R, bool ACC>
@@ -334,11 +355,14 @@ hipError_t hs_launch_update(const DGroup<R> *groups, int ng, int total_tiles, Hs
 }
 
 template <typename R>
-hipError_t hs_launch_deliver(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w, int fused, int block,
-                             int grid, hipStream_t s)
+hipError_t hs_launch_deliver(const DPath<R> *paths, int np, const DGroup<R> *groups, int ng, HsCtl *ctl, int koff,
+                             int w, int fused, int block, int grid, hipStream_t s)
 {
-    if (fused) hipLaunchKernelGGL((hs_k_deliver<R, true>), dim3(grid), dim3(block), 0, s, paths, np, ctl, koff, w);
-    else hipLaunchKernelGGL((hs_k_deliver<R, false>), dim3(grid), dim3(block), 0, s, paths, np, ctl, koff, w);
+    if (fused)
+        hipLaunchKernelGGL((hs_k_deliver<R, true>), dim3(grid), dim3(block), 0, s, paths, np, groups, ng, ctl, koff, w);
+    else
+        hipLaunchKernelGGL((hs_k_deliver<R, false>), dim3(grid), dim3(block), 0, s, paths, np, groups, ng, ctl, koff,
+                           w);
     return hipGetLastError();
 }
 
@@ -393,7 +417,8 @@ hipError_t hs_launch_tick(HsCtl *ctl, long long k, hipStream_t s)
 #define HS_INSTANTIATE(R)                                                                                         \
     template hipError_t hs_launch_update<R>(const DGroup<R> *, int, int, HsCtl *, int, int, int, int, int, int, int, \
                                             hipStream_t);                                                         \
-    template hipError_t hs_launch_deliver<R>(const DPath<R> *, int, HsCtl *, int, int, int, int, int, hipStream_t); \
+    template hipError_t hs_launch_deliver<R>(const DPath<R> *, int, const DGroup<R> *, int, HsCtl *, int, int, int, \
+                                             int, int, hipStream_t);                                              \
     template hipError_t hs_launch_gather<R>(const DPath<R> *, int, HsCtl *, int, int, int, int, int, hipStream_t);  \
     template hipError_t hs_launch_fused<R>(const DGroup<R> *, int, int, const DPath<R> *, int, HsCtl *, int, int,   \
                                            int, int, int, hipStream_t);                                           \

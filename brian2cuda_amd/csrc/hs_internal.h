@@ -74,6 +74,7 @@ struct DGroup {
     long long st_t0[HS_MAX_STATE_MONS], st_rows[HS_MAX_STATE_MONS];
     R *st_buf[HS_MAX_STATE_MONS];
     int tile_begin; /* first tile index of this group in the concatenated tile space */
+    R *xi;          /* windowed form: the window's noise [w][n] drawn by the delivery launch, or nullptr */
 };
 
 template <typename R>
@@ -133,9 +134,10 @@ hipError_t hs_launch_deliver_one_ctl(const DPath<R> &p, HsCtl *ctl, unsigned lon
 template <typename R>
 hipError_t hs_launch_update(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int koff, int w,
                             int wmax, int fused, int ms, int block, int grid, hipStream_t s);
+/* groups/ng: groups whose xi buffer the launch fills for steps t .. t+w-1 (may be 0) */
 template <typename R>
-hipError_t hs_launch_deliver(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w, int fused, int block,
-                             int grid, hipStream_t s);
+hipError_t hs_launch_deliver(const DPath<R> *paths, int np, const DGroup<R> *groups, int ng, HsCtl *ctl, int koff,
+                             int w, int fused, int block, int grid, hipStream_t s);
 template <typename R>
 hipError_t hs_launch_fused(const DGroup<R> *groups, int ng, int total_tiles, const DPath<R> *paths, int np,
                            HsCtl *ctl, int koff, int ms, int block, int n_upd, int n_del, hipStream_t s);

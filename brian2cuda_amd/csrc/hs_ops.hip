@@ -116,9 +116,9 @@ __global__ void hs_k_randn(unsigned long long seed, unsigned int stream, long lo
                            int uniform)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        double z = uniform ? hs_rand(seed, stream, (uint32_t)i, step) : hs_randn(seed, stream, (uint32_t)i, step);
-        if (prec == HS_F64) ((double *)out)[i] = z;
-        else ((float *)out)[i] = (float)z;
+        if (uniform) ((double *)out)[i] = hs_rand(seed, stream, (uint32_t)i, step);
+        else if (prec == HS_F64) ((double *)out)[i] = hs_randn(seed, stream, (uint32_t)i, step);
+        else ((float *)out)[i] = hs_randn_f32(seed, stream, (uint32_t)i, step);
     }
 }
 
@@ -193,7 +193,10 @@ __global__ void hs_k_math(int fn, int n, const double *x, double *y)
         else if (fn == 1) r = hs_log(a);
         else if (fn == 2) r = hs_sqrt(a);
         else if (fn == 3) r = hs_cos2pi(a);
-        else r = (double)hs_expf((float)a);
+        else if (fn == 4) r = (double)hs_expf((float)a);
+        else if (fn == 5) r = (double)hs_logf((float)a);
+        else if (fn == 6) r = (double)hs_sqrtf((float)a);
+        else r = (double)hs_cos2pif((float)a);
         y[i] = r;
     }
 }

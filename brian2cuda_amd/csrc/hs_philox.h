@@ -7,11 +7,13 @@
  * (brian2cuda/templates/rand.cu:46-561, brian2cuda/brianlib/curand_buffer.h:20-160)
  * and the per-thread curand states (rand.cu:25-42): a draw is a pure function of
  * (seed, stream, index, step), so there is no buffer to refill, no state to
- * initialise and no dependence on launch geometry or precision.
+ * initialise and no dependence on launch geometry (both precisions draw from the
+ * same Philox words).
  *
  * Counter layout of one draw:  ctr = (index, step_lo, step_hi, stream),
  *                              key = (seed_lo, seed_hi).
- * One Philox call gives 4 words -> two 53-bit uniforms -> one Box-Muller normal.
+ * One Philox call gives 4 words -> two 53-bit uniforms -> one Box-Muller normal
+ * (the FP32 form: two 24-bit uniforms and float arithmetic).
  */
 #ifndef HS_PHILOX_H
 #define HS_PHILOX_H
@@ -69,6 +71,22 @@ HS_HD double hs_randn(uint64_t seed, uint32_t stream, uint32_t index, int64_t st
     double u1 = hs_u53(w.v[0], w.v[1]) + HS_TWO_M53; /* (0, 1] */
     double u2 = hs_u53(w.v[2], w.v[3]);              /* [0, 1) */
     return hs_box_muller(u1, u2);
+}
+
+/* the FP32 form's standard normal: the same Philox words, 24-bit uniforms
+   u1 = (w0>>8 + 1) 2^-24 in (0, 1], u2 = (w2>>8) 2^-24 in [0, 1), and the
+   Box-Muller transform in float arithmetic */
+HS_HD float hs_randn_f32(uint64_t seed, uint32_t stream, uint32_t index, int64_t step)
+{
+    hs_u32x4 c;
+    c.v[0] = index;
+    c.v[1] = (uint32_t)((uint64_t)step);
+    c.v[2] = (uint32_t)((uint64_t)step >> 32);
+    c.v[3] = stream;
+    hs_u32x4 w = hs_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    float u1 = (float)((w.v[0] >> 8) + 1u) * 5.9604645e-08f; /* 2^-24 */
+    float u2 = (float)(w.v[2] >> 8) * 5.9604645e-08f;
+    return hs_box_mullerf(u1, u2);
 }
 
 /* uniform in [0,1) for (seed, stream, index, step) */

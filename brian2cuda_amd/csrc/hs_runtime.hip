@@ -271,7 +271,7 @@ extern "C" int hs_stateupdate(int model, int precision, int n, void *const *vars
 extern "C" int hs_math_eval(int fn, int n, const double *x, double *y)
 {
     HS_TRY(hs_need_init());
-    if (fn < 0 || fn > 4 || n < 0 || !x || !y) return hs_fail(HS_ERR_ARG, "hs_math_eval: bad arguments");
+    if (fn < 0 || fn > 7 || n < 0 || !x || !y) return hs_fail(HS_ERR_ARG, "hs_math_eval: bad arguments");
     if (n == 0) return HS_OK;
     HS_HIP(hs_launch_math(fn, n, x, y, g_rt.stream));
     HS_HIP(hipStreamSynchronize(g_rt.stream));
@@ -342,6 +342,7 @@ struct HGroup {
     int Q = 0;
     int smon = -1;
     std::vector<int> stmons;
+    void *xi = nullptr; /* windowed form: noise of the window [W][n], drawn by the delivery launch */
 };
 
 struct HPath {
@@ -808,6 +809,13 @@ static int hs_build(hs_net *net)
                 HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)gt.n * gt.nacc, &gt.acc[P.var], true));
         }
     }
+    /* windowed form with a scatter launch in every window: that launch draws the
+       window's noise (its waves wait on memory most of the time) and the update
+       reads it, instead of evaluating Philox + Box-Muller on its own chain */
+    if (net->mode == 3 && net->n_scatter > 0)
+        for (HGroup &g : net->g)
+            if (g.model == HS_MODEL_LIF_NOISE)
+                HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)g.n * W, &g.xi, false));
     HsCtl c;
     memset(&c, 0, sizeof(c));
     c.base = net->step;
@@ -864,6 +872,7 @@ static int hs_upload_desc_t(hs_net *net)
             d.st_buf[s] = (R *)M.buf;
         }
         d.tile_begin = tiles;
+        d.xi = (R *)g.xi;
         tiles += (g.n + block - 1) / block;
     }
     std::vector<DPath<R>> dp(net->p.size());
@@ -1024,8 +1033,8 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
     auto deliver = [&](int koff, int w) -> int {
         if (net->n_scatter)
             HS_TRY(hs_kt_launch(net, 1, [&]() -> int {
-                HS_HIP(hs_launch_deliver<R>(dp, np, net->ctl, koff, w, fused, 256, hs_deliver_grid(net),
-                                            g_rt.stream));
+                HS_HIP(hs_launch_deliver<R>(dp, np, dg, net->mode == 3 ? ng : 0, net->ctl, koff, w, fused, 256,
+                                            hs_deliver_grid(net), g_rt.stream));
                 return HS_OK;
             }));
         if (net->n_gather)

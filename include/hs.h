@@ -76,7 +76,7 @@ int hs_threshold_compact(int precision, const void *v, int n, double vth, int32_
 /*
  * n standard normals out[i] = N(seed, stream, i, step) (Philox4x32-10 +
  * Box-Muller, the deterministic math of the numerical contract); out is double
- * or float by precision (float values are the double draws rounded).
+ * or float by precision (float: the FP32 form's draw, 24-bit uniforms and float math).
  * Replaces the cuRAND buffer refill (rand.cu:395-463, curand_buffer.h:20-160).
  */
 int hs_randn(uint64_t seed, uint32_t stream, int64_t step, int n, int precision, void *out);
@@ -111,7 +111,8 @@ int hs_deliver(int precision, int n_src, int dmin, int D, int Q, const int32_t *
 
 /*
  * Device evaluation of the deterministic math (for bit-equality checks):
- * fn 0 exp, 1 log, 2 sqrt, 3 cos(2 pi x), 4 expf (x and y read/written as double).
+ * fn 0 exp, 1 log, 2 sqrt, 3 cos(2 pi x), 4 expf, 5 logf, 6 sqrtf, 7 cos(2 pi x)
+ * in float (x and y read/written as double).
  */
 int hs_math_eval(int fn, int n, const double *x, double *y);
 

@@ -38,6 +38,7 @@ def lib():
     sig = {
         "ref_philox4x32_10": (None, [U32P, U32P, U32P]),
         "ref_randn": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64]),
+        "ref_randn_f32": (C.c_float, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64]),
         "ref_rand": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64]),
         "ref_quantize_delay": (C.c_int, [C.c_double, C.c_double]),
         "ref_timestep": (C.c_longlong, [C.c_double, C.c_double]),
@@ -46,6 +47,9 @@ def lib():
         "ref_log": (C.c_double, [C.c_double]),
         "ref_sqrt": (C.c_double, [C.c_double]),
         "ref_cos2pi": (C.c_double, [C.c_double]),
+        "ref_logf": (C.c_float, [C.c_float]),
+        "ref_sqrtf": (C.c_float, [C.c_float]),
+        "ref_cos2pif": (C.c_float, [C.c_float]),
         "ref_model_constants": (C.c_int, [C.c_int, DP, C.c_double, DP]),
         "ref_stateupdate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(DP), I32P, DP, C.c_double,
                                       C.c_uint64, C.c_int64]),
@@ -89,7 +93,10 @@ def philox4x32_10(ctr, key):
     return tuple(o)
 
 
-def randn(seed, stream, index, step):
+def randn(seed, stream, index, step, precision=64):
+    """One draw; precision 32 gives the FP32 form's draw (24-bit uniforms, float math)."""
+    if precision == 32:
+        return lib().ref_randn_f32(seed, stream, index, step)
     return lib().ref_randn(seed, stream, index, step)
 
 

@@ -82,6 +82,18 @@ double ref_randn(uint64_t seed, uint32_t stream, uint32_t index, int64_t step)
     return hs_box_muller(u1, u2);
 }
 
+/* the FP32 form's draw: 24-bit uniforms from words 0 and 2, float Box-Muller */
+float ref_randn_f32(uint64_t seed, uint32_t stream, uint32_t index, int64_t step)
+{
+    uint32_t c[4] = {index, (uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32), stream};
+    uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t w[4];
+    ref_philox4x32_10(c, k, w);
+    float u1 = ldexpf((float)((w[0] >> 8) + 1u), -24);
+    float u2 = ldexpf((float)(w[2] >> 8), -24);
+    return hs_box_mullerf(u1, u2);
+}
+
 double ref_rand(uint64_t seed, uint32_t stream, uint32_t index, int64_t step)
 {
     uint32_t c[4] = {index, (uint32_t)(uint64_t)step, (uint32_t)((uint64_t)step >> 32), stream};
@@ -100,6 +112,9 @@ float ref_expf(float x) { return hs_expf(x); }
 double ref_log(double x) { return hs_log(x); }
 double ref_sqrt(double x) { return hs_sqrt(x); }
 double ref_cos2pi(double u) { return hs_cos2pi(u); }
+float ref_logf(float x) { return hs_logf(x); }
+float ref_sqrtf(float x) { return hs_sqrtf(x); }
+float ref_cos2pif(float u) { return hs_cos2pif(u); }
 
 int ref_model_constants(int model, const double *params, double dt, double *k)
 {

@@ -12,6 +12,7 @@ extern "C" {
 
 void ref_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 double ref_randn(uint64_t seed, uint32_t stream, uint32_t index, int64_t step);
+float ref_randn_f32(uint64_t seed, uint32_t stream, uint32_t index, int64_t step);
 double ref_rand(uint64_t seed, uint32_t stream, uint32_t index, int64_t step);
 int ref_quantize_delay(double delay, double dt);
 long long ref_timestep(double t, double dt);
@@ -20,6 +21,9 @@ float ref_expf(float x);
 double ref_log(double x);
 double ref_sqrt(double x);
 double ref_cos2pi(double u);
+float ref_logf(float x);
+float ref_sqrtf(float x);
+float ref_cos2pif(float u);
 int ref_model_constants(int model, const double *params, double dt, double *k);
 int ref_stateupdate(int model, int precision, int n, double **vars, const int32_t *last, const double *params,
                     double dt, uint64_t seed, int64_t step);

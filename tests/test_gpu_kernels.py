@@ -8,17 +8,25 @@ F64, F32 = 64, 32
 
 
 def test_detmath_bits_equal_host(hs, oracle_lib):
-    """exp/log/sqrt/cos2pi/expf evaluated on gfx950 give the host oracle's bits."""
+    """exp/log/sqrt/cos2pi and the float expf/logf/sqrtf/cos2pif on gfx950 give the host oracle's bits."""
     L = oracle_lib.lib()
     rng = np.random.default_rng(7)
     cases = {
         0: rng.uniform(-700, 700, 20000),
         1: np.concatenate([rng.random(10000) + 1e-300, 10.0 ** rng.uniform(-300, 300, 10000)]),
-        2: 10.0 ** rng.uniform(-300, 300, 20000),
+        2: np.concatenate([10.0 ** rng.uniform(-300, 300, 20000), rng.uniform(0, 80, 20000),
+                           np.nextafter(np.arange(1.0, 2000.0) ** 2, 0), np.nextafter(np.arange(1.0, 2000.0) ** 2, 1e9),
+                           rng.uniform(0, 1, 2000) * 2.0 ** -1060]),
         3: rng.random(20000),
         4: rng.uniform(-80, 80, 20000).astype(np.float32).astype(np.float64),
+        5: np.concatenate([(np.arange(1, 20001) * 2.0 ** -24), rng.random(20000) * 1e30 + 1e-30]).astype(
+            np.float32).astype(np.float64),
+        6: np.concatenate([rng.uniform(0, 80, 20000), np.arange(1.0, 4000.0) ** 2]).astype(np.float32).astype(
+            np.float64),
+        7: (rng.integers(0, 2**24, 20000) * 2.0 ** -24),
     }
-    host = {0: L.ref_exp, 1: L.ref_log, 2: L.ref_sqrt, 3: L.ref_cos2pi, 4: lambda x: float(L.ref_expf(x))}
+    host = {0: L.ref_exp, 1: L.ref_log, 2: L.ref_sqrt, 3: L.ref_cos2pi, 4: lambda x: float(L.ref_expf(x)),
+            5: lambda x: float(L.ref_logf(x)), 6: lambda x: float(L.ref_sqrtf(x)), 7: lambda x: float(L.ref_cos2pif(x))}
     for fn, x in cases.items():
         y = hs.math_eval(fn, x)
         ref = np.array([host[fn](float(a)) for a in x])
@@ -30,11 +38,11 @@ def test_philox_randn_matches_oracle(hs, oracle_lib, prec):
     for seed, stream, step in [(1234, 0, 0), (1234, 0, 7), (2**40 + 5, 3, 123456789)]:
         n = 5000
         z = hs.randn(seed, stream, step, n, prec)
-        ref = np.array([oracle_lib.randn(seed, stream, i, step) for i in range(n)])
+        ref = np.array([oracle_lib.randn(seed, stream, i, step, precision=prec) for i in range(n)])
         if prec == F64:
             assert np.array_equal(z, ref)
         else:
-            assert np.array_equal(z, ref.astype(np.float32))
+            assert np.array_equal(z, ref.astype(np.float32))  # float draws, exactly representable
 
 
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 1000, 100_003])
