@@ -77,9 +77,7 @@ __global__ __launch_bounds__(64) void hs_k_hines(HinesDesc<R> d, int nsteps)
     double *Vs = A + CM;        /* right-hand side, then v_star */
     double *Up = Vs + CM;
     double *Um = Up + CM;
-    double *Ab0 = Um + CM;
-    double *Ab2 = Ab0 + CM;
-    double *Pd = Ab2 + CM;      /* SM + 1 */
+    double *Pd = Um + CM;       /* SM + 1 */
     double *Pp = Pd + SM + 1;   /* SM */
     double *Pc = Pp + SM;       /* (SM + 1) * RL */
     double *Bn = Pc + (SM + 1) * HS_HINES_RL;
@@ -95,10 +93,11 @@ __global__ __launch_bounds__(64) void hs_k_hines(HinesDesc<R> d, int nsteps)
     int *ord = chl + (SM + 1) * HS_HINES_RL; /* SM + 1 */
     int *lvs = ord + SM + 1;    /* SM + 2 */
 
-    for (int j = lane; j < nc; j += HS_WAVE) {
-        Ab0[j] = d.ab0[c0 + j];
-        Ab2[j] = d.ab2[c0 + j];
-    }
+    /* the off-diagonal bands are constant: read from global memory (L1/L2
+       resident after the first step) to keep the workgroup's LDS small enough
+       for every tree of a large batch to be resident at once */
+    const double *__restrict__ gab0 = d.ab0 + c0;
+    const double *__restrict__ gab2 = d.ab2 + c0;
     for (int s = lane; s < ns; s += HS_WAVE) {
         fst[s] = d.first[s0 + s];
         lst[s] = d.end[s0 + s];
@@ -161,12 +160,23 @@ __global__ __launch_bounds__(64) void hs_k_hines(HinesDesc<R> d, int nsteps)
         for (int s = lane; s < ns; s += HS_WAVE) {
             const int js = fst[s], je = lst[s], last = je - 1;
             double cp = 0.0, vp = 0.0, upp = 0.0, ump = 0.0;
+            /* the inputs of compartment j+1 are loaded while j is computed (they
+               are not written before their own iteration) */
+            double b_n = A[js], vs_n = Vs[js];
+            double c_n = (js < nc - 1) ? gab0[js + 1] : 0.0;
+            double a_n = js > 0 ? gab2[js - 1] : 0.0;
             for (int j = js; j < je; ++j) {
-                const double b = A[j];
-                double vs = Vs[j];
+                const double b = b_n, ai = a_n;
+                double vs = vs_n;
+                double cj = c_n;
+                if (j + 1 < je) {
+                    b_n = A[j + 1];
+                    vs_n = Vs[j + 1];
+                    c_n = (j + 1 < nc - 1) ? gab0[j + 2] : 0.0;
+                    a_n = gab2[j];
+                }
                 double up = (j == last) ? bps[s] : 0.0;
                 double um = (j == js) ? bms[s] : 0.0;
-                double cj = (j < nc - 1) ? Ab0[j + 1] : 0.0;
                 if (j > 0) {
                     if (j == js) {
                         /* section start: the subdiagonal is exactly 0 (its axial
@@ -177,7 +187,6 @@ __global__ __launch_bounds__(64) void hs_k_hines(HinesDesc<R> d, int nsteps)
                         up = up * m;
                         um = um * m;
                     } else {
-                        const double ai = Ab2[j - 1];
                         const double m = 1.0 / (b - ai * cp);
                         cj = cj * m;
                         vs = (vs - ai * vp) * m;
@@ -199,14 +208,23 @@ __global__ __launch_bounds__(64) void hs_k_hines(HinesDesc<R> d, int nsteps)
                 upp = up;
                 ump = um;
             }
-            for (int j = je - 2; j >= js; --j) {
-                const double c = A[j];
-                vp = Vs[j] - c * vp;
-                upp = Up[j] - c * upp;
-                ump = Um[j] - c * ump;
-                Vs[j] = vp;
-                Up[j] = upp;
-                Um[j] = ump;
+            if (je - 2 >= js) {
+                double c_b = A[je - 2], v_b = Vs[je - 2], u_b = Up[je - 2], m_b = Um[je - 2];
+                for (int j = je - 2; j >= js; --j) {
+                    const double c = c_b, v0 = v_b, u0 = u_b, m0 = m_b;
+                    if (j - 1 >= js) {
+                        c_b = A[j - 1];
+                        v_b = Vs[j - 1];
+                        u_b = Up[j - 1];
+                        m_b = Um[j - 1];
+                    }
+                    vp = v0 - c * vp;
+                    upp = u0 - c * upp;
+                    ump = m0 - c * ump;
+                    Vs[j] = vp;
+                    Up[j] = upp;
+                    Um[j] = ump;
+                }
             }
         }
         __syncthreads();
@@ -242,12 +260,26 @@ __global__ __launch_bounds__(64) void hs_k_hines(HinesDesc<R> d, int nsteps)
         for (int L = nlev - 1; L >= 0; --L) {
             for (int q = lvs[L] + lane; q < lvs[L + 1]; q += HS_WAVE) {
                 const int i = ord[q];
+                const int nci = nch[i];
                 double pd = Pd[i], b = Bn[i];
-                for (int c = 0; c < nch[i]; ++c) {
-                    const int j = chl[i * HS_HINES_RL + c];
-                    const double subfac = Pc[i * HS_HINES_RL + c] / Pd[j];
-                    pd = pd - subfac * Pp[j - 1];
-                    b = b - subfac * Bn[j];
+                /* every child's operands are loaded (and its factor divided out)
+                   before the row update, which then runs in child order */
+                double sf[HS_HINES_RL], pp[HS_HINES_RL], bj[HS_HINES_RL];
+#pragma unroll
+                for (int c = 0; c < HS_HINES_RL; ++c) {
+                    if (c < nci) {
+                        const int j = chl[i * HS_HINES_RL + c];
+                        sf[c] = Pc[i * HS_HINES_RL + c] / Pd[j];
+                        pp[c] = Pp[j - 1];
+                        bj[c] = Bn[j];
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < HS_HINES_RL; ++c) {
+                    if (c < nci) {
+                        pd = pd - sf[c] * pp[c];
+                        b = b - sf[c] * bj[c];
+                    }
                 }
                 Pd[i] = pd;
                 Bn[i] = b;
@@ -619,7 +651,7 @@ extern "C" int hs_hines_run(hs_hines *H, int64_t nsteps)
 {
     if (!H || nsteps < 0) return hs_rt_fail(HS_ERR_ARG, "bad arguments");
     const int CM = H->cmax, SM = H->smax;
-    const size_t lds = sizeof(double) * (6 * (size_t)CM + (SM + 1) + SM + (SM + 1) * HS_HINES_RL + (SM + 1) + 4 * SM) +
+    const size_t lds = sizeof(double) * (4 * (size_t)CM + (SM + 1) + SM + (SM + 1) * HS_HINES_RL + (SM + 1) + 4 * SM) +
                        sizeof(int) * (3 * (size_t)SM + (SM + 1) + (SM + 1) * HS_HINES_RL + (SM + 1) + (SM + 2));
     if (lds > 160 * 1024) return hs_rt_fail(HS_ERR_UNSUPPORTED, "tree too large for LDS (%zu bytes)", lds);
     hipStream_t st = hs_rt_stream();
