@@ -7,8 +7,10 @@ delays U[0.5, 5] ms quantised to dt=0.1 ms (5..50 steps), sigma=1 mV, mu=25 mV,
 Philox noise; 1 s of simulated time (10,000 steps) after a warm-up.
 
   fp32 batched : FP32 state, weights and arithmetic; effects accumulated per
-                 step, W = min(8, min delay) steps per update launch and per
-                 delivery launch, K steps per captured HIP graph
+                 step; one launch per window of W = min(8, min delay) steps
+                 (update + push of the delays < 2W | pull of the longer delays
+                 for the next window | the next window's noise draws), K steps
+                 per captured HIP graph
   fp64 plain   : FP64 state, weights and arithmetic; update kernel + delivery
                  kernel per step, effects applied in place (Brian2's order)
 
@@ -25,12 +27,19 @@ of a launch, with n_spk and E the measured means of the timed run:
             (FP32) 4+4+1, w_s = V + not_ref = 4+1 (halved FP fields of 17/9)
   delivery: n_spk*(8 + 12*b) + E*(4 + w_b)   b = delay slots per spike (46),
             w_b = 4 (FP32 per-synapse weight)
+  window  : update + delivery (the pipelined form does both in one launch)
 The average launch duration comes from HIP events around every launch on the
 runtime's stream (hs_net_set_option(net, 7, 1): no graphs in that run).
 
 Drift: both forms are re-run with spike monitors over `--drift-steps` steps;
 spikes of each neuron are matched greedily in time order (pairs within 5 ms) and
-the RMS of the matched time differences is reported with the unmatched fraction.
+the RMS of the matched time differences is reported with the unmatched fraction,
+with each form's population rate and mean ISI CV, and the same matching for the
+FP64 fused form against FP64 plain (no change of arithmetic: the divergence the
+network's own sensitivity produces from a one-step reordering of additions).
+
+Multicompartment: `--hines-trees` random branched HH trees (100-400
+compartments) advanced `--hines-steps` steps, FP64 and FP32 channel arithmetic.
 
 CPU baseline: the oracle (serial C, cpp_standalone semantics, FP64) on the same
 network, timed over a bounded window of steps on one host core.
@@ -145,12 +154,32 @@ def spike_trains(spec, prec, engine, steps):
     return i, t
 
 
+def train_stats(i, t, n, steps, dt):
+    """Population rate (Hz) and mean coefficient of variation of the inter-spike intervals."""
+    o = np.lexsort((t, i))
+    i, t = i[o], t[o]
+    isi = np.diff(t).astype(np.float64)
+    cvs = []
+    b = np.searchsorted(i, np.arange(n + 1))
+    for k in np.nonzero(np.diff(b) >= 3)[0]:
+        x = isi[b[k]:b[k + 1] - 1]
+        cvs.append(x.std() / x.mean())
+    return {"rate_hz": len(t) / (n * steps * dt), "cv_isi": float(np.mean(cvs)) if cvs else None}
+
+
 def drift(spec, steps, tol_steps=50):
-    """Greedy time-ordered matching of FP32-batched to FP64-plain spikes per neuron."""
+    """Greedy time-ordered matching of FP32-batched to FP64-plain spikes per neuron,
+    and the same for FP64 fused vs FP64 plain (same precision, effects applied one
+    step later): the second isolates the divergence that the network's own
+    sensitivity produces from any change of arithmetic."""
     i64, t64 = spike_trains(spec, 64, "plain", steps)
     i32, t32 = spike_trains(spec, 32, "fused", steps)
     res = match_spikes(i64, t64, i32, t32, spec.n_neurons, tol_steps)
     res["steps"] = steps
+    res["fp64_plain"] = train_stats(i64, t64, spec.n_neurons, steps, spec.dt)
+    res["fp32_batched"] = train_stats(i32, t32, spec.n_neurons, steps, spec.dt)
+    f64, g64 = spike_trains(spec, 64, "fused", steps)
+    res["fp64_fused_vs_fp64_plain"] = match_spikes(i64, t64, f64, g64, spec.n_neurons, tol_steps)
     return res
 
 
@@ -187,6 +216,34 @@ def match_spikes(i64, t64, i32, t32, n, tol_steps):
             "unmatched_frac": unmatched / total if total else 0.0, "match_window_steps": tol_steps}
 
 
+HINES_BYTES_PER_COMP_STEP = 68  # a, d, v, rhs r (32) + v w (8) + parent index (4) + m, h, n r/w (24)
+
+
+def hines_bench(n_trees, steps, seed):
+    """Multicompartment batch (random branched HH trees, 100-400 compartments,
+    dt = 0.025 ms): device time per step for FP64 and FP32 channel arithmetic."""
+    from brian2cuda_amd import morphology as mo
+    tb = mo.random_trees(n_trees=n_trees, seed=seed)
+    out = {"trees": n_trees, "compartments": tb.n_comp, "sections": tb.n_sections, "steps": steps}
+    for prec in (64, 32):
+        h = mo.build_hs(tb, prec)
+        h.run(min(200, steps))
+        h.run(steps)
+        s = h.stats()
+        step_s = s["last_run_gpu_ms"] * 1e-3 / steps
+        out[f"fp{prec}"] = {"us_per_step": round(step_s * 1e6, 3),
+                            "compartment_steps_per_s": tb.n_comp / step_s,
+                            "sim_s_per_wall_s": steps * tb.dt / s["last_run_wall_s"],
+                            "soma_spikes": int(h.spike_counts().sum())}
+        h.close()
+    step_s = out["fp64"]["us_per_step"] * 1e-6
+    ach = HINES_BYTES_PER_COMP_STEP * tb.n_comp / step_s / 1e9
+    out["roofline_fp64"] = {"bound": "hbm", "achieved": round(ach, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(ach / HBM_PEAK_GBS, 6), "kernel": "hs_k_hines",
+                            "bytes_per_compartment_step": HINES_BYTES_PER_COMP_STEP}
+    return out
+
+
 def cpu_baseline(spec, skip_steps, steps):
     """Oracle (cpp_standalone semantics, FP64, 1 core) on a window of steps."""
     import oracle
@@ -218,6 +275,8 @@ def main():
     ap.add_argument("--cpu-skip", type=int, default=300, help="oracle untimed steps before its window")
     ap.add_argument("--kt-steps", type=int, default=2000, help="steps of the kernel-timed run (0: skip)")
     ap.add_argument("--drift-steps", type=int, default=10000, help="steps of the drift comparison (0: skip)")
+    ap.add_argument("--hines-trees", type=int, default=2048, help="trees of the multicompartment batch")
+    ap.add_argument("--hines-steps", type=int, default=2000, help="steps of the multicompartment run (0: skip)")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -255,6 +314,7 @@ def main():
             "hs_k_deliver": n_spk * (8 + 12 * D) + E * (4 + w_b),
             "hs_k_gather": n_spk * (8 + 12 * D) + E * (4 + w_b),
             "hs_k_fused": N * (r_s + w_s) + n_spk * (12 + 12 * D) + E * (4 + w_b),
+            "hs_k_window": N * (r_s + w_s) + n_spk * (12 + 12 * D) + E * (4 + w_b),
         }
         name = max(kt, key=lambda k: kt[k]["total_ms"])
         steps_per_launch = args.kt_steps / kt[name]["launches"]
@@ -270,6 +330,11 @@ def main():
     if args.drift_steps > 0 and not args.no_fp64 and args.workload == "brunel":
         dr = drift(spec, args.drift_steps)
         log("drift", dr)
+
+    hines = None
+    if args.hines_steps > 0:
+        hines = hines_bench(args.hines_trees, args.hines_steps, args.seed)
+        log("hines", hines)
 
     cpu = None
     if args.cpu_steps > 0:
@@ -304,6 +369,7 @@ def main():
                                                if fp64 and fp64["events_per_s"] > 0 else None),
         "drift_fp32_batched_vs_fp64_plain": dr,
         "roofline": roofline,
+        "multicompartment": hines,
         "cpu_baseline": cpu,
         "device": info,
     }

@@ -168,10 +168,12 @@ struct HsDeliverLds {
 template <typename R, bool ACC>
 __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, int w, HsDeliverLds *L,
                                                 long long wave_rank,
-                                long long n_waves, unsigned long long *ev_lane)
+                                long long n_waves, unsigned long long *ev_lane, int di0 = 0, int dn = -1)
 {
+    /* delay slots di0 .. di0+dn-1 of the pathway (all by default) */
     const int D = p.D;
-    const int npairs = w * D;
+    if (dn < 0) dn = D;
+    const int npairs = w * dn;
     const int tq = (int)hs_mod(T, p.Q);                 /* ring slot of step T */
     const int ta = ACC ? (int)hs_mod(T, p.nacc) : 0;    /* accumulator slot of step T */
     {
@@ -183,8 +185,8 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
             const int q = c + threadIdx.x;
             int cnt = 0;
             if (q < npairs) {
-                const int j = q / D, di = q - j * D;
-                cnt = p.hist_cnt[hs_wrap(tq + j - (p.dmin + di), p.Q)];
+                const int j = q / dn, di = di0 + (q - j * dn);
+                cnt = p.hist_cnt[hs_wrap(tq + j - (p.dmin + di), p.Q) * p.cstride];
             }
             int tot;
             const int ex = hs_wave_excl_scan(cnt, &tot);
@@ -219,8 +221,8 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
                     else hi = mid;
                 }
                 const int kk = (int)(wi - L->pref[lo]);
-                jw = lo / D;
-                const int di = lo - jw * D;
+                jw = lo / dn;
+                const int di = di0 + (lo - jw * dn);
                 const long long slot = hs_wrap(tq + jw - (p.dmin + di), p.Q);
                 pre = p.hist[slot * p.n_src + kk];
                 const long long b = (long long)pre * D + di;
@@ -316,7 +318,7 @@ __device__ __forceinline__ void hs_gather_tile(const DPath<R> &p, int tile, long
         int carry = 0;
         for (int c = 0; c < nU; c += HS_WAVE) {
             const int q = c + hs_lane();
-            const int cnt = q < nU ? p.hist_cnt[hs_wrap(tq - dmax + q, p.Q)] : 0;
+            const int cnt = q < nU ? p.hist_cnt[hs_wrap(tq - dmax + q, p.Q) * p.cstride] : 0;
             int tot;
             const int ex = hs_wave_excl_scan(cnt, &tot);
             if (q < nU) L->pref[q] = carry + ex;

@@ -35,6 +35,102 @@ struct HsCompactLdsW {
     unsigned long long mbase[HS_MAX_WINDOW];
 };
 
+/* pipelined windows: the pathways a group's spikes push into */
+template <typename R>
+struct HsPush {
+    const DPath<R> *paths;
+    int np;
+    unsigned long long *emit; /* events counter (this block's spread slot) */
+};
+
+/*
+ * The near synapses (delay slots 0 .. nnear-1, delays < 2W) of the spikes this
+ * wave emitted in steps T .. T+w-1, pushed into the accumulator slots of their
+ * arrival steps -- all >= T + dmin >= T + W, none of which this launch reads.
+ * Every spike's whole out-degree is counted as emitted events (a run's delivered
+ * count is emitted minus what is still in flight at its end).
+ */
+template <typename R>
+__device__ __forceinline__ void hs_push_near(const HsPush<R> &ps, int gid, int i, unsigned int bits, long long T,
+                                             int w)
+{
+    (void)w;
+    const int cnt = __popc(bits); /* this lane's spikes in the window (same neuron, same synapses) */
+    if (!__ballot(cnt != 0)) return;
+    constexpr int UNR = 4; /* passes whose loads are issued together */
+    unsigned long long emitted = 0;
+    for (int pi = 0; pi < ps.np; ++pi) {
+        const DPath<R> &p = ps.paths[pi];
+        if (p.src != gid) continue;
+        /* the spiking lane loads its bundle starts 0 .. nnear (nnear <= HS_MAX_WINDOW)
+           and its end at once; item lanes read them by cross-lane shuffles */
+        int o[HS_MAX_WINDOW + 1];
+        int s_end = 0;
+        if (cnt) {
+            const long long base = (long long)i * p.D;
+#pragma unroll
+            for (int q = 0; q <= HS_MAX_WINDOW; ++q) o[q] = q <= p.nnear ? p.bstart[base + q] : 0;
+            s_end = p.bstart[base + p.D];
+        } else {
+#pragma unroll
+            for (int q = 0; q <= HS_MAX_WINDOW; ++q) o[q] = 0;
+        }
+        const int s0 = o[0];
+        int len = 0;
+#pragma unroll
+        for (int q = 1; q <= HS_MAX_WINDOW; ++q)
+            if (q == p.nnear) len = o[q] - s0;
+        if (cnt) emitted += (unsigned long long)cnt * (unsigned long long)(s_end - s0);
+        /* the wave's (spike, near synapse) items, flattened, 64 per pass */
+        int tot;
+        const int off = hs_wave_excl_scan(cnt * len, &tot);
+        const int ta = (int)hs_mod(T, p.nacc);
+        for (int c0 = 0; c0 < tot; c0 += UNR * HS_WAVE) {
+            int tg[UNR], slot[UNR];
+            R wv[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int idx = c0 + u * HS_WAVE + hs_lane();
+                int lo = 0; /* owner: the last lane whose items start at or before idx */
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) {
+                    const int cand = lo + step;
+                    if (__shfl(off, cand, HS_WAVE) <= idx) lo = cand;
+                }
+                const int ob = __shfl(off, lo, HS_WAVE);
+                const int lb = __shfl(len, lo, HS_WAVE);
+                const unsigned int bb = (unsigned int)__shfl((int)bits, lo, HS_WAVE);
+                const int pb = __shfl(i, lo, HS_WAVE);
+                int ob_q[HS_MAX_WINDOW + 1];
+#pragma unroll
+                for (int q = 0; q <= HS_MAX_WINDOW; ++q) ob_q[q] = __shfl(o[q], lo, HS_WAVE);
+                tg[u] = -1;
+                if (idx < tot) {
+                    const int k = idx - ob;
+                    const int r = k / lb; /* the owner's r-th spike of the window, synapse k - r*lb */
+                    const int s = ob_q[0] + (k - r * lb);
+                    unsigned int bm = bb;
+                    for (int q = 0; q < r; ++q) bm &= bm - 1;
+                    const int j = __ffs(bm) - 1;
+                    int di = 0;
+#pragma unroll
+                    for (int q = 1; q < HS_MAX_WINDOW; ++q) di += (q < p.nnear && ob_q[q] <= s) ? 1 : 0;
+                    if (p.wmode == HS_W_SCALAR) wv[u] = p.wscalar;
+                    else if (p.wmode == HS_W_SYNAPSE) wv[u] = p.w[s];
+                    else wv[u] = p.w[pb];
+                    slot[u] = hs_wrap(ta + j + p.dmin + di, p.nacc);
+                    tg[u] = p.post[s];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u)
+                if (tg[u] >= 0) unsafeAtomicAdd(&p.acc[(long long)slot[u] * p.n_tgt + tg[u]], wv[u]);
+        }
+    }
+    emitted = hs_wave_sum(emitted);
+    if (hs_lane() == 0 && emitted) atomicAdd(ps.emit, emitted);
+}
+
 /* one step of one neuron inside the update window; returns the spike */
 template <typename R, int MODEL, bool FUSED>
 __device__ __forceinline__ bool hs_step_one(const DGroup<R> &g, R *s, const R *k, int32_t &last, long long t, int i,
@@ -60,10 +156,10 @@ __device__ __forceinline__ bool hs_step_one(const DGroup<R> &g, R *s, const R *k
     return spk;
 }
 
-template <typename R, int MODEL, bool FUSED>
-__device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int tile, long long T, int w, int wmax,
-                                               long long no_reset_before,
-                               HsCompactLdsW *L, unsigned long long *spk_total)
+template <typename R, int MODEL, bool FUSED, bool PUSH>
+__device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int tile, long long T, int w, int wmax,
+                                               long long no_reset_before, HsCompactLdsW *L,
+                                               unsigned long long *spk_total, const HsPush<R> &push)
 {
     constexpr int NV = HsNV<MODEL>::value;
     const int i = tile * blockDim.x + threadIdx.x;
@@ -71,7 +167,7 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int tile, lon
     const int tq = (int)hs_mod(T, g.Q);  /* ring slot of step T */
     /* ring slots of the steps after this launch start from an empty list */
     if (tile == 0 && threadIdx.x == 0)
-        for (int m = 0; m < wmax; ++m) g.hist_cnt[hs_wrap(tq + w + m, g.Q)] = 0;
+        for (int m = 0; m < wmax; ++m) g.hist_cnt[hs_wrap(tq + w + m, g.Q) * HS_CNT_STRIDE] = 0;
     R s[NV];
     int32_t last = HS_NEVER_SPIKED;
 #pragma unroll
@@ -89,8 +185,10 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int tile, lon
         R xi[NOISY ? HS_MAX_WINDOW : 1];
         if (NOISY) {
             if (g.xi) {
+                const int x0 = (int)hs_mod(T, g.xi_slots);
 #pragma unroll
-                for (int j = 0; j < HS_MAX_WINDOW; ++j) xi[j] = j < w ? g.xi[(long long)j * g.n + i] : (R)0;
+                for (int j = 0; j < HS_MAX_WINDOW; ++j)
+                    xi[j] = j < w ? g.xi[(long long)hs_wrap(x0 + j, g.xi_slots) * g.n + i] : (R)0;
             } else {
 #pragma unroll
                 for (int j = 0; j < HS_MAX_WINDOW; ++j)
@@ -159,17 +257,20 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int tile, lon
         if (hs_lane() == 0) L->wave_cnt[j][wid] = __popcll(m);
     }
     __syncthreads();
+    int tot = 0;
     if ((int)threadIdx.x < w) {
         const int j = threadIdx.x;
-        int tot = 0;
         for (int v = 0; v < nw; ++v) {
             const int c = L->wave_cnt[j][v];
             L->wave_cnt[j][v] = tot;
             tot += c;
         }
-        L->base[j] = tot ? atomicAdd(g.hist_cnt + hs_wrap(tq + j, g.Q), tot) : 0;
-        if (tot && spk_total) atomicAdd(spk_total, (unsigned long long)tot);
+        L->base[j] = tot ? atomicAdd(g.hist_cnt + hs_wrap(tq + j, g.Q) * HS_CNT_STRIDE, tot) : 0;
         L->mbase[j] = (tot && g.smon_n) ? atomicAdd(g.smon_n, (unsigned long long)tot) : 0ull;
+    }
+    if (threadIdx.x < HS_WAVE) { /* one count per block and window */
+        const unsigned long long s = hs_wave_sum((unsigned long long)tot);
+        if (threadIdx.x == 0 && s && spk_total) atomicAdd(spk_total, s);
     }
     __syncthreads();
     for (int j = 0; j < w; ++j) {
@@ -187,16 +288,17 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int tile, lon
             }
         }
     }
+    if (PUSH) hs_push_near<R>(push, gid, i, bits, T, w);
     __syncthreads();
 }
 
 /* MS: the model of every group (kernel specialised for it, so the register
    budget is that model's), or 0 for networks mixing models */
-template <typename R, bool FUSED, int MS>
+template <typename R, bool FUSED, int MS, bool PUSH = false>
 __device__ __forceinline__ void hs_update_phase(const DGroup<R> *groups, int ng, int total_tiles, long long T, int w,
                                                 int wmax,
                                 long long no_reset_before, HsCompactLdsW *L, unsigned long long *spk_total,
-                                int block_rank, int n_blocks)
+                                int block_rank, int n_blocks, const HsPush<R> &push = HsPush<R>{nullptr, 0, nullptr})
 {
     for (int tile = block_rank; tile < total_tiles; tile += n_blocks) {
         int gi = 0;
@@ -204,55 +306,162 @@ __device__ __forceinline__ void hs_update_phase(const DGroup<R> *groups, int ng,
         const DGroup<R> &g = groups[gi];
         const int lt = tile - g.tile_begin;
         if (MS != 0) {
-            hs_update_tile<R, (MS != 0 ? MS : 1), FUSED>(g, lt, T, w, wmax, no_reset_before, L, spk_total);
+            hs_update_tile<R, (MS != 0 ? MS : 1), FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
+                                                               push);
         } else if (g.model == HS_MODEL_LIF_CUBA) {
-            hs_update_tile<R, HS_MODEL_LIF_CUBA, FUSED>(g, lt, T, w, wmax, no_reset_before, L, spk_total);
+            hs_update_tile<R, HS_MODEL_LIF_CUBA, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
+                                                              push);
         } else if (g.model == HS_MODEL_LIF_NOISE) {
-            hs_update_tile<R, HS_MODEL_LIF_NOISE, FUSED>(g, lt, T, w, wmax, no_reset_before, L, spk_total);
+            hs_update_tile<R, HS_MODEL_LIF_NOISE, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
+                                                               push);
         } else {
-            hs_update_tile<R, HS_MODEL_HH_COBA, FUSED>(g, lt, T, w, wmax, no_reset_before, L, spk_total);
+            hs_update_tile<R, HS_MODEL_HH_COBA, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
+                                                             push);
         }
     }
 }
 
+/* noise draws of steps T0 .. T0+nsteps-1 of every group with a noise ring, by
+   blocks brank of nblk: xi[(t mod xi_slots)*n + i] = draw of neuron i at step t */
+template <typename R>
+__device__ __forceinline__ void hs_noise_ahead(const DGroup<R> *groups, int ng, long long T0, int nsteps, int brank,
+                                               int nblk)
+{
+    const long long nthr = (long long)nblk * blockDim.x;
+    const long long tid = (long long)brank * blockDim.x + threadIdx.x;
+    for (int gi = 0; gi < ng; ++gi) {
+        const DGroup<R> &g = groups[gi];
+        if (!g.xi) continue;
+        const int s0 = (int)hs_mod(T0, g.xi_slots);
+        const long long tot = (long long)g.n * nsteps;
+        for (long long q = tid; q < tot; q += nthr) {
+            const int j = (int)(q / g.n);
+            const int i = (int)(q - (long long)j * g.n);
+            g.xi[(long long)hs_wrap(s0 + j, g.xi_slots) * g.n + i] =
+                hs_noise<R, HS_MODEL_LIF_NOISE>(g.seed, g.stream, i, T0 + j);
+        }
+    }
+}
+
+/* the draws of the first window of a chunk (the later windows' draws are made
+   by the update launch of the window before) */
+template <typename R>
+__global__ __launch_bounds__(256) void hs_k_noise(const DGroup<R> *groups, int ng, HsCtl *ctl, int koff, int nsteps)
+{
+    hs_noise_ahead<R>(groups, ng, ctl->base + koff, nsteps, blockIdx.x, gridDim.x);
+}
+
+/* blocks [0, n_upd): the update of steps T .. T+w-1; blocks [n_upd, grid): the
+   noise draws of the next window, steps T+w .. T+w+wmax-1 (no dependence on the
+   state, so they run beside the update on otherwise idle waves) */
 template <typename R, bool FUSED, int MS>
 __global__ __launch_bounds__(256) void hs_k_update(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl,
-                                                    int koff, int w, int wmax)
+                                                    int koff, int w, int wmax, int n_upd)
 {
     __shared__ HsCompactLdsW L;
     const long long T = ctl->base + koff;
-    hs_update_phase<R, FUSED, MS>(groups, ng, total_tiles, T, w, wmax, ctl->no_reset_before, &L, &ctl->spikes,
-                                  blockIdx.x, gridDim.x);
+    if ((int)blockIdx.x < n_upd)
+        hs_update_phase<R, FUSED, MS>(groups, ng, total_tiles, T, w, wmax, ctl->no_reset_before, &L,
+                                      ctl->spk_slot + (blockIdx.x % HS_EV_SLOTS) * HS_EV_STRIDE, blockIdx.x, n_upd);
+    else
+        hs_noise_ahead<R>(groups, ng, T + w, wmax, blockIdx.x - n_upd, gridDim.x - n_upd);
 }
 
+/*
+ * Pipelined windows, one launch per window of steps T .. T+w-1 (W = the engine's
+ * window <= every delay):
+ *   blocks [0, n_upd)           update of the window, then the push of the near
+ *                               synapses (delays < 2W) of its spikes
+ *   blocks [n_upd, +n_far)      pull delivery of the far synapses (delays >= 2W)
+ *                               arriving in T+W .. T+W+w-1, whose spikes are all
+ *                               from steps <= T-1: finished by earlier launches
+ *   the remaining blocks        the next window's noise draws
+ * The update reads accumulator slots T-1 .. T+w-2; every write of this launch
+ * goes to slots >= T+W (nacc = 3W+1 keeps them apart).  After the launch every
+ * effect arriving before T+w+W has been applied or accumulated.
+ */
+template <typename R, int MS>
+__global__ __launch_bounds__(256) void hs_k_window(const DGroup<R> *groups, int ng, int total_tiles,
+                                                    const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w, int W,
+                                                    int n_upd, int n_far)
+{
+    __shared__ union {
+        HsCompactLdsW c;
+        HsDeliverLds d;
+    } L;
+    const long long T = ctl->base + koff;
+    const int b = blockIdx.x;
+    if (b < n_upd) {
+        const HsPush<R> push{paths, np, ctl->ev_slot + (b % HS_EV_SLOTS) * HS_EV_STRIDE};
+        hs_update_phase<R, true, MS, true>(groups, ng, total_tiles, T, w, W, ctl->no_reset_before, &L.c,
+                                           ctl->spk_slot + (b % HS_EV_SLOTS) * HS_EV_STRIDE, b, n_upd, push);
+    } else if (b < n_upd + n_far) {
+        const int nwb = blockDim.x / HS_WAVE;
+        const long long rank = (long long)(b - n_upd) * nwb + threadIdx.x / HS_WAVE;
+        const long long nw = (long long)n_far * nwb;
+        unsigned long long ev = 0; /* counted at emission instead */
+        for (int pi = 0; pi < np; ++pi) {
+            const DPath<R> &p = paths[pi];
+            if (p.nnear < p.D) hs_deliver_path<R, true>(p, T + W, w, &L.d, rank, nw, &ev, p.nnear, p.D - p.nnear);
+        }
+    } else {
+        hs_noise_ahead<R>(groups, ng, T + w, W, b - n_upd - n_far, gridDim.x - n_upd - n_far);
+    }
+}
+
+/* events still in flight at step E = ctl->base (arrival >= E) of spikes emitted
+   before E: every spike (i, u) with u >= E - dmax still owes its bundles of
+   delay >= E - u.  One block; the result goes to ctl->pending. */
+template <typename R>
+__global__ __launch_bounds__(256) void hs_k_pending(const DPath<R> *paths, int np, HsCtl *ctl)
+{
+    __shared__ unsigned long long part[4];
+    const long long E = ctl->base;
+    unsigned long long acc = 0;
+    for (int pi = 0; pi < np; ++pi) {
+        const DPath<R> &p = paths[pi];
+        const int dmax = p.dmin + p.D - 1;
+        for (long long u = E - dmax; u < E; ++u) {
+            const int slot = (int)hs_mod(u, p.Q);
+            const int cnt = p.hist_cnt[(long long)slot * p.cstride];
+            const int f = (int)max(0LL, E - u - p.dmin); /* first delay slot arriving at or after E */
+            for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+                const long long base = (long long)p.hist[(long long)slot * p.n_src + k] * p.D;
+                acc += (unsigned long long)(p.bstart[base + p.D] - p.bstart[base + f]);
+            }
+        }
+    }
+    acc = hs_wave_sum(acc);
+    if (hs_lane() == 0) part[threadIdx.x / HS_WAVE] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int v = 0; v < (int)(blockDim.x / HS_WAVE); ++v) t += part[v];
+        ctl->pending = t;
+    }
+}
+
+/* blocks [0, n_del): delivery of the bundles due in steps T .. T+w-1; blocks
+   [n_del, grid): the noise draws of the same steps, read by the update launch
+   that follows (the delivery is bound by atomic throughput and leaves the
+   vector ALUs idle) */
 template <typename R, bool ACC>
 __global__ __launch_bounds__(1024) void hs_k_deliver(const DPath<R> *paths, int np, const DGroup<R> *groups, int ng,
-                                                     HsCtl *ctl, int koff, int w)
+                                                     HsCtl *ctl, int koff, int w, int n_del)
 {
     __shared__ HsDeliverLds L;
     const long long T = ctl->base + koff;
+    if ((int)blockIdx.x >= n_del) {
+        hs_noise_ahead<R>(groups, ng, T, w, blockIdx.x - n_del, gridDim.x - n_del);
+        return;
+    }
     const int nwb = blockDim.x / HS_WAVE;
     const long long rank = (long long)blockIdx.x * nwb + threadIdx.x / HS_WAVE;
-    const long long nw = (long long)gridDim.x * nwb;
+    const long long nw = (long long)n_del * nwb;
     unsigned long long ev = 0;
     for (int pi = 0; pi < np; ++pi)
         if (paths[pi].dmode == HS_DELIVER_SCATTER) hs_deliver_path<R, ACC>(paths[pi], T, w, &L, rank, nw, &ev);
     hs_flush_events(ev, ctl, rank);
-    /* the window's noise draws: independent of the state, so they can be made
-       here, by waves whose delivery chains are waiting on memory; the update of
-       the window reads them (xi[j*n + i] = draw of neuron i at step T+j) */
-    const long long nthr = (long long)gridDim.x * blockDim.x;
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (int gi = 0; gi < ng; ++gi) {
-        const DGroup<R> &g = groups[gi];
-        if (!g.xi) continue;
-        const long long tot = (long long)g.n * w;
-        for (long long q = tid; q < tot; q += nthr) {
-            const int j = (int)(q / g.n);
-            const int i = (int)(q - (long long)j * g.n);
-            g.xi[q] = hs_noise<R, HS_MODEL_LIF_NOISE>(g.seed, g.stream, i, T + j);
-        }
-    }
 }
 
 template <typename R, bool ACC>
@@ -281,8 +490,8 @@ __global__ __launch_bounds__(256) void hs_k_fused(const DGroup<R> *groups, int n
     } L;
     const long long T = ctl->base + koff;
     if ((int)blockIdx.x < n_upd) {
-        hs_update_phase<R, true, MS>(groups, ng, total_tiles, T, 1, 1, ctl->no_reset_before, &L.c, &ctl->spikes,
-                                     blockIdx.x, n_upd);
+        hs_update_phase<R, true, MS>(groups, ng, total_tiles, T, 1, 1, ctl->no_reset_before, &L.c,
+                                     ctl->spk_slot + (blockIdx.x % HS_EV_SLOTS) * HS_EV_STRIDE, blockIdx.x, n_upd);
     } else {
         const int nwb = blockDim.x / HS_WAVE;
         const long long rank = (long long)(blockIdx.x - n_upd) * nwb + threadIdx.x / HS_WAVE;
@@ -327,18 +536,19 @@ __global__ void hs_k_mark_reset(HsCtl *ctl) { ctl->no_reset_before = ctl->base; 
 /* ------------------------------------------------------------------------ */
 template <typename R, bool FUSED, int MS>
 static void hs_update_ms(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int koff, int w, int wmax,
-                         int block, int grid, hipStream_t s)
+                         int block, int grid, int n_noise, hipStream_t s)
 {
-    hipLaunchKernelGGL((hs_k_update<R, FUSED, MS>), dim3(grid), dim3(block), 0, s, groups, ng, total_tiles, ctl, koff,
-                       w, wmax);
+    hipLaunchKernelGGL((hs_k_update<R, FUSED, MS>), dim3(grid + n_noise), dim3(block), 0, s, groups, ng, total_tiles,
+                       ctl, koff, w, wmax, grid);
 }
 
 template <typename R>
 hipError_t hs_launch_update(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int koff, int w, int wmax,
-                            int fused, int ms, int block, int grid, hipStream_t s)
+                            int fused, int ms, int block, int grid, int n_noise, hipStream_t s)
 {
-    if (w < 1 || w > HS_MAX_WINDOW || wmax < w || wmax > HS_MAX_WINDOW || block > 256) return hipErrorInvalidValue;
-#define HS_UPD(F, M) hs_update_ms<R, F, M>(groups, ng, total_tiles, ctl, koff, w, wmax, block, grid, s)
+    if (w < 1 || w > HS_MAX_WINDOW || wmax < w || wmax > HS_MAX_WINDOW || block > 256 || grid < 1 || n_noise < 0)
+        return hipErrorInvalidValue;
+#define HS_UPD(F, M) hs_update_ms<R, F, M>(groups, ng, total_tiles, ctl, koff, w, wmax, block, grid, n_noise, s)
     if (fused) {
         if (ms == HS_MODEL_LIF_CUBA) HS_UPD(true, HS_MODEL_LIF_CUBA);
         else if (ms == HS_MODEL_LIF_NOISE) HS_UPD(true, HS_MODEL_LIF_NOISE);
@@ -355,14 +565,24 @@ hipError_t hs_launch_update(const DGroup<R> *groups, int ng, int total_tiles, Hs
 }
 
 template <typename R>
-hipError_t hs_launch_deliver(const DPath<R> *paths, int np, const DGroup<R> *groups, int ng, HsCtl *ctl, int koff,
-                             int w, int fused, int block, int grid, hipStream_t s)
+hipError_t hs_launch_noise(const DGroup<R> *groups, int ng, HsCtl *ctl, int koff, int nsteps, int grid, hipStream_t s)
 {
+    if (nsteps < 1 || grid < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(hs_k_noise<R>, dim3(grid), dim3(256), 0, s, groups, ng, ctl, koff, nsteps);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t hs_launch_deliver(const DPath<R> *paths, int np, const DGroup<R> *groups, int ng, HsCtl *ctl, int koff,
+                             int w, int fused, int block, int grid, int n_noise, hipStream_t s)
+{
+    if (grid < 1 || n_noise < 0) return hipErrorInvalidValue;
     if (fused)
-        hipLaunchKernelGGL((hs_k_deliver<R, true>), dim3(grid), dim3(block), 0, s, paths, np, groups, ng, ctl, koff, w);
+        hipLaunchKernelGGL((hs_k_deliver<R, true>), dim3(grid + n_noise), dim3(block), 0, s, paths, np, groups, ng, ctl,
+                           koff, w, grid);
     else
-        hipLaunchKernelGGL((hs_k_deliver<R, false>), dim3(grid), dim3(block), 0, s, paths, np, groups, ng, ctl, koff,
-                           w);
+        hipLaunchKernelGGL((hs_k_deliver<R, false>), dim3(grid + n_noise), dim3(block), 0, s, paths, np, groups, ng,
+                           ctl, koff, w, grid);
     return hipGetLastError();
 }
 
@@ -408,6 +628,33 @@ hipError_t hs_launch_finalize(const DGroup<R> *groups, int ng, int total_tiles, 
     return hipGetLastError();
 }
 
+template <typename R>
+hipError_t hs_launch_window(const DGroup<R> *groups, int ng, int total_tiles, const DPath<R> *paths, int np,
+                            HsCtl *ctl, int koff, int w, int W, int ms, int block, int n_upd, int n_far, int n_noise,
+                            hipStream_t s)
+{
+    if (w < 1 || w > W || W > HS_MAX_WINDOW || block > 256 || block < 64 || block % 64 || n_upd < 1 || n_far < 0 ||
+        n_noise < 0)
+        return hipErrorInvalidValue;
+    const dim3 grid(n_upd + n_far + n_noise);
+#define HS_WIN(M)                                                                                                    \
+    hipLaunchKernelGGL((hs_k_window<R, M>), grid, dim3(block), 0, s, groups, ng, total_tiles, paths, np, ctl, koff, w, \
+                       W, n_upd, n_far)
+    if (ms == HS_MODEL_LIF_CUBA) HS_WIN(HS_MODEL_LIF_CUBA);
+    else if (ms == HS_MODEL_LIF_NOISE) HS_WIN(HS_MODEL_LIF_NOISE);
+    else if (ms == HS_MODEL_HH_COBA) HS_WIN(HS_MODEL_HH_COBA);
+    else HS_WIN(0);
+#undef HS_WIN
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t hs_launch_pending(const DPath<R> *paths, int np, HsCtl *ctl, hipStream_t s)
+{
+    hipLaunchKernelGGL(hs_k_pending<R>, dim3(1), dim3(256), 0, s, paths, np, ctl);
+    return hipGetLastError();
+}
+
 hipError_t hs_launch_tick(HsCtl *ctl, long long k, hipStream_t s)
 {
     hipLaunchKernelGGL(hs_k_tick, dim3(1), dim3(1), 0, s, ctl, k);
@@ -416,12 +663,16 @@ hipError_t hs_launch_tick(HsCtl *ctl, long long k, hipStream_t s)
 
 #define HS_INSTANTIATE(R)                                                                                         \
     template hipError_t hs_launch_update<R>(const DGroup<R> *, int, int, HsCtl *, int, int, int, int, int, int, int, \
-                                            hipStream_t);                                                         \
+                                            int, hipStream_t);                                                    \
+    template hipError_t hs_launch_noise<R>(const DGroup<R> *, int, HsCtl *, int, int, int, hipStream_t);           \
     template hipError_t hs_launch_deliver<R>(const DPath<R> *, int, const DGroup<R> *, int, HsCtl *, int, int, int, \
-                                             int, int, hipStream_t);                                              \
+                                             int, int, int, hipStream_t);                                         \
     template hipError_t hs_launch_gather<R>(const DPath<R> *, int, HsCtl *, int, int, int, int, int, hipStream_t);  \
     template hipError_t hs_launch_fused<R>(const DGroup<R> *, int, int, const DPath<R> *, int, HsCtl *, int, int,   \
                                            int, int, int, hipStream_t);                                           \
-    template hipError_t hs_launch_finalize<R>(const DGroup<R> *, int, int, HsCtl *, int, int, hipStream_t);
+    template hipError_t hs_launch_finalize<R>(const DGroup<R> *, int, int, HsCtl *, int, int, hipStream_t);     \
+    template hipError_t hs_launch_window<R>(const DGroup<R> *, int, int, const DPath<R> *, int, HsCtl *, int, int, \
+                                            int, int, int, int, int, int, hipStream_t);                           \
+    template hipError_t hs_launch_pending<R>(const DPath<R> *, int, HsCtl *, hipStream_t);
 HS_INSTANTIATE(double)
 HS_INSTANTIATE(float)

@@ -33,19 +33,30 @@
    serialise in the L2 (one counter cost ~10 ns per wave). */
 #define HS_EV_SLOTS 256
 #define HS_EV_STRIDE 16
+/* the spike count of each step of a group's ring sits on its own 128-byte line:
+   the blocks of an update launch reserve list space in w counters at once */
+#define HS_CNT_STRIDE 32
 
 struct HsCtl {
     long long base;            /* simulation step of chunk offset 0 */
     long long no_reset_before; /* resets of spikes at steps < this were already applied */
-    unsigned long long spikes; /* spikes emitted */
-    unsigned long long pad_;
-    unsigned long long ev_slot[HS_EV_SLOTS * HS_EV_STRIDE]; /* synaptic events delivered: sum of ev_slot[k*STRIDE] */
+    unsigned long long pending; /* pipelined windows: events of emitted spikes still in flight at base */
+    unsigned long long pad_[13];
+    unsigned long long ev_slot[HS_EV_SLOTS * HS_EV_STRIDE];  /* synaptic events delivered: sum of ev_slot[k*STRIDE] */
+    unsigned long long spk_slot[HS_EV_SLOTS * HS_EV_STRIDE]; /* spikes emitted, spread the same way */
 };
 
 static inline unsigned long long hs_ctl_events(const HsCtl &c)
 {
     unsigned long long e = 0;
     for (int k = 0; k < HS_EV_SLOTS; ++k) e += c.ev_slot[k * HS_EV_STRIDE];
+    return e;
+}
+
+static inline unsigned long long hs_ctl_spikes(const HsCtl &c)
+{
+    unsigned long long e = 0;
+    for (int k = 0; k < HS_EV_SLOTS; ++k) e += c.spk_slot[k * HS_EV_STRIDE];
     return e;
 }
 
@@ -74,7 +85,8 @@ struct DGroup {
     long long st_t0[HS_MAX_STATE_MONS], st_rows[HS_MAX_STATE_MONS];
     R *st_buf[HS_MAX_STATE_MONS];
     int tile_begin; /* first tile index of this group in the concatenated tile space */
-    R *xi;          /* windowed form: the window's noise [w][n] drawn by the delivery launch, or nullptr */
+    R *xi;          /* windowed form: noise ring [xi_slots][n], draw of step t in slot t mod xi_slots (or nullptr) */
+    int xi_slots;
 };
 
 template <typename R>
@@ -82,7 +94,8 @@ struct DPath {
     int src, tgt, var;
     int n_src, dmin, D, Q, wmode;
     const int32_t *hist;     /* source group's spike ring */
-    const int32_t *hist_cnt;
+    const int32_t *hist_cnt; /* count of ring slot s at hist_cnt[s * cstride] */
+    int cstride;
     const int32_t *bstart;
     const int32_t *post;
     const R *w;
@@ -97,6 +110,9 @@ struct DPath {
     int dmode, n_tiles;
     const int32_t *goff;
     const uint32_t *gsyn;
+    /* pipelined windows: delay slots [0, nnear) are pushed by the emitting
+       update, [nnear, D) pulled by the delivery blocks */
+    int nnear;
 };
 
 /* runtime services for the translation units (hs_runtime.hip) */
@@ -131,13 +147,18 @@ hipError_t hs_launch_deliver_one_ctl(const DPath<R> &p, HsCtl *ctl, unsigned lon
  *            blocks [n_upd, n_upd + n_del), one launch (every delay >= 1 step)
  */
 /* ms: the model shared by every group (specialised kernel), 0 for mixed networks */
+/* grid update blocks plus n_noise blocks drawing the noise of steps t+w .. t+w+wmax-1 */
 template <typename R>
 hipError_t hs_launch_update(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int koff, int w,
-                            int wmax, int fused, int ms, int block, int grid, hipStream_t s);
-/* groups/ng: groups whose xi buffer the launch fills for steps t .. t+w-1 (may be 0) */
+                            int wmax, int fused, int ms, int block, int grid, int n_noise, hipStream_t s);
+/* noise draws of steps t .. t+nsteps-1 into the groups' noise rings */
+template <typename R>
+hipError_t hs_launch_noise(const DGroup<R> *groups, int ng, HsCtl *ctl, int koff, int nsteps, int grid,
+                           hipStream_t s);
+/* grid delivery blocks plus n_noise blocks drawing the noise of steps t .. t+w-1 */
 template <typename R>
 hipError_t hs_launch_deliver(const DPath<R> *paths, int np, const DGroup<R> *groups, int ng, HsCtl *ctl, int koff,
-                             int w, int fused, int block, int grid, hipStream_t s);
+                             int w, int fused, int block, int grid, int n_noise, hipStream_t s);
 template <typename R>
 hipError_t hs_launch_fused(const DGroup<R> *groups, int ng, int total_tiles, const DPath<R> *paths, int np,
                            HsCtl *ctl, int koff, int ms, int block, int n_upd, int n_del, hipStream_t s);
@@ -148,6 +169,14 @@ hipError_t hs_launch_gather(const DPath<R> *paths, int np, HsCtl *ctl, int koff,
 template <typename R>
 hipError_t hs_launch_finalize(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int fused,
                               int block, hipStream_t s);
+/* pipelined windows: one launch per window (update + near push | far pull | next noise) */
+template <typename R>
+hipError_t hs_launch_window(const DGroup<R> *groups, int ng, int total_tiles, const DPath<R> *paths, int np,
+                            HsCtl *ctl, int koff, int w, int W, int ms, int block, int n_upd, int n_far, int n_noise,
+                            hipStream_t s);
+/* events in flight at ctl->base into ctl->pending */
+template <typename R>
+hipError_t hs_launch_pending(const DPath<R> *paths, int np, HsCtl *ctl, hipStream_t s);
 hipError_t hs_launch_tick(HsCtl *ctl, long long k, hipStream_t s);
 
 #endif /* HS_INTERNAL_H */

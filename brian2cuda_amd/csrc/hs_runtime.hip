@@ -292,6 +292,7 @@ static int hs_deliver_t(int n_src, int dmin, int D, int Q, const int32_t *hist, 
     p.wmode = wmode;
     p.hist = hist;
     p.hist_cnt = hist_cnt;
+    p.cstride = 1;
     p.bstart = bstart;
     p.post = post;
     p.w = (const R *)w;
@@ -342,7 +343,8 @@ struct HGroup {
     int Q = 0;
     int smon = -1;
     std::vector<int> stmons;
-    void *xi = nullptr; /* windowed form: noise of the window [W][n], drawn by the delivery launch */
+    void *xi = nullptr; /* windowed form: noise ring [xi_slots][n], drawn one window ahead */
+    int xi_slots = 0;
 };
 
 struct HPath {
@@ -386,15 +388,17 @@ struct hs_net {
     int window = 1;        /* steps per update launch (fused form) */
     int dmin_all = 0;      /* smallest delay over all pathways */
     int mode = 0;          /* 0 plain, 1 fused per-step pair, 2 fused single launch, 3 fused windows */
+    bool noise_ahead = false; /* mode 3 with a noise model: draws made one window ahead */
     int64_t step = 0;
     /* 0 update block, 1 scatter grid, 2 graph chunk, 3 fused delivery blocks,
-       4 window, 5 delivery mode, 6 gather block, 7 kernel timing */
-    long long opt[8] = {256, 0, 256, 0, 0, 0, 256, 0};
+       4 window, 5 delivery mode, 6 gather block, 7 kernel timing, 8 noise blocks,
+       9 pipelined windows */
+    long long opt[10] = {256, 0, 256, 0, 0, 0, 256, 0, 0, 1};
     /* kernel timing: event pairs of the current run, totals per kind */
     std::vector<hipEvent_t> kt_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> kt_open;
-    double kt_ms[4] = {0, 0, 0, 0};
-    int64_t kt_n[4] = {0, 0, 0, 0};
+    double kt_ms[5] = {0, 0, 0, 0, 0};
+    int64_t kt_n[5] = {0, 0, 0, 0, 0};
     int n_scatter = 0, n_gather = 0, gather_tiles = 0;
     std::map<int64_t, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -734,10 +738,13 @@ extern "C" int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start
 
 extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
 {
-    if (!net || key < 0 || key > 7) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (!net || key < 0 || key > 9) return hs_fail(HS_ERR_ARG, "bad option key");
     if (key == 7 && (value < 0 || value > 1)) return hs_fail(HS_ERR_ARG, "kernel timing: 0 off, 1 on");
+    if (key == 8 && (value < 0 || value > (1 << 16))) return hs_fail(HS_ERR_ARG, "bad noise block count");
+    if (key == 9 && (value < 0 || value > 1)) return hs_fail(HS_ERR_ARG, "pipelined windows: 0 off, 1 on");
+    if (key == 9 && net->built) return hs_fail(HS_ERR_STATE, "the engine form is fixed at the first run");
     if (key == 7)
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < 5; ++q) {
             net->kt_ms[q] = 0;
             net->kt_n[q] = 0;
         }
@@ -758,7 +765,7 @@ extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
 
 extern "C" int hs_net_get_option(hs_net *net, int key, long long *value)
 {
-    if (!net || key < 0 || key > 7 || !value) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (!net || key < 0 || key > 9 || !value) return hs_fail(HS_ERR_ARG, "bad option key");
     *value = net->opt[key];
     return HS_OK;
 }
@@ -789,8 +796,10 @@ static int hs_build(hs_net *net)
         w = std::min(w, HS_MAX_DELAY_SLOTS / dspan);
         w = std::max(w, 1);
         net->window = w;
-        /* the single-launch kernel carries scatter delivery only */
-        net->mode = (w == 1 && net->n_gather == 0) ? 2 : 3;
+        /* the single-launch kernels carry scatter delivery only */
+        if (w == 1 && net->n_gather == 0) net->mode = 2;
+        else if (w >= 2 && net->n_gather == 0 && net->opt[9]) net->mode = 4;
+        else net->mode = 3;
     }
     const int W = net->window;
     for (HGroup &g : net->g) {
@@ -799,23 +808,27 @@ static int hs_build(hs_net *net)
             if (P.src == (int)(&g - &net->g[0])) dmax = std::max(dmax, P.dmax);
         g.Q = dmax + 2 * W; /* see the ring invariants in hs_engine.hip */
         HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.Q * g.n, (void **)&g.hist, false));
-        HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.Q, (void **)&g.hist_cnt, true));
+        HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.Q * HS_CNT_STRIDE, (void **)&g.hist_cnt, true));
     }
     if (net->engine == HS_ENGINE_FUSED) {
         for (const HPath &P : net->p) {
             HGroup &gt = net->g[P.tgt];
-            gt.nacc = W + 1;
+            /* pipelined windows write up to 3W-2 steps ahead of the window they update */
+            gt.nacc = net->mode == 4 ? 3 * W + 1 : W + 1;
             if (!gt.acc[P.var])
                 HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)gt.n * gt.nacc, &gt.acc[P.var], true));
         }
     }
-    /* windowed form with a scatter launch in every window: that launch draws the
-       window's noise (its waves wait on memory most of the time) and the update
-       reads it, instead of evaluating Philox + Box-Muller on its own chain */
-    if (net->mode == 3 && net->n_scatter > 0)
+    /* windowed form: the noise of a window is drawn one window ahead, by extra
+       blocks of the previous update launch (the update's own blocks leave most
+       of the chip idle), into a ring of 2W steps that the update reads */
+    if (net->mode == 3 || net->mode == 4)
         for (HGroup &g : net->g)
-            if (g.model == HS_MODEL_LIF_NOISE)
-                HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)g.n * W, &g.xi, false));
+            if (g.model == HS_MODEL_LIF_NOISE) {
+                g.xi_slots = 2 * W;
+                HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)g.n * g.xi_slots, &g.xi, false));
+                net->noise_ahead = true;
+            }
     HsCtl c;
     memset(&c, 0, sizeof(c));
     c.base = net->step;
@@ -873,6 +886,7 @@ static int hs_upload_desc_t(hs_net *net)
         }
         d.tile_begin = tiles;
         d.xi = (R *)g.xi;
+        d.xi_slots = g.xi_slots;
         tiles += (g.n + block - 1) / block;
     }
     std::vector<DPath<R>> dp(net->p.size());
@@ -892,6 +906,7 @@ static int hs_upload_desc_t(hs_net *net)
         d.wmode = P.wmode;
         d.hist = gs.hist;
         d.hist_cnt = gs.hist_cnt;
+        d.cstride = HS_CNT_STRIDE;
         d.bstart = P.bstart;
         d.post = P.post;
         d.w = (const R *)P.w;
@@ -904,6 +919,8 @@ static int hs_upload_desc_t(hs_net *net)
         d.n_tiles = P.n_tiles;
         d.goff = P.goff;
         d.gsyn = P.gsyn;
+        /* pipelined windows: delays < 2W are pushed at emission */
+        d.nnear = net->mode == 4 ? std::max(0, std::min(P.D, 2 * net->window - P.dmin)) : 0;
     }
     if (!net->dgroups) HS_TRY(hs_dalloc(net, sizeof(DGroup<R>) * std::max<size_t>(1, dg.size()), &net->dgroups, false));
     if (!net->dpaths) HS_TRY(hs_dalloc(net, sizeof(DPath<R>) * std::max<size_t>(1, dp.size()), &net->dpaths, false));
@@ -957,6 +974,13 @@ static int hs_fused_deliver_blocks(const hs_net *net)
     return g_rt.n_cu * 2;
 }
 
+/* blocks drawing the next window's noise beside the update blocks */
+static int hs_noise_grid(const hs_net *net)
+{
+    if (net->opt[8] > 0) return (int)net->opt[8];
+    return g_rt.n_cu * 2;
+}
+
 static int hs_update_grid(const hs_net *net)
 {
     return std::max(1, std::min(net->total_tiles, 1 << 20));
@@ -1005,7 +1029,8 @@ static int hs_kt_collect(hs_net *net)
 
 extern "C" int hs_net_kernel_time(hs_net *net, int kind, double *total_ms, int64_t *launches)
 {
-    if (!net || kind < 0 || kind > 3) return hs_fail(HS_ERR_ARG, "kernel kind: 0 update, 1 scatter, 2 gather, 3 fused");
+    if (!net || kind < 0 || kind > 4)
+        return hs_fail(HS_ERR_ARG, "kernel kind: 0 update, 1 scatter, 2 gather, 3 fused, 4 window");
     if (total_ms) *total_ms = net->kt_ms[kind];
     if (launches) *launches = net->kt_n[kind];
     return HS_OK;
@@ -1030,11 +1055,18 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
     for (const HGroup &g : net->g)
         if (g.model != ms) ms = 0;
     const int ggrid = std::max(1, std::min(net->gather_tiles, 1 << 20));
+    /* where a window's noise is drawn: by extra blocks of its scatter-delivery
+       launch (two-launch windows), else one window ahead by the update or window
+       launch, after a first launch for the chunk's first window */
+    const int n_noise = net->noise_ahead ? hs_noise_grid(net) : 0;
+    const bool noise_in_deliver = n_noise > 0 && net->mode == 3 && net->n_scatter > 0;
+    const int n_noise_del = noise_in_deliver ? n_noise : 0;
+    const int n_noise_ahead = noise_in_deliver ? 0 : n_noise;
     auto deliver = [&](int koff, int w) -> int {
         if (net->n_scatter)
             HS_TRY(hs_kt_launch(net, 1, [&]() -> int {
-                HS_HIP(hs_launch_deliver<R>(dp, np, dg, net->mode == 3 ? ng : 0, net->ctl, koff, w, fused, 256,
-                                            hs_deliver_grid(net), g_rt.stream));
+                HS_HIP(hs_launch_deliver<R>(dp, np, dg, ng, net->ctl, koff, w, fused, 256, hs_deliver_grid(net),
+                                            n_noise_del, g_rt.stream));
                 return HS_OK;
             }));
         if (net->n_gather)
@@ -1047,15 +1079,25 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
     auto update = [&](int koff, int w, int wmax, int fu) -> int {
         return hs_kt_launch(net, 0, [&]() -> int {
             HS_HIP(hs_launch_update<R>(dg, ng, net->total_tiles, net->ctl, koff, w, wmax, fu, ms, block,
-                                       hs_update_grid(net), g_rt.stream));
+                                       hs_update_grid(net), n_noise_ahead, g_rt.stream));
             return HS_OK;
         });
     };
-    if (net->mode == 3) {
+    if (net->mode == 3 || net->mode == 4) {
+        if (n_noise_ahead) HS_HIP(hs_launch_noise<R>(dg, ng, net->ctl, 0, net->window, n_noise, g_rt.stream));
         for (int64_t j = 0; j < k; j += net->window) {
             const int w = (int)std::min<int64_t>(net->window, k - j);
-            HS_TRY(deliver((int)j, w));
-            HS_TRY(update((int)j, w, net->window, 1));
+            if (net->mode == 4) {
+                HS_TRY(hs_kt_launch(net, 4, [&]() -> int {
+                    HS_HIP(hs_launch_window<R>(dg, ng, net->total_tiles, dp, np, net->ctl, (int)j, w, net->window, ms,
+                                               block, hs_update_grid(net), hs_deliver_grid(net), n_noise,
+                                               g_rt.stream));
+                    return HS_OK;
+                }));
+            } else {
+                HS_TRY(deliver((int)j, w));
+                HS_TRY(update((int)j, w, net->window, 1));
+            }
         }
     } else {
         for (int64_t j = 0; j < k; ++j) {
@@ -1178,6 +1220,15 @@ extern "C" int hs_net_run(hs_net *net, int64_t nsteps)
     HS_HIP(hipStreamSynchronize(g_rt.stream));
     auto t1 = std::chrono::steady_clock::now();
     HS_TRY(hs_kt_collect(net));
+    if (net->mode == 4 && !net->p.empty()) {
+        /* pipelined windows count events at emission: subtract those still in flight */
+        if (net->prec == HS_F64)
+            HS_HIP(hs_launch_pending<double>((const DPath<double> *)net->dpaths, (int)net->p.size(), net->ctl,
+                                             g_rt.stream));
+        else
+            HS_HIP(hs_launch_pending<float>((const DPath<float> *)net->dpaths, (int)net->p.size(), net->ctl,
+                                            g_rt.stream));
+    }
     float ms = 0;
     HS_HIP(hipEventElapsedTime(&ms, net->ev0, net->ev1));
     net->last_gpu_ms = ms;
@@ -1188,8 +1239,8 @@ extern "C" int hs_net_run(hs_net *net, int64_t nsteps)
     HsCtl &c = cbuf[0];
     HS_HIP(hipMemcpyAsync(&c, net->ctl, sizeof(c), hipMemcpyDeviceToHost, g_rt.stream));
     HS_HIP(hipStreamSynchronize(g_rt.stream));
-    net->events = hs_ctl_events(c);
-    net->spikes = c.spikes;
+    net->events = hs_ctl_events(c) - (net->mode == 4 ? c.pending : 0ull);
+    net->spikes = hs_ctl_spikes(c);
     if (c.base != net->step) return hs_fail(HS_ERR_STATE, "device step %lld != host step %lld", c.base, (long long)net->step);
     for (const HMon &M : net->m) {
         if (M.kind != 0) continue;
@@ -1213,6 +1264,7 @@ extern "C" int hs_net_stats(hs_net *net, hs_stats *out)
     out->last_run_gpu_ms = net->last_gpu_ms;
     out->last_run_steps = net->last_steps;
     if (net->p.empty() || net->mode == 2) out->launches_per_step = 1.0;
+    else if (net->mode == 4) out->launches_per_step = 1.0 / net->window;
     else {
         const double per = 1.0 + (net->n_scatter ? 1 : 0) + (net->n_gather ? 1 : 0);
         out->launches_per_step = net->mode == 3 ? per / net->window : per;

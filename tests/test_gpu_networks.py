@@ -70,18 +70,34 @@ def test_brunel_parity_small(hs, oracle_lib, prec, engine, sem):
     assert ref.events > 0
 
 
+@pytest.mark.parametrize("pipelined", [1, 0])
 @pytest.mark.parametrize("window", [1, 2, 3, 5])
 @pytest.mark.parametrize("prec", [64, 32])
-def test_brunel_windows(hs, oracle_lib, window, prec):
-    """The windowed fused form (delivery of w steps, then w update steps per tile)
-    is bit-identical to the oracle's ACC semantics for every window <= the minimum
-    delay (5 steps), including runs whose length is not a multiple of the window."""
+def test_brunel_windows(hs, oracle_lib, window, prec, pipelined):
+    """The windowed fused forms are bit-identical to the oracle's ACC semantics for
+    every window <= the minimum delay (5 steps), including runs whose length is
+    not a multiple of the window:
+      pipelined = 1: one launch per window (update + push of delays < 2W, pull of
+                     the longer delays for the next window, next window's noise)
+      pipelined = 0: delivery of w steps, then w update steps per tile"""
     spec = nw.brunel(N=3000, K=300, seed=4, monitors=True)
-    net, ref = _compare(spec, prec, "fused", 1, None, oracle_lib, options={4: window, 2: 64},
+    net, ref = _compare(spec, prec, "fused", 1, None, oracle_lib, options={4: window, 2: 64, 9: pipelined},
                         chunks=[7, 600, 393, 1000])
     s = net.stats()
     assert s["window"] == window
     assert s["spikes"] == len(ref.spikes(0)[0])
+    if window >= 2:
+        assert s["launches_per_step"] == pytest.approx((1.0 if pipelined else 2.0) / window)
+
+
+@pytest.mark.parametrize("dlo,dhi", [(0.2e-3, 0.3e-3), (0.2e-3, 0.7e-3), (0.4e-3, 2.5e-3)])
+def test_pipelined_near_far_split(hs, oracle_lib, dlo, dhi):
+    """Delay spans entirely below 2W (push only), straddling 2W, and mostly above
+    it (pull only beyond 2W), with the window at the minimum delay."""
+    spec = nw.brunel(N=2000, K=200, seed=12, monitors=True, dlo=dlo, dhi=dhi)
+    for prec in (64, 32):
+        net, ref = _compare(spec, prec, "fused", 1, None, oracle_lib, options={2: 50}, chunks=[3, 333, 164])
+        assert net.stats()["window"] == int(round(dlo / 1e-4))
 
 
 @pytest.mark.parametrize("delivery", [1, 2])
@@ -119,7 +135,8 @@ def test_batch_parity_small(hs, oracle_lib, engine, sem):
 @pytest.mark.parametrize("engine,sem,delivery", [("plain", 0, 1), ("fused", 1, 2), ("fused", 1, 1)])
 def test_kernel_timing_keeps_results(hs, oracle_lib, engine, sem, delivery):
     """Kernel timing (no graphs, an event pair per launch) changes no bits and
-    counts one update launch per window and one delivery launch per window."""
+    counts the launches of each kind: an update and a delivery launch per window
+    (plain form, gather), or one window launch (pipelined fused form)."""
     from brian2cuda_amd import runtime as rt
     spec = nw.brunel(N=2500, K=250, seed=6, monitors=True)
     net, _ = _compare(spec, 32, engine, sem, None, oracle_lib, options={5: delivery, 7: 1}, chunks=[300, 211])
@@ -127,13 +144,14 @@ def test_kernel_timing_keeps_results(hs, oracle_lib, engine, sem, delivery):
     chunk = net.get_option(rt.OPT_GRAPH_CHUNK)  # a run advances in chunks; a window never spans two
     pieces = [min(chunk, k - j) for k in (300, 211) for j in range(0, k, chunk)]
     launches = sum(-(-p // w) for p in pieces)
-    ms_u, n_u = net.kernel_time(rt.KT_UPDATE)
-    kind = rt.KT_SCATTER if delivery == 1 else rt.KT_GATHER
-    ms_d, n_d = net.kernel_time(kind)
-    assert n_u == launches and n_d == launches
-    assert ms_u > 0 and ms_d > 0
-    other = rt.KT_GATHER if delivery == 1 else rt.KT_SCATTER
-    assert net.kernel_time(other) == (0.0, 0)
+    counts = {k: net.kernel_time(k) for k in rt.KERNEL_NAMES}
+    if engine == "fused" and delivery == 1:
+        expect = {rt.KT_WINDOW: launches}
+    else:
+        expect = {rt.KT_UPDATE: launches, (rt.KT_SCATTER if delivery == 1 else rt.KT_GATHER): launches}
+    for k, (ms, n) in counts.items():
+        assert n == expect.get(k, 0), (rt.KERNEL_NAMES[k], n)
+        assert (ms > 0) == (k in expect)
 
 
 @pytest.mark.parametrize("case", ["push_outer_loop", "push_outer_loop2", "circular_eventspaces"])
