@@ -7,10 +7,11 @@ delays U[0.5, 5] ms quantised to dt=0.1 ms (5..50 steps), sigma=1 mV, mu=25 mV,
 Philox noise; 1 s of simulated time (10,000 steps) after a warm-up.
 
   fp32 batched : FP32 state, weights and arithmetic; effects accumulated per
-                 step; one launch per window of W = min(8, min delay) steps
-                 (update + push of the delays < 2W | pull of the longer delays
-                 for the next window | the next window's noise draws), K steps
-                 per captured HIP graph
+                 step; windows of W = min(8, min delay) steps, each one
+                 delivery launch (all bundles due in the window, plus the
+                 window's noise draws on otherwise idle blocks) and one update
+                 launch (W steps per neuron with the state in registers);
+                 K steps per captured HIP graph
   fp64 plain   : FP64 state, weights and arithmetic; update kernel + delivery
                  kernel per step, effects applied in place (Brian2's order)
 

@@ -393,7 +393,7 @@ struct hs_net {
     /* 0 update block, 1 scatter grid, 2 graph chunk, 3 fused delivery blocks,
        4 window, 5 delivery mode, 6 gather block, 7 kernel timing, 8 noise blocks,
        9 pipelined windows */
-    long long opt[10] = {256, 0, 256, 0, 0, 0, 256, 0, 0, 1};
+    long long opt[10] = {256, 0, 256, 0, 0, 0, 256, 0, 0, 0};
     /* kernel timing: event pairs of the current run, totals per kind */
     std::vector<hipEvent_t> kt_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> kt_open;

@@ -163,7 +163,10 @@ int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start, int stop,
  * kernel, 4 window (steps per update launch, 0: largest allowed), 5 delivery
  * layout (0 auto, 1 scatter, 2 gather), 6 gather block size, 7 kernel timing
  * (1: no graphs, every launch bracketed by HIP events; totals reset when set),
- * 8 blocks drawing the next window's noise beside the update (0: 2 per CU). */
+ * 8 blocks drawing a window's noise beside the delivery or update (0: 2 per CU),
+ * 9 pipelined windows (1: one launch per window -- update and push of the
+ * delays < 2W, pull of the longer delays for the next window; 0, the default:
+ * a delivery launch and an update launch per window). */
 int hs_net_set_option(hs_net *net, int key, long long value);
 int hs_net_get_option(hs_net *net, int key, long long *value);
 /* With option 7 on: device time and launch count, summed over the runs since

@@ -132,20 +132,22 @@ def test_batch_parity_small(hs, oracle_lib, engine, sem):
     _compare(spec, 32, engine, sem, 1000, oracle_lib)
 
 
-@pytest.mark.parametrize("engine,sem,delivery", [("plain", 0, 1), ("fused", 1, 2), ("fused", 1, 1)])
-def test_kernel_timing_keeps_results(hs, oracle_lib, engine, sem, delivery):
+@pytest.mark.parametrize("engine,sem,delivery,pipelined",
+                         [("plain", 0, 1, 0), ("fused", 1, 2, 0), ("fused", 1, 1, 0), ("fused", 1, 1, 1)])
+def test_kernel_timing_keeps_results(hs, oracle_lib, engine, sem, delivery, pipelined):
     """Kernel timing (no graphs, an event pair per launch) changes no bits and
-    counts the launches of each kind: an update and a delivery launch per window
-    (plain form, gather), or one window launch (pipelined fused form)."""
+    counts the launches of each kind: an update and a delivery launch per window,
+    or one window launch (pipelined fused form)."""
     from brian2cuda_amd import runtime as rt
     spec = nw.brunel(N=2500, K=250, seed=6, monitors=True)
-    net, _ = _compare(spec, 32, engine, sem, None, oracle_lib, options={5: delivery, 7: 1}, chunks=[300, 211])
+    net, _ = _compare(spec, 32, engine, sem, None, oracle_lib, options={5: delivery, 7: 1, 9: pipelined},
+                      chunks=[300, 211])
     w = net.stats()["window"]
     chunk = net.get_option(rt.OPT_GRAPH_CHUNK)  # a run advances in chunks; a window never spans two
     pieces = [min(chunk, k - j) for k in (300, 211) for j in range(0, k, chunk)]
     launches = sum(-(-p // w) for p in pieces)
     counts = {k: net.kernel_time(k) for k in rt.KERNEL_NAMES}
-    if engine == "fused" and delivery == 1:
+    if pipelined:
         expect = {rt.KT_WINDOW: launches}
     else:
         expect = {rt.KT_UPDATE: launches, (rt.KT_SCATTER if delivery == 1 else rt.KT_GATHER): launches}
