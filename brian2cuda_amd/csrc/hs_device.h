@@ -170,10 +170,13 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
                                                 long long wave_rank,
                                 long long n_waves, unsigned long long *ev_lane, int di0 = 0, int dn = -1)
 {
-    /* delay slots di0 .. di0+dn-1 of the pathway (all by default) */
+    /* delay slots di0 .. di0+dn-1 of the pathway (all by default); the items
+       are indexed ((j * dn) + (di - di0)) * nsub + sub-list */
     const int D = p.D;
     if (dn < 0) dn = D;
-    const int npairs = w * dn;
+    const int ns = p.nsub;
+    const int npairs = w * dn * ns;
+    const long long rowlen = (long long)ns * p.subcap;
     const int tq = (int)hs_mod(T, p.Q);                 /* ring slot of step T */
     const int ta = ACC ? (int)hs_mod(T, p.nacc) : 0;    /* accumulator slot of step T */
     {
@@ -185,8 +188,9 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
             const int q = c + threadIdx.x;
             int cnt = 0;
             if (q < npairs) {
-                const int j = q / dn, di = di0 + (q - j * dn);
-                cnt = p.hist_cnt[hs_wrap(tq + j - (p.dmin + di), p.Q) * p.cstride];
+                const int pq = q / ns, sq = q - pq * ns;
+                const int j = pq / dn, di = di0 + (pq - j * dn);
+                cnt = p.hist_cnt[((long long)hs_wrap(tq + j - (p.dmin + di), p.Q) * ns + sq) * p.cstride];
             }
             int tot;
             const int ex = hs_wave_excl_scan(cnt, &tot);
@@ -221,10 +225,11 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
                     else hi = mid;
                 }
                 const int kk = (int)(wi - L->pref[lo]);
-                jw = lo / dn;
-                const int di = di0 + (lo - jw * dn);
+                const int pq = lo / ns, sq = lo - pq * ns;
+                jw = pq / dn;
+                const int di = di0 + (pq - jw * dn);
                 const long long slot = hs_wrap(tq + jw - (p.dmin + di), p.Q);
-                pre = p.hist[slot * p.n_src + kk];
+                pre = p.hist[slot * rowlen + (long long)sq * p.subcap + kk];
                 const long long b = (long long)pre * D + di;
                 s0 = p.bstart[b];
                 len = p.bstart[b + 1] - s0;
@@ -311,14 +316,17 @@ __device__ __forceinline__ void hs_gather_tile(const DPath<R> &p, int tile, long
         for (int i = threadIdx.x; i < HS_GTILE; i += blockDim.x) lacc[i] = i < ntl ? p.target[base + i] : (R)0;
     }
     const int dmax = p.dmin + p.D - 1;
-    const int nU = p.D - 1 + w; /* candidate emission steps T-dmax .. T+w-1-dmin */
+    const int ns = p.nsub;
+    const int nU = (p.D - 1 + w) * ns; /* (candidate emission step T-dmax .. T+w-1-dmin, sub-list) */
+    const long long rowlen = (long long)ns * p.subcap;
     const int tq = (int)hs_mod(T, p.Q);
     const int ta = ACC ? (int)hs_mod(T, p.nacc) : 0;
     if (threadIdx.x < HS_WAVE) {
         int carry = 0;
         for (int c = 0; c < nU; c += HS_WAVE) {
             const int q = c + hs_lane();
-            const int cnt = q < nU ? p.hist_cnt[hs_wrap(tq - dmax + q, p.Q) * p.cstride] : 0;
+            const int cnt =
+                q < nU ? p.hist_cnt[((long long)hs_wrap(tq - dmax + q / ns, p.Q) * ns + q % ns) * p.cstride] : 0;
             int tot;
             const int ex = hs_wave_excl_scan(cnt, &tot);
             if (q < nU) L->pref[q] = carry + ex;
@@ -336,8 +344,10 @@ __device__ __forceinline__ void hs_gather_tile(const DPath<R> &p, int tile, long
             if (L->pref[mid] <= c) lo = mid;
             else hi = mid;
         }
-        const long long u = T - dmax + lo;
-        const int pre = p.hist[(long long)hs_wrap(tq - dmax + lo, p.Q) * p.n_src + (c - L->pref[lo])];
+        const int lu = lo / ns, ls = lo - lu * ns;
+        const long long u = T - dmax + lu;
+        const int pre = p.hist[(long long)hs_wrap(tq - dmax + lu, p.Q) * rowlen + (long long)ls * p.subcap +
+                               (c - L->pref[lo])];
         const long long gi = (long long)tile * p.n_src + pre; /* tile-major: a block stays in its tile's rows */
         const int a = p.goff[gi], b = p.goff[gi + 1];
         for (int s = a; s < b; ++s) {

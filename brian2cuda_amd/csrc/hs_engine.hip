@@ -165,9 +165,13 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
     const int i = tile * blockDim.x + threadIdx.x;
     const bool valid = i < g.n;
     const int tq = (int)hs_mod(T, g.Q);  /* ring slot of step T */
-    /* ring slots of the steps after this launch start from an empty list */
-    if (tile == 0 && threadIdx.x == 0)
-        for (int m = 0; m < wmax; ++m) g.hist_cnt[hs_wrap(tq + w + m, g.Q) * HS_CNT_STRIDE] = 0;
+    /* ring slots of the steps after this launch start from empty lists */
+    if (tile == 0)
+        for (int q = threadIdx.x; q < wmax * g.nsub; q += blockDim.x)
+            g.hist_cnt[((long long)hs_wrap(tq + w + q / g.nsub, g.Q) * g.nsub + q % g.nsub) * HS_CNT_STRIDE] = 0;
+    /* this tile's sub-list (a tile lies inside one 256-neuron chunk) */
+    const int sub = (int)(((long long)tile * blockDim.x >> 8) % g.nsub);
+    const long long rowlen = (long long)g.nsub * g.subcap;
     R s[NV];
     int32_t last = HS_NEVER_SPIKED;
 #pragma unroll
@@ -265,7 +269,9 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
             L->wave_cnt[j][v] = tot;
             tot += c;
         }
-        L->base[j] = tot ? atomicAdd(g.hist_cnt + hs_wrap(tq + j, g.Q) * HS_CNT_STRIDE, tot) : 0;
+        L->base[j] = tot ? atomicAdd(g.hist_cnt + ((long long)hs_wrap(tq + j, g.Q) * g.nsub + sub) * HS_CNT_STRIDE,
+                                     tot)
+                         : 0;
         L->mbase[j] = (tot && g.smon_n) ? atomicAdd(g.smon_n, (unsigned long long)tot) : 0ull;
     }
     if (threadIdx.x < HS_WAVE) { /* one count per block and window */
@@ -278,7 +284,7 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
         const unsigned long long m = __ballot(spk);
         if (spk) {
             const int pos = L->wave_cnt[j][wid] + __popcll(m & hs_lanemask_lt());
-            g.hist[(long long)hs_wrap(tq + j, g.Q) * g.n + L->base[j] + pos] = i;
+            g.hist[(long long)hs_wrap(tq + j, g.Q) * rowlen + (long long)sub * g.subcap + L->base[j] + pos] = i;
             if (g.smon_n) {
                 const unsigned long long r = L->mbase[j] + (unsigned long long)pos;
                 if ((long long)r < g.smon_cap) {
@@ -423,11 +429,14 @@ __global__ __launch_bounds__(256) void hs_k_pending(const DPath<R> *paths, int n
         const int dmax = p.dmin + p.D - 1;
         for (long long u = E - dmax; u < E; ++u) {
             const int slot = (int)hs_mod(u, p.Q);
-            const int cnt = p.hist_cnt[(long long)slot * p.cstride];
             const int f = (int)max(0LL, E - u - p.dmin); /* first delay slot arriving at or after E */
-            for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-                const long long base = (long long)p.hist[(long long)slot * p.n_src + k] * p.D;
-                acc += (unsigned long long)(p.bstart[base + p.D] - p.bstart[base + f]);
+            for (int s = 0; s < p.nsub; ++s) {
+                const int cnt = p.hist_cnt[((long long)slot * p.nsub + s) * p.cstride];
+                const int32_t *list = p.hist + (long long)slot * p.nsub * p.subcap + (long long)s * p.subcap;
+                for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+                    const long long base = (long long)list[k] * p.D;
+                    acc += (unsigned long long)(p.bstart[base + p.D] - p.bstart[base + f]);
+                }
             }
         }
     }

@@ -87,15 +87,21 @@ struct DGroup {
     int tile_begin; /* first tile index of this group in the concatenated tile space */
     R *xi;          /* windowed form: noise ring [xi_slots][n], draw of step t in slot t mod xi_slots (or nullptr) */
     int xi_slots;
+    /* the spike list of a ring slot is split into nsub sub-lists of subcap
+       entries: neuron i goes to sub-list (i / 256) mod nsub, so that the blocks
+       of a large group reserve list space on nsub counters, not one.  List of
+       (slot, s): hist[slot*nsub*subcap + s*subcap ..], count at
+       hist_cnt[(slot*nsub + s) * HS_CNT_STRIDE] */
+    int nsub, subcap;
 };
 
 template <typename R>
 struct DPath {
     int src, tgt, var;
     int n_src, dmin, D, Q, wmode;
-    const int32_t *hist;     /* source group's spike ring */
-    const int32_t *hist_cnt; /* count of ring slot s at hist_cnt[s * cstride] */
-    int cstride;
+    const int32_t *hist;     /* source group's spike ring (sub-lists as in DGroup) */
+    const int32_t *hist_cnt; /* count of (ring slot, sub-list) at hist_cnt[(slot*nsub + s) * cstride] */
+    int cstride, nsub, subcap;
     const int32_t *bstart;
     const int32_t *post;
     const R *w;

@@ -293,6 +293,8 @@ static int hs_deliver_t(int n_src, int dmin, int D, int Q, const int32_t *hist, 
     p.hist = hist;
     p.hist_cnt = hist_cnt;
     p.cstride = 1;
+    p.nsub = 1;
+    p.subcap = n_src;
     p.bstart = bstart;
     p.post = post;
     p.w = (const R *)w;
@@ -345,6 +347,7 @@ struct HGroup {
     std::vector<int> stmons;
     void *xi = nullptr; /* windowed form: noise ring [xi_slots][n], drawn one window ahead */
     int xi_slots = 0;
+    int nsub = 1, subcap = 0; /* spike sub-lists per ring slot (see DGroup) */
 };
 
 struct HPath {
@@ -751,8 +754,8 @@ extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
     if (key == 6 && (value < 64 || value > 1024 || value % 64))
         return hs_fail(HS_ERR_ARG, "gather block size must be a multiple of 64 in [64,1024]");
     if (key == 5 && (value < 0 || value > 2)) return hs_fail(HS_ERR_ARG, "delivery mode: 0 auto, 1 scatter, 2 gather");
-    if (key == 0 && (value < 64 || value > 256 || value % 64))
-        return hs_fail(HS_ERR_ARG, "update block size must be 64, 128, 192 or 256");
+    if (key == 0 && value != 64 && value != 128 && value != 256)
+        return hs_fail(HS_ERR_ARG, "update block size must be 64, 128 or 256 (a divisor of the 256-neuron chunk)");
     if ((key == 1 || key == 3) && (value < 0 || value > (1 << 20))) return hs_fail(HS_ERR_ARG, "bad grid size");
     if (key == 2 && (value < 0 || value > 100000)) return hs_fail(HS_ERR_ARG, "bad graph chunk");
     if (key == 4 && (value < 0 || value > HS_MAX_DELAY_SLOTS)) return hs_fail(HS_ERR_ARG, "bad window");
@@ -803,12 +806,27 @@ static int hs_build(hs_net *net)
     }
     const int W = net->window;
     for (HGroup &g : net->g) {
+        const int gid = (int)(&g - &net->g[0]);
         int dmax = 0;
         for (const HPath &P : net->p)
-            if (P.src == (int)(&g - &net->g[0])) dmax = std::max(dmax, P.dmax);
+            if (P.src == gid) dmax = std::max(dmax, P.dmax);
         g.Q = dmax + 2 * W; /* see the ring invariants in hs_engine.hip */
-        HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.Q * g.n, (void **)&g.hist, false));
-        HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.Q * HS_CNT_STRIDE, (void **)&g.hist_cnt, true));
+        /* large groups split each step's spike list into sub-lists so that
+           thousands of blocks do not reserve list space on one counter; the
+           delivery prefix tables bound the (step, delay, sub-list) items */
+        const int nchunks = (g.n + 255) / 256;
+        int nsub = 1;
+        if (nchunks >= 512)
+            while (nsub < 32 && nsub * 2 * 128 <= nchunks) nsub *= 2;
+        for (const HPath &P : net->p) {
+            if (P.src != gid) continue;
+            const int items = P.dmode == HS_DELIVER_GATHER ? P.D - 1 + W : W * P.D;
+            while (nsub > 1 && items * nsub > HS_MAX_DELAY_SLOTS) nsub /= 2;
+        }
+        g.nsub = nsub;
+        g.subcap = (nchunks + nsub - 1) / nsub * 256;
+        HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.Q * nsub * g.subcap, (void **)&g.hist, false));
+        HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.Q * nsub * HS_CNT_STRIDE, (void **)&g.hist_cnt, true));
     }
     if (net->engine == HS_ENGINE_FUSED) {
         for (const HPath &P : net->p) {
@@ -863,6 +881,8 @@ static int hs_upload_desc_t(hs_net *net)
         d.last = g.last;
         d.hist = g.hist;
         d.hist_cnt = g.hist_cnt;
+        d.nsub = g.nsub;
+        d.subcap = g.subcap;
         for (int j = 0; j < 16; ++j) d.k[j] = (R)g.k[j];
         d.vth = (R)g.params[g.mi.i_vth];
         d.vr = g.mi.i_vr >= 0 ? (R)g.params[g.mi.i_vr] : (R)0;
@@ -907,6 +927,8 @@ static int hs_upload_desc_t(hs_net *net)
         d.hist = gs.hist;
         d.hist_cnt = gs.hist_cnt;
         d.cstride = HS_CNT_STRIDE;
+        d.nsub = gs.nsub;
+        d.subcap = gs.subcap;
         d.bstart = P.bstart;
         d.post = P.post;
         d.w = (const R *)P.w;

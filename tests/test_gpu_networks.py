@@ -182,3 +182,22 @@ def test_reference_effects_cancel_gpu(hs, engine):
     assert len(net.spikes(sm[0])[0]) == 0
     assert np.all(net.get_state(1, 0) == 0.0)
     assert net.stats()["events"] == 2 * 15360 * (steps - 1)
+
+
+@pytest.mark.parametrize("engine,sem,opts", [("plain", 0, {}), ("fused", 1, {}), ("fused", 1, {9: 1}),
+                                             ("fused", 1, {5: 2}), ("plain", 0, {5: 2})])
+def test_large_group_sub_lists(hs, oracle_lib, engine, sem, opts):
+    """A group of 140,000 neurons (547 chunks of 256) splits each step's spike
+    list into sub-lists; every delivery form still gives the oracle's bits."""
+    spec = nw.brunel(N=140_000, K=20, seed=31, monitors=True)
+    for prec in (32, 64):
+        _compare(spec, prec, engine, sem, None, oracle_lib, options=opts, chunks=[150, 101])
+
+
+def test_batch_sub_lists(hs, oracle_lib):
+    """40 CUBA networks of up to 16,000 neurons in one index space (about 700
+    chunks): sub-listed spike rings in the single-launch fused form."""
+    spec = nw.batch_cuba(n_nets=40, seed=17)
+    assert spec.n_neurons >= 512 * 256
+    _compare(spec, 32, "fused", 1, 400, oracle_lib)
+    _compare(spec, 64, "plain", 0, 400, oracle_lib)
