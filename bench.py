@@ -245,6 +245,29 @@ def hines_bench(n_trees, steps, seed):
     return out
 
 
+def connect_bench(args, repeats=3):
+    """Device build of the same connectivity (connect(p=K/N), delays U[0.5, 5] ms,
+    per-synapse FP32 weights): wall time of hs_net_connect_random (count pass,
+    prefix sum, write pass, two scalar readbacks), best of `repeats` fresh nets."""
+    from brian2cuda_amd import networks as nw
+    from brian2cuda_amd import runtime as rt
+    spec = nw.brunel(N=args.N, K=args.K, seed=args.seed, device_connect=True)
+    r, g = spec.paths[0].random, spec.groups[0]
+    best, nsyn, nslots = None, 0, 0
+    for _ in range(repeats):
+        net = rt.Net(32, "fused", spec.dt, spec.seed)
+        net.add_group(g.model, g.n, g.params)
+        t0 = time.perf_counter()
+        pid = net.connect_random(0, 0, 0, r.p, r.pre, r.post, r.autapses, r.seed, r.stream, r.delay, r.wmode, r.w)
+        dt_s = time.perf_counter() - t0
+        pi = net.pathway_info(pid)
+        nsyn, nslots = pi["nsyn"], pi["D"]
+        net.close()
+        best = dt_s if best is None else min(best, dt_s)
+    return {"device_s": round(best, 5), "synapses": int(nsyn), "synapses_per_s": round(nsyn / best, 1),
+            "bytes_written": int(nsyn * 8 + 4 * (args.N * nslots + 1))}
+
+
 def cpu_baseline(spec, skip_steps, steps):
     """Oracle (cpp_standalone semantics, FP64, 1 core) on a window of steps."""
     import oracle
@@ -279,6 +302,7 @@ def main():
     ap.add_argument("--hines-trees", type=int, default=2048, help="trees of the multicompartment batch")
     ap.add_argument("--hines-steps", type=int, default=2000, help="steps of the multicompartment run (0: skip)")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-connect", action="store_true", help="skip the device connectivity-build timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the engine has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -293,7 +317,8 @@ def main():
     log("device", info)
     t0 = time.time()
     spec = make_spec(args.workload, args)
-    log(f"built {spec.name}: N={spec.n_neurons} S={spec.n_synapses} in {time.time() - t0:.1f}s")
+    spec_s = time.time() - t0
+    log(f"built {spec.name}: N={spec.n_neurons} S={spec.n_synapses} in {spec_s:.1f}s")
     D = max((p.delay_steps.max() - p.delay_steps.min() + 1) if p.delay_steps is not None else 1 for p in spec.paths)
 
     fp32 = run_form(spec, 32, "fused", args.steps, args.warmup)
@@ -331,6 +356,13 @@ def main():
     if args.drift_steps > 0 and not args.no_fp64 and args.workload == "brunel":
         dr = drift(spec, args.drift_steps)
         log("drift", dr)
+
+    conn = None
+    if args.workload == "brunel" and not args.no_connect:
+        conn = connect_bench(args)
+        conn["host_numpy_generate_s"] = round(spec_s, 4)
+        conn["host_bundle_upload_s"] = round(fp32["build_s"], 4)
+        log("connectivity build", conn)
 
     hines = None
     if args.hines_steps > 0:
@@ -371,6 +403,7 @@ def main():
         "drift_fp32_batched_vs_fp64_plain": dr,
         "roofline": roofline,
         "multicompartment": hines,
+        "connectivity_build": conn,
         "cpu_baseline": cpu,
         "device": info,
     }

@@ -81,6 +81,12 @@ SIGNATURES = {
     "hs_net_run": (C.c_int, [P, C.c_int64]),
     "hs_net_stats": (C.c_int, [P, C.POINTER(HsStats)]),
     "hs_net_kernel_time": (C.c_int, [P, C.c_int, DP, I64P]),
+    "hs_net_connect_random": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_double, C.c_int, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_int,
+                                        C.c_double, C.c_double, C.POINTER(C.c_int)]),
+    "hs_net_pathway_info": (C.c_int, [P, C.c_int, I64P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
+    "hs_net_get_synapses": (C.c_int, [P, C.c_int, I32P, I32P, I32P, DP, C.c_int64]),
     "hs_net_mon_count": (C.c_int, [P, C.c_int, I64P]),
     "hs_net_get_spikes": (C.c_int, [P, C.c_int, I32P, I64P, C.c_int64]),
     "hs_net_get_state_monitor": (C.c_int, [P, C.c_int, DP, C.c_int64]),

@@ -121,6 +121,50 @@ struct DPath {
     int nnear;
 };
 
+/*
+ * Random connectivity built on the device (hs_connect.hip).  Row i (a
+ * presynaptic neuron in [pre_lo, pre_hi)) selects targets j in [post_lo, post_hi)
+ * with probability p; draw k of row i comes from the Philox counter
+ * (i, k, HS_CONN_CTR | tag, stream) under key (seed_lo, seed_hi):
+ *   HS_CONN_TAG_SKIP  the geometric skips of p < 0.25 (two draws per counter: words 0-1, then 2-3)
+ *   HS_CONN_TAG_PAIR  one test per pair for p >= 0.25 (counter k = (j - post_lo) / 2, same halves)
+ *   HS_CONN_TAG_SYN   per synapse (i, j), counter k = j: delay uniform from words 0-1,
+ *                     weight uniform from words 2-3
+ * The high half of word 2 marks the counter as a connectivity draw, so it never
+ * meets a noise draw (index, step_lo, step_hi, stream) of a run shorter than 2^62 steps.
+ */
+#define HS_CONN_CTR 0x434E0000u
+#define HS_CONN_TAG_SKIP 0u
+#define HS_CONN_TAG_SYN 1u
+#define HS_CONN_TAG_PAIR 2u
+#define HS_CONN_LDS_SLOTS 128 /* delay slots kept per thread in LDS (D above: tables in global memory) */
+
+struct HsConnSpec {
+    int pre_lo, pre_hi, post_lo, post_hi;
+    int autapses;    /* 0: j == i is dropped (source and target are the same group) */
+    int jump;        /* p < 0.25: geometric skips; else one draw per pair */
+    double p;
+    double pconst;   /* 1 / log(1 - p) (jump form) */
+    unsigned long long seed;
+    unsigned int stream;
+    double dlo, dspan, dt; /* delay = (int)((dlo + u*dspan)/dt + 0.5) steps */
+    int dmin, D;           /* delay slot = delay - dmin, in [0, D) */
+    double wlo, wspan;     /* weight = wlo + u*wspan */
+};
+
+/* count pass: cnt[i*D + slot] (int32, zeroed by the caller) for rows [pre_lo, pre_hi);
+   *err becomes nonzero if a slot falls outside [0, D) */
+hipError_t hs_launch_conn_count(const HsConnSpec &c, int32_t *cnt, int *err, hipStream_t s);
+/* write pass: post[] and w[] (R = double or float by prec, nullptr: no weights) at
+   bstart[i*D+slot] + running count; cursor: int32[n_src*D] holding a copy of
+   bstart when D > HS_CONN_LDS_SLOTS (else unused) */
+hipError_t hs_launch_conn_write(int prec, const HsConnSpec &c, const int32_t *bstart, int32_t *cursor,
+                                int32_t *post, void *w, int *err, hipStream_t s);
+/* exclusive prefix sum of n int32 counts: out[0..n] (out[n] = total, also at
+   bsum[hs_scan_blocks(n)] as int64); bsum holds hs_scan_blocks(n) + 1 entries */
+int hs_scan_blocks(long long n);
+hipError_t hs_launch_scan(const int32_t *in, long long n, long long *bsum, int32_t *out, hipStream_t s);
+
 /* runtime services for the translation units (hs_runtime.hip) */
 hipStream_t hs_rt_stream();
 int hs_rt_cus();

@@ -693,6 +693,199 @@ extern "C" int hs_net_add_pathway(hs_net *net, int src, int tgt, int var, int64_
     return HS_OK;
 }
 
+/*
+ * Random connectivity on the device (kernels in hs_connect.hip): count per
+ * (pre, delay slot) -> prefix sum -> bstart -> write post/w in bundle order.
+ * The host touches two scalars (the total and an error flag); temporaries are
+ * freed before return.
+ */
+static int hs_connect_q(double d, double dt, int *q)
+{
+    const double x = d / dt + 0.5;
+    if (!(x >= 0.0) || x >= 2147483647.0) return hs_fail(HS_ERR_ARG, "delay %g s out of range", d);
+    *q = (int)x;
+    return HS_OK;
+}
+
+extern "C" int hs_net_connect_random(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo,
+                                     int post_hi, double p, int autapses, uint64_t seed, uint32_t stream,
+                                     double delay_lo, double delay_hi, int wmode, double w_lo, double w_hi, int *pid)
+{
+    if (!net) return hs_fail(HS_ERR_ARG, "null net");
+    if (net->built) return hs_fail(HS_ERR_STATE, "pathways must be added before the first run");
+    if (src < 0 || src >= (int)net->g.size() || tgt < 0 || tgt >= (int)net->g.size())
+        return hs_fail(HS_ERR_ARG, "bad source/target group");
+    HGroup &gs = net->g[src];
+    HGroup &gt = net->g[tgt];
+    if (var < 0 || var >= gt.mi.nvars) return hs_fail(HS_ERR_ARG, "bad target variable %d", var);
+    if (pre_lo < 0 || pre_lo > pre_hi || pre_hi > gs.n) return hs_fail(HS_ERR_ARG, "bad source range");
+    if (post_lo < 0 || post_lo > post_hi || post_hi > gt.n) return hs_fail(HS_ERR_ARG, "bad target range");
+    if (!(p >= 0.0) || !(p <= 1.0)) return hs_fail(HS_ERR_ARG, "connection probability %g outside [0, 1]", p);
+    if (!(delay_lo >= 0.0) || !(delay_hi >= delay_lo) || delay_hi > 1e300)
+        return hs_fail(HS_ERR_ARG, "bad delay range [%g, %g]", delay_lo, delay_hi);
+    if (wmode != HS_W_SCALAR && wmode != HS_W_SYNAPSE)
+        return hs_fail(HS_ERR_UNSUPPORTED, "random connectivity takes a scalar or per-synapse weight");
+    if (!(w_hi >= w_lo) || w_lo < -1e300 || w_hi > 1e300) return hs_fail(HS_ERR_ARG, "bad weight range");
+    if (net->opt[5] == HS_DELIVER_GATHER)
+        return hs_fail(HS_ERR_UNSUPPORTED, "random connectivity builds scatter bundles (delivery option 1)");
+    HsConnSpec c = {};
+    c.pre_lo = pre_lo;
+    c.pre_hi = pre_hi;
+    c.post_lo = post_lo;
+    c.post_hi = post_hi;
+    c.autapses = (autapses || src != tgt) ? 1 : 0;
+    c.p = p;
+    c.jump = p < 0.25 ? 1 : 0;
+    c.pconst = c.jump && p > 0.0 ? 1.0 / hs_log(1.0 - p) : 0.0;
+    c.seed = seed;
+    c.stream = stream;
+    c.dt = net->dt;
+    c.dlo = delay_lo;
+    c.dspan = delay_hi - delay_lo;
+    c.wlo = w_lo;
+    c.wspan = wmode == HS_W_SYNAPSE ? w_hi - w_lo : 0.0;
+    int dmin, dmax;
+    HS_TRY(hs_connect_q(c.dlo, c.dt, &dmin));
+    HS_TRY(hs_connect_q(c.dlo + 1.0 * c.dspan, c.dt, &dmax)); /* the slot formula at u = 1: a bound */
+    const int D = dmax - dmin + 1;
+    if (D > HS_MAX_DELAY_SLOTS) return hs_fail(HS_ERR_UNSUPPORTED, "delay range %d steps > %d", D, HS_MAX_DELAY_SLOTS);
+    c.dmin = dmin;
+    c.D = D;
+    if (p == 0.0) c.pre_hi = c.pre_lo; /* no rows: the reference skips p == 0 (synapses_create_generator.cu:315) */
+
+    HPath P;
+    P.src = src;
+    P.tgt = tgt;
+    P.var = var;
+    P.wmode = wmode;
+    P.wscalar = w_lo;
+    P.dmin = dmin;
+    P.dmax = dmax;
+    P.D = D;
+    P.dmode = HS_DELIVER_SCATTER;
+    const long long nb = (long long)gs.n * D;
+    const int nblk = hs_scan_blocks(nb);
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)(nb + 1), (void **)&P.bstart, false));
+    int32_t *cnt = nullptr;
+    long long *bsum = nullptr;
+    int *err = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(g_rt.stream);
+        if (cnt) (void)hipFree(cnt);
+        if (bsum) (void)hipFree(bsum);
+        if (err) (void)hipFree(err);
+    };
+    auto fail_hip = [&](hipError_t e, const char *what) {
+        release();
+        return hs_fail(HS_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc((void **)&cnt, sizeof(int32_t) * (size_t)(nb ? nb : 1))) != hipSuccess ||
+        (e = hipMalloc((void **)&bsum, sizeof(long long) * (size_t)(nblk + 1))) != hipSuccess ||
+        (e = hipMalloc((void **)&err, sizeof(int))) != hipSuccess) {
+        (void)hipGetLastError();
+        release();
+        return hs_fail(HS_ERR_NOMEM, "connectivity temporaries (%lld counts): %s", nb, hipGetErrorString(e));
+    }
+    if ((e = hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)(nb ? nb : 1), g_rt.stream)) != hipSuccess ||
+        (e = hipMemsetAsync(err, 0, sizeof(int), g_rt.stream)) != hipSuccess)
+        return fail_hip(e, "clear");
+    if ((e = hs_launch_conn_count(c, cnt, err, g_rt.stream)) != hipSuccess) return fail_hip(e, "count pass");
+    if ((e = hs_launch_scan(cnt, nb, bsum, P.bstart, g_rt.stream)) != hipSuccess) return fail_hip(e, "prefix sum");
+    long long total = 0;
+    int herr = 0;
+    if ((e = hipMemcpyAsync(&total, bsum + nblk, sizeof(long long), hipMemcpyDeviceToHost, g_rt.stream)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, g_rt.stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(g_rt.stream)) != hipSuccess)
+        return fail_hip(e, "count readback");
+    if (herr) {
+        release();
+        return hs_fail(HS_ERR_HIP, "count pass: delay slot outside [0, %d)", D);
+    }
+    if (total >= 0x7fffffffLL) {
+        release();
+        return hs_fail(HS_ERR_UNSUPPORTED, "%lld synapses: more than 2^31 - 1 in one pathway", total);
+    }
+    P.nsyn = total;
+    const size_t rs = hs_rsize(net);
+    int r;
+    if ((r = hs_dalloc(net, sizeof(int32_t) * (size_t)(total ? total : 1), (void **)&P.post, false)) != HS_OK) {
+        release();
+        return r;
+    }
+    if (wmode == HS_W_SYNAPSE &&
+        (r = hs_dalloc(net, rs * (size_t)(total ? total : 1), &P.w, false)) != HS_OK) {
+        release();
+        return r;
+    }
+    if (D > HS_CONN_LDS_SLOTS && nb &&
+        (e = hipMemcpyAsync(cnt, P.bstart, sizeof(int32_t) * (size_t)nb, hipMemcpyDeviceToDevice, g_rt.stream)) !=
+            hipSuccess)
+        return fail_hip(e, "cursor copy");
+    if ((e = hs_launch_conn_write(net->prec == HS_F64 ? 64 : 32, c, P.bstart, cnt, P.post,
+                                  wmode == HS_W_SYNAPSE ? P.w : nullptr, err, g_rt.stream)) != hipSuccess)
+        return fail_hip(e, "write pass");
+    if ((e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, g_rt.stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(g_rt.stream)) != hipSuccess)
+        return fail_hip(e, "write readback");
+    release();
+    if (herr) return hs_fail(HS_ERR_HIP, "write pass: count and write passes disagree (code %d)", herr);
+    net->n_scatter++;
+    net->p.push_back(P);
+    net->desc_dirty = true;
+    if (pid) *pid = (int)net->p.size() - 1;
+    return HS_OK;
+}
+
+extern "C" int hs_net_pathway_info(hs_net *net, int pid, int64_t *nsyn, int *dmin, int *D, int *dmode)
+{
+    if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");
+    const HPath &P = net->p[pid];
+    if (nsyn) *nsyn = P.nsyn;
+    if (dmin) *dmin = P.dmin;
+    if (D) *D = P.D;
+    if (dmode) *dmode = P.dmode;
+    return HS_OK;
+}
+
+/* the synapses of a scatter-mode pathway in bundle order: by (pre, delay), creation order within a bundle */
+extern "C" int hs_net_get_synapses(hs_net *net, int pid, int32_t *pre, int32_t *post, int32_t *delay_steps,
+                                   double *w, int64_t cap)
+{
+    if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");
+    const HPath &P = net->p[pid];
+    if (P.dmode != HS_DELIVER_SCATTER) return hs_fail(HS_ERR_UNSUPPORTED, "synapse readback of gather-mode pathways");
+    if (cap < P.nsyn) return hs_fail(HS_ERR_ARG, "capacity %lld < %lld synapses", (long long)cap, (long long)P.nsyn);
+    const int n_src = net->g[P.src].n;
+    const size_t nb = (size_t)n_src * P.D;
+    std::vector<int32_t> bs(nb + 1), po((size_t)(P.nsyn ? P.nsyn : 1));
+    HS_HIP(hipMemcpyAsync(bs.data(), P.bstart, sizeof(int32_t) * (nb + 1), hipMemcpyDeviceToHost, g_rt.stream));
+    if (P.nsyn)
+        HS_HIP(hipMemcpyAsync(po.data(), P.post, sizeof(int32_t) * (size_t)P.nsyn, hipMemcpyDeviceToHost,
+                              g_rt.stream));
+    const size_t rs = hs_rsize(net);
+    std::vector<unsigned char> wv;
+    if (w && P.wmode != HS_W_SCALAR) {
+        const size_t nw = P.wmode == HS_W_SYNAPSE ? (size_t)P.nsyn : (size_t)n_src;
+        wv.resize(rs * (nw ? nw : 1));
+        if (nw) HS_HIP(hipMemcpyAsync(wv.data(), P.w, rs * nw, hipMemcpyDeviceToHost, g_rt.stream));
+    }
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    auto wat = [&](size_t k) {
+        return net->prec == HS_F64 ? ((const double *)wv.data())[k] : (double)((const float *)wv.data())[k];
+    };
+    for (int i = 0; i < n_src; ++i)
+        for (int d = 0; d < P.D; ++d)
+            for (int32_t s = bs[(size_t)i * P.D + d]; s < bs[(size_t)i * P.D + d + 1]; ++s) {
+                if (pre) pre[s] = i;
+                if (post) post[s] = po[s];
+                if (delay_steps) delay_steps[s] = P.dmin + d;
+                if (w) w[s] = P.wmode == HS_W_SCALAR ? P.wscalar : P.wmode == HS_W_SYNAPSE ? wat((size_t)s) : wat(i);
+            }
+    return HS_OK;
+}
+
 extern "C" int hs_net_add_spike_monitor(hs_net *net, int gid, int64_t capacity, int *mid)
 {
     if (!net || gid < 0 || gid >= (int)net->g.size()) return hs_fail(HS_ERR_ARG, "bad group");

@@ -31,21 +31,40 @@ class GroupSpec:
 
 
 @dataclass
+class RandomConnect:
+    """Synapses.connect(p=p) generated where the pathway is built (on the device
+    by the runtime, by the oracle's restatement for the oracle), with delays
+    uniform in delay=(lo, hi) seconds and weights w=(lo, hi) (wmode SYNAPSE:
+    uniform per synapse; SCALAR: w[0])."""
+    p: float
+    pre: tuple
+    post: tuple
+    seed: int
+    stream: int = 0
+    autapses: bool = True
+    delay: tuple = (0.0, 0.0)
+    wmode: int = HS_W_SCALAR
+    w: tuple = (0.0, 0.0)
+
+
+@dataclass
 class PathSpec:
     src: int
     tgt: int
     var: int
-    pre: np.ndarray
-    post: np.ndarray
+    pre: Optional[np.ndarray]
+    post: Optional[np.ndarray]
     delay_steps: Optional[np.ndarray] = None
     scalar_delay: int = 0
     wmode: int = HS_W_SCALAR
     weights: Optional[np.ndarray] = None
     wscalar: float = 0.0
+    random: Optional[RandomConnect] = None
 
     @property
     def nsyn(self):
-        return int(self.pre.size)
+        """synapse count (None for a random pathway: known once built)"""
+        return None if self.pre is None else int(self.pre.size)
 
 
 @dataclass
@@ -65,7 +84,8 @@ class NetSpec:
 
     @property
     def n_synapses(self):
-        return sum(p.nsyn for p in self.paths)
+        n = [p.nsyn for p in self.paths]
+        return None if None in n else sum(n)
 
 
 def quantise_delay(d, dt):
@@ -167,13 +187,23 @@ def brunel_params(tau=20 * ms, mu=25 * mV, sigma=1 * mV, theta=20 * mV, Vr=10 * 
 
 
 def brunel(N=100_000, K=1000, seed=1234, dt=0.1 * ms, J=-0.1 * mV, dlo=0.5 * ms, dhi=5.0 * ms,
-           per_synapse_weights=True, monitors=False, sigma=1 * mV, mu=25 * mV):
+           per_synapse_weights=True, monitors=False, sigma=1 * mV, mu=25 * mV, device_connect=False):
     """C3: sparse inhibitory LIF network with noise and heterogeneous delays.
 
     connect(p=K/N) (examples/brunelhakim.py:144-156), delays U[dlo, dhi] quantised
-    to dt, on_pre 'V += -|J|' with a per-synapse weight array."""
-    rng = np.random.default_rng(seed)
+    to dt, on_pre 'V += -|J|' with a per-synapse weight array.  device_connect:
+    the synapses are generated where the network is built (RandomConnect)
+    instead of by numpy here."""
     g = GroupSpec(HS_MODEL_LIF_NOISE, N, brunel_params(sigma=sigma, mu=mu), {0: 10 * mV})
+    if device_connect:
+        rc = RandomConnect(K / N, (0, N), (0, N), seed, stream=1, delay=(dlo, dhi),
+                           wmode=HS_W_SYNAPSE if per_synapse_weights else HS_W_SCALAR, w=(J, J))
+        ps = PathSpec(0, 0, 0, None, None, wmode=rc.wmode, wscalar=J, random=rc)
+        spec = NetSpec("brunel", dt, seed, [g], [ps], info={"N": N, "K": K})
+        if monitors:
+            spec.spike_monitors = [0]
+        return spec
+    rng = np.random.default_rng(seed)
     pre, post = bernoulli_connect(rng, N, N, K / N)
     d = quantise_delay(dlo + (dhi - dlo) * rng.random(pre.size), dt)
     if per_synapse_weights:
@@ -244,8 +274,13 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
         for var, val in g.init.items():
             net.set_state(gid, var, val)
     for p in spec.paths:
-        net.add_pathway(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode, p.weights,
-                        p.wscalar)
+        if p.random is not None:
+            r = p.random
+            net.connect_random(p.src, p.tgt, p.var, r.p, r.pre, r.post, r.autapses, r.seed, r.stream, r.delay,
+                               r.wmode, r.w)
+        else:
+            net.add_pathway(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode, p.weights,
+                            p.wscalar)
     smons = [net.add_spike_monitor(g, spike_capacity) for g in spec.spike_monitors]
     stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
     return net, smons, stmons

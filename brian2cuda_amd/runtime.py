@@ -214,6 +214,38 @@ class Net:
              (_f64(w) if w is not None else None), float(wscalar), C.byref(pid))
         return pid.value
 
+    def connect_random(self, src, tgt, var, p, pre=None, post=None, autapses=True, seed=None, stream=0,
+                       delay=0.0, wmode=HS_W_SCALAR, w=0.0):
+        """Synapses.connect(p=p) generated on the device, with delays uniform in
+        delay=(lo, hi) seconds (or one value) and weights w (scalar, or (lo, hi)
+        uniform per synapse with wmode HS_W_SYNAPSE).  pre/post: (lo, hi) index
+        ranges inside the groups (default: whole group).  Returns the pathway id."""
+        n_src, n_tgt = self.groups[src][1], self.groups[tgt][1]
+        pre = (0, n_src) if pre is None else pre
+        post = (0, n_tgt) if post is None else post
+        dlo, dhi = (delay, delay) if np.isscalar(delay) else delay
+        wlo, whi = (w, w) if np.isscalar(w) else w
+        pid = C.c_int()
+        call("hs_net_connect_random", self.h, src, tgt, var, int(pre[0]), int(pre[1]), int(post[0]), int(post[1]),
+             float(p), int(bool(autapses)), int(self.seed if seed is None else seed), int(stream), float(dlo),
+             float(dhi), int(wmode), float(wlo), float(whi), C.byref(pid))
+        return pid.value
+
+    def pathway_info(self, pid):
+        n = C.c_int64()
+        dmin, D, mode = C.c_int(), C.c_int(), C.c_int()
+        call("hs_net_pathway_info", self.h, pid, C.byref(n), C.byref(dmin), C.byref(D), C.byref(mode))
+        return {"nsyn": n.value, "dmin": dmin.value, "D": D.value, "delivery": mode.value}
+
+    def synapses(self, pid):
+        """(pre, post, delay_steps, w) of a scatter-mode pathway in bundle order:
+        by (pre, delay), creation order within a bundle."""
+        n = self.pathway_info(pid)["nsyn"]
+        pre, post, d = (np.empty(n, np.int32) for _ in range(3))
+        w = np.empty(n, np.float64)
+        call("hs_net_get_synapses", self.h, pid, _i32(pre), _i32(post), _i32(d), _f64(w), n)
+        return pre, post, d, w
+
     def add_spike_monitor(self, gid, capacity=1 << 20):
         mid = C.c_int()
         call("hs_net_add_spike_monitor", self.h, gid, int(capacity), C.byref(mid))

@@ -153,6 +153,37 @@ int hs_net_add_pathway(hs_net *net, int src, int tgt, int var, int64_t nsyn, con
                        const int32_t *post, const int32_t *delay_steps, int scalar_delay, int wmode,
                        const double *weights, double wscalar, int *pid);
 
+/*
+ * Add a pathway with random connectivity generated on the device
+ * (Synapses.connect(p=p) followed by delay = 'delay_lo + (delay_hi-delay_lo)*rand()').
+ * Every source neuron i in [pre_lo, pre_hi) connects to each target j in
+ * [post_lo, post_hi) with probability p: geometric skips for p < 0.25, one draw
+ * per pair otherwise -- the two algorithms of the reference's generator
+ * (synapses_create_generator.cu:315-333).  autapses = 0 drops j == i when src ==
+ * tgt (connect(condition='i != j', p=p)).  Draws: Philox-4x32-10 keyed by
+ * (seed, stream, i, draw), so the synapses are a pure function of the arguments.
+ * Delays in seconds, quantised (int)(d/dt + 0.5) (synapses_push_spikes.cu:375);
+ * the bundle table spans the quantised bounds [q(delay_lo), q(delay_hi)].
+ * wmode HS_W_SCALAR: weight w_lo; HS_W_SYNAPSE: per-synapse uniform in
+ * [w_lo, w_hi) (w_lo == w_hi: a constant array).  Bundles are written on the
+ * device (scatter delivery only): no synapse array crosses PCIe.
+ * Replaces the host generation + copies of synapses_create_generator.cu:164-465
+ * and the bundle construction of _before_run_<push_spikes> (synapses_push_spikes.cu:171-993).
+ */
+int hs_net_connect_random(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo, int post_hi,
+                          double p, int autapses, uint64_t seed, uint32_t stream, double delay_lo, double delay_hi,
+                          int wmode, double w_lo, double w_hi, int *pid);
+/* synapse count, first delay slot (steps), delay slots and delivery mode of a pathway */
+int hs_net_pathway_info(hs_net *net, int pid, int64_t *nsyn, int *dmin, int *D, int *dmode);
+/*
+ * The synapses of a scatter-mode pathway, host arrays of cap >= nsyn entries (any
+ * may be NULL), in bundle order: by (pre, delay), creation order within a bundle.
+ * Replaces the write-out of the Synapses' dynamic arrays (_synaptic_pre,
+ * _synaptic_post, delay, w) in _write_arrays (brian2cuda/templates/objects.cu:550-563).
+ */
+int hs_net_get_synapses(hs_net *net, int pid, int32_t *pre, int32_t *post, int32_t *delay_steps, double *w,
+                        int64_t cap);
+
 /* SpikeMonitor on a group: capacity = device records kept (i, step). */
 int hs_net_add_spike_monitor(hs_net *net, int gid, int64_t capacity, int *mid);
 /* StateMonitor (when='start') of var over neurons [start, stop). */

@@ -41,6 +41,9 @@ def lib():
         "ref_randn_f32": (C.c_float, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64]),
         "ref_rand": (C.c_double, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64]),
         "ref_quantize_delay": (C.c_int, [C.c_double, C.c_double]),
+        "ref_connect_random": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64,
+                                           C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                           I32P, I32P, I32P, DP, C.c_int64]),
         "ref_timestep": (C.c_longlong, [C.c_double, C.c_double]),
         "ref_exp": (C.c_double, [C.c_double]),
         "ref_expf": (C.c_float, [C.c_float]),
@@ -91,6 +94,24 @@ def philox4x32_10(ctr, key):
     o = (C.c_uint32 * 4)()
     lib().ref_philox4x32_10(c, k, o)
     return tuple(o)
+
+
+def connect_random(pre_range, post_range, p, skip_self, seed, stream, delay, dt, weight):
+    """Synapses of Synapses.connect(p) with drawn delays and weights, creation
+    order (pre ascending, post ascending): dict of pre, post, delay_steps (int32)
+    and w (float64).  delay, weight: (lo, hi) ranges."""
+    L = lib()
+    args = (int(pre_range[0]), int(pre_range[1]), int(post_range[0]), int(post_range[1]), float(p),
+            int(bool(skip_self)), int(seed), int(stream), float(delay[0]), float(delay[1]), float(dt),
+            float(weight[0]), float(weight[1]))
+    n = L.ref_connect_random(*args, None, None, None, None, 0)
+    out = {"pre": np.empty(n, np.int32), "post": np.empty(n, np.int32), "delay_steps": np.empty(n, np.int32),
+           "w": np.empty(n, np.float64)}
+    I32P, DP = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    m = L.ref_connect_random(*args, out["pre"].ctypes.data_as(I32P), out["post"].ctypes.data_as(I32P),
+                             out["delay_steps"].ctypes.data_as(I32P), out["w"].ctypes.data_as(DP), n)
+    assert m == n
+    return out
 
 
 def randn(seed, stream, index, step, precision=64):
@@ -218,6 +239,15 @@ class RefNet:
         return out[: rows * width].reshape(rows, width)
 
 
+def random_pathway(spec, path, rows=None):
+    """The synapses of a RandomConnect pathway (creation order); rows: a
+    sub-range of source rows (rows are generated independently of each other)."""
+    r = path.random
+    pre = r.pre if rows is None else rows
+    return connect_random(pre, r.post, r.p, (not r.autapses) and path.src == path.tgt, r.seed, r.stream, r.delay,
+                          spec.dt, r.w)
+
+
 def build_ref(spec, precision=64, semantics=SEQ):
     """Instantiate a brian2cuda_amd.networks.NetSpec on the oracle."""
     net = RefNet(precision, semantics, spec.dt, spec.seed)
@@ -226,8 +256,13 @@ def build_ref(spec, precision=64, semantics=SEQ):
         for var, val in g.init.items():
             net.set_state(gid, var, val)
     for p in spec.paths:
-        net.add_pathway(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode, p.weights,
-                        p.wscalar)
+        if p.random is not None:
+            syn = random_pathway(spec, p)
+            net.add_pathway(p.src, p.tgt, p.var, syn["pre"], syn["post"], syn["delay_steps"], 0, p.random.wmode,
+                            syn["w"], p.random.w[0])
+        else:
+            net.add_pathway(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode, p.weights,
+                            p.wscalar)
     smons = [net.add_spike_monitor(g) for g in spec.spike_monitors]
     stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
     return net, smons, stmons

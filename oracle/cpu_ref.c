@@ -105,6 +105,78 @@ double ref_rand(uint64_t seed, uint32_t stream, uint32_t index, int64_t step)
 
 int ref_quantize_delay(double delay, double dt) { return (int)(delay / dt + 0.5); }
 
+/*
+ * Random connectivity, restated from the reference's generator
+ * (brian2cuda/templates/synapses_create_generator.cu:315-333), outer loop over
+ * presynaptic neurons, inner over targets in increasing j:
+ *   p == 0    no synapses (line 315)
+ *   p < 0.25  jump algorithm: r = rand(); r == 0 ends the row; j += floor(log(r) *
+ *             1/log(1-p)); j past the end ends the row; else j is selected
+ *   p >= 0.25 rand() >= p skips the pair
+ * The draws (the product's documented layout, not the reference's cuRAND buffer):
+ *   skip draw k of row i : Philox((i, k/2, 0x434E0000|0, stream)), words 0-1 (k even) or 2-3
+ *   pair test of (i, j)  : Philox((i, (j-post_lo)/2, 0x434E0000|2, stream)), halves the same way
+ *   synapse (i, j)       : Philox((i, j, 0x434E0000|1, stream)): delay uniform from words 0-1,
+ *                          weight uniform from words 2-3
+ * delay = delay_lo + u*(delay_hi - delay_lo) seconds, quantised (int)(d/dt + 0.5)
+ * (synapses_push_spikes.cu:375); weight = w_lo + u*(w_hi - w_lo).
+ */
+static void ref_conn_words(uint64_t seed, uint32_t stream, uint32_t i, uint32_t k, uint32_t tag, uint32_t *w)
+{
+    uint32_t c[4] = {i, k, 0x434E0000u | tag, stream};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    ref_philox4x32_10(c, key, w);
+}
+
+int64_t ref_connect_random(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int skip_self, uint64_t seed,
+                           uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo, double w_hi,
+                           int32_t *pre, int32_t *post, int32_t *delay_steps, double *w, int64_t cap)
+{
+    int64_t n = 0;
+    if (p == 0.0) return 0;
+    const int jump = p < 0.25;
+    const double pconst = jump ? 1.0 / ref_log(1.0 - p) : 0.0;
+    const double dspan = delay_hi - delay_lo, wspan = w_hi - w_lo;
+    for (int i = pre_lo; i < pre_hi; ++i) {
+        uint32_t words[4] = {0, 0, 0, 0};
+        uint32_t draw = 0;
+        long long j = post_lo;
+        while (j < post_hi) {
+            if (jump) {
+                if ((draw & 1u) == 0) ref_conn_words(seed, stream, (uint32_t)i, draw / 2, 0, words);
+                const double r = (draw & 1u) ? ref_u53(words[2], words[3]) : ref_u53(words[0], words[1]);
+                draw++;
+                if (r == 0.0) break;
+                const double skip = floor(ref_log(r) * pconst);
+                if (skip >= (double)(post_hi - j)) break;
+                j += (long long)skip;
+            } else {
+                const uint32_t m = (uint32_t)(j - post_lo);
+                if ((m & 1u) == 0) ref_conn_words(seed, stream, (uint32_t)i, m / 2, 2, words);
+                const double r = (m & 1u) ? ref_u53(words[2], words[3]) : ref_u53(words[0], words[1]);
+                if (r >= p) {
+                    ++j;
+                    continue;
+                }
+            }
+            if (!(skip_self && j == i)) {
+                if (n < cap) {
+                    uint32_t sw[4];
+                    ref_conn_words(seed, stream, (uint32_t)i, (uint32_t)j, 1, sw);
+                    const double ud = ref_u53(sw[0], sw[1]), uw = ref_u53(sw[2], sw[3]);
+                    if (pre) pre[n] = i;
+                    if (post) post[n] = (int32_t)j;
+                    if (delay_steps) delay_steps[n] = ref_quantize_delay(delay_lo + ud * dspan, dt);
+                    if (w) w[n] = w_lo + uw * wspan;
+                }
+                n++;
+            }
+            ++j;
+        }
+    }
+    return n;
+}
+
 long long ref_timestep(double t, double dt) { return hs_timestep(t, dt); }
 
 double ref_exp(double x) { return hs_exp(x); }
