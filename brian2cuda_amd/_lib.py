@@ -87,6 +87,9 @@ SIGNATURES = {
     "hs_net_pathway_info": (C.c_int, [P, C.c_int, I64P, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
     "hs_net_get_synapses": (C.c_int, [P, C.c_int, I32P, I32P, I32P, DP, C.c_int64]),
+    "hs_net_set_spikegen": (C.c_int, [P, C.c_int, C.c_int64, I32P, I64P]),
+    "hs_net_pathway_stdp": (C.c_int, [P, C.c_int, DP, C.c_int]),
+    "hs_net_get_synapse_var": (C.c_int, [P, C.c_int, C.c_int, DP, C.c_int64]),
     "hs_net_mon_count": (C.c_int, [P, C.c_int, I64P]),
     "hs_net_get_spikes": (C.c_int, [P, C.c_int, I32P, I64P, C.c_int64]),
     "hs_net_get_state_monitor": (C.c_int, [P, C.c_int, DP, C.c_int64]),

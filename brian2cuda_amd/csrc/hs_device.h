@@ -71,6 +71,24 @@ template <int MODEL> struct HsNV;
 template <> struct HsNV<HS_MODEL_LIF_CUBA> { static constexpr int value = HS_LIF_CUBA_NVARS; };
 template <> struct HsNV<HS_MODEL_LIF_NOISE> { static constexpr int value = HS_LIF_NOISE_NVARS; };
 template <> struct HsNV<HS_MODEL_HH_COBA> { static constexpr int value = HS_HH_NVARS; };
+template <> struct HsNV<HS_MODEL_SPIKEGEN> { static constexpr int value = HS_SPIKEGEN_NVARS; };
+
+/* SpikeGeneratorGroup: does neuron i fire at step t (binary search in the step's ascending list) */
+template <typename R>
+__device__ __forceinline__ bool hs_spikegen_fires(const DGroup<R> &g, long long t, int i)
+{
+    const long long s = g.gen_period > 0 ? t % g.gen_period : t;
+    if (s < 0 || s >= g.gen_steps) return false;
+    int lo = g.gen_off[s], hi = g.gen_off[s + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = g.gen_idx[mid];
+        if (v == i) return true;
+        if (v < i) lo = mid + 1;
+        else hi = mid;
+    }
+    return false;
+}
 
 /* the model's noise draw for neuron i at step t (0 for noiseless models) */
 template <typename R, int MODEL>

@@ -137,23 +137,29 @@ __device__ __forceinline__ bool hs_step_one(const DGroup<R> &g, R *s, const R *k
                                             long long no_reset_before, R xi)
 {
     constexpr int NV = HsNV<MODEL>::value;
-    if (g.has_reset && (long long)last == t - 1 && t - 1 >= no_reset_before) s[0] = g.vr;
-    for (int m = 0; m < g.nst; ++m) {
-        if (i >= g.st_start[m] && i < g.st_stop[m]) {
-            const long long row = t - g.st_t0[m];
-            if (row >= 0 && row < g.st_rows[m]) {
-                R val = s[0];
+    if constexpr (MODEL == HS_MODEL_SPIKEGEN) { /* spikegenerator.cu:43-80: no state, no refractoriness */
+        const bool fire = hs_spikegen_fires(g, t, i);
+        if (fire) last = (int32_t)t;
+        return fire;
+    } else {
+        if (g.has_reset && (long long)last == t - 1 && t - 1 >= no_reset_before) s[0] = g.vr;
+        for (int m = 0; m < g.nst; ++m) {
+            if (i >= g.st_start[m] && i < g.st_stop[m]) {
+                const long long row = t - g.st_t0[m];
+                if (row >= 0 && row < g.st_rows[m]) {
+                    R val = s[0];
 #pragma unroll
-                for (int v = 1; v < NV; ++v)
-                    if (v == g.st_var[m]) val = s[v];
-                g.st_buf[m][row * (long long)(g.st_stop[m] - g.st_start[m]) + (i - g.st_start[m])] = val;
+                    for (int v = 1; v < NV; ++v)
+                        if (v == g.st_var[m]) val = s[v];
+                    g.st_buf[m][row * (long long)(g.st_stop[m] - g.st_start[m]) + (i - g.st_start[m])] = val;
+                }
             }
         }
+        const int not_ref = (t - (long long)last) >= (long long)g.ref_steps;
+        const bool spk = hs_neuron_step<R, MODEL>(s, k, not_ref, g.vth, xi);
+        if (spk) last = (int32_t)t;
+        return spk;
     }
-    const int not_ref = (t - (long long)last) >= (long long)g.ref_steps;
-    const bool spk = hs_neuron_step<R, MODEL>(s, k, not_ref, g.vth, xi);
-    if (spk) last = (int32_t)t;
-    return spk;
 }
 
 template <typename R, int MODEL, bool FUSED, bool PUSH>
@@ -320,6 +326,9 @@ __device__ __forceinline__ void hs_update_phase(const DGroup<R> *groups, int ng,
         } else if (g.model == HS_MODEL_LIF_NOISE) {
             hs_update_tile<R, HS_MODEL_LIF_NOISE, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
                                                                push);
+        } else if (g.model == HS_MODEL_SPIKEGEN) {
+            hs_update_tile<R, HS_MODEL_SPIKEGEN, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
+                                                              push);
         } else {
             hs_update_tile<R, HS_MODEL_HH_COBA, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
                                                              push);

@@ -27,6 +27,7 @@
 /* delivery modes of a pathway */
 #define HS_DELIVER_SCATTER 1 /* per (pre, delay) bundles, global atomics on the targets */
 #define HS_DELIVER_GATHER 2  /* per (pre, target tile) segments, LDS accumulation per tile */
+#define HS_DELIVER_PLASTIC 3 /* scatter bundles with STDP (on_pre and on_post kernels, plain form) */
 
 /* The delivered-event count is split over HS_EV_SLOTS counters, one per 128-byte
    line: thousands of waves add to it per launch, and same-address atomics
@@ -93,6 +94,11 @@ struct DGroup {
        (slot, s): hist[slot*nsub*subcap + s*subcap ..], count at
        hist_cnt[(slot*nsub + s) * HS_CNT_STRIDE] */
     int nsub, subcap;
+    /* SpikeGeneratorGroup: neurons gen_idx[gen_off[s] .. gen_off[s+1]) (ascending)
+       fire at step s < gen_steps; step t plays s = t mod gen_period (0: s = t) */
+    const int32_t *gen_off;
+    const int32_t *gen_idx;
+    long long gen_steps, gen_period;
 };
 
 template <typename R>
@@ -119,6 +125,17 @@ struct DPath {
     /* pipelined windows: delay slots [0, nnear) are pushed by the emitting
        update, [nnear, D) pulled by the delivery blocks */
     int nnear;
+    /* STDP (dmode HS_DELIVER_PLASTIC): weights wm (the array w), traces and
+       last update step per synapse in bundle order; the synapses of target
+       neuron j are qsyn[qstart[j] .. qstart[j+1]); the target group's spike
+       ring for on_post */
+    R *wm, *apre, *apost;
+    int32_t *lastup;
+    const int32_t *qstart, *qsyn;
+    double taupre, taupost, dt;
+    R sk[5]; /* dApre, dApost, wmin, wmax, scale */
+    const int32_t *thist, *thist_cnt;
+    int tQ, tnsub, tsubcap;
 };
 
 /*
@@ -228,5 +245,9 @@ hipError_t hs_launch_window(const DGroup<R> *groups, int ng, int total_tiles, co
 template <typename R>
 hipError_t hs_launch_pending(const DPath<R> *paths, int np, HsCtl *ctl, hipStream_t s);
 hipError_t hs_launch_tick(HsCtl *ctl, long long k, hipStream_t s);
+/* STDP pathways at step ctl->base + koff (plain form, after the delivery of the
+   step): on_pre of the bundles due now, then on_post of the targets' spikes */
+template <typename R>
+hipError_t hs_launch_stdp(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int grid, hipStream_t s);
 
 #endif /* HS_INTERNAL_H */

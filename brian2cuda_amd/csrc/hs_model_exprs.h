@@ -37,7 +37,8 @@ HS_HD long long hs_timestep(double t, double dt)
  * Per-group constants, computed once on the host in double with the
  * deterministic exp so that every caller derives the same bits.
  * Returns the number of constants written to k (<= 16), or -1 for a bad model.
- *   LIF_CUBA : params El,Vt,Vr,taum,taue,taui,ref -> k = Em,Ee,Ei,Ae,Ai,El
+ * p holds every parameter of the model, optional ones included (hs_model_params).
+ *   LIF_CUBA : params El,Vt,Vr,taum,taue,taui,ref,free -> k = Em,Ee,Ei,Ae,Ai,El,free
  *   LIF_NOISE: params tau,mu,sigma,theta,Vr,ref,stream -> k = a,b,mu
  *   HH_COBA  : params Cm,gl,El,EK,ENa,gNa,gK,VT,Ee,Ei,taue,taui,vth,ref
  *              -> k = Cm,gl,El,EK,ENa,gNa,gK,VT,Ee,Ei,dec_e,dec_i,dt
@@ -53,7 +54,8 @@ static inline int hs_model_constants(int model, const double *p, double dt, doub
         k[3] = taue / (taue - taum) * (Ee - Em);
         k[4] = taui / (taui - taum) * (Ei - Em);
         k[5] = p[0];
-        return 6;
+        k[6] = p[7];
+        return 7;
     }
     if (model == 2) { /* HS_MODEL_LIF_NOISE */
         double tau = p[0], mu = p[1], sigma = p[2];
@@ -69,7 +71,27 @@ static inline int hs_model_constants(int model, const double *p, double dt, doub
         k[12] = dt;
         return 13;
     }
+    if (model == 4) /* HS_MODEL_SPIKEGEN: no state update */
+        return 0;
     return -1;
+}
+
+/*
+ * A model's parameter vector with its optional trailing parameters filled in
+ * (zero = the default): out[16] from np given values.  Returns the model's full
+ * count, or -1 if np is not a count the model accepts.
+ */
+static inline int hs_model_params(int model, const double *p, int np, double *out)
+{
+    int lo, hi;
+    if (model == 1) { lo = 7; hi = 8; }       /* LIF_CUBA [, free] */
+    else if (model == 2) { lo = hi = 7; }     /* LIF_NOISE */
+    else if (model == 3) { lo = hi = 14; }    /* HH_COBA */
+    else if (model == 4) { lo = hi = 1; }     /* SPIKEGEN */
+    else return -1;
+    if (np < lo || np > hi || (np > 0 && !p)) return -1;
+    for (int i = 0; i < 16; ++i) out[i] = i < np ? p[i] : 0.0;
+    return hi;
 }
 
 #endif /* HS_MODEL_EXPRS_H */

@@ -202,6 +202,7 @@ static bool hs_model_info(int model, HsModelInfo *mi)
     if (model == HS_MODEL_LIF_CUBA) { *mi = {HS_LIF_CUBA_NVARS, HS_LIF_CUBA_NPARAMS, 1, 1, 2, 6, -1}; return true; }
     if (model == HS_MODEL_LIF_NOISE) { *mi = {HS_LIF_NOISE_NVARS, HS_LIF_NOISE_NPARAMS, 1, 3, 4, 5, 6}; return true; }
     if (model == HS_MODEL_HH_COBA) { *mi = {HS_HH_NVARS, HS_HH_NPARAMS, 0, 12, -1, 13, -1}; return true; }
+    if (model == HS_MODEL_SPIKEGEN) { *mi = {HS_SPIKEGEN_NVARS, HS_SPIKEGEN_NPARAMS, 0, -1, -1, -1, -1}; return true; }
     return false;
 }
 
@@ -247,9 +248,13 @@ extern "C" int hs_stateupdate(int model, int precision, int n, void *const *vars
 {
     HS_TRY(hs_need_init());
     HsModelInfo mi;
-    if (!hs_model_info(model, &mi)) return hs_fail(HS_ERR_UNSUPPORTED, "unknown model %d", model);
-    if (nparams != mi.nparams || !params || !vars || !last || n < 0 || (precision != HS_F64 && precision != HS_F32))
+    if (!hs_model_info(model, &mi) || model == HS_MODEL_SPIKEGEN)
+        return hs_fail(HS_ERR_UNSUPPORTED, "no state update for model %d", model);
+    double pp[16];
+    if (hs_model_params(model, params, nparams, pp) < 0 || !vars || !last || n < 0 ||
+        (precision != HS_F64 && precision != HS_F32))
         return hs_fail(HS_ERR_ARG, "hs_stateupdate: bad arguments (model %d expects %d params)", model, mi.nparams);
+    params = pp;
     double k[16] = {0};
     if (hs_model_constants(model, params, dt, k) < 0) return hs_fail(HS_ERR_ARG, "bad model");
     int ref_steps = hs_ref_steps(params[mi.i_ref], dt);
@@ -348,6 +353,8 @@ struct HGroup {
     void *xi = nullptr; /* windowed form: noise ring [xi_slots][n], drawn one window ahead */
     int xi_slots = 0;
     int nsub = 1, subcap = 0; /* spike sub-lists per ring slot (see DGroup) */
+    int32_t *gen_off = nullptr, *gen_idx = nullptr; /* SpikeGeneratorGroup pattern (see DGroup) */
+    long long gen_steps = 0, gen_period = 0;
 };
 
 struct HPath {
@@ -363,6 +370,10 @@ struct HPath {
     int n_tiles = 0;
     int32_t *goff = nullptr;
     uint32_t *gsyn = nullptr;
+    /* STDP (dmode HS_DELIVER_PLASTIC) */
+    void *apre = nullptr, *apost = nullptr;
+    int32_t *lastup = nullptr, *qstart = nullptr, *qsyn = nullptr;
+    double stdp[HS_STDP_NPARAMS] = {0};
 };
 
 struct HMon {
@@ -402,7 +413,7 @@ struct hs_net {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> kt_open;
     double kt_ms[5] = {0, 0, 0, 0, 0};
     int64_t kt_n[5] = {0, 0, 0, 0, 0};
-    int n_scatter = 0, n_gather = 0, gather_tiles = 0;
+    int n_scatter = 0, n_gather = 0, gather_tiles = 0, n_plastic = 0;
     std::map<int64_t, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_wall = 0, last_gpu_ms = 0;
@@ -483,17 +494,17 @@ extern "C" int hs_net_add_group(hs_net *net, int model, int n, const double *par
     if (net->built) return hs_fail(HS_ERR_STATE, "groups must be added before the first run");
     HsModelInfo mi;
     if (!hs_model_info(model, &mi)) return hs_fail(HS_ERR_UNSUPPORTED, "unknown model %d", model);
-    if (n <= 0 || !params || nparams != mi.nparams)
-        return hs_fail(HS_ERR_ARG, "hs_net_add_group: model %d expects %d params, n > 0", model, mi.nparams);
     HGroup g;
+    if (n <= 0 || hs_model_params(model, params, nparams, g.params) < 0)
+        return hs_fail(HS_ERR_ARG, "hs_net_add_group: model %d expects %d params, n > 0", model, mi.nparams);
+    params = g.params; /* every parameter, optional ones filled in */
     g.model = model;
     g.n = n;
     g.mi = mi;
-    memset(g.params, 0, sizeof(g.params));
-    memcpy(g.params, params, sizeof(double) * nparams);
     memset(g.k, 0, sizeof(g.k));
     hs_model_constants(model, params, net->dt, g.k);
-    g.ref_steps = hs_ref_steps(params[mi.i_ref], net->dt);
+    g.ref_steps = mi.i_ref >= 0 ? hs_ref_steps(params[mi.i_ref], net->dt) : 0;
+    if (model == HS_MODEL_SPIKEGEN) g.gen_period = hs_timestep(params[0], net->dt);
     g.stream = mi.i_stream >= 0 ? (unsigned int)params[mi.i_stream] : 0u;
     for (int v = 0; v < mi.nvars; ++v) HS_TRY(hs_dalloc(net, hs_rsize(net) * n, &g.var[v], true));
     HS_TRY(hs_dalloc(net, sizeof(int32_t) * n, (void **)&g.last, false));
@@ -855,7 +866,7 @@ extern "C" int hs_net_get_synapses(hs_net *net, int pid, int32_t *pre, int32_t *
 {
     if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");
     const HPath &P = net->p[pid];
-    if (P.dmode != HS_DELIVER_SCATTER) return hs_fail(HS_ERR_UNSUPPORTED, "synapse readback of gather-mode pathways");
+    if (P.dmode == HS_DELIVER_GATHER) return hs_fail(HS_ERR_UNSUPPORTED, "synapse readback of gather-mode pathways");
     if (cap < P.nsyn) return hs_fail(HS_ERR_ARG, "capacity %lld < %lld synapses", (long long)cap, (long long)P.nsyn);
     const int n_src = net->g[P.src].n;
     const size_t nb = (size_t)n_src * P.D;
@@ -883,6 +894,121 @@ extern "C" int hs_net_get_synapses(hs_net *net, int pid, int32_t *pre, int32_t *
                 if (delay_steps) delay_steps[s] = P.dmin + d;
                 if (w) w[s] = P.wmode == HS_W_SCALAR ? P.wscalar : P.wmode == HS_W_SYNAPSE ? wat((size_t)s) : wat(i);
             }
+    return HS_OK;
+}
+
+/* SpikeGeneratorGroup pattern: host counting sort by step, ascending neurons
+   within a step, one upload; may be replaced between runs */
+extern "C" int hs_net_set_spikegen(hs_net *net, int gid, int64_t n, const int32_t *indices, const int64_t *steps)
+{
+    if (!net || gid < 0 || gid >= (int)net->g.size()) return hs_fail(HS_ERR_ARG, "bad group");
+    HGroup &g = net->g[gid];
+    if (g.model != HS_MODEL_SPIKEGEN) return hs_fail(HS_ERR_ARG, "group %d is not a spike generator", gid);
+    if (n < 0 || n >= 0x7fffffffLL || (n > 0 && (!indices || !steps))) return hs_fail(HS_ERR_ARG, "bad spike list");
+    long long smax = -1;
+    for (int64_t k = 0; k < n; ++k) {
+        if (indices[k] < 0 || indices[k] >= g.n) return hs_fail(HS_ERR_ARG, "spike %lld: neuron out of range", (long long)k);
+        if (steps[k] < 0 || steps[k] >= 0x7ffffffeLL) return hs_fail(HS_ERR_ARG, "spike %lld: step out of range", (long long)k);
+        smax = std::max<long long>(smax, steps[k]);
+    }
+    const long long ns = smax + 1;
+    std::vector<int32_t> off((size_t)ns + 1, 0), idx((size_t)(n ? n : 1));
+    for (int64_t k = 0; k < n; ++k) off[(size_t)steps[k] + 1]++;
+    for (long long s = 0; s < ns; ++s) off[(size_t)s + 1] += off[(size_t)s];
+    std::vector<int32_t> fill(off.begin(), off.end() - 1);
+    for (int64_t k = 0; k < n; ++k) idx[(size_t)fill[(size_t)steps[k]]++] = indices[k];
+    for (long long s = 0; s < ns; ++s) {
+        std::sort(idx.begin() + off[(size_t)s], idx.begin() + off[(size_t)s + 1]);
+        for (int32_t q = off[(size_t)s] + 1; q < off[(size_t)s + 1]; ++q)
+            if (idx[(size_t)q] == idx[(size_t)q - 1])
+                return hs_fail(HS_ERR_ARG, "neuron %d given twice at step %lld", idx[(size_t)q], s);
+    }
+    HS_HIP(hipStreamSynchronize(g_rt.stream)); /* no launch still reads the old pattern */
+    hs_dfree(net, g.gen_off);
+    hs_dfree(net, g.gen_idx);
+    g.gen_off = g.gen_idx = nullptr;
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * off.size(), (void **)&g.gen_off, false));
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * idx.size(), (void **)&g.gen_idx, false));
+    HS_HIP(hipMemcpyAsync(g.gen_off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipMemcpyAsync(g.gen_idx, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    g.gen_steps = ns;
+    net->desc_dirty = true;
+    hs_drop_graphs(net);
+    return HS_OK;
+}
+
+/* STDP on a pathway: traces and last-update steps allocated (zero: Brian2's
+   initial values), the target-indexed synapse table built from the bundles */
+extern "C" int hs_net_pathway_stdp(hs_net *net, int pid, const double *params, int nparams)
+{
+    if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");
+    if (net->built) return hs_fail(HS_ERR_STATE, "plasticity must be set before the first run");
+    if (!params || nparams != HS_STDP_NPARAMS) return hs_fail(HS_ERR_ARG, "STDP takes %d parameters", HS_STDP_NPARAMS);
+    if (net->engine != HS_ENGINE_PLAIN) return hs_fail(HS_ERR_UNSUPPORTED, "STDP pathways run in the plain form");
+    HPath &P = net->p[pid];
+    if (P.dmode != HS_DELIVER_SCATTER) return hs_fail(HS_ERR_UNSUPPORTED, "STDP needs a scatter-mode pathway");
+    if (P.wmode != HS_W_SYNAPSE) return hs_fail(HS_ERR_ARG, "STDP needs per-synapse weights (HS_W_SYNAPSE)");
+    if (!(params[0] > 0.0) || !(params[1] > 0.0)) return hs_fail(HS_ERR_ARG, "trace time constants must be > 0");
+    if (!(params[4] <= params[5])) return hs_fail(HS_ERR_ARG, "wmin > wmax");
+    const HGroup &gt = net->g[P.tgt];
+    const size_t ns = (size_t)(P.nsyn ? P.nsyn : 1);
+    std::vector<int32_t> po(ns), qs((size_t)gt.n + 1, 0), qy(ns);
+    if (P.nsyn)
+        HS_HIP(hipMemcpyAsync(po.data(), P.post, sizeof(int32_t) * (size_t)P.nsyn, hipMemcpyDeviceToHost, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    for (int64_t s = 0; s < P.nsyn; ++s) qs[(size_t)po[(size_t)s] + 1]++;
+    for (int j = 0; j < gt.n; ++j) qs[(size_t)j + 1] += qs[(size_t)j];
+    std::vector<int32_t> fill(qs.begin(), qs.end() - 1);
+    for (int64_t s = 0; s < P.nsyn; ++s) qy[(size_t)fill[(size_t)po[(size_t)s]]++] = (int32_t)s;
+    const size_t rs = hs_rsize(net);
+    HS_TRY(hs_dalloc(net, rs * ns, &P.apre, true));
+    HS_TRY(hs_dalloc(net, rs * ns, &P.apost, true));
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * ns, (void **)&P.lastup, true));
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * qs.size(), (void **)&P.qstart, false));
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * ns, (void **)&P.qsyn, false));
+    HS_HIP(hipMemcpyAsync(P.qstart, qs.data(), sizeof(int32_t) * qs.size(), hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipMemcpyAsync(P.qsyn, qy.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    for (int j = 0; j < HS_STDP_NPARAMS; ++j) P.stdp[j] = params[j];
+    P.dmode = HS_DELIVER_PLASTIC;
+    net->n_scatter--;
+    net->n_plastic++;
+    net->desc_dirty = true;
+    return HS_OK;
+}
+
+/* a synapse variable (HS_SYN_*) of a pathway in bundle order (hs_net_get_synapses' order) */
+extern "C" int hs_net_get_synapse_var(hs_net *net, int pid, int var, double *out, int64_t cap)
+{
+    if (!net || pid < 0 || pid >= (int)net->p.size() || !out) return hs_fail(HS_ERR_ARG, "bad pathway");
+    const HPath &P = net->p[pid];
+    if (P.dmode == HS_DELIVER_GATHER) return hs_fail(HS_ERR_UNSUPPORTED, "synapse readback of gather-mode pathways");
+    if (cap < P.nsyn) return hs_fail(HS_ERR_ARG, "capacity %lld < %lld synapses", (long long)cap, (long long)P.nsyn);
+    if (var < HS_SYN_W || var > HS_SYN_LASTUPDATE) return hs_fail(HS_ERR_ARG, "bad synapse variable %d", var);
+    if (var != HS_SYN_W && P.dmode != HS_DELIVER_PLASTIC)
+        return hs_fail(HS_ERR_ARG, "synapse variable %d exists on STDP pathways only", var);
+    if (P.nsyn == 0) return HS_OK;
+    if (var == HS_SYN_W && P.wmode != HS_W_SYNAPSE)
+        return hs_net_get_synapses(net, pid, nullptr, nullptr, nullptr, out, cap);
+    const size_t n = (size_t)P.nsyn;
+    if (var == HS_SYN_LASTUPDATE) {
+        std::vector<int32_t> v(n);
+        HS_HIP(hipMemcpyAsync(v.data(), P.lastup, sizeof(int32_t) * n, hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+        for (size_t s = 0; s < n; ++s) out[s] = (double)v[s];
+        return HS_OK;
+    }
+    const void *src = var == HS_SYN_W ? P.w : (var == HS_SYN_APRE ? P.apre : P.apost);
+    if (net->prec == HS_F64) {
+        HS_HIP(hipMemcpyAsync(out, src, sizeof(double) * n, hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    } else {
+        std::vector<float> v(n);
+        HS_HIP(hipMemcpyAsync(v.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+        for (size_t s = 0; s < n; ++s) out[s] = (double)v[s];
+    }
     return HS_OK;
 }
 
@@ -970,6 +1096,8 @@ extern "C" int hs_net_get_option(hs_net *net, int key, long long *value)
 static int hs_build(hs_net *net)
 {
     if (net->built) return HS_OK;
+    if (net->n_plastic && net->engine != HS_ENGINE_PLAIN)
+        return hs_fail(HS_ERR_UNSUPPORTED, "STDP pathways run in the plain form (HS_ENGINE_PLAIN)");
     /* form of the step loop */
     int dmin_all = 1 << 30, dspan = 1;
     for (const HPath &P : net->p) {
@@ -1077,7 +1205,11 @@ static int hs_upload_desc_t(hs_net *net)
         d.nsub = g.nsub;
         d.subcap = g.subcap;
         for (int j = 0; j < 16; ++j) d.k[j] = (R)g.k[j];
-        d.vth = (R)g.params[g.mi.i_vth];
+        d.vth = g.mi.i_vth >= 0 ? (R)g.params[g.mi.i_vth] : (R)0;
+        d.gen_off = g.gen_off;
+        d.gen_idx = g.gen_idx;
+        d.gen_steps = g.gen_steps;
+        d.gen_period = g.gen_period;
         d.vr = g.mi.i_vr >= 0 ? (R)g.params[g.mi.i_vr] : (R)0;
         d.seed = net->seed;
         if (g.smon >= 0) {
@@ -1136,6 +1268,23 @@ static int hs_upload_desc_t(hs_net *net)
         d.gsyn = P.gsyn;
         /* pipelined windows: delays < 2W are pushed at emission */
         d.nnear = net->mode == 4 ? std::max(0, std::min(P.D, 2 * net->window - P.dmin)) : 0;
+        if (P.dmode == HS_DELIVER_PLASTIC) {
+            d.wm = (R *)P.w;
+            d.apre = (R *)P.apre;
+            d.apost = (R *)P.apost;
+            d.lastup = P.lastup;
+            d.qstart = P.qstart;
+            d.qsyn = P.qsyn;
+            d.taupre = P.stdp[0];
+            d.taupost = P.stdp[1];
+            d.dt = net->dt;
+            for (int j = 0; j < 5; ++j) d.sk[j] = (R)P.stdp[2 + j];
+            d.thist = gt.hist;
+            d.thist_cnt = gt.hist_cnt;
+            d.tQ = gt.Q;
+            d.tnsub = gt.nsub;
+            d.tsubcap = gt.subcap;
+        }
     }
     if (!net->dgroups) HS_TRY(hs_dalloc(net, sizeof(DGroup<R>) * std::max<size_t>(1, dg.size()), &net->dgroups, false));
     if (!net->dpaths) HS_TRY(hs_dalloc(net, sizeof(DPath<R>) * std::max<size_t>(1, dp.size()), &net->dpaths, false));
@@ -1325,6 +1474,8 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
             } else {
                 HS_TRY(update((int)j, 1, 1, fused));
                 if (np > 0) HS_TRY(deliver((int)j, 1));
+                if (net->n_plastic)
+                    HS_HIP(hs_launch_stdp<R>(dp, np, net->ctl, (int)j, std::max(1, hs_rt_cus()), g_rt.stream));
             }
         }
     }

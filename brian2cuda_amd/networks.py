@@ -16,8 +16,8 @@ from typing import List, Optional
 
 import numpy as np
 
-from .runtime import (HS_MODEL_HH_COBA, HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_W_PRE, HS_W_SCALAR,
-                      HS_W_SYNAPSE)
+from .runtime import (HS_MODEL_HH_COBA, HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_SPIKEGEN, HS_W_PRE,
+                      HS_W_SCALAR, HS_W_SYNAPSE)
 
 mV, ms, nS = 1e-3, 1e-3, 1e-9
 
@@ -28,6 +28,7 @@ class GroupSpec:
     n: int
     params: list
     init: dict = field(default_factory=dict)  # var index -> array (or scalar)
+    spikes: Optional[tuple] = None            # spike generator: (indices, steps)
 
 
 @dataclass
@@ -60,6 +61,7 @@ class PathSpec:
     weights: Optional[np.ndarray] = None
     wscalar: float = 0.0
     random: Optional[RandomConnect] = None
+    stdp: Optional[tuple] = None  # taupre, taupost, dApre, dApost, wmin, wmax, scale
 
     @property
     def nsyn(self):
@@ -261,6 +263,52 @@ def batch_cuba(n_nets=256, seed=1234, nmin=500, nmax=16000, p=0.02, dt=0.1 * ms,
     return spec
 
 
+# ---- plasticity: the reference's STDP consistency networks ---------------------------
+
+def stdp_example(heterog_delays=False, n_cells=100, seed=42, dt=0.1 * ms):
+    """The networks of brian2cuda/tests/test_cpp_cuda_consistency.py:10-89
+    (test_stdp_example) and :92-172 (test_stdp_heterog_delays_example), with the
+    test's own numpy draws (legacy RandomState(42), same order):
+
+      P: n_cells LIF, dv/dt = (g-(v-El))/taum, dg/dt = -g/taus, refractory 5 ms
+         without "(unless refractory)" -> CUBA with gi unused and free = 1
+      Q: SpikeGeneratorGroup(n_cells, sources, times), one spike per ms at most
+      S: P -> P all-to-all, STDP (event-driven traces), w = fac*randn, on_pre
+         g += w*mV; heterog_delays: delay = U[0, 2) ms per synapse
+      T: Q -> P one-to-one, w = 10*fac, on_pre g += w*mV
+    Pathway order: T then S (Brian2 sorts the synapses slot by order, then by
+    name: synapses_1_pre before synapses_pre), S's post pathway after both."""
+    rs = np.random.RandomState(seed)
+    taum, taus = 20 * ms, 5 * ms
+    Vt, Vr, El = -50 * mV, -60 * mV, -49 * mV
+    fac = 60 * 0.27 / 10
+    gmax = 20 * fac
+    dApre, taupre = 0.01, 20 * ms
+    taupost = taupre
+    dApost = -dApre * taupre / taupost * 1.05
+    dApost *= 0.1 * gmax
+    dApre *= 0.1 * gmax
+    connectivity = rs.randn(n_cells, n_cells)
+    sources = rs.randint(0, n_cells, 10 * n_cells)
+    times_ms = rs.choice(np.arange(10 * n_cells), 10 * n_cells, replace=False)
+    v_init = Vr + rs.rand(n_cells) * (Vt - Vr)
+    delay = rs.rand(n_cells ** 2) * 2 * ms if heterog_delays else None
+    P = GroupSpec(HS_MODEL_LIF_CUBA, n_cells, [El, Vt, Vr, taum, taus, taus, 5 * ms, 1.0], {0: v_init, 1: 0.0})
+    Q = GroupSpec(HS_MODEL_SPIKEGEN, n_cells, [0.0],
+                  spikes=(sources.astype(np.int32), np.rint(times_ms * ms / dt).astype(np.int64)))
+    ii, jj = np.meshgrid(np.arange(n_cells), np.arange(n_cells), indexing="ij")
+    T = PathSpec(1, 0, 1, np.arange(n_cells, dtype=np.int32), np.arange(n_cells, dtype=np.int32),
+                 wmode=HS_W_SCALAR, wscalar=10 * fac * mV)
+    S = PathSpec(0, 0, 1, ii.ravel().astype(np.int32), jj.ravel().astype(np.int32),
+                 delay_steps=None if delay is None else quantise_delay(delay, dt),
+                 wmode=HS_W_SYNAPSE, weights=fac * connectivity.ravel(),
+                 stdp=(taupre, taupost, dApre, dApost, -np.inf, np.inf, mV))
+    spec = NetSpec("stdp" + ("_heterog" if heterog_delays else ""), dt, seed, [P, Q], [T, S],
+                   spike_monitors=[0], state_monitors=[(0, 0, 0, 10)],
+                   info={"n_cells": n_cells, "record_synapses": list(range(10)), "record_every_steps": 1000})
+    return spec
+
+
 # ---- instantiate on the device --------------------------------------------------------
 
 def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options=None):
@@ -273,14 +321,18 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
         gid = net.add_group(g.model, g.n, g.params)
         for var, val in g.init.items():
             net.set_state(gid, var, val)
+        if g.spikes is not None:
+            net.set_spikegen(gid, *g.spikes)
     for p in spec.paths:
         if p.random is not None:
             r = p.random
-            net.connect_random(p.src, p.tgt, p.var, r.p, r.pre, r.post, r.autapses, r.seed, r.stream, r.delay,
-                               r.wmode, r.w)
+            pid = net.connect_random(p.src, p.tgt, p.var, r.p, r.pre, r.post, r.autapses, r.seed, r.stream, r.delay,
+                                     r.wmode, r.w)
         else:
-            net.add_pathway(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode, p.weights,
-                            p.wscalar)
+            pid = net.add_pathway(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode,
+                                  p.weights, p.wscalar)
+        if p.stdp is not None:
+            net.pathway_stdp(pid, *p.stdp)
     smons = [net.add_spike_monitor(g, spike_capacity) for g in spec.spike_monitors]
     stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
     return net, smons, stmons

@@ -16,14 +16,17 @@ HS_F64, HS_F32 = 64, 32
 HS_ENGINE_PLAIN, HS_ENGINE_FUSED = 0, 1
 HS_W_SCALAR, HS_W_SYNAPSE, HS_W_PRE = 0, 1, 2
 
-HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_HH_COBA = 1, 2, 3
-MODEL_NVARS = {HS_MODEL_LIF_CUBA: 3, HS_MODEL_LIF_NOISE: 1, HS_MODEL_HH_COBA: 6}
-MODEL_NPARAMS = {HS_MODEL_LIF_CUBA: 7, HS_MODEL_LIF_NOISE: 7, HS_MODEL_HH_COBA: 14}
+HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_HH_COBA, HS_MODEL_SPIKEGEN = 1, 2, 3, 4
+MODEL_NVARS = {HS_MODEL_LIF_CUBA: 3, HS_MODEL_LIF_NOISE: 1, HS_MODEL_HH_COBA: 6, HS_MODEL_SPIKEGEN: 1}
+MODEL_NPARAMS = {HS_MODEL_LIF_CUBA: 7, HS_MODEL_LIF_NOISE: 7, HS_MODEL_HH_COBA: 14, HS_MODEL_SPIKEGEN: 1}
 MODEL_VARS = {
     HS_MODEL_LIF_CUBA: ("v", "ge", "gi"),
     HS_MODEL_LIF_NOISE: ("V",),
     HS_MODEL_HH_COBA: ("v", "m", "n", "h", "ge", "gi"),
+    HS_MODEL_SPIKEGEN: ("_unused",),
 }
+HS_STDP_NPARAMS = 7  # taupre, taupost, dApre, dApost, wmin, wmax, scale
+HS_SYN_W, HS_SYN_APRE, HS_SYN_APOST, HS_SYN_LASTUPDATE = 0, 1, 2, 3
 NEVER_SPIKED = -1073741824
 
 OPT_UPDATE_BLOCK, OPT_DELIVER_GRID, OPT_GRAPH_CHUNK, OPT_FUSED_GRID, OPT_WINDOW, OPT_DELIVERY = 0, 1, 2, 3, 4, 5
@@ -230,6 +233,26 @@ class Net:
              float(p), int(bool(autapses)), int(self.seed if seed is None else seed), int(stream), float(dlo),
              float(dhi), int(wmode), float(wlo), float(whi), C.byref(pid))
         return pid.value
+
+    def set_spikegen(self, gid, indices, steps):
+        """SpikeGeneratorGroup pattern: neuron indices[k] fires at step steps[k]."""
+        i = np.ascontiguousarray(indices, dtype=np.int32)
+        s = np.ascontiguousarray(steps, dtype=np.int64)
+        if i.size != s.size:
+            raise ValueError("indices/steps size mismatch")
+        call("hs_net_set_spikegen", self.h, gid, i.size, _i32(i), s.ctypes.data_as(C.POINTER(C.c_int64)))
+
+    def pathway_stdp(self, pid, taupre, taupost, dApre, dApost, wmin=-np.inf, wmax=np.inf, scale=1.0):
+        """STDP with event-driven traces on a per-synapse-weight pathway (plain form)."""
+        p = np.array([taupre, taupost, dApre, dApost, wmin, wmax, scale], dtype=np.float64)
+        call("hs_net_pathway_stdp", self.h, pid, _f64(p), p.size)
+
+    def synapse_var(self, pid, var):
+        """A synapse variable (HS_SYN_*) in bundle order (the order of synapses())."""
+        n = self.pathway_info(pid)["nsyn"]
+        out = np.empty(max(n, 1))
+        call("hs_net_get_synapse_var", self.h, pid, int(var), _f64(out), n)
+        return out[:n]
 
     def pathway_info(self, pid):
         n = C.c_int64()

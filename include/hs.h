@@ -184,6 +184,31 @@ int hs_net_pathway_info(hs_net *net, int pid, int64_t *nsyn, int *dmin, int *D, 
 int hs_net_get_synapses(hs_net *net, int pid, int32_t *pre, int32_t *post, int32_t *delay_steps, double *w,
                         int64_t cap);
 
+/*
+ * SpikeGeneratorGroup pattern of a HS_MODEL_SPIKEGEN group: neuron indices[k]
+ * fires at step steps[k] (host arrays of n; a (neuron, step) pair at most once),
+ * repeated every period steps if the group's period parameter is > 0.  May be
+ * replaced between runs.  Replaces the generator's arrays and its per-step kernel
+ * (brian2cuda/templates/spikegenerator.cu:43-80): the update launch tests each
+ * neuron against the step's sorted list.
+ */
+int hs_net_set_spikegen(hs_net *net, int gid, int64_t n, const int32_t *indices, const int64_t *steps);
+/*
+ * Make a pathway plastic: spike-timing-dependent plasticity with event-driven
+ * traces (hs_models.h, HS_STDP_NPARAMS parameters), the synapse model of
+ * brian2cuda/tests/test_cpp_cuda_consistency.py:45-61.  Needs a scatter pathway
+ * with per-synapse weights and the plain form; before the first run.  The
+ * pathway then runs its on_pre code (target += w*scale, trace and weight update)
+ * and an on_post pathway over the target group's spikes, after the step's other
+ * deliveries (Brian2 orders the pre pathway before the post pathway).
+ * Replaces the pre- and post-pathway effect kernels of synapses.cu:12-164 for
+ * such synapses.
+ */
+int hs_net_pathway_stdp(hs_net *net, int pid, const double *params, int nparams);
+/* a synapse variable (HS_SYN_W, _APRE, _APOST, _LASTUPDATE) in bundle order
+   (the order of hs_net_get_synapses); cap >= nsyn */
+int hs_net_get_synapse_var(hs_net *net, int pid, int var, double *out, int64_t cap);
+
 /* SpikeMonitor on a group: capacity = device records kept (i, step). */
 int hs_net_add_spike_monitor(hs_net *net, int gid, int64_t capacity, int *mid);
 /* StateMonitor (when='start') of var over neurons [start, stop). */

@@ -29,11 +29,15 @@
  *   dv/dt  = (ge+gi-(v-El))/taum : volt (unless refractory)
  *   dge/dt = -ge/taue ; dgi/dt = -gi/taui
  *   threshold v>Vt, reset v=Vr, refractory
- * params: El, Vt, Vr, taum, taue, taui, refractory
+ * params: El, Vt, Vr, taum, taue, taui, refractory[, free]
+ *   free (optional, default 0): 1 drops "(unless refractory)" -- v integrates
+ *   while refractory and refractoriness only blocks the threshold, as in the
+ *   STDP networks of brian2cuda/tests/test_cpp_cuda_consistency.py:41-44
  */
 #define HS_MODEL_LIF_CUBA 1
 #define HS_LIF_CUBA_NVARS 3
 #define HS_LIF_CUBA_NPARAMS 7
+#define HS_LIF_CUBA_NPARAMS_MAX 8
 #define HS_V 0
 #define HS_GE 1
 #define HS_GI 2
@@ -66,7 +70,34 @@
 #define HS_HH_GE 4
 #define HS_HH_GI 5
 
+/*
+ * HS_MODEL_SPIKEGEN -- SpikeGeneratorGroup (brian2cuda/templates/spikegenerator.cu:43-80):
+ *   neuron i spikes at step s for every (i, s) given to hs_net_set_spikegen;
+ *   with a period of P steps the pattern repeats (step t fires the spikes of
+ *   t mod P).  No state: one placeholder variable; no reset, no refractoriness.
+ * params: period (seconds; 0: no repetition)
+ */
+#define HS_MODEL_SPIKEGEN 4
+#define HS_SPIKEGEN_NVARS 1
+#define HS_SPIKEGEN_NPARAMS 1
+
 #define HS_MAX_VARS 6
+
+/*
+ * Spike-timing-dependent plasticity on a pathway (hs_net_pathway_stdp), the
+ * synapse model of brian2cuda/tests/test_cpp_cuda_consistency.py:45-61:
+ *   dApre/dt = -Apre/taupre (event-driven); dApost/dt = -Apost/taupost (event-driven); w
+ *   on_pre : target += w*scale; Apre += dApre; w = clip(w + Apost, wmin, wmax)
+ *   on_post: Apost += dApost;   w = clip(w + Apre, wmin, wmax)
+ * Before either, both traces decay from the synapse's last update to now.
+ * params: taupre, taupost, dApre, dApost, wmin, wmax, scale
+ * synapse variables (hs_net_get_synapse_var): w, Apre, Apost, lastupdate (step)
+ */
+#define HS_STDP_NPARAMS 7
+#define HS_SYN_W 0
+#define HS_SYN_APRE 1
+#define HS_SYN_APOST 2
+#define HS_SYN_LASTUPDATE 3
 
 /* "never spiked" marker for the integer last-spike step */
 #define HS_NEVER_SPIKED (-1073741824)

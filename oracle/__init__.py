@@ -53,7 +53,10 @@ def lib():
         "ref_logf": (C.c_float, [C.c_float]),
         "ref_sqrtf": (C.c_float, [C.c_float]),
         "ref_cos2pif": (C.c_float, [C.c_float]),
-        "ref_model_constants": (C.c_int, [C.c_int, DP, C.c_double, DP]),
+        "ref_model_constants": (C.c_int, [C.c_int, DP, C.c_int, C.c_double, DP]),
+        "ref_net_set_spikegen": (C.c_int, [V, C.c_int, C.c_int64, I32P, I64P]),
+        "ref_net_pathway_stdp": (C.c_int, [V, C.c_int, DP, C.c_int]),
+        "ref_net_get_synapse_var": (C.c_int, [V, C.c_int, C.c_int, DP]),
         "ref_stateupdate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(DP), I32P, DP, C.c_double,
                                       C.c_uint64, C.c_int64]),
         "ref_net_create": (V, [C.c_int, C.c_int, C.c_double, C.c_uint64]),
@@ -124,8 +127,10 @@ def randn(seed, stream, index, step, precision=64):
 def model_constants(model, params, dt):
     p = np.ascontiguousarray(params, dtype=np.float64)
     k = np.zeros(16)
-    n = lib().ref_model_constants(model, p.ctypes.data_as(C.POINTER(C.c_double)), dt,
+    n = lib().ref_model_constants(model, p.ctypes.data_as(C.POINTER(C.c_double)), p.size, dt,
                                   k.ctypes.data_as(C.POINTER(C.c_double)))
+    if n < 0:
+        raise ValueError("bad model or parameter count")
     return k[:n]
 
 
@@ -135,7 +140,8 @@ def stateupdate(model, precision, arrays, last, params, dt, seed, step):
     DP = C.POINTER(C.c_double)
     ptrs = (DP * len(arrays))(*[a.ctypes.data_as(DP) for a in arrays])
     last = np.ascontiguousarray(last, dtype=np.int32)
-    p = np.ascontiguousarray(params, dtype=np.float64)
+    p = np.zeros(16)  # optional trailing parameters default to 0
+    p[:len(params)] = params
     r = lib().ref_stateupdate(model, precision, arrays[0].size, ptrs, last.ctypes.data_as(C.POINTER(C.c_int32)),
                               p.ctypes.data_as(DP), dt, seed, step)
     if r < 0:
@@ -204,6 +210,27 @@ class RefNet:
             raise ValueError(f"bad pathway ({r})")
         return r
 
+    def set_spikegen(self, gid, indices, steps):
+        i = np.ascontiguousarray(indices, dtype=np.int32)
+        s = np.ascontiguousarray(steps, dtype=np.int64)
+        r = self.L.ref_net_set_spikegen(self.h, gid, i.size, i.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        s.ctypes.data_as(C.POINTER(C.c_int64)))
+        if r < 0:
+            raise ValueError(f"bad spike generator pattern ({r})")
+
+    def pathway_stdp(self, pid, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        r = self.L.ref_net_pathway_stdp(self.h, pid, p.ctypes.data_as(C.POINTER(C.c_double)), p.size)
+        if r < 0:
+            raise ValueError(f"bad plasticity ({r})")
+
+    def synapse_var(self, pid, var, nsyn):
+        """a synapse variable (HS_SYN_*) in synapse (creation) order"""
+        out = np.empty(nsyn)
+        if self.L.ref_net_get_synapse_var(self.h, pid, var, out.ctypes.data_as(C.POINTER(C.c_double))) < 0:
+            raise ValueError("bad synapse variable")
+        return out
+
     def add_spike_monitor(self, gid):
         return self.L.ref_net_add_spike_monitor(self.h, gid)
 
@@ -255,14 +282,18 @@ def build_ref(spec, precision=64, semantics=SEQ):
         gid = net.add_group(g.model, g.n, g.params)
         for var, val in g.init.items():
             net.set_state(gid, var, val)
+        if getattr(g, "spikes", None) is not None:
+            net.set_spikegen(gid, *g.spikes)
     for p in spec.paths:
         if p.random is not None:
             syn = random_pathway(spec, p)
-            net.add_pathway(p.src, p.tgt, p.var, syn["pre"], syn["post"], syn["delay_steps"], 0, p.random.wmode,
-                            syn["w"], p.random.w[0])
+            pid = net.add_pathway(p.src, p.tgt, p.var, syn["pre"], syn["post"], syn["delay_steps"], 0,
+                                  p.random.wmode, syn["w"], p.random.w[0])
         else:
-            net.add_pathway(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode, p.weights,
-                            p.wscalar)
+            pid = net.add_pathway(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode,
+                                  p.weights, p.wscalar)
+        if getattr(p, "stdp", None) is not None:
+            net.pathway_stdp(pid, p.stdp)
     smons = [net.add_spike_monitor(g) for g in spec.spike_monitors]
     stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
     return net, smons, stmons

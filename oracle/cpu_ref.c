@@ -188,9 +188,11 @@ float ref_logf(float x) { return hs_logf(x); }
 float ref_sqrtf(float x) { return hs_sqrtf(x); }
 float ref_cos2pif(float u) { return hs_cos2pif(u); }
 
-int ref_model_constants(int model, const double *params, double dt, double *k)
+int ref_model_constants(int model, const double *params, int nparams, double dt, double *k)
 {
-    return hs_model_constants(model, params, dt, k);
+    double p[16];
+    if (hs_model_params(model, params, nparams, p) < 0) return -1;
+    return hs_model_constants(model, p, dt, k);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -216,6 +218,12 @@ typedef struct {
     int64_t nspk;
     int32_t *pend;       /* spikes whose reset is pending (ACC) */
     int64_t npend;
+    /* SpikeGeneratorGroup: neurons gen_idx[gen_off[s] .. gen_off[s+1]) fire at
+       step s (ascending), s < gen_steps; the pattern repeats every gen_period
+       steps (0: once) */
+    int64_t *gen_off;
+    int32_t *gen_idx;
+    int64_t gen_steps, gen_period;
 } ref_group;
 
 typedef struct {
@@ -230,6 +238,14 @@ typedef struct {
     int Q, offset;
     int64_t **q;
     int64_t *qlen, *qcap;
+    /* STDP (plastic != 0): traces and last update step per synapse, and the
+       synapses of each postsynaptic neuron (CSR, synapse order) for on_post */
+    int plastic;
+    double taupre, taupost;
+    double sk[5]; /* dApre, dApost, wmin, wmax, scale */
+    void *apre, *apost;
+    int64_t *lastup;
+    int64_t *post_start, *by_post;
 } ref_path;
 
 typedef struct {
@@ -276,6 +292,7 @@ static int model_nvars(int model)
     if (model == HS_MODEL_LIF_CUBA) return HS_LIF_CUBA_NVARS;
     if (model == HS_MODEL_LIF_NOISE) return HS_LIF_NOISE_NVARS;
     if (model == HS_MODEL_HH_COBA) return HS_HH_NVARS;
+    if (model == HS_MODEL_SPIKEGEN) return HS_SPIKEGEN_NVARS;
     return -1;
 }
 
@@ -283,19 +300,23 @@ int ref_net_add_group(ref_net *net, int model, int n, const double *params, int 
 {
     if (net->ng >= REF_MAX_GROUPS || n <= 0) return -1;
     int nv = model_nvars(model);
-    if (nv < 0 || nparams > 16) return -1;
+    if (nv < 0) return -1;
     ref_group *g = &net->g[net->ng];
     memset(g, 0, sizeof(*g));
+    if (hs_model_params(model, params, nparams, g->p) < 0) return -1;
+    params = g->p; /* every parameter, optional ones filled in */
     g->model = model;
     g->n = n;
     g->nvars = nv;
-    memcpy(g->p, params, sizeof(double) * nparams);
     g->nk = hs_model_constants(model, params, net->dt, g->k);
     if (model == HS_MODEL_LIF_CUBA) {
         g->vth = params[1]; g->vreset = params[2]; g->refractory = params[6]; g->has_reset = 1;
     } else if (model == HS_MODEL_LIF_NOISE) {
         g->vth = params[3]; g->vreset = params[4]; g->refractory = params[5]; g->has_reset = 1;
         g->stream = (uint32_t)params[6];
+    } else if (model == HS_MODEL_SPIKEGEN) {
+        g->refractory = 0.0; g->has_reset = 0;
+        g->gen_period = hs_timestep(params[0], net->dt);
     } else {
         g->vth = params[12]; g->refractory = params[13]; g->has_reset = 0;
     }
@@ -382,6 +403,84 @@ int ref_net_add_pathway(ref_net *net, int src, int tgt, int var, int64_t nsyn, c
     p->qcap = (int64_t *)calloc(p->Q, sizeof(int64_t));
     if (!gt->acc[var]) gt->acc[var] = calloc((size_t)gt->n, rsz(net));
     return net->np++;
+}
+
+/* spikegenerator.cu:43-80: neuron idx[k] fires at step steps[k]; a pair given
+   twice is an error (one spike per neuron and step) */
+int ref_net_set_spikegen(ref_net *net, int gid, int64_t n, const int32_t *idx, const int64_t *steps)
+{
+    if (gid < 0 || gid >= net->ng || net->g[gid].model != HS_MODEL_SPIKEGEN || n < 0) return -1;
+    ref_group *g = &net->g[gid];
+    int64_t smax = -1;
+    for (int64_t k = 0; k < n; ++k) {
+        if (idx[k] < 0 || idx[k] >= g->n || steps[k] < 0) return -2;
+        if (steps[k] > smax) smax = steps[k];
+    }
+    free(g->gen_off);
+    free(g->gen_idx);
+    g->gen_steps = smax + 1;
+    g->gen_off = (int64_t *)calloc((size_t)g->gen_steps + 1, sizeof(int64_t));
+    g->gen_idx = (int32_t *)malloc(sizeof(int32_t) * (n ? n : 1));
+    for (int64_t k = 0; k < n; ++k) g->gen_off[steps[k] + 1]++;
+    for (int64_t s = 0; s < g->gen_steps; ++s) g->gen_off[s + 1] += g->gen_off[s];
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)(g->gen_steps ? g->gen_steps : 1));
+    memcpy(fill, g->gen_off, sizeof(int64_t) * (size_t)g->gen_steps);
+    for (int64_t k = 0; k < n; ++k) {
+        /* insertion into the step's ascending list */
+        int64_t pos = fill[steps[k]]++;
+        while (pos > g->gen_off[steps[k]] && g->gen_idx[pos - 1] > idx[k]) {
+            g->gen_idx[pos] = g->gen_idx[pos - 1];
+            --pos;
+        }
+        if (pos > g->gen_off[steps[k]] && g->gen_idx[pos - 1] == idx[k]) {
+            free(fill);
+            return -3;
+        }
+        g->gen_idx[pos] = idx[k];
+    }
+    free(fill);
+    return 0;
+}
+
+/* STDP on a pathway with per-synapse weights (SEQ semantics only) */
+int ref_net_pathway_stdp(ref_net *net, int pid, const double *params, int nparams)
+{
+    if (pid < 0 || pid >= net->np || nparams != HS_STDP_NPARAMS || net->sem != 0) return -1;
+    ref_path *p = &net->pw[pid];
+    if (p->wmode != HS_W_SYNAPSE || p->plastic) return -2;
+    const ref_group *gt = &net->g[p->tgt];
+    p->plastic = 1;
+    p->taupre = params[0];
+    p->taupost = params[1];
+    for (int j = 0; j < 5; ++j) p->sk[j] = params[2 + j];
+    const int64_t n = p->nsyn ? p->nsyn : 1;
+    p->apre = calloc((size_t)n, rsz(net));
+    p->apost = calloc((size_t)n, rsz(net));
+    p->lastup = (int64_t *)calloc((size_t)n, sizeof(int64_t)); /* lastupdate = 0 s */
+    p->post_start = (int64_t *)calloc((size_t)gt->n + 1, sizeof(int64_t));
+    for (int64_t s = 0; s < p->nsyn; ++s) p->post_start[p->post[s] + 1]++;
+    for (int i = 0; i < gt->n; ++i) p->post_start[i + 1] += p->post_start[i];
+    p->by_post = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)gt->n);
+    memcpy(fill, p->post_start, sizeof(int64_t) * (size_t)gt->n);
+    for (int64_t s = 0; s < p->nsyn; ++s) p->by_post[fill[p->post[s]]++] = s;
+    free(fill);
+    return 0;
+}
+
+/* synapse variable in synapse (creation) order: 0 w, 1 Apre, 2 Apost, 3 lastupdate (step) */
+int ref_net_get_synapse_var(ref_net *net, int pid, int var, double *out)
+{
+    if (pid < 0 || pid >= net->np) return -1;
+    ref_path *p = &net->pw[pid];
+    for (int64_t s = 0; s < p->nsyn; ++s) {
+        if (var == HS_SYN_W) out[s] = p->wmode == HS_W_SYNAPSE ? rget(net, p->w, s) : p->wscalar;
+        else if (var == HS_SYN_APRE && p->plastic) out[s] = rget(net, p->apre, s);
+        else if (var == HS_SYN_APOST && p->plastic) out[s] = rget(net, p->apost, s);
+        else if (var == HS_SYN_LASTUPDATE && p->plastic) out[s] = (double)p->lastup[s];
+        else return -2;
+    }
+    return 0;
 }
 
 int ref_net_add_spike_monitor(ref_net *net, int gid)
@@ -512,12 +611,14 @@ void ref_net_destroy(ref_net *net)
     for (int g = 0; g < net->ng; ++g) {
         for (int v = 0; v < HS_MAX_VARS; ++v) { free(net->g[g].var[v]); free(net->g[g].acc[v]); }
         free(net->g[g].lastspike); free(net->g[g].spk); free(net->g[g].pend);
+        free(net->g[g].gen_off); free(net->g[g].gen_idx);
     }
     for (int p = 0; p < net->np; ++p) {
         ref_path *pw = &net->pw[p];
         free(pw->pre); free(pw->post); free(pw->delay); free(pw->pre_start); free(pw->by_pre); free(pw->w);
         for (int q = 0; q < pw->Q; ++q) free(pw->q[q]);
         free(pw->q); free(pw->qlen); free(pw->qcap);
+        free(pw->apre); free(pw->apost); free(pw->lastup); free(pw->post_start); free(pw->by_post);
     }
     for (int m = 0; m < net->nm; ++m) { free(net->mon[m].i); free(net->mon[m].step); free(net->mon[m].data); }
     free(net);

@@ -29,7 +29,7 @@ double ref_cos2pi(double u);
 float ref_logf(float x);
 float ref_sqrtf(float x);
 float ref_cos2pif(float u);
-int ref_model_constants(int model, const double *params, double dt, double *k);
+int ref_model_constants(int model, const double *params, int nparams, double dt, double *k);
 int ref_stateupdate(int model, int precision, int n, double **vars, const int32_t *last, const double *params,
                     double dt, uint64_t seed, int64_t step);
 
@@ -42,6 +42,9 @@ int ref_net_get_lastspike(ref_net *net, int gid, double *out);
 int ref_net_add_pathway(ref_net *net, int src, int tgt, int var, int64_t nsyn, const int32_t *pre,
                         const int32_t *post, const int32_t *delay_steps, int scalar_delay,
                         int wmode, const double *weights, double wscalar);
+int ref_net_set_spikegen(ref_net *net, int gid, int64_t n, const int32_t *idx, const int64_t *steps);
+int ref_net_pathway_stdp(ref_net *net, int pid, const double *params, int nparams);
+int ref_net_get_synapse_var(ref_net *net, int pid, int var, double *out);
 int ref_net_add_spike_monitor(ref_net *net, int gid);
 int ref_net_add_state_monitor(ref_net *net, int gid, int var, int start, int stop);
 int ref_net_run(ref_net *net, int64_t nsteps);

@@ -1,0 +1,149 @@
+"""GPU parity of spike generators and STDP pathways (plain form) against the oracle.
+
+Bit-exact where the arithmetic is order-free: a generator's spikes, and a
+feed-forward STDP network whose input fires at most once per step (each target
+then receives at most one plastic effect per step, so no two floating-point
+additions meet on one target).  The reference's recurrent STDP networks
+(test_cpp_cuda_consistency.py) sum several effects into one conductance in the
+same step in atomic order, as the reference's CUDA kernels do; they are compared
+the way that test compares cpp_standalone with cuda_standalone: same spikes,
+weights and voltages within a relative tolerance.
+"""
+import numpy as np
+import pytest
+
+from brian2cuda_amd import networks as nw
+from brian2cuda_amd import runtime as rt
+from brian2cuda_amd._lib import HsError
+
+pytestmark = pytest.mark.gpu
+
+DT = 1e-4
+
+
+def _bundle_order(spec, path):
+    """permutation from synapse (creation) order to the device's bundle order"""
+    d = path.delay_steps if path.delay_steps is not None else np.full(path.nsyn, path.scalar_delay)
+    return np.lexsort((d, path.pre))
+
+
+@pytest.mark.parametrize("engine", ["plain", "fused"])
+def test_spikegen_pattern(hs, oracle_lib, engine):
+    rng = np.random.default_rng(3)
+    n = 700
+    idx = rng.integers(0, n, 4000)
+    steps = rng.integers(0, 900, 4000)
+    key = np.unique(steps * n + idx)
+    idx, steps = (key % n).astype(np.int32), (key // n).astype(np.int64)
+    for period in (0.0, 500 * DT):
+        spec = nw.NetSpec("gen", DT, 1, [nw.GroupSpec(rt.HS_MODEL_SPIKEGEN, n, [period], spikes=(idx, steps))], [],
+                          spike_monitors=[0])
+        net, sm, _ = nw.build_hs(spec, 32, engine)
+        ref, rsm, _ = oracle_lib.build_ref(spec, 32, oracle_lib.SEQ)
+        net.run(1200)
+        ref.run(1200)
+        gi, gt = net.spikes(sm[0])
+        ri, rt_ = ref.spikes(rsm[0])
+        assert np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+        if period == 0.0:
+            assert len(gi) == idx.size
+        net.close()
+
+
+def _feedforward(delay_steps, n_in=100, n_out=60, seed=5):
+    """generator (one spike per step at most) -> LIF, all-to-all STDP"""
+    rs = np.random.RandomState(seed)
+    times = rs.choice(np.arange(3000), 1500, replace=False)
+    src = rs.randint(0, n_in, times.size)
+    G = nw.GroupSpec(rt.HS_MODEL_SPIKEGEN, n_in, [0.0], spikes=(src.astype(np.int32), times.astype(np.int64)))
+    P = nw.GroupSpec(rt.HS_MODEL_LIF_CUBA, n_out, [-49e-3, -50e-3, -60e-3, 20e-3, 5e-3, 5e-3, 5e-3, 1.0],
+                     {0: -60e-3 + 10e-3 * rs.rand(n_out), 1: 0.0})
+    ii, jj = np.meshgrid(np.arange(n_in), np.arange(n_out), indexing="ij")
+    S = nw.PathSpec(0, 1, 1, ii.ravel().astype(np.int32), jj.ravel().astype(np.int32), scalar_delay=delay_steps,
+                    wmode=rt.HS_W_SYNAPSE, weights=1.5 + 2.0 * rs.rand(n_in * n_out),
+                    stdp=(20e-3, 20e-3, 0.05, -0.055, 0.0, 4.0, 1e-3))
+    return nw.NetSpec("stdp_ff", DT, seed, [G, P], [S], spike_monitors=[1], state_monitors=[(1, 0, 0, 20)])
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("delay", [0, 3])
+def test_stdp_feedforward_bit_exact(hs, oracle_lib, prec, delay):
+    spec = _feedforward(delay)
+    net, sm, st = nw.build_hs(spec, prec, "plain")
+    ref, rsm, rst = oracle_lib.build_ref(spec, prec, oracle_lib.SEQ)
+    S = spec.paths[0]
+    perm = _bundle_order(spec, S)
+    for k in (700, 1100, 1300):
+        net.run(k)
+        ref.run(k)
+        for var in (rt.HS_SYN_W, rt.HS_SYN_APRE, rt.HS_SYN_APOST, rt.HS_SYN_LASTUPDATE):
+            np.testing.assert_array_equal(net.synapse_var(0, var), ref.synapse_var(0, var, S.nsyn)[perm],
+                                          err_msg=f"synapse variable {var} after {net.stats()['steps']} steps")
+    gi, gt = net.spikes(sm[0])
+    ri, rt_ = ref.spikes(rsm[0])
+    assert len(gi) > 50
+    assert np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    assert np.array_equal(net.state_monitor(st[0], 20), ref.state_monitor(rst[0], 20))
+    assert net.stats()["events"] == ref.events
+    net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("heterog", [False, True])
+def test_reference_stdp_examples(hs, oracle_lib, prec, heterog):
+    """test_stdp_example / test_stdp_heterog_delays_example: w of synapses 0..9
+    at t = 0 and 0.1 s, v of neurons 0..9 every step, spike count and rate."""
+    spec = nw.stdp_example(heterog)
+    net, sm, st = nw.build_hs(spec, prec, "plain")
+    ref, rsm, rst = oracle_lib.build_ref(spec, prec, oracle_lib.SEQ)
+    S = spec.paths[1]
+    perm = _bundle_order(spec, S)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)  # creation index -> bundle position
+    rtol = 1e-9
+    for rec in range(2):  # StateMonitor(S, 'w', record=range(10), dt=0.1*second): t = 0, 0.1 s
+        gw, rw = net.synapse_var(1, rt.HS_SYN_W)[inv[:10]], ref.synapse_var(1, 0, S.nsyn)[:10]
+        if prec == 64 or rec == 0:
+            np.testing.assert_allclose(gw, rw, rtol=rtol if prec == 64 else 1e-7)
+        else:  # FP32: a spike moved by a step moves w by about one trace increment (0.03)
+            assert np.mean(np.abs(gw - rw)) < 0.02
+        net.run(1000)
+        ref.run(1000)
+    gi, gt = net.spikes(sm[0])
+    ri, rt_ = ref.spikes(rsm[0])
+    if prec == 64:
+        assert np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    else:
+        assert abs(len(gi) - len(ri)) <= 0.02 * len(ri)
+    rate_g = np.bincount(gt, minlength=2000) / (100 * DT)
+    rate_r = np.bincount(rt_, minlength=2000) / (100 * DT)
+    if prec == 64:
+        np.testing.assert_array_equal(rate_g, rate_r)
+        np.testing.assert_allclose(net.state_monitor(st[0], 10), ref.state_monitor(rst[0], 10), rtol=rtol,
+                                   atol=1e-12)
+        np.testing.assert_allclose(net.synapse_var(1, rt.HS_SYN_W), ref.synapse_var(1, 0, S.nsyn)[perm], rtol=rtol,
+                                   atol=1e-12)
+    net.close()
+
+
+def test_stdp_rejects_unsupported(hs):
+    spec = _feedforward(0)
+    net = rt.Net(64, "fused", DT, 1)
+    for g in spec.groups:
+        net.add_group(g.model, g.n, g.params)
+    S = spec.paths[0]
+    pid = net.add_pathway(S.src, S.tgt, S.var, S.pre, S.post, None, 0, S.wmode, S.weights, 0.0)
+    with pytest.raises(HsError):
+        net.pathway_stdp(pid, *S.stdp)  # fused form
+    net.close()
+    net = rt.Net(64, "plain", DT, 1)
+    for g in spec.groups:
+        net.add_group(g.model, g.n, g.params)
+    pid = net.add_pathway(S.src, S.tgt, S.var, S.pre, S.post, None, 0, rt.HS_W_SCALAR, None, 1.0)
+    with pytest.raises(HsError):
+        net.pathway_stdp(pid, *S.stdp)  # scalar weight
+    with pytest.raises(HsError):
+        net.set_spikegen(0, [1, 1], [5, 5])  # twice in one step
+    with pytest.raises(HsError):
+        net.set_spikegen(1, [1], [5])  # not a generator
+    net.close()
