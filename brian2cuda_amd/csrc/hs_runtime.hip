@@ -1679,6 +1679,74 @@ extern "C" int hs_net_get_spikes(hs_net *net, int mid, int32_t *i, int64_t *step
     return HS_OK;
 }
 
+/* raw little-endian array file, as _write_arrays writes each array (objects.cu:550-563) */
+static int hs_write_raw(const char *path, const void *data, size_t bytes)
+{
+    if (!path) return HS_OK;
+    FILE *f = fopen(path, "wb");
+    if (!f) return hs_fail(HS_ERR_ARG, "cannot open %s for writing", path);
+    const size_t put = bytes ? fwrite(data, 1, bytes, f) : 0;
+    const int closed = fclose(f);
+    if (put != bytes || closed != 0) return hs_fail(HS_ERR_ARG, "short write to %s", path);
+    return HS_OK;
+}
+
+extern "C" int hs_net_write_monitor(hs_net *net, int mid, const char *path_a, const char *path_b,
+                                    const char *path_c, const char *path_d)
+{
+    if (!net || mid < 0 || mid >= (int)net->m.size()) return hs_fail(HS_ERR_ARG, "bad monitor");
+    const HMon &M = net->m[mid];
+    int64_t n = 0;
+    HS_TRY(hs_net_mon_count(net, mid, &n));
+    if (M.kind == 0) {
+        /* SpikeMonitor: i (int32), t (double seconds), count (int32 per neuron), N */
+        std::vector<int32_t> i((size_t)(n ? n : 1));
+        std::vector<int64_t> st((size_t)(n ? n : 1));
+        HS_TRY(hs_net_get_spikes(net, mid, i.data(), st.data(), n));
+        std::vector<double> t((size_t)(n ? n : 1));
+        std::vector<int32_t> count((size_t)net->g[M.gid].n, 0);
+        for (int64_t r = 0; r < n; ++r) {
+            t[(size_t)r] = (double)st[(size_t)r] * net->dt;
+            count[(size_t)i[(size_t)r]]++;
+        }
+        const int32_t N = (int32_t)n;
+        HS_TRY(hs_write_raw(path_a, i.data(), sizeof(int32_t) * (size_t)n));
+        HS_TRY(hs_write_raw(path_b, t.data(), sizeof(double) * (size_t)n));
+        HS_TRY(hs_write_raw(path_c, count.data(), sizeof(int32_t) * count.size()));
+        HS_TRY(hs_write_raw(path_d, &N, sizeof(N)));
+        return HS_OK;
+    }
+    /* StateMonitor: t (double), values (the variable's type, time-major rows of
+       the recorded indices, objects.cu:589-595), indices (int32), N (rows) */
+    const int64_t w = M.stop - M.start;
+    std::vector<double> t((size_t)(n ? n : 1));
+    for (int64_t r = 0; r < n; ++r) t[(size_t)r] = (double)(M.t0 + r) * net->dt;
+    std::vector<unsigned char> vals(hs_rsize(net) * (size_t)(n * w ? n * w : 1));
+    if (n * w) {
+        HS_HIP(hipMemcpyAsync(vals.data(), M.buf, hs_rsize(net) * (size_t)(n * w), hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    }
+    std::vector<int32_t> idx((size_t)(w ? w : 1));
+    for (int64_t k = 0; k < w; ++k) idx[(size_t)k] = (int32_t)(M.start + k);
+    const int32_t N = (int32_t)n;
+    HS_TRY(hs_write_raw(path_a, t.data(), sizeof(double) * (size_t)n));
+    HS_TRY(hs_write_raw(path_b, vals.data(), hs_rsize(net) * (size_t)(n * w)));
+    HS_TRY(hs_write_raw(path_c, idx.data(), sizeof(int32_t) * (size_t)w));
+    HS_TRY(hs_write_raw(path_d, &N, sizeof(N)));
+    return HS_OK;
+}
+
+extern "C" int hs_net_write_last_run_info(hs_net *net, const char *path)
+{
+    if (!net || !path) return hs_fail(HS_ERR_ARG, "bad arguments");
+    FILE *f = fopen(path, "w");
+    if (!f) return hs_fail(HS_ERR_ARG, "cannot open %s for writing", path);
+    /* Network::_last_run_time and _last_run_completed_fraction as std::ostream prints them */
+    fprintf(f, "%g %g\n", net->last_wall, net->last_steps > 0 || net->step > 0 ? 1.0 : 0.0);
+    if (fclose(f) != 0) return hs_fail(HS_ERR_ARG, "short write to %s", path);
+    return HS_OK;
+}
+
 extern "C" int hs_net_get_state_monitor(hs_net *net, int mid, double *out, int64_t cap)
 {
     if (!net || mid < 0 || mid >= (int)net->m.size() || net->m[mid].kind != 1) return hs_fail(HS_ERR_ARG, "bad monitor");

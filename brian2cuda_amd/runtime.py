@@ -254,6 +254,16 @@ class Net:
         call("hs_net_get_synapse_var", self.h, pid, int(var), _f64(out), n)
         return out[:n]
 
+    def write_monitor(self, mid, paths):
+        """Write a monitor's result files (raw little-endian arrays, the
+        reference's format): paths = (a, b, c, d) as in hs_net_write_monitor,
+        None to skip one."""
+        enc = [None if p is None else str(p).encode() for p in paths]
+        call("hs_net_write_monitor", self.h, mid, *enc)
+
+    def write_last_run_info(self, path):
+        call("hs_net_write_last_run_info", self.h, str(path).encode())
+
     def pathway_info(self, pid):
         n = C.c_int64()
         dmin, D, mode = C.c_int(), C.c_int(), C.c_int()

@@ -258,6 +258,21 @@ int hs_net_mon_count(hs_net *net, int mid, int64_t *n);
 int hs_net_get_spikes(hs_net *net, int mid, int32_t *i, int64_t *step, int64_t cap);
 /* State samples, time-major [n_steps][stop-start], as double. */
 int hs_net_get_state_monitor(hs_net *net, int mid, double *out, int64_t cap);
+/*
+ * Write a monitor's arrays as the reference's result files: one raw
+ * little-endian array per file, as _write_arrays does
+ * (brian2cuda/templates/objects.cu:550-563; 2-D monitor arrays time-major, one
+ * row of the recorded indices per recorded step, :589-595).  NULL path: skipped.
+ *   spike monitor: a = i (int32[N]), b = t (double[N], seconds), c = count
+ *                  (int32 per neuron of the group), d = N (int32[1]);
+ *                  records in cpp_standalone order (by step, then index)
+ *   state monitor: a = t (double[rows]), b = values (the network's precision,
+ *                  [rows][stop-start]), c = recorded indices (int32), d = rows (int32[1])
+ */
+int hs_net_write_monitor(hs_net *net, int mid, const char *path_a, const char *path_b, const char *path_c,
+                         const char *path_d);
+/* last_run_info.txt: "<wall seconds of the last run> <completed fraction>" (brian2cuda/templates/objects.cu:631-636) */
+int hs_net_write_last_run_info(hs_net *net, const char *path);
 
 /* ---- multicompartment cells: the Hines engine ----------------------------- */
 
