@@ -912,6 +912,8 @@ extern "C" int hs_net_set_spikegen(hs_net *net, int gid, int64_t n, const int32_
         if (steps[k] < 0 || steps[k] >= 0x7ffffffeLL) return hs_fail(HS_ERR_ARG, "spike %lld: step out of range", (long long)k);
         smax = std::max<long long>(smax, steps[k]);
     }
+    if (g.gen_period > 0 && smax >= g.gen_period)
+        return hs_fail(HS_ERR_ARG, "spike at step %lld is not inside the period of %lld steps", smax, g.gen_period);
     const long long ns = smax + 1;
     std::vector<int32_t> off((size_t)ns + 1, 0), idx((size_t)(n ? n : 1));
     for (int64_t k = 0; k < n; ++k) off[(size_t)steps[k] + 1]++;

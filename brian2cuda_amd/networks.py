@@ -17,7 +17,7 @@ from typing import List, Optional
 import numpy as np
 
 from .runtime import (HS_MODEL_HH_COBA, HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_SPIKEGEN, HS_W_PRE,
-                      HS_W_SCALAR, HS_W_SYNAPSE)
+                      HS_W_SCALAR, HS_W_SYNAPSE, spikegen_steps)
 
 mV, ms, nS = 1e-3, 1e-3, 1e-9
 
@@ -295,7 +295,7 @@ def stdp_example(heterog_delays=False, n_cells=100, seed=42, dt=0.1 * ms):
     delay = rs.rand(n_cells ** 2) * 2 * ms if heterog_delays else None
     P = GroupSpec(HS_MODEL_LIF_CUBA, n_cells, [El, Vt, Vr, taum, taus, taus, 5 * ms, 1.0], {0: v_init, 1: 0.0})
     Q = GroupSpec(HS_MODEL_SPIKEGEN, n_cells, [0.0],
-                  spikes=(sources.astype(np.int32), np.rint(times_ms * ms / dt).astype(np.int64)))
+                  spikes=(sources.astype(np.int32), spikegen_steps(times_ms * ms, dt)))
     ii, jj = np.meshgrid(np.arange(n_cells), np.arange(n_cells), indexing="ij")
     T = PathSpec(1, 0, 1, np.arange(n_cells, dtype=np.int32), np.arange(n_cells, dtype=np.int32),
                  wmode=HS_W_SCALAR, wscalar=10 * fac * mV)

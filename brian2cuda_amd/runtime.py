@@ -25,6 +25,15 @@ MODEL_VARS = {
     HS_MODEL_HH_COBA: ("v", "m", "n", "h", "ge", "gi"),
     HS_MODEL_SPIKEGEN: ("_unused",),
 }
+
+
+def spikegen_steps(times, dt):
+    """Time bins of SpikeGeneratorGroup spike times: Brian2's timestep(),
+    (int)((t + 1e-3*dt)/dt) (brian2cuda/cuda_generator.py:997-1006), so a time
+    just below a bin edge by rounding error still lands in that bin."""
+    return ((np.asarray(times, dtype=np.float64) + 1e-3 * dt) / dt).astype(np.int64)
+
+
 HS_STDP_NPARAMS = 7  # taupre, taupost, dApre, dApost, wmin, wmax, scale
 HS_SYN_W, HS_SYN_APRE, HS_SYN_APOST, HS_SYN_LASTUPDATE = 0, 1, 2, 3
 NEVER_SPIKED = -1073741824

@@ -416,6 +416,7 @@ int ref_net_set_spikegen(ref_net *net, int gid, int64_t n, const int32_t *idx, c
         if (idx[k] < 0 || idx[k] >= g->n || steps[k] < 0) return -2;
         if (steps[k] > smax) smax = steps[k];
     }
+    if (g->gen_period > 0 && smax >= g->gen_period) return -4; /* every spike inside the period */
     free(g->gen_off);
     free(g->gen_idx);
     g->gen_steps = smax + 1;

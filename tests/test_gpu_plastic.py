@@ -36,8 +36,9 @@ def test_spikegen_pattern(hs, oracle_lib, engine):
     key = np.unique(steps * n + idx)
     idx, steps = (key % n).astype(np.int32), (key // n).astype(np.int64)
     for period in (0.0, 500 * DT):
-        spec = nw.NetSpec("gen", DT, 1, [nw.GroupSpec(rt.HS_MODEL_SPIKEGEN, n, [period], spikes=(idx, steps))], [],
-                          spike_monitors=[0])
+        keep = steps < 500 if period > 0 else np.ones(steps.size, bool)  # a pattern lies inside its period
+        spec = nw.NetSpec("gen", DT, 1, [nw.GroupSpec(rt.HS_MODEL_SPIKEGEN, n, [period],
+                                                      spikes=(idx[keep], steps[keep]))], [], spike_monitors=[0])
         net, sm, _ = nw.build_hs(spec, 32, engine)
         ref, rsm, _ = oracle_lib.build_ref(spec, 32, oracle_lib.SEQ)
         net.run(1200)
@@ -146,4 +147,38 @@ def test_stdp_rejects_unsupported(hs):
         net.set_spikegen(0, [1, 1], [5, 5])  # twice in one step
     with pytest.raises(HsError):
         net.set_spikegen(1, [1], [5])  # not a generator
+    net.close()
+
+
+@pytest.mark.parametrize("case", ["extreme_period", "period_repeat", "rounding_first_bin", "rounding_separate"])
+def test_reference_spikegenerator_cases_gpu(hs, case):
+    """The expectations of the reference's test_spikegenerator.py on the device."""
+    n, idx, times, period, steps = {
+        "extreme_period": (5, [0, 1, 2], np.array([0, 1, 2]) * 1e-3, 1e6, 100),
+        "period_repeat": (1, np.zeros(10, int), np.arange(0, 1, 0.1) * 1e-3, 1e-3, 50),
+        "rounding_first_bin": (100, np.arange(100), np.linspace(0, 0.1e-3, 100, endpoint=False), 0.0, 1),
+        "rounding_separate": (1, np.zeros(10000, int), np.arange(10000) * DT, 0.0, 10000),
+    }[case]
+    net = rt.Net(64, "plain", DT, 1)
+    g = net.add_group(rt.HS_MODEL_SPIKEGEN, n, [period])
+    net.set_spikegen(g, idx, rt.spikegen_steps(times, DT))
+    m = net.add_spike_monitor(g)
+    net.run(steps)
+    i, t = net.spikes(m)
+    if case == "extreme_period":
+        assert i.tolist() == [0, 1, 2] and t.tolist() == [0, 10, 20]
+    elif case == "period_repeat":
+        assert np.array_equal(np.bincount(t // 10, minlength=5), [10] * 5)
+    elif case == "rounding_first_bin":
+        assert np.array_equal(np.bincount(i, minlength=100), np.ones(100)) and np.all(t == 0)
+    else:
+        assert np.array_equal(t, np.arange(10000))
+    net.close()
+
+
+def test_spikegen_period_validation_gpu(hs):
+    net = rt.Net(64, "plain", DT, 1)
+    g = net.add_group(rt.HS_MODEL_SPIKEGEN, 2, [10 * DT])
+    with pytest.raises(HsError):
+        net.set_spikegen(g, [0], [10])
     net.close()

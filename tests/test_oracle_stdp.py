@@ -150,3 +150,45 @@ def test_stdp_example_network_runs_and_learns():
         assert len(i) > 100
         w1 = ref.synapse_var(1, HS_SYN_W, 10000)
         assert np.mean(np.abs(w1 - w0)) > 1e-3
+
+
+# ---- the reference's SpikeGeneratorGroup expectations (test_spikegenerator.py) ----
+
+def _gen_run(n, indices, times, period, steps, prec=64):
+    from brian2cuda_amd.runtime import spikegen_steps
+    net = oracle.RefNet(prec, oracle.SEQ, DT, 1)
+    g = net.add_group(HS_MODEL_SPIKEGEN, n, [period])
+    net.set_spikegen(g, indices, spikegen_steps(times, DT))
+    m = net.add_spike_monitor(g)
+    net.run(steps)
+    return net.spikes(m)
+
+
+def test_reference_spikegenerator_extreme_period():
+    """test_spikegenerator_extreme_period: period 1e6 s, spikes at 0, 1, 2 ms."""
+    i, t = _gen_run(5, [0, 1, 2], np.array([0, 1, 2]) * 1e-3, 1e6, 100)
+    assert i.tolist() == [0, 1, 2]
+    np.testing.assert_allclose(t * DT, [0, 1e-3, 2e-3])
+
+
+def test_reference_spikegenerator_period_repeat():
+    """test_spikegenerator_period_repeat: 10 spikes of neuron 0 every 0.1 ms,
+    period 1 ms, five runs of 1 ms -> 10 spikes in each."""
+    i, t = _gen_run(1, np.zeros(10, int), np.arange(0, 1, 0.1) * 1e-3, 1e-3, 50)
+    assert np.array_equal(np.bincount(t // 10, minlength=5), [10] * 5)
+
+
+def test_reference_spikegenerator_rounding():
+    """test_spikegenerator_rounding: 100 times in [0, dt) all fall in the first
+    bin; 10,000 times k*dt fall in 10,000 separate bins."""
+    i, t = _gen_run(100, np.arange(100), np.linspace(0, 0.1e-3, 100, endpoint=False), 0.0, 1)
+    assert np.array_equal(np.bincount(i, minlength=100), np.ones(100))
+    i, t = _gen_run(1, np.zeros(10000, int), np.arange(10000) * DT, 0.0, 10000)
+    assert np.array_equal(t, np.arange(10000))
+
+
+def test_spikegen_rejects_times_outside_the_period():
+    net = oracle.RefNet(64, oracle.SEQ, DT, 1)
+    g = net.add_group(HS_MODEL_SPIKEGEN, 2, [10 * DT])
+    with pytest.raises(ValueError):
+        net.set_spikegen(g, [0], [10])
