@@ -245,6 +245,22 @@ def hines_bench(n_trees, steps, seed):
     return out
 
 
+def measured_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each scaled by what the counter
+    reads for a plain copy of known size), for the workload it was measured on;
+    None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "brunel_c3_deliver_traffic.json")
+    if args.workload != "brunel" or args.N != 100_000 or args.K != 1000 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t["kernel"] != kernel:
+        return None
+    return int(1024 * (t["fetch_size_kib_per_dispatch"] * t["fetch_scale"] +
+                       t["write_size_kib_per_dispatch"] * t["write_scale"]))
+
+
 def connect_bench(args, repeats=3):
     """Device build of the same connectivity (connect(p=K/N), delays U[0.5, 5] ms,
     per-synapse FP32 weights): wall time of hs_net_connect_random (count pass,
@@ -347,7 +363,7 @@ def main():
         bpl = per_step[name] * steps_per_launch
         achieved = bpl / (kt[name]["avg_us"] * 1e-6) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(args, name),
                     "kernel": name, "bytes_per_launch": int(bpl), "steps_per_launch": round(steps_per_launch, 3),
                     "avg_launch_us": round(kt[name]["avg_us"], 3),
                     "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kt.items()}}

@@ -10,23 +10,27 @@
  *     (brian2cuda/templates/synapses_push_spikes.cu:171-993): a host sort per
  *     presynaptic neuron and one copy per (pre, delay) bundle.
  *
- * Row i (one thread per presynaptic neuron) selects targets in increasing j with
- * the reference's two algorithms (synapses_create_generator.cu:315-333):
- *   p <  0.25: j starts at post_lo; each draw r in [0, 1) advances j by
- *              floor(log(r) / log(1 - p)) and selects it (r == 0 ends the row;
- *              a skip past post_hi ends it too)
+ * Row i selects targets in increasing j with the reference's two algorithms
+ * (synapses_create_generator.cu:315-333):
+ *   p <  0.25: j starts at post_lo; draw k (r in [0, 1)) advances j by
+ *              skip_k = floor(log(r) / log(1 - p)) and selects it, then j += 1
+ *              (r == 0 ends the row; a skip past post_hi ends it too)
  *   p >= 0.25: every j is tested with its own draw and selected when r < p
  * The draws are Philox words keyed by (seed, stream, i, k) (layout in
  * hs_internal.h), so a row's synapses do not depend on launch geometry or on the
  * other rows, and the oracle regenerates any subset of rows.
  *
- * Three passes:
- *   count: synapses per (row, delay slot), accumulated per thread in LDS
- *          (D <= 128 slots) or in the row's own range of the global table
+ * One wave per row.  The 64 lanes take 64 consecutive draws of the row at once:
+ * the position of draw k is post_lo + sum_{m<=k} (skip_m + 1) - 1, an inclusive
+ * wave scan, so the batch reproduces the serial chain exactly; the first draw
+ * with r == 0 or a position past the end stops the row.  Three passes:
+ *   count: synapses per (row, delay slot), LDS counters of the wave
  *   scan : exclusive prefix sum of the n_src*D counts -> bstart (int64 block sums)
  *   write: the rows again; synapse (i, j) goes to bstart[i*D + slot] + its rank
- *          among the row's earlier synapses of that slot, so each bundle keeps
- *          creation (ascending j) order
+ *          among the row's earlier synapses of that slot (ballots per distinct
+ *          slot of the batch, LDS cursors), so each bundle keeps creation
+ *          (ascending j) order and a row's writes stay in the row's own region
+ *          of the table (one wave, a few kB: the L2 merges them into full lines)
  * Bounds: the delay quantisation is monotone in the uniform, so every slot lies
  * in [0, D) with D taken from the quantised bounds; both passes still check it,
  * and the write pass checks each position against the end of its bundle, so a
@@ -36,10 +40,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "hs_device.h"
 #include "hs_internal.h"
 #include "hs_philox.h"
 
-#define HS_CONN_BLOCK 64 /* one wave: thread t owns row blockIdx.x*64 + t and LDS column t */
+#define HS_CONN_WAVES 4 /* rows in flight per block, one per wave; LDS: D ints per wave */
 
 __device__ __forceinline__ hs_u32x4 hs_conn_words(const HsConnSpec &c, uint32_t i, uint32_t k, uint32_t tag)
 {
@@ -51,30 +56,54 @@ __device__ __forceinline__ hs_u32x4 hs_conn_words(const HsConnSpec &c, uint32_t 
     return hs_philox4x32_10(ctr, (uint32_t)c.seed, (uint32_t)(c.seed >> 32));
 }
 
-/* f(j) for every target selected in row i, in increasing j */
-template <class F>
-__device__ __forceinline__ void hs_conn_row(const HsConnSpec &c, int i, F &f)
+/*
+ * One batch of row i: 64 draws (from draw k0, position jb) or 64 pair tests
+ * (targets jb .. jb+63).  Lane l learns whether its draw selects a target and
+ * which (sel, j); jb/k0 advance to the next batch and done is set (wave-uniform)
+ * when the row has ended.
+ */
+__device__ __forceinline__ void hs_conn_batch(const HsConnSpec &c, int i, long long &jb, uint32_t &k0, bool &done,
+                                              bool &sel, long long &j)
 {
+    const int lane = hs_lane();
     const long long hi = c.post_hi;
-    hs_u32x4 wd = {{0u, 0u, 0u, 0u}};
     if (c.jump) {
-        uint32_t k = 0;
-        for (long long j = c.post_lo; j < hi; ++j, ++k) {
-            if (!(k & 1u)) wd = hs_conn_words(c, (uint32_t)i, k >> 1, HS_CONN_TAG_SKIP);
-            const double r = (k & 1u) ? hs_u53(wd.v[2], wd.v[3]) : hs_u53(wd.v[0], wd.v[1]);
-            if (r == 0.0) break;
+        const uint32_t k = k0 + (uint32_t)lane;
+        const hs_u32x4 wd = hs_conn_words(c, (uint32_t)i, k >> 1, HS_CONN_TAG_SKIP);
+        const double r = (k & 1u) ? hs_u53(wd.v[2], wd.v[3]) : hs_u53(wd.v[0], wd.v[1]);
+        const bool zero = r == 0.0;
+        long long inc = 1; /* skip + 1; a skip at or past the row length only has to end the row */
+        if (!zero) {
             const double skip = floor(hs_log(r) * c.pconst);
-            if (skip >= (double)(hi - j)) break;
-            j += (long long)skip;
-            if (c.autapses || j != i) f((int)j);
+            const double cap = (double)(hi - c.post_lo);
+            inc = (long long)(skip < cap ? skip : cap) + 1;
+        }
+        long long incl = inc;
+#pragma unroll
+        for (int o = 1; o < HS_WAVE; o <<= 1) {
+            const long long y = __shfl_up(incl, o, HS_WAVE);
+            if (lane >= o) incl += y;
+        }
+        const long long pos = jb + incl - 1;
+        const unsigned long long stop = __ballot(zero || pos >= hi);
+        const int first = stop ? __ffsll((long long)stop) - 1 : HS_WAVE;
+        sel = lane < first && (c.autapses || pos != i);
+        j = pos;
+        if (stop) {
+            done = true;
+        } else {
+            jb += __shfl(incl, HS_WAVE - 1, HS_WAVE);
+            k0 += HS_WAVE;
         }
     } else {
-        for (long long j = c.post_lo; j < hi; ++j) {
-            const uint32_t m = (uint32_t)(j - c.post_lo);
-            if (!(m & 1u)) wd = hs_conn_words(c, (uint32_t)i, m >> 1, HS_CONN_TAG_PAIR);
-            const double r = (m & 1u) ? hs_u53(wd.v[2], wd.v[3]) : hs_u53(wd.v[0], wd.v[1]);
-            if (r < c.p && (c.autapses || j != i)) f((int)j);
-        }
+        const long long jj = jb + lane;
+        const uint32_t m = (uint32_t)(jj - c.post_lo);
+        const hs_u32x4 wd = hs_conn_words(c, (uint32_t)i, m >> 1, HS_CONN_TAG_PAIR);
+        const double r = (m & 1u) ? hs_u53(wd.v[2], wd.v[3]) : hs_u53(wd.v[0], wd.v[1]);
+        sel = jj < hi && r < c.p && (c.autapses || jj != i);
+        j = jj;
+        jb += HS_WAVE;
+        if (jb >= hi) done = true;
     }
 }
 
@@ -85,103 +114,127 @@ __device__ __forceinline__ int hs_conn_slot(const HsConnSpec &c, double ud)
     return (int)((c.dlo + ud * c.dspan) / c.dt + 0.5) - c.dmin;
 }
 
-template <bool LDS>
-__global__ void __launch_bounds__(HS_CONN_BLOCK) hs_k_conn_count(HsConnSpec c, int32_t *__restrict__ cnt, int *err)
+__global__ void __launch_bounds__(HS_CONN_WAVES * HS_WAVE) hs_k_conn_count(HsConnSpec c, int32_t *__restrict__ cnt,
+                                                                           int *err)
 {
-    extern __shared__ int32_t tab[]; /* [D][64] */
-    const int t = threadIdx.x;
-    const int i = c.pre_lo + (int)blockIdx.x * HS_CONN_BLOCK + t;
-    if (i >= c.pre_hi) return; /* no barrier below: each thread uses its own column only */
-    const size_t row = (size_t)i * (size_t)c.D;
-    if (LDS)
-        for (int d = 0; d < c.D; ++d) tab[d * HS_CONN_BLOCK + t] = 0;
-    auto f = [&](int j) {
-        double ud = 0.0;
-        if (c.dspan != 0.0) {
-            const hs_u32x4 x = hs_conn_words(c, (uint32_t)i, (uint32_t)j, HS_CONN_TAG_SYN);
-            ud = hs_u53(x.v[0], x.v[1]);
+    extern __shared__ int32_t tab[]; /* [waves][D] */
+    const int lane = hs_lane(), wv = threadIdx.x / HS_WAVE;
+    int32_t *t = tab + wv * c.D;
+    const int rows = c.pre_hi - c.pre_lo;
+    for (int r = blockIdx.x * HS_CONN_WAVES + wv; r < rows; r += gridDim.x * HS_CONN_WAVES) {
+        const int i = c.pre_lo + r;
+        const size_t row = (size_t)i * (size_t)c.D;
+        for (int d = lane; d < c.D; d += HS_WAVE) t[d] = 0;
+        long long jb = c.post_lo;
+        uint32_t k0 = 0;
+        bool done = jb >= c.post_hi;
+        while (!done) {
+            bool sel;
+            long long j;
+            hs_conn_batch(c, i, jb, k0, done, sel, j);
+            if (sel) {
+                double ud = 0.0;
+                if (c.dspan != 0.0) {
+                    const hs_u32x4 x = hs_conn_words(c, (uint32_t)i, (uint32_t)j, HS_CONN_TAG_SYN);
+                    ud = hs_u53(x.v[0], x.v[1]);
+                }
+                const int s = hs_conn_slot(c, ud);
+                if ((unsigned)s >= (unsigned)c.D) *err = 1;
+                else atomicAdd(&t[s], 1);
+            }
         }
-        const int s = hs_conn_slot(c, ud);
-        if ((unsigned)s >= (unsigned)c.D) {
-            *err = 1;
-            return;
-        }
-        if (LDS) tab[s * HS_CONN_BLOCK + t] += 1;
-        else cnt[row + s] += 1;
-    };
-    hs_conn_row(c, i, f);
-    if (LDS)
-        for (int d = 0; d < c.D; ++d) cnt[row + d] = tab[d * HS_CONN_BLOCK + t];
+        for (int d = lane; d < c.D; d += HS_WAVE) cnt[row + d] = t[d];
+    }
 }
 
-template <typename R, bool LDS>
-__global__ void __launch_bounds__(HS_CONN_BLOCK)
-    hs_k_conn_write(HsConnSpec c, const int32_t *__restrict__ bstart, int32_t *__restrict__ cursor,
-                    int32_t *__restrict__ post, R *__restrict__ w, int *err)
+template <typename R>
+__global__ void __launch_bounds__(HS_CONN_WAVES * HS_WAVE)
+    hs_k_conn_write(HsConnSpec c, const int32_t *__restrict__ bstart, int32_t *__restrict__ post, R *__restrict__ w,
+                    int *err)
 {
-    extern __shared__ int32_t cur[]; /* [D][64] */
-    const int t = threadIdx.x;
-    const int i = c.pre_lo + (int)blockIdx.x * HS_CONN_BLOCK + t;
-    if (i >= c.pre_hi) return;
-    const size_t row = (size_t)i * (size_t)c.D;
-    if (LDS)
-        for (int d = 0; d < c.D; ++d) cur[d * HS_CONN_BLOCK + t] = bstart[row + d];
+    extern __shared__ int32_t cur[]; /* [waves][D]: next free position of each bundle of the row */
+    const int lane = hs_lane(), wv = threadIdx.x / HS_WAVE;
+    int32_t *q = cur + wv * c.D;
+    const int rows = c.pre_hi - c.pre_lo;
     const bool draw = c.dspan != 0.0 || (w && c.wspan != 0.0);
-    auto f = [&](int j) {
-        double ud = 0.0, uw = 0.0;
-        if (draw) {
-            const hs_u32x4 x = hs_conn_words(c, (uint32_t)i, (uint32_t)j, HS_CONN_TAG_SYN);
-            ud = hs_u53(x.v[0], x.v[1]);
-            uw = hs_u53(x.v[2], x.v[3]);
+    for (int r = blockIdx.x * HS_CONN_WAVES + wv; r < rows; r += gridDim.x * HS_CONN_WAVES) {
+        const int i = c.pre_lo + r;
+        const size_t row = (size_t)i * (size_t)c.D;
+        for (int d = lane; d < c.D; d += HS_WAVE) q[d] = bstart[row + d];
+        long long jb = c.post_lo;
+        uint32_t k0 = 0;
+        bool done = jb >= c.post_hi;
+        while (!done) {
+            bool sel;
+            long long j;
+            hs_conn_batch(c, i, jb, k0, done, sel, j);
+            double ud = 0.0, uw = 0.0;
+            int s = -1;
+            if (sel) {
+                if (draw) {
+                    const hs_u32x4 x = hs_conn_words(c, (uint32_t)i, (uint32_t)j, HS_CONN_TAG_SYN);
+                    ud = hs_u53(x.v[0], x.v[1]);
+                    uw = hs_u53(x.v[2], x.v[3]);
+                }
+                s = hs_conn_slot(c, ud);
+                if ((unsigned)s >= (unsigned)c.D) {
+                    *err = 1;
+                    sel = false;
+                }
+            }
+            /* rank of each synapse among the batch's synapses of its slot, in lane
+               (= creation) order: one ballot per distinct slot */
+            int pos = -1;
+            unsigned long long act = __ballot(sel);
+            while (act) {
+                const int leader = __ffsll((long long)act) - 1;
+                const int s0 = __shfl(s, leader, HS_WAVE);
+                const bool mine = sel && s == s0;
+                const unsigned long long m = __ballot(mine);
+                if (mine) pos = q[s0] + __popcll(m & hs_lanemask_lt());
+                if (lane == leader) q[s0] += __popcll(m);
+                act &= ~m;
+            }
+            if (sel) {
+                if (pos >= bstart[row + s + 1]) {
+                    *err = 2;
+                } else {
+                    post[pos] = (int32_t)j;
+                    if (w) w[pos] = (R)(c.wlo + uw * c.wspan);
+                }
+            }
         }
-        const int s = hs_conn_slot(c, ud);
-        if ((unsigned)s >= (unsigned)c.D) {
-            *err = 1;
-            return;
-        }
-        const int pos = LDS ? cur[s * HS_CONN_BLOCK + t]++ : cursor[row + s]++;
-        if (pos >= bstart[row + s + 1]) {
-            *err = 2;
-            return;
-        }
-        post[pos] = j;
-        if (w) w[pos] = (R)(c.wlo + uw * c.wspan);
-    };
-    hs_conn_row(c, i, f);
+    }
+}
+
+static int hs_conn_grid(const HsConnSpec &c)
+{
+    const long long rows = c.pre_hi - c.pre_lo;
+    const long long g = (rows + HS_CONN_WAVES - 1) / HS_CONN_WAVES;
+    return (int)(g < 65536 ? g : 65536);
 }
 
 hipError_t hs_launch_conn_count(const HsConnSpec &c, int32_t *cnt, int *err, hipStream_t s)
 {
-    const int rows = c.pre_hi - c.pre_lo;
-    if (rows <= 0) return hipSuccess;
-    const int grid = (rows + HS_CONN_BLOCK - 1) / HS_CONN_BLOCK;
-    if (c.D <= HS_CONN_LDS_SLOTS)
-        hs_k_conn_count<true><<<grid, HS_CONN_BLOCK, sizeof(int32_t) * c.D * HS_CONN_BLOCK, s>>>(c, cnt, err);
-    else
-        hs_k_conn_count<false><<<grid, HS_CONN_BLOCK, 0, s>>>(c, cnt, err);
+    if (c.pre_hi <= c.pre_lo) return hipSuccess;
+    if (c.D < 1 || c.D > HS_MAX_DELAY_SLOTS) return hipErrorInvalidValue;
+    hs_k_conn_count<<<hs_conn_grid(c), HS_CONN_WAVES * HS_WAVE, sizeof(int32_t) * c.D * HS_CONN_WAVES, s>>>(c, cnt,
+                                                                                                           err);
     return hipGetLastError();
 }
 
-template <typename R>
-static hipError_t hs_conn_write_t(const HsConnSpec &c, const int32_t *bstart, int32_t *cursor, int32_t *post, R *w,
-                                  int *err, hipStream_t s)
+hipError_t hs_launch_conn_write(int prec, const HsConnSpec &c, const int32_t *bstart, int32_t *post, void *w, int *err,
+                                hipStream_t s)
 {
-    const int rows = c.pre_hi - c.pre_lo;
-    if (rows <= 0) return hipSuccess;
-    const int grid = (rows + HS_CONN_BLOCK - 1) / HS_CONN_BLOCK;
-    if (c.D <= HS_CONN_LDS_SLOTS)
-        hs_k_conn_write<R, true><<<grid, HS_CONN_BLOCK, sizeof(int32_t) * c.D * HS_CONN_BLOCK, s>>>(c, bstart, cursor,
-                                                                                                   post, w, err);
+    if (c.pre_hi <= c.pre_lo) return hipSuccess;
+    if (c.D < 1 || c.D > HS_MAX_DELAY_SLOTS) return hipErrorInvalidValue;
+    const size_t lds = sizeof(int32_t) * c.D * HS_CONN_WAVES;
+    if (prec == 64)
+        hs_k_conn_write<double><<<hs_conn_grid(c), HS_CONN_WAVES * HS_WAVE, lds, s>>>(c, bstart, post, (double *)w,
+                                                                                    err);
     else
-        hs_k_conn_write<R, false><<<grid, HS_CONN_BLOCK, 0, s>>>(c, bstart, cursor, post, w, err);
+        hs_k_conn_write<float><<<hs_conn_grid(c), HS_CONN_WAVES * HS_WAVE, lds, s>>>(c, bstart, post, (float *)w, err);
     return hipGetLastError();
-}
-
-hipError_t hs_launch_conn_write(int prec, const HsConnSpec &c, const int32_t *bstart, int32_t *cursor,
-                                int32_t *post, void *w, int *err, hipStream_t s)
-{
-    if (prec == 64) return hs_conn_write_t<double>(c, bstart, cursor, post, (double *)w, err, s);
-    return hs_conn_write_t<float>(c, bstart, cursor, post, (float *)w, err, s);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -154,7 +154,6 @@ struct DPath {
 #define HS_CONN_TAG_SKIP 0u
 #define HS_CONN_TAG_SYN 1u
 #define HS_CONN_TAG_PAIR 2u
-#define HS_CONN_LDS_SLOTS 128 /* delay slots kept per thread in LDS (D above: tables in global memory) */
 
 struct HsConnSpec {
     int pre_lo, pre_hi, post_lo, post_hi;
@@ -173,10 +172,9 @@ struct HsConnSpec {
    *err becomes nonzero if a slot falls outside [0, D) */
 hipError_t hs_launch_conn_count(const HsConnSpec &c, int32_t *cnt, int *err, hipStream_t s);
 /* write pass: post[] and w[] (R = double or float by prec, nullptr: no weights) at
-   bstart[i*D+slot] + running count; cursor: int32[n_src*D] holding a copy of
-   bstart when D > HS_CONN_LDS_SLOTS (else unused) */
-hipError_t hs_launch_conn_write(int prec, const HsConnSpec &c, const int32_t *bstart, int32_t *cursor,
-                                int32_t *post, void *w, int *err, hipStream_t s);
+   bstart[i*D+slot] + the synapse's rank in its bundle */
+hipError_t hs_launch_conn_write(int prec, const HsConnSpec &c, const int32_t *bstart, int32_t *post, void *w,
+                                int *err, hipStream_t s);
 /* exclusive prefix sum of n int32 counts: out[0..n] (out[n] = total, also at
    bsum[hs_scan_blocks(n)] as int64); bsum holds hs_scan_blocks(n) + 1 entries */
 int hs_scan_blocks(long long n);

@@ -831,11 +831,7 @@ extern "C" int hs_net_connect_random(hs_net *net, int src, int tgt, int var, int
         release();
         return r;
     }
-    if (D > HS_CONN_LDS_SLOTS && nb &&
-        (e = hipMemcpyAsync(cnt, P.bstart, sizeof(int32_t) * (size_t)nb, hipMemcpyDeviceToDevice, g_rt.stream)) !=
-            hipSuccess)
-        return fail_hip(e, "cursor copy");
-    if ((e = hs_launch_conn_write(net->prec == HS_F64 ? 64 : 32, c, P.bstart, cnt, P.post,
+    if ((e = hs_launch_conn_write(net->prec == HS_F64 ? 64 : 32, c, P.bstart, P.post,
                                   wmode == HS_W_SYNAPSE ? P.w : nullptr, err, g_rt.stream)) != hipSuccess)
         return fail_hip(e, "write pass");
     if ((e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, g_rt.stream)) != hipSuccess ||
