@@ -311,7 +311,7 @@ def main():
     ap.add_argument("--K", type=int, default=1000)
     ap.add_argument("--nets", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--cpu-steps", type=int, default=300, help="oracle timed steps (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=3000, help="oracle timed steps (0: skip); 3000 is about 15 s of one core")
     ap.add_argument("--cpu-skip", type=int, default=300, help="oracle untimed steps before its window")
     ap.add_argument("--kt-steps", type=int, default=2000, help="steps of the kernel-timed run (0: skip)")
     ap.add_argument("--drift-steps", type=int, default=10000, help="steps of the drift comparison (0: skip)")
