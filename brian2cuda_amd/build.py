@@ -16,7 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhs.so")
-SOURCES = ["hs_runtime.hip", "hs_engine.hip", "hs_ops.hip", "hs_hines.hip", "hs_connect.hip", "hs_plastic.hip"]
+SOURCES = ["hs_runtime.hip", "hs_engine.hip", "hs_ops.hip", "hs_hines.hip", "hs_connect.hip", "hs_plastic.hip", "hs_sort.hip"]
 ARCH = os.environ.get("HS_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

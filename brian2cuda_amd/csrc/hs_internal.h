@@ -180,6 +180,20 @@ hipError_t hs_launch_conn_write(int prec, const HsConnSpec &c, const int32_t *bs
 int hs_scan_blocks(long long n);
 hipError_t hs_launch_scan(const int32_t *in, long long n, long long *bsum, int32_t *out, hipStream_t s);
 
+/* bundle table from synapse arrays in HBM (hs_sort.hip); minmax = {INT_MAX, -1}
+   and err = 0 on entry to the check */
+hipError_t hs_launch_syn_check(const int32_t *pre, const int32_t *post, const int32_t *delay, long long n, int n_src,
+                               int n_tgt, int *minmax, int *err, hipStream_t s);
+hipError_t hs_launch_syn_keys(const int32_t *pre, const int32_t *delay, long long n, int D, int dmin, uint32_t *keys,
+                              int32_t *vals, hipStream_t s);
+/* stable radix sort of (key, value) pairs on key bits [0, bits); tmp == nullptr: size query */
+hipError_t hs_radix_sort_pairs(void *tmp, size_t *tmp_bytes, const uint32_t *keys, uint32_t *keys_out,
+                               const int32_t *vals, int32_t *vals_out, long long n, int bits, hipStream_t s);
+hipError_t hs_launch_syn_gather(int prec, const int32_t *order, long long n, const int32_t *post, const double *w,
+                                int32_t *post_out, void *w_out, hipStream_t s);
+hipError_t hs_launch_syn_convert(int prec, const double *w, long long n, void *out, hipStream_t s);
+hipError_t hs_launch_syn_bstart(const uint32_t *keys, long long n, long long nb, int32_t *bstart, hipStream_t s);
+
 /* runtime services for the translation units (hs_runtime.hip) */
 hipStream_t hs_rt_stream();
 int hs_rt_cus();

@@ -846,6 +846,127 @@ extern "C" int hs_net_connect_random(hs_net *net, int src, int tgt, int var, int
     return HS_OK;
 }
 
+/*
+ * Bundle table from synapse arrays already in HBM (kernels in hs_sort.hip):
+ * check + delay range, keys (pre, delay slot), stable radix sort, gather of
+ * post and weights, bstart from the sorted keys.  Temporaries are freed before
+ * return; the caller's arrays are only read.
+ */
+extern "C" int hs_net_add_pathway_device(hs_net *net, int src, int tgt, int var, int64_t nsyn, const int32_t *pre,
+                                         const int32_t *post, const int32_t *delay_steps, int scalar_delay, int wmode,
+                                         const double *weights, double wscalar, int *pid)
+{
+    if (!net) return hs_fail(HS_ERR_ARG, "null net");
+    if (net->built) return hs_fail(HS_ERR_STATE, "pathways must be added before the first run");
+    if (src < 0 || src >= (int)net->g.size() || tgt < 0 || tgt >= (int)net->g.size())
+        return hs_fail(HS_ERR_ARG, "bad source/target group");
+    HGroup &gs = net->g[src];
+    HGroup &gt = net->g[tgt];
+    if (var < 0 || var >= gt.mi.nvars) return hs_fail(HS_ERR_ARG, "bad target variable %d", var);
+    if (nsyn < 0 || nsyn >= 0x7fffffffLL) return hs_fail(HS_ERR_ARG, "synapse count %lld out of range", (long long)nsyn);
+    if (nsyn > 0 && (!pre || !post)) return hs_fail(HS_ERR_ARG, "null pre/post");
+    if (wmode < 0 || wmode > 2) return hs_fail(HS_ERR_ARG, "bad weight mode");
+    if (wmode != HS_W_SCALAR && !weights) return hs_fail(HS_ERR_ARG, "weights required for weight mode %d", wmode);
+    if (!delay_steps && scalar_delay < 0) return hs_fail(HS_ERR_ARG, "negative delay");
+    if (net->opt[5] == HS_DELIVER_GATHER)
+        return hs_fail(HS_ERR_UNSUPPORTED, "device-built pathways use scatter bundles (delivery option 1)");
+    const long long n = nsyn;
+    int32_t *tmp_i = nullptr;      /* minmax[2], err */
+    uint32_t *keys = nullptr, *keys_s = nullptr;
+    int32_t *vals = nullptr, *vals_s = nullptr;
+    void *cub = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(g_rt.stream);
+        for (void *q : {(void *)tmp_i, (void *)keys, (void *)keys_s, (void *)vals, (void *)vals_s, cub})
+            if (q) (void)hipFree(q);
+    };
+    auto fail_hip = [&](hipError_t e, const char *what) {
+        (void)hipGetLastError();
+        release();
+        return hs_fail(HS_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc((void **)&tmp_i, 3 * sizeof(int32_t))) != hipSuccess) return fail_hip(e, "temporaries");
+    const int32_t init[3] = {0x7fffffff, -1, 0};
+    int32_t got[3];
+    if ((e = hipMemcpyAsync(tmp_i, init, sizeof(init), hipMemcpyHostToDevice, g_rt.stream)) != hipSuccess ||
+        (e = hs_launch_syn_check(pre, post, delay_steps, n, gs.n, gt.n, tmp_i, tmp_i + 2, g_rt.stream)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(got, tmp_i, sizeof(got), hipMemcpyDeviceToHost, g_rt.stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(g_rt.stream)) != hipSuccess)
+        return fail_hip(e, "check");
+    if (got[2]) {
+        release();
+        return hs_fail(HS_ERR_ARG, "%s", (got[2] & 1) ? "pre index out of range"
+                                         : (got[2] & 2) ? "post index out of range" : "negative delay");
+    }
+    int dmin = scalar_delay, dmax = scalar_delay;
+    if (delay_steps && n > 0) {
+        dmin = got[0];
+        dmax = got[1];
+    }
+    const int D = dmax - dmin + 1;
+    if (D > HS_MAX_DELAY_SLOTS) {
+        release();
+        return hs_fail(HS_ERR_UNSUPPORTED, "delay range %d steps > %d", D, HS_MAX_DELAY_SLOTS);
+    }
+    const long long nb = (long long)gs.n * D;
+    if (nb >= 0xffffffffLL) {
+        release();
+        return hs_fail(HS_ERR_UNSUPPORTED, "%lld bundles: keys beyond 32 bits", nb);
+    }
+    int bits = 1;
+    while (bits < 32 && (1LL << bits) < nb) ++bits;
+    HPath P;
+    P.src = src;
+    P.tgt = tgt;
+    P.var = var;
+    P.nsyn = nsyn;
+    P.wmode = wmode;
+    P.wscalar = wscalar;
+    P.dmin = dmin;
+    P.dmax = dmax;
+    P.D = D;
+    P.dmode = HS_DELIVER_SCATTER;
+    const size_t rs = hs_rsize(net);
+    int r;
+    if ((r = hs_dalloc(net, sizeof(int32_t) * (size_t)(nb + 1), (void **)&P.bstart, false)) != HS_OK ||
+        (r = hs_dalloc(net, sizeof(int32_t) * (size_t)(n ? n : 1), (void **)&P.post, false)) != HS_OK ||
+        (wmode == HS_W_SYNAPSE && (r = hs_dalloc(net, rs * (size_t)(n ? n : 1), &P.w, false)) != HS_OK) ||
+        (wmode == HS_W_PRE && (r = hs_dalloc(net, rs * (size_t)gs.n, &P.w, false)) != HS_OK)) {
+        release();
+        return r;
+    }
+    const size_t nn = (size_t)(n ? n : 1);
+    size_t cub_bytes = 0;
+    if ((e = hipMalloc((void **)&keys, sizeof(uint32_t) * nn)) != hipSuccess ||
+        (e = hipMalloc((void **)&keys_s, sizeof(uint32_t) * nn)) != hipSuccess ||
+        (e = hipMalloc((void **)&vals, sizeof(int32_t) * nn)) != hipSuccess ||
+        (e = hipMalloc((void **)&vals_s, sizeof(int32_t) * nn)) != hipSuccess ||
+        (e = hs_radix_sort_pairs(nullptr, &cub_bytes, keys, keys_s, vals, vals_s, n, bits, g_rt.stream)) !=
+            hipSuccess ||
+        (e = hipMalloc(&cub, cub_bytes ? cub_bytes : 1)) != hipSuccess)
+        return fail_hip(e, "sort temporaries");
+    if ((e = hs_launch_syn_keys(pre, delay_steps, n, D, dmin, keys, vals, g_rt.stream)) != hipSuccess)
+        return fail_hip(e, "keys");
+    if (n > 0 && (e = hs_radix_sort_pairs(cub, &cub_bytes, keys, keys_s, vals, vals_s, n, bits, g_rt.stream)) !=
+                     hipSuccess)
+        return fail_hip(e, "radix sort");
+    if ((e = hs_launch_syn_gather(net->prec == HS_F64 ? 64 : 32, vals_s, n, post, weights, P.post,
+                                  wmode == HS_W_SYNAPSE ? P.w : nullptr, g_rt.stream)) != hipSuccess)
+        return fail_hip(e, "gather");
+    if (wmode == HS_W_PRE &&
+        (e = hs_launch_syn_convert(net->prec == HS_F64 ? 64 : 32, weights, gs.n, P.w, g_rt.stream)) != hipSuccess)
+        return fail_hip(e, "per-pre weights");
+    if ((e = hs_launch_syn_bstart(keys_s, n, nb, P.bstart, g_rt.stream)) != hipSuccess) return fail_hip(e, "bstart");
+    release(); /* synchronises: the table is complete */
+    net->n_scatter++;
+    net->p.push_back(P);
+    net->desc_dirty = true;
+    if (pid) *pid = (int)net->p.size() - 1;
+    return HS_OK;
+}
+
 extern "C" int hs_net_pathway_info(hs_net *net, int pid, int64_t *nsyn, int *dmin, int *D, int *dmode)
 {
     if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");

@@ -311,8 +311,10 @@ def stdp_example(heterog_delays=False, n_cells=100, seed=42, dt=0.1 * ms):
 
 # ---- instantiate on the device --------------------------------------------------------
 
-def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options=None):
-    """Create a runtime.Net from a spec.  Returns (net, spike_monitor_ids, state_monitor_ids)."""
+def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options=None, device_sort=False):
+    """Create a runtime.Net from a spec.  Returns (net, spike_monitor_ids, state_monitor_ids).
+    device_sort: array pathways are uploaded and bundled on the device
+    (add_pathway_device) instead of by the host counting sort."""
     from .runtime import Net
     net = Net(precision, engine, spec.dt, spec.seed)
     for k, v in (options or {}).items():  # before the pathways: the delivery mode is chosen per pathway
@@ -329,8 +331,9 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
             pid = net.connect_random(p.src, p.tgt, p.var, r.p, r.pre, r.post, r.autapses, r.seed, r.stream, r.delay,
                                      r.wmode, r.w)
         else:
-            pid = net.add_pathway(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode,
-                                  p.weights, p.wscalar)
+            add = net.add_pathway_device if device_sort else net.add_pathway
+            pid = add(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode, p.weights,
+                      p.wscalar)
         if p.stdp is not None:
             net.pathway_stdp(pid, *p.stdp)
     smons = [net.add_spike_monitor(g, spike_capacity) for g in spec.spike_monitors]

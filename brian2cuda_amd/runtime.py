@@ -288,6 +288,32 @@ class Net:
         call("hs_net_get_synapses", self.h, pid, _i32(pre), _i32(post), _i32(d), _f64(w), n)
         return pre, post, d, w
 
+    def add_pathway_device(self, src, tgt, var, pre, post, delay_steps=None, scalar_delay=0, wmode=HS_W_SCALAR,
+                           weights=None, wscalar=0.0):
+        """add_pathway with the bundle table built on the device: the arrays
+        (numpy, or DeviceArray already in HBM) are sorted by (pre, delay) there."""
+        held = []
+
+        def dev(a, dtype):
+            if a is None:
+                return None
+            if isinstance(a, DeviceArray):
+                return a.ptr
+            d = DeviceArray.from_numpy(np.ascontiguousarray(a, dtype=dtype))
+            held.append(d)
+            return d.ptr
+
+        n = pre.n if isinstance(pre, DeviceArray) else int(np.asarray(pre).size)
+        pid = C.c_int()
+        try:
+            call("hs_net_add_pathway_device", self.h, src, tgt, var, n, dev(pre, np.int32), dev(post, np.int32),
+                 dev(delay_steps, np.int32), int(scalar_delay), int(wmode),
+                 dev(weights, np.float64) if wmode != HS_W_SCALAR else None, float(wscalar), C.byref(pid))
+        finally:
+            for d in held:
+                d.free()
+        return pid.value
+
     def add_spike_monitor(self, gid, capacity=1 << 20):
         mid = C.c_int()
         call("hs_net_add_spike_monitor", self.h, gid, int(capacity), C.byref(mid))

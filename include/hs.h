@@ -173,6 +173,19 @@ int hs_net_add_pathway(hs_net *net, int src, int tgt, int var, int64_t nsyn, con
 int hs_net_connect_random(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo, int post_hi,
                           double p, int autapses, uint64_t seed, uint32_t stream, double delay_lo, double delay_hi,
                           int wmode, double w_lo, double w_hi, int *pid);
+/*
+ * hs_net_add_pathway with the bundle table built on the device: pre, post,
+ * delay_steps (or NULL: scalar_delay) and weights are DEVICE pointers (hs_alloc
+ * or any allocation on the device) holding int32 / double arrays of the same
+ * meaning; they are only read.  Indices and delays are checked on the device,
+ * then the synapses are stably radix-sorted by (pre, delay) there, so the table
+ * equals the host-built one for the same arrays.  Scatter delivery only.
+ * Replaces the host sort and per-bundle copies of _before_run_<push_spikes>
+ * (synapses_push_spikes.cu:441-575).
+ */
+int hs_net_add_pathway_device(hs_net *net, int src, int tgt, int var, int64_t nsyn, const int32_t *pre,
+                              const int32_t *post, const int32_t *delay_steps, int scalar_delay, int wmode,
+                              const double *weights, double wscalar, int *pid);
 /* synapse count, first delay slot (steps), delay slots and delivery mode of a pathway */
 int hs_net_pathway_info(hs_net *net, int pid, int64_t *nsyn, int *dmin, int *D, int *dmode);
 /*
