@@ -92,6 +92,8 @@ SIGNATURES = {
     "hs_net_set_spikegen": (C.c_int, [P, C.c_int, C.c_int64, I32P, I64P]),
     "hs_net_pathway_stdp": (C.c_int, [P, C.c_int, DP, C.c_int]),
     "hs_net_get_synapse_var": (C.c_int, [P, C.c_int, C.c_int, DP, C.c_int64]),
+    "hs_net_add_rate_monitor": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "hs_net_get_rate_monitor": (C.c_int, [P, C.c_int, DP, C.c_int64]),
     "hs_net_write_monitor": (C.c_int, [P, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p]),
     "hs_net_write_last_run_info": (C.c_int, [P, C.c_char_p]),
     "hs_net_mon_count": (C.c_int, [P, C.c_int, I64P]),

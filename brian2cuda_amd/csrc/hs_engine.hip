@@ -300,6 +300,14 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
             }
         }
     }
+    if (g.rmon_cnt) { /* PopulationRateMonitor: one count per wave and step */
+        const bool in = valid && i >= g.rmon_start && i < g.rmon_stop;
+        for (int j = 0; j < w; ++j) {
+            const unsigned long long m = __ballot(in && ((bits >> j) & 1u));
+            const long long row = T + j - g.rmon_t0;
+            if (hs_lane() == 0 && m && row >= 0 && row < g.rmon_rows) atomicAdd(&g.rmon_cnt[row], (int)__popcll(m));
+        }
+    }
     if (PUSH) hs_push_near<R>(push, gid, i, bits, T, w);
     __syncthreads();
 }

@@ -99,6 +99,11 @@ struct DGroup {
     const int32_t *gen_off;
     const int32_t *gen_idx;
     long long gen_steps, gen_period;
+    /* PopulationRateMonitor: spikes of neurons [rmon_start, rmon_stop) at step t
+       are added to rmon_cnt[t - rmon_t0] (rows [0, rmon_rows)), or nullptr */
+    int32_t *rmon_cnt;
+    long long rmon_t0, rmon_rows;
+    int rmon_start, rmon_stop;
 };
 
 template <typename R>

@@ -355,6 +355,7 @@ struct HGroup {
     int nsub = 1, subcap = 0; /* spike sub-lists per ring slot (see DGroup) */
     int32_t *gen_off = nullptr, *gen_idx = nullptr; /* SpikeGeneratorGroup pattern (see DGroup) */
     long long gen_steps = 0, gen_period = 0;
+    int rmon = -1; /* PopulationRateMonitor */
 };
 
 struct HPath {
@@ -377,7 +378,7 @@ struct HPath {
 };
 
 struct HMon {
-    int kind; /* 0 spikes, 1 state */
+    int kind; /* 0 spikes, 1 state, 2 population rate (buf: int32 spike count per row) */
     int gid, var, start, stop;
     int64_t cap = 0;                   /* spike records */
     int32_t *mi = nullptr, *mt = nullptr;
@@ -1178,6 +1179,44 @@ extern "C" int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start
     return HS_OK;
 }
 
+/* PopulationRateMonitor (ratemonitor.cu): spikes per step of neurons [start, stop) */
+extern "C" int hs_net_add_rate_monitor(hs_net *net, int gid, int start, int stop, int *mid)
+{
+    if (!net || gid < 0 || gid >= (int)net->g.size()) return hs_fail(HS_ERR_ARG, "bad group");
+    HGroup &g = net->g[gid];
+    if (start < 0 || stop > g.n || stop <= start) return hs_fail(HS_ERR_ARG, "bad rate monitor range");
+    if (g.rmon >= 0) return hs_fail(HS_ERR_UNSUPPORTED, "one rate monitor per group");
+    HMon M;
+    M.kind = 2;
+    M.gid = gid;
+    M.var = 0;
+    M.start = start;
+    M.stop = stop;
+    M.t0 = net->step;
+    net->m.push_back(M);
+    g.rmon = (int)net->m.size() - 1;
+    net->desc_dirty = true;
+    hs_drop_graphs(net);
+    if (mid) *mid = (int)net->m.size() - 1;
+    return HS_OK;
+}
+
+/* rate[k] = count / dt / (stop - start) for each recorded step, as ratemonitor.cu:98 computes it */
+extern "C" int hs_net_get_rate_monitor(hs_net *net, int mid, double *rate, int64_t cap)
+{
+    if (!net || mid < 0 || mid >= (int)net->m.size() || net->m[mid].kind != 2 || !rate)
+        return hs_fail(HS_ERR_ARG, "bad rate monitor");
+    const HMon &M = net->m[mid];
+    const int64_t rows = std::max<int64_t>(0, net->step - M.t0);
+    if (cap < rows) return hs_fail(HS_ERR_ARG, "output too small");
+    if (!rows) return HS_OK;
+    std::vector<int32_t> c((size_t)rows);
+    HS_HIP(hipMemcpyAsync(c.data(), M.buf, sizeof(int32_t) * (size_t)rows, hipMemcpyDeviceToHost, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    for (int64_t k = 0; k < rows; ++k) rate[k] = 1.0 * c[(size_t)k] / net->dt / (double)(M.stop - M.start);
+    return HS_OK;
+}
+
 extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
 {
     if (!net || key < 0 || key > 9) return hs_fail(HS_ERR_ARG, "bad option key");
@@ -1330,6 +1369,14 @@ static int hs_upload_desc_t(hs_net *net)
         d.gen_idx = g.gen_idx;
         d.gen_steps = g.gen_steps;
         d.gen_period = g.gen_period;
+        if (g.rmon >= 0) {
+            const HMon &M = net->m[g.rmon];
+            d.rmon_cnt = (int32_t *)M.buf;
+            d.rmon_t0 = M.t0;
+            d.rmon_rows = M.rows;
+            d.rmon_start = M.start;
+            d.rmon_stop = M.stop;
+        }
         d.vr = g.mi.i_vr >= 0 ? (R)g.params[g.mi.i_vr] : (R)0;
         d.seed = net->seed;
         if (g.smon >= 0) {
@@ -1426,15 +1473,16 @@ static int hs_upload_desc(hs_net *net)
 static int hs_grow_state_monitors(hs_net *net, int64_t until)
 {
     for (HMon &M : net->m) {
-        if (M.kind != 1) continue;
+        if (M.kind == 0) continue;
         const int64_t need = until - M.t0;
         if (need <= M.rows) continue;
         int64_t rows = std::max<int64_t>(need, M.rows * 2);
-        const size_t w = (size_t)(M.stop - M.start);
+        /* state rows: one value per recorded neuron; rate rows: one int32 count, zeroed */
+        const size_t rowb = M.kind == 1 ? hs_rsize(net) * (size_t)(M.stop - M.start) : sizeof(int32_t);
         void *nb = nullptr;
-        HS_TRY(hs_dalloc(net, hs_rsize(net) * w * (size_t)rows, &nb, false));
+        HS_TRY(hs_dalloc(net, rowb * (size_t)rows, &nb, M.kind == 2));
         if (M.buf && M.rows)
-            HS_HIP(hipMemcpyAsync(nb, M.buf, hs_rsize(net) * w * (size_t)M.rows, hipMemcpyDeviceToDevice, g_rt.stream));
+            HS_HIP(hipMemcpyAsync(nb, M.buf, rowb * (size_t)M.rows, hipMemcpyDeviceToDevice, g_rt.stream));
         HS_HIP(hipStreamSynchronize(g_rt.stream));
         hs_dfree(net, M.buf);
         M.buf = nb;
@@ -1833,6 +1881,17 @@ extern "C" int hs_net_write_monitor(hs_net *net, int mid, const char *path_a, co
         HS_TRY(hs_write_raw(path_a, i.data(), sizeof(int32_t) * (size_t)n));
         HS_TRY(hs_write_raw(path_b, t.data(), sizeof(double) * (size_t)n));
         HS_TRY(hs_write_raw(path_c, count.data(), sizeof(int32_t) * count.size()));
+        HS_TRY(hs_write_raw(path_d, &N, sizeof(N)));
+        return HS_OK;
+    }
+    if (M.kind == 2) {
+        /* PopulationRateMonitor: t (double), rate (double), -, N (rows) */
+        std::vector<double> t((size_t)(n ? n : 1)), rate((size_t)(n ? n : 1));
+        for (int64_t r = 0; r < n; ++r) t[(size_t)r] = (double)(M.t0 + r) * net->dt;
+        HS_TRY(hs_net_get_rate_monitor(net, mid, rate.data(), n));
+        const int32_t N = (int32_t)n;
+        HS_TRY(hs_write_raw(path_a, t.data(), sizeof(double) * (size_t)n));
+        HS_TRY(hs_write_raw(path_b, rate.data(), sizeof(double) * (size_t)n));
         HS_TRY(hs_write_raw(path_d, &N, sizeof(N)));
         return HS_OK;
     }

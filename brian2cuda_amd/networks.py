@@ -78,6 +78,7 @@ class NetSpec:
     paths: List[PathSpec]
     spike_monitors: List[int] = field(default_factory=list)         # group ids
     state_monitors: List[tuple] = field(default_factory=list)       # (gid, var, start, stop)
+    rate_monitors: List[tuple] = field(default_factory=list)        # (gid, start, stop)
     info: dict = field(default_factory=dict)
 
     @property
@@ -304,7 +305,7 @@ def stdp_example(heterog_delays=False, n_cells=100, seed=42, dt=0.1 * ms):
                  wmode=HS_W_SYNAPSE, weights=fac * connectivity.ravel(),
                  stdp=(taupre, taupost, dApre, dApost, -np.inf, np.inf, mV))
     spec = NetSpec("stdp" + ("_heterog" if heterog_delays else ""), dt, seed, [P, Q], [T, S],
-                   spike_monitors=[0], state_monitors=[(0, 0, 0, 10)],
+                   spike_monitors=[0], state_monitors=[(0, 0, 0, 10)], rate_monitors=[(0, 0, n_cells)],
                    info={"n_cells": n_cells, "record_synapses": list(range(10)), "record_every_steps": 1000})
     return spec
 
@@ -338,4 +339,5 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
             net.pathway_stdp(pid, *p.stdp)
     smons = [net.add_spike_monitor(g, spike_capacity) for g in spec.spike_monitors]
     stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
+    net.rate_mids = [net.add_rate_monitor(*m) for m in spec.rate_monitors]
     return net, smons, stmons

@@ -319,6 +319,20 @@ class Net:
         call("hs_net_add_spike_monitor", self.h, gid, int(capacity), C.byref(mid))
         return mid.value
 
+    def add_rate_monitor(self, gid, start=0, stop=None):
+        """PopulationRateMonitor over neurons [start, stop) of a group."""
+        stop = self.groups[gid][1] if stop is None else stop
+        mid = C.c_int()
+        call("hs_net_add_rate_monitor", self.h, gid, int(start), int(stop), C.byref(mid))
+        return mid.value
+
+    def rate_monitor(self, mid):
+        """population rate (Hz) of every step since the monitor was added"""
+        n = self.mon_count(mid)
+        out = np.empty(max(n, 1))
+        call("hs_net_get_rate_monitor", self.h, mid, _f64(out), n)
+        return out[:n]
+
     def add_state_monitor(self, gid, var, start, stop):
         mid = C.c_int()
         call("hs_net_add_state_monitor", self.h, gid, var, int(start), int(stop), C.byref(mid))

@@ -224,6 +224,14 @@ int hs_net_get_synapse_var(hs_net *net, int pid, int var, double *out, int64_t c
 
 /* SpikeMonitor on a group: capacity = device records kept (i, step). */
 int hs_net_add_spike_monitor(hs_net *net, int gid, int64_t capacity, int *mid);
+/*
+ * PopulationRateMonitor of neurons [start, stop) of a group: the update kernel
+ * counts their spikes per step; hs_net_get_rate_monitor gives
+ * rate = 1.0*count/dt/(stop-start) per recorded step (host double[cap]).
+ * Replaces ratemonitor.cu:52-99 (a one-thread kernel per step).
+ */
+int hs_net_add_rate_monitor(hs_net *net, int gid, int start, int stop, int *mid);
+int hs_net_get_rate_monitor(hs_net *net, int mid, double *rate, int64_t cap);
 /* StateMonitor (when='start') of var over neurons [start, stop). */
 int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start, int stop, int *mid);
 
@@ -281,6 +289,7 @@ int hs_net_get_state_monitor(hs_net *net, int mid, double *out, int64_t cap);
  *                  records in cpp_standalone order (by step, then index)
  *   state monitor: a = t (double[rows]), b = values (the network's precision,
  *                  [rows][stop-start]), c = recorded indices (int32), d = rows (int32[1])
+ *   rate monitor : a = t (double[rows]), b = rate (double[rows], Hz), c unused, d = rows
  */
 int hs_net_write_monitor(hs_net *net, int mid, const char *path_a, const char *path_b, const char *path_c,
                          const char *path_d);

@@ -67,6 +67,8 @@ def lib():
         "ref_net_add_pathway": (C.c_int, [V, C.c_int, C.c_int, C.c_int, C.c_int64, I32P, I32P, I32P, C.c_int,
                                           C.c_int, DP, C.c_double]),
         "ref_net_add_spike_monitor": (C.c_int, [V, C.c_int]),
+        "ref_net_add_rate_monitor": (C.c_int, [V, C.c_int, C.c_int, C.c_int]),
+        "ref_mon_get_rate": (C.c_int, [V, C.c_int, DP]),
         "ref_net_add_state_monitor": (C.c_int, [V, C.c_int, C.c_int, C.c_int, C.c_int]),
         "ref_net_run": (C.c_int, [V, C.c_int64]),
         "ref_net_step": (C.c_int64, [V]),
@@ -234,6 +236,19 @@ class RefNet:
     def add_spike_monitor(self, gid):
         return self.L.ref_net_add_spike_monitor(self.h, gid)
 
+    def add_rate_monitor(self, gid, start, stop):
+        m = self.L.ref_net_add_rate_monitor(self.h, gid, start, stop)
+        if m < 0:
+            raise ValueError("bad rate monitor")
+        return m
+
+    def rate_monitor(self, mid):
+        """population rate (Hz) of every step since the monitor was added"""
+        n = self.L.ref_mon_count(self.h, mid)
+        out = np.empty(max(n, 1))
+        self.L.ref_mon_get_rate(self.h, mid, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out[:n]
+
     def add_state_monitor(self, gid, var, start, stop):
         m = self.L.ref_net_add_state_monitor(self.h, gid, var, start, stop)
         if m < 0:
@@ -296,6 +311,7 @@ def build_ref(spec, precision=64, semantics=SEQ):
             net.pathway_stdp(pid, p.stdp)
     smons = [net.add_spike_monitor(g) for g in spec.spike_monitors]
     stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
+    net.rate_mids = [net.add_rate_monitor(*m) for m in getattr(spec, "rate_monitors", [])]
     return net, smons, stmons
 
 

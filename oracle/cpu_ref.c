@@ -534,6 +534,40 @@ static void mon_state(ref_net *net, ref_mon *m, const ref_group *g)
     m->n++;
 }
 
+/* ratemonitor.cu:52-99: spikes of neurons [start, stop) this step (stored as a count) */
+static void mon_rate(ref_net *net, ref_mon *m, const ref_group *g)
+{
+    (void)net;
+    if (m->n + 1 > m->cap) {
+        int64_t c = m->cap ? 2 * m->cap : 1024;
+        m->data = (double *)realloc(m->data, sizeof(double) * c);
+        m->cap = c;
+    }
+    int64_t c = 0;
+    for (int64_t k = 0; k < g->nspk; ++k) c += g->spk[k] >= m->start && g->spk[k] < m->stop;
+    m->data[m->n++] = (double)c;
+}
+
+int ref_net_add_rate_monitor(ref_net *net, int gid, int start, int stop)
+{
+    if (net->nm >= REF_MAX_MONS || gid < 0 || gid >= net->ng) return -1;
+    if (start < 0 || stop > net->g[gid].n || stop <= start) return -1;
+    ref_mon *m = &net->mon[net->nm];
+    memset(m, 0, sizeof(*m));
+    m->kind = 2;
+    m->gid = gid; m->start = start; m->stop = stop;
+    return net->nm++;
+}
+
+/* rate = 1.0*num_spikes/dt/N (ratemonitor.cu:98) */
+int ref_mon_get_rate(ref_net *net, int mid, double *out)
+{
+    if (mid < 0 || mid >= net->nm || net->mon[mid].kind != 2) return -1;
+    ref_mon *m = &net->mon[mid];
+    for (int64_t k = 0; k < m->n; ++k) out[k] = 1.0 * m->data[k] / net->dt / (double)(m->stop - m->start);
+    return 0;
+}
+
 static void queue_push(ref_path *p, int slot, int64_t syn)
 {
     if (p->qlen[slot] == p->qcap[slot]) {

@@ -45,6 +45,8 @@ int ref_net_add_pathway(ref_net *net, int src, int tgt, int var, int64_t nsyn, c
 int ref_net_set_spikegen(ref_net *net, int gid, int64_t n, const int32_t *idx, const int64_t *steps);
 int ref_net_pathway_stdp(ref_net *net, int pid, const double *params, int nparams);
 int ref_net_get_synapse_var(ref_net *net, int pid, int var, double *out);
+int ref_net_add_rate_monitor(ref_net *net, int gid, int start, int stop);
+int ref_mon_get_rate(ref_net *net, int mid, double *out);
 int ref_net_add_spike_monitor(ref_net *net, int gid);
 int ref_net_add_state_monitor(ref_net *net, int gid, int var, int start, int stop);
 int ref_net_run(ref_net *net, int64_t nsteps);

@@ -120,6 +120,9 @@ def test_reference_stdp_examples(hs, oracle_lib, prec, heterog):
     rate_r = np.bincount(rt_, minlength=2000) / (100 * DT)
     if prec == 64:
         np.testing.assert_array_equal(rate_g, rate_r)
+        # PopulationRateMonitor(P), as the reference's test records it
+        np.testing.assert_array_equal(net.rate_monitor(net.rate_mids[0]), ref.rate_monitor(ref.rate_mids[0]))
+        np.testing.assert_array_equal(net.rate_monitor(net.rate_mids[0]), rate_g)
         np.testing.assert_allclose(net.state_monitor(st[0], 10), ref.state_monitor(rst[0], 10), rtol=rtol,
                                    atol=1e-12)
         np.testing.assert_allclose(net.synapse_var(1, rt.HS_SYN_W), ref.synapse_var(1, 0, S.nsyn)[perm], rtol=rtol,

@@ -48,3 +48,40 @@ def test_partial_paths_and_errors(hs, tmp_path):
     with pytest.raises(HsError):
         net.write_monitor(sm[0], (tmp_path / "no_such_dir" / "i", None, None, None))
     net.close()
+
+
+@pytest.mark.parametrize("engine,sem", [("plain", 0), ("fused", 1)])
+def test_rate_monitor(hs, oracle_lib, tmp_path, engine, sem):
+    """PopulationRateMonitor over the whole group: the device counts give the
+    oracle's rates, 1.0*count/dt/N, to the bit; added after 200 steps, it
+    records from then on; its result files follow the format."""
+    spec = nw.brunel(N=3000, K=300, seed=5, monitors=True)
+    net, sm, _ = nw.build_hs(spec, 32, engine)
+    ref, rsm, _ = oracle_lib.build_ref(spec, 32, sem)
+    net.run(200)
+    ref.run(200)
+    a = net.add_rate_monitor(0)
+    ra = ref.add_rate_monitor(0, 0, 3000)
+    net.run(700)
+    ref.run(700)
+    ga, rra = net.rate_monitor(a), ref.rate_monitor(ra)
+    assert ga.size == 700 and np.array_equal(ga, rra) and ga.sum() > 0
+    gi, gt = net.spikes(sm[0])
+    np.testing.assert_allclose(ga * 3000 * spec.dt, np.bincount(gt[gt >= 200] - 200, minlength=700)[:700],
+                               rtol=1e-12)
+    net.write_monitor(a, (tmp_path / "t", tmp_path / "rate", None, tmp_path / "N"))
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "rate", "<f8"), ga)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "t", "<f8"), (200 + np.arange(700)) * spec.dt)
+    net.close()
+
+
+def test_rate_monitor_subgroup(hs, oracle_lib):
+    spec = nw.cuba(N=2000, seed=4)
+    spec.rate_monitors = [(0, 100, 1600)]
+    net, _, _ = nw.build_hs(spec, 64, "plain")
+    ref, _, _ = oracle_lib.build_ref(spec, 64, oracle_lib.SEQ)
+    net.run(1000)
+    ref.run(1000)
+    g, r = net.rate_monitor(net.rate_mids[0]), ref.rate_monitor(ref.rate_mids[0])
+    assert np.array_equal(g, r) and g.sum() > 0
+    net.close()
