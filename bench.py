@@ -319,7 +319,19 @@ def main():
     ap.add_argument("--hines-steps", type=int, default=2000, help="steps of the multicompartment run (0: skip)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-connect", action="store_true", help="skip the device connectivity-build timing")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 256 MiB device-to-device copies (the reference for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 3 x 256 MiB")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the engine has no multi-GPU path (asked for --gpus {args.gpus}). "
               "Run with --gpus 1.", file=sys.stderr)
