@@ -13,7 +13,7 @@ reference's expectations state.
 import numpy as np
 
 from brian2cuda_amd.networks import GroupSpec, NetSpec, PathSpec
-from brian2cuda_amd.runtime import HS_MODEL_LIF_CUBA, HS_W_SCALAR, HS_W_SYNAPSE
+from brian2cuda_amd.runtime import HS_MODEL_LIF_CUBA, HS_MODEL_SPIKEGEN, HS_W_SCALAR, HS_W_SYNAPSE
 
 INF_TAU = 1e30  # exp(-dt/tau) rounds to 1.0: v is exactly constant
 
@@ -88,6 +88,57 @@ def circular_eventspaces(dt=1e-4):
     spec = NetSpec("circular_eventspaces", dt, 1, [g_in, g], paths, spike_monitors=[1])
     expect = {1: np.array([0]), 3: np.array([1]), 4: np.array([3]), 5: np.array([4]), 6: np.array([2])}
     return spec, 7, expect
+
+
+def circular_eventspaces_spikegenerator(sem=0, dt=1e-4):
+    """test_circular_eventspaces_spikegenerator (test_synaptic_propagations.py:185-229):
+    a SpikeGeneratorGroup neuron fires at steps 0..11 into five targets with
+    delays 0, 2, 5 (homogeneous) and 3, 4 (heterogeneous); each arrival adds 1.1.
+    The reference runs it with resets scheduled before synapses, so a target
+    spikes at every step d+1 .. d+12.  Brian2's default schedule (synapses, then
+    resets) makes it spike every second step, d+1, d+3, .., d+11: the arrival
+    of the spiking step is erased by the reset (brian2 issue 1332, cited there).
+    Both semantics here restate the default schedule (ACC adds a step's effects
+    at the start of the next step, before the deferred reset), so both expect
+    every second step; sem is kept for symmetry with the other cases."""
+    n = 12
+    g_in = GroupSpec(HS_MODEL_SPIKEGEN, 1, [0.0], spikes=(np.zeros(n, np.int32), np.arange(n, dtype=np.int64)))
+    g = GroupSpec(HS_MODEL_LIF_CUBA, 5, static_params(), {0: 0.0, 1: 0.0, 2: 0.0})
+    z = np.zeros(1, np.int32)
+    paths = [
+        PathSpec(0, 1, 0, z, np.array([0], np.int32), scalar_delay=0, wscalar=1.1),
+        PathSpec(0, 1, 0, z, np.array([1], np.int32), scalar_delay=2, wscalar=1.1),
+        PathSpec(0, 1, 0, z, np.array([2], np.int32), scalar_delay=5, wscalar=1.1),
+        PathSpec(0, 1, 0, np.zeros(2, np.int32), np.array([3, 4], np.int32),
+                 delay_steps=np.array([3, 4], np.int32), wscalar=1.1),
+    ]
+    spec = NetSpec("circular_eventspaces_spikegenerator", dt, 1, [g_in, g], paths, spike_monitors=[1])
+    expect = {}
+    for j, d in enumerate([0, 2, 5, 3, 4]):
+        steps = range(d + 1, d + n, 2)
+        for t in steps:
+            expect.setdefault(t, []).append(j)
+    expect = {k: np.sort(np.array(v)) for k, v in expect.items()}
+    return spec, n + 6, expect
+
+
+def multiple_runs(delays, dt=1e-4):
+    """test_multiple_runs_with_scalar_delay / _with_heterogeneous_delay
+    (test_network_multiple_runs.py:12-31, :91-108): an input neuron spiking at
+    every step (threshold 'True': a generator with a one-step period) adds 1 to
+    'v : 1' of len(delays) targets through synapses of the given delays; a
+    StateMonitor (when='start') records v over two runs of two steps.
+    Returns the spec and the expected trace [target][step]."""
+    g_in = GroupSpec(HS_MODEL_SPIKEGEN, 1, [dt], spikes=(np.zeros(1, np.int32), np.zeros(1, np.int64)))
+    n = len(delays)
+    params = static_params()
+    params[1] = 1e30  # 'v : 1' without threshold
+    g = GroupSpec(HS_MODEL_LIF_CUBA, n, params, {0: 0.0, 1: 0.0, 2: 0.0})
+    p = PathSpec(0, 1, 0, np.zeros(n, np.int32), np.arange(n, dtype=np.int32),
+                 delay_steps=np.array(delays, np.int32), wscalar=1.0)
+    spec = NetSpec("multiple_runs", dt, 1, [g_in, g], [p], state_monitors=[(1, 0, 0, n)])
+    expect = np.array([[max(0, t - d) for t in range(4)] for d in delays], dtype=np.float64)
+    return spec, expect
 
 
 def effects_cancel(n_pre=15360, n_post=64, dt=1e-4):

@@ -172,6 +172,30 @@ def test_reference_propagation_cases_gpu(hs, case, engine, prec):
         np.testing.assert_array_equal(got[k], expect[k])
 
 
+@pytest.mark.parametrize("engine,sem", [("plain", 0), ("fused", 1)])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_reference_circular_eventspaces_spikegenerator_gpu(hs, engine, sem, prec):
+    import refcases as rc
+    spec, steps, expect = rc.circular_eventspaces_spikegenerator(sem)
+    net, sm, _ = nw.build_hs(spec, prec, engine)
+    net.run(steps)
+    got = rc.spikes_by_step(*net.spikes(sm[0]))
+    assert sorted(got) == sorted(expect)
+    for k in expect:
+        np.testing.assert_array_equal(got[k], expect[k])
+
+
+@pytest.mark.parametrize("delays", [[1], [0, 1]])
+@pytest.mark.parametrize("engine", ["plain", "fused"])
+def test_reference_multiple_runs_gpu(hs, delays, engine):
+    import refcases as rc
+    spec, expect = rc.multiple_runs(delays)
+    net, _, st = nw.build_hs(spec, 64, engine)
+    net.run(2)
+    net.run(2)
+    np.testing.assert_array_equal(net.state_monitor(st[0], len(delays)).T, expect)
+
+
 @pytest.mark.parametrize("engine", ["plain", "fused"])
 def test_reference_effects_cancel_gpu(hs, engine):
     """test_synaptic_effect_modes: +1 and -1 onto every target each step cancel exactly."""

@@ -260,6 +260,29 @@ def test_reference_propagation_cases(case, sem):
         np.testing.assert_array_equal(got[k], expect[k])
 
 
+@pytest.mark.parametrize("sem", [oracle.SEQ, oracle.ACC])
+def test_reference_circular_eventspaces_spikegenerator(sem):
+    spec, steps, expect = rc.circular_eventspaces_spikegenerator(sem)
+    ref, sm, _ = oracle.build_ref(spec, 64, sem)
+    ref.run(steps)
+    got = rc.spikes_by_step(*ref.spikes(sm[0]))
+    assert sorted(got) == sorted(expect)
+    for k in expect:
+        np.testing.assert_array_equal(got[k], expect[k])
+
+
+@pytest.mark.parametrize("delays", [[1], [0, 1]])
+@pytest.mark.parametrize("sem", [oracle.SEQ, oracle.ACC])
+def test_reference_multiple_runs(delays, sem):
+    """test_multiple_runs_with_scalar_delay ([[0, 0, 1, 2]]) and
+    _with_heterogeneous_delay ([0, 1, 2, 3], [0, 0, 1, 2]): two runs of two steps."""
+    spec, expect = rc.multiple_runs(delays)
+    ref, _, st = oracle.build_ref(spec, 64, sem)
+    ref.run(2)
+    ref.run(2)
+    np.testing.assert_array_equal(ref.state_monitor(st[0], len(delays)).T, expect)
+
+
 def test_reference_effects_cancel():
     spec, steps, _ = rc.effects_cancel()
     for sem in (oracle.SEQ, oracle.ACC):
