@@ -123,11 +123,11 @@ def run_form(spec, prec, engine, steps, warmup, options=None):
     return res
 
 
-def kernel_times(spec, prec, engine, steps, warmup):
+def kernel_times(spec, prec, engine, steps, warmup, options=None):
     """Per-kernel device time with an event pair around every launch."""
     from brian2cuda_amd import networks as nw
     from brian2cuda_amd import runtime as rt
-    net, _, _ = nw.build_hs(spec, prec, engine)
+    net, _, _ = nw.build_hs(spec, prec, engine, options=options)
     net.run(warmup)
     net.set_option(rt.OPT_KERNEL_TIMING, 1)
     s0 = net.stats()
@@ -250,15 +250,16 @@ def measured_traffic(args, kernel):
     (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each scaled by what the counter
     reads for a plain copy of known size), for the workload it was measured on;
     None otherwise."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "brunel_c3_deliver_traffic.json")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "brunel_c3_traffic.json")
     if args.workload != "brunel" or args.N != 100_000 or args.K != 1000 or not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    if t["kernel"] != kernel:
+    k = t["kernels"].get(kernel)
+    if k is None:
         return None
-    return int(1024 * (t["fetch_size_kib_per_dispatch"] * t["fetch_scale"] +
-                       t["write_size_kib_per_dispatch"] * t["write_scale"]))
+    return int(1024 * (k["fetch_size_kib_per_dispatch"] * t["fetch_scale"] +
+                       k["write_size_kib_per_dispatch"] * t["write_scale"]))
 
 
 def connect_bench(args, repeats=3):
@@ -319,6 +320,11 @@ def main():
     ap.add_argument("--hines-steps", type=int, default=2000, help="steps of the multicompartment run (0: skip)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-connect", action="store_true", help="skip the device connectivity-build timing")
+    ap.add_argument("--options", default=None,
+                    help='JSON {"key": value} of engine options for the FP32 batched form instead of autotuning '
+                         '(e.g. what a previous run reported as tuned_options.fp32_batched)')
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time both forms with the engine's default launch shapes instead of autotuned ones")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three 256 MiB device-to-device copies (the reference for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc) and exit")
@@ -349,17 +355,32 @@ def main():
     log(f"built {spec.name}: N={spec.n_neurons} S={spec.n_synapses} in {spec_s:.1f}s")
     D = max((p.delay_steps.max() - p.delay_steps.min() + 1) if p.delay_steps is not None else 1 for p in spec.paths)
 
-    fp32 = run_form(spec, 32, "fused", args.steps, args.warmup)
+    tuned = {"fp32_batched": {}, "fp64_plain": {}}
+    if args.options:
+        tuned["fp32_batched"] = {int(k): v for k, v in json.loads(args.options).items()}
+    elif not args.no_autotune:
+        from brian2cuda_amd import autotune as at
+        from brian2cuda_amd import networks as nw
+
+        def build(sp, prec, engine, options=None):  # same tables as the host build, bundled on the device
+            return nw.build_hs(sp, prec, engine, options=options, device_sort=True)
+
+        t0 = time.time()
+        tuned["fp32_batched"] = at.autotune(spec, 32, "fused", build=build, log=log)
+        if not args.no_fp64:
+            tuned["fp64_plain"] = at.autotune(spec, 64, "plain", build=build, log=log)
+        log(f"autotuned in {time.time() - t0:.1f}s: {tuned}")
+    fp32 = run_form(spec, 32, "fused", args.steps, args.warmup, options=tuned["fp32_batched"])
     log("fp32 batched", fp32)
     fp64 = None
     if not args.no_fp64:
-        fp64 = run_form(spec, 64, "plain", args.steps, args.warmup)
+        fp64 = run_form(spec, 64, "plain", args.steps, args.warmup, options=tuned["fp64_plain"])
         log("fp64 plain", fp64)
 
     N = spec.n_neurons
     roofline = None
     if args.kt_steps > 0:
-        kt, W, n_spk, E = kernel_times(spec, 32, "fused", args.kt_steps, args.warmup)
+        kt, W, n_spk, E = kernel_times(spec, 32, "fused", args.kt_steps, args.warmup, options=tuned["fp32_batched"])
         log("kernel times (fp32 batched)", kt)
         r_s, w_s = state_bytes(args.workload, 32)
         w_b = weight_bytes(spec, 32)
@@ -430,6 +451,7 @@ def main():
                                                if fp64 and fp64["events_per_s"] > 0 else None),
         "drift_fp32_batched_vs_fp64_plain": dr,
         "roofline": roofline,
+        "tuned_options": {k: {str(kk): vv for kk, vv in v.items()} for k, v in tuned.items()},
         "multicompartment": hines,
         "connectivity_build": conn,
         "cpu_baseline": cpu,

@@ -408,7 +408,7 @@ struct hs_net {
     /* 0 update block, 1 scatter grid, 2 graph chunk, 3 fused delivery blocks,
        4 window, 5 delivery mode, 6 gather block, 7 kernel timing, 8 noise blocks,
        9 pipelined windows */
-    long long opt[10] = {256, 0, 256, 0, 0, 0, 256, 0, 0, 0};
+    long long opt[11] = {256, 0, 256, 0, 0, 0, 256, 0, 0, 0, 0};
     /* kernel timing: event pairs of the current run, totals per kind */
     std::vector<hipEvent_t> kt_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> kt_open;
@@ -1219,7 +1219,7 @@ extern "C" int hs_net_get_rate_monitor(hs_net *net, int mid, double *rate, int64
 
 extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
 {
-    if (!net || key < 0 || key > 9) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (!net || key < 0 || key > 10) return hs_fail(HS_ERR_ARG, "bad option key");
     if (key == 7 && (value < 0 || value > 1)) return hs_fail(HS_ERR_ARG, "kernel timing: 0 off, 1 on");
     if (key == 8 && (value < 0 || value > (1 << 16))) return hs_fail(HS_ERR_ARG, "bad noise block count");
     if (key == 9 && (value < 0 || value > 1)) return hs_fail(HS_ERR_ARG, "pipelined windows: 0 off, 1 on");
@@ -1234,7 +1234,8 @@ extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
     if (key == 5 && (value < 0 || value > 2)) return hs_fail(HS_ERR_ARG, "delivery mode: 0 auto, 1 scatter, 2 gather");
     if (key == 0 && value != 64 && value != 128 && value != 256)
         return hs_fail(HS_ERR_ARG, "update block size must be 64, 128 or 256 (a divisor of the 256-neuron chunk)");
-    if ((key == 1 || key == 3) && (value < 0 || value > (1 << 20))) return hs_fail(HS_ERR_ARG, "bad grid size");
+    if ((key == 1 || key == 3 || key == 10) && (value < 0 || value > (1 << 20)))
+        return hs_fail(HS_ERR_ARG, "bad grid size");
     if (key == 2 && (value < 0 || value > 100000)) return hs_fail(HS_ERR_ARG, "bad graph chunk");
     if (key == 4 && (value < 0 || value > HS_MAX_DELAY_SLOTS)) return hs_fail(HS_ERR_ARG, "bad window");
     if (key == 4 && net->built) return hs_fail(HS_ERR_STATE, "the window is fixed at the first run");
@@ -1246,7 +1247,7 @@ extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
 
 extern "C" int hs_net_get_option(hs_net *net, int key, long long *value)
 {
-    if (!net || key < 0 || key > 9 || !value) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (!net || key < 0 || key > 10 || !value) return hs_fail(HS_ERR_ARG, "bad option key");
     *value = net->opt[key];
     return HS_OK;
 }
@@ -1513,9 +1514,11 @@ static int hs_noise_grid(const hs_net *net)
     return g_rt.n_cu * 2;
 }
 
+/* one block per tile, or at most opt[10] blocks striding over the tiles */
 static int hs_update_grid(const hs_net *net)
 {
-    return std::max(1, std::min(net->total_tiles, 1 << 20));
+    const int cap = net->opt[10] > 0 ? (int)net->opt[10] : (1 << 20);
+    return std::max(1, std::min(net->total_tiles, cap));
 }
 
 /* kernel timing (option 7): an event pair around one launch of kind `kind` */

@@ -39,7 +39,7 @@ HS_SYN_W, HS_SYN_APRE, HS_SYN_APOST, HS_SYN_LASTUPDATE = 0, 1, 2, 3
 NEVER_SPIKED = -1073741824
 
 OPT_UPDATE_BLOCK, OPT_DELIVER_GRID, OPT_GRAPH_CHUNK, OPT_FUSED_GRID, OPT_WINDOW, OPT_DELIVERY = 0, 1, 2, 3, 4, 5
-OPT_GATHER_BLOCK, OPT_KERNEL_TIMING, OPT_NOISE_BLOCKS, OPT_PIPELINED = 6, 7, 8, 9
+OPT_GATHER_BLOCK, OPT_KERNEL_TIMING, OPT_NOISE_BLOCKS, OPT_PIPELINED, OPT_UPDATE_GRID = 6, 7, 8, 9, 10
 DELIVERY_AUTO, DELIVERY_SCATTER, DELIVERY_GATHER = 0, 1, 2
 KT_UPDATE, KT_SCATTER, KT_GATHER, KT_FUSED, KT_WINDOW = 0, 1, 2, 3, 4
 KERNEL_NAMES = {KT_UPDATE: "hs_k_update", KT_SCATTER: "hs_k_deliver", KT_GATHER: "hs_k_gather", KT_FUSED: "hs_k_fused",

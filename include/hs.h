@@ -243,7 +243,9 @@ int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start, int stop,
  * 8 blocks drawing a window's noise beside the delivery or update (0: 2 per CU),
  * 9 pipelined windows (1: one launch per window -- update and push of the
  * delays < 2W, pull of the longer delays for the next window; 0, the default:
- * a delivery launch and an update launch per window). */
+ * a delivery launch and an update launch per window), 10 update grid (0: one
+ * block per tile; n > 0: at most n blocks, each striding over tiles).
+ * brian2cuda_amd/autotune.py picks keys 0, 1, 3, 4, 8, 9, 10 by measurement. */
 int hs_net_set_option(hs_net *net, int key, long long value);
 int hs_net_get_option(hs_net *net, int key, long long *value);
 /* With option 7 on: device time and launch count, summed over the runs since
