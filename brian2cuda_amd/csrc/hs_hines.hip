@@ -30,6 +30,10 @@
 #define HS_HINES_MAXC 1024
 #define HS_HINES_MAXS 255
 #define HS_HINES_RL 4 /* children per node */
+#define HS_HINES_TU 16 /* sections up to this length run the unrolled Thomas sweep */
+#ifndef HS_HINES_PF
+#define HS_HINES_PF 4 /* its prefetch distance for the off-diagonal bands */
+#endif
 
 template <typename R>
 struct HinesDesc {
@@ -64,8 +68,12 @@ __device__ __forceinline__ void hs_mc_chan<float>(float v, float *g, const float
     hs_mc_channels_f32(v, g, k, gt, i0);
 }
 
+/* up to 8 compartments per lane: two waves per SIMD (2048 trees then fit the
+   chip at once: 256 CUs x 4 SIMDs x 2); the register budget this sets covers
+   the unrolled Thomas sweep without spills.  16 per lane: one wave per SIMD and
+   the plain sweep (the unrolled one would spill) */
 template <typename R, int CPL>
-__global__ __launch_bounds__(64) void hs_k_hines(HinesDesc<R> d, int nsteps)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CPL <= 8 ? 2 : 1))) void hs_k_hines(HinesDesc<R> d, int nsteps)
 {
     extern __shared__ double lds[];
     const int tree = blockIdx.x, lane = threadIdx.x;
@@ -160,6 +168,80 @@ __global__ __launch_bounds__(64) void hs_k_hines(HinesDesc<R> d, int nsteps)
         for (int s = lane; s < ns; s += HS_WAVE) {
             const int js = fst[s], je = lst[s], last = je - 1;
             double cp = 0.0, vp = 0.0, upp = 0.0, ump = 0.0;
+            if (CPL <= 8 && je - js <= HS_HINES_TU) {
+                /* a short section: the two off-diagonal bands (global memory,
+                   an L2 round trip each) are loaded HS_HINES_PF compartments
+                   ahead, not one; the diagonal and right-hand side come from LDS
+                   as below; same operations, same order */
+                double Ga[HS_HINES_TU], Gc[HS_HINES_TU];
+#pragma unroll
+                for (int q = 0; q < HS_HINES_TU; ++q) Ga[q] = Gc[q] = 0.0;
+#pragma unroll
+                for (int q = 0; q < HS_HINES_PF; ++q) {
+                    const int j = js + q;
+                    if (j < je) {
+                        Gc[q] = (j < nc - 1) ? gab0[j + 1] : 0.0;
+                        Ga[q] = j > 0 ? gab2[j - 1] : 0.0;
+                    }
+                }
+                const double bp = bps[s], bm = bms[s];
+#pragma unroll
+                for (int q = 0; q < HS_HINES_TU; ++q) {
+                    const int j = js + q;
+                    if (q + HS_HINES_PF < HS_HINES_TU && j + HS_HINES_PF < je) {
+                        const int jn = j + HS_HINES_PF;
+                        Gc[q + HS_HINES_PF] = (jn < nc - 1) ? gab0[jn + 1] : 0.0;
+                        Ga[q + HS_HINES_PF] = gab2[jn - 1];
+                    }
+                    if (j < je) {
+                        const double b = A[j], ai = Ga[q];
+                        double cj = Gc[q], vs = Vs[j];
+                        double up = (j == last) ? bp : 0.0;
+                        double um = (q == 0) ? bm : 0.0;
+                        if (j > 0) {
+                            if (q == 0) {
+                                const double m = 1.0 / b;
+                                cj = cj * m;
+                                vs = vs * m;
+                                up = up * m;
+                                um = um * m;
+                            } else {
+                                const double m = 1.0 / (b - ai * cp);
+                                cj = cj * m;
+                                vs = (vs - ai * vp) * m;
+                                up = (up - ai * upp) * m;
+                                um = (um - ai * ump) * m;
+                            }
+                        } else {
+                            cj = cj / b;
+                            vs = vs / b;
+                            up = up / b;
+                            um = um / b;
+                        }
+                        A[j] = cj;
+                        Vs[j] = vs;
+                        Up[j] = up;
+                        Um[j] = um;
+                        cp = cj;
+                        vp = vs;
+                        upp = up;
+                        ump = um;
+                    }
+                }
+#pragma unroll
+                for (int q = HS_HINES_TU - 2; q >= 0; --q) {
+                    const int j = js + q;
+                    if (j <= je - 2) {
+                        vp = Vs[j] - A[j] * vp;
+                        upp = Up[j] - A[j] * upp;
+                        ump = Um[j] - A[j] * ump;
+                        Vs[j] = vp;
+                        Up[j] = upp;
+                        Um[j] = ump;
+                    }
+                }
+                continue;
+            }
             /* the inputs of compartment j+1 are loaded while j is computed (they
                are not written before their own iteration) */
             double b_n = A[js], vs_n = Vs[js];
