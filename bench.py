@@ -37,7 +37,11 @@ spikes of each neuron are matched greedily in time order (pairs within 5 ms) and
 the RMS of the matched time differences is reported with the unmatched fraction,
 with each form's population rate and mean ISI CV, and the same matching for the
 FP64 fused form against FP64 plain (no change of arithmetic: the divergence the
-network's own sensitivity produces from a one-step reordering of additions).
+network's own sensitivity produces from a one-step reordering of additions), and
+for FP64 plain with neuron 0's initial V moved by one FP32 ulp, or by one
+synaptic event (|J|), against FP64 plain; the FP32 and the perturbed comparisons
+are also given over the first 500,
+1000, 2000 and 5000 steps.
 
 Multicompartment: `--hines-trees` random branched HH trees (100-400
 compartments) advanced `--hines-steps` steps, FP64 and FP32 channel arithmetic.
@@ -171,8 +175,10 @@ def train_stats(i, t, n, steps, dt):
 def drift(spec, steps, tol_steps=50):
     """Greedy time-ordered matching of FP32-batched to FP64-plain spikes per neuron,
     and the same for FP64 fused vs FP64 plain (same precision, effects applied one
-    step later): the second isolates the divergence that the network's own
-    sensitivity produces from any change of arithmetic."""
+    step later), and for FP64 plain with one initial value moved by one FP32 ulp
+    or by one synaptic event vs FP64 plain: the last two separate the divergence that the network's own
+    sensitivity produces from any change of arithmetic.  by_horizon: the first
+    and last comparison over the first h steps."""
     i64, t64 = spike_trains(spec, 64, "plain", steps)
     i32, t32 = spike_trains(spec, 32, "fused", steps)
     res = match_spikes(i64, t64, i32, t32, spec.n_neurons, tol_steps)
@@ -181,6 +187,35 @@ def drift(spec, steps, tol_steps=50):
     res["fp32_batched"] = train_stats(i32, t32, spec.n_neurons, steps, spec.dt)
     f64, g64 = spike_trains(spec, 64, "fused", steps)
     res["fp64_fused_vs_fp64_plain"] = match_spikes(i64, t64, f64, g64, spec.n_neurons, tol_steps)
+    # FP64 plain against itself with neuron 0's initial V raised by (a) one FP32
+    # ulp and (b) |J| = 0.1 mV, one synaptic event: how far the network alone
+    # carries a perturbation of FP32's rounding size, and one that moves a spike
+    g0 = spec.groups[0]
+    base = np.full(g0.n, g0.init.get(0, 0.0), dtype=np.float64)
+    pert = {}
+    for name, eps in (("fp32_ulp", float(np.spacing(np.float32(base[0])))), ("one_event", 1e-4)):
+        v0 = base.copy()
+        v0[0] += eps
+        sp1 = dataclasses.replace(spec, groups=[dataclasses.replace(g0, init={**g0.init, 0: v0})]
+                                  + list(spec.groups[1:]))
+        u64, w64 = spike_trains(sp1, 64, "plain", steps)
+        pert[name] = (u64, w64)
+        r = match_spikes(i64, t64, u64, w64, spec.n_neurons, tol_steps)
+        r["perturbation"] = {"neuron": 0, "var": "V", "delta": eps}
+        res[f"fp64_plain_{name}_vs_fp64_plain"] = r
+    # the comparisons over the first h steps only
+    res["by_horizon"] = {}
+    for h in (500, 1000, 2000, 5000):
+        if h >= steps:
+            continue
+        a, b = t64 < h, t32 < h
+        r32 = match_spikes(i64[a], t64[a], i32[b], t32[b], spec.n_neurons, tol_steps)
+        row = {"fp32_batched": {"rms_dt_steps": r32["rms_dt_steps"], "unmatched_frac": r32["unmatched_frac"]}}
+        for name, (u64, w64) in pert.items():
+            c = w64 < h
+            r1 = match_spikes(i64[a], t64[a], u64[c], w64[c], spec.n_neurons, tol_steps)
+            row[f"fp64_plain_{name}"] = {"rms_dt_steps": r1["rms_dt_steps"], "unmatched_frac": r1["unmatched_frac"]}
+        res["by_horizon"][str(h)] = row
     return res
 
 

@@ -7,6 +7,10 @@ properties that do not depend on size.
              the synapses of i whose arrival step s + d falls inside the run)
   refractoriness : two spikes of one neuron are >= 2 ms apart
   forms    : FP32 batched and FP64 plain fire at the same rate (within 3 %)
+and, over the first 1000 steps (100 ms: the oracle finishes them in seconds),
+bit for bit against the oracle in the form's own precision and semantics: every
+spike, the final state and the event count, with default and with tuned
+launch shapes.
 """
 import dataclasses
 
@@ -63,3 +67,27 @@ def test_fullsize_forms_fire_alike(hs, c3):
     n32 = len(_run(spec, 32, "fused", STEPS)[0])
     n64 = len(_run(spec, 64, "plain", STEPS)[0])
     assert abs(n32 - n64) <= 0.03 * n64
+
+
+# the launch shapes the autotuner picks for the FP32 form on this network
+# (pipelined windows, 64-thread update tiles, 768 noise and 768 pull blocks)
+TUNED_FP32 = {9: 1, 0: 64, 8: 768, 1: 768}
+
+
+@pytest.mark.parametrize("prec,engine,sem,options", [(64, "plain", 0, None), (32, "fused", 1, None),
+                                                     (32, "fused", 1, TUNED_FP32), (64, "plain", 0, {0: 64})])
+def test_fullsize_bit_exact_vs_oracle(hs, oracle_lib, c3, prec, engine, sem, options):
+    spec = c3[0]
+    steps = 1000
+    net, smon, _ = nw.build_hs(spec, prec, engine, spike_capacity=1 << 22, options=options)
+    ref, rsm, _ = oracle_lib.build_ref(spec, prec, sem)
+    net.run(steps)
+    ref.run(steps)
+    gi, gt = net.spikes(smon[0])
+    ri, rt_ = ref.spikes(rsm[0])
+    assert len(ri) > 10_000
+    assert np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    for v in range(hs.MODEL_NVARS[spec.groups[0].model]):
+        assert np.array_equal(net.get_state(0, v), ref.get_state(0, v)), f"var {v}"
+    assert net.stats()["events"] == ref.events
+    net.close()
