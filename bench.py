@@ -38,8 +38,8 @@ the RMS of the matched time differences is reported with the unmatched fraction,
 with each form's population rate and mean ISI CV, and the same matching for the
 FP64 fused form against FP64 plain (no change of arithmetic: the divergence the
 network's own sensitivity produces from a one-step reordering of additions), and
-for FP64 plain with neuron 0's initial V moved by one FP32 ulp, or by one
-synaptic event (|J|), against FP64 plain; the FP32 and the perturbed comparisons
+for FP64 plain with neuron 0's initial V moved by one FP32 ulp, by one
+synaptic event (|J|) or by ten, against FP64 plain; the FP32 and the perturbed comparisons
 are also given over the first 500,
 1000, 2000 and 5000 steps.
 
@@ -175,8 +175,8 @@ def train_stats(i, t, n, steps, dt):
 def drift(spec, steps, tol_steps=50):
     """Greedy time-ordered matching of FP32-batched to FP64-plain spikes per neuron,
     and the same for FP64 fused vs FP64 plain (same precision, effects applied one
-    step later), and for FP64 plain with one initial value moved by one FP32 ulp
-    or by one synaptic event vs FP64 plain: the last two separate the divergence that the network's own
+    step later), and for FP64 plain with one initial value moved by one FP32 ulp,
+    one synaptic event or ten vs FP64 plain: the last two separate the divergence that the network's own
     sensitivity produces from any change of arithmetic.  by_horizon: the first
     and last comparison over the first h steps."""
     i64, t64 = spike_trains(spec, 64, "plain", steps)
@@ -188,12 +188,13 @@ def drift(spec, steps, tol_steps=50):
     f64, g64 = spike_trains(spec, 64, "fused", steps)
     res["fp64_fused_vs_fp64_plain"] = match_spikes(i64, t64, f64, g64, spec.n_neurons, tol_steps)
     # FP64 plain against itself with neuron 0's initial V raised by (a) one FP32
-    # ulp and (b) |J| = 0.1 mV, one synaptic event: how far the network alone
-    # carries a perturbation of FP32's rounding size, and one that moves a spike
+    # ulp, (b) |J| = 0.1 mV (one synaptic event) and (c) 1 mV, which moves that
+    # neuron's first spike: how far the network alone carries each
     g0 = spec.groups[0]
     base = np.full(g0.n, g0.init.get(0, 0.0), dtype=np.float64)
     pert = {}
-    for name, eps in (("fp32_ulp", float(np.spacing(np.float32(base[0])))), ("one_event", 1e-4)):
+    for name, eps in (("fp32_ulp", float(np.spacing(np.float32(base[0])))), ("one_event", 1e-4),
+                      ("ten_events", 1e-3)):
         v0 = base.copy()
         v0[0] += eps
         sp1 = dataclasses.replace(spec, groups=[dataclasses.replace(g0, init={**g0.init, 0: v0})]
