@@ -64,3 +64,14 @@ def test_hines_cable_and_state_roundtrip(hs):
     v = np.linspace(-0.01, 0.01, tb.n_comp)
     g.set_state(0, v)
     np.testing.assert_array_equal(g.get_state(0), v)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_hines_full_batch(hs, prec):
+    """The timed batch itself (2048 trees, 528,986 compartments, seed 1234) over
+    40 steps in two launches: every compartment's state and every soma's spike
+    count bit for bit."""
+    tb = mo.random_trees(n_trees=2048, seed=1234)
+    g, r = _run_both(tb, prec, 40, chunks=[25, 15])
+    assert r.spike_counts().sum() > 0
+    _assert_same(g, r)
