@@ -31,6 +31,9 @@ of a launch, with n_spk and E the measured means of the timed run:
   window  : update + delivery (the pipelined form does both in one launch)
 The average launch duration comes from HIP events around every launch on the
 runtime's stream (hs_net_set_option(net, 7, 1): no graphs in that run).
+roofline.atomics sets the same launch's synaptic events per second (one
+accumulator atomic each) against hs_scatter_add, random FP32 atomics alone into
+an array of N values: the path is bound by atomics, not by HBM bytes.
 
 Drift: both forms are re-run with spike monitors over `--drift-steps` steps;
 spikes of each neuron are matched greedily in time order (pairs within 5 ms) and
@@ -218,6 +221,29 @@ def drift(spec, steps, tol_steps=50):
             row[f"fp64_plain_{name}"] = {"rms_dt_steps": r1["rms_dt_steps"], "unmatched_frac": r1["unmatched_frac"]}
         res["by_horizon"][str(h)] = row
     return res
+
+
+def atomic_ceiling(n_target, ev_launch, launch_us, seed):
+    """The dominant kernel's synaptic events per second (one accumulator atomic
+    each) against random FP32 atomics alone (hs_scatter_add) into an array of the
+    same size, both timed with an event pair around the launch: at the launch's
+    own event count (one launch, its fixed cost included) and at 64 times that
+    count (the sustained rate)."""
+    from brian2cuda_amd import runtime as rt
+    rng = np.random.default_rng(seed)
+    out = {"events_per_launch": int(ev_launch), "achieved_per_s": ev_launch / (launch_us * 1e-6)}
+    for key, n in (("same_count", int(ev_launch)), ("sustained", 64 * int(ev_launch))):
+        idx = rt.DeviceArray.from_numpy(rng.integers(0, n_target, max(n, 1)).astype(np.int32))
+        acc = rt.DeviceArray(n_target, np.float32)
+        acc.zero()
+        rt.scatter_add(idx, 1e-4, acc)
+        ms = min(rt.scatter_add(idx, 1e-4, acc) for _ in range(5))
+        out[f"scatter_{key}_us"] = round(ms * 1e3, 3)
+        out[f"scatter_{key}_per_s"] = n / (ms * 1e-3)
+        idx.free()
+        acc.free()
+    out["frac_of_sustained"] = round(out["achieved_per_s"] / out["scatter_sustained_per_s"], 4)
+    return out
 
 
 def match_spikes(i64, t64, i32, t32, n, tol_steps):
@@ -436,6 +462,8 @@ def main():
                     "kernel": name, "bytes_per_launch": int(bpl), "steps_per_launch": round(steps_per_launch, 3),
                     "avg_launch_us": round(kt[name]["avg_us"], 3),
                     "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kt.items()}}
+        roofline["atomics"] = atomic_ceiling(N, E * steps_per_launch, kt[name]["avg_us"], args.seed)
+        log("atomics", roofline["atomics"])
 
     dr = None
     if args.drift_steps > 0 and not args.no_fp64 and args.workload == "brunel":

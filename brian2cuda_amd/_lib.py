@@ -66,6 +66,7 @@ SIGNATURES = {
     "hs_deliver": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, C.c_int64, P, P, C.c_int, P,
                              C.c_double, P, P]),
     "hs_math_eval": (C.c_int, [C.c_int, C.c_int, P, P]),
+    "hs_scatter_add": (C.c_int, [C.c_int, C.c_int64, P, C.c_int, C.c_double, P, C.POINTER(C.c_double)]),
     "hs_net_create": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_uint64, C.POINTER(C.c_void_p)]),
     "hs_net_destroy": (None, [P]),
     "hs_net_add_group": (C.c_int, [P, C.c_int, C.c_int, DP, C.c_int, C.POINTER(C.c_int)]),

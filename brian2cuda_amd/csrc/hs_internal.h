@@ -42,7 +42,8 @@ struct HsCtl {
     long long base;            /* simulation step of chunk offset 0 */
     long long no_reset_before; /* resets of spikes at steps < this were already applied */
     unsigned long long pending; /* pipelined windows: events of emitted spikes still in flight at base */
-    unsigned long long pad_[13];
+    unsigned long long scatter_bad; /* hs_scatter_add: indices outside the target array */
+    unsigned long long pad_[12];
     unsigned long long ev_slot[HS_EV_SLOTS * HS_EV_STRIDE];  /* synaptic events delivered: sum of ev_slot[k*STRIDE] */
     unsigned long long spk_slot[HS_EV_SLOTS * HS_EV_STRIDE]; /* spikes emitted, spread the same way */
 };
@@ -215,6 +216,8 @@ hipError_t hs_launch_stateupdate(int model, int prec, int n, void *const *vars, 
                                  const double *k, int nk, unsigned long long seed, unsigned int stream,
                                  int ref_steps, long long step, hipStream_t s);
 hipError_t hs_launch_math(int fn, int n, const double *x, double *y, hipStream_t s);
+hipError_t hs_launch_scatter_add(int prec, const int32_t *idx, long long n, int n_target, double w, void *target,
+                                 unsigned long long *bad, hipStream_t s);
 
 /* one pathway at step ctl->base, effects in place */
 template <typename R>

@@ -233,3 +233,36 @@ template hipError_t hs_launch_deliver_one_ctl<double>(const DPath<double> &, HsC
                                                       hipStream_t);
 template hipError_t hs_launch_deliver_one_ctl<float>(const DPath<float> &, HsCtl *, unsigned long long *, int,
                                                      hipStream_t);
+
+/* effect application of a flat event list: target[idx[k]] += w for k < n with
+   hardware atomics (no spike list, no bundle lookup: the throughput of random
+   atomics alone).  An index outside [0, n_target) is skipped and counted in bad. */
+template <typename R>
+__global__ __launch_bounds__(256) void hs_k_scatter_add(const int32_t *__restrict__ idx, long long n, int n_target,
+                                                         R w, R *__restrict__ target, unsigned long long *bad)
+{
+    unsigned long long nb = 0;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int t = idx[k];
+        if ((unsigned int)t < (unsigned int)n_target) unsafeAtomicAdd(&target[t], w);
+        else ++nb;
+    }
+    nb = hs_wave_sum(nb);
+    if (hs_lane() == 0 && nb) atomicAdd(bad, nb);
+}
+
+hipError_t hs_launch_scatter_add(int prec, const int32_t *idx, long long n, int n_target, double w, void *target,
+                                 unsigned long long *bad, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const long long g = (n + 255) / 256;
+    const int grid = (int)(g > 65536 ? 65536 : g);
+    if (prec == 64)
+        hipLaunchKernelGGL(hs_k_scatter_add<double>, dim3(grid), dim3(256), 0, s, idx, n, n_target, w,
+                           (double *)target, bad);
+    else
+        hipLaunchKernelGGL(hs_k_scatter_add<float>, dim3(grid), dim3(256), 0, s, idx, n, n_target, (float)w,
+                           (float *)target, bad);
+    return hipGetLastError();
+}

@@ -331,6 +331,40 @@ extern "C" int hs_deliver(int precision, int n_src, int dmin, int D, int Q, cons
     return hs_fail(HS_ERR_ARG, "bad precision");
 }
 
+extern "C" int hs_scatter_add(int precision, int64_t n, const int32_t *idx, int n_target, double w, void *target,
+                              double *kernel_ms)
+{
+    HS_TRY(hs_need_init());
+    if (n < 0 || n_target <= 0 || (n > 0 && (!idx || !target)))
+        return hs_fail(HS_ERR_ARG, "hs_scatter_add: bad arguments");
+    if (precision != HS_F32 && precision != HS_F64) return hs_fail(HS_ERR_ARG, "bad precision");
+    if (kernel_ms) *kernel_ms = 0.0;
+    if (n == 0) return HS_OK;
+    HsCtl c;
+    memset(&c, 0, sizeof(c));
+    HS_HIP(hipMemcpyAsync(g_rt.ctl, &c, sizeof(c), hipMemcpyHostToDevice, g_rt.stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (kernel_ms) {
+        HS_HIP(hipEventCreate(&e0));
+        HS_HIP(hipEventCreate(&e1));
+        HS_HIP(hipEventRecord(e0, g_rt.stream));
+    }
+    HS_HIP(hs_launch_scatter_add(precision, idx, n, n_target, w, target, &g_rt.ctl->scatter_bad, g_rt.stream));
+    if (kernel_ms) HS_HIP(hipEventRecord(e1, g_rt.stream));
+    unsigned long long bad = 0;
+    HS_HIP(hipMemcpyAsync(&bad, &g_rt.ctl->scatter_bad, sizeof(bad), hipMemcpyDeviceToHost, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    if (kernel_ms) {
+        float ms = 0.0f;
+        HS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *kernel_ms = ms;
+        HS_HIP(hipEventDestroy(e0));
+        HS_HIP(hipEventDestroy(e1));
+    }
+    if (bad) return hs_fail(HS_ERR_ARG, "hs_scatter_add: %llu indices outside [0, %d)", bad, n_target);
+    return HS_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* networks                                                                  */
 /* ------------------------------------------------------------------------ */

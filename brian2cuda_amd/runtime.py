@@ -152,6 +152,15 @@ def deliver(precision, n_src, dmin, D, Q, hist, hist_cnt, step, bstart, post, wm
          wmode, (w.ptr if w is not None else None), float(wscalar), target.ptr, events.ptr)
 
 
+def scatter_add(idx, w, target):
+    """hs_scatter_add: target[idx[k]] += w on device arrays (idx int32; target
+    f64 or f32).  Returns the kernel's device time in ms."""
+    prec = HS_F64 if target.dtype == np.float64 else HS_F32
+    ms = C.c_double(0.0)
+    call("hs_scatter_add", prec, int(idx.n), idx.ptr, int(target.n), float(w), target.ptr, C.byref(ms))
+    return ms.value
+
+
 # ---- networks -------------------------------------------------------------------
 
 class Net:

@@ -110,6 +110,20 @@ int hs_deliver(int precision, int n_src, int dmin, int D, int Q, const int32_t *
                int wmode, const void *w, double wscalar, void *target, uint64_t *events);
 
 /*
+ * Effect application of a flat event list: target[idx[k]] += w for k < n
+ * (device arrays; target holds n_target values of the given precision), with
+ * hardware atomics and no spike list or bundle lookup.  It measures what random
+ * atomics alone sustain; hs_deliver and the network kernels add the lookups.
+ * An index outside [0, n_target) is skipped, and the call then returns
+ * HS_ERR_ARG.  kernel_ms (nullable) receives the kernel's device time.
+ * Replaces the atomic effect statement of _run_kernel_<synapses_pre> when it
+ * runs with atomics (synapses.cu:202-215; _brian_atomicAdd,
+ * cuda_generator.py:33-150).
+ */
+int hs_scatter_add(int precision, int64_t n, const int32_t *idx, int n_target, double w, void *target,
+                   double *kernel_ms);
+
+/*
  * Device evaluation of the deterministic math (for bit-equality checks):
  * fn 0 exp, 1 log, 2 sqrt, 3 cos(2 pi x), 4 expf, 5 logf, 6 sqrtf, 7 cos(2 pi x)
  * in float (x and y read/written as double).

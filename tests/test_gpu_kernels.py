@@ -110,3 +110,34 @@ def test_stateupdate_one_step_vs_oracle(hs, oracle_lib, prec, model):
                                  params, 1e-4, 99, step)
     for v in range(nv):
         assert np.array_equal(got[v], ref[v].astype(dtype)), f"var {v}: {np.sum(got[v] != ref[v].astype(dtype))}"
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("n_events,n_target", [(0, 10), (1, 1), (1000, 7), (300_000, 100_000)])
+def test_scatter_add_vs_serial_sum(hs, prec, n_events, n_target):
+    """hs_scatter_add against numpy.add.at: every addend is the same weight, so
+    the atomic sum is exact whatever the order."""
+    dt = np.float64 if prec == 64 else np.float32
+    rng = np.random.default_rng(n_events + n_target)
+    idx = rng.integers(0, n_target, n_events).astype(np.int32)
+    t0 = rng.standard_normal(n_target).astype(dt)
+    w = dt(-1e-4)
+    d_idx = hs.DeviceArray.from_numpy(idx if n_events else np.zeros(1, np.int32))
+    d_idx.n = n_events
+    d_t = hs.DeviceArray.from_numpy(t0)
+    ms = hs.scatter_add(d_idx, float(w), d_t)
+    want = t0.copy()
+    np.add.at(want, idx, w)
+    assert np.array_equal(d_t.numpy(), want)
+    assert ms >= 0.0
+
+
+def test_scatter_add_rejects_out_of_range(hs):
+    from brian2cuda_amd._lib import HsError
+    idx = np.array([0, 5, -1, 3, 9], np.int32)
+    d_idx = hs.DeviceArray.from_numpy(idx)
+    d_t = hs.DeviceArray.from_numpy(np.zeros(5, np.float64))
+    with pytest.raises(HsError):
+        hs.scatter_add(d_idx, 1.0, d_t)
+    # the in-range entries were applied, the others skipped
+    assert np.array_equal(d_t.numpy(), np.array([1.0, 0.0, 0.0, 1.0, 0.0]))
