@@ -36,7 +36,9 @@ HS_HD long long hs_timestep(double t, double dt)
 /*
  * Per-group constants, computed once on the host in double with the
  * deterministic exp so that every caller derives the same bits.
- * Returns the number of constants written to k (<= 16), or -1 for a bad model.
+ * Returns the number of constants written to k (<= 16), or -1 for a bad model
+ * or parameters outside the closed form (LIF_CUBA: taum, taue, taui > 0 and
+ * taue, taui != taum, whose Ax divide by taux - taum).
  * p holds every parameter of the model, optional ones included (hs_model_params).
  *   LIF_CUBA : params El,Vt,Vr,taum,taue,taui,ref,free -> k = Em,Ee,Ei,Ae,Ai,El,free
  *   LIF_NOISE: params tau,mu,sigma,theta,Vr,ref,stream -> k = a,b,mu
@@ -47,6 +49,7 @@ static inline int hs_model_constants(int model, const double *p, double dt, doub
 {
     if (model == 1) { /* HS_MODEL_LIF_CUBA */
         double taum = p[3], taue = p[4], taui = p[5];
+        if (!(taum > 0.0 && taue > 0.0 && taui > 0.0) || taue == taum || taui == taum) return -1;
         double Em = hs_exp(-dt / taum), Ee = hs_exp(-dt / taue), Ei = hs_exp(-dt / taui);
         k[0] = Em;
         k[1] = Ee;

@@ -537,7 +537,9 @@ extern "C" int hs_net_add_group(hs_net *net, int model, int n, const double *par
     g.n = n;
     g.mi = mi;
     memset(g.k, 0, sizeof(g.k));
-    hs_model_constants(model, params, net->dt, g.k);
+    if (hs_model_constants(model, params, net->dt, g.k) < 0)
+        return hs_fail(HS_ERR_ARG, "hs_net_add_group: parameters outside model %d's closed form "
+                                   "(LIF_CUBA: time constants > 0, taue and taui != taum)", model);
     g.ref_steps = mi.i_ref >= 0 ? hs_ref_steps(params[mi.i_ref], net->dt) : 0;
     if (model == HS_MODEL_SPIKEGEN) g.gen_period = hs_timestep(params[0], net->dt);
     g.stream = mi.i_stream >= 0 ? (unsigned int)params[mi.i_stream] : 0u;

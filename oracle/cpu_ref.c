@@ -309,6 +309,7 @@ int ref_net_add_group(ref_net *net, int model, int n, const double *params, int 
     g->n = n;
     g->nvars = nv;
     g->nk = hs_model_constants(model, params, net->dt, g->k);
+    if (g->nk < 0) return -1;
     if (model == HS_MODEL_LIF_CUBA) {
         g->vth = params[1]; g->vreset = params[2]; g->refractory = params[6]; g->has_reset = 1;
     } else if (model == HS_MODEL_LIF_NOISE) {

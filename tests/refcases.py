@@ -161,3 +161,72 @@ def spikes_by_step(i, t):
     for step in np.unique(t):
         out[int(step)] = np.sort(i[t == step])
     return out
+
+
+def bundle_prefs(dt=1e-4):
+    """test_threads_per_synapse_bundle_prefs (test_synaptic_propagations.py:343-374):
+    a SpikeGeneratorGroup neuron fires once at step 0 into targets 3 and 4 of
+    five static neurons with delays j*dt; each arrival adds 1.1, so neuron 3
+    spikes at step 4 and neuron 4 at step 5 over a 6-step run, whatever the
+    launch shapes (the reference varies the threads per bundle)."""
+    g_in = GroupSpec(HS_MODEL_SPIKEGEN, 1, [0.0], spikes=(np.zeros(1, np.int32), np.zeros(1, np.int64)))
+    g = GroupSpec(HS_MODEL_LIF_CUBA, 5, static_params(), {0: 0.0, 1: 0.0, 2: 0.0})
+    p = PathSpec(0, 1, 0, np.zeros(2, np.int32), np.array([3, 4], np.int32),
+                 delay_steps=np.array([3, 4], np.int32), wscalar=1.1)
+    spec = NetSpec("bundle_prefs", dt, 1, [g_in, g], [p], spike_monitors=[1])
+    return spec, 6, {4: np.array([3]), 5: np.array([4])}
+
+
+def ring(N=1000, dt=1e-4, w=0.5e-3):
+    """test_cuda_standalone (test_cuda_standalone.py:16-49): dV/dt = (-40 mV - V)/tau
+    (unless refractory), tau 1 ms, threshold V > -50 mV, reset V = -60 mV,
+    refractory 5 ms, V(0) = -i mV; on_pre V += w (0.5 mV) onto the neighbours
+    |i - j| < 5, i != j, delay 0; 100 ms.  The reference expects 17,000 to 18,000
+    spikes, the first at t = 0."""
+    mV, ms = 1e-3, 1e-3
+    g = GroupSpec(HS_MODEL_LIF_CUBA, N, [-40 * mV, -50 * mV, -60 * mV, 1 * ms, 5 * ms, 10 * ms, 5 * ms],
+                  {0: -np.arange(N) * mV, 1: 0.0, 2: 0.0})
+    i = np.repeat(np.arange(N), 9)
+    j = np.repeat(np.arange(N), 9) + np.tile(np.arange(-4, 5), N)
+    ok = (j >= 0) & (j < N) & (j != i)
+    p = PathSpec(0, 0, 0, i[ok].astype(np.int32), j[ok].astype(np.int32), scalar_delay=0, wscalar=w)
+    spec = NetSpec("ring", dt, 1, [g], [p], spike_monitors=[0])
+    return spec, 1000
+
+
+def ring_numpy(N=1000, steps=1000, dt=1e-4, w=0.5e-3):
+    """The ring restated independently in numpy on Brian2's schedule (state
+    update with refractoriness timestep(t - lastspike) >= timestep(5 ms), threshold,
+    synapses with delay 0, reset), FP64.  Returns (spike count, first spike step)."""
+    mV = 1e-3
+    E = np.exp(-dt / 1e-3)
+    V = -np.arange(N) * mV
+    last = np.full(N, -10 ** 9)
+    n, first = 0, None
+    for s in range(steps):
+        nr = (s - last) >= 50
+        V = np.where(nr, -40 * mV + (V + 40 * mV) * E, V)
+        spk = np.nonzero((V > -50 * mV) & nr)[0]
+        if first is None and spk.size:
+            first = s
+        n += spk.size
+        last[spk] = s
+        add = np.zeros(N)
+        for d in (-4, -3, -2, -1, 1, 2, 3, 4):
+            j = spk + d
+            np.add.at(add, j[(j >= 0) & (j < N)], w)
+        V = V + add
+        V[spk] = -60 * mV
+    return n, first
+
+
+def example_run(N=100, dt=1e-4):
+    """brian2cuda's example_run (brian2cuda/__init__.py:40-77): dv/dt = -v/tau,
+    tau 10 ms, threshold v > 1, reset v = 0, refractory 5 ms, v(0) = i/100, run
+    1 ms; the reference asserts v == arange(N)/N * exp(-1 ms/tau)."""
+    ms = 1e-3
+    # taue, taui: unused (ge = gi = 0), but they must differ from taum
+    g = GroupSpec(HS_MODEL_LIF_CUBA, N, [0.0, 1.0, 0.0, 10 * ms, 5 * ms, 20 * ms, 5 * ms],
+                  {0: np.arange(N) / 100.0, 1: 0.0, 2: 0.0})
+    spec = NetSpec("example_run", dt, 1, [g], [], spike_monitors=[0])
+    return spec, 10, np.arange(N) / N * np.exp(-1 * ms / (10 * ms))

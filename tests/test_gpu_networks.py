@@ -159,7 +159,7 @@ def test_kernel_timing_keeps_results(hs, oracle_lib, engine, sem, delivery, pipe
         assert (ms > 0) == (k in expect)
 
 
-@pytest.mark.parametrize("case", ["push_outer_loop", "push_outer_loop2", "circular_eventspaces"])
+@pytest.mark.parametrize("case", ["push_outer_loop", "push_outer_loop2", "circular_eventspaces", "bundle_prefs"])
 @pytest.mark.parametrize("engine", ["plain", "fused"])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_reference_propagation_cases_gpu(hs, case, engine, prec):
@@ -253,3 +253,53 @@ def test_cobahh_64k(hs, oracle_lib, prec, engine, sem):
     net, ref = _compare(spec, prec, engine, sem, None, oracle_lib, chunks=[180, 120])
     if prec == 64:
         assert not np.isnan(ref.get_state(0, 0)).any()
+
+
+
+@pytest.mark.parametrize("engine,options", [("plain", None), ("plain", {1: 1}), ("plain", {1: 4096}),
+                                            ("plain", {5: 2}), ("fused", {0: 64}), ("fused", {1: 2048})])
+def test_reference_bundle_prefs_launch_shapes(hs, engine, options):
+    """test_threads_per_synapse_bundle_prefs varies the threads per bundle; here
+    the delivery grid (1 block, 4096 blocks), gather delivery and update tiles
+    vary, and neuron 3 still spikes at step 4, neuron 4 at step 5."""
+    import refcases as rc
+    spec, steps, expect = rc.bundle_prefs()
+    net, sm, _ = nw.build_hs(spec, 64, engine, options=options)
+    net.run(steps)
+    got = rc.spikes_by_step(*net.spikes(sm[0]))
+    assert sorted(got) == sorted(expect)
+    for k in expect:
+        np.testing.assert_array_equal(got[k], expect[k])
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine,sem", ENGINES)
+def test_reference_ring_gpu(hs, oracle_lib, prec, engine, sem):
+    """test_cuda_standalone's ring: bit for bit against the oracle, the first
+    spike at t = 0 and at least 17,000 spikes (18,068 in FP64)."""
+    import refcases as rc
+    spec, steps = rc.ring()
+    net, ref = _compare(spec, prec, engine, sem, steps, oracle_lib)
+    i, t = net.spikes(0)
+    assert len(i) >= 17000 and t[0] == 0
+    if prec == 64:
+        assert len(i) == 18068
+
+
+@pytest.mark.parametrize("prec,rtol", [(64, 1e-7), (32, 2e-6)])
+@pytest.mark.parametrize("engine,sem", ENGINES)
+def test_reference_example_run_gpu(hs, oracle_lib, prec, rtol, engine, sem):
+    """brian2cuda's example_run: v == arange(N)/N exp(-1 ms/tau) after 1 ms."""
+    import refcases as rc
+    spec, steps, expect = rc.example_run()
+    net, ref = _compare(spec, prec, engine, sem, steps, oracle_lib)
+    np.testing.assert_allclose(net.get_state(0, 0), expect, rtol=rtol)
+
+
+def test_cuba_singular_time_constants_refused(hs):
+    """taue or taui equal to taum: the closed form divides by their difference."""
+    from brian2cuda_amd._lib import HsError
+    net = hs.Net(64, "plain", 1e-4, 1)
+    with pytest.raises(HsError):
+        net.add_group(hs.HS_MODEL_LIF_CUBA, 10, nw.cuba_params(taum=10e-3, taui=10e-3))
+    net.close()

@@ -3,6 +3,8 @@ published known-answer vectors, closed-form solutions, textbook integrators,
 the reference's own test expectations, and libm.  (No GPU.)"""
 import math
 
+import dataclasses
+
 import numpy as np
 import pytest
 from scipy import linalg, stats
@@ -248,7 +250,7 @@ def test_float32_form_rounds_like_float32():
 # ---------------------------------------------------------------------------
 # the reference's spike-propagation expectations
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("case", ["push_outer_loop", "push_outer_loop2", "circular_eventspaces"])
+@pytest.mark.parametrize("case", ["push_outer_loop", "push_outer_loop2", "circular_eventspaces", "bundle_prefs"])
 @pytest.mark.parametrize("sem", [oracle.SEQ, oracle.ACC])
 def test_reference_propagation_cases(case, sem):
     spec, steps, expect = getattr(rc, case)()
@@ -347,3 +349,55 @@ def test_cuba_network_rates_plausible():
     assert v.shape == (5000, 100)
     # sampled at the start of a step: after resets, so never above threshold
     assert v.max() <= -0.050 and v.min() > -0.1
+
+
+@pytest.mark.parametrize("sem", [oracle.SEQ, oracle.ACC])
+def test_reference_ring_uncoupled_closed_form(sem):
+    """The ring without its synapses: after a spike a neuron is held 50 steps,
+    then needs 7 exact steps from -60 mV to pass -50 mV (-40 - 20 e^{-0.7} mV),
+    so it fires every 56 steps; every first spike comes by step 47, so each of
+    the 1000 neurons fires 18 times in 1000 steps."""
+    spec, steps = rc.ring(w=0.0)
+    ref, sm, _ = oracle.build_ref(spec, 64, sem)
+    ref.run(steps)
+    i, t = ref.spikes(sm[0])
+    assert np.array_equal(np.bincount(i, minlength=1000), np.full(1000, 18))
+    assert t[0] == 0 and t.max() <= 999
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("sem", [oracle.SEQ, oracle.ACC])
+def test_reference_ring(prec, sem):
+    """test_cuda_standalone: the first spike at t = 0 and at least 17,000 spikes
+    in 100 ms, as the reference expects; the count itself (18,068) equals an
+    independent numpy restatement of the schedule.  The reference's upper bound
+    of 18,000 is not met: the uncoupled ring already fires exactly 18,000 times
+    under timestep-based refractoriness, and excitatory coupling can only add."""
+    spec, steps = rc.ring()
+    ref, sm, _ = oracle.build_ref(spec, prec, sem)
+    ref.run(steps)
+    i, t = ref.spikes(sm[0])
+    assert len(i) >= 17000
+    assert t[0] == 0
+    if prec == 64:
+        assert (len(i), int(t[0])) == rc.ring_numpy()
+
+
+@pytest.mark.parametrize("prec,rtol", [(64, 1e-7), (32, 2e-6)])
+def test_reference_example_run(prec, rtol):
+    """example_run's closed form v = arange(N)/N exp(-1 ms/tau) after 1 ms (FP64 at
+    the reference's rtol 1e-7; FP32 at 2e-6: ten roundings of a float product)."""
+    spec, steps, expect = rc.example_run()
+    ref, sm, _ = oracle.build_ref(spec, prec, oracle.SEQ)
+    ref.run(steps)
+    np.testing.assert_allclose(ref.get_state(0, 0), expect, rtol=rtol)
+    assert len(ref.spikes(sm[0])[0]) == 0
+
+
+def test_cuba_singular_time_constants_refused():
+    """The shared closed-form constants reject taue or taui equal to taum."""
+    spec, steps, _ = rc.example_run()
+    g = spec.groups[0]
+    bad = dataclasses.replace(spec, groups=[dataclasses.replace(g, params=g.params[:5] + [g.params[3]] + g.params[6:])])
+    with pytest.raises(Exception):
+        oracle.build_ref(bad, 64, oracle.SEQ)
