@@ -420,6 +420,8 @@ struct HMon {
     void *buf = nullptr;               /* state rows */
     int64_t rows = 0;                  /* capacity in rows */
     int64_t t0 = 0;
+    int rec = -1; /* spike monitor of an index range: the monitor recording the whole group
+                     (the device records each group's spikes once; a range is a filter on read) */
 };
 
 struct hs_net {
@@ -1169,28 +1171,55 @@ extern "C" int hs_net_get_synapse_var(hs_net *net, int pid, int var, double *out
     return HS_OK;
 }
 
+/* the group's recording monitor (created on first use), then for a range other
+   than the whole group a view on it */
+static int hs_add_spike_monitor(hs_net *net, int gid, int start, int stop, int64_t capacity, int *mid)
+{
+    if (!net || gid < 0 || gid >= (int)net->g.size()) return hs_fail(HS_ERR_ARG, "bad group");
+    HGroup &g = net->g[gid];
+    if (start < 0 || stop > g.n || stop <= start) return hs_fail(HS_ERR_ARG, "bad spike monitor range");
+    if (capacity <= 0) capacity = 1 << 20;
+    if (g.smon < 0) {
+        HMon M;
+        M.kind = 0;
+        M.gid = gid;
+        M.var = 0;
+        M.start = 0;
+        M.stop = g.n;
+        M.cap = capacity;
+        HS_TRY(hs_dalloc(net, sizeof(int32_t) * capacity, (void **)&M.mi, false));
+        HS_TRY(hs_dalloc(net, sizeof(int32_t) * capacity, (void **)&M.mt, false));
+        HS_TRY(hs_dalloc(net, sizeof(unsigned long long), (void **)&M.mn, true));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+        net->m.push_back(M);
+        g.smon = (int)net->m.size() - 1;
+        net->desc_dirty = true;
+        hs_drop_graphs(net);
+        if (start == 0 && stop == g.n) {
+            if (mid) *mid = g.smon;
+            return HS_OK;
+        }
+    }
+    HMon V;
+    V.kind = 0;
+    V.gid = gid;
+    V.start = start;
+    V.stop = stop;
+    V.rec = g.smon;
+    net->m.push_back(V);
+    if (mid) *mid = (int)net->m.size() - 1;
+    return HS_OK;
+}
+
 extern "C" int hs_net_add_spike_monitor(hs_net *net, int gid, int64_t capacity, int *mid)
 {
     if (!net || gid < 0 || gid >= (int)net->g.size()) return hs_fail(HS_ERR_ARG, "bad group");
-    if (net->g[gid].smon >= 0) return hs_fail(HS_ERR_UNSUPPORTED, "one spike monitor per group");
-    if (capacity <= 0) capacity = 1 << 20;
-    HMon M;
-    M.kind = 0;
-    M.gid = gid;
-    M.var = 0;
-    M.start = 0;
-    M.stop = net->g[gid].n;
-    M.cap = capacity;
-    HS_TRY(hs_dalloc(net, sizeof(int32_t) * capacity, (void **)&M.mi, false));
-    HS_TRY(hs_dalloc(net, sizeof(int32_t) * capacity, (void **)&M.mt, false));
-    HS_TRY(hs_dalloc(net, sizeof(unsigned long long), (void **)&M.mn, true));
-    HS_HIP(hipStreamSynchronize(g_rt.stream));
-    net->m.push_back(M);
-    net->g[gid].smon = (int)net->m.size() - 1;
-    net->desc_dirty = true;
-    hs_drop_graphs(net);
-    if (mid) *mid = (int)net->m.size() - 1;
-    return HS_OK;
+    return hs_add_spike_monitor(net, gid, 0, net->g[gid].n, capacity, mid);
+}
+
+extern "C" int hs_net_add_spike_monitor_range(hs_net *net, int gid, int start, int stop, int64_t capacity, int *mid)
+{
+    return hs_add_spike_monitor(net, gid, start, stop, capacity, mid);
 }
 
 extern "C" int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start, int stop, int *mid)
@@ -1721,7 +1750,7 @@ static int hs_check_spike_monitors(hs_net *net, int64_t k, bool *resized)
 {
     *resized = false;
     for (HMon &M : net->m) {
-        if (M.kind != 0) continue;
+        if (M.kind != 0 || M.rec >= 0) continue;
         unsigned long long n = 0;
         HS_HIP(hipMemcpyAsync(&n, M.mn, sizeof(n), hipMemcpyDeviceToHost, g_rt.stream));
         HS_HIP(hipStreamSynchronize(g_rt.stream));
@@ -1816,7 +1845,7 @@ extern "C" int hs_net_run(hs_net *net, int64_t nsteps)
     net->spikes = hs_ctl_spikes(c);
     if (c.base != net->step) return hs_fail(HS_ERR_STATE, "device step %lld != host step %lld", c.base, (long long)net->step);
     for (const HMon &M : net->m) {
-        if (M.kind != 0) continue;
+        if (M.kind != 0 || M.rec >= 0) continue;
         unsigned long long n = 0;
         HS_HIP(hipMemcpyAsync(&n, M.mn, sizeof(n), hipMemcpyDeviceToHost, g_rt.stream));
         HS_HIP(hipStreamSynchronize(g_rt.stream));
@@ -1846,11 +1875,42 @@ extern "C" int hs_net_stats(hs_net *net, hs_stats *out)
     return HS_OK;
 }
 
+/* every record of a group's recording monitor, as (step << 32 | index) keys in
+   the order cpp_standalone writes them (by step, then index) */
+static int hs_spike_keys(hs_net *net, const HMon &M, std::vector<uint64_t> &key)
+{
+    unsigned long long c = 0;
+    HS_HIP(hipMemcpyAsync(&c, M.mn, sizeof(c), hipMemcpyDeviceToHost, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    const int64_t n = (int64_t)c;
+    if (n > M.cap) return hs_fail(HS_ERR_OVERFLOW, "spike monitor overflow: %lld records for capacity %lld", (long long)n, (long long)M.cap);
+    std::vector<int32_t> hi((size_t)n), ht((size_t)n);
+    if (n) {
+        HS_HIP(hipMemcpyAsync(hi.data(), M.mi, sizeof(int32_t) * n, hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipMemcpyAsync(ht.data(), M.mt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    }
+    /* records of one step arrive tile by tile; order them as cpp_standalone does */
+    key.resize((size_t)n);
+    for (int64_t r = 0; r < n; ++r) key[r] = ((uint64_t)(uint32_t)ht[r] << 32) | (uint32_t)hi[r];
+    std::sort(key.begin(), key.end());
+    return HS_OK;
+}
+
 extern "C" int hs_net_mon_count(hs_net *net, int mid, int64_t *n)
 {
     if (!net || mid < 0 || mid >= (int)net->m.size() || !n) return hs_fail(HS_ERR_ARG, "bad monitor");
     const HMon &M = net->m[mid];
-    if (M.kind == 0) {
+    if (M.kind == 0 && M.rec >= 0) {
+        std::vector<uint64_t> key;
+        HS_TRY(hs_spike_keys(net, net->m[M.rec], key));
+        int64_t c = 0;
+        for (uint64_t k : key) {
+            const int32_t i = (int32_t)(k & 0xffffffffu);
+            c += (i >= M.start && i < M.stop);
+        }
+        *n = c;
+    } else if (M.kind == 0) {
         unsigned long long c = 0;
         HS_HIP(hipMemcpyAsync(&c, M.mn, sizeof(c), hipMemcpyDeviceToHost, g_rt.stream));
         HS_HIP(hipStreamSynchronize(g_rt.stream));
@@ -1865,23 +1925,23 @@ extern "C" int hs_net_get_spikes(hs_net *net, int mid, int32_t *i, int64_t *step
 {
     if (!net || mid < 0 || mid >= (int)net->m.size() || net->m[mid].kind != 0) return hs_fail(HS_ERR_ARG, "bad monitor");
     const HMon &M = net->m[mid];
+    std::vector<uint64_t> key;
+    HS_TRY(hs_spike_keys(net, M.rec >= 0 ? net->m[M.rec] : M, key));
+    /* a range monitor keeps the records of its indices, relative to its start
+       (spikemonitor.cu:15-22 is_in_subgroup, :195 count[_idx - _source_start]) */
     int64_t n = 0;
-    HS_TRY(hs_net_mon_count(net, mid, &n));
-    if (n > M.cap) return hs_fail(HS_ERR_OVERFLOW, "spike monitor overflow: %lld records for capacity %lld", (long long)n, (long long)M.cap);
-    if (cap < n) return hs_fail(HS_ERR_ARG, "output arrays too small (%lld < %lld)", (long long)cap, (long long)n);
-    std::vector<int32_t> hi((size_t)n), ht((size_t)n);
-    if (n) {
-        HS_HIP(hipMemcpyAsync(hi.data(), M.mi, sizeof(int32_t) * n, hipMemcpyDeviceToHost, g_rt.stream));
-        HS_HIP(hipMemcpyAsync(ht.data(), M.mt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, g_rt.stream));
-        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    for (uint64_t k : key) {
+        const int32_t ii = (int32_t)(k & 0xffffffffu);
+        n += (ii >= M.start && ii < M.stop);
     }
-    /* records of one step arrive tile by tile; order them as cpp_standalone does */
-    std::vector<uint64_t> key((size_t)n);
-    for (int64_t r = 0; r < n; ++r) key[r] = ((uint64_t)(uint32_t)ht[r] << 32) | (uint32_t)hi[r];
-    std::sort(key.begin(), key.end());
-    for (int64_t r = 0; r < n; ++r) {
-        step[r] = (int64_t)(key[r] >> 32);
-        i[r] = (int32_t)(key[r] & 0xffffffffu);
+    if (cap < n) return hs_fail(HS_ERR_ARG, "output arrays too small (%lld < %lld)", (long long)cap, (long long)n);
+    int64_t r = 0;
+    for (uint64_t k : key) {
+        const int32_t ii = (int32_t)(k & 0xffffffffu);
+        if (ii < M.start || ii >= M.stop) continue;
+        step[r] = (int64_t)(k >> 32);
+        i[r] = ii - M.start;
+        ++r;
     }
     return HS_OK;
 }
@@ -1911,7 +1971,7 @@ extern "C" int hs_net_write_monitor(hs_net *net, int mid, const char *path_a, co
         std::vector<int64_t> st((size_t)(n ? n : 1));
         HS_TRY(hs_net_get_spikes(net, mid, i.data(), st.data(), n));
         std::vector<double> t((size_t)(n ? n : 1));
-        std::vector<int32_t> count((size_t)net->g[M.gid].n, 0);
+        std::vector<int32_t> count((size_t)(M.stop - M.start), 0);
         for (int64_t r = 0; r < n; ++r) {
             t[(size_t)r] = (double)st[(size_t)r] * net->dt;
             count[(size_t)i[(size_t)r]]++;

@@ -323,9 +323,16 @@ class Net:
                 d.free()
         return pid.value
 
-    def add_spike_monitor(self, gid, capacity=1 << 20):
+    def add_spike_monitor(self, gid, capacity=1 << 20, start=None, stop=None):
+        """SpikeMonitor of a group, or of its neurons [start, stop) (a Subgroup:
+        recorded indices relative to start)."""
         mid = C.c_int()
-        call("hs_net_add_spike_monitor", self.h, gid, int(capacity), C.byref(mid))
+        if start is None and stop is None:
+            call("hs_net_add_spike_monitor", self.h, gid, int(capacity), C.byref(mid))
+        else:
+            start = 0 if start is None else start
+            stop = self.groups[gid][1] if stop is None else stop
+            call("hs_net_add_spike_monitor_range", self.h, gid, int(start), int(stop), int(capacity), C.byref(mid))
         return mid.value
 
     def add_rate_monitor(self, gid, start=0, stop=None):

@@ -236,8 +236,19 @@ int hs_net_pathway_stdp(hs_net *net, int pid, const double *params, int nparams)
    (the order of hs_net_get_synapses); cap >= nsyn */
 int hs_net_get_synapse_var(hs_net *net, int pid, int var, double *out, int64_t cap);
 
-/* SpikeMonitor on a group: capacity = device records kept (i, step). */
+/* SpikeMonitor on a group: capacity = device records kept (i, step) at first;
+   the buffers grow before a run that could fill them. */
 int hs_net_add_spike_monitor(hs_net *net, int gid, int64_t capacity, int *mid);
+/*
+ * SpikeMonitor on neurons [start, stop) of a group (a Brian2 Subgroup G[start:stop]):
+ * its records are those of the range, indices relative to start.  The device
+ * records each group's spikes once (in the group's whole-group monitor, created
+ * here if absent); a range monitor is a filter applied on read.  Any number of
+ * monitors, overlapping or not, may watch one group.
+ * Replaces the subgroup filter of the spike monitor (spikemonitor.cu:15-22
+ * is_in_subgroup, :195 count[_idx - _source_start]).
+ */
+int hs_net_add_spike_monitor_range(hs_net *net, int gid, int start, int stop, int64_t capacity, int *mid);
 /*
  * PopulationRateMonitor of neurons [start, stop) of a group: the update kernel
  * counts their spikes per step; hs_net_get_rate_monitor gives

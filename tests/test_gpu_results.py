@@ -85,3 +85,67 @@ def test_rate_monitor_subgroup(hs, oracle_lib):
     g, r = net.rate_monitor(net.rate_mids[0]), ref.rate_monitor(ref.rate_mids[0])
     assert np.array_equal(g, r) and g.sum() > 0
     net.close()
+
+
+@pytest.mark.parametrize("engine", ["plain", "fused"])
+def test_spike_monitor_subgroups_all_attr(hs, engine):
+    """test_spike_monitor_subgroups_all_attr (test_monitor.py:23-52): neurons 0, 4
+    and 5 of six spike at t = 0 (here a generator group); monitors of the whole
+    group and of G[:2], G[2:4], G[4:] record relative indices."""
+    from brian2cuda_amd import runtime as rt
+    net = rt.Net(64, engine, 1e-4, 1)
+    gid = net.add_group(rt.HS_MODEL_SPIKEGEN, 6, [0.0])
+    net.set_spikegen(gid, np.array([0, 4, 5], np.int32), np.zeros(3, np.int64))
+    m1 = net.add_spike_monitor(gid, start=0, stop=2)  # before the whole-group monitor
+    m_all = net.add_spike_monitor(gid)
+    m2 = net.add_spike_monitor(gid, start=2, stop=4)
+    m3 = net.add_spike_monitor(gid, start=4, stop=6)
+    net.run(1)
+    want = {m_all: ([0, 4, 5], [1, 0, 0, 0, 1, 1]), m1: ([0], [1, 0]), m2: ([], [0, 0]), m3: ([0, 1], [1, 1])}
+    for mid, (i_want, count_want) in want.items():
+        i, t = net.spikes(mid)
+        np.testing.assert_array_equal(i, i_want)
+        np.testing.assert_array_equal(t, np.zeros(len(i_want)))
+        np.testing.assert_array_equal(np.bincount(i, minlength=len(count_want)), count_want)
+        assert net.mon_count(mid) == len(i_want)
+    net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_range_monitors_vs_oracle(hs, oracle_lib, tmp_path, prec):
+    """Overlapping range monitors on a live network (created before and after
+    the whole-group one) hold the oracle's records of their ranges, and their
+    result files count per range index."""
+    spec = nw.cuba(N=800, seed=13)
+    spec = nw.NetSpec(spec.name, spec.dt, spec.seed, spec.groups, spec.paths, spike_monitors=[0])
+    net, sm, _ = nw.build_hs(spec, prec, "plain")
+    ref, rsm, _ = oracle_lib.build_ref(spec, prec, oracle_lib.SEQ)
+    ranges = [(0, 100), (50, 300), (799, 800), (0, 800)]
+    mids = [net.add_spike_monitor(0, start=a, stop=b) for a, b in ranges]
+    net.run(600)
+    ref.run(600)
+    ri, rt_ = ref.spikes(rsm[0])
+    assert len(ri) > 0
+    for (a, b), mid in zip(ranges, mids):
+        keep = (ri >= a) & (ri < b)
+        i, t = net.spikes(mid)
+        np.testing.assert_array_equal(i, ri[keep] - a)
+        np.testing.assert_array_equal(t, rt_[keep])
+    p = {k: tmp_path / k for k in ("i", "t", "count", "N")}
+    net.write_monitor(mids[1], (p["i"], p["t"], p["count"], p["N"]))
+    s = results.read_spike_monitor(p["i"], p["t"], p["count"], p["N"])
+    keep = (ri >= 50) & (ri < 300)
+    assert s["N"] == keep.sum()
+    np.testing.assert_array_equal(s["count"], np.bincount(ri[keep] - 50, minlength=250))
+    net.close()
+
+
+def test_range_monitor_bad_range(hs):
+    from brian2cuda_amd import runtime as rt
+    from brian2cuda_amd._lib import HsError
+    net = rt.Net(64, "plain", 1e-4, 1)
+    gid = net.add_group(rt.HS_MODEL_SPIKEGEN, 6, [0.0])
+    for a, b in [(-1, 3), (3, 3), (4, 2), (0, 7)]:
+        with pytest.raises(HsError):
+            net.add_spike_monitor(gid, start=a, stop=b)
+    net.close()
