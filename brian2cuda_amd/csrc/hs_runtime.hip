@@ -343,23 +343,28 @@ extern "C" int hs_scatter_add(int precision, int64_t n, const int32_t *idx, int 
     HsCtl c;
     memset(&c, 0, sizeof(c));
     HS_HIP(hipMemcpyAsync(g_rt.ctl, &c, sizeof(c), hipMemcpyHostToDevice, g_rt.stream));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    struct Events { /* destroyed on every return path */
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
     if (kernel_ms) {
-        HS_HIP(hipEventCreate(&e0));
-        HS_HIP(hipEventCreate(&e1));
-        HS_HIP(hipEventRecord(e0, g_rt.stream));
+        HS_HIP(hipEventCreate(&ev.e[0]));
+        HS_HIP(hipEventCreate(&ev.e[1]));
+        HS_HIP(hipEventRecord(ev.e[0], g_rt.stream));
     }
     HS_HIP(hs_launch_scatter_add(precision, idx, n, n_target, w, target, &g_rt.ctl->scatter_bad, g_rt.stream));
-    if (kernel_ms) HS_HIP(hipEventRecord(e1, g_rt.stream));
+    if (kernel_ms) HS_HIP(hipEventRecord(ev.e[1], g_rt.stream));
     unsigned long long bad = 0;
     HS_HIP(hipMemcpyAsync(&bad, &g_rt.ctl->scatter_bad, sizeof(bad), hipMemcpyDeviceToHost, g_rt.stream));
     HS_HIP(hipStreamSynchronize(g_rt.stream));
     if (kernel_ms) {
         float ms = 0.0f;
-        HS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        HS_HIP(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
         *kernel_ms = ms;
-        HS_HIP(hipEventDestroy(e0));
-        HS_HIP(hipEventDestroy(e1));
     }
     if (bad) return hs_fail(HS_ERR_ARG, "hs_scatter_add: %llu indices outside [0, %d)", bad, n_target);
     return HS_OK;
