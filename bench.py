@@ -47,7 +47,7 @@ network's own sensitivity produces from a one-step reordering of additions), and
 for FP64 plain with neuron 0's initial V moved by one FP32 ulp, by one
 synaptic event (|J|) or by ten, against FP64 plain; the FP32 and the perturbed comparisons
 are also given over the first 500,
-1000, 2000 and 5000 steps.
+1000, 2000 and 5000 steps, with the RMS of V between the two forms after each.
 
 Multicompartment: `--hines-trees` random branched HH trees (100-400
 compartments) advanced `--hines-steps` steps, FP64 and FP32 channel arithmetic.
@@ -155,14 +155,24 @@ def kernel_times(spec, prec, engine, steps, warmup, options=None):
     return out, window, spikes / steps, events / steps
 
 
-def spike_trains(spec, prec, engine, steps):
+def spike_trains(spec, prec, engine, steps, v_at=None):
+    """(i, step) of every spike over `steps` steps; with v_at, also a dict
+    {h: V of every neuron after h steps} for each h in v_at (h < steps)."""
     from brian2cuda_amd import networks as nw
     sp = dataclasses.replace(spec, spike_monitors=[0])
     net, smon, _ = nw.build_hs(sp, prec, engine, spike_capacity=1 << 24)
-    net.run(steps)
+    vs, done = {}, 0
+    for h in sorted(v_at or ()):
+        if 0 < h < steps:
+            net.run(h - done)
+            done = h
+            vs[h] = net.get_state(0, 0)
+    net.run(steps - done)
+    if v_at is not None:
+        vs[steps] = net.get_state(0, 0)
     i, t = net.spikes(smon[0])
     net.close()
-    return i, t
+    return (i, t, vs) if v_at is not None else (i, t)
 
 
 def train_stats(i, t, n, steps, dt):
@@ -185,10 +195,13 @@ def drift(spec, steps, tol_steps=50):
     one synaptic event or ten vs FP64 plain: the last two separate the divergence that the network's own
     sensitivity produces from any change of arithmetic.  by_horizon: the first
     and last comparison over the first h steps."""
-    i64, t64 = spike_trains(spec, 64, "plain", steps)
-    i32, t32 = spike_trains(spec, 32, "fused", steps)
+    horizons = (500, 1000, 2000, 5000)
+    i64, t64, v64 = spike_trains(spec, 64, "plain", steps, v_at=horizons)
+    i32, t32, v32 = spike_trains(spec, 32, "fused", steps, v_at=horizons)
     res = match_spikes(i64, t64, i32, t32, spec.n_neurons, tol_steps)
     res["steps"] = steps
+    # RMS of V (FP32 batched - FP64 plain) over all neurons after h steps, in mV
+    res["v_rms_mV"] = {str(h): float(np.sqrt(np.mean((v32[h] - v64[h]) ** 2))) * 1e3 for h in sorted(v64)}
     res["fp64_plain"] = train_stats(i64, t64, spec.n_neurons, steps, spec.dt)
     res["fp32_batched"] = train_stats(i32, t32, spec.n_neurons, steps, spec.dt)
     f64, g64 = spike_trains(spec, 64, "fused", steps)
@@ -212,7 +225,7 @@ def drift(spec, steps, tol_steps=50):
         res[f"fp64_plain_{name}_vs_fp64_plain"] = r
     # the comparisons over the first h steps only
     res["by_horizon"] = {}
-    for h in (500, 1000, 2000, 5000):
+    for h in horizons:
         if h >= steps:
             continue
         a, b = t64 < h, t32 < h
