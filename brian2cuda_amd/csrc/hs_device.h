@@ -72,6 +72,7 @@ template <> struct HsNV<HS_MODEL_LIF_CUBA> { static constexpr int value = HS_LIF
 template <> struct HsNV<HS_MODEL_LIF_NOISE> { static constexpr int value = HS_LIF_NOISE_NVARS; };
 template <> struct HsNV<HS_MODEL_HH_COBA> { static constexpr int value = HS_HH_NVARS; };
 template <> struct HsNV<HS_MODEL_SPIKEGEN> { static constexpr int value = HS_SPIKEGEN_NVARS; };
+template <> struct HsNV<HS_MODEL_POISSON> { static constexpr int value = HS_POISSON_NVARS; };
 
 /* SpikeGeneratorGroup: does neuron i fire at step t (binary search in the step's ascending list) */
 template <typename R>

@@ -141,6 +141,11 @@ __device__ __forceinline__ bool hs_step_one(const DGroup<R> &g, R *s, const R *k
         const bool fire = hs_spikegen_fires(g, t, i);
         if (fire) last = (int32_t)t;
         return fire;
+    } else if constexpr (MODEL == HS_MODEL_POISSON) { /* PoissonGroup: rand() < rates*dt */
+        const R p = s[0] * k[0];
+        const bool fire = hs_rand(g.seed, g.stream, (uint32_t)i, t) < (double)p;
+        if (fire) last = (int32_t)t;
+        return fire;
     } else {
         if (g.has_reset && (long long)last == t - 1 && t - 1 >= no_reset_before) s[0] = g.vr;
         for (int m = 0; m < g.nst; ++m) {
@@ -337,6 +342,9 @@ __device__ __forceinline__ void hs_update_phase(const DGroup<R> *groups, int ng,
         } else if (g.model == HS_MODEL_SPIKEGEN) {
             hs_update_tile<R, HS_MODEL_SPIKEGEN, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
                                                               push);
+        } else if (g.model == HS_MODEL_POISSON) {
+            hs_update_tile<R, HS_MODEL_POISSON, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
+                                                             push);
         } else {
             hs_update_tile<R, HS_MODEL_HH_COBA, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
                                                              push);

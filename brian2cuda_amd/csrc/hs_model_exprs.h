@@ -42,6 +42,7 @@ HS_HD long long hs_timestep(double t, double dt)
  * p holds every parameter of the model, optional ones included (hs_model_params).
  *   LIF_CUBA : params El,Vt,Vr,taum,taue,taui,ref,free -> k = Em,Ee,Ei,Ae,Ai,El,free
  *   LIF_NOISE: params tau,mu,sigma,theta,Vr,ref,stream -> k = a,b,mu
+ *   POISSON  : params rate [,stream] -> k = dt
  *   HH_COBA  : params Cm,gl,El,EK,ENa,gNa,gK,VT,Ee,Ei,taue,taui,vth,ref
  *              -> k = Cm,gl,El,EK,ENa,gNa,gK,VT,Ee,Ei,dec_e,dec_i,dt
  */
@@ -76,6 +77,11 @@ static inline int hs_model_constants(int model, const double *p, double dt, doub
     }
     if (model == 4) /* HS_MODEL_SPIKEGEN: no state update */
         return 0;
+    if (model == 5) { /* HS_MODEL_POISSON: the firing probability is rates * k[0] */
+        if (!(p[0] >= 0.0)) return -1;
+        k[0] = dt;
+        return 1;
+    }
     return -1;
 }
 
@@ -91,6 +97,7 @@ static inline int hs_model_params(int model, const double *p, int np, double *ou
     else if (model == 2) { lo = hi = 7; }     /* LIF_NOISE */
     else if (model == 3) { lo = hi = 14; }    /* HH_COBA */
     else if (model == 4) { lo = hi = 1; }     /* SPIKEGEN */
+    else if (model == 5) { lo = 1; hi = 2; }  /* POISSON [, stream] */
     else return -1;
     if (np < lo || np > hi || (np > 0 && !p)) return -1;
     for (int i = 0; i < 16; ++i) out[i] = i < np ? p[i] : 0.0;

@@ -203,6 +203,7 @@ static bool hs_model_info(int model, HsModelInfo *mi)
     if (model == HS_MODEL_LIF_NOISE) { *mi = {HS_LIF_NOISE_NVARS, HS_LIF_NOISE_NPARAMS, 1, 3, 4, 5, 6}; return true; }
     if (model == HS_MODEL_HH_COBA) { *mi = {HS_HH_NVARS, HS_HH_NPARAMS, 0, 12, -1, 13, -1}; return true; }
     if (model == HS_MODEL_SPIKEGEN) { *mi = {HS_SPIKEGEN_NVARS, HS_SPIKEGEN_NPARAMS, 0, -1, -1, -1, -1}; return true; }
+    if (model == HS_MODEL_POISSON) { *mi = {HS_POISSON_NVARS, HS_POISSON_NPARAMS, 0, -1, -1, -1, 1}; return true; }
     return false;
 }
 
@@ -248,7 +249,7 @@ extern "C" int hs_stateupdate(int model, int precision, int n, void *const *vars
 {
     HS_TRY(hs_need_init());
     HsModelInfo mi;
-    if (!hs_model_info(model, &mi) || model == HS_MODEL_SPIKEGEN)
+    if (!hs_model_info(model, &mi) || model == HS_MODEL_SPIKEGEN || model == HS_MODEL_POISSON)
         return hs_fail(HS_ERR_UNSUPPORTED, "no state update for model %d", model);
     double pp[16];
     if (hs_model_params(model, params, nparams, pp) < 0 || !vars || !last || n < 0 ||
@@ -551,6 +552,15 @@ extern "C" int hs_net_add_group(hs_net *net, int model, int n, const double *par
     if (model == HS_MODEL_SPIKEGEN) g.gen_period = hs_timestep(params[0], net->dt);
     g.stream = mi.i_stream >= 0 ? (unsigned int)params[mi.i_stream] : 0u;
     for (int v = 0; v < mi.nvars; ++v) HS_TRY(hs_dalloc(net, hs_rsize(net) * n, &g.var[v], true));
+    if (model == HS_MODEL_POISSON) { /* every neuron starts at the rate parameter */
+        std::vector<double> r8((size_t)n, params[0]);
+        std::vector<float> r4((size_t)n, (float)params[0]);
+        if (net->prec == HS_F64)
+            HS_HIP(hipMemcpyAsync(g.var[0], r8.data(), sizeof(double) * n, hipMemcpyHostToDevice, g_rt.stream));
+        else
+            HS_HIP(hipMemcpyAsync(g.var[0], r4.data(), sizeof(float) * n, hipMemcpyHostToDevice, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    }
     HS_TRY(hs_dalloc(net, sizeof(int32_t) * n, (void **)&g.last, false));
     std::vector<int32_t> never((size_t)n, HS_NEVER_SPIKED);
     HS_HIP(hipMemcpyAsync(g.last, never.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, g_rt.stream));

@@ -16,14 +16,17 @@ HS_F64, HS_F32 = 64, 32
 HS_ENGINE_PLAIN, HS_ENGINE_FUSED = 0, 1
 HS_W_SCALAR, HS_W_SYNAPSE, HS_W_PRE = 0, 1, 2
 
-HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_HH_COBA, HS_MODEL_SPIKEGEN = 1, 2, 3, 4
-MODEL_NVARS = {HS_MODEL_LIF_CUBA: 3, HS_MODEL_LIF_NOISE: 1, HS_MODEL_HH_COBA: 6, HS_MODEL_SPIKEGEN: 1}
-MODEL_NPARAMS = {HS_MODEL_LIF_CUBA: 7, HS_MODEL_LIF_NOISE: 7, HS_MODEL_HH_COBA: 14, HS_MODEL_SPIKEGEN: 1}
+HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_HH_COBA, HS_MODEL_SPIKEGEN, HS_MODEL_POISSON = 1, 2, 3, 4, 5
+MODEL_NVARS = {HS_MODEL_LIF_CUBA: 3, HS_MODEL_LIF_NOISE: 1, HS_MODEL_HH_COBA: 6, HS_MODEL_SPIKEGEN: 1,
+               HS_MODEL_POISSON: 1}
+MODEL_NPARAMS = {HS_MODEL_LIF_CUBA: 7, HS_MODEL_LIF_NOISE: 7, HS_MODEL_HH_COBA: 14, HS_MODEL_SPIKEGEN: 1,
+                 HS_MODEL_POISSON: 1}
 MODEL_VARS = {
     HS_MODEL_LIF_CUBA: ("v", "ge", "gi"),
     HS_MODEL_LIF_NOISE: ("V",),
     HS_MODEL_HH_COBA: ("v", "m", "n", "h", "ge", "gi"),
     HS_MODEL_SPIKEGEN: ("_unused",),
+    HS_MODEL_POISSON: ("rates",),
 }
 
 

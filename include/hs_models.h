@@ -81,6 +81,22 @@
 #define HS_SPIKEGEN_NVARS 1
 #define HS_SPIKEGEN_NPARAMS 1
 
+/*
+ * HS_MODEL_POISSON -- PoissonGroup(N, rates) (Brian2's PoissonGroup: threshold
+ * 'rand() < rates*dt', used as the input of examples/stdp.py:162 and
+ * tests/features/speed.py:519,649): neuron i spikes at step t when
+ * u(seed, stream, i, t) < rates_i * dt, u the 53-bit Philox uniform in [0, 1)
+ * (hs_rand) and rates_i * dt computed in the group's precision.  No reset, no
+ * refractoriness.
+ * vars: rates (Hz; every neuron starts at the rate parameter)
+ * params: rate (Hz) [, stream (Philox stream, default 0)]
+ */
+#define HS_MODEL_POISSON 5
+#define HS_POISSON_NVARS 1
+#define HS_POISSON_NPARAMS 1
+#define HS_POISSON_NPARAMS_MAX 2
+#define HS_POISSON_RATES 0
+
 #define HS_MAX_VARS 6
 
 /*

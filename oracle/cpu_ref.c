@@ -293,6 +293,7 @@ static int model_nvars(int model)
     if (model == HS_MODEL_LIF_NOISE) return HS_LIF_NOISE_NVARS;
     if (model == HS_MODEL_HH_COBA) return HS_HH_NVARS;
     if (model == HS_MODEL_SPIKEGEN) return HS_SPIKEGEN_NVARS;
+    if (model == HS_MODEL_POISSON) return HS_POISSON_NVARS;
     return -1;
 }
 
@@ -318,11 +319,19 @@ int ref_net_add_group(ref_net *net, int model, int n, const double *params, int 
     } else if (model == HS_MODEL_SPIKEGEN) {
         g->refractory = 0.0; g->has_reset = 0;
         g->gen_period = hs_timestep(params[0], net->dt);
+    } else if (model == HS_MODEL_POISSON) {
+        g->refractory = 0.0; g->has_reset = 0;
+        g->stream = (uint32_t)params[1];
     } else {
         g->vth = params[12]; g->refractory = params[13]; g->has_reset = 0;
     }
     g->ref_steps = hs_timestep(g->refractory, net->dt);
     for (int v = 0; v < nv; ++v) g->var[v] = calloc((size_t)n, rsz(net));
+    if (model == HS_MODEL_POISSON) /* rates start at the rate parameter */
+        for (int i = 0; i < n; ++i) {
+            if (net->prec == HS_F64) ((double *)g->var[0])[i] = params[0];
+            else ((float *)g->var[0])[i] = (float)params[0];
+        }
     g->lastspike = (double *)malloc(sizeof(double) * n);
     for (int i = 0; i < n; ++i) g->lastspike[i] = -1e4; /* never spiked */
     g->spk = (int32_t *)malloc(sizeof(int32_t) * n);
