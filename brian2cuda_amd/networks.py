@@ -16,7 +16,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from .runtime import (HS_MODEL_HH_COBA, HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_SPIKEGEN, HS_W_PRE,
+from .runtime import (HS_MODEL_HH_COBA, HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_POISSON, HS_MODEL_SPIKEGEN, HS_W_PRE,
                       HS_W_SCALAR, HS_W_SYNAPSE, spikegen_steps)
 
 mV, ms, nS = 1e-3, 1e-3, 1e-9
@@ -265,6 +265,38 @@ def batch_cuba(n_nets=256, seed=1234, nmin=500, nmax=16000, p=0.02, dt=0.1 * ms,
 
 
 # ---- plasticity: the reference's STDP consistency networks ---------------------------
+
+def stdp_poisson_example(N=10_000, K=1000, seed=1234, dt=0.1 * ms, delay=0.0, n_lif=None):
+    """The network of examples/stdp.py:143-182 (post_effects, delays 'none' or a
+    homogeneous delay in seconds):
+
+      input  : PoissonGroup(N, rates=F), F = 15 Hz
+      neurons: N/K LIF, dv/dt = (ge*(Ee-vr) + El - v)/taum, dge/dt = -ge/taue,
+               threshold v > vt, reset v = vr, method 'exact'.  The equation is
+               linear ((Ee - vr) is a constant): it is the CUBA model with
+               ge_CUBA = ge*(Ee - vr), so the pathway scales w by Ee - vr
+      S      : input -> neurons, connect(p = K/N), w = rand()*gmax, STDP
+               (taupre = taupost = 20 ms, dApre = 0.01 gmax,
+               dApost = -1.05 dApre, w clipped to [0, gmax])
+    Brian2's default v = 0 starts every neuron above threshold."""
+    taum, taupre, taupost = 10 * ms, 20 * ms, 20 * ms
+    Ee, vt, vr, El, taue = 0 * mV, -54 * mV, -60 * mV, -74 * mV, 5 * ms
+    F, gmax = 15.0, 0.01
+    dApre = 0.01
+    dApost = -dApre * taupre / taupost * 1.05
+    dApost *= gmax
+    dApre *= gmax
+    n_lif = N // K if n_lif is None else n_lif
+    inp = GroupSpec(HS_MODEL_POISSON, N, [F, 1.0])
+    # taui is unused (gi = 0); it must only differ from taum
+    lif = GroupSpec(HS_MODEL_LIF_CUBA, n_lif, [El, vt, vr, taum, taue, taue, 0.0], {0: 0.0, 1: 0.0, 2: 0.0})
+    rc = RandomConnect(p=K / N, pre=(0, N), post=(0, n_lif), seed=seed, stream=2, delay=(delay, delay),
+                       wmode=HS_W_SYNAPSE, w=(0.0, gmax))
+    S = PathSpec(0, 1, 1, None, None, wmode=HS_W_SYNAPSE, random=rc,
+                 stdp=(taupre, taupost, dApre, dApost, 0.0, gmax, Ee - vr))
+    return NetSpec("stdp_poisson", dt, seed, [inp, lif], [S], spike_monitors=[1],
+                   info={"gmax": gmax, "N": N, "K": K})
+
 
 def stdp_example(heterog_delays=False, n_cells=100, seed=42, dt=0.1 * ms):
     """The networks of brian2cuda/tests/test_cpp_cuda_consistency.py:10-89

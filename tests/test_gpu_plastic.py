@@ -185,3 +185,52 @@ def test_spikegen_period_validation_gpu(hs):
     with pytest.raises(HsError):
         net.set_spikegen(g, [0], [10])
     net.close()
+
+
+@pytest.fixture(scope="module")
+def stdp_poisson():
+    return nw.stdp_poisson_example()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_stdp_poisson_example_vs_oracle(hs, oracle_lib, stdp_poisson, prec):
+    """examples/stdp.py at its size (10,000 Poisson inputs, 10 neurons, p = 0.1,
+    STDP): spikes, weights and events bit for bit over 2000 steps."""
+    spec = stdp_poisson
+    net, sm, _ = nw.build_hs(spec, prec, "plain")
+    ref, rsm, _ = oracle_lib.build_ref(spec, prec, oracle_lib.SEQ)
+    net.run(2000)
+    ref.run(2000)
+    gi, gt = net.spikes(sm[0])
+    ri, rt_ = ref.spikes(rsm[0])
+    assert len(ri) > 50
+    assert np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    nsyn = net.pathway_info(0)["nsyn"]
+    for var in range(4):
+        assert np.array_equal(net.synapse_var(0, var), ref.synapse_var(0, var, nsyn)), var
+    assert net.stats()["events"] == ref.events
+    net.close()
+
+
+def test_stdp_poisson_example_weights_split(hs, stdp_poisson):
+    """What examples/stdp.py shows (Song, Miller & Abbott 2000): under STDP with
+    depression slightly stronger than potentiation (dApost = -1.05 dApre) the
+    uniform initial weights move toward both bounds and the output rate falls.
+    Over 20 s: the share of weights within 10% of 0 or gmax grows from 0.20 to
+    over 0.27, both tails grow, and the rate of the last 10 s is below the
+    first's."""
+    spec = stdp_poisson
+    gmax = spec.info["gmax"]
+    net, sm, _ = nw.build_hs(spec, 64, "plain")
+    w0 = net.synapse_var(0, 0)
+    assert abs(np.mean((w0 < 0.1 * gmax) | (w0 > 0.9 * gmax)) - 0.2) < 0.03
+    net.run(100_000)
+    n1 = len(net.spikes(sm[0])[0])
+    net.run(100_000)
+    n2 = len(net.spikes(sm[0])[0]) - n1
+    w = net.synapse_var(0, 0)
+    assert np.all((w >= 0) & (w <= gmax))
+    assert np.mean((w < 0.1 * gmax) | (w > 0.9 * gmax)) > 0.27
+    assert np.mean(w < 0.1 * gmax) > 0.15 and np.mean(w > 0.9 * gmax) > 0.1
+    assert n2 < n1
+    net.close()
