@@ -76,6 +76,7 @@ SIGNATURES = {
     "hs_net_add_pathway": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int64, I32P, I32P, I32P, C.c_int, C.c_int,
                                      DP, C.c_double, C.POINTER(C.c_int)]),
     "hs_net_add_spike_monitor": (C.c_int, [P, C.c_int, C.c_int64, C.POINTER(C.c_int)]),
+    "hs_net_add_synapse_monitor": (C.c_int, [P, C.c_int, C.c_int, P, C.c_int, C.POINTER(C.c_int)]),
     "hs_net_add_spike_monitor_range": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_int)]),
     "hs_net_add_state_monitor": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "hs_net_set_option": (C.c_int, [P, C.c_int, C.c_longlong]),

@@ -216,6 +216,8 @@ hipError_t hs_launch_stateupdate(int model, int prec, int n, void *const *vars, 
                                  const double *k, int nk, unsigned long long seed, unsigned int stream,
                                  int ref_steps, long long step, hipStream_t s);
 hipError_t hs_launch_math(int fn, int n, const double *x, double *y, hipStream_t s);
+hipError_t hs_launch_synmon(int prec, const void *src, const int32_t *idx, int n, void *buf, long long rows,
+                            long long t0, const HsCtl *ctl, int koff, hipStream_t s);
 hipError_t hs_launch_scatter_add(int prec, const int32_t *idx, long long n, int n_target, double w, void *target,
                                  unsigned long long *bad, hipStream_t s);
 

@@ -145,3 +145,31 @@ hipError_t hs_launch_stdp(const DPath<R> *paths, int np, HsCtl *ctl, int koff, i
 
 template hipError_t hs_launch_stdp<double>(const DPath<double> *, int, HsCtl *, int, int, hipStream_t);
 template hipError_t hs_launch_stdp<float>(const DPath<float> *, int, HsCtl *, int, int, hipStream_t);
+
+/* StateMonitor of a synapse variable (Brian2's default when='start'): at the
+   start of step T, row T - t0 of buf takes src[idx[k]] for the n recorded
+   synapses (positions in bundle order) */
+template <typename R>
+__global__ __launch_bounds__(256) void hs_k_synmon(const R *__restrict__ src, const int32_t *__restrict__ idx, int n,
+                                                    R *__restrict__ buf, long long rows, long long t0, const HsCtl *ctl,
+                                                    int koff)
+{
+    const long long row = ctl->base + koff - t0;
+    if (row < 0 || row >= rows) return;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        buf[row * n + k] = src[idx[k]];
+}
+
+hipError_t hs_launch_synmon(int prec, const void *src, const int32_t *idx, int n, void *buf, long long rows,
+                            long long t0, const HsCtl *ctl, int koff, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int grid = (n + 255) / 256 > 1024 ? 1024 : (n + 255) / 256;
+    if (prec == 64)
+        hipLaunchKernelGGL(hs_k_synmon<double>, dim3(grid), dim3(256), 0, s, (const double *)src, idx, n,
+                           (double *)buf, rows, t0, ctl, koff);
+    else
+        hipLaunchKernelGGL(hs_k_synmon<float>, dim3(grid), dim3(256), 0, s, (const float *)src, idx, n,
+                           (float *)buf, rows, t0, ctl, koff);
+    return hipGetLastError();
+}

@@ -418,7 +418,8 @@ struct HPath {
 };
 
 struct HMon {
-    int kind; /* 0 spikes, 1 state, 2 population rate (buf: int32 spike count per row) */
+    int kind; /* 0 spikes, 1 state, 2 population rate (buf: int32 spike count per row),
+                 3 synapse variable (buf: rows of the selected synapses' values) */
     int gid, var, start, stop;
     int64_t cap = 0;                   /* spike records */
     int32_t *mi = nullptr, *mt = nullptr;
@@ -428,6 +429,9 @@ struct HMon {
     int64_t t0 = 0;
     int rec = -1; /* spike monitor of an index range: the monitor recording the whole group
                      (the device records each group's spikes once; a range is a filter on read) */
+    int pid = -1;                      /* kind 3 (synapse variable): pathway, selected synapses */
+    int32_t *sidx = nullptr;
+    std::vector<int32_t> hsel;
 };
 
 struct hs_net {
@@ -1237,6 +1241,38 @@ extern "C" int hs_net_add_spike_monitor_range(hs_net *net, int gid, int start, i
     return hs_add_spike_monitor(net, gid, start, stop, capacity, mid);
 }
 
+extern "C" int hs_net_add_synapse_monitor(hs_net *net, int pid, int var, const int32_t *indices, int n, int *mid)
+{
+    if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");
+    const HPath &P = net->p[pid];
+    if (net->engine != HS_ENGINE_PLAIN)
+        return hs_fail(HS_ERR_UNSUPPORTED, "synapse StateMonitors run in the plain form (as STDP does)");
+    if (var < HS_SYN_W || var > HS_SYN_APOST) return hs_fail(HS_ERR_ARG, "bad synapse variable %d", var);
+    if (var == HS_SYN_W && P.wmode != HS_W_SYNAPSE)
+        return hs_fail(HS_ERR_ARG, "w is recorded on pathways with per-synapse weights");
+    if (var != HS_SYN_W && P.dmode != HS_DELIVER_PLASTIC)
+        return hs_fail(HS_ERR_ARG, "synapse variable %d exists on STDP pathways only", var);
+    if (P.dmode == HS_DELIVER_GATHER) return hs_fail(HS_ERR_UNSUPPORTED, "gather-mode pathways");
+    if (n <= 0 || !indices) return hs_fail(HS_ERR_ARG, "no synapses to record");
+    HMon M;
+    M.kind = 3;
+    M.gid = P.tgt;
+    M.var = var;
+    M.pid = pid;
+    M.hsel.assign(indices, indices + n);
+    for (int32_t x : M.hsel)
+        if (x < 0 || (int64_t)x >= P.nsyn) return hs_fail(HS_ERR_ARG, "synapse %d outside [0, %lld)", x, (long long)P.nsyn);
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)n, (void **)&M.sidx, false));
+    HS_HIP(hipMemcpyAsync(M.sidx, M.hsel.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    M.t0 = net->step;
+    net->m.push_back(M);
+    net->desc_dirty = true;
+    hs_drop_graphs(net);
+    if (mid) *mid = (int)net->m.size() - 1;
+    return HS_OK;
+}
+
 extern "C" int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start, int stop, int *mid)
 {
     if (!net || gid < 0 || gid >= (int)net->g.size()) return hs_fail(HS_ERR_ARG, "bad group");
@@ -1559,7 +1595,9 @@ static int hs_grow_state_monitors(hs_net *net, int64_t until)
         if (need <= M.rows) continue;
         int64_t rows = std::max<int64_t>(need, M.rows * 2);
         /* state rows: one value per recorded neuron; rate rows: one int32 count, zeroed */
-        const size_t rowb = M.kind == 1 ? hs_rsize(net) * (size_t)(M.stop - M.start) : sizeof(int32_t);
+        const size_t rowb = M.kind == 1   ? hs_rsize(net) * (size_t)(M.stop - M.start)
+                            : M.kind == 3 ? hs_rsize(net) * M.hsel.size()
+                                          : sizeof(int32_t);
         void *nb = nullptr;
         HS_TRY(hs_dalloc(net, rowb * (size_t)rows, &nb, M.kind == 2));
         if (M.buf && M.rows)
@@ -1723,6 +1761,13 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
                     return HS_OK;
                 }));
             } else {
+                for (const HMon &M : net->m) /* synapse StateMonitors, when='start' */
+                    if (M.kind == 3) {
+                        const HPath &P = net->p[M.pid];
+                        const void *src = M.var == HS_SYN_W ? P.w : (M.var == HS_SYN_APRE ? P.apre : P.apost);
+                        HS_HIP(hs_launch_synmon(net->prec, src, M.sidx, (int)M.hsel.size(), M.buf, M.rows, M.t0,
+                                                net->ctl, (int)j, g_rt.stream));
+                    }
                 HS_TRY(update((int)j, 1, 1, fused));
                 if (np > 0) HS_TRY(deliver((int)j, 1));
                 if (net->n_plastic)
@@ -2010,8 +2055,9 @@ extern "C" int hs_net_write_monitor(hs_net *net, int mid, const char *path_a, co
         return HS_OK;
     }
     /* StateMonitor: t (double), values (the variable's type, time-major rows of
-       the recorded indices, objects.cu:589-595), indices (int32), N (rows) */
-    const int64_t w = M.stop - M.start;
+       the recorded indices, objects.cu:589-595), indices (int32), N (rows); a
+       synapse StateMonitor records its synapses' positions as indices */
+    const int64_t w = M.kind == 3 ? (int64_t)M.hsel.size() : M.stop - M.start;
     std::vector<double> t((size_t)(n ? n : 1));
     for (int64_t r = 0; r < n; ++r) t[(size_t)r] = (double)(M.t0 + r) * net->dt;
     std::vector<unsigned char> vals(hs_rsize(net) * (size_t)(n * w ? n * w : 1));
@@ -2020,7 +2066,7 @@ extern "C" int hs_net_write_monitor(hs_net *net, int mid, const char *path_a, co
         HS_HIP(hipStreamSynchronize(g_rt.stream));
     }
     std::vector<int32_t> idx((size_t)(w ? w : 1));
-    for (int64_t k = 0; k < w; ++k) idx[(size_t)k] = (int32_t)(M.start + k);
+    for (int64_t k = 0; k < w; ++k) idx[(size_t)k] = M.kind == 3 ? M.hsel[(size_t)k] : (int32_t)(M.start + k);
     const int32_t N = (int32_t)n;
     HS_TRY(hs_write_raw(path_a, t.data(), sizeof(double) * (size_t)n));
     HS_TRY(hs_write_raw(path_b, vals.data(), hs_rsize(net) * (size_t)(n * w)));
@@ -2042,10 +2088,11 @@ extern "C" int hs_net_write_last_run_info(hs_net *net, const char *path)
 
 extern "C" int hs_net_get_state_monitor(hs_net *net, int mid, double *out, int64_t cap)
 {
-    if (!net || mid < 0 || mid >= (int)net->m.size() || net->m[mid].kind != 1) return hs_fail(HS_ERR_ARG, "bad monitor");
+    if (!net || mid < 0 || mid >= (int)net->m.size() || (net->m[mid].kind != 1 && net->m[mid].kind != 3))
+        return hs_fail(HS_ERR_ARG, "bad monitor");
     const HMon &M = net->m[mid];
     const int64_t rows = std::max<int64_t>(0, net->step - M.t0);
-    const int64_t w = M.stop - M.start;
+    const int64_t w = M.kind == 3 ? (int64_t)M.hsel.size() : M.stop - M.start;
     if (cap < rows * w) return hs_fail(HS_ERR_ARG, "output too small");
     if (!rows) return HS_OK;
     if (net->prec == HS_F64) {

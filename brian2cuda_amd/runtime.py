@@ -338,6 +338,14 @@ class Net:
             call("hs_net_add_spike_monitor_range", self.h, gid, int(start), int(stop), int(capacity), C.byref(mid))
         return mid.value
 
+    def add_synapse_monitor(self, pid, var, indices):
+        """StateMonitor of synapse variable var (HS_SYN_W, _APRE, _APOST) of the
+        synapses at `indices` (bundle order); read with state_monitor(mid, len(indices))."""
+        idx = np.ascontiguousarray(indices, dtype=np.int32)
+        mid = C.c_int()
+        call("hs_net_add_synapse_monitor", self.h, int(pid), int(var), _i32(idx), idx.size, C.byref(mid))
+        return mid.value
+
     def add_rate_monitor(self, gid, start=0, stop=None):
         """PopulationRateMonitor over neurons [start, stop) of a group."""
         stop = self.groups[gid][1] if stop is None else stop

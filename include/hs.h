@@ -240,6 +240,17 @@ int hs_net_get_synapse_var(hs_net *net, int pid, int var, double *out, int64_t c
    the buffers grow before a run that could fill them. */
 int hs_net_add_spike_monitor(hs_net *net, int gid, int64_t capacity, int *mid);
 /*
+ * StateMonitor of a synapse variable (HS_SYN_W, HS_SYN_APRE or HS_SYN_APOST) of
+ * the synapses at positions indices[0..n) of a pathway (bundle order, the
+ * order of hs_net_get_synapses).  Like Brian2's StateMonitor (when='start')
+ * each step records the values before that step's updates; read with
+ * hs_net_get_state_monitor (rows x n, time-major) or hs_net_write_monitor.
+ * Plain form only (where STDP runs); w needs per-synapse weights.
+ * Replaces a StateMonitor on a Synapses object (statemonitor.cu:9-100, as
+ * test_cpp_cuda_consistency.py:177-207 and examples/stdp.py:182 use it).
+ */
+int hs_net_add_synapse_monitor(hs_net *net, int pid, int var, const int32_t *indices, int n, int *mid);
+/*
  * SpikeMonitor on neurons [start, stop) of a group (a Brian2 Subgroup G[start:stop]):
  * its records are those of the range, indices relative to start.  The device
  * records each group's spikes once (in the group's whole-group monitor, created

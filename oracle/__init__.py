@@ -70,6 +70,7 @@ def lib():
         "ref_net_add_rate_monitor": (C.c_int, [V, C.c_int, C.c_int, C.c_int]),
         "ref_mon_get_rate": (C.c_int, [V, C.c_int, DP]),
         "ref_net_add_state_monitor": (C.c_int, [V, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "ref_net_add_synapse_monitor": (C.c_int, [V, C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int]),
         "ref_net_run": (C.c_int, [V, C.c_int64]),
         "ref_net_step": (C.c_int64, [V]),
         "ref_net_events": (C.c_uint64, [V]),
@@ -253,6 +254,13 @@ class RefNet:
         m = self.L.ref_net_add_state_monitor(self.h, gid, var, start, stop)
         if m < 0:
             raise ValueError("bad state monitor")
+        return m
+
+    def add_synapse_monitor(self, pid, var, indices):
+        idx = np.ascontiguousarray(indices, dtype=np.int32)
+        m = self.L.ref_net_add_synapse_monitor(self.h, pid, var, idx.ctypes.data_as(C.POINTER(C.c_int32)), idx.size)
+        if m < 0:
+            raise ValueError("bad synapse monitor")
         return m
 
     def run(self, nsteps):

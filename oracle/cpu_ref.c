@@ -249,12 +249,14 @@ typedef struct {
 } ref_path;
 
 typedef struct {
-    int kind; /* 0 spikes, 1 state */
+    int kind; /* 0 spikes, 1 state, 2 rate, 3 synapse variable */
     int gid, var, start, stop;
     int64_t n, cap;
     int32_t *i;
     int64_t *step;
     double *data;
+    int pid, nsel; /* kind 3: pathway and recorded synapse positions */
+    int32_t *sel;
 } ref_mon;
 
 struct ref_net {
@@ -515,6 +517,40 @@ int ref_net_add_state_monitor(ref_net *net, int gid, int var, int start, int sto
     return net->nm++;
 }
 
+/* StateMonitor of a synapse variable (w, Apre or Apost) of the synapses sel[],
+   when='start' (statemonitor.cu:9-100 on a Synapses source) */
+int ref_net_add_synapse_monitor(ref_net *net, int pid, int var, const int32_t *sel, int n)
+{
+    if (net->nm >= REF_MAX_MONS || pid < 0 || pid >= net->np || var < 0 || var > 2 || n <= 0) return -1;
+    const ref_path *p = &net->pw[pid];
+    if (var == 0 && p->wmode != HS_W_SYNAPSE) return -1;
+    if (var > 0 && !p->plastic) return -1;
+    for (int k = 0; k < n; ++k)
+        if (sel[k] < 0 || sel[k] >= p->nsyn) return -1;
+    ref_mon *m = &net->mon[net->nm];
+    memset(m, 0, sizeof(*m));
+    m->kind = 3;
+    m->gid = p->tgt; m->var = var; m->pid = pid; m->nsel = n;
+    m->sel = (int32_t *)malloc(sizeof(int32_t) * n);
+    memcpy(m->sel, sel, sizeof(int32_t) * n);
+    return net->nm++;
+}
+
+static void mon_syn(ref_net *net, ref_mon *m)
+{
+    const ref_path *p = &net->pw[m->pid];
+    const void *src = m->var == 0 ? p->w : (m->var == 1 ? p->apre : p->apost);
+    int64_t w = m->nsel;
+    if ((m->n + 1) * w > m->cap) {
+        int64_t c = m->cap ? m->cap : 1024 * w;
+        while (c < (m->n + 1) * w) c *= 2;
+        m->data = (double *)realloc(m->data, sizeof(double) * c);
+        m->cap = c;
+    }
+    for (int64_t j = 0; j < w; ++j) m->data[m->n * w + j] = rget(net, src, m->sel[j]);
+    m->n++;
+}
+
 static void mon_spikes(ref_net *net, ref_mon *m, const ref_group *g)
 {
     if (m->n + g->nspk > m->cap) {
@@ -644,9 +680,10 @@ int ref_mon_get_spikes(ref_net *net, int mid, int32_t *i, int64_t *step)
 
 int ref_mon_get_state(ref_net *net, int mid, double *out)
 {
-    if (mid < 0 || mid >= net->nm || net->mon[mid].kind != 1) return -1;
+    if (mid < 0 || mid >= net->nm || (net->mon[mid].kind != 1 && net->mon[mid].kind != 3)) return -1;
     ref_mon *m = &net->mon[mid];
-    if (m->n) memcpy(out, m->data, sizeof(double) * m->n * (m->stop - m->start));
+    const int64_t w = m->kind == 3 ? m->nsel : m->stop - m->start;
+    if (m->n) memcpy(out, m->data, sizeof(double) * m->n * w);
     return 0;
 }
 
@@ -665,6 +702,8 @@ void ref_net_destroy(ref_net *net)
         free(pw->q); free(pw->qlen); free(pw->qcap);
         free(pw->apre); free(pw->apost); free(pw->lastup); free(pw->post_start); free(pw->by_post);
     }
-    for (int m = 0; m < net->nm; ++m) { free(net->mon[m].i); free(net->mon[m].step); free(net->mon[m].data); }
+    for (int m = 0; m < net->nm; ++m) {
+        free(net->mon[m].i); free(net->mon[m].step); free(net->mon[m].data); free(net->mon[m].sel);
+    }
     free(net);
 }

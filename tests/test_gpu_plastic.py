@@ -234,3 +234,43 @@ def test_stdp_poisson_example_weights_split(hs, stdp_poisson):
     assert np.mean(w < 0.1 * gmax) > 0.15 and np.mean(w > 0.9 * gmax) > 0.1
     assert n2 < n1
     net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_synapse_monitor_vs_oracle(hs, oracle_lib, stdp_poisson, prec):
+    """StateMonitor(S, 'w' / 'Apre' / 'Apost', record=...) of examples/stdp.py:182
+    on the full-size example: every recorded row bit for bit against the oracle,
+    and row k equals the variable read after k steps."""
+    spec = stdp_poisson
+    net, _, _ = nw.build_hs(spec, prec, "plain")
+    ref, _, _ = oracle_lib.build_ref(spec, prec, oracle_lib.SEQ)
+    nsyn = net.pathway_info(0)["nsyn"]
+    sel = np.array([0, 1, 2, 3, 4, nsyn // 2, nsyn - 1])
+    gm = [net.add_synapse_monitor(0, v, sel) for v in (rt.HS_SYN_W, rt.HS_SYN_APRE, rt.HS_SYN_APOST)]
+    rm = [ref.add_synapse_monitor(0, v, sel) for v in (rt.HS_SYN_W, rt.HS_SYN_APRE, rt.HS_SYN_APOST)]
+    net.run(700)
+    ref.run(700)
+    w700 = net.synapse_var(0, rt.HS_SYN_W)[sel]
+    net.run(800)
+    ref.run(800)
+    for a, b in zip(gm, rm):
+        g = net.state_monitor(a, sel.size)
+        assert g.shape == (1500, sel.size)
+        assert np.array_equal(g, ref.state_monitor(b, sel.size))
+    np.testing.assert_array_equal(net.state_monitor(gm[0], sel.size)[700], w700)
+    net.close()
+
+
+def test_synapse_monitor_refusals(hs, stdp_poisson):
+    spec = stdp_poisson
+    net, _, _ = nw.build_hs(spec, 64, "plain")
+    nsyn = net.pathway_info(0)["nsyn"]
+    for args in ((0, rt.HS_SYN_W, [nsyn]), (0, rt.HS_SYN_W, [-1]), (0, rt.HS_SYN_LASTUPDATE, [0]), (1, 0, [0])):
+        with pytest.raises(HsError):
+            net.add_synapse_monitor(*args)
+    net.close()
+    fused = nw.cuba(N=300, seed=2)
+    net, _, _ = nw.build_hs(fused, 64, "fused")
+    with pytest.raises(HsError):
+        net.add_synapse_monitor(0, rt.HS_SYN_W, [0])
+    net.close()

@@ -192,3 +192,31 @@ def test_spikegen_rejects_times_outside_the_period():
     g = net.add_group(HS_MODEL_SPIKEGEN, 2, [10 * DT])
     with pytest.raises(ValueError):
         net.set_spikegen(g, [0], [10])
+
+
+def test_synapse_monitor_records_at_step_start():
+    """A synapse StateMonitor (when='start') records, at the start of step k, the
+    values the synapses hold after k steps: row k equals the variable read after
+    run(k), for w and both traces."""
+    spec = nw.stdp_example(n_cells=40, seed=3)
+    ref, _, _ = oracle.build_ref(spec, 64, oracle.SEQ)
+    pid = len(spec.paths) - 1  # the STDP pathway S
+    nsyn = spec.paths[pid].pre.size
+    sel = np.array([0, 7, nsyn // 2, nsyn - 1])
+    mids = [ref.add_synapse_monitor(pid, v, sel) for v in (0, 1, 2)]
+    snaps = []
+    for k in (300, 450, 250):
+        ref.run(k)
+        snaps.append((ref.step, [ref.synapse_var(pid, v, nsyn)[sel] for v in (0, 1, 2)]))
+    ref.run(1)
+    changed = False
+    for v, mid in enumerate(mids):
+        rows = ref.state_monitor(mid, sel.size)
+        assert rows.shape == (ref.step, sel.size)
+        for step, vals in snaps:
+            np.testing.assert_array_equal(rows[step], vals[v])
+        changed |= bool(np.any(rows[0] != rows[-1]))
+    assert changed
+    with pytest.raises(ValueError):
+        ref.add_synapse_monitor(pid, 0, [nsyn])
+    ref.close()
