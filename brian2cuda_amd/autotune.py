@@ -120,9 +120,11 @@ def autotune(spec, precision, engine, steps=4000, warmup=300, cache=None, rounds
                                                 os.path.join(os.path.expanduser("~"), ".cache", "brian2cuda_amd")),
                                  "autotune.json")
     table = {}
-    if os.path.exists(path):
+    try:
         with open(path) as f:
             table = json.load(f)
+    except (OSError, ValueError):
+        table = {}  # no cache yet, or unreadable: tune afresh
     if key in table:
         return {int(k): v for k, v in table[key]["options"].items()}
     space = candidate_space(info["compute_units"], engine)
@@ -142,9 +144,13 @@ def autotune(spec, precision, engine, steps=4000, warmup=300, cache=None, rounds
         best_t = min(b) if opts else min(d)
     table[key] = {"options": {str(k): v for k, v in opts.items()}, "us_per_step": best_t,
                   "default_us_per_step": trials[0][1], "trials": len(trials), "seconds": time.time() - t0}
-    os.makedirs(os.path.dirname(path), exist_ok=True)
-    tmp = path + ".tmp"
-    with open(tmp, "w") as f:
-        json.dump(table, f, indent=1)
-    os.replace(tmp, path)
+    try:  # the cache is an optimisation: a read-only home must not fail the caller
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1)
+        os.replace(tmp, path)
+    except OSError as e:
+        if log:
+            log(f"autotune: cache not written ({e})")
     return opts

@@ -44,3 +44,19 @@ def test_tune_forms_finds_a_form_that_only_wins_with_its_own_shapes():
     best, t, trials = at.tune_forms(space, cost, 9)
     assert best == {9: 1, 8: 1024} and t == 4.7
     assert len(trials) >= 4
+
+
+def test_cache_write_failure_is_not_fatal(tmp_path, monkeypatch):
+    """An unwritable cache location only loses the cache: the tuned options are
+    still returned."""
+    from brian2cuda_amd import autotune as at
+    from brian2cuda_amd import _lib
+    monkeypatch.setattr(_lib, "device_info", lambda: {"arch": "test", "compute_units": 4})
+    monkeypatch.setattr(at, "measure_us_per_step", lambda *a, **k: 1.0)
+    blocked = tmp_path / "file"
+    blocked.write_text("not a directory")
+    logs = []
+    opts = at.autotune(nw.cuba(N=200, seed=1), 64, "plain", cache=str(blocked / "sub" / "autotune.json"),
+                       log=logs.append)
+    assert opts == {}
+    assert any("cache not written" in m for m in logs)
