@@ -319,7 +319,28 @@ def hines_bench(n_trees, steps, seed):
     ach = HINES_BYTES_PER_COMP_STEP * tb.n_comp / step_s / 1e9
     out["roofline_fp64"] = {"bound": "hbm", "achieved": round(ach, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(ach / HBM_PEAK_GBS, 6), "kernel": "hs_k_hines",
-                            "bytes_per_compartment_step": HINES_BYTES_PER_COMP_STEP}
+                            "bytes_per_compartment_step": HINES_BYTES_PER_COMP_STEP,
+                            "traffic": hines_traffic(n_trees, seed)}
+    return out
+
+
+def hines_traffic(n_trees, seed):
+    """Measured HBM bytes of one hs_k_hines launch (FP64 and FP32, rocprofv3
+    FETCH_SIZE / WRITE_SIZE scaled by a copy of known size) for the batch they
+    were measured on (profiles/hines_c5_traffic.json); None for another batch.
+    A launch reads the trees once and writes them back once, whatever its step
+    count."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hines_c5_traffic.json")
+    if n_trees != 2048 or seed != 1234 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    out = {}
+    for prec in ("f64", "f32"):
+        k = t["kernels"][f"hs_k_hines_{prec}"]
+        out[f"{prec}_bytes_per_launch"] = int(round(k["fetch_size_kib_per_dispatch"] * 1024 * t["fetch_scale"]
+                                                  + k["write_size_kib_per_dispatch"] * 1024 * t["write_scale"]))
+        out[f"{prec}_steps_per_launch_measured"] = k["steps_per_dispatch"]
     return out
 
 
