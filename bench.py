@@ -4,56 +4,61 @@ Metric: synaptic events/s and sim-s/wall-s, FP32 batched form vs FP64 plain form
 Workload (default): the Brunel-Hakim network -- N=100,000 noise-driven LIF
 neurons, connect(p=1000/N) (about 1e8 synapses, float32 weights J=-0.1 mV),
 delays U[0.5, 5] ms quantised to dt=0.1 ms (5..50 steps), sigma=1 mV, mu=25 mV,
-Philox noise; 1 s of simulated time (10,000 steps) after a warm-up.
+Philox noise; 1 s of simulated time (10,000 steps).
 
   fp32 batched : FP32 state, weights and arithmetic; effects accumulated per
-                 step; windows of W = min(8, min delay) steps, each one
-                 delivery launch (all bundles due in the window, plus the
-                 window's noise draws on otherwise idle blocks) and one update
-                 launch (W steps per neuron with the state in registers);
-                 K steps per captured HIP graph
+                 step; windows of W = min(8, min delay) steps (autotuned: one
+                 pipelined launch per window, or a delivery launch and an
+                 update launch); K steps per captured HIP graph
   fp64 plain   : FP64 state, weights and arithmetic; update kernel + delivery
                  kernel per step, effects applied in place (Brian2's order)
 
-A "step" is one simulated time step of the whole network (dt = 0.1 ms).  Inputs
-are built on the host from a seed and copied to HBM before timing; the timed
-region is the step loop only (like Network::_last_run_time).  `value` is the
-FP32 batched form's synaptic events per wall-clock second over the K timed steps.
+A "step" is one simulated time step of the whole network (dt = 0.1 ms).  The
+network starts quiescent (V0 = Vr: about 220 steps to the first spike), so every
+form first runs at least 1000 untimed steps; then every HIP graph the timed run
+replays is instantiated (hs_net_prepare), and the timed region is the
+configuration's whole 1 s (10,000 steps) -- more if --steps asks for more --
+with inputs already in HBM: the step loop only, like Network::_last_run_time.
+`value` is the FP32 batched form's synaptic events per wall-clock second over
+the timed steps.  --exact-steps times exactly --steps after --warmup (profiling).
 
 Roofline (dominant kernel of the FP32 form, the one with the largest device time
 in a kernel-timed run of the same network): algorithmic bytes per launch are the
-terms of the per-step byte model that the kernel touches, summed over the W steps
-of a launch, with n_spk and E the measured means of the timed run:
+terms of the per-step byte model that the kernel touches, summed over the steps
+of a launch:
   update  : N*(r_s + w_s) + n_spk*4          r_s = V + lastspike + not_ref =
             (FP32) 4+4+1, w_s = V + not_ref = 4+1 (halved FP fields of 17/9)
   delivery: n_spk*(8 + 12*b) + E*(4 + w_b)   b = delay slots per spike (46),
             w_b = 4 (FP32 per-synapse weight)
   window  : update + delivery (the pipelined form does both in one launch)
-The average launch duration: when the dominant kernel is the only kernel of
-the timed form (the pipelined windows), the timed run's HIP events around its
-K steps on the runtime's stream, divided by its launches; otherwise HIP events
-around every launch on the runtime's stream (hs_net_set_option(net, 7, 1): no
-graphs in that run), which add 2-3 us to each launch.
-roofline.atomics sets the same launch's synaptic events per second (one
-accumulator atomic each) against hs_scatter_add, random FP32 atomics alone into
-an array of N values: the path is bound by atomics, not by HBM bytes.
+Bytes and launch time come from one run: when the dominant kernel is the timed
+form's only kernel (the pipelined windows), the timed run itself -- n_spk and E
+its own means, the launch time its HIP events over its launches; otherwise the
+kernel-timed run (an event pair around every launch), with its own n_spk and E.
+roofline.atomics sets the launch's synaptic events per second (one accumulator
+atomic each) against hs_scatter_add, random FP32 atomics alone into an array of
+N values.
+
+other_configs: CUBA (4000), COBAHH (4k, 64k, 512k) and the 256-network batch at
+full size, both forms, default launch shapes.
 
 Drift: both forms are re-run with spike monitors over `--drift-steps` steps;
 spikes of each neuron are matched greedily in time order (pairs within 5 ms) and
 the RMS of the matched time differences is reported with the unmatched fraction,
-with each form's population rate and mean ISI CV, and the same matching for the
-FP64 fused form against FP64 plain (no change of arithmetic: the divergence the
-network's own sensitivity produces from a one-step reordering of additions), and
-for FP64 plain with neuron 0's initial V moved by one FP32 ulp, by one
-synaptic event (|J|) or by ten, against FP64 plain; the FP32 and the perturbed comparisons
-are also given over the first 500,
-1000, 2000 and 5000 steps, with the RMS of V between the two forms after each.
+by horizon, and the longest horizon over which it stays below one dt -- for the
+Brunel-Hakim network (with controls: FP64 fused vs FP64 plain, and FP64 plain
+with neuron 0's initial V moved by one FP32 ulp, one synaptic event or ten),
+and for the noise-free CUBA and COBAHH networks of 4000 neurons.
 
 Multicompartment: `--hines-trees` random branched HH trees (100-400
-compartments) advanced `--hines-steps` steps, FP64 and FP32 channel arithmetic.
+compartments) advanced `--hines-steps` steps, FP64 and FP32 channel arithmetic;
+its roofline is VALU issue (committed SQ counters per tree-step over the
+measured step time), since the trees stay on chip.
 
-CPU baseline: the oracle (serial C, cpp_standalone semantics, FP64) on the same
-network, timed over a bounded window of steps on one host core.
+CPU baseline: the oracle (C, cpp_standalone semantics, FP64) on the same
+network -- its OpenMP step loop (bit-identical to the serial one) over
+min(nproc, OMP_NUM_THREADS) host threads on the GPU run's timed window, and the
+serial loop on one thread over a bounded window.
 """
 import argparse
 import dataclasses
@@ -102,20 +107,36 @@ def weight_bytes(spec, prec):
     return (8 if prec == 64 else 4) if any(p.wmode == HS_W_SYNAPSE for p in spec.paths) else 0
 
 
-def run_form(spec, prec, engine, steps, warmup, options=None):
+def chunk_launches(steps, chunk, window):
+    """Launches of the window kernel in a run of `steps` steps replayed as graph
+    chunks of `chunk` steps, each chunk in windows of `window` steps."""
+    if chunk <= 0:
+        chunk = steps
+    sizes = [chunk] * (steps // chunk) + ([steps % chunk] if steps % chunk else [])
+    return sum(-(-k // window) for k in sizes)
+
+
+def run_form(spec, prec, engine, steps, settle, options=None):
+    """Build, settle `settle` untimed steps (the network leaves its quiescent
+    start), instantiate every graph the timed run launches (hs_net_prepare), then
+    time `steps` steps: the step loop only, as Network::_last_run_time."""
     from brian2cuda_amd import networks as nw
+    from brian2cuda_amd import runtime as rt
     t0 = time.time()
     net, _, _ = nw.build_hs(spec, prec, engine, options=options)
     build_s = time.time() - t0
-    net.run(warmup)
+    net.run(settle)
+    net.prepare(steps)
     s0 = net.stats()
     t1 = time.perf_counter()
     net.run(steps)
     wall_py = time.perf_counter() - t1
     s1 = net.stats()
+    window = int(s1["window"])
     res = {
         "engine": engine,
         "precision": prec,
+        "settle_steps": settle,
         "steps": steps,
         "wall_s": s1["last_run_wall_s"],
         "wall_s_python": wall_py,
@@ -123,12 +144,15 @@ def run_form(spec, prec, engine, steps, warmup, options=None):
         "events": int(s1["events"] - s0["events"]),
         "spikes": int(s1["spikes"] - s0["spikes"]),
         "launches_per_step": float(s1["launches_per_step"]),
-        "window": int(s1["window"]),
+        "window": window,
+        "window_launches": chunk_launches(steps, net.get_option(rt.OPT_GRAPH_CHUNK), window),
         "build_s": build_s,
     }
     res["events_per_s"] = res["events"] / res["wall_s"]
     res["sim_s_per_wall_s"] = steps * spec.dt / res["wall_s"]
     res["ms_per_step"] = 1e3 * res["wall_s"] / steps
+    res["spikes_per_step"] = res["spikes"] / steps
+    res["events_per_step"] = res["events"] / steps
     net.close()
     return res
 
@@ -140,6 +164,7 @@ def kernel_times(spec, prec, engine, steps, warmup, options=None):
     net, _, _ = nw.build_hs(spec, prec, engine, options=options)
     net.run(warmup)
     net.set_option(rt.OPT_KERNEL_TIMING, 1)
+    net.prepare(steps)
     s0 = net.stats()
     net.run(steps)
     s1 = net.stats()
@@ -204,6 +229,8 @@ def drift(spec, steps, tol_steps=50):
     res["v_rms_mV"] = {str(h): float(np.sqrt(np.mean((v32[h] - v64[h]) ** 2))) * 1e3 for h in sorted(v64)}
     res["fp64_plain"] = train_stats(i64, t64, spec.n_neurons, steps, spec.dt)
     res["fp32_batched"] = train_stats(i32, t32, spec.n_neurons, steps, spec.dt)
+    res["fp32_by_horizon"], res["rms_below_1dt_through_steps"] = drift_horizons(i64, t64, i32, t32,
+                                                                                 spec.n_neurons, steps, tol_steps)
     f64, g64 = spike_trains(spec, 64, "fused", steps)
     res["fp64_fused_vs_fp64_plain"] = match_spikes(i64, t64, f64, g64, spec.n_neurons, tol_steps)
     # FP64 plain against itself with neuron 0's initial V raised by (a) one FP32
@@ -295,6 +322,43 @@ def match_spikes(i64, t64, i32, t32, n, tol_steps):
             "unmatched_frac": unmatched / total if total else 0.0, "match_window_steps": tol_steps}
 
 
+DRIFT_GRID = (100, 200, 300, 500, 700, 1000, 1500, 2000, 3000, 5000, 7000, 10000)
+
+
+def drift_horizons(i64, t64, i32, t32, n, steps, tol_steps=50):
+    """The FP32-vs-FP64 matching over the first h steps for h on a grid, and the
+    longest prefix of the grid over which the RMS spike-time drift stays below
+    one step (dt)."""
+    rows, below, ok = {}, 0, True
+    for h in DRIFT_GRID:
+        if h > steps:
+            break
+        a, b = t64 < h, t32 < h
+        r = match_spikes(i64[a], t64[a], i32[b], t32[b], n, tol_steps)
+        rows[str(h)] = {"rms_dt_steps": r["rms_dt_steps"], "unmatched_frac": r["unmatched_frac"],
+                        "spikes_fp64": r["spikes_fp64"]}
+        ok = ok and (r["rms_dt_steps"] or 0.0) < 1.0
+        if ok:
+            below = h
+    return rows, below
+
+
+def drift_config(spec, steps, tol_steps=50):
+    """FP32 batched against FP64 plain for a configuration without noise: spike
+    matching over the whole run and by horizon, the V RMS after the run, and
+    each form's rate and CV."""
+    i64, t64, v64 = spike_trains(spec, 64, "plain", steps, v_at=())
+    i32, t32, v32 = spike_trains(spec, 32, "fused", steps, v_at=())
+    res = match_spikes(i64, t64, i32, t32, spec.n_neurons, tol_steps)
+    res["steps"] = steps
+    res["v_rms_mV_at_end"] = float(np.sqrt(np.nanmean((v32[steps] - v64[steps]) ** 2))) * 1e3
+    res["by_horizon"], res["rms_below_1dt_through_steps"] = drift_horizons(i64, t64, i32, t32, spec.n_neurons,
+                                                                            steps, tol_steps)
+    res["fp64_plain"] = train_stats(i64, t64, spec.n_neurons, steps, spec.dt)
+    res["fp32_batched"] = train_stats(i32, t32, spec.n_neurons, steps, spec.dt)
+    return res
+
+
 HINES_BYTES_PER_COMP_STEP = 68  # a, d, v, rhs r (32) + v w (8) + parent index (4) + m, h, n r/w (24)
 
 
@@ -315,13 +379,45 @@ def hines_bench(n_trees, steps, seed):
                             "sim_s_per_wall_s": steps * tb.dt / s["last_run_wall_s"],
                             "soma_spikes": int(h.spike_counts().sum())}
         h.close()
-    step_s = out["fp64"]["us_per_step"] * 1e-6
-    ach = HINES_BYTES_PER_COMP_STEP * tb.n_comp / step_s / 1e9
-    out["roofline_fp64"] = {"bound": "hbm", "achieved": round(ach, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(ach / HBM_PEAK_GBS, 6), "kernel": "hs_k_hines",
-                            "bytes_per_compartment_step": HINES_BYTES_PER_COMP_STEP,
-                            "traffic": hines_traffic(n_trees, seed)}
+    out["roofline_fp64"] = hines_valu_roofline(out, "f64", n_trees, seed)
+    out["roofline_fp32"] = hines_valu_roofline(out, "f32", n_trees, seed)
     return out
+
+
+# MI355X: 256 CUs x 4 SIMDs at the 2.4 GHz peak engine clock; a wave64 VALU
+# instruction (FP64, or FP32 unpacked) occupies its SIMD for 4 cycles
+SIMDS, CLOCK_HZ, VALU_CYCLES = 1024, 2.4e9, 4
+VALU_PEAK_GINSTR = SIMDS * CLOCK_HZ / VALU_CYCLES / 1e9
+
+
+def hines_valu_roofline(out, prec, n_trees, seed):
+    """hs_k_hines is bound by instruction issue, not by HBM (the trees stay on
+    chip: measured traffic is under 1 B per compartment-step).  achieved = wave
+    VALU instructions issued per second: the committed SQ_INSTS_VALU per
+    tree-step (rocprofv3, profiles/hines_c5_counters.json, the same 2048 trees)
+    x trees / this run's step time; peak = what the SIMDs can issue.  None for
+    another batch.  The HBM figure (68 B per compartment-step, the algorithmic
+    byte model) is kept beside it for reference."""
+    key = "fp64" if prec == "f64" else "fp32"
+    step_s = out[key]["us_per_step"] * 1e-6
+    hbm = HINES_BYTES_PER_COMP_STEP * out["compartments"] / step_s / 1e9
+    base = {"kernel": "hs_k_hines", "precision_channels": key, "hbm_model_gbs": round(hbm, 3),
+            "hbm_model_frac": round(hbm / HBM_PEAK_GBS, 6), "bytes_per_compartment_step_model": HINES_BYTES_PER_COMP_STEP,
+            "traffic": hines_traffic(n_trees, seed)}
+    path = os.path.join(ROOT, "profiles", "hines_c5_counters.json")
+    if n_trees != 2048 or seed != 1234 or not os.path.exists(path):
+        return {**base, "bound": "valu", "achieved": None, "peak": round(VALU_PEAK_GINSTR, 3),
+                "unit": "G wave-instr/s", "frac": None}
+    with open(path) as f:
+        c = json.load(f)
+    k = c["kernels"][f"hs_k_hines_{prec}"]
+    per_tree_step = k["SQ_INSTS_VALU"] / k["SQ_WAVES"] / (c["steps_counted"] / 2)
+    ach = per_tree_step * n_trees / step_s / 1e9
+    return {**base, "bound": "valu", "achieved": round(ach, 3), "peak": round(VALU_PEAK_GINSTR, 3),
+            "unit": "G wave-instr/s", "frac": round(ach / VALU_PEAK_GINSTR, 4),
+            "valu_instr_per_tree_step": round(per_tree_step, 1),
+            "valu_active_frac_per_wave": round(k["SQ_ACTIVE_INST_VALU"] / k["SQ_WAVE_CYCLES"], 4),
+            "wait_frac_per_wave": round(k["SQ_WAIT_INST_ANY"] / k["SQ_WAVE_CYCLES"], 4)}
 
 
 def hines_traffic(n_trees, seed):
@@ -384,44 +480,132 @@ def connect_bench(args, repeats=3):
             "bytes_written": int(nsyn * 8 + 4 * (args.N * nslots + 1))}
 
 
-def cpu_baseline(spec, skip_steps, steps):
-    """Oracle (cpp_standalone semantics, FP64, 1 core) on a window of steps."""
+def cpu_info():
+    """host CPU model, nproc (CPUs this process may run on) and OMP_NUM_THREADS"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return {"model": model, "nproc": nproc, "omp_num_threads": int(omp) if omp and omp.isdigit() else None}
+
+
+def cpu_baseline(spec, settle, steps, threads, serial_skip, serial_steps):
+    """The oracle (cpp_standalone semantics, FP64, SEQ order) on the same network:
+    its OpenMP step loop over `threads` threads on the timed window of the GPU
+    run (settle, then `steps` steps), and the serial loop (one thread, the
+    cpp_standalone default) on a bounded window.  Both give the same bits."""
     import oracle
-    t0 = time.time()
+    out = {}
+    if threads > 1:
+        t0 = time.time()
+        ref, _, _ = oracle.build_ref(spec, 64, oracle.SEQ, threads=threads)
+        build_s = time.time() - t0
+        ref.run(settle)
+        e0 = ref.events
+        t1 = time.perf_counter()
+        ref.run(steps)
+        wall = time.perf_counter() - t1
+        ev = ref.events - e0
+        ref.close()
+        out["threaded"] = {"threads": threads, "value": ev / wall, "wall_s": wall, "events": int(ev), "steps": steps,
+                           "skip": settle, "sim_s_per_wall_s": steps * spec.dt / wall, "build_s": build_s}
     ref, _, _ = oracle.build_ref(spec, 64, oracle.SEQ)
-    build_s = time.time() - t0
-    ref.run(skip_steps)
+    ref.run(serial_skip)
     e0 = ref.events
     t1 = time.perf_counter()
-    ref.run(steps)
+    ref.run(serial_steps)
     wall = time.perf_counter() - t1
     ev = ref.events - e0
     ref.close()
-    return {"value": ev / wall, "wall_s": wall, "events": int(ev), "steps": steps, "skip": skip_steps,
-            "sim_s_per_wall_s": steps * spec.dt / wall, "build_s": build_s}
+    out["serial"] = {"threads": 1, "value": ev / wall, "wall_s": wall, "events": int(ev), "steps": serial_steps,
+                     "skip": serial_skip, "sim_s_per_wall_s": serial_steps * spec.dt / wall}
+    return out
+
+
+def other_spec(name, seed):
+    """the other configurations at full size (no monitors)"""
+    from brian2cuda_amd import networks as nw
+    if name == "cuba_4k":
+        return nw.cuba(N=4000, seed=seed, monitors=False)
+    if name.startswith("cobahh_"):
+        return nw.cobahh(N={"cobahh_4k": 4000, "cobahh_64k": 64000, "cobahh_512k": 512000}[name], seed=seed,
+                         monitors=False)
+    if name == "batch_256_cuba":
+        return nw.batch_cuba(n_nets=256, seed=seed, monitors=False)
+    raise ValueError(name)
+
+
+OTHER_CONFIGS = ("cuba_4k", "cobahh_4k", "cobahh_64k", "cobahh_512k", "batch_256_cuba")
+
+
+def other_configs(args):
+    """The other configurations at their full sizes: FP32 batched and FP64 plain,
+    `--other-steps` timed steps after `--other-settle` (default launch shapes),
+    with the per-step byte model's HBM fraction of the FP32 form's step."""
+    out = {}
+    for name in OTHER_CONFIGS:
+        t0 = time.time()
+        spec = other_spec(name, args.seed)
+        gen_s = time.time() - t0
+        row = {"neurons": spec.n_neurons, "synapses": spec.n_synapses, "spec_s": round(gen_s, 2)}
+        for key, prec, engine in (("fp32_batched", 32, "fused"), ("fp64_plain", 64, "plain")):
+            r = run_form(spec, prec, engine, args.other_steps, args.other_settle)
+            row[key] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
+        f, p = row["fp32_batched"], row["fp64_plain"]
+        row["speedup_fp32_batched_vs_fp64_plain"] = round(p["ms_per_step"] / f["ms_per_step"], 4)
+        wl = "batch" if name.startswith("batch") else ("cuba" if name.startswith("cuba") else "cobahh")
+        r_s, w_s = state_bytes(wl, 32)
+        bstep = spec.n_neurons * (r_s + w_s) + f["spikes_per_step"] * 12 + f["events_per_step"] * 4
+        row["fp32_step_bytes_model"] = int(bstep)
+        row["fp32_step_hbm_frac"] = round(bstep / (f["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
+        out[name] = row
+        log(name, row)
+    return out
+
+
+CONFIG_STEPS = 10000  # every configuration simulates 1 s at dt = 0.1 ms
+SETTLE_MIN = 1000     # the networks start quiescent (C3: V0 = Vr, about 220 steps to the first spike)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000, help="timed simulation steps (dt=0.1 ms)")
-    ap.add_argument("--warmup", type=int, default=1000, help="untimed steps before timing")
+    ap.add_argument("--steps", type=int, default=CONFIG_STEPS,
+                    help="timed simulation steps requested (dt=0.1 ms); fewer than the configuration's "
+                         f"{CONFIG_STEPS} (1 s) time the {CONFIG_STEPS} anyway")
+    ap.add_argument("--warmup", type=int, default=SETTLE_MIN,
+                    help=f"untimed steps before timing requested; fewer than {SETTLE_MIN} settle {SETTLE_MIN}")
     ap.add_argument("--workload", default="brunel", choices=["brunel", "cuba", "batch", "cobahh"])
     ap.add_argument("--N", type=int, default=None)
     ap.add_argument("--K", type=int, default=1000)
     ap.add_argument("--nets", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--cpu-steps", type=int, default=3000, help="oracle timed steps (0: skip); 3000 is about 15 s of one core")
-    ap.add_argument("--cpu-skip", type=int, default=300, help="oracle untimed steps before its window")
+    ap.add_argument("--exact-steps", action="store_true",
+                    help="time exactly --steps after exactly --warmup steps (profiling runs)")
+    ap.add_argument("--cpu-steps", type=int, default=2000,
+                    help="serial oracle timed steps (0: skip every CPU leg); 2000 is about 10 s of one core")
+    ap.add_argument("--cpu-skip", type=int, default=300, help="serial oracle untimed steps before its window")
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="threads of the OpenMP oracle run (default: min(nproc, OMP_NUM_THREADS))")
     ap.add_argument("--kt-steps", type=int, default=2000, help="steps of the kernel-timed run (0: skip)")
-    ap.add_argument("--drift-steps", type=int, default=10000, help="steps of the drift comparison (0: skip)")
+    ap.add_argument("--drift-steps", type=int, default=CONFIG_STEPS, help="steps of the drift comparisons (0: skip)")
     ap.add_argument("--hines-trees", type=int, default=2048, help="trees of the multicompartment batch")
     ap.add_argument("--hines-steps", type=int, default=2000, help="steps of the multicompartment run (0: skip)")
+    ap.add_argument("--other-steps", type=int, default=3000, help="timed steps of the other configurations (0: skip)")
+    ap.add_argument("--other-settle", type=int, default=500, help="untimed steps of the other configurations")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-connect", action="store_true", help="skip the device connectivity-build timing")
     ap.add_argument("--options", default=None,
                     help='JSON {"key": value} of engine options for the FP32 batched form instead of autotuning '
                          '(e.g. what a previous run reported as tuned_options.fp32_batched)')
+    ap.add_argument("--fp64-options", default=None, help="the same for the FP64 plain form")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time both forms with the engine's default launch shapes instead of autotuned ones")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -443,6 +627,9 @@ def main():
         sys.exit(2)
     if args.N is None:
         args.N = {"brunel": 100_000, "cuba": 4000, "cobahh": 4000, "batch": 0}[args.workload]
+    steps = args.steps if args.exact_steps else max(args.steps, CONFIG_STEPS)
+    settle = args.warmup if args.exact_steps else max(args.warmup, SETTLE_MIN)
+    log(f"timing {steps} steps after {settle} settle steps (asked: --steps {args.steps} --warmup {args.warmup})")
 
     import brian2cuda_amd
     brian2cuda_amd.init(0)
@@ -455,8 +642,11 @@ def main():
     D = max((p.delay_steps.max() - p.delay_steps.min() + 1) if p.delay_steps is not None else 1 for p in spec.paths)
 
     tuned = {"fp32_batched": {}, "fp64_plain": {}}
-    if args.options:
-        tuned["fp32_batched"] = {int(k): v for k, v in json.loads(args.options).items()}
+    if args.options or args.fp64_options:
+        if args.options:
+            tuned["fp32_batched"] = {int(k): v for k, v in json.loads(args.options).items()}
+        if args.fp64_options:
+            tuned["fp64_plain"] = {int(k): v for k, v in json.loads(args.fp64_options).items()}
     elif not args.no_autotune:
         from brian2cuda_amd import autotune as at
         from brian2cuda_amd import networks as nw
@@ -469,52 +659,69 @@ def main():
         if not args.no_fp64:
             tuned["fp64_plain"] = at.autotune(spec, 64, "plain", build=build, log=log)
         log(f"autotuned in {time.time() - t0:.1f}s: {tuned}")
-    fp32 = run_form(spec, 32, "fused", args.steps, args.warmup, options=tuned["fp32_batched"])
+    fp32 = run_form(spec, 32, "fused", steps, settle, options=tuned["fp32_batched"])
     log("fp32 batched", fp32)
     fp64 = None
     if not args.no_fp64:
-        fp64 = run_form(spec, 64, "plain", args.steps, args.warmup, options=tuned["fp64_plain"])
+        fp64 = run_form(spec, 64, "plain", steps, settle, options=tuned["fp64_plain"])
         log("fp64 plain", fp64)
 
     N = spec.n_neurons
     roofline = None
     if args.kt_steps > 0:
-        kt, W, n_spk, E = kernel_times(spec, 32, "fused", args.kt_steps, args.warmup, options=tuned["fp32_batched"])
+        kt, W, kn_spk, kE = kernel_times(spec, 32, "fused", args.kt_steps, settle, options=tuned["fp32_batched"])
         log("kernel times (fp32 batched)", kt)
         r_s, w_s = state_bytes(args.workload, 32)
         w_b = weight_bytes(spec, 32)
-        per_step = {
-            "hs_k_update": N * (r_s + w_s) + n_spk * 4,
-            "hs_k_deliver": n_spk * (8 + 12 * D) + E * (4 + w_b),
-            "hs_k_gather": n_spk * (8 + 12 * D) + E * (4 + w_b),
-            "hs_k_fused": N * (r_s + w_s) + n_spk * (12 + 12 * D) + E * (4 + w_b),
-            "hs_k_window": N * (r_s + w_s) + n_spk * (12 + 12 * D) + E * (4 + w_b),
-        }
+
+        def per_step(name, n_spk, E):
+            return {
+                "hs_k_update": N * (r_s + w_s) + n_spk * 4,
+                "hs_k_deliver": n_spk * (8 + 12 * D) + E * (4 + w_b),
+                "hs_k_gather": n_spk * (8 + 12 * D) + E * (4 + w_b),
+                "hs_k_fused": N * (r_s + w_s) + n_spk * (12 + 12 * D) + E * (4 + w_b),
+                "hs_k_window": N * (r_s + w_s) + n_spk * (12 + 12 * D) + E * (4 + w_b),
+            }[name]
+
         name = max(kt, key=lambda k: kt[k]["total_ms"])
-        steps_per_launch = args.kt_steps / kt[name]["launches"]
-        bpl = per_step[name] * steps_per_launch
-        # launch duration: when the dominant kernel is the only one timed, the
-        # timed run's own HIP events (around all its steps, on the runtime's
-        # stream) over its launches; an event pair around every launch adds its
-        # own 2-3 us.  Otherwise the per-launch event pairs.
-        alone = len(kt) == 1
-        avg_us = fp32["gpu_ms"] * 1e3 / (args.steps / steps_per_launch) if alone else kt[name]["avg_us"]
+        if len(kt) == 1 and name == "hs_k_window":
+            # the dominant kernel is the timed form's only kernel: its bytes and
+            # its launch time both from the timed run itself (HIP events around
+            # its steps on the runtime's stream, over its launches)
+            launches = fp32["window_launches"]
+            steps_per_launch = steps / launches
+            bpl = per_step(name, fp32["spikes_per_step"], fp32["events_per_step"]) * steps_per_launch
+            avg_us = fp32["gpu_ms"] * 1e3 / launches
+            src = "timed run: its HIP events over its window launches; bytes from its own spike and event counts"
+            ev_launch = fp32["events_per_step"] * steps_per_launch
+        else:
+            # several kernels per step: the kernel-timed run (an event pair per
+            # launch), bytes from that run's own counts
+            steps_per_launch = args.kt_steps / kt[name]["launches"]
+            bpl = per_step(name, kn_spk, kE) * steps_per_launch
+            avg_us = kt[name]["avg_us"]
+            src = "kernel-timed run: an event pair per launch; bytes from its own spike and event counts"
+            ev_launch = kE * steps_per_launch
         achieved = bpl / (avg_us * 1e-6) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(args, name),
                     "kernel": name, "bytes_per_launch": int(bpl), "steps_per_launch": round(steps_per_launch, 3),
-                    "avg_launch_us": round(avg_us, 3),
-                    "avg_launch_from": "timed region / launches" if alone else "event pair per launch",
+                    "avg_launch_us": round(avg_us, 3), "avg_launch_from": src,
                     "avg_launch_us_event_pairs": round(kt[name]["avg_us"], 3),
                     "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kt.items()}}
-        roofline["atomics"] = atomic_ceiling(N, E * steps_per_launch, kt[name]["avg_us"], args.seed)
-        roofline["atomics"]["achieved_per_s_timed_region"] = E * steps_per_launch / (avg_us * 1e-6)
+        roofline["atomics"] = atomic_ceiling(N, ev_launch, avg_us, args.seed)
         log("atomics", roofline["atomics"])
 
     dr = None
     if args.drift_steps > 0 and not args.no_fp64 and args.workload == "brunel":
-        dr = drift(spec, args.drift_steps)
+        dr = {"brunel_hakim_c3": drift(spec, args.drift_steps)}
+        for name in ("cuba_4k", "cobahh_4k"):
+            dr[name] = drift_config(other_spec(name, args.seed), args.drift_steps)
         log("drift", dr)
+
+    others = None
+    if args.other_steps > 0 and not args.no_fp64 and args.workload == "brunel":
+        others = other_configs(args)
 
     conn = None
     if args.workload == "brunel" and not args.no_connect:
@@ -530,12 +737,23 @@ def main():
 
     cpu = None
     if args.cpu_steps > 0:
-        cb = cpu_baseline(spec, args.cpu_skip, args.cpu_steps)
+        ci = cpu_info()
+        threads = args.cpu_threads or min(ci["nproc"], ci["omp_num_threads"] or ci["nproc"])
+        cb = cpu_baseline(spec, settle, steps, threads, args.cpu_skip, args.cpu_steps)
         log("cpu baseline", cb)
-        cpu = {"value": round(cb["value"], 1), "unit": "synaptic events/s", "cores": 1, "kind": "port",
-               "sample": f"oracle (serial C, FP64, cpp_standalone semantics) on the same network, steps "
-                         f"{args.cpu_skip}..{args.cpu_skip + args.cpu_steps} ({cb['events']} events in "
-                         f"{cb['wall_s']:.2f} s; {cb['sim_s_per_wall_s']:.4f} sim-s/wall-s)"}
+        main_leg = cb.get("threaded", cb["serial"])
+        ser = cb["serial"]
+        cpu = {"value": round(main_leg["value"], 1), "unit": "synaptic events/s", "cores": main_leg["threads"],
+               "kind": "port",
+               "sample": f"oracle (C, FP64, cpp_standalone semantics and order) on the same network: OpenMP step "
+                         f"loop over {main_leg['threads']} threads, steps {main_leg['skip']}.."
+                         f"{main_leg['skip'] + main_leg['steps']} (the GPU run's timed window; "
+                         f"{main_leg['events']} events in {main_leg['wall_s']:.2f} s, "
+                         f"{main_leg['sim_s_per_wall_s']:.4f} sim-s/wall-s); serial loop on one thread, steps "
+                         f"{ser['skip']}..{ser['skip'] + ser['steps']} ({ser['value']:.4g} events/s, "
+                         f"{ser['sim_s_per_wall_s']:.4f} sim-s/wall-s)",
+               "serial_1_thread": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in ser.items()},
+               "cpu": ci}
 
     out = {
         "metric": "synaptic events/s and sim-s/wall-s: FP32 batched vs FP64 plain, 1 MI355X",
@@ -544,6 +762,12 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "steps_timed": steps,
+        "settle_steps_run": settle,
+        "step_accounting": (f"a step is one 0.1 ms simulation step of the whole network; the timed region is "
+                            f"{steps} steps ({steps * spec.dt:g} s simulated, the configuration's duration) after "
+                            f"{settle} untimed steps, whatever smaller --steps/--warmup ask for "
+                            f"(--exact-steps times exactly what is asked)"),
         "ms_per_step": round(fp32["ms_per_step"], 6),
         "higher_is_better": True,
         "scaling": "weak",
@@ -553,18 +777,19 @@ def main():
         "config": {"workload": {"brunel": "brunel_hakim_heterog_delays", "cuba": "cuba", "batch": "batch_cuba",
                                 "cobahh": "cobahh"}[args.workload],
                    "neurons": N, "synapses": spec.n_synapses, "dt_ms": spec.dt * 1e3,
-                   "sim_s": args.steps * spec.dt, "delay_slots": int(D), "seed": args.seed},
+                   "sim_s": steps * spec.dt, "delay_slots": int(D), "seed": args.seed},
         "sim_s_per_wall_s": round(fp32["sim_s_per_wall_s"], 4),
         "fp32_batched": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in fp32.items()},
         "fp64_plain": ({k: (round(v, 6) if isinstance(v, float) else v) for k, v in fp64.items()} if fp64 else None),
         "speedup_fp32_batched_vs_fp64_plain": (round(fp32["events_per_s"] / fp64["events_per_s"], 4)
                                                if fp64 and fp64["events_per_s"] > 0 else None),
-        "drift_fp32_batched_vs_fp64_plain": dr,
         "roofline": roofline,
+        "cpu_baseline": cpu,
+        "other_configs": others,
+        "drift_fp32_batched_vs_fp64_plain": dr,
         "tuned_options": {k: {str(kk): vv for kk, vv in v.items()} for k, v in tuned.items()},
         "multicompartment": hines,
         "connectivity_build": conn,
-        "cpu_baseline": cpu,
         "device": info,
     }
     print(json.dumps(out), flush=True)

@@ -7,16 +7,10 @@ This package holds the Python host side:
 
   runtime    ctypes-level objects (device arrays, standalone kernels, Net)
   networks   seeded synthetic workloads (CUBA, COBAHH, Brunel-Hakim, batches)
-  device     the Brian2 Device plugin (registered when brian2 is importable)
   build      builds libhs.so for gfx950
+
+Importing the package never needs Brian2 and never touches the GPU.
 """
 from ._lib import HsError, device_info, init, load_library  # noqa: F401
 
 __all__ = ["HsError", "device_info", "init", "load_library"]
-
-try:  # register set_device("hip_standalone") when Brian2 is present
-    import brian2  # noqa: F401
-except ImportError:  # Brian2 is optional: the runtime works without it
-    brian2 = None
-else:
-    from . import device  # noqa: F401

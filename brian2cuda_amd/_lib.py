@@ -82,6 +82,7 @@ SIGNATURES = {
     "hs_net_set_option": (C.c_int, [P, C.c_int, C.c_longlong]),
     "hs_net_get_option": (C.c_int, [P, C.c_int, C.POINTER(C.c_longlong)]),
     "hs_net_run": (C.c_int, [P, C.c_int64]),
+    "hs_net_prepare": (C.c_int, [P, C.c_int64]),
     "hs_net_stats": (C.c_int, [P, C.POINTER(HsStats)]),
     "hs_net_kernel_time": (C.c_int, [P, C.c_int, DP, I64P]),
     "hs_net_connect_random": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

@@ -1784,9 +1784,9 @@ static int hs_enqueue_steps(hs_net *net, int64_t k)
     return net->prec == HS_F64 ? hs_enqueue_steps_t<double>(net, k) : hs_enqueue_steps_t<float>(net, k);
 }
 
-static int hs_chunk(hs_net *net, int64_t k)
+/* the instantiated graph of a chunk of k steps, captured on first use */
+static int hs_chunk_graph(hs_net *net, int64_t k, hipGraphExec_t *out)
 {
-    if (net->opt[2] <= 0 || net->opt[7]) return hs_enqueue_steps(net, k);
     auto it = net->graphs.find(k);
     if (it == net->graphs.end()) {
         hipGraph_t graph;
@@ -1801,7 +1801,43 @@ static int hs_chunk(hs_net *net, int64_t k)
         if (e != hipSuccess) return hs_fail(HS_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
         it = net->graphs.emplace(k, exec).first;
     }
-    HS_HIP(hipGraphLaunch(it->second, g_rt.stream));
+    *out = it->second;
+    return HS_OK;
+}
+
+static int hs_chunk(hs_net *net, int64_t k)
+{
+    if (net->opt[2] <= 0 || net->opt[7]) return hs_enqueue_steps(net, k);
+    hipGraphExec_t exec;
+    HS_TRY(hs_chunk_graph(net, k, &exec));
+    HS_HIP(hipGraphLaunch(exec, g_rt.stream));
+    return HS_OK;
+}
+
+/* device state, state-monitor rows and descriptors for a run of nsteps steps */
+static int hs_prepare_run(hs_net *net, int64_t nsteps)
+{
+    HS_TRY(hs_build(net));
+    HS_TRY(hs_grow_state_monitors(net, net->step + nsteps));
+    if (net->desc_dirty) {
+        hs_drop_graphs(net);
+        HS_TRY(hs_upload_desc(net));
+    }
+    return HS_OK;
+}
+
+extern "C" int hs_net_prepare(hs_net *net, int64_t nsteps)
+{
+    if (!net) return hs_fail(HS_ERR_ARG, "null net");
+    if (nsteps < 0) return hs_fail(HS_ERR_ARG, "negative step count");
+    if (net->g.empty()) return hs_fail(HS_ERR_STATE, "network has no groups");
+    HS_TRY(hs_prepare_run(net, nsteps));
+    if (net->opt[2] <= 0 || net->opt[7] || nsteps == 0) return HS_OK;
+    const int64_t K = net->opt[2];
+    hipGraphExec_t exec;
+    if (nsteps >= K) HS_TRY(hs_chunk_graph(net, K, &exec));
+    if (nsteps % K) HS_TRY(hs_chunk_graph(net, nsteps % K, &exec));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
     return HS_OK;
 }
 
@@ -1845,12 +1881,7 @@ extern "C" int hs_net_run(hs_net *net, int64_t nsteps)
     if (!net) return hs_fail(HS_ERR_ARG, "null net");
     if (nsteps < 0) return hs_fail(HS_ERR_ARG, "negative step count");
     if (net->g.empty()) return hs_fail(HS_ERR_STATE, "network has no groups");
-    HS_TRY(hs_build(net));
-    HS_TRY(hs_grow_state_monitors(net, net->step + nsteps));
-    if (net->desc_dirty) {
-        hs_drop_graphs(net);
-        HS_TRY(hs_upload_desc(net));
-    }
+    HS_TRY(hs_prepare_run(net, nsteps));
     bool have_smon = false;
     for (const HMon &M : net->m) have_smon |= (M.kind == 0);
     const int64_t K = net->opt[2] > 0 ? net->opt[2] : 256;

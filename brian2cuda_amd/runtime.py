@@ -376,6 +376,10 @@ class Net:
     def run(self, nsteps):
         call("hs_net_run", self.h, int(nsteps))
 
+    def prepare(self, nsteps):
+        """Instantiate everything a following run(nsteps) launches (hs_net_prepare)."""
+        call("hs_net_prepare", self.h, int(nsteps))
+
     def stats(self):
         s = HsStats()
         call("hs_net_stats", self.h, C.byref(s))

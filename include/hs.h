@@ -298,6 +298,19 @@ int hs_net_kernel_time(hs_net *net, int kind, double *total_ms, int64_t *launche
  */
 int hs_net_run(hs_net *net, int64_t nsteps);
 
+/*
+ * Everything a following hs_net_run(net, nsteps) needs before its first step,
+ * without advancing the network: the device rings and accumulators, the
+ * state-monitor rows, the descriptors, and the instantiated HIP graph of every
+ * chunk size that run launches.  That run then only replays graphs (unless a
+ * spike monitor has to grow first).  The generated code's Network::run has no
+ * such split: its set-up (the _before_run_* code objects, network.cu:40-60)
+ * and its step loop run in one call, and Network::_last_run_time times the loop
+ * (network.cu:118-126); this call keeps graph instantiation out of the loop
+ * that hs_stats.last_run_wall_s times.
+ */
+int hs_net_prepare(hs_net *net, int64_t nsteps);
+
 /* Run statistics. */
 typedef struct {
     int64_t steps;           /* steps simulated so far */

@@ -72,6 +72,7 @@ def lib():
         "ref_net_add_state_monitor": (C.c_int, [V, C.c_int, C.c_int, C.c_int, C.c_int]),
         "ref_net_add_synapse_monitor": (C.c_int, [V, C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int]),
         "ref_net_run": (C.c_int, [V, C.c_int64]),
+        "ref_net_set_threads": (C.c_int, [V, C.c_int]),
         "ref_net_step": (C.c_int64, [V]),
         "ref_net_events": (C.c_uint64, [V]),
         "ref_mon_count": (C.c_int64, [V, C.c_int]),
@@ -264,7 +265,16 @@ class RefNet:
         return m
 
     def run(self, nsteps):
-        self.L.ref_net_run(self.h, int(nsteps))
+        r = self.L.ref_net_run(self.h, int(nsteps))
+        if r < 0:
+            raise RuntimeError(f"ref_net_run failed ({r})")
+
+    def set_threads(self, n):
+        """Run the step loop over n OpenMP threads (before the first step);
+        the results stay bit-identical to the serial loop."""
+        r = self.L.ref_net_set_threads(self.h, int(n))
+        if r < 0:
+            raise ValueError(f"ref_net_set_threads({n}) failed ({r})")
 
     @property
     def step(self):
@@ -298,9 +308,12 @@ def random_pathway(spec, path, rows=None):
                           spec.dt, r.w)
 
 
-def build_ref(spec, precision=64, semantics=SEQ):
-    """Instantiate a brian2cuda_amd.networks.NetSpec on the oracle."""
+def build_ref(spec, precision=64, semantics=SEQ, threads=1):
+    """Instantiate a brian2cuda_amd.networks.NetSpec on the oracle (its step
+    loop over `threads` OpenMP threads)."""
     net = RefNet(precision, semantics, spec.dt, spec.seed)
+    if threads > 1:
+        net.set_threads(threads)
     for g in spec.groups:
         gid = net.add_group(g.model, g.n, g.params)
         for var, val in g.init.items():

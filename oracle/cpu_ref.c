@@ -37,6 +37,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "../include/hs_models.h"
 #include "../brian2cuda_amd/csrc/hs_model_exprs.h"
@@ -201,6 +204,7 @@ int ref_model_constants(int model, const double *params, int nparams, double dt,
 #define REF_MAX_GROUPS 16
 #define REF_MAX_PATHS 32
 #define REF_MAX_MONS 32
+#define REF_MAX_THREADS 256
 
 typedef struct {
     int model, n, nvars;
@@ -238,6 +242,11 @@ typedef struct {
     int Q, offset;
     int64_t **q;
     int64_t *qlen, *qcap;
+    /* threaded form (nthr > 1): thread t owns the targets [n_tgt*t/nthr,
+       n_tgt*(t+1)/nthr) and a queue of its own (tq[t*Q + slot]) holding only
+       the synapses onto them, pushed in the serial order */
+    int64_t **tq;
+    int64_t *tqlen, *tqcap;
     /* STDP (plastic != 0): traces and last update step per synapse, and the
        synapses of each postsynaptic neuron (CSR, synapse order) for on_post */
     int plastic;
@@ -261,6 +270,8 @@ typedef struct {
 
 struct ref_net {
     int prec, sem;
+    int nthr;    /* OpenMP threads of the step loop (1: the serial restatement) */
+    int thr_err; /* a parallel region ran with fewer threads than nthr */
     double dt;
     uint64_t seed;
     int64_t step;
@@ -286,7 +297,31 @@ ref_net *ref_net_create(int precision, int semantics, double dt, uint64_t seed)
     net->sem = semantics;
     net->dt = dt;
     net->seed = seed;
+    net->nthr = 1;
     return net;
+}
+
+/*
+ * The step loop over nthreads OpenMP threads, as cpp_standalone runs with
+ * devices.cpp_standalone.openmp_threads (the reference's CPU benchmark,
+ * brian2cuda/tests/features/cuda_configuration.py:396-418).  The results stay
+ * identical to the serial loop, bit for bit: each thread updates a contiguous
+ * range of neurons and the spike lists are joined in range order; each thread
+ * owns a contiguous range of targets and a spike queue of its own holding the
+ * synapses onto them, pushed and delivered in the serial order, so every
+ * target receives its additions in the same order as with one thread.  Only
+ * before the first step; STDP pathways stay serial.
+ */
+int ref_net_set_threads(ref_net *net, int nthreads)
+{
+    if (!net || nthreads < 1 || nthreads > REF_MAX_THREADS || net->step != 0) return -1;
+#ifndef _OPENMP
+    if (nthreads > 1) return -2;
+#else
+    omp_set_dynamic(0); /* exactly nthreads per region: the target ranges own queue state across steps */
+#endif
+    net->nthr = nthreads;
+    return 0;
 }
 
 static int model_nvars(int model)
@@ -614,14 +649,23 @@ int ref_mon_get_rate(ref_net *net, int mid, double *out)
     return 0;
 }
 
-static void queue_push(ref_path *p, int slot, int64_t syn)
+static void queue_push_in(int64_t **q, int64_t *qlen, int64_t *qcap, int slot, int64_t syn)
 {
-    if (p->qlen[slot] == p->qcap[slot]) {
-        int64_t c = p->qcap[slot] ? 2 * p->qcap[slot] : 256;
-        p->q[slot] = (int64_t *)realloc(p->q[slot], sizeof(int64_t) * c);
-        p->qcap[slot] = c;
+    if (qlen[slot] == qcap[slot]) {
+        int64_t c = qcap[slot] ? 2 * qcap[slot] : 256;
+        q[slot] = (int64_t *)realloc(q[slot], sizeof(int64_t) * c);
+        qcap[slot] = c;
     }
-    p->q[slot][p->qlen[slot]++] = syn;
+    q[slot][qlen[slot]++] = syn;
+}
+
+static void queue_push(ref_path *p, int slot, int64_t syn) { queue_push_in(p->q, p->qlen, p->qcap, slot, syn); }
+
+/* contiguous share [*lo, *hi) of n items for thread t of T */
+static void ref_share(int64_t n, int t, int T, int64_t *lo, int64_t *hi)
+{
+    *lo = n * t / T;
+    *hi = n * (t + 1) / T;
 }
 
 /* the per-precision body: update groups, push/deliver, reset */
@@ -650,6 +694,7 @@ int ref_net_run(ref_net *net, int64_t nsteps)
     for (int64_t s = 0; s < nsteps; ++s) {
         if (net->prec == HS_F64) ref_step_f64(net);
         else ref_step_f32(net);
+        if (net->thr_err) return -5;
     }
     if (net->sem == 1) {
         if (net->prec == HS_F64) ref_flush_f64(net);
@@ -700,6 +745,9 @@ void ref_net_destroy(ref_net *net)
         free(pw->pre); free(pw->post); free(pw->delay); free(pw->pre_start); free(pw->by_pre); free(pw->w);
         for (int q = 0; q < pw->Q; ++q) free(pw->q[q]);
         free(pw->q); free(pw->qlen); free(pw->qcap);
+        if (pw->tq)
+            for (int64_t q = 0; q < (int64_t)net->nthr * pw->Q; ++q) free(pw->tq[q]);
+        free(pw->tq); free(pw->tqlen); free(pw->tqcap);
         free(pw->apre); free(pw->apost); free(pw->lastup); free(pw->post_start); free(pw->by_post);
     }
     for (int m = 0; m < net->nm; ++m) {

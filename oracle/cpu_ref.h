@@ -35,6 +35,9 @@ int ref_stateupdate(int model, int precision, int n, double **vars, const int32_
 
 typedef struct ref_net ref_net;
 ref_net *ref_net_create(int precision, int semantics, double dt, uint64_t seed);
+/* the step loop over nthreads OpenMP threads, bit-identical to the serial one;
+   before the first step (-2: built without OpenMP) */
+int ref_net_set_threads(ref_net *net, int nthreads);
 int ref_net_add_group(ref_net *net, int model, int n, const double *params, int nparams);
 int ref_net_set_state(ref_net *net, int gid, int var, const double *values);
 int ref_net_get_state(ref_net *net, int gid, int var, double *out);

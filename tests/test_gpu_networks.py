@@ -303,3 +303,24 @@ def test_cuba_singular_time_constants_refused(hs):
     with pytest.raises(HsError):
         net.add_group(hs.HS_MODEL_LIF_CUBA, 10, nw.cuba_params(taum=10e-3, taui=10e-3))
     net.close()
+
+
+@pytest.mark.parametrize("engine,options", [("plain", None), ("fused", None), ("fused", {9: 1}), ("fused", {2: 0})])
+def test_prepare_then_run_same_bits(hs, engine, options):
+    """hs_net_prepare instantiates the graphs a run launches without advancing
+    the network: prepare + run gives the bits of run alone, and the step count
+    is untouched by prepare."""
+    spec = nw.brunel(N=3000, K=100, seed=9, monitors=True)
+    out = []
+    for prep in (False, True):
+        net, smon, _ = nw.build_hs(spec, 32, engine, options=options)
+        net.run(300)
+        if prep:
+            net.prepare(517)
+            assert net.stats()["steps"] == 300
+        net.run(517)
+        out.append((*net.spikes(smon[0]), net.get_state(0, 0), net.stats()["events"]))
+        net.close()
+    (i0, t0, v0, e0), (i1, t1, v1, e1) = out
+    assert len(i0) > 100
+    assert np.array_equal(i0, i1) and np.array_equal(t0, t1) and np.array_equal(v0, v1) and e0 == e1

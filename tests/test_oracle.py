@@ -401,3 +401,31 @@ def test_cuba_singular_time_constants_refused():
     bad = dataclasses.replace(spec, groups=[dataclasses.replace(g, params=g.params[:5] + [g.params[3]] + g.params[6:])])
     with pytest.raises(Exception):
         oracle.build_ref(bad, 64, oracle.SEQ)
+
+
+@pytest.mark.parametrize("name,prec,sem,steps", [("cuba", 64, 0, 800), ("cuba", 32, 1, 800),
+                                                 ("cobahh", 64, 0, 300), ("brunel", 32, 1, 400)])
+def test_threaded_oracle_equals_serial(oracle_lib, name, prec, sem, steps):
+    """The OpenMP step loop (ref_net_set_threads, the CPU baseline's form) gives
+    the serial loop's results bit for bit: spikes, state and events."""
+    from brian2cuda_amd import networks as nw
+    spec = {"cuba": lambda: nw.cuba(N=3000, seed=3), "cobahh": lambda: nw.cobahh(N=3000, seed=3),
+            "brunel": lambda: nw.brunel(N=12000, K=400, seed=3, monitors=True)}[name]()
+    out = []
+    for th in (1, 5):
+        ref, sm, _ = oracle_lib.build_ref(spec, prec, sem, threads=th)
+        ref.run(steps)
+        out.append((*ref.spikes(sm[0]), ref.get_state(0, 0), ref.events))
+        ref.close()
+    (i1, t1, v1, e1), (i5, t5, v5, e5) = out
+    assert len(i1) > 100
+    assert np.array_equal(i1, i5) and np.array_equal(t1, t5)
+    assert np.array_equal(v1, v5) and e1 == e5
+
+
+def test_threaded_oracle_only_before_the_first_step(oracle_lib):
+    from brian2cuda_amd import networks as nw
+    ref, _, _ = oracle_lib.build_ref(nw.cuba(N=200, seed=1), 64, 0)
+    ref.run(3)
+    with pytest.raises(ValueError):
+        ref.set_threads(4)
