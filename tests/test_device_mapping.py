@@ -1,0 +1,177 @@
+"""CPU checks of the hip_standalone device's mapping from object descriptions
+(equation text, threshold/reset, namespace, connect, on_pre, monitors) onto the
+engine's network spec: the reference examples' models are recognised with their
+parameters, unsupported code is refused with NotImplementedError, and the
+module imports with and without Brian2.  (No GPU.)"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from brian2cuda_amd import device as dv
+from brian2cuda_amd import networks as nw
+from brian2cuda_amd.runtime import (HS_MODEL_HH_COBA, HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_POISSON,
+                                    HS_MODEL_SPIKEGEN, HS_W_SCALAR, HS_W_SYNAPSE)
+
+mV, ms, nS = 1e-3, 1e-3, 1e-9
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+CUBA_EQS = """
+dv/dt  = (ge+gi-(v-El))/taum : volt (unless refractory)
+dge/dt = -ge/taue : volt
+dgi/dt = -gi/taui : volt
+"""
+CUBA_NS = {"taum": 20 * ms, "taue": 5 * ms, "taui": 10 * ms, "Vt": -50 * mV, "Vr": -60 * mV, "El": -49 * mV,
+           "we": 1.62 * mV, "wi": -9 * mV, "Ne": 3200}
+
+COBAHH_EQS = """
+dv/dt = (gl*(El-v)+ge*(Ee-v)+gi*(Ei-v)-
+         g_na*(m*m*m)*h*(v-ENa)-
+         g_kd*(n*n*n*n)*(v-EK))/Cm : volt
+dm/dt = alpha_m*(1-m)-beta_m*m : 1
+dn/dt = alpha_n*(1-n)-beta_n*n : 1
+dh/dt = alpha_h*(1-h)-beta_h*h : 1
+dge/dt = -ge*(1./taue) : siemens
+dgi/dt = -gi*(1./taui) : siemens
+alpha_m = 0.32*(mV**-1)*(13.*mV-v+VT)/
+         (exp((13.*mV-v+VT)/(4.*mV))-1.)/ms : Hz
+beta_m = 0.28*(mV**-1.)*(v-VT-40.*mV)/
+        (exp((v-VT-40.*mV)/(5.*mV))-1.)/ms : Hz
+alpha_h = 0.128*exp((17.*mV-v+VT)/(18.*mV))/ms : Hz
+beta_h = 4./(1.+exp((40.*mV-v+VT)/(5.*mV)))/ms : Hz
+alpha_n = 0.032*(mV**-1.)*(15.*mV-v+VT)/
+         (exp((15.*mV-v+VT)/(5.*mV))-1.)/ms : Hz
+beta_n = .5*exp((10.*mV-v+VT)/(40.*mV))/ms : Hz
+"""
+
+
+def _cobahh_ns():
+    Cm, gl, El, EK, ENa, gNa, gK, VT, Ee, Ei, taue, taui, vth, refr = nw.cobahh_params()
+    return {"Cm": Cm, "gl": gl, "El": El, "EK": EK, "ENa": ENa, "g_na": gNa, "g_kd": gK, "VT": VT, "Ee": Ee,
+            "Ei": Ei, "taue": taue, "taui": taui}
+
+
+def test_cuba_example_maps_to_the_c1_model():
+    """examples/cuba.py:118-143: the model, its parameters, connect('i<Ne', p=0.02)
+    as a pre range, on_pre 'ge += we' as a scalar weight, monitors."""
+    v0 = np.linspace(-60 * mV, -50 * mV, 4000)
+    g = dv.GroupDesc("neurongroup", 4000, equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr",
+                     refractory=5 * ms, method="exact", namespace=CUBA_NS, init={"v": v0, "ge": 0.0, "gi": 0.0})
+    ce = dv.SynapsesDesc("synapses", "neurongroup", "neurongroup", "ge += we", namespace=CUBA_NS, p=0.02,
+                         condition="i<Ne", delay=0.1 * ms, seed=7)
+    ci = dv.SynapsesDesc("synapses_1", "neurongroup", "neurongroup", "gi += wi", namespace=CUBA_NS, p=0.02,
+                         condition="i>=Ne", delay=0.1 * ms, seed=7)
+    mons = [dv.MonitorDesc("spikemonitor", "spike", "neurongroup"),
+            dv.MonitorDesc("statemonitor", "state", "neurongroup", variable="v", record=range(100))]
+    spec, names = dv.network_spec([g], [ce, ci], mons, dt=0.1 * ms, seed=7)
+    gs = spec.groups[0]
+    assert gs.model == HS_MODEL_LIF_CUBA and gs.params == nw.cuba_params()
+    np.testing.assert_array_equal(gs.init[0], v0)
+    assert gs.init[1] == 0.0 and gs.init[2] == 0.0
+    pe, pi = spec.paths
+    assert (pe.var, pi.var) == (1, 2)
+    assert pe.random.pre == (0, 3200) and pe.random.post == (0, 4000) and pe.wscalar == 1.62 * mV
+    assert pi.random.pre == (3200, 4000) and pi.wscalar == -9 * mV
+    assert pe.random.delay == (0.1 * ms, 0.1 * ms) and pe.random.wmode == HS_W_SCALAR
+    assert spec.spike_monitors == [0] and spec.state_monitors == [(0, 0, 0, 100)]
+    assert names["paths"] == ["synapses", "synapses_1"] and names["state_monitors"] == ["statemonitor"]
+
+
+def test_cuba_without_unless_refractory_sets_free():
+    eqs = CUBA_EQS.replace("(unless refractory)", "")
+    g = dv.GroupDesc("g", 10, equations=eqs, threshold="v > Vt", reset="v=Vr", refractory=2 * ms,
+                     namespace=CUBA_NS)
+    spec, _ = dv.group_spec(g)
+    assert spec.params == nw.cuba_params(refractory=2 * ms) + [1.0]
+
+
+def test_brunel_hakim_example_maps_to_the_c3_model():
+    """examples/brunelhakim.py:140-183 (V += -J, connect(p), delay as an array)."""
+    eqs = "\ndV/dt = (-V+muext + sigmaext * sqrt(tau) * xi)/tau : volt\n"
+    ns = {"Vr": 10 * mV, "theta": 20 * mV, "tau": 20 * ms, "J": 0.1 * mV, "muext": 25 * mV, "sigmaext": 1 * mV}
+    g = dv.GroupDesc("neurongroup", 500, equations=eqs, threshold="V>theta", reset="V=Vr", refractory=2 * ms,
+                     method="euler", namespace=ns, init={"V": 10 * mV})
+    rng = np.random.default_rng(1)
+    i, j = rng.integers(0, 500, 3000), rng.integers(0, 500, 3000)
+    d = rng.uniform(0.5 * ms, 5 * ms, 3000)
+    s = dv.SynapsesDesc("synapses", "neurongroup", "neurongroup", "V += -J", namespace=ns, i=i, j=j, delay=d)
+    spec, _ = dv.network_spec([g], [s], [], dt=0.1 * ms)
+    gs, ps = spec.groups[0], spec.paths[0]
+    assert gs.model == HS_MODEL_LIF_NOISE and gs.params == nw.brunel_params()
+    assert ps.wmode == HS_W_SCALAR and ps.wscalar == -0.1 * mV
+    np.testing.assert_array_equal(ps.pre, i)
+    np.testing.assert_array_equal(ps.delay_steps, nw.quantise_delay(d, 0.1 * ms))
+
+
+def test_cobahh_example_maps_to_the_c2_model():
+    """examples/cobahh.py:160-190 with subgroup sources Pe = P[:Ne], Pi = P[Ne:]
+    and per-synapse constant weights 'we : siemens (constant)'."""
+    g = dv.GroupDesc("neurongroup", 400, equations=COBAHH_EQS, threshold="v>-20*mV", refractory=3 * ms,
+                     method="exponential_euler", namespace=_cobahh_ns())
+    rng = np.random.default_rng(2)
+    i, j = rng.integers(0, 320, 2000), rng.integers(0, 400, 2000)
+    ce = dv.SynapsesDesc("synapses", "neurongroup", "neurongroup", "ge+=we", source_range=(0, 320), i=i, j=j,
+                         values={"we": 6 * nS})
+    ci = dv.SynapsesDesc("synapses_1", "neurongroup", "neurongroup", "gi+=wi", source_range=(320, 400),
+                         i=i[:500] % 80, j=j[:500], values={"wi": np.full(500, 67 * nS)})
+    spec, _ = dv.network_spec([g], [ce, ci], [], dt=0.1 * ms)
+    assert spec.groups[0].model == HS_MODEL_HH_COBA and spec.groups[0].params == nw.cobahh_params()
+    pe, pi = spec.paths
+    assert (pe.var, pi.var) == (4, 5) and pe.wmode == HS_W_SYNAPSE
+    np.testing.assert_array_equal(pe.weights, np.full(2000, 6 * nS))
+    np.testing.assert_array_equal(pi.pre, i[:500] % 80 + 320)
+    assert pe.scalar_delay == 0
+
+
+def test_poisson_and_spike_generator_groups():
+    p = dv.GroupDesc("poissongroup", 100, kind="poisson", rates=15.0)
+    sg = dv.GroupDesc("spikegeneratorgroup", 5, kind="spikegenerator", indices=[0, 4, 2],
+                      times=[0.0, 0.5 * ms, 1.0 * ms])
+    spec, _ = dv.network_spec([p, sg], [], [], dt=0.1 * ms)
+    assert spec.groups[0].model == HS_MODEL_POISSON and spec.groups[0].params == [15.0]
+    assert spec.groups[1].model == HS_MODEL_SPIKEGEN
+    np.testing.assert_array_equal(spec.groups[1].spikes[1], [0, 5, 10])
+
+
+@pytest.mark.parametrize("change,msg", [
+    (dict(equations="dv/dt = -v/tau : volt"), "not one of the built-in models"),
+    (dict(threshold="v > Vt and v < 0*mV"), "threshold"),
+    (dict(reset="v = Vr; w += 1"), "reset"),
+    (dict(method="euler"), "integrated 'exact'"),
+    (dict(namespace={k: v for k, v in CUBA_NS.items() if k != "taum"}), "taum"),
+])
+def test_unsupported_groups_are_refused(change, msg):
+    kw = dict(equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr", refractory=5 * ms, namespace=CUBA_NS)
+    kw.update(change)
+    with pytest.raises(NotImplementedError, match=msg):
+        dv.group_spec(dv.GroupDesc("g", 10, **kw))
+
+
+@pytest.mark.parametrize("on_pre,kw,msg", [
+    ("w += 1", {}, "not 'var \\+= w'"),
+    ("ge += we", dict(p=0.1, condition="i % 2 == 0"), "condition"),
+    ("ge += w", dict(p=0.1, values={"w": 1.0}), "'lo \\+ \\(hi-lo\\)\\*rand\\(\\)'"),
+])
+def test_unsupported_synapses_are_refused(on_pre, kw, msg):
+    g = dv.GroupDesc("g", 10, equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr", namespace=CUBA_NS)
+    s = dv.SynapsesDesc("s", "g", "g", on_pre, namespace=CUBA_NS, **kw)
+    with pytest.raises(NotImplementedError, match=msg):
+        dv.network_spec([g], [s], [], dt=0.1 * ms)
+
+
+def test_non_contiguous_state_monitor_is_refused():
+    g = dv.GroupDesc("g", 10, equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr", namespace=CUBA_NS)
+    m = dv.MonitorDesc("statemonitor", "state", "g", variable="v", record=[0, 2, 4])
+    with pytest.raises(NotImplementedError, match="contiguous"):
+        dv.network_spec([g], [], [m], dt=0.1 * ms)
+
+
+def test_device_module_imports_with_and_without_brian2():
+    for code in ["import sys; sys.modules['brian2'] = None\n",
+                 "import sys, types; sys.modules['brian2'] = types.ModuleType('brian2')\n"]:
+        r = subprocess.run([sys.executable, "-c", code + "import brian2cuda_amd.device as d\n"
+                            "print(hasattr(d, 'HipStandaloneDevice'))"], cwd=ROOT, capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0 and r.stdout.strip() == "False", r.stderr
