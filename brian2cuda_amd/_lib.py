@@ -59,6 +59,8 @@ SIGNATURES = {
     "hs_zero": (C.c_int, [P, C.c_size_t]),
     "hs_sync": (C.c_int, []),
     "hs_threshold_compact": (C.c_int, [C.c_int, P, C.c_int, C.c_double, P, C.c_int64, C.c_int, C.c_int, P, P]),
+    "hs_threshold_compact_noalloc": (C.c_int, [C.c_int, P, C.c_int, C.c_double, P, C.c_int64, C.c_int, C.c_int, P,
+                                                P]),
     "hs_randn": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int64, C.c_int, C.c_int, P]),
     "hs_rand": (C.c_int, [C.c_uint64, C.c_uint32, C.c_int64, C.c_int, P]),
     "hs_stateupdate": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p), P, DP, C.c_int, C.c_double,

@@ -66,6 +66,15 @@ struct HsRuntime {
 };
 static HsRuntime g_rt;
 
+/* Device memory of the allocation-free entry points (hs_threshold_compact_noalloc):
+   one int32 spike count per 256-neuron tile, HS_NOALLOC_MAX_N neurons at most
+   (512 KiB).  Each GPU holds its own copy until the process ends.  In host code
+   the name is not a device address: hs_init resolves the address once per device
+   (g_scratch_dev) and host code uses only that. */
+static __device__ __align__(256) int32_t scratch[HS_NOALLOC_MAX_N / 256];
+#define HS_MAX_DEVICES 64
+static int32_t *g_scratch_dev[HS_MAX_DEVICES];
+
 static int hs_need_init()
 {
     if (!g_rt.stream) return hs_fail(HS_ERR_STATE, "hs_init() has not been called");
@@ -93,7 +102,14 @@ extern "C" int hs_init(int device)
     int n = 0;
     HS_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return hs_fail(HS_ERR_ARG, "device %d not present (%d devices)", device, n);
+    if (device >= HS_MAX_DEVICES) return hs_fail(HS_ERR_UNSUPPORTED, "device %d: at most %d devices", device, HS_MAX_DEVICES);
     HS_HIP(hipSetDevice(device));
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        HS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return hs_fail(HS_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_dev[device] = (int32_t *)p;
+    }
     hipDeviceProp_t prop;
     HS_HIP(hipGetDeviceProperties(&prop, device));
     g_rt.n_cu = prop.multiProcessorCount;
@@ -219,6 +235,26 @@ extern "C" int hs_threshold_compact(int precision, const void *v, int n, double 
     if (n == 0) return hs_zero(count, sizeof(int32_t));
     if (sorted) HS_TRY(hs_scratch((size_t)(n + 255) / 256 + 1));
     HS_HIP(hs_launch_threshold(precision, v, n, vth, last, step, ref_steps, sorted, spikes, count, g_rt.scratch,
+                               g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    return HS_OK;
+}
+
+extern "C" int hs_threshold_compact_noalloc(int precision, const void *v, int n, double vth, int32_t *last,
+                                            int64_t step, int ref_steps, int sorted, int32_t *spikes, int32_t *count)
+{
+    HS_TRY(hs_need_init());
+    if (n < 0 || (precision != HS_F64 && precision != HS_F32) || !v || !last || !spikes || !count)
+        return hs_fail(HS_ERR_ARG, "hs_threshold_compact_noalloc: bad arguments");
+    if (n > HS_NOALLOC_MAX_N)
+        return hs_fail(HS_ERR_ARG, "hs_threshold_compact_noalloc: n = %d exceeds the maximum of %d neurons", n,
+                       HS_NOALLOC_MAX_N);
+    int32_t *tiles = g_scratch_dev[g_rt.device];
+    if (!tiles) return hs_fail(HS_ERR_STATE, "hs_threshold_compact_noalloc: scratch address not resolved");
+    if (n == 0) return hs_zero(count, sizeof(int32_t));
+    /* the tile counts are written in full by the count pass before the scan reads
+       them: nothing left by an earlier call is read */
+    HS_HIP(hs_launch_threshold(precision, v, n, vth, last, step, ref_steps, sorted, spikes, count, tiles,
                                g_rt.stream));
     HS_HIP(hipStreamSynchronize(g_rt.stream));
     return HS_OK;

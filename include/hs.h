@@ -74,6 +74,20 @@ int hs_threshold_compact(int precision, const void *v, int n, double vth, int32_
                          int64_t step, int ref_steps, int sorted, int32_t *spikes, int32_t *count);
 
 /*
+ * hs_threshold_compact without any allocation: the same kernels and results,
+ * with the sorted form's per-tile counts in a static device array of the
+ * library (512 KiB per GPU, held until the process ends) instead of memory
+ * the library allocates.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  n above HS_NOALLOC_MAX_N
+ * returns HS_ERR_ARG.  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (from several threads) race.
+ */
+#define HS_NOALLOC_MAX_N (1 << 25) /* 33,554,432 neurons */
+int hs_threshold_compact_noalloc(int precision, const void *v, int n, double vth, int32_t *last,
+                                 int64_t step, int ref_steps, int sorted, int32_t *spikes, int32_t *count);
+
+/*
  * n standard normals out[i] = N(seed, stream, i, step) (Philox4x32-10 +
  * Box-Muller, the deterministic math of the numerical contract); out is double
  * or float by precision (float: the FP32 form's draw, 24-bit uniforms and float math).
