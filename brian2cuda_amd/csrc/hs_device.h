@@ -14,6 +14,53 @@
 
 __device__ __forceinline__ int hs_lane() { return threadIdx.x & (HS_WAVE - 1); }
 
+/*
+ * Accesses to HBM arrays through pointers held in the DGroup / DPath
+ * descriptors.  Such pointers are generic to the compiler, which then emits
+ * FLAT instructions: those count against the LDS/scalar counter as well, so
+ * every wait for a descriptor field also waits for the data loads in flight
+ * and the state loads of a neuron serialise.  Casting to the global address
+ * space gives global_load / global_store / global_atomic instead.
+ */
+#define HS_GLOBAL __attribute__((address_space(1)))
+
+template <typename T>
+__device__ __forceinline__ T hs_ld(const T *p)
+{
+    return *(const HS_GLOBAL T *)p;
+}
+
+template <typename T>
+__device__ __forceinline__ void hs_st(T *p, T v)
+{
+    *(HS_GLOBAL T *)p = v;
+}
+
+/* relaxed device-scope atomic add whose result is not used (hardware FP
+   atomics with -munsafe-fp-atomics) */
+template <typename T>
+__device__ __forceinline__ void hs_atomic_add(T *p, T v)
+{
+    (void)__hip_atomic_fetch_add((HS_GLOBAL T *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* relaxed device-scope atomic add returning the old value */
+template <typename T>
+__device__ __forceinline__ T hs_atomic_fetch_add(T *p, T v)
+{
+    return __hip_atomic_fetch_add((HS_GLOBAL T *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* x unchanged, but opaque to the optimiser (no instruction is emitted): a
+   choice among values stays a choice among registers instead of becoming a
+   load from a computed address, which would move the array to LDS */
+template <typename T>
+__device__ __forceinline__ T hs_opaque(T x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 __device__ __forceinline__ unsigned long long hs_lanemask_lt() { return (1ull << hs_lane()) - 1ull; }
 
 /* exclusive prefix sum across the 64 lanes; *total gets the wave sum */
@@ -80,10 +127,10 @@ __device__ __forceinline__ bool hs_spikegen_fires(const DGroup<R> &g, long long 
 {
     const long long s = g.gen_period > 0 ? t % g.gen_period : t;
     if (s < 0 || s >= g.gen_steps) return false;
-    int lo = g.gen_off[s], hi = g.gen_off[s + 1];
+    int lo = hs_ld(g.gen_off + s), hi = hs_ld(g.gen_off + s + 1);
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        const int v = g.gen_idx[mid];
+        const int v = hs_ld(g.gen_idx + mid);
         if (v == i) return true;
         if (v < i) lo = mid + 1;
         else hi = mid;
@@ -209,7 +256,7 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
             if (q < npairs) {
                 const int pq = q / ns, sq = q - pq * ns;
                 const int j = pq / dn, di = di0 + (pq - j * dn);
-                cnt = p.hist_cnt[((long long)hs_wrap(tq + j - (p.dmin + di), p.Q) * ns + sq) * p.cstride];
+                cnt = hs_ld(p.hist_cnt + ((long long)hs_wrap(tq + j - (p.dmin + di), p.Q) * ns + sq) * p.cstride);
             }
             int tot;
             const int ex = hs_wave_excl_scan(cnt, &tot);
@@ -248,10 +295,10 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
                 jw = pq / dn;
                 const int di = di0 + (pq - jw * dn);
                 const long long slot = hs_wrap(tq + jw - (p.dmin + di), p.Q);
-                pre = p.hist[slot * rowlen + (long long)sq * p.subcap + kk];
+                pre = hs_ld(p.hist + slot * rowlen + (long long)sq * p.subcap + kk);
                 const long long b = (long long)pre * D + di;
-                s0 = p.bstart[b];
-                len = p.bstart[b + 1] - s0;
+                s0 = hs_ld(p.bstart + b);
+                len = hs_ld(p.bstart + b + 1) - s0;
             }
         }
         int tot;
@@ -271,13 +318,13 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
             const int jb = __shfl(jw, lo, HS_WAVE);
             if (j < tot) {
                 const int syn = sb + (j - ob);
-                const int tg = p.post[syn];
+                const int tg = hs_ld(p.post + syn);
                 R wv;
                 if (p.wmode == HS_W_SCALAR) wv = p.wscalar;
-                else if (p.wmode == HS_W_SYNAPSE) wv = p.w[syn];
-                else wv = p.w[pb];
+                else if (p.wmode == HS_W_SYNAPSE) wv = hs_ld(p.w + syn);
+                else wv = hs_ld(p.w + pb);
                 R *target = ACC ? p.acc + (long long)hs_wrap(ta + jb, p.nacc) * p.n_tgt : p.target;
-                unsafeAtomicAdd(&target[tg], wv);
+                hs_atomic_add(target + tg, wv);
             }
         }
         *ev_lane += (unsigned long long)tot;
@@ -288,13 +335,13 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
 __device__ __forceinline__ void hs_flush_events(unsigned long long ev, unsigned long long *events)
 {
     /* every lane of a wave holds the wave's count; lane 0 publishes it */
-    if (hs_lane() == 0 && ev) atomicAdd(events, ev);
+    if (hs_lane() == 0 && ev) hs_atomic_add(events, ev);
 }
 
 /* the same into the control block's spread counters, slot by wave rank */
 __device__ __forceinline__ void hs_flush_events(unsigned long long ev, HsCtl *ctl, long long rank)
 {
-    if (hs_lane() == 0 && ev) atomicAdd(&ctl->ev_slot[(int)(rank % HS_EV_SLOTS) * HS_EV_STRIDE], ev);
+    if (hs_lane() == 0 && ev) hs_atomic_add(&ctl->ev_slot[(int)(rank % HS_EV_SLOTS) * HS_EV_STRIDE], ev);
 }
 
 __device__ __forceinline__ unsigned long long hs_wave_sum(unsigned long long x)
@@ -332,7 +379,7 @@ __device__ __forceinline__ void hs_gather_tile(const DPath<R> &p, int tile, long
     if (ACC) {
         for (int i = threadIdx.x; i < w * HS_GTILE; i += blockDim.x) lacc[i] = (R)0;
     } else {
-        for (int i = threadIdx.x; i < HS_GTILE; i += blockDim.x) lacc[i] = i < ntl ? p.target[base + i] : (R)0;
+        for (int i = threadIdx.x; i < HS_GTILE; i += blockDim.x) lacc[i] = i < ntl ? hs_ld(p.target + base + i) : (R)0;
     }
     const int dmax = p.dmin + p.D - 1;
     const int ns = p.nsub;
@@ -345,7 +392,7 @@ __device__ __forceinline__ void hs_gather_tile(const DPath<R> &p, int tile, long
         for (int c = 0; c < nU; c += HS_WAVE) {
             const int q = c + hs_lane();
             const int cnt =
-                q < nU ? p.hist_cnt[((long long)hs_wrap(tq - dmax + q / ns, p.Q) * ns + q % ns) * p.cstride] : 0;
+                q < nU ? hs_ld(p.hist_cnt + ((long long)hs_wrap(tq - dmax + q / ns, p.Q) * ns + q % ns) * p.cstride) : 0;
             int tot;
             const int ex = hs_wave_excl_scan(cnt, &tot);
             if (q < nU) L->pref[q] = carry + ex;
@@ -365,18 +412,18 @@ __device__ __forceinline__ void hs_gather_tile(const DPath<R> &p, int tile, long
         }
         const int lu = lo / ns, ls = lo - lu * ns;
         const long long u = T - dmax + lu;
-        const int pre = p.hist[(long long)hs_wrap(tq - dmax + lu, p.Q) * rowlen + (long long)ls * p.subcap +
-                               (c - L->pref[lo])];
+        const int pre = hs_ld(p.hist + (long long)hs_wrap(tq - dmax + lu, p.Q) * rowlen + (long long)ls * p.subcap +
+                              (c - L->pref[lo]));
         const long long gi = (long long)tile * p.n_src + pre; /* tile-major: a block stays in its tile's rows */
-        const int a = p.goff[gi], b = p.goff[gi + 1];
+        const int a = hs_ld(p.goff + gi), b = hs_ld(p.goff + gi + 1);
         for (int s = a; s < b; ++s) {
-            const uint32_t e = p.gsyn[s];
+            const uint32_t e = hs_ld(p.gsyn + s);
             const long long arr = u + (long long)(p.dmin + (int)(e >> 16)) - T;
             if (arr >= 0 && arr < w) {
                 R wv;
                 if (p.wmode == HS_W_SCALAR) wv = p.wscalar;
-                else if (p.wmode == HS_W_SYNAPSE) wv = p.w[s];
-                else wv = p.w[pre];
+                else if (p.wmode == HS_W_SYNAPSE) wv = hs_ld(p.w + s);
+                else wv = hs_ld(p.w + pre);
                 atomicAdd(&lacc[(ACC ? (int)arr * HS_GTILE : 0) + (int)(e & 0xffffu)], wv);
                 ++ev;
             }
@@ -389,11 +436,11 @@ __device__ __forceinline__ void hs_gather_tile(const DPath<R> &p, int tile, long
             const int j = i / HS_GTILE, l = i - j * HS_GTILE;
             if (l < ntl) {
                 R *a = p.acc + (long long)hs_wrap(ta + j, p.nacc) * p.n_tgt + base + l;
-                *a = *a + lacc[i];
+                hs_st(a, hs_ld(a) + lacc[i]);
             }
         }
     } else {
-        for (int i = threadIdx.x; i < ntl; i += blockDim.x) p.target[base + i] = lacc[i];
+        for (int i = threadIdx.x; i < ntl; i += blockDim.x) hs_st(p.target + base + i, lacc[i]);
     }
     __syncthreads();
 }

@@ -69,8 +69,8 @@ __device__ __forceinline__ void hs_push_near(const HsPush<R> &ps, int gid, int i
         if (cnt) {
             const long long base = (long long)i * p.D;
 #pragma unroll
-            for (int q = 0; q <= HS_MAX_WINDOW; ++q) o[q] = q <= p.nnear ? p.bstart[base + q] : 0;
-            s_end = p.bstart[base + p.D];
+            for (int q = 0; q <= HS_MAX_WINDOW; ++q) o[q] = q <= p.nnear ? hs_ld(p.bstart + base + q) : 0;
+            s_end = hs_ld(p.bstart + base + p.D);
         } else {
 #pragma unroll
             for (int q = 0; q <= HS_MAX_WINDOW; ++q) o[q] = 0;
@@ -116,19 +116,19 @@ __device__ __forceinline__ void hs_push_near(const HsPush<R> &ps, int gid, int i
 #pragma unroll
                     for (int q = 1; q < HS_MAX_WINDOW; ++q) di += (q < p.nnear && ob_q[q] <= s) ? 1 : 0;
                     if (p.wmode == HS_W_SCALAR) wv[u] = p.wscalar;
-                    else if (p.wmode == HS_W_SYNAPSE) wv[u] = p.w[s];
-                    else wv[u] = p.w[pb];
+                    else if (p.wmode == HS_W_SYNAPSE) wv[u] = hs_ld(p.w + s);
+                    else wv[u] = hs_ld(p.w + pb);
                     slot[u] = hs_wrap(ta + j + p.dmin + di, p.nacc);
-                    tg[u] = p.post[s];
+                    tg[u] = hs_ld(p.post + s);
                 }
             }
 #pragma unroll
             for (int u = 0; u < UNR; ++u)
-                if (tg[u] >= 0) unsafeAtomicAdd(&p.acc[(long long)slot[u] * p.n_tgt + tg[u]], wv[u]);
+                if (tg[u] >= 0) hs_atomic_add(p.acc + (long long)slot[u] * p.n_tgt + tg[u], wv[u]);
         }
     }
     emitted = hs_wave_sum(emitted);
-    if (hs_lane() == 0 && emitted) atomicAdd(ps.emit, emitted);
+    if (hs_lane() == 0 && emitted) hs_atomic_add(ps.emit, emitted);
 }
 
 /* one step of one neuron inside the update window; returns the spike */
@@ -152,11 +152,11 @@ __device__ __forceinline__ bool hs_step_one(const DGroup<R> &g, R *s, const R *k
             if (i >= g.st_start[m] && i < g.st_stop[m]) {
                 const long long row = t - g.st_t0[m];
                 if (row >= 0 && row < g.st_rows[m]) {
-                    R val = s[0];
+                    const int sv = g.st_var[m];
+                    R val = hs_opaque(s[0]);
 #pragma unroll
-                    for (int v = 1; v < NV; ++v)
-                        if (v == g.st_var[m]) val = s[v];
-                    g.st_buf[m][row * (long long)(g.st_stop[m] - g.st_start[m]) + (i - g.st_start[m])] = val;
+                    for (int v = 1; v < NV; ++v) val = sv == v ? hs_opaque(s[v]) : val;
+                    hs_st(g.st_buf[m] + row * (long long)(g.st_stop[m] - g.st_start[m]) + (i - g.st_start[m]), val);
                 }
             }
         }
@@ -179,15 +179,16 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
     /* ring slots of the steps after this launch start from empty lists */
     if (tile == 0)
         for (int q = threadIdx.x; q < wmax * g.nsub; q += blockDim.x)
-            g.hist_cnt[((long long)hs_wrap(tq + w + q / g.nsub, g.Q) * g.nsub + q % g.nsub) * HS_CNT_STRIDE] = 0;
+            hs_st(g.hist_cnt + ((long long)hs_wrap(tq + w + q / g.nsub, g.Q) * g.nsub + q % g.nsub) * HS_CNT_STRIDE,
+                  0);
     /* this tile's sub-list (a tile lies inside one 256-neuron chunk) */
     const int sub = (int)(((long long)tile * blockDim.x >> 8) % g.nsub);
     const long long rowlen = (long long)g.nsub * g.subcap;
     R s[NV];
     int32_t last = HS_NEVER_SPIKED;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) s[v] = valid ? g.var[v][i] : (R)0;
-    if (valid) last = g.last[i];
+    for (int v = 0; v < NV; ++v) s[v] = valid ? hs_ld(g.var[v] + i) : (R)0;
+    if (valid) last = hs_ld(g.last + i);
     R k[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) k[j] = g.k[j];
@@ -203,7 +204,7 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
                 const int x0 = (int)hs_mod(T, g.xi_slots);
 #pragma unroll
                 for (int j = 0; j < HS_MAX_WINDOW; ++j)
-                    xi[j] = j < w ? g.xi[(long long)hs_wrap(x0 + j, g.xi_slots) * g.n + i] : (R)0;
+                    xi[j] = j < w ? hs_ld(g.xi + (long long)hs_wrap(x0 + j, g.xi_slots) * g.n + i) : (R)0;
             } else {
 #pragma unroll
                 for (int j = 0; j < HS_MAX_WINDOW; ++j)
@@ -222,7 +223,7 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
             for (int j = 0; j < PF; ++j) {
                 const long long sj = (long long)hs_wrap(a0 + (j < w ? j : 0), g.nacc) * g.n + i;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) pf[j][v] = (j < w && g.acc[v]) ? g.acc[v][sj] : (R)0;
+                for (int v = 0; v < NV; ++v) pf[j][v] = (j < w && g.acc[v]) ? hs_ld(g.acc[v] + sj) : (R)0;
             }
         }
 #pragma unroll 1
@@ -240,13 +241,13 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
                 } else if (j + 1 < w) {
                     const long long sn = (long long)hs_wrap(a0 + j + 1, g.nacc) * g.n + i;
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) pf[0][v] = g.acc[v] ? g.acc[v][sn] : (R)0;
+                    for (int v = 0; v < NV; ++v) pf[0][v] = g.acc[v] ? hs_ld(g.acc[v] + sn) : (R)0;
                 }
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
                     if (g.acc[v]) {
                         s[v] = s[v] + cur[v];
-                        g.acc[v][slot] = (R)0;
+                        hs_st(g.acc[v] + slot, (R)0);
                     }
                 }
             }
@@ -261,8 +262,8 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
     }
     if (valid) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) g.var[v][i] = s[v];
-        g.last[i] = last;
+        for (int v = 0; v < NV; ++v) hs_st(g.var[v] + i, s[v]);
+        hs_st(g.last + i, last);
     }
     /* compaction of the w spike lists: counts per wave, then w reservations */
     const int wid = threadIdx.x / HS_WAVE;
@@ -280,14 +281,15 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
             L->wave_cnt[j][v] = tot;
             tot += c;
         }
-        L->base[j] = tot ? atomicAdd(g.hist_cnt + ((long long)hs_wrap(tq + j, g.Q) * g.nsub + sub) * HS_CNT_STRIDE,
-                                     tot)
+        L->base[j] = tot ? hs_atomic_fetch_add(g.hist_cnt + ((long long)hs_wrap(tq + j, g.Q) * g.nsub + sub) *
+                                                                HS_CNT_STRIDE,
+                                               (int32_t)tot)
                          : 0;
-        L->mbase[j] = (tot && g.smon_n) ? atomicAdd(g.smon_n, (unsigned long long)tot) : 0ull;
+        L->mbase[j] = (tot && g.smon_n) ? hs_atomic_fetch_add(g.smon_n, (unsigned long long)tot) : 0ull;
     }
     if (threadIdx.x < HS_WAVE) { /* one count per block and window */
         const unsigned long long s = hs_wave_sum((unsigned long long)tot);
-        if (threadIdx.x == 0 && s && spk_total) atomicAdd(spk_total, s);
+        if (threadIdx.x == 0 && s && spk_total) hs_atomic_add(spk_total, s);
     }
     __syncthreads();
     for (int j = 0; j < w; ++j) {
@@ -295,12 +297,13 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
         const unsigned long long m = __ballot(spk);
         if (spk) {
             const int pos = L->wave_cnt[j][wid] + __popcll(m & hs_lanemask_lt());
-            g.hist[(long long)hs_wrap(tq + j, g.Q) * rowlen + (long long)sub * g.subcap + L->base[j] + pos] = i;
+            hs_st(g.hist + (long long)hs_wrap(tq + j, g.Q) * rowlen + (long long)sub * g.subcap + L->base[j] + pos,
+                  (int32_t)i);
             if (g.smon_n) {
                 const unsigned long long r = L->mbase[j] + (unsigned long long)pos;
                 if ((long long)r < g.smon_cap) {
-                    g.smon_i[r] = i;
-                    g.smon_t[r] = (int32_t)(T + j);
+                    hs_st(g.smon_i + r, (int32_t)i);
+                    hs_st(g.smon_t + r, (int32_t)(T + j));
                 }
             }
         }
@@ -310,7 +313,7 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
         for (int j = 0; j < w; ++j) {
             const unsigned long long m = __ballot(in && ((bits >> j) & 1u));
             const long long row = T + j - g.rmon_t0;
-            if (hs_lane() == 0 && m && row >= 0 && row < g.rmon_rows) atomicAdd(&g.rmon_cnt[row], (int)__popcll(m));
+            if (hs_lane() == 0 && m && row >= 0 && row < g.rmon_rows) hs_atomic_add(g.rmon_cnt + row, (int32_t)__popcll(m));
         }
     }
     if (PUSH) hs_push_near<R>(push, gid, i, bits, T, w);
@@ -368,8 +371,8 @@ __device__ __forceinline__ void hs_noise_ahead(const DGroup<R> *groups, int ng, 
         for (long long q = tid; q < tot; q += nthr) {
             const int j = (int)(q / g.n);
             const int i = (int)(q - (long long)j * g.n);
-            g.xi[(long long)hs_wrap(s0 + j, g.xi_slots) * g.n + i] =
-                hs_noise<R, HS_MODEL_LIF_NOISE>(g.seed, g.stream, i, T0 + j);
+            hs_st(g.xi + (long long)hs_wrap(s0 + j, g.xi_slots) * g.n + i,
+                  hs_noise<R, HS_MODEL_LIF_NOISE>(g.seed, g.stream, i, T0 + j));
         }
     }
 }
@@ -377,7 +380,7 @@ __device__ __forceinline__ void hs_noise_ahead(const DGroup<R> *groups, int ng, 
 /* the draws of the first window of a chunk (the later windows' draws are made
    by the update launch of the window before) */
 template <typename R>
-__global__ __launch_bounds__(256) void hs_k_noise(const DGroup<R> *groups, int ng, HsCtl *ctl, int koff, int nsteps)
+__global__ __launch_bounds__(256) void hs_k_noise(const DGroup<R> *__restrict__ groups, int ng, HsCtl *ctl, int koff, int nsteps)
 {
     hs_noise_ahead<R>(groups, ng, ctl->base + koff, nsteps, blockIdx.x, gridDim.x);
 }
@@ -386,7 +389,7 @@ __global__ __launch_bounds__(256) void hs_k_noise(const DGroup<R> *groups, int n
    noise draws of the next window, steps T+w .. T+w+wmax-1 (no dependence on the
    state, so they run beside the update on otherwise idle waves) */
 template <typename R, bool FUSED, int MS>
-__global__ __launch_bounds__(256) void hs_k_update(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl,
+__global__ __launch_bounds__(256) void hs_k_update(const DGroup<R> *__restrict__ groups, int ng, int total_tiles, HsCtl *ctl,
                                                     int koff, int w, int wmax, int n_upd)
 {
     __shared__ HsCompactLdsW L;
@@ -412,8 +415,8 @@ __global__ __launch_bounds__(256) void hs_k_update(const DGroup<R> *groups, int 
  * effect arriving before T+w+W has been applied or accumulated.
  */
 template <typename R, int MS>
-__global__ __launch_bounds__(256) void hs_k_window(const DGroup<R> *groups, int ng, int total_tiles,
-                                                    const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w, int W,
+__global__ __launch_bounds__(256) void hs_k_window(const DGroup<R> *__restrict__ groups, int ng, int total_tiles,
+                                                    const DPath<R> *__restrict__ paths, int np, HsCtl *ctl, int koff, int w, int W,
                                                     int n_upd, int n_far)
 {
     __shared__ union {
@@ -444,7 +447,7 @@ __global__ __launch_bounds__(256) void hs_k_window(const DGroup<R> *groups, int 
    before E: every spike (i, u) with u >= E - dmax still owes its bundles of
    delay >= E - u.  One block; the result goes to ctl->pending. */
 template <typename R>
-__global__ __launch_bounds__(256) void hs_k_pending(const DPath<R> *paths, int np, HsCtl *ctl)
+__global__ __launch_bounds__(256) void hs_k_pending(const DPath<R> *__restrict__ paths, int np, HsCtl *ctl)
 {
     __shared__ unsigned long long part[4];
     const long long E = ctl->base;
@@ -480,7 +483,7 @@ __global__ __launch_bounds__(256) void hs_k_pending(const DPath<R> *paths, int n
    that follows (the delivery is bound by atomic throughput and leaves the
    vector ALUs idle) */
 template <typename R, bool ACC>
-__global__ __launch_bounds__(1024) void hs_k_deliver(const DPath<R> *paths, int np, const DGroup<R> *groups, int ng,
+__global__ __launch_bounds__(1024) void hs_k_deliver(const DPath<R> *__restrict__ paths, int np, const DGroup<R> *__restrict__ groups, int ng,
                                                      HsCtl *ctl, int koff, int w, int n_del)
 {
     __shared__ HsDeliverLds L;
@@ -499,7 +502,7 @@ __global__ __launch_bounds__(1024) void hs_k_deliver(const DPath<R> *paths, int 
 }
 
 template <typename R, bool ACC>
-__global__ __launch_bounds__(1024) void hs_k_gather(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int w)
+__global__ __launch_bounds__(1024) void hs_k_gather(const DPath<R> *__restrict__ paths, int np, HsCtl *ctl, int koff, int w)
 {
     __shared__ HsGatherLds L;
     __shared__ R lacc[HS_MAX_WINDOW * HS_GTILE];
@@ -515,8 +518,8 @@ __global__ __launch_bounds__(1024) void hs_k_gather(const DPath<R> *paths, int n
 }
 
 template <typename R, int MS>
-__global__ __launch_bounds__(256) void hs_k_fused(const DGroup<R> *groups, int ng, int total_tiles,
-                                                   const DPath<R> *paths, int np, HsCtl *ctl, int koff, int n_upd)
+__global__ __launch_bounds__(256) void hs_k_fused(const DGroup<R> *__restrict__ groups, int ng, int total_tiles,
+                                                   const DPath<R> *__restrict__ paths, int np, HsCtl *ctl, int koff, int n_upd)
 {
     __shared__ union {
         HsCompactLdsW c;
@@ -539,7 +542,7 @@ __global__ __launch_bounds__(256) void hs_k_fused(const DGroup<R> *groups, int n
 /* end of a run at step t_end = ctl->base: apply the last step's accumulated
    effects (fused form) and the resets of its spikers */
 template <typename R, bool FUSED>
-__global__ __launch_bounds__(256) void hs_k_finalize(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl)
+__global__ __launch_bounds__(256) void hs_k_finalize(const DGroup<R> *__restrict__ groups, int ng, int total_tiles, HsCtl *ctl)
 {
     const long long t = ctl->base;
     for (int tile = blockIdx.x; tile < total_tiles; tile += gridDim.x) {
@@ -569,7 +572,7 @@ __global__ void hs_k_mark_reset(HsCtl *ctl) { ctl->no_reset_before = ctl->base; 
 /* launchers                                                                 */
 /* ------------------------------------------------------------------------ */
 template <typename R, bool FUSED, int MS>
-static void hs_update_ms(const DGroup<R> *groups, int ng, int total_tiles, HsCtl *ctl, int koff, int w, int wmax,
+static void hs_update_ms(const DGroup<R> *__restrict__ groups, int ng, int total_tiles, HsCtl *ctl, int koff, int w, int wmax,
                          int block, int grid, int n_noise, hipStream_t s)
 {
     hipLaunchKernelGGL((hs_k_update<R, FUSED, MS>), dim3(grid + n_noise), dim3(block), 0, s, groups, ng, total_tiles,

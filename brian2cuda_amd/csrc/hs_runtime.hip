@@ -1422,7 +1422,10 @@ static int hs_build(hs_net *net)
     if (net->engine == HS_ENGINE_PLAIN) {
         net->mode = 0;
     } else if (net->p.empty() || dmin_all < 1) {
-        net->mode = 1; /* effects due in the same step: update, then deliver */
+        /* effects due in the same step: update, then deliver -- or, pipelined,
+           one launch per step whose update blocks push every synapse of delay
+           0 and 1 of their own spikes (all due after this launch's update) */
+        net->mode = (!net->p.empty() && net->opt[9] && net->n_gather == 0) ? 4 : 1;
     } else {
         /* the window cannot exceed the smallest delay, and its (step, delay) pairs
            must fit the delivery kernel's prefix table */
@@ -1433,8 +1436,8 @@ static int hs_build(hs_net *net)
         w = std::max(w, 1);
         net->window = w;
         /* the single-launch kernels carry scatter delivery only */
-        if (w == 1 && net->n_gather == 0) net->mode = 2;
-        else if (w >= 2 && net->n_gather == 0 && net->opt[9]) net->mode = 4;
+        if (net->n_gather == 0 && net->opt[9]) net->mode = 4;
+        else if (w == 1 && net->n_gather == 0) net->mode = 2;
         else net->mode = 3;
     }
     const int W = net->window;
@@ -1654,6 +1657,15 @@ static int hs_deliver_grid(const hs_net *net)
     return g_rt.n_cu * 4;
 }
 
+/* pipelined windows: does any pathway have synapses of delay >= 2W, which
+   the pull blocks deliver (the update blocks push the nearer ones)? */
+static bool hs_has_far(const hs_net *net)
+{
+    for (const HPath &P : net->p)
+        if (std::max(0, std::min(P.D, 2 * net->window - P.dmin)) < P.D) return true;
+    return false;
+}
+
 /* delivery blocks of the single-launch fused kernel */
 static int hs_fused_deliver_blocks(const hs_net *net)
 {
@@ -1779,8 +1791,8 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
             if (net->mode == 4) {
                 HS_TRY(hs_kt_launch(net, 4, [&]() -> int {
                     HS_HIP(hs_launch_window<R>(dg, ng, net->total_tiles, dp, np, net->ctl, (int)j, w, net->window, ms,
-                                               block, hs_update_grid(net), hs_deliver_grid(net), n_noise,
-                                               g_rt.stream));
+                                               block, hs_update_grid(net), hs_has_far(net) ? hs_deliver_grid(net) : 0,
+                                               n_noise, g_rt.stream));
                     return HS_OK;
                 }));
             } else {

@@ -324,3 +324,20 @@ def test_prepare_then_run_same_bits(hs, engine, options):
     (i0, t0, v0, e0), (i1, t1, v1, e1) = out
     assert len(i0) > 100
     assert np.array_equal(i0, i1) and np.array_equal(t0, t1) and np.array_equal(v0, v1) and e0 == e1
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("which", ["cuba", "cobahh", "cobahh_64k", "batch"])
+def test_pipelined_single_step_windows(hs, oracle_lib, prec, which):
+    """The pipelined form with windows of one step (option 9): one launch per
+    step whose update blocks push the synapses of delays 0 and 1 of their own
+    spikes -- for COBAHH (delay 0, otherwise two launches per step), CUBA and the
+    batch (delay 1) -- equals the oracle's ACC semantics bit for bit, over
+    chunked runs."""
+    spec = {"cuba": lambda: nw.cuba(N=4000, seed=11), "cobahh": lambda: nw.cobahh(N=4000, seed=3),
+            "cobahh_64k": lambda: nw.cobahh(N=64000, seed=1234),
+            "batch": lambda: nw.batch_cuba(n_nets=24, seed=8)}[which]()
+    net, ref = _compare(spec, prec, "fused", 1, None, oracle_lib, options={9: 1}, chunks=[1, 256, 343])
+    st = net.stats()
+    assert st["window"] == 1 and st["launches_per_step"] <= 1.0 + 1e-9
+    assert ref.events > 1000
