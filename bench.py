@@ -180,12 +180,12 @@ def kernel_times(spec, prec, engine, steps, warmup, options=None):
     return out, window, spikes / steps, events / steps
 
 
-def spike_trains(spec, prec, engine, steps, v_at=None):
+def spike_trains(spec, prec, engine, steps, v_at=None, options=None):
     """(i, step) of every spike over `steps` steps; with v_at, also a dict
     {h: V of every neuron after h steps} for each h in v_at (h < steps)."""
     from brian2cuda_amd import networks as nw
     sp = dataclasses.replace(spec, spike_monitors=[0])
-    net, smon, _ = nw.build_hs(sp, prec, engine, spike_capacity=1 << 24)
+    net, smon, _ = nw.build_hs(sp, prec, engine, spike_capacity=1 << 24, options=options)
     vs, done = {}, 0
     for h in sorted(v_at or ()):
         if 0 < h < steps:
@@ -343,14 +343,18 @@ def drift_horizons(i64, t64, i32, t32, n, steps, tol_steps=50):
     return rows, below
 
 
-def drift_config(spec, steps, tol_steps=50):
-    """FP32 batched against FP64 plain for a configuration without noise: spike
-    matching over the whole run and by horizon, the V RMS after the run, and
-    each form's rate and CV."""
-    i64, t64, v64 = spike_trains(spec, 64, "plain", steps, v_at=())
-    i32, t32, v32 = spike_trains(spec, 32, "fused", steps, v_at=())
+def drift_config(spec, steps, tol_steps=50, options=None):
+    """FP32 batched (with `options`, e.g. the fast HH form) against FP64 plain
+    for a configuration without noise: spike matching over the whole run and by
+    horizon, the V RMS after 100 steps and after the run, and each form's rate
+    and CV."""
+    i64, t64, v64 = spike_trains(spec, 64, "plain", steps, v_at=(100,))
+    i32, t32, v32 = spike_trains(spec, 32, "fused", steps, v_at=(100,), options=options)
     res = match_spikes(i64, t64, i32, t32, spec.n_neurons, tol_steps)
     res["steps"] = steps
+    if options:
+        res["fp32_options"] = {str(k): v for k, v in options.items()}
+    res["v_rms_mV_after_100_steps"] = float(np.sqrt(np.nanmean((v32[100] - v64[100]) ** 2))) * 1e3
     res["v_rms_mV_at_end"] = float(np.sqrt(np.nanmean((v32[steps] - v64[steps]) ** 2))) * 1e3
     res["by_horizon"], res["rms_below_1dt_through_steps"] = drift_horizons(i64, t64, i32, t32, spec.n_neurons,
                                                                             steps, tol_steps)
@@ -555,11 +559,25 @@ def other_configs(args):
         spec = other_spec(name, args.seed)
         gen_s = time.time() - t0
         row = {"neurons": spec.n_neurons, "synapses": spec.n_synapses, "spec_s": round(gen_s, 2)}
-        for key, prec, engine in (("fp32_batched", 32, "fused"), ("fp64_plain", 64, "plain")):
-            r = run_form(spec, prec, engine, args.other_steps, args.other_settle)
-            row[key] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
+        rnd = lambda r: {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()}
+        # FP32 batched: the faster of the two exact forms (one launch per step
+        # with update and delivery blocks, or the pipelined form whose update
+        # blocks push their spikes' synapses); COBAHH also in the fast form
+        forms = {"default": {}, "pipelined": {9: 1}}
+        if name.startswith("cobahh"):
+            forms.update({"fast": {11: 1}, "fast_pipelined": {9: 1, 11: 1}})
+        res = {fn: rnd(run_form(spec, 32, "fused", args.other_steps, args.other_settle, options=o))
+               for fn, o in forms.items()}
+        row["fp32_forms_ms_per_step"] = {fn: r["ms_per_step"] for fn, r in res.items()}
+        exact = min(("default", "pipelined"), key=lambda fn: res[fn]["ms_per_step"])
+        row["fp32_batched"] = dict(res[exact], options={str(k): v for k, v in forms[exact].items()})
+        row["fp64_plain"] = rnd(run_form(spec, 64, "plain", args.other_steps, args.other_settle))
         f, p = row["fp32_batched"], row["fp64_plain"]
         row["speedup_fp32_batched_vs_fp64_plain"] = round(p["ms_per_step"] / f["ms_per_step"], 4)
+        if "fast" in res:
+            fast = min(("fast", "fast_pipelined"), key=lambda fn: res[fn]["ms_per_step"])
+            row["fp32_fast"] = dict(res[fast], options={str(k): v for k, v in forms[fast].items()})
+            row["speedup_fp32_fast_vs_fp64_plain"] = round(p["ms_per_step"] / res[fast]["ms_per_step"], 4)
         wl = "batch" if name.startswith("batch") else ("cuba" if name.startswith("cuba") else "cobahh")
         r_s, w_s = state_bytes(wl, 32)
         bstep = spec.n_neurons * (r_s + w_s) + f["spikes_per_step"] * 12 + f["events_per_step"] * 4
@@ -717,6 +735,9 @@ def main():
         dr = {"brunel_hakim_c3": drift(spec, args.drift_steps)}
         for name in ("cuba_4k", "cobahh_4k"):
             dr[name] = drift_config(other_spec(name, args.seed), args.drift_steps)
+        # the FP32 fast form of the HH update (option 11): its error against FP64
+        dr["cobahh_4k_fp32_fast"] = drift_config(other_spec("cobahh_4k", args.seed), args.drift_steps,
+                                                 options={9: 1, 11: 1})
         log("drift", dr)
 
     others = None

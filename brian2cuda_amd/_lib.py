@@ -81,6 +81,8 @@ SIGNATURES = {
     "hs_net_add_synapse_monitor": (C.c_int, [P, C.c_int, C.c_int, P, C.c_int, C.POINTER(C.c_int)]),
     "hs_net_add_spike_monitor_range": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int64, C.POINTER(C.c_int)]),
     "hs_net_add_state_monitor": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "hs_net_add_state_monitor_indices": (C.c_int, [P, C.c_int, C.c_int, I32P, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "hs_net_set_monitor_interval": (C.c_int, [P, C.c_int, C.c_int]),
     "hs_net_set_option": (C.c_int, [P, C.c_int, C.c_longlong]),
     "hs_net_get_option": (C.c_int, [P, C.c_int, C.POINTER(C.c_longlong)]),
     "hs_net_run": (C.c_int, [P, C.c_int64]),

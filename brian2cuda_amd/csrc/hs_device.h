@@ -90,6 +90,29 @@ __device__ __forceinline__ int hs_wrap(int x, int m)
     return x;
 }
 
+/*
+ * The FP32 fast form of the COBA Hodgkin-Huxley update (option HS_OPT_FAST_MATH):
+ * the same expression text with the device exponential __expf, division as a
+ * product with the hardware reciprocal (v_rcp_f32, 1 ulp) and FMA contraction.
+ * Not bit-reproducible on the host: its error is measured against the FP64
+ * oracle instead.
+ */
+#pragma push_macro("HS_HD")
+#undef HS_HD
+#define HS_HD __device__ static inline __attribute__((always_inline))
+#define R float
+#define HS_SFX(name) name##_f32fast
+#define HS_EXP __expf
+#define HS_DIV(a, b) ((a) * __builtin_amdgcn_rcpf(b))
+#define HS_FAST_CONTRACT _Pragma("clang fp contract(fast)")
+#include "hs_model_body.inc"
+#undef R
+#undef HS_SFX
+#undef HS_EXP
+#undef HS_DIV
+#undef HS_FAST_CONTRACT
+#pragma pop_macro("HS_HD")
+
 template <typename R> struct HsOps;
 template <> struct HsOps<double> {
     static __device__ __forceinline__ void cuba(double *v, double *ge, double *gi, int nr, const double *k)
@@ -101,6 +124,8 @@ template <> struct HsOps<double> {
         return hs_lif_noise_update_f64(V, xi, k);
     }
     static __device__ __forceinline__ void hh(double *s, const double *k) { hs_hh_coba_update_f64(s, k); }
+    /* no fast form in FP64: the exact update */
+    static __device__ __forceinline__ void hh_fast(double *s, const double *k) { hs_hh_coba_update_f64(s, k); }
 };
 template <> struct HsOps<float> {
     static __device__ __forceinline__ void cuba(float *v, float *ge, float *gi, int nr, const float *k)
@@ -112,12 +137,14 @@ template <> struct HsOps<float> {
         return hs_lif_noise_update_f32(V, xi, k);
     }
     static __device__ __forceinline__ void hh(float *s, const float *k) { hs_hh_coba_update_f32(s, k); }
+    static __device__ __forceinline__ void hh_fast(float *s, const float *k) { hs_hh_coba_update_f32fast(s, k); }
 };
 
 template <int MODEL> struct HsNV;
 template <> struct HsNV<HS_MODEL_LIF_CUBA> { static constexpr int value = HS_LIF_CUBA_NVARS; };
 template <> struct HsNV<HS_MODEL_LIF_NOISE> { static constexpr int value = HS_LIF_NOISE_NVARS; };
 template <> struct HsNV<HS_MODEL_HH_COBA> { static constexpr int value = HS_HH_NVARS; };
+template <> struct HsNV<HS_MODEL_HH_FAST> { static constexpr int value = HS_HH_NVARS; };
 template <> struct HsNV<HS_MODEL_SPIKEGEN> { static constexpr int value = HS_SPIKEGEN_NVARS; };
 template <> struct HsNV<HS_MODEL_POISSON> { static constexpr int value = HS_POISSON_NVARS; };
 
@@ -157,6 +184,8 @@ __device__ __forceinline__ bool hs_neuron_step(R *s, const R *k, int not_ref, R 
         HsOps<R>::cuba(&s[0], &s[1], &s[2], not_ref, k);
     } else if (MODEL == HS_MODEL_LIF_NOISE) {
         s[0] = HsOps<R>::noise(s[0], xi, k);
+    } else if (MODEL == HS_MODEL_HH_FAST) {
+        HsOps<R>::hh_fast(s, k);
     } else {
         HsOps<R>::hh(s, k);
     }

@@ -149,15 +149,21 @@ __device__ __forceinline__ bool hs_step_one(const DGroup<R> &g, R *s, const R *k
     } else {
         if (g.has_reset && (long long)last == t - 1 && t - 1 >= no_reset_before) s[0] = g.vr;
         for (int m = 0; m < g.nst; ++m) {
-            if (i >= g.st_start[m] && i < g.st_stop[m]) {
-                const long long row = t - g.st_t0[m];
-                if (row >= 0 && row < g.st_rows[m]) {
-                    const int sv = g.st_var[m];
-                    R val = hs_opaque(s[0]);
+            /* the row of step t (uniform over the launch), then this neuron's column */
+            const long long dt0 = t - g.st_t0[m];
+            if (dt0 < 0) continue;
+            const int every = g.st_every[m];
+            const long long row = every == 1 ? dt0 : dt0 / every;
+            if (row * every != dt0 || row >= g.st_rows[m]) continue;
+            int col;
+            if (g.st_map[m]) col = hs_ld(g.st_map[m] + i);
+            else col = (i >= g.st_start[m] && i < g.st_stop[m]) ? i - g.st_start[m] : -1;
+            if (col >= 0) {
+                const int sv = g.st_var[m];
+                R val = hs_opaque(s[0]);
 #pragma unroll
-                    for (int v = 1; v < NV; ++v) val = sv == v ? hs_opaque(s[v]) : val;
-                    hs_st(g.st_buf[m] + row * (long long)(g.st_stop[m] - g.st_start[m]) + (i - g.st_start[m]), val);
-                }
+                for (int v = 1; v < NV; ++v) val = sv == v ? hs_opaque(s[v]) : val;
+                hs_st(g.st_buf[m] + row * (long long)g.st_width[m] + col, val);
             }
         }
         const int not_ref = (t - (long long)last) >= (long long)g.ref_steps;
@@ -348,6 +354,9 @@ __device__ __forceinline__ void hs_update_phase(const DGroup<R> *groups, int ng,
         } else if (g.model == HS_MODEL_POISSON) {
             hs_update_tile<R, HS_MODEL_POISSON, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
                                                              push);
+        } else if (g.model == HS_MODEL_HH_FAST) {
+            hs_update_tile<R, HS_MODEL_HH_FAST, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
+                                                             push);
         } else {
             hs_update_tile<R, HS_MODEL_HH_COBA, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
                                                              push);
@@ -526,13 +535,18 @@ __global__ __launch_bounds__(256) void hs_k_fused(const DGroup<R> *__restrict__ 
         HsDeliverLds d;
     } L;
     const long long T = ctl->base + koff;
-    if ((int)blockIdx.x < n_upd) {
+    /* the delivery blocks come first: they do not depend on this launch's
+       update, so they are dispatched at once and overlap with it instead of
+       starting after the last update block */
+    const int n_del = (int)gridDim.x - n_upd;
+    if ((int)blockIdx.x >= n_del) {
+        const int b = (int)blockIdx.x - n_del;
         hs_update_phase<R, true, MS>(groups, ng, total_tiles, T, 1, 1, ctl->no_reset_before, &L.c,
-                                     ctl->spk_slot + (blockIdx.x % HS_EV_SLOTS) * HS_EV_STRIDE, blockIdx.x, n_upd);
+                                     ctl->spk_slot + (b % HS_EV_SLOTS) * HS_EV_STRIDE, b, n_upd);
     } else {
         const int nwb = blockDim.x / HS_WAVE;
-        const long long rank = (long long)(blockIdx.x - n_upd) * nwb + threadIdx.x / HS_WAVE;
-        const long long nw = (long long)(gridDim.x - n_upd) * nwb;
+        const long long rank = (long long)blockIdx.x * nwb + threadIdx.x / HS_WAVE;
+        const long long nw = (long long)n_del * nwb;
         unsigned long long ev = 0;
         for (int pi = 0; pi < np; ++pi) hs_deliver_path<R, true>(paths[pi], T, 1, &L.d, rank, nw, &ev);
         hs_flush_events(ev, ctl, rank);
@@ -590,11 +604,13 @@ hipError_t hs_launch_update(const DGroup<R> *groups, int ng, int total_tiles, Hs
         if (ms == HS_MODEL_LIF_CUBA) HS_UPD(true, HS_MODEL_LIF_CUBA);
         else if (ms == HS_MODEL_LIF_NOISE) HS_UPD(true, HS_MODEL_LIF_NOISE);
         else if (ms == HS_MODEL_HH_COBA) HS_UPD(true, HS_MODEL_HH_COBA);
+        else if (ms == HS_MODEL_HH_FAST) HS_UPD(true, HS_MODEL_HH_FAST);
         else HS_UPD(true, 0);
     } else {
         if (ms == HS_MODEL_LIF_CUBA) HS_UPD(false, HS_MODEL_LIF_CUBA);
         else if (ms == HS_MODEL_LIF_NOISE) HS_UPD(false, HS_MODEL_LIF_NOISE);
         else if (ms == HS_MODEL_HH_COBA) HS_UPD(false, HS_MODEL_HH_COBA);
+        else if (ms == HS_MODEL_HH_FAST) HS_UPD(false, HS_MODEL_HH_FAST);
         else HS_UPD(false, 0);
     }
 #undef HS_UPD
@@ -644,6 +660,7 @@ hipError_t hs_launch_fused(const DGroup<R> *groups, int ng, int total_tiles, con
     if (ms == HS_MODEL_LIF_CUBA) HS_FUS(HS_MODEL_LIF_CUBA);
     else if (ms == HS_MODEL_LIF_NOISE) HS_FUS(HS_MODEL_LIF_NOISE);
     else if (ms == HS_MODEL_HH_COBA) HS_FUS(HS_MODEL_HH_COBA);
+    else if (ms == HS_MODEL_HH_FAST) HS_FUS(HS_MODEL_HH_FAST);
     else HS_FUS(0);
 #undef HS_FUS
     return hipGetLastError();
@@ -680,6 +697,7 @@ hipError_t hs_launch_window(const DGroup<R> *groups, int ng, int total_tiles, co
     if (ms == HS_MODEL_LIF_CUBA) HS_WIN(HS_MODEL_LIF_CUBA);
     else if (ms == HS_MODEL_LIF_NOISE) HS_WIN(HS_MODEL_LIF_NOISE);
     else if (ms == HS_MODEL_HH_COBA) HS_WIN(HS_MODEL_HH_COBA);
+    else if (ms == HS_MODEL_HH_FAST) HS_WIN(HS_MODEL_HH_FAST);
     else HS_WIN(0);
 #undef HS_WIN
     return hipGetLastError();

@@ -20,6 +20,9 @@
 #include "../../include/hs_models.h"
 
 #define HS_MAX_STATE_MONS 4
+/* kernel specialisation of HS_MODEL_HH_COBA in the FP32 fast form (option 11);
+   groups keep their model id, the runtime selects this one for the kernels */
+#define HS_MODEL_HH_FAST 103
 #define HS_MAX_DELAY_SLOTS 4096 /* D per pathway (LDS prefix table) */
 #define HS_MAX_WINDOW 8         /* steps per update launch of the windowed form */
 #define HS_GTILE 256            /* targets per block of the gather delivery */
@@ -86,6 +89,10 @@ struct DGroup {
     int st_var[HS_MAX_STATE_MONS], st_start[HS_MAX_STATE_MONS], st_stop[HS_MAX_STATE_MONS];
     long long st_t0[HS_MAX_STATE_MONS], st_rows[HS_MAX_STATE_MONS];
     R *st_buf[HS_MAX_STATE_MONS];
+    /* recorded indices: st_map[m][i] = column of neuron i or -1 (nullptr: the
+       range [st_start, st_stop)); a row every st_every steps from st_t0, st_width columns */
+    const int32_t *st_map[HS_MAX_STATE_MONS];
+    int st_every[HS_MAX_STATE_MONS], st_width[HS_MAX_STATE_MONS];
     int tile_begin; /* first tile index of this group in the concatenated tile space */
     R *xi;          /* windowed form: noise ring [xi_slots][n], draw of step t in slot t mod xi_slots (or nullptr) */
     int xi_slots;
@@ -217,7 +224,7 @@ hipError_t hs_launch_stateupdate(int model, int prec, int n, void *const *vars, 
                                  int ref_steps, long long step, hipStream_t s);
 hipError_t hs_launch_math(int fn, int n, const double *x, double *y, hipStream_t s);
 hipError_t hs_launch_synmon(int prec, const void *src, const int32_t *idx, int n, void *buf, long long rows,
-                            long long t0, const HsCtl *ctl, int koff, hipStream_t s);
+                            long long t0, int every, const HsCtl *ctl, int koff, hipStream_t s);
 hipError_t hs_launch_scatter_add(int prec, const int32_t *idx, long long n, int n_target, double w, void *target,
                                  unsigned long long *bad, hipStream_t s);
 

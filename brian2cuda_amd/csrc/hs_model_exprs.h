@@ -8,6 +8,10 @@
 
 #include "hs_detmath.h"
 
+/* the exact instances: IEEE division, no contraction (the numerical contract) */
+#define HS_DIV(a, b) ((a) / (b))
+#define HS_FAST_CONTRACT
+
 #define R double
 #define HS_SFX(name) name##_f64
 #define HS_EXP hs_exp
@@ -23,6 +27,8 @@
 #undef R
 #undef HS_SFX
 #undef HS_EXP
+#undef HS_DIV
+#undef HS_FAST_CONTRACT
 
 /*
  * Brian2's `timestep` function (brian2cuda/cuda_generator.py:997-1006):

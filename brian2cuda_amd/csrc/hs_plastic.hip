@@ -151,25 +151,27 @@ template hipError_t hs_launch_stdp<float>(const DPath<float> *, int, HsCtl *, in
    synapses (positions in bundle order) */
 template <typename R>
 __global__ __launch_bounds__(256) void hs_k_synmon(const R *__restrict__ src, const int32_t *__restrict__ idx, int n,
-                                                    R *__restrict__ buf, long long rows, long long t0, const HsCtl *ctl,
-                                                    int koff)
+                                                    R *__restrict__ buf, long long rows, long long t0, int every,
+                                                    const HsCtl *ctl, int koff)
 {
-    const long long row = ctl->base + koff - t0;
-    if (row < 0 || row >= rows) return;
+    const long long dt0 = ctl->base + koff - t0;
+    if (dt0 < 0 || dt0 % every != 0) return;
+    const long long row = dt0 / every;
+    if (row >= rows) return;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
         buf[row * n + k] = src[idx[k]];
 }
 
 hipError_t hs_launch_synmon(int prec, const void *src, const int32_t *idx, int n, void *buf, long long rows,
-                            long long t0, const HsCtl *ctl, int koff, hipStream_t s)
+                            long long t0, int every, const HsCtl *ctl, int koff, hipStream_t s)
 {
-    if (n <= 0) return hipSuccess;
+    if (n <= 0 || every < 1) return hipSuccess;
     const int grid = (n + 255) / 256 > 1024 ? 1024 : (n + 255) / 256;
     if (prec == 64)
         hipLaunchKernelGGL(hs_k_synmon<double>, dim3(grid), dim3(256), 0, s, (const double *)src, idx, n,
-                           (double *)buf, rows, t0, ctl, koff);
+                           (double *)buf, rows, t0, every, ctl, koff);
     else
         hipLaunchKernelGGL(hs_k_synmon<float>, dim3(grid), dim3(256), 0, s, (const float *)src, idx, n,
-                           (float *)buf, rows, t0, ctl, koff);
+                           (float *)buf, rows, t0, every, ctl, koff);
     return hipGetLastError();
 }

@@ -468,7 +468,21 @@ struct HMon {
     int pid = -1;                      /* kind 3 (synapse variable): pathway, selected synapses */
     int32_t *sidx = nullptr;
     std::vector<int32_t> hsel;
+    /* kind 1: a row every `every` steps from t0 (t0 mod every == 0), `width`
+       columns; recorded neurons [start, stop) or, with hidx, hidx[] in that
+       order (map: device int32[n] of columns, -1 if not recorded) */
+    int every = 1;
+    int64_t width = 0;
+    std::vector<int32_t> hidx;
+    int32_t *map = nullptr;
 };
+
+/* rows a monitor holds after the network reached `step` */
+static int64_t hs_mon_rows(const HMon &M, int64_t step)
+{
+    if (step <= M.t0) return 0;
+    return (step - M.t0 + M.every - 1) / M.every;
+}
 
 struct hs_net {
     int prec, engine;
@@ -490,7 +504,7 @@ struct hs_net {
     /* 0 update block, 1 scatter grid, 2 graph chunk, 3 fused delivery blocks,
        4 window, 5 delivery mode, 6 gather block, 7 kernel timing, 8 noise blocks,
        9 pipelined windows */
-    long long opt[11] = {256, 0, 256, 0, 0, 0, 256, 0, 0, 0, 0};
+    long long opt[12] = {256, 0, 256, 0, 0, 0, 256, 0, 0, 0, 0, 0};
     /* kernel timing: event pairs of the current run, totals per kind */
     std::vector<hipEvent_t> kt_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> kt_open;
@@ -1322,12 +1336,65 @@ extern "C" int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start
     M.var = var;
     M.start = start;
     M.stop = stop;
+    M.width = stop - start;
     M.t0 = net->step;
     net->m.push_back(M);
     g.stmons.push_back((int)net->m.size() - 1);
     net->desc_dirty = true;
     hs_drop_graphs(net);
     if (mid) *mid = (int)net->m.size() - 1;
+    return HS_OK;
+}
+
+extern "C" int hs_net_add_state_monitor_indices(hs_net *net, int gid, int var, const int32_t *indices, int n,
+                                                int every, int *mid)
+{
+    if (!net || gid < 0 || gid >= (int)net->g.size()) return hs_fail(HS_ERR_ARG, "bad group");
+    HGroup &g = net->g[gid];
+    if (var < 0 || var >= g.mi.nvars || n <= 0 || !indices || every < 1)
+        return hs_fail(HS_ERR_ARG, "bad state monitor (variable, %d indices, every %d steps)", n, every);
+    if ((int)g.stmons.size() >= HS_MAX_STATE_MONS)
+        return hs_fail(HS_ERR_UNSUPPORTED, "at most %d state monitors per group", HS_MAX_STATE_MONS);
+    std::vector<int32_t> map((size_t)g.n, -1);
+    for (int k = 0; k < n; ++k) {
+        const int32_t x = indices[k];
+        if (x < 0 || x >= g.n) return hs_fail(HS_ERR_ARG, "record index %d outside [0, %d)", x, g.n);
+        if (map[(size_t)x] >= 0) return hs_fail(HS_ERR_ARG, "record index %d given twice", x);
+        map[(size_t)x] = k;
+    }
+    HMon M;
+    M.kind = 1;
+    M.gid = gid;
+    M.var = var;
+    M.start = 0;
+    M.stop = g.n;
+    M.width = n;
+    M.every = every;
+    M.hidx.assign(indices, indices + n);
+    M.t0 = (net->step + every - 1) / every * every; /* the monitor's clock ticks at multiples of its dt */
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * (size_t)g.n, (void **)&M.map, false));
+    HS_HIP(hipMemcpyAsync(M.map, map.data(), sizeof(int32_t) * (size_t)g.n, hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    net->m.push_back(M);
+    g.stmons.push_back((int)net->m.size() - 1);
+    net->desc_dirty = true;
+    hs_drop_graphs(net);
+    if (mid) *mid = (int)net->m.size() - 1;
+    return HS_OK;
+}
+
+extern "C" int hs_net_set_monitor_interval(hs_net *net, int mid, int every)
+{
+    if (!net || mid < 0 || mid >= (int)net->m.size() || every < 1) return hs_fail(HS_ERR_ARG, "bad monitor or interval");
+    HMon &M = net->m[mid];
+    if (M.kind != 1 && M.kind != 3)
+        return hs_fail(HS_ERR_UNSUPPORTED, "only StateMonitors (of neurons or synapses) take an interval");
+    if (hs_mon_rows(M, net->step) > 0)
+        return hs_fail(HS_ERR_STATE, "the interval is set before the monitor records its first row");
+    M.every = every;
+    M.t0 = (net->step + every - 1) / every * every;
+    net->desc_dirty = true;
+    hs_drop_graphs(net);
     return HS_OK;
 }
 
@@ -1371,8 +1438,11 @@ extern "C" int hs_net_get_rate_monitor(hs_net *net, int mid, double *rate, int64
 
 extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
 {
-    if (!net || key < 0 || key > 10) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (!net || key < 0 || key > 11) return hs_fail(HS_ERR_ARG, "bad option key");
     if (key == 7 && (value < 0 || value > 1)) return hs_fail(HS_ERR_ARG, "kernel timing: 0 off, 1 on");
+    if (key == 11 && (value < 0 || value > 1)) return hs_fail(HS_ERR_ARG, "fast math: 0 off, 1 on");
+    if (key == 11 && value && net->prec != HS_F32)
+        return hs_fail(HS_ERR_UNSUPPORTED, "the fast form of the HH update is an FP32 form");
     if (key == 8 && (value < 0 || value > (1 << 16))) return hs_fail(HS_ERR_ARG, "bad noise block count");
     if (key == 9 && (value < 0 || value > 1)) return hs_fail(HS_ERR_ARG, "pipelined windows: 0 off, 1 on");
     if (key == 9 && net->built) return hs_fail(HS_ERR_STATE, "the engine form is fixed at the first run");
@@ -1399,9 +1469,16 @@ extern "C" int hs_net_set_option(hs_net *net, int key, long long value)
 
 extern "C" int hs_net_get_option(hs_net *net, int key, long long *value)
 {
-    if (!net || key < 0 || key > 10 || !value) return hs_fail(HS_ERR_ARG, "bad option key");
+    if (!net || key < 0 || key > 11 || !value) return hs_fail(HS_ERR_ARG, "bad option key");
     *value = net->opt[key];
     return HS_OK;
+}
+
+/* the model id the kernels are specialised for: the HH model runs its fast
+   form when option 11 is on (FP32) */
+static int hs_kernel_model(const hs_net *net, int model)
+{
+    return (model == HS_MODEL_HH_COBA && net->opt[11] && net->prec == HS_F32) ? HS_MODEL_HH_FAST : model;
 }
 
 /* allocate rings/accumulators once (first run) */
@@ -1502,7 +1579,7 @@ static int hs_upload_desc_t(hs_net *net)
         const HGroup &g = net->g[gi];
         DGroup<R> &d = dg[gi];
         memset(&d, 0, sizeof(d));
-        d.model = g.model;
+        d.model = hs_kernel_model(net, g.model);
         d.n = g.n;
         d.nvars = g.mi.nvars;
         d.Q = g.Q;
@@ -1551,6 +1628,9 @@ static int hs_upload_desc_t(hs_net *net)
             d.st_t0[s] = M.t0;
             d.st_rows[s] = M.rows;
             d.st_buf[s] = (R *)M.buf;
+            d.st_map[s] = M.map;
+            d.st_every[s] = M.every;
+            d.st_width[s] = (int)M.width;
         }
         d.tile_begin = tiles;
         d.xi = (R *)g.xi;
@@ -1630,11 +1710,11 @@ static int hs_grow_state_monitors(hs_net *net, int64_t until)
 {
     for (HMon &M : net->m) {
         if (M.kind == 0) continue;
-        const int64_t need = until - M.t0;
+        const int64_t need = hs_mon_rows(M, until);
         if (need <= M.rows) continue;
         int64_t rows = std::max<int64_t>(need, M.rows * 2);
         /* state rows: one value per recorded neuron; rate rows: one int32 count, zeroed */
-        const size_t rowb = M.kind == 1   ? hs_rsize(net) * (size_t)(M.stop - M.start)
+        const size_t rowb = M.kind == 1   ? hs_rsize(net) * (size_t)M.width
                             : M.kind == 3 ? hs_rsize(net) * M.hsel.size()
                                           : sizeof(int32_t);
         void *nb = nullptr;
@@ -1752,9 +1832,9 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
     const DGroup<R> *dg = (const DGroup<R> *)net->dgroups;
     const DPath<R> *dp = (const DPath<R> *)net->dpaths;
     const int fused = net->engine == HS_ENGINE_FUSED;
-    int ms = net->g.empty() ? 0 : net->g[0].model; /* one model for all groups: specialised kernels */
+    int ms = net->g.empty() ? 0 : hs_kernel_model(net, net->g[0].model); /* one model for all groups: specialised kernels */
     for (const HGroup &g : net->g)
-        if (g.model != ms) ms = 0;
+        if (hs_kernel_model(net, g.model) != ms) ms = 0;
     const int ggrid = std::max(1, std::min(net->gather_tiles, 1 << 20));
     /* where a window's noise is drawn: by extra blocks of its scatter-delivery
        launch (two-launch windows), else one window ahead by the update or window
@@ -1814,7 +1894,7 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
                         const HPath &P = net->p[M.pid];
                         const void *src = M.var == HS_SYN_W ? P.w : (M.var == HS_SYN_APRE ? P.apre : P.apost);
                         HS_HIP(hs_launch_synmon(net->prec, src, M.sidx, (int)M.hsel.size(), M.buf, M.rows, M.t0,
-                                                net->ctl, (int)j, g_rt.stream));
+                                                M.every, net->ctl, (int)j, g_rt.stream));
                     }
                 HS_TRY(update((int)j, 1, 1, fused));
                 if (np > 0) HS_TRY(deliver((int)j, 1));
@@ -2055,7 +2135,7 @@ extern "C" int hs_net_mon_count(hs_net *net, int mid, int64_t *n)
         HS_HIP(hipStreamSynchronize(g_rt.stream));
         *n = (int64_t)c;
     } else {
-        *n = std::max<int64_t>(0, net->step - M.t0);
+        *n = hs_mon_rows(M, net->step);
     }
     return HS_OK;
 }
@@ -2136,16 +2216,17 @@ extern "C" int hs_net_write_monitor(hs_net *net, int mid, const char *path_a, co
     /* StateMonitor: t (double), values (the variable's type, time-major rows of
        the recorded indices, objects.cu:589-595), indices (int32), N (rows); a
        synapse StateMonitor records its synapses' positions as indices */
-    const int64_t w = M.kind == 3 ? (int64_t)M.hsel.size() : M.stop - M.start;
+    const int64_t w = M.kind == 3 ? (int64_t)M.hsel.size() : M.width;
     std::vector<double> t((size_t)(n ? n : 1));
-    for (int64_t r = 0; r < n; ++r) t[(size_t)r] = (double)(M.t0 + r) * net->dt;
+    for (int64_t r = 0; r < n; ++r) t[(size_t)r] = (double)(M.t0 + r * M.every) * net->dt;
     std::vector<unsigned char> vals(hs_rsize(net) * (size_t)(n * w ? n * w : 1));
     if (n * w) {
         HS_HIP(hipMemcpyAsync(vals.data(), M.buf, hs_rsize(net) * (size_t)(n * w), hipMemcpyDeviceToHost, g_rt.stream));
         HS_HIP(hipStreamSynchronize(g_rt.stream));
     }
     std::vector<int32_t> idx((size_t)(w ? w : 1));
-    for (int64_t k = 0; k < w; ++k) idx[(size_t)k] = M.kind == 3 ? M.hsel[(size_t)k] : (int32_t)(M.start + k);
+    for (int64_t k = 0; k < w; ++k)
+        idx[(size_t)k] = M.kind == 3 ? M.hsel[(size_t)k] : (!M.hidx.empty() ? M.hidx[(size_t)k] : (int32_t)(M.start + k));
     const int32_t N = (int32_t)n;
     HS_TRY(hs_write_raw(path_a, t.data(), sizeof(double) * (size_t)n));
     HS_TRY(hs_write_raw(path_b, vals.data(), hs_rsize(net) * (size_t)(n * w)));
@@ -2170,8 +2251,8 @@ extern "C" int hs_net_get_state_monitor(hs_net *net, int mid, double *out, int64
     if (!net || mid < 0 || mid >= (int)net->m.size() || (net->m[mid].kind != 1 && net->m[mid].kind != 3))
         return hs_fail(HS_ERR_ARG, "bad monitor");
     const HMon &M = net->m[mid];
-    const int64_t rows = std::max<int64_t>(0, net->step - M.t0);
-    const int64_t w = M.kind == 3 ? (int64_t)M.hsel.size() : M.stop - M.start;
+    const int64_t rows = hs_mon_rows(M, net->step);
+    const int64_t w = M.kind == 3 ? (int64_t)M.hsel.size() : M.width;
     if (cap < rows * w) return hs_fail(HS_ERR_ARG, "output too small");
     if (!rows) return HS_OK;
     if (net->prec == HS_F64) {

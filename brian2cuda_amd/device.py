@@ -85,7 +85,8 @@ class MonitorDesc:
     source: str
     source_range: Optional[tuple] = None
     variable: Optional[str] = None
-    record: object = None                 # state monitor: recorded indices (a contiguous range)
+    record: object = None                 # state monitor: recorded indices (within the source)
+    dt: Optional[float] = None            # state monitor: its own clock's dt (a multiple of the network's)
 
 
 # ---- built-in models ------------------------------------------------------------
@@ -374,10 +375,19 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
         elif m.kind == "state":
             if m.variable not in gvars[gid]:
                 raise NotImplementedError(f"StateMonitor {m.name!r}: no variable {m.variable!r}")
-            rec = np.atleast_1d(np.asarray(m.record if m.record is not None else np.arange(lo, hi), np.int64))
-            if rec.size == 0 or not np.array_equal(rec, np.arange(rec[0], rec[0] + rec.size)):
-                raise NotImplementedError(f"StateMonitor {m.name!r}: record must be a contiguous range")
-            spec.state_monitors.append((gid, gvars[gid].index(m.variable), int(rec[0]), int(rec[-1]) + 1))
+            rec = np.atleast_1d(np.asarray(m.record if m.record is not None else np.arange(hi - lo), np.int64)) + lo
+            if rec.size == 0 or len(np.unique(rec)) != rec.size:
+                raise NotImplementedError(f"StateMonitor {m.name!r}: record must name distinct neurons")
+            every = 1
+            if m.dt is not None:
+                every = int(round(float(m.dt) / dt))
+                if every < 1 or abs(every * dt - float(m.dt)) > 1e-9 * float(m.dt):
+                    raise NotImplementedError(f"StateMonitor {m.name!r}: its dt must be a multiple of the clock's dt")
+            var = gvars[gid].index(m.variable)
+            if every == 1 and np.array_equal(rec, np.arange(rec[0], rec[0] + rec.size)):
+                spec.state_monitors.append((gid, var, int(rec[0]), int(rec[-1]) + 1))
+            else:
+                spec.state_monitors.append(nw.StateMon(gid, var, rec.astype(np.int32), every))
             names["state_monitors"].append(m.name)
         elif m.kind == "rate":
             spec.rate_monitors.append((gid, int(lo), int(hi)))

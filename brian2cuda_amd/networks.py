@@ -70,6 +70,27 @@ class PathSpec:
 
 
 @dataclass
+class StateMon:
+    """StateMonitor(G, var, record=indices, dt=every*dt): the columns are the
+    given neurons in the given order, a row at every step t with t % every == 0.
+    (A plain tuple (gid, var, start, stop) in NetSpec.state_monitors is the
+    contiguous range [start, stop) recorded every step.)"""
+    gid: int
+    var: int
+    indices: np.ndarray
+    every: int = 1
+
+    @property
+    def width(self):
+        return int(np.asarray(self.indices).size)
+
+
+def state_monitor_width(m):
+    """columns of a NetSpec state monitor entry"""
+    return m.width if isinstance(m, StateMon) else m[3] - m[2]
+
+
+@dataclass
 class NetSpec:
     name: str
     dt: float
@@ -77,7 +98,7 @@ class NetSpec:
     groups: List[GroupSpec]
     paths: List[PathSpec]
     spike_monitors: List[int] = field(default_factory=list)         # group ids
-    state_monitors: List[tuple] = field(default_factory=list)       # (gid, var, start, stop)
+    state_monitors: List[object] = field(default_factory=list)      # (gid, var, start, stop) or StateMon
     rate_monitors: List[tuple] = field(default_factory=list)        # (gid, start, stop)
     info: dict = field(default_factory=dict)
 
@@ -370,6 +391,7 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
         if p.stdp is not None:
             net.pathway_stdp(pid, *p.stdp)
     smons = [net.add_spike_monitor(g, spike_capacity) for g in spec.spike_monitors]
-    stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
+    stmons = [net.add_state_monitor_indices(m.gid, m.var, m.indices, m.every) if isinstance(m, StateMon)
+              else net.add_state_monitor(*m) for m in spec.state_monitors]
     net.rate_mids = [net.add_rate_monitor(*m) for m in spec.rate_monitors]
     return net, smons, stmons

@@ -34,6 +34,7 @@ import subprocess
 
 import numpy as np
 
+from . import networks as nw
 from .runtime import (HS_ENGINE_FUSED, HS_ENGINE_PLAIN, HS_MODEL_HH_COBA, HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE,
                       HS_MODEL_POISSON, HS_MODEL_SPIKEGEN, HS_W_SCALAR)
 
@@ -237,16 +238,29 @@ class ProjectWriter:
                      "count": f"_array_{mname}_count", "N": f"_array_{mname}_N"}
             writes.append((mname, [files["i"], files["t"], files["count"], files["N"]]))
             info["monitors"].append({"name": mname, "kind": "spike", "source": gname, "files": files})
-        for mi, (gid, var, start, stop) in enumerate(spec.state_monitors):
-            mname, gname = self.stnames[mi], self.gnames[gid]
-            vname = MODEL_VARS[spec.groups[gid].model][var]
+        for mi, m in enumerate(spec.state_monitors):
+            mname = self.stnames[mi]
             L.append(f"    int m_{mname};")
-            L.append(f"    HS_CHECK(hs_net_add_state_monitor(net, g_{gname}, {var}, {start}, {stop}, &m_{mname}));")
+            if isinstance(m, nw.StateMon):
+                gid, var = m.gid, m.var
+                gname = self.gnames[gid]
+                ri, n = self._static(directory, f"_array_{mname}__indices", m.indices, np.int32)
+                L.append(f"    {{ std::vector<int32_t> idx({n});")
+                L.append(f"      HS_IO(read_array(\"{ri}\", idx.data(), sizeof(int32_t) * {n}));")
+                L.append(f"      HS_CHECK(hs_net_add_state_monitor_indices(net, g_{gname}, {var}, idx.data(), {n}, "
+                         f"{int(m.every)}, &m_{mname})); }}")
+                width, every = n, int(m.every)
+            else:
+                gid, var, start, stop = m
+                gname = self.gnames[gid]
+                L.append(f"    HS_CHECK(hs_net_add_state_monitor(net, g_{gname}, {var}, {start}, {stop}, &m_{mname}));")
+                width, every = int(stop - start), 1
+            vname = MODEL_VARS[spec.groups[gid].model][var]
             files = {"t": f"_dynamic_array_{mname}_t", vname: f"_dynamic_array_{mname}_{vname}",
                      "indices": f"_array_{mname}__indices", "N": f"_array_{mname}_N"}
             writes.append((mname, [files["t"], files[vname], files["indices"], files["N"]]))
             info["monitors"].append({"name": mname, "kind": "state", "source": gname, "var": vname,
-                                     "indices": [int(start), int(stop)], "files": files})
+                                     "width": width, "every": every, "files": files})
         for mi, (gid, start, stop) in enumerate(spec.rate_monitors):
             mname, gname = self.rmnames[mi], self.gnames[gid]
             L.append(f"    int m_{mname};")
@@ -448,7 +462,7 @@ def read_results(directory, results_dir="results"):
                               "N": int(np.fromfile(os.path.join(rd, files["N"]), "<i4")[0])}
         elif m["kind"] == "state":
             t = np.fromfile(os.path.join(rd, files["t"]), "<f8")
-            width = m["indices"][1] - m["indices"][0]
+            width = m["width"]
             v = np.fromfile(os.path.join(rd, files[m["var"]]), real).reshape(t.size, width)
             out[m["name"]] = {"t": t, m["var"]: v, "indices": np.fromfile(os.path.join(rd, files["indices"]), "<i4"),
                               "N": int(np.fromfile(os.path.join(rd, files["N"]), "<i4")[0])}

@@ -43,6 +43,7 @@ NEVER_SPIKED = -1073741824
 
 OPT_UPDATE_BLOCK, OPT_DELIVER_GRID, OPT_GRAPH_CHUNK, OPT_FUSED_GRID, OPT_WINDOW, OPT_DELIVERY = 0, 1, 2, 3, 4, 5
 OPT_GATHER_BLOCK, OPT_KERNEL_TIMING, OPT_NOISE_BLOCKS, OPT_PIPELINED, OPT_UPDATE_GRID = 6, 7, 8, 9, 10
+OPT_FAST_MATH = 11
 DELIVERY_AUTO, DELIVERY_SCATTER, DELIVERY_GATHER = 0, 1, 2
 KT_UPDATE, KT_SCATTER, KT_GATHER, KT_FUSED, KT_WINDOW = 0, 1, 2, 3, 4
 KERNEL_NAMES = {KT_UPDATE: "hs_k_update", KT_SCATTER: "hs_k_deliver", KT_GATHER: "hs_k_gather", KT_FUSED: "hs_k_fused",
@@ -363,6 +364,18 @@ class Net:
     def add_state_monitor(self, gid, var, start, stop):
         mid = C.c_int()
         call("hs_net_add_state_monitor", self.h, gid, var, int(start), int(stop), C.byref(mid))
+        return mid.value
+
+    def set_monitor_interval(self, mid, every):
+        """a StateMonitor's own dt as every*dt (before its first row)"""
+        call("hs_net_set_monitor_interval", self.h, int(mid), int(every))
+
+    def add_state_monitor_indices(self, gid, var, indices, every=1):
+        """StateMonitor(record=indices, dt=every*dt): columns in the given order,
+        a row at every step t with t % every == 0."""
+        idx = np.ascontiguousarray(indices, dtype=np.int32)
+        mid = C.c_int()
+        call("hs_net_add_state_monitor_indices", self.h, gid, var, _i32(idx), idx.size, int(every), C.byref(mid))
         return mid.value
 
     def set_option(self, key, value):

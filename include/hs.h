@@ -282,8 +282,23 @@ int hs_net_add_spike_monitor_range(hs_net *net, int gid, int start, int stop, in
  */
 int hs_net_add_rate_monitor(hs_net *net, int gid, int start, int stop, int *mid);
 int hs_net_get_rate_monitor(hs_net *net, int mid, double *rate, int64_t cap);
-/* StateMonitor (when='start') of var over neurons [start, stop). */
+/* StateMonitor (when='start') of var over neurons [start, stop), every step. */
 int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start, int stop, int *mid);
+/*
+ * StateMonitor(G, var, record=indices, dt=every*dt) (when='start'): the values
+ * of neurons indices[0..n) (distinct, in the given order: the columns) at the
+ * start of every step t with t mod every == 0, from the step the monitor is
+ * added (the monitor's clock ticks at multiples of its dt).  Rows are read with
+ * hs_net_get_state_monitor (rows x n, time-major) or hs_net_write_monitor
+ * (t = row step * dt; indices as given).
+ * Replaces statemonitor.cu:9-100 with its _indices array and its own clock.
+ */
+int hs_net_add_state_monitor_indices(hs_net *net, int gid, int var, const int32_t *indices, int n, int every,
+                                     int *mid);
+/* The interval of a StateMonitor (of neurons or of synapses) in steps, as its
+   own dt = every*dt: rows at steps t with t mod every == 0.  Set before the
+   monitor records its first row. */
+int hs_net_set_monitor_interval(hs_net *net, int mid, int every);
 
 /* Tuning knobs: key 0 update block size, 1 scatter-delivery grid, 2 steps per
  * captured graph (0: no graph), 3 delivery blocks of the single-launch fused
@@ -294,7 +309,10 @@ int hs_net_add_state_monitor(hs_net *net, int gid, int var, int start, int stop,
  * 9 pipelined windows (1: one launch per window -- update and push of the
  * delays < 2W, pull of the longer delays for the next window; 0, the default:
  * a delivery launch and an update launch per window), 10 update grid (0: one
- * block per tile; n > 0: at most n blocks, each striding over tiles).
+ * block per tile; n > 0: at most n blocks, each striding over tiles),
+ * 11 fast math (FP32 only; 1: the COBA Hodgkin-Huxley update uses the device
+ * exponential, reciprocal division and FMA contraction -- faster, not
+ * bit-reproducible by the oracle; its error is measured against FP64).
  * brian2cuda_amd/autotune.py picks keys 0, 1, 3, 4, 8, 9, 10 by measurement. */
 int hs_net_set_option(hs_net *net, int key, long long value);
 int hs_net_get_option(hs_net *net, int key, long long *value);

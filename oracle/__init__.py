@@ -70,6 +70,8 @@ def lib():
         "ref_net_add_rate_monitor": (C.c_int, [V, C.c_int, C.c_int, C.c_int]),
         "ref_mon_get_rate": (C.c_int, [V, C.c_int, DP]),
         "ref_net_add_state_monitor": (C.c_int, [V, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "ref_net_add_state_monitor_indices": (C.c_int, [V, C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int, C.c_int]),
+        "ref_net_set_monitor_interval": (C.c_int, [V, C.c_int, C.c_int]),
         "ref_net_add_synapse_monitor": (C.c_int, [V, C.c_int, C.c_int, C.POINTER(C.c_int32), C.c_int]),
         "ref_net_run": (C.c_int, [V, C.c_int64]),
         "ref_net_set_threads": (C.c_int, [V, C.c_int]),
@@ -257,6 +259,19 @@ class RefNet:
             raise ValueError("bad state monitor")
         return m
 
+    def set_monitor_interval(self, mid, every):
+        if self.L.ref_net_set_monitor_interval(self.h, int(mid), int(every)) < 0:
+            raise ValueError("bad monitor interval")
+
+    def add_state_monitor_indices(self, gid, var, indices, every=1):
+        """StateMonitor(record=indices, dt=every*dt): rows at steps t with t % every == 0"""
+        idx = np.ascontiguousarray(indices, dtype=np.int32)
+        m = self.L.ref_net_add_state_monitor_indices(self.h, gid, var, idx.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                     idx.size, int(every))
+        if m < 0:
+            raise ValueError("bad state monitor")
+        return m
+
     def add_synapse_monitor(self, pid, var, indices):
         idx = np.ascontiguousarray(indices, dtype=np.int32)
         m = self.L.ref_net_add_synapse_monitor(self.h, pid, var, idx.ctypes.data_as(C.POINTER(C.c_int32)), idx.size)
@@ -331,7 +346,8 @@ def build_ref(spec, precision=64, semantics=SEQ, threads=1):
         if getattr(p, "stdp", None) is not None:
             net.pathway_stdp(pid, p.stdp)
     smons = [net.add_spike_monitor(g) for g in spec.spike_monitors]
-    stmons = [net.add_state_monitor(*m) for m in spec.state_monitors]
+    stmons = [net.add_state_monitor_indices(m.gid, m.var, m.indices, m.every) if hasattr(m, "indices")
+              else net.add_state_monitor(*m) for m in spec.state_monitors]
     net.rate_mids = [net.add_rate_monitor(*m) for m in getattr(spec, "rate_monitors", [])]
     return net, smons, stmons
 

@@ -264,8 +264,10 @@ typedef struct {
     int32_t *i;
     int64_t *step;
     double *data;
-    int pid, nsel; /* kind 3: pathway and recorded synapse positions */
+    int pid, nsel; /* kind 3: pathway and recorded synapse positions;
+                      kind 1 with indices: nsel recorded neurons sel[] */
     int32_t *sel;
+    int every;     /* kind 1: record at steps t with t mod every == 0 (StateMonitor dt) */
 } ref_mon;
 
 struct ref_net {
@@ -549,6 +551,37 @@ int ref_net_add_state_monitor(ref_net *net, int gid, int var, int start, int sto
     memset(m, 0, sizeof(*m));
     m->kind = 1;
     m->gid = gid; m->var = var; m->start = start; m->stop = stop;
+    m->every = 1;
+    return net->nm++;
+}
+
+/* StateMonitor(G, var, record=indices, dt=every*dt) (statemonitor.cu:9-100 with
+   _indices): neurons idx[0..n) in the given order, at the start of every step t
+   with t mod every == 0 */
+/* a StateMonitor's own dt: rows at steps t with t mod every == 0 */
+int ref_net_set_monitor_interval(ref_net *net, int mid, int every)
+{
+    if (mid < 0 || mid >= net->nm || every < 1) return -1;
+    ref_mon *m = &net->mon[mid];
+    if ((m->kind != 1 && m->kind != 3) || m->n) return -1;
+    m->every = every;
+    return 0;
+}
+
+int ref_net_add_state_monitor_indices(ref_net *net, int gid, int var, const int32_t *idx, int n, int every)
+{
+    if (net->nm >= REF_MAX_MONS || gid < 0 || gid >= net->ng || n <= 0 || every < 1) return -1;
+    if (var < 0 || var >= net->g[gid].nvars) return -1;
+    for (int k = 0; k < n; ++k)
+        if (idx[k] < 0 || idx[k] >= net->g[gid].n) return -1;
+    ref_mon *m = &net->mon[net->nm];
+    memset(m, 0, sizeof(*m));
+    m->kind = 1;
+    m->gid = gid; m->var = var; m->start = 0; m->stop = n;
+    m->nsel = n;
+    m->sel = (int32_t *)malloc(sizeof(int32_t) * n);
+    memcpy(m->sel, idx, sizeof(int32_t) * n);
+    m->every = every;
     return net->nm++;
 }
 
@@ -566,6 +599,7 @@ int ref_net_add_synapse_monitor(ref_net *net, int pid, int var, const int32_t *s
     memset(m, 0, sizeof(*m));
     m->kind = 3;
     m->gid = p->tgt; m->var = var; m->pid = pid; m->nsel = n;
+    m->every = 1;
     m->sel = (int32_t *)malloc(sizeof(int32_t) * n);
     memcpy(m->sel, sel, sizeof(int32_t) * n);
     return net->nm++;
@@ -573,6 +607,7 @@ int ref_net_add_synapse_monitor(ref_net *net, int pid, int var, const int32_t *s
 
 static void mon_syn(ref_net *net, ref_mon *m)
 {
+    if (m->every > 1 && net->step % m->every != 0) return;
     const ref_path *p = &net->pw[m->pid];
     const void *src = m->var == 0 ? p->w : (m->var == 1 ? p->apre : p->apost);
     int64_t w = m->nsel;
@@ -604,6 +639,7 @@ static void mon_spikes(ref_net *net, ref_mon *m, const ref_group *g)
 
 static void mon_state(ref_net *net, ref_mon *m, const ref_group *g)
 {
+    if (m->every > 1 && net->step % m->every != 0) return;
     int64_t w = m->stop - m->start;
     if ((m->n + 1) * w > m->cap) {
         int64_t c = m->cap ? m->cap : 1024 * w;
@@ -611,7 +647,8 @@ static void mon_state(ref_net *net, ref_mon *m, const ref_group *g)
         m->data = (double *)realloc(m->data, sizeof(double) * c);
         m->cap = c;
     }
-    for (int64_t j = 0; j < w; ++j) m->data[m->n * w + j] = rget(net, g->var[m->var], m->start + j);
+    for (int64_t j = 0; j < w; ++j)
+        m->data[m->n * w + j] = rget(net, g->var[m->var], m->sel ? m->sel[j] : m->start + j);
     m->n++;
 }
 

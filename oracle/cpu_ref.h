@@ -52,6 +52,8 @@ int ref_net_add_rate_monitor(ref_net *net, int gid, int start, int stop);
 int ref_mon_get_rate(ref_net *net, int mid, double *out);
 int ref_net_add_spike_monitor(ref_net *net, int gid);
 int ref_net_add_state_monitor(ref_net *net, int gid, int var, int start, int stop);
+int ref_net_add_state_monitor_indices(ref_net *net, int gid, int var, const int32_t *idx, int n, int every);
+int ref_net_set_monitor_interval(ref_net *net, int mid, int every);
 int ref_net_add_synapse_monitor(ref_net *net, int pid, int var, const int32_t *sel, int n);
 int ref_net_run(ref_net *net, int64_t nsteps);
 int64_t ref_net_step(ref_net *net);

@@ -161,11 +161,24 @@ def test_unsupported_synapses_are_refused(on_pre, kw, msg):
         dv.network_spec([g], [s], [], dt=0.1 * ms)
 
 
-def test_non_contiguous_state_monitor_is_refused():
+def test_state_monitor_indices_and_dt():
+    """StateMonitor(S, 'v', record=[...], dt=...) as the reference's consistency
+    tests use it (test_cpp_cuda_consistency.py:72): arbitrary indices and a
+    monitor dt become an indexed monitor; a contiguous range every step stays a
+    range; repeated indices and a dt that is not a multiple are refused."""
     g = dv.GroupDesc("g", 10, equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr", namespace=CUBA_NS)
-    m = dv.MonitorDesc("statemonitor", "state", "g", variable="v", record=[0, 2, 4])
-    with pytest.raises(NotImplementedError, match="contiguous"):
-        dv.network_spec([g], [], [m], dt=0.1 * ms)
+    ms_ = [dv.MonitorDesc("m0", "state", "g", variable="v", record=[0, 4, 2]),
+           dv.MonitorDesc("m1", "state", "g", variable="ge", record=range(3), dt=1 * ms),
+           dv.MonitorDesc("m2", "state", "g", variable="gi", record=range(2, 5))]
+    spec, _ = dv.network_spec([g], [], ms_, dt=0.1 * ms)
+    a, b, c = spec.state_monitors
+    assert isinstance(a, nw.StateMon) and list(a.indices) == [0, 4, 2] and a.every == 1 and a.var == 0
+    assert isinstance(b, nw.StateMon) and list(b.indices) == [0, 1, 2] and b.every == 10 and b.var == 1
+    assert c == (0, 2, 2, 5)
+    for bad, msg in [(dict(record=[1, 1]), "distinct"), (dict(record=[1], dt=0.25 * ms), "multiple")]:
+        m = dv.MonitorDesc("m", "state", "g", variable="v", **bad)
+        with pytest.raises(NotImplementedError, match=msg):
+            dv.network_spec([g], [], [m], dt=0.1 * ms)
 
 
 def test_device_module_imports_with_and_without_brian2():
