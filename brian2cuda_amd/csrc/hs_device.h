@@ -361,6 +361,28 @@ __device__ __forceinline__ void hs_deliver_path(const DPath<R> &p, long long T, 
     __syncthreads();
 }
 
+/*
+ * Delivery blocks split over the pathways of a launch: with at least as many
+ * blocks as pathways, block b serves pathway b mod np only, its waves ranked
+ * among that pathway's blocks, so a block pays one pass of count prefix and
+ * bundle lookups instead of one per pathway; with fewer blocks, every block
+ * serves every pathway in turn.  Loop: for (p = p0; p < np; p += pstep).
+ */
+struct HsSplit {
+    int p0, pstep;
+    long long rank, n_waves;
+};
+
+__device__ __forceinline__ HsSplit hs_split(int np, int b, int n_blk)
+{
+    const int nwb = blockDim.x / HS_WAVE, wv = threadIdx.x / HS_WAVE;
+    if (np > 1 && n_blk >= np) {
+        const int p = b % np, nb = (n_blk - p + np - 1) / np;
+        return HsSplit{p, np, (long long)(b / np) * nwb + wv, (long long)nb * nwb};
+    }
+    return HsSplit{0, 1, (long long)b * nwb + wv, (long long)n_blk * nwb};
+}
+
 __device__ __forceinline__ void hs_flush_events(unsigned long long ev, unsigned long long *events)
 {
     /* every lane of a wave holds the wave's count; lane 0 publishes it */

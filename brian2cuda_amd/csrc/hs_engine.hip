@@ -439,13 +439,12 @@ __global__ __launch_bounds__(256) void hs_k_window(const DGroup<R> *__restrict__
         hs_update_phase<R, true, MS, true>(groups, ng, total_tiles, T, w, W, ctl->no_reset_before, &L.c,
                                            ctl->spk_slot + (b % HS_EV_SLOTS) * HS_EV_STRIDE, b, n_upd, push);
     } else if (b < n_upd + n_far) {
-        const int nwb = blockDim.x / HS_WAVE;
-        const long long rank = (long long)(b - n_upd) * nwb + threadIdx.x / HS_WAVE;
-        const long long nw = (long long)n_far * nwb;
+        const HsSplit sp = hs_split(np, b - n_upd, n_far);
         unsigned long long ev = 0; /* counted at emission instead */
-        for (int pi = 0; pi < np; ++pi) {
+        for (int pi = sp.p0; pi < np; pi += sp.pstep) {
             const DPath<R> &p = paths[pi];
-            if (p.nnear < p.D) hs_deliver_path<R, true>(p, T + W, w, &L.d, rank, nw, &ev, p.nnear, p.D - p.nnear);
+            if (p.nnear < p.D)
+                hs_deliver_path<R, true>(p, T + W, w, &L.d, sp.rank, sp.n_waves, &ev, p.nnear, p.D - p.nnear);
         }
     } else {
         hs_noise_ahead<R>(groups, ng, T + w, W, b - n_upd - n_far, gridDim.x - n_upd - n_far);
@@ -501,13 +500,12 @@ __global__ __launch_bounds__(1024) void hs_k_deliver(const DPath<R> *__restrict_
         hs_noise_ahead<R>(groups, ng, T, w, blockIdx.x - n_del, gridDim.x - n_del);
         return;
     }
-    const int nwb = blockDim.x / HS_WAVE;
-    const long long rank = (long long)blockIdx.x * nwb + threadIdx.x / HS_WAVE;
-    const long long nw = (long long)n_del * nwb;
+    const HsSplit sp = hs_split(np, blockIdx.x, n_del);
     unsigned long long ev = 0;
-    for (int pi = 0; pi < np; ++pi)
-        if (paths[pi].dmode == HS_DELIVER_SCATTER) hs_deliver_path<R, ACC>(paths[pi], T, w, &L, rank, nw, &ev);
-    hs_flush_events(ev, ctl, rank);
+    for (int pi = sp.p0; pi < np; pi += sp.pstep)
+        if (paths[pi].dmode == HS_DELIVER_SCATTER)
+            hs_deliver_path<R, ACC>(paths[pi], T, w, &L, sp.rank, sp.n_waves, &ev);
+    hs_flush_events(ev, ctl, (long long)blockIdx.x * (blockDim.x / HS_WAVE) + threadIdx.x / HS_WAVE);
 }
 
 template <typename R, bool ACC>
@@ -544,12 +542,11 @@ __global__ __launch_bounds__(256) void hs_k_fused(const DGroup<R> *__restrict__ 
         hs_update_phase<R, true, MS>(groups, ng, total_tiles, T, 1, 1, ctl->no_reset_before, &L.c,
                                      ctl->spk_slot + (b % HS_EV_SLOTS) * HS_EV_STRIDE, b, n_upd);
     } else {
-        const int nwb = blockDim.x / HS_WAVE;
-        const long long rank = (long long)blockIdx.x * nwb + threadIdx.x / HS_WAVE;
-        const long long nw = (long long)n_del * nwb;
+        const HsSplit sp = hs_split(np, blockIdx.x, n_del);
         unsigned long long ev = 0;
-        for (int pi = 0; pi < np; ++pi) hs_deliver_path<R, true>(paths[pi], T, 1, &L.d, rank, nw, &ev);
-        hs_flush_events(ev, ctl, rank);
+        for (int pi = sp.p0; pi < np; pi += sp.pstep)
+            hs_deliver_path<R, true>(paths[pi], T, 1, &L.d, sp.rank, sp.n_waves, &ev);
+        hs_flush_events(ev, ctl, (long long)blockIdx.x * (blockDim.x / HS_WAVE) + threadIdx.x / HS_WAVE);
     }
 }
 
