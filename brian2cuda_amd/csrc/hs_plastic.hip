@@ -1,7 +1,8 @@
 /*
  * hs_plastic.hip -- spike-timing-dependent plasticity on scatter pathways: the
- * on_pre and on_post code of an STDP Synapses object, run by the plain form after
- * the step's delivery launch.
+ * on_pre and on_post code of an STDP Synapses object, run after the step's
+ * update and delivery launches (plain form: effects on the state; fused form
+ * with one-step windows: effects into the accumulator of the step).
  *
  * Replaces, for these synapses, the effect kernel of the pre pathway
  * (brian2cuda/templates/synapses.cu:12-164 with synaptic_effects "synapse"
@@ -70,6 +71,9 @@ __global__ __launch_bounds__(256) void hs_k_stdp_pre(const DPath<R> *paths, int 
         R sk[5];
 #pragma unroll
         for (int j = 0; j < 5; ++j) sk[j] = p.sk[j];
+        /* plain form: the target state; fused form: the accumulator of step T,
+           added to the state at the start of step T + 1 */
+        R *tgt = p.acc ? p.acc + hs_mod(T, p.nacc) * p.n_tgt : p.target;
         for (int d = blockIdx.x; d < p.D; d += gridDim.x) {
             const long long ts = T - p.dmin - d; /* emission step of the bundles due now */
             if (ts < 0) continue;
@@ -86,7 +90,7 @@ __global__ __launch_bounds__(256) void hs_k_stdp_pre(const DPath<R> *paths, int 
                         const R fb = HsStdp<R>::decay(n, p.dt, p.taupost);
                         R w = p.wm[q], a = p.apre[q], b = p.apost[q];
                         const R e = HsStdp<R>::pre(&w, &a, &b, fa, fb, sk);
-                        unsafeAtomicAdd(&p.target[p.post[q]], e);
+                        hs_atomic_add(tgt + p.post[q], e);
                         p.wm[q] = w;
                         p.apre[q] = a;
                         p.apost[q] = b;

@@ -1173,7 +1173,6 @@ extern "C" int hs_net_pathway_stdp(hs_net *net, int pid, const double *params, i
     if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");
     if (net->built) return hs_fail(HS_ERR_STATE, "plasticity must be set before the first run");
     if (!params || nparams != HS_STDP_NPARAMS) return hs_fail(HS_ERR_ARG, "STDP takes %d parameters", HS_STDP_NPARAMS);
-    if (net->engine != HS_ENGINE_PLAIN) return hs_fail(HS_ERR_UNSUPPORTED, "STDP pathways run in the plain form");
     HPath &P = net->p[pid];
     if (P.dmode != HS_DELIVER_SCATTER) return hs_fail(HS_ERR_UNSUPPORTED, "STDP needs a scatter-mode pathway");
     if (P.wmode != HS_W_SYNAPSE) return hs_fail(HS_ERR_ARG, "STDP needs per-synapse weights (HS_W_SYNAPSE)");
@@ -1295,8 +1294,9 @@ extern "C" int hs_net_add_synapse_monitor(hs_net *net, int pid, int var, const i
 {
     if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");
     const HPath &P = net->p[pid];
-    if (net->engine != HS_ENGINE_PLAIN)
-        return hs_fail(HS_ERR_UNSUPPORTED, "synapse StateMonitors run in the plain form (as STDP does)");
+    if (net->engine != HS_ENGINE_PLAIN && net->built && net->mode != 1)
+        return hs_fail(HS_ERR_UNSUPPORTED, "synapse StateMonitors run in the per-step forms: add them before the "
+                                           "first run of a fused network");
     if (var < HS_SYN_W || var > HS_SYN_APOST) return hs_fail(HS_ERR_ARG, "bad synapse variable %d", var);
     if (var == HS_SYN_W && P.wmode != HS_W_SYNAPSE)
         return hs_fail(HS_ERR_ARG, "w is recorded on pathways with per-synapse weights");
@@ -1485,8 +1485,8 @@ static int hs_kernel_model(const hs_net *net, int model)
 static int hs_build(hs_net *net)
 {
     if (net->built) return HS_OK;
-    if (net->n_plastic && net->engine != HS_ENGINE_PLAIN)
-        return hs_fail(HS_ERR_UNSUPPORTED, "STDP pathways run in the plain form (HS_ENGINE_PLAIN)");
+    if (net->n_plastic && net->engine != HS_ENGINE_PLAIN && net->n_gather)
+        return hs_fail(HS_ERR_UNSUPPORTED, "STDP pathways in the fused form need scatter delivery");
     /* form of the step loop */
     int dmin_all = 1 << 30, dspan = 1;
     for (const HPath &P : net->p) {
@@ -1498,6 +1498,11 @@ static int hs_build(hs_net *net)
     net->window = 1;
     if (net->engine == HS_ENGINE_PLAIN) {
         net->mode = 0;
+    } else if (net->n_plastic || std::any_of(net->m.begin(), net->m.end(), [](const HMon &M) { return M.kind == 3; })) {
+        /* STDP: its pre and post code run after each step's update (the post
+           code needs the step's spikes), effects into the step's accumulator;
+           synapse StateMonitors record at the start of each step */
+        net->mode = 1;
     } else if (net->p.empty() || dmin_all < 1) {
         /* effects due in the same step: update, then deliver -- or, pipelined,
            one launch per step whose update blocks push every synapse of delay

@@ -238,10 +238,12 @@ int hs_net_set_spikegen(hs_net *net, int gid, int64_t n, const int32_t *indices,
  * Make a pathway plastic: spike-timing-dependent plasticity with event-driven
  * traces (hs_models.h, HS_STDP_NPARAMS parameters), the synapse model of
  * brian2cuda/tests/test_cpp_cuda_consistency.py:45-61.  Needs a scatter pathway
- * with per-synapse weights and the plain form; before the first run.  The
- * pathway then runs its on_pre code (target += w*scale, trace and weight update)
- * and an on_post pathway over the target group's spikes, after the step's other
- * deliveries (Brian2 orders the pre pathway before the post pathway).
+ * with per-synapse weights; before the first run.  The pathway then runs its
+ * on_pre code (target += w*scale, trace and weight update) and an on_post
+ * pathway over the target group's spikes, after the step's other deliveries
+ * (Brian2 orders the pre pathway before the post pathway).  In the fused form
+ * the network then advances one step per update launch and the on_pre effects
+ * go to the step's accumulator, as every other effect of that form does.
  * Replaces the pre- and post-pathway effect kernels of synapses.cu:12-164 for
  * such synapses.
  */
@@ -259,7 +261,8 @@ int hs_net_add_spike_monitor(hs_net *net, int gid, int64_t capacity, int *mid);
  * order of hs_net_get_synapses).  Like Brian2's StateMonitor (when='start')
  * each step records the values before that step's updates; read with
  * hs_net_get_state_monitor (rows x n, time-major) or hs_net_write_monitor.
- * Plain form only (where STDP runs); w needs per-synapse weights.
+ * w needs per-synapse weights.  In the fused form the monitor is added before
+ * the first run (the network then advances one step per update launch).
  * Replaces a StateMonitor on a Synapses object (statemonitor.cu:9-100, as
  * test_cpp_cuda_consistency.py:177-207 and examples/stdp.py:182 use it).
  */

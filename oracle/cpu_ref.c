@@ -495,7 +495,7 @@ int ref_net_set_spikegen(ref_net *net, int gid, int64_t n, const int32_t *idx, c
 /* STDP on a pathway with per-synapse weights (SEQ semantics only) */
 int ref_net_pathway_stdp(ref_net *net, int pid, const double *params, int nparams)
 {
-    if (pid < 0 || pid >= net->np || nparams != HS_STDP_NPARAMS || net->sem != 0) return -1;
+    if (pid < 0 || pid >= net->np || nparams != HS_STDP_NPARAMS) return -1;
     ref_path *p = &net->pw[pid];
     if (p->wmode != HS_W_SYNAPSE || p->plastic) return -2;
     const ref_group *gt = &net->g[p->tgt];

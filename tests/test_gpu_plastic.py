@@ -132,14 +132,7 @@ def test_reference_stdp_examples(hs, oracle_lib, prec, heterog):
 
 def test_stdp_rejects_unsupported(hs):
     spec = _feedforward(0)
-    net = rt.Net(64, "fused", DT, 1)
-    for g in spec.groups:
-        net.add_group(g.model, g.n, g.params)
     S = spec.paths[0]
-    pid = net.add_pathway(S.src, S.tgt, S.var, S.pre, S.post, None, 0, S.wmode, S.weights, 0.0)
-    with pytest.raises(HsError):
-        net.pathway_stdp(pid, *S.stdp)  # fused form
-    net.close()
     net = rt.Net(64, "plain", DT, 1)
     for g in spec.groups:
         net.add_group(g.model, g.n, g.params)
@@ -271,6 +264,53 @@ def test_synapse_monitor_refusals(hs, stdp_poisson):
     net.close()
     fused = nw.cuba(N=300, seed=2)
     net, _, _ = nw.build_hs(fused, 64, "fused")
+    net.run(10)  # built in the one-launch form: no per-step synapse monitor any more
     with pytest.raises(HsError):
         net.add_synapse_monitor(0, rt.HS_SYN_W, [0])
+    net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_stdp_fused_form_vs_oracle(hs, oracle_lib, stdp_poisson, prec):
+    """STDP in the fused (batched) form: one step per update launch, on_pre
+    effects into the step's accumulator -- examples/stdp.py at full size against
+    the oracle's accumulator semantics: spikes, weights, traces, last updates,
+    events and a synapse StateMonitor with an interval, over 2000 steps."""
+    spec = stdp_poisson
+    net, sm, _ = nw.build_hs(spec, prec, "fused")
+    ref, rsm, _ = oracle_lib.build_ref(spec, prec, oracle_lib.ACC)
+    sel = np.arange(0, 5000, 250)
+    g = net.add_synapse_monitor(0, rt.HS_SYN_W, sel)
+    r = ref.add_synapse_monitor(0, rt.HS_SYN_W, sel)
+    net.set_monitor_interval(g, 10)
+    ref.set_monitor_interval(r, 10)
+    for k in (700, 1300):
+        net.run(k)
+        ref.run(k)
+    gi, gt = net.spikes(sm[0])
+    ri, rt_ = ref.spikes(rsm[0])
+    assert len(ri) > 50
+    assert np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    nsyn = net.pathway_info(0)["nsyn"]
+    for var in range(4):
+        assert np.array_equal(net.synapse_var(0, var), ref.synapse_var(0, var, nsyn)), var
+    assert np.array_equal(net.state_monitor(g, sel.size), ref.state_monitor(r, sel.size))
+    assert net.stats()["events"] == ref.events
+    assert net.stats()["window"] == 1
+    net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("delay", [0, 2])
+def test_stdp_feedforward_fused_vs_oracle(hs, oracle_lib, prec, delay):
+    spec = _feedforward(delay)
+    net, sm, st = nw.build_hs(spec, prec, "fused")
+    ref, rsm, rst = oracle_lib.build_ref(spec, prec, oracle_lib.ACC)
+    net.run(1500)
+    ref.run(1500)
+    assert np.array_equal(net.spikes(sm[0])[0], ref.spikes(rsm[0])[0])
+    nsyn = net.pathway_info(0)["nsyn"]
+    for var in range(4):
+        assert np.array_equal(net.synapse_var(0, var), ref.synapse_var(0, var, nsyn)), var
+    assert np.array_equal(net.state_monitor(st[0], 20), ref.state_monitor(rst[0], 20))
     net.close()
