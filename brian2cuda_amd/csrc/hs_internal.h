@@ -167,6 +167,16 @@ struct DPath {
 #define HS_CONN_TAG_SKIP 0u
 #define HS_CONN_TAG_SYN 1u
 #define HS_CONN_TAG_PAIR 2u
+#define HS_CONN_TAG_FIX 3u
+/* fixed out-degree (connect(j='k for k in sample(N_post, size=K)')): row i's draw
+   c (c = 0, 1, ...) is the 53-bit uniform u of words 0-1 of counter
+   (i, c, HS_CONN_CTR | HS_CONN_TAG_FIX, stream), value min(floor(u * n), n - 1),
+   n = post_hi - post_lo.  Draws 0..K-1 first; after sorting, repeated values
+   are dropped and the missing ones drawn with the next counters, until K
+   distinct values remain; value x is target post_lo + x, visited in ascending
+   order, and j == i is then dropped when autapses == 0 (the condition
+   "if k != i" of the generator). */
+#define HS_CONN_KMAX 2048
 
 struct HsConnSpec {
     int pre_lo, pre_hi, post_lo, post_hi;
@@ -179,6 +189,7 @@ struct HsConnSpec {
     double dlo, dspan, dt; /* delay = (int)((dlo + u*dspan)/dt + 0.5) steps */
     int dmin, D;           /* delay slot = delay - dmin, in [0, D) */
     double wlo, wspan;     /* weight = wlo + u*wspan */
+    int fixed_k;           /* > 0: fixed out-degree K (then p and jump are unused) */
 };
 
 /* count pass: cnt[i*D + slot] (int32, zeroed by the caller) for rows [pre_lo, pre_hi);

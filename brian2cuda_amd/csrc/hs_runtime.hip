@@ -814,8 +814,9 @@ extern "C" int hs_net_add_pathway(hs_net *net, int src, int tgt, int var, int64_
 }
 
 /*
- * Random connectivity on the device (kernels in hs_connect.hip): count per
- * (pre, delay slot) -> prefix sum -> bstart -> write post/w in bundle order.
+ * Random connectivity on the device (kernels in hs_connect.hip), connect(p) and
+ * fixed out-degree: count per (pre, delay slot) -> prefix sum -> bstart ->
+ * write post/w in bundle order.
  * The host touches two scalars (the total and an error flag); temporaries are
  * freed before return.
  */
@@ -827,9 +828,10 @@ static int hs_connect_q(double d, double dt, int *q)
     return HS_OK;
 }
 
-extern "C" int hs_net_connect_random(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo,
-                                     int post_hi, double p, int autapses, uint64_t seed, uint32_t stream,
-                                     double delay_lo, double delay_hi, int wmode, double w_lo, double w_hi, int *pid)
+/* fixed_k > 0: K distinct targets per source neuron (p unused); else connect(p) */
+static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo, int post_hi,
+                             double p, int fixed_k, int autapses, uint64_t seed, uint32_t stream, double delay_lo,
+                             double delay_hi, int wmode, double w_lo, double w_hi, int *pid)
 {
     if (!net) return hs_fail(HS_ERR_ARG, "null net");
     if (net->built) return hs_fail(HS_ERR_STATE, "pathways must be added before the first run");
@@ -840,7 +842,9 @@ extern "C" int hs_net_connect_random(hs_net *net, int src, int tgt, int var, int
     if (var < 0 || var >= gt.mi.nvars) return hs_fail(HS_ERR_ARG, "bad target variable %d", var);
     if (pre_lo < 0 || pre_lo > pre_hi || pre_hi > gs.n) return hs_fail(HS_ERR_ARG, "bad source range");
     if (post_lo < 0 || post_lo > post_hi || post_hi > gt.n) return hs_fail(HS_ERR_ARG, "bad target range");
-    if (!(p >= 0.0) || !(p <= 1.0)) return hs_fail(HS_ERR_ARG, "connection probability %g outside [0, 1]", p);
+    if (fixed_k < 0) return hs_fail(HS_ERR_ARG, "out-degree %d < 0", fixed_k);
+    if (!fixed_k && (!(p >= 0.0) || !(p <= 1.0)))
+        return hs_fail(HS_ERR_ARG, "connection probability %g outside [0, 1]", p);
     if (!(delay_lo >= 0.0) || !(delay_hi >= delay_lo) || delay_hi > 1e300)
         return hs_fail(HS_ERR_ARG, "bad delay range [%g, %g]", delay_lo, delay_hi);
     if (wmode != HS_W_SCALAR && wmode != HS_W_SYNAPSE)
@@ -854,8 +858,16 @@ extern "C" int hs_net_connect_random(hs_net *net, int src, int tgt, int var, int
     c.post_lo = post_lo;
     c.post_hi = post_hi;
     c.autapses = (autapses || src != tgt) ? 1 : 0;
-    c.p = p;
-    c.jump = p < 0.25 ? 1 : 0;
+    if (fixed_k > 0 && pre_hi > pre_lo) {
+        if (fixed_k > post_hi - post_lo)
+            return hs_fail(HS_ERR_ARG, "sample size %d bigger than the population size %d", fixed_k,
+                           post_hi - post_lo);
+        if (fixed_k > HS_CONN_KMAX)
+            return hs_fail(HS_ERR_UNSUPPORTED, "out-degree %d > %d", fixed_k, HS_CONN_KMAX);
+    }
+    c.fixed_k = fixed_k;
+    c.p = fixed_k ? 0.0 : p;
+    c.jump = !fixed_k && p < 0.25 ? 1 : 0;
     c.pconst = c.jump && p > 0.0 ? 1.0 / hs_log(1.0 - p) : 0.0;
     c.seed = seed;
     c.stream = stream;
@@ -952,6 +964,24 @@ extern "C" int hs_net_connect_random(hs_net *net, int src, int tgt, int var, int
     net->desc_dirty = true;
     if (pid) *pid = (int)net->p.size() - 1;
     return HS_OK;
+}
+
+extern "C" int hs_net_connect_random(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo,
+                                     int post_hi, double p, int autapses, uint64_t seed, uint32_t stream,
+                                     double delay_lo, double delay_hi, int wmode, double w_lo, double w_hi, int *pid)
+{
+    return hs_connect_device(net, src, tgt, var, pre_lo, pre_hi, post_lo, post_hi, p, 0, autapses, seed, stream,
+                             delay_lo, delay_hi, wmode, w_lo, w_hi, pid);
+}
+
+extern "C" int hs_net_connect_fixed(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo,
+                                    int post_hi, int k, int autapses, uint64_t seed, uint32_t stream, double delay_lo,
+                                    double delay_hi, int wmode, double w_lo, double w_hi, int *pid)
+{
+    if (k < 0) return hs_fail(HS_ERR_ARG, "out-degree %d < 0", k);
+    /* K = 0 creates no synapse: the connect(p = 0) form */
+    return hs_connect_device(net, src, tgt, var, pre_lo, pre_hi, post_lo, post_hi, k ? 1.0 : 0.0, k, autapses, seed,
+                             stream, delay_lo, delay_hi, wmode, w_lo, w_hi, pid);
 }
 
 /*

@@ -71,6 +71,8 @@ class SynapsesDesc:
     i: object = None                      # connect(i=..., j=...) (within the (sub)groups)
     j: object = None
     p: Optional[float] = None             # connect(p=...), optionally with a condition
+    k: Optional[int] = None               # connect(j='k for k in sample(N_post, size=K)'), optionally
+                                          # with condition 'i != j' (its 'if k != i') and i/j ranges
     condition: Optional[str] = None
     values: Dict[str, object] = field(default_factory=dict)  # per-synapse variables (w ...)
     weight_range: Optional[tuple] = None  # w = 'lo + (hi-lo)*rand()' with connect(p)
@@ -319,27 +321,30 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
         sr = s.source_range or (0, n_src)
         tr = s.target_range or (0, n_tgt)
         var, wkind, wval, neg = _on_pre(s.on_pre, gvars[tgt], s)
-        if s.p is not None:
+        if s.p is not None or s.k is not None:
+            if s.p is not None and s.k is not None:
+                raise ValueError(f"Synapses {s.name!r}: connect() takes p or a sample size K, not both")
             pre, post, autapses = _condition_ranges(s.condition, sr[1] - sr[0], tr[1] - tr[0], s)
             pre = (sr[0] + pre[0], sr[0] + pre[1])
             post = (tr[0] + post[0], tr[0] + post[1])
             delay = s.delay if isinstance(s.delay, tuple) else (float(s.delay), float(s.delay))
             if wkind == "synapse":
                 if s.weight_range is None:
-                    raise NotImplementedError(f"Synapses {s.name!r}: with connect(p) the weight is a constant or "
-                                              "'lo + (hi-lo)*rand()'")
+                    raise NotImplementedError(f"Synapses {s.name!r}: with connect(p) or a sample size the weight is a "
+                                              "constant or 'lo + (hi-lo)*rand()'")
                 lo, hi = s.weight_range
                 w = (-hi, -lo) if neg else (lo, hi)
-                rc = nw.RandomConnect(float(s.p), pre, post, int(s.seed), stream=len(paths) + 1, autapses=autapses,
-                                      delay=delay, wmode=HS_W_SYNAPSE, w=w)
+                rc = nw.RandomConnect(float(s.p or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
+                                      autapses=autapses, delay=delay, wmode=HS_W_SYNAPSE, w=w, k=int(s.k or 0))
                 paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SYNAPSE, random=rc))
             else:
-                rc = nw.RandomConnect(float(s.p), pre, post, int(s.seed), stream=len(paths) + 1, autapses=autapses,
-                                      delay=delay, wmode=HS_W_SCALAR, w=(wval, wval))
+                rc = nw.RandomConnect(float(s.p or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
+                                      autapses=autapses, delay=delay, wmode=HS_W_SCALAR, w=(wval, wval),
+                                      k=int(s.k or 0))
                 paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SCALAR, wscalar=wval, random=rc))
             continue
         if s.i is None or s.j is None:
-            raise NotImplementedError(f"Synapses {s.name!r}: connect() with i/j arrays or p= only")
+            raise NotImplementedError(f"Synapses {s.name!r}: connect() with i/j arrays, p= or a sample size only")
         pre = np.asarray(s.i, np.int64) + sr[0]
         post = np.asarray(s.j, np.int64) + tr[0]
         if pre.size != post.size:

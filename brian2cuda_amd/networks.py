@@ -36,7 +36,9 @@ class RandomConnect:
     """Synapses.connect(p=p) generated where the pathway is built (on the device
     by the runtime, by the oracle's restatement for the oracle), with delays
     uniform in delay=(lo, hi) seconds and weights w=(lo, hi) (wmode SYNAPSE:
-    uniform per synapse; SCALAR: w[0])."""
+    uniform per synapse; SCALAR: w[0]).  k > 0: a fixed out-degree instead,
+    K distinct targets per source (connect(j='k for k in sample(N_post,
+    size=K)'))."""
     p: float
     pre: tuple
     post: tuple
@@ -46,6 +48,7 @@ class RandomConnect:
     delay: tuple = (0.0, 0.0)
     wmode: int = HS_W_SCALAR
     w: tuple = (0.0, 0.0)
+    k: int = 0  # > 0: fixed out-degree K instead of p (sample(N_post, size=K))
 
 
 @dataclass
@@ -211,17 +214,22 @@ def brunel_params(tau=20 * ms, mu=25 * mV, sigma=1 * mV, theta=20 * mV, Vr=10 * 
 
 
 def brunel(N=100_000, K=1000, seed=1234, dt=0.1 * ms, J=-0.1 * mV, dlo=0.5 * ms, dhi=5.0 * ms,
-           per_synapse_weights=True, monitors=False, sigma=1 * mV, mu=25 * mV, device_connect=False):
+           per_synapse_weights=True, monitors=False, sigma=1 * mV, mu=25 * mV, device_connect=False,
+           fixed_outdegree=False):
     """C3: sparse inhibitory LIF network with noise and heterogeneous delays.
 
     connect(p=K/N) (examples/brunelhakim.py:144-156), delays U[dlo, dhi] quantised
     to dt, on_pre 'V += -|J|' with a per-synapse weight array.  device_connect:
     the synapses are generated where the network is built (RandomConnect)
-    instead of by numpy here."""
+    instead of by numpy here; with fixed_outdegree, exactly K targets per neuron
+    (connect(j='k for k in sample(N, size=K)')) in place of connect(p)."""
     g = GroupSpec(HS_MODEL_LIF_NOISE, N, brunel_params(sigma=sigma, mu=mu), {0: 10 * mV})
+    if fixed_outdegree and not device_connect:
+        raise ValueError("fixed_outdegree is generated where the network is built (device_connect=True)")
     if device_connect:
         rc = RandomConnect(K / N, (0, N), (0, N), seed, stream=1, delay=(dlo, dhi),
-                           wmode=HS_W_SYNAPSE if per_synapse_weights else HS_W_SCALAR, w=(J, J))
+                           wmode=HS_W_SYNAPSE if per_synapse_weights else HS_W_SCALAR, w=(J, J),
+                           k=K if fixed_outdegree else 0)
         ps = PathSpec(0, 0, 0, None, None, wmode=rc.wmode, wscalar=J, random=rc)
         spec = NetSpec("brunel", dt, seed, [g], [ps], info={"N": N, "K": K})
         if monitors:
@@ -382,8 +390,12 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
     for p in spec.paths:
         if p.random is not None:
             r = p.random
-            pid = net.connect_random(p.src, p.tgt, p.var, r.p, r.pre, r.post, r.autapses, r.seed, r.stream, r.delay,
-                                     r.wmode, r.w)
+            if r.k:
+                pid = net.connect_fixed(p.src, p.tgt, p.var, r.k, r.pre, r.post, r.autapses, r.seed, r.stream,
+                                        r.delay, r.wmode, r.w)
+            else:
+                pid = net.connect_random(p.src, p.tgt, p.var, r.p, r.pre, r.post, r.autapses, r.seed, r.stream,
+                                         r.delay, r.wmode, r.w)
         else:
             add = net.add_pathway_device if device_sort else net.add_pathway
             pid = add(p.src, p.tgt, p.var, p.pre, p.post, p.delay_steps, p.scalar_delay, p.wmode, p.weights,

@@ -191,11 +191,13 @@ class ProjectWriter:
             L.append(f"    int p_{pname};")
             if p.random is not None:
                 r = p.random
-                L.append(f"    HS_CHECK(hs_net_connect_random(net, g_{src}, g_{tgt}, {p.var}, {r.pre[0]}, {r.pre[1]}, "
-                         f"{r.post[0]}, {r.post[1]}, {_c_double(r.p)}, {int(bool(r.autapses))}, {int(r.seed)}ULL, "
+                k = int(getattr(r, "k", 0) or 0)
+                fn, arg = ("hs_net_connect_fixed", str(k)) if k else ("hs_net_connect_random", _c_double(r.p))
+                L.append(f"    HS_CHECK({fn}(net, g_{src}, g_{tgt}, {p.var}, {r.pre[0]}, {r.pre[1]}, "
+                         f"{r.post[0]}, {r.post[1]}, {arg}, {int(bool(r.autapses))}, {int(r.seed)}ULL, "
                          f"{int(r.stream)}u, {_c_double(r.delay[0])}, {_c_double(r.delay[1])}, {int(r.wmode)}, "
                          f"{_c_double(r.w[0])}, {_c_double(r.w[1])}, &p_{pname}));")
-                pinfo["connect"] = "random"
+                pinfo["connect"] = "fixed" if k else "random"
             else:
                 rp, n = self._static(directory, f"_dynamic_array_{pname}__synaptic_pre", p.pre, np.int32)
                 rq, _ = self._static(directory, f"_dynamic_array_{pname}__synaptic_post", p.post, np.int32)

@@ -256,6 +256,24 @@ class Net:
              float(dhi), int(wmode), float(wlo), float(whi), C.byref(pid))
         return pid.value
 
+    def connect_fixed(self, src, tgt, var, K, pre=None, post=None, autapses=True, seed=None, stream=0,
+                      delay=0.0, wmode=HS_W_SCALAR, w=0.0):
+        """A fixed out-degree connect generated on the device: every source
+        neuron of pre samples K distinct targets of post, uniformly
+        (connect(j='k for k in sample(N_post, size=K)')); autapses=False drops
+        a sampled j == i; delays and weights as connect_random.  Returns the
+        pathway id."""
+        n_src, n_tgt = self.groups[src][1], self.groups[tgt][1]
+        pre = (0, n_src) if pre is None else pre
+        post = (0, n_tgt) if post is None else post
+        dlo, dhi = (delay, delay) if np.isscalar(delay) else delay
+        wlo, whi = (w, w) if np.isscalar(w) else w
+        pid = C.c_int()
+        call("hs_net_connect_fixed", self.h, src, tgt, var, int(pre[0]), int(pre[1]), int(post[0]), int(post[1]),
+             int(K), int(bool(autapses)), int(self.seed if seed is None else seed), int(stream), float(dlo),
+             float(dhi), int(wmode), float(wlo), float(whi), C.byref(pid))
+        return pid.value
+
     def set_spikegen(self, gid, indices, steps):
         """SpikeGeneratorGroup pattern: neuron indices[k] fires at step steps[k]."""
         i = np.ascontiguousarray(indices, dtype=np.int32)

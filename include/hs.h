@@ -202,6 +202,21 @@ int hs_net_connect_random(hs_net *net, int src, int tgt, int var, int pre_lo, in
                           double p, int autapses, uint64_t seed, uint32_t stream, double delay_lo, double delay_hi,
                           int wmode, double w_lo, double w_hi, int *pid);
 /*
+ * Fixed out-degree on the device (Synapses.connect(j='k for k in sample(N_post,
+ * size=K)')): every source neuron i in [pre_lo, pre_hi) samples K distinct
+ * targets of [post_lo, post_hi), uniformly over the K-subsets; autapses = 0 then
+ * drops a sampled j == i when src == tgt (the generator's 'if k != i', so such
+ * a row keeps K - 1).  K larger than post_hi - post_lo is an error
+ * (HS_ERR_ARG, the reference's "sample size bigger than the population size"),
+ * K = 0 creates nothing; K <= 2048.  Draws, delays, weights and the bundle
+ * table as hs_net_connect_random.
+ * Replaces the fixed-size sampling of synapses_create_generator.cu:250-312
+ * (selection sampling or a tracked set, both uniform over the K-subsets).
+ */
+int hs_net_connect_fixed(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo, int post_hi,
+                         int k, int autapses, uint64_t seed, uint32_t stream, double delay_lo, double delay_hi,
+                         int wmode, double w_lo, double w_hi, int *pid);
+/*
  * hs_net_add_pathway with the bundle table built on the device: pre, post,
  * delay_steps (or NULL: scalar_delay) and weights are DEVICE pointers (hs_alloc
  * or any allocation on the device) holding int32 / double arrays of the same

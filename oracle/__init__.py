@@ -44,6 +44,9 @@ def lib():
         "ref_connect_random": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64,
                                            C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                            I32P, I32P, I32P, DP, C.c_int64]),
+        "ref_connect_fixed": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
+                                          C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                          I32P, I32P, I32P, DP, C.c_int64]),
         "ref_timestep": (C.c_longlong, [C.c_double, C.c_double]),
         "ref_exp": (C.c_double, [C.c_double]),
         "ref_expf": (C.c_float, [C.c_float]),
@@ -119,6 +122,27 @@ def connect_random(pre_range, post_range, p, skip_self, seed, stream, delay, dt,
     I32P, DP = C.POINTER(C.c_int32), C.POINTER(C.c_double)
     m = L.ref_connect_random(*args, out["pre"].ctypes.data_as(I32P), out["post"].ctypes.data_as(I32P),
                              out["delay_steps"].ctypes.data_as(I32P), out["w"].ctypes.data_as(DP), n)
+    assert m == n
+    return out
+
+
+def connect_fixed(pre_range, post_range, K, skip_self, seed, stream, delay, dt, weight):
+    """Synapses of a fixed out-degree connect (K distinct targets sampled per
+    source, connect(j='k for k in sample(N_post, size=K)'); skip_self drops a
+    sampled j == i), creation order (pre ascending, targets ascending): dict of
+    pre, post, delay_steps, w."""
+    L = lib()
+    args = (int(pre_range[0]), int(pre_range[1]), int(post_range[0]), int(post_range[1]), int(K),
+            int(bool(skip_self)), int(seed), int(stream), float(delay[0]), float(delay[1]), float(dt),
+            float(weight[0]), float(weight[1]))
+    n = L.ref_connect_fixed(*args, None, None, None, None, 0)
+    if n < 0:
+        raise ValueError("sample size K bigger than the population size")
+    out = {"pre": np.empty(n, np.int32), "post": np.empty(n, np.int32), "delay_steps": np.empty(n, np.int32),
+           "w": np.empty(n, np.float64)}
+    I32P, DP = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    m = L.ref_connect_fixed(*args, out["pre"].ctypes.data_as(I32P), out["post"].ctypes.data_as(I32P),
+                            out["delay_steps"].ctypes.data_as(I32P), out["w"].ctypes.data_as(DP), n)
     assert m == n
     return out
 
@@ -319,8 +343,10 @@ def random_pathway(spec, path, rows=None):
     sub-range of source rows (rows are generated independently of each other)."""
     r = path.random
     pre = r.pre if rows is None else rows
-    return connect_random(pre, r.post, r.p, (not r.autapses) and path.src == path.tgt, r.seed, r.stream, r.delay,
-                          spec.dt, r.w)
+    skip_self = (not r.autapses) and path.src == path.tgt
+    if getattr(r, "k", 0):
+        return connect_fixed(pre, r.post, r.k, skip_self, r.seed, r.stream, r.delay, spec.dt, r.w)
+    return connect_random(pre, r.post, r.p, skip_self, r.seed, r.stream, r.delay, spec.dt, r.w)
 
 
 def build_ref(spec, precision=64, semantics=SEQ, threads=1):

@@ -180,6 +180,70 @@ int64_t ref_connect_random(int pre_lo, int pre_hi, int post_lo, int post_hi, dou
     return n;
 }
 
+static int ref_cmp_i32(const void *a, const void *b)
+{
+    const int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+    return (x > y) - (x < y);
+}
+
+/*
+ * Fixed out-degree (connect(j='k for k in sample(N_post, size=K)'), the
+ * reference's fixed-size sampling, synapses_create_generator.cu:250-312, with
+ * its own draws): every source i in [pre_lo, pre_hi) samples K distinct targets
+ * of the n = post_hi - post_lo candidates, uniform over the K-subsets; with
+ * skip_self the sampled j == i is then dropped (the generator's condition
+ * "if k != i", so such a row keeps K - 1).  Draw c of row i is
+ * min(floor(u * n), n - 1), u the 53-bit uniform of words 0-1 of Philox counter
+ * (i, c, tag 3); draws 0..K-1, then, after each sort, the repeated values are
+ * dropped and the missing ones drawn with the next counters.  The targets are
+ * listed in ascending order; delays and weights are drawn per (i, j) as for
+ * connect(p).  Returns the synapse count, -1 if K > n (the reference's error
+ * "sample size bigger than the population size").
+ */
+int64_t ref_connect_fixed(int pre_lo, int pre_hi, int post_lo, int post_hi, int K, int skip_self, uint64_t seed,
+                          uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo, double w_hi,
+                          int32_t *pre, int32_t *post, int32_t *delay_steps, double *w, int64_t cap)
+{
+    int64_t n = 0;
+    const int cand = post_hi - post_lo;
+    if (K > cand && pre_hi > pre_lo) return -1;
+    if (K <= 0 || pre_hi <= pre_lo) return 0;
+    const double dspan = delay_hi - delay_lo, wspan = w_hi - w_lo;
+    int32_t *x = (int32_t *)malloc(sizeof(int32_t) * (size_t)K);
+    for (int i = pre_lo; i < pre_hi; ++i) {
+        uint32_t c = 0;
+        int m = 0; /* distinct values so far, sorted in x[0..m) */
+        while (m < K) {
+            for (int k = m; k < K; ++k) {
+                uint32_t wd[4];
+                ref_conn_words(seed, stream, (uint32_t)i, c++, 3, wd);
+                const int32_t v = (int32_t)floor(ref_u53(wd[0], wd[1]) * (double)cand);
+                x[k] = v < cand - 1 ? v : cand - 1;
+            }
+            qsort(x, (size_t)K, sizeof(int32_t), ref_cmp_i32);
+            m = 0;
+            for (int k = 0; k < K; ++k)
+                if (k == 0 || x[k] != x[k - 1]) x[m++] = x[k];
+        }
+        for (int k = 0; k < K; ++k) {
+            const int32_t j = post_lo + x[k];
+            if (skip_self && j == i) continue;
+            if (n < cap) {
+                uint32_t sw[4];
+                ref_conn_words(seed, stream, (uint32_t)i, (uint32_t)j, 1, sw);
+                const double ud = ref_u53(sw[0], sw[1]), uw = ref_u53(sw[2], sw[3]);
+                if (pre) pre[n] = i;
+                if (post) post[n] = j;
+                if (delay_steps) delay_steps[n] = ref_quantize_delay(delay_lo + ud * dspan, dt);
+                if (w) w[n] = w_lo + uw * wspan;
+            }
+            n++;
+        }
+    }
+    free(x);
+    return n;
+}
+
 long long ref_timestep(double t, double dt) { return hs_timestep(t, dt); }
 
 double ref_exp(double x) { return hs_exp(x); }

@@ -20,6 +20,9 @@ int ref_quantize_delay(double delay, double dt);
 int64_t ref_connect_random(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int skip_self, uint64_t seed,
                            uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo, double w_hi,
                            int32_t *pre, int32_t *post, int32_t *delay_steps, double *w, int64_t cap);
+int64_t ref_connect_fixed(int pre_lo, int pre_hi, int post_lo, int post_hi, int K, int skip_self, uint64_t seed,
+                          uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo, double w_hi,
+                          int32_t *pre, int32_t *post, int32_t *delay_steps, double *w, int64_t cap);
 long long ref_timestep(double t, double dt);
 double ref_exp(double x);
 float ref_expf(float x);

@@ -105,6 +105,24 @@ def test_brunel_hakim_example_maps_to_the_c3_model():
     np.testing.assert_array_equal(ps.delay_steps, nw.quantise_delay(d, 0.1 * ms))
 
 
+def test_fixed_outdegree_connect_maps_to_the_device_sampler():
+    """connect(condition='i != j', j='k for k in sample(N, size=K)') with drawn
+    delays and weights: a RandomConnect with k = K that drops j == i."""
+    eqs = "\ndV/dt = (-V+muext + sigmaext * sqrt(tau) * xi)/tau : volt\n"
+    ns = {"Vr": 10 * mV, "theta": 20 * mV, "tau": 20 * ms, "J": 0.1 * mV, "muext": 25 * mV, "sigmaext": 1 * mV}
+    g = dv.GroupDesc("neurongroup", 500, equations=eqs, threshold="V>theta", reset="V=Vr", refractory=2 * ms,
+                     method="euler", namespace=ns, init={"V": 10 * mV})
+    s = dv.SynapsesDesc("synapses", "neurongroup", "neurongroup", "V += w", k=50, condition="i != j",
+                        values={"w": None}, weight_range=(0.1 * mV, 0.2 * mV), delay=(0.5 * ms, 5 * ms), seed=3)
+    spec, _ = dv.network_spec([g], [s], [], dt=0.1 * ms)
+    r = spec.paths[0].random
+    assert (r.k, r.autapses, r.pre, r.post, r.seed) == (50, False, (0, 500), (0, 500), 3)
+    assert r.wmode == HS_W_SYNAPSE and r.w == (0.1 * mV, 0.2 * mV) and r.delay == (0.5 * ms, 5 * ms)
+    with pytest.raises(ValueError):
+        dv.network_spec([g], [dv.SynapsesDesc("synapses", "neurongroup", "neurongroup", "V += J", namespace=ns,
+                                              k=5, p=0.1)], [], dt=0.1 * ms)
+
+
 def test_cobahh_example_maps_to_the_c2_model():
     """examples/cobahh.py:160-190 with subgroup sources Pe = P[:Ne], Pi = P[Ne:]
     and per-synapse constant weights 'we : siemens (constant)'."""

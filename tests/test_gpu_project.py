@@ -81,6 +81,22 @@ def test_device_connect_and_noise_project(hs, oracle_lib, tmp_path):
     np.testing.assert_array_equal(res["neurongroup.V"], ref.get_state(0, 0).astype(np.float32).astype(np.float64))
 
 
+def test_fixed_outdegree_project(hs, oracle_lib, tmp_path):
+    """The fixed out-degree connect through the project (hs_net_connect_fixed in
+    main.cpp): spikes and state equal the oracle's."""
+    spec = nw.brunel(N=3000, K=300, seed=23, device_connect=True, fixed_outdegree=True, monitors=True)
+    project.write_project(spec, tmp_path, 500, precision=32, engine="fused")
+    project.build_project(tmp_path)
+    project.run_project(tmp_path, "results", timeout=120)
+    res = project.read_results(tmp_path)
+    ref, rsm, _ = _oracle(oracle_lib, spec, 32, oracle_lib.ACC, 500)
+    ri, rstep = ref.spikes(rsm[0])
+    assert len(ri) > 0
+    np.testing.assert_array_equal(res["spikemonitor"]["i"], ri)
+    np.testing.assert_array_equal(res["spikemonitor"]["t"], rstep * spec.dt)
+    np.testing.assert_array_equal(res["neurongroup.V"], ref.get_state(0, 0).astype(np.float32).astype(np.float64))
+
+
 def test_described_cuba_example_through_the_device_layer(hs, oracle_lib, tmp_path):
     """examples/cuba.py described object by object (equation text, threshold,
     reset, namespace, connect('i<Ne', p=0.02), on_pre 'ge += we') goes through

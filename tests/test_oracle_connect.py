@@ -9,6 +9,11 @@ with per-synapse delay and weight draws) against facts it does not compute itsel
   - the distributions the algorithm must produce (Bernoulli counts, geometric
     gaps, quantised uniform delays, uniform weights), by chi-square tests;
   - special cases: p = 0, p = 1, dropping j == i, rows independent of the range.
+
+The fixed out-degree form (oracle.connect_fixed) is pinned the same way: a
+numpy restatement of its draw/dedupe/redraw rounds, K distinct sorted targets
+per row, uniformity over the K-subsets (chi-square over all 56 subsets of a
+small case), and K = 0, K = n, K > n.
 """
 import numpy as np
 import pytest
@@ -175,3 +180,101 @@ def test_seed_and_stream_change_the_draws():
     c = oracle.connect_random((0, 50), (0, 2000), 0.01, False, 6, 0, (0, 0), 1e-4, (0, 0))
     assert not np.array_equal(a["post"][:50], b["post"][:50])
     assert not np.array_equal(a["post"][:50], c["post"][:50])
+
+
+# --------------------------------------------------------------------------
+# fixed out-degree: connect(j='k for k in sample(N_post, size=K)'), the
+# reference's fixed-size sampling (synapses_create_generator.cu:250-312)
+# --------------------------------------------------------------------------
+TAG_FIX = 3
+
+
+def restate_fixed(rows, post, K, seed, stream, delay, dt, weight, skip_self=False):
+    """Draw K values, keep the distinct ones (np.unique sorts), draw the missing
+    ones with the row's next counters; then the per-(i, j) delay and weight."""
+    lo, hi = post
+    n = hi - lo
+    out = {k: [] for k in ("pre", "post", "delay_steps", "w")}
+    for i in range(*rows):
+        vals = np.empty(0, np.int64)
+        c = 0
+        while vals.size < K:
+            need = K - vals.size
+            w = philox_np(i, np.arange(c, c + need), CTR | TAG_FIX, stream, seed)
+            c += need
+            new = np.minimum(np.floor(u53(w[0], w[1]) * n).astype(np.int64), n - 1)
+            vals = np.unique(np.r_[vals, new])
+        js = lo + vals
+        if skip_self:
+            js = js[js != i]
+        w = philox_np(i, js, CTR | TAG_SYN, stream, seed)
+        out["pre"].append(np.full(js.size, i, np.int32))
+        out["post"].append(js.astype(np.int32))
+        out["delay_steps"].append(nw.quantise_delay(delay[0] + u53(w[0], w[1]) * (delay[1] - delay[0]), dt))
+        out["w"].append(weight[0] + u53(w[2], w[3]) * (weight[1] - weight[0]))
+    return {k: np.concatenate(v) for k, v in out.items()}
+
+
+@pytest.mark.parametrize("K,post,skip_self", [(10, (0, 2000), False), (100, (0, 300), True), (250, (50, 350), False),
+                                              (300, (0, 300), False), (1, (0, 7), True), (64, (0, 1000), True)])
+def test_fixed_rows_match_vectorised_restatement(K, post, skip_self):
+    rows = (0, 60)
+    a = oracle.connect_fixed(rows, post, K, skip_self, 123456789, 7, (0.1e-3, 2.2e-3), 1e-4, (-1.0, 1.0))
+    b = restate_fixed(rows, post, K, 123456789, 7, (0.1e-3, 2.2e-3), 1e-4, (-1.0, 1.0), skip_self)
+    for k in a:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_fixed_rows_hold_k_distinct_sorted_targets():
+    n, K = 500, 40
+    s = oracle.connect_fixed((0, n), (0, n), K, True, 3, 1, (0, 0), 1e-4, (0, 0))
+    bounds = np.searchsorted(s["pre"], np.arange(n + 1))
+    for i in range(n):
+        t = s["post"][bounds[i]:bounds[i + 1]]
+        assert np.all(np.diff(t) > 0) and t.min() >= 0 and t.max() < n and i not in t
+        assert t.size in (K - 1, K)
+    a = oracle.connect_fixed((0, n), (0, n), K, False, 3, 1, (0, 0), 1e-4, (0, 0))
+    assert a["pre"].size == n * K
+    keep = a["pre"] != a["post"]
+    for k in a:  # dropping j == i changes nothing else
+        np.testing.assert_array_equal(a[k][keep], s[k])
+    # i is among the K sampled targets of its own row with probability K / n
+    drops = n * K - s["pre"].size
+    assert drops == np.sum(~keep) and abs(drops - K) < 5 * np.sqrt(K * (1 - K / n))
+
+
+def test_fixed_subsets_are_uniform():
+    """K = 3 of n = 8: all 56 subsets equally likely (chi-square over 28,000 rows);
+    a sampler that favours low values, or repeats, fails it."""
+    from itertools import combinations
+    n, K, rows = 8, 3, 28000
+    s = oracle.connect_fixed((0, rows), (0, n), K, False, 99, 4, (0, 0), 1e-4, (0, 0))
+    t = s["post"].reshape(rows, K)
+    code = (1 << t).sum(1)
+    subsets = [sum(1 << j for j in c) for c in combinations(range(n), K)]
+    obs = np.array([np.sum(code == c) for c in subsets])
+    assert obs.sum() == rows
+    assert stats.chisquare(obs).pvalue > 1e-4
+    # large K / n (many redraw rounds): every target equally often
+    s = oracle.connect_fixed((0, 4000), (0, 100), 90, False, 5, 0, (0, 0), 1e-4, (0, 0))
+    col = np.bincount(s["post"], minlength=100)
+    assert stats.chisquare(col).pvalue > 1e-4
+
+
+def test_fixed_special_cases():
+    n = 200
+    full = oracle.connect_fixed((0, 50), (0, n), n, False, 1, 0, (0, 0), 1e-4, (0, 0))
+    np.testing.assert_array_equal(full["post"], np.tile(np.arange(n), 50))
+    noself = oracle.connect_fixed((0, n), (0, n), n, True, 1, 0, (0, 0), 1e-4, (0, 0))
+    assert noself["pre"].size == n * n - n and np.all(noself["pre"] != noself["post"])
+    assert oracle.connect_fixed((0, n), (0, n), 0, False, 1, 0, (0, 0), 1e-4, (0, 0))["pre"].size == 0
+    with pytest.raises(ValueError):
+        oracle.connect_fixed((0, n), (0, n), n + 1, False, 1, 0, (0, 0), 1e-4, (0, 0))
+    a = oracle.connect_fixed((0, n), (10, 190), 30, False, 9, 2, (0.1e-3, 0.9e-3), 1e-4, (0, 1))
+    c = oracle.connect_fixed((120, 170), (10, 190), 30, False, 9, 2, (0.1e-3, 0.9e-3), 1e-4, (0, 1))
+    sel = (a["pre"] >= 120) & (a["pre"] < 170)
+    for k in a:  # a row does not depend on the other rows
+        np.testing.assert_array_equal(a[k][sel], c[k])
+    assert a["post"].min() >= 10 and a["post"].max() < 190
+    d = oracle.connect_fixed((0, n), (10, 190), 30, False, 9, 3, (0.1e-3, 0.9e-3), 1e-4, (0, 1))
+    assert not np.array_equal(a["post"], d["post"])
