@@ -1,0 +1,521 @@
+"""Code models from Brian2 model descriptions: a NeuronGroup's equations,
+threshold, reset and refractory period turned into the C statements of
+hs_net_add_group_code (update, threshold, reset), which the runtime compiles
+into the engine's kernels for the device.
+
+The reference hands the equations to Brian2's state-update methods and
+translates the resulting abstract code statement by statement into CUDA
+(cuda_generator.py:393-461, translate_one_statement_sequence; the kernels of
+stateupdate.cu, threshold.cu and reset.cu).  This module does both steps for
+the methods Brian2 offers for point neurons:
+
+  euler              x += dt*f(x)  (+ sqrt(dt)*g*xi for additive noise: Euler-Maruyama)
+  rk2                midpoint: k = dt*f(x, t); x += dt*f(x + k/2, t + dt/2)
+  rk4                the classical four-stage Runge-Kutta
+  exponential_euler  dx/dt = A x + B with A, B free of x: x = -B/A + (x + B/A) exp(A dt)
+  exact              linear systems with constant coefficients: X' = M X + c solved
+                     over dt with the matrix exponential (numerically, at code
+                     generation), X(t+dt) = E X + b
+
+Equations are Brian2's text form: 'dx/dt = expr : unit [(flags)]',
+'name = expr : unit' (subexpressions, substituted where used) and
+'name : unit [(constant)]' (per-neuron parameters, state variables the update
+leaves alone).  Names resolve to the group's variables, its subexpressions,
+the namespace (SI values), Brian2's units, and t, dt, xi, i.  Functions: exp,
+log, log10, sqrt, abs, clip, floor, ceil, sign.  '(unless refractory)' holds
+a variable while the neuron is refractory.  Anything else raises
+NotImplementedError naming it.
+
+Constants and units are folded in at generation time; every literal is cast to
+the network's precision R, so an FP32 network computes in FP32 throughout.
+"""
+import ast
+import re
+from dataclasses import dataclass
+from typing import Dict, List, Optional
+
+import numpy as np
+import sympy as sp
+
+
+@dataclass
+class CodeModel:
+    """The arguments of hs_net_add_group_code for one group."""
+    names: List[str]
+    update: str
+    threshold: str = ""
+    reset: str = ""
+    refractory: float = 0.0
+    noise: bool = False
+    stream: int = 0
+
+
+# ---- units ------------------------------------------------------------------------
+def _unit_table():
+    base = {"volt": 1.0, "V": 1.0, "amp": 1.0, "ampere": 1.0, "A": 1.0, "second": 1.0, "s": 1.0,
+            "siemens": 1.0, "S": 1.0, "farad": 1.0, "F": 1.0, "ohm": 1.0, "hertz": 1.0, "Hz": 1.0,
+            "metre": 1.0, "meter": 1.0, "m": 1.0, "gram": 1e-3, "gramme": 1e-3, "g": 1e-3, "mole": 1.0, "mol": 1.0,
+            "molar": 1e3, "M": 1e3, "coulomb": 1.0, "C": 1.0, "newton": 1.0, "N": 1.0, "joule": 1.0, "J": 1.0,
+            "watt": 1.0, "W": 1.0, "liter": 1e-3, "litre": 1e-3, "l": 1e-3, "L": 1e-3}
+    prefixes = {"y": 1e-24, "z": 1e-21, "a": 1e-18, "f": 1e-15, "p": 1e-12, "n": 1e-9, "u": 1e-6, "m": 1e-3,
+                "c": 1e-2, "d": 1e-1, "da": 1e1, "h": 1e2, "k": 1e3, "M": 1e6, "G": 1e9, "T": 1e12}
+    out = dict(base)
+    for b, v in base.items():
+        for p, f in prefixes.items():
+            out.setdefault(p + b, f * v)
+    for b in ("metre", "meter", "m"):  # area and volume forms (cm2, um**2 spelled as words)
+        for p, f in list(prefixes.items()) + [("", 1.0)]:
+            out.setdefault(p + b + "2", (f * base[b]) ** 2)
+            out.setdefault(p + b + "3", (f * base[b]) ** 3)
+    return out
+
+
+UNITS = _unit_table()
+RESERVED = {"t", "dt", "xi", "i", "not_refractory"}
+
+
+class clip(sp.Function):  # noqa: N801 -- printed as the runtime's clip()
+    nargs = 3
+
+
+class log10(sp.Function):  # noqa: N801
+    nargs = 1
+
+
+_FUNCS = {"exp": sp.exp, "log": sp.log, "sqrt": sp.sqrt, "abs": sp.Abs, "floor": sp.floor, "ceil": sp.ceiling,
+          "sign": sp.sign, "clip": clip, "log10": log10}
+
+
+# ---- parsing ------------------------------------------------------------------------
+@dataclass
+class _Eq:
+    kind: str          # "diff" | "sub" | "param"
+    name: str
+    expr: Optional[str]
+    flags: List[str]
+
+
+def parse_equations(text):
+    """Brian2 equation text -> list of _Eq (continuation lines joined, comments dropped)."""
+    start = re.compile(r"^\s*(d[A-Za-z_]\w*\s*/\s*dt\s*=|[A-Za-z_]\w*\s*=|[A-Za-z_]\w*\s*:)")
+    stmts, cur = [], ""
+    for raw in text.replace("\r", "").split("\n"):
+        line = raw.split("#")[0].rstrip()
+        if not line.strip():
+            continue
+        if start.match(line) and cur.strip() and ":" in cur:
+            stmts.append(cur)
+            cur = line
+        else:
+            cur += " " + line
+    if cur.strip():
+        stmts.append(cur)
+    eqs = []
+    for st in stmts:
+        st = st.strip()
+        flags = []
+        m = re.match(r"^(.*):([^:]*?)(\(([^()]*)\))?\s*$", st)
+        if not m:
+            raise NotImplementedError(f"equation {st!r}: no ': unit'")
+        body = m.group(1).strip()
+        if m.group(4):
+            flags = [f.strip() for f in m.group(4).split(",") if f.strip()]
+        d = re.match(r"^d([A-Za-z_]\w*)\s*/\s*dt\s*=\s*(.+)$", body)
+        s = re.match(r"^([A-Za-z_]\w*)\s*=\s*(.+)$", body)
+        p = re.match(r"^([A-Za-z_]\w*)$", body)
+        if d:
+            eqs.append(_Eq("diff", d.group(1), d.group(2), flags))
+        elif s:
+            eqs.append(_Eq("sub", s.group(1), s.group(2), flags))
+        elif p:
+            eqs.append(_Eq("param", p.group(1), None, flags))
+        else:
+            raise NotImplementedError(f"equation {st!r} is not 'dx/dt = ...', 'x = ...' or 'x : unit'")
+        for f in flags:
+            if f not in ("unless refractory", "constant"):
+                raise NotImplementedError(f"equation {st!r}: flag {f!r} is not supported")
+    names = [e.name for e in eqs]
+    dup = {n for n in names if names.count(n) > 1}
+    if dup:
+        raise NotImplementedError(f"equations define {sorted(dup)} more than once")
+    return eqs
+
+
+class _Conv(ast.NodeVisitor):
+    """Python/Brian2 expression AST -> sympy, names through `sym`."""
+
+    def __init__(self, sym, what):
+        self.sym, self.what = sym, what
+
+    def visit_Expression(self, n):
+        return self.visit(n.body)
+
+    def visit_Constant(self, n):
+        if isinstance(n.value, bool):
+            return sp.Integer(int(n.value))
+        if isinstance(n.value, int):
+            return sp.Integer(n.value)
+        if isinstance(n.value, float):
+            return sp.Float(n.value)
+        raise NotImplementedError(f"{self.what}: constant {n.value!r}")
+
+    def visit_Name(self, n):
+        return self.sym(n.id)
+
+    def visit_UnaryOp(self, n):
+        x = self.visit(n.operand)
+        if isinstance(n.op, ast.USub):
+            return -x
+        if isinstance(n.op, ast.UAdd):
+            return x
+        raise NotImplementedError(f"{self.what}: operator {type(n.op).__name__}")
+
+    def visit_BinOp(self, n):
+        a, b = self.visit(n.left), self.visit(n.right)
+        ops = {ast.Add: lambda: a + b, ast.Sub: lambda: a - b, ast.Mult: lambda: a * b, ast.Div: lambda: a / b,
+               ast.Pow: lambda: a ** b}
+        f = ops.get(type(n.op))
+        if f is None:
+            raise NotImplementedError(f"{self.what}: operator {type(n.op).__name__}")
+        return f()
+
+    def visit_Call(self, n):
+        if not isinstance(n.func, ast.Name) or n.func.id not in _FUNCS or n.keywords:
+            raise NotImplementedError(f"{self.what}: function {ast.unparse(n.func)!r} is not supported")
+        return _FUNCS[n.func.id](*[self.visit(a) for a in n.args])
+
+    def generic_visit(self, n):
+        raise NotImplementedError(f"{self.what}: {type(n).__name__} is not supported here")
+
+
+def _parse(expr, sym, what):
+    try:
+        tree = ast.parse(expr.strip(), mode="eval")
+    except SyntaxError as e:
+        raise NotImplementedError(f"{what}: cannot parse {expr!r} ({e.msg})")
+    return _Conv(sym, what).visit(tree)
+
+
+# ---- printing ---------------------------------------------------------------------
+def _lit(x):
+    x = float(x)
+    if not np.isfinite(x):
+        raise NotImplementedError(f"non-finite constant {x}")
+    return f"((R){x!r})"
+
+
+class _Printer:
+    """sympy -> C over the code model's names (R: the network's precision)."""
+
+    def __init__(self, local):
+        self.local = local  # Symbol -> C expression
+
+    def __call__(self, e):
+        return self.p(e)
+
+    def p(self, e):
+        if e in self.local:
+            return self.local[e]
+        if e.is_Symbol:
+            raise NotImplementedError(f"unresolved name {e.name!r}")
+        if e.is_Integer:
+            return f"((R){int(e)})"
+        if e.is_Rational:
+            return f"(((R){int(e.p)})/((R){int(e.q)}))"
+        if e.is_Number:
+            return _lit(e)
+        if e.is_Add:
+            return "(" + " + ".join(self.p(a) for a in e.args) + ")"
+        if e.is_Mul:
+            num, den, sign = [], [], 1
+            for a in e.args:
+                if a.is_Number and a == -1:
+                    sign = -sign
+                elif a.is_Pow and a.exp.is_Integer and a.exp < 0:
+                    den.append(self.pow(a.base, -a.exp))
+                elif a.is_Rational and not a.is_Integer and a.p in (1, -1):
+                    sign *= int(a.p)
+                    den.append(f"((R){int(a.q)})")
+                else:
+                    num.append(self.p(a))
+            s = "*".join(num) if num else "((R)1)"
+            if den:
+                s = f"({s})/({'*'.join(den)})"
+            return f"(-({s}))" if sign < 0 else f"({s})"
+        if e.is_Pow:
+            return self.pow(e.base, e.exp)
+        if isinstance(e, sp.exp):
+            return f"exp({self.p(e.args[0])})"
+        if isinstance(e, sp.log):
+            return f"log({self.p(e.args[0])})"
+        if isinstance(e, log10):
+            return f"log10({self.p(e.args[0])})"
+        if isinstance(e, sp.Abs):
+            return f"abs({self.p(e.args[0])})"
+        if isinstance(e, sp.floor):
+            return f"floor({self.p(e.args[0])})"
+        if isinstance(e, sp.ceiling):
+            return f"ceil({self.p(e.args[0])})"
+        if isinstance(e, sp.sign):
+            return f"sign({self.p(e.args[0])})"
+        if isinstance(e, clip):
+            return "clip(" + ", ".join(self.p(a) for a in e.args) + ")"
+        raise NotImplementedError(f"cannot print {e}")
+
+    def pow(self, b, x):
+        bs = self.p(b)
+        if x.is_Integer:
+            n = int(x)
+            if n == 0:
+                return "((R)1)"
+            if 0 < n <= 8:
+                return "(" + "*".join([bs] * n) + ")"
+            if -8 <= n < 0:
+                return "(((R)1)/(" + "*".join([bs] * -n) + "))"
+        if x == sp.Rational(1, 2):
+            return f"sqrt({bs})"
+        if x == sp.Rational(-1, 2):
+            return f"(((R)1)/sqrt({bs}))"
+        return f"exp({self.p(x)}*log({bs}))"
+
+
+# ---- the model ----------------------------------------------------------------------
+def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=None, namespace=None, dt=None,
+                 stream=0, what="NeuronGroup"):
+    """CodeModel of a NeuronGroup description (method None: 'exact' where the
+    system is linear with constant coefficients, else 'euler').  dt (seconds)
+    is needed by 'exact' only."""
+    ns = dict(namespace or {})
+    eqs = parse_equations(equations)
+    diff = [e for e in eqs if e.kind == "diff"]
+    params = [e for e in eqs if e.kind == "param"]
+    subs = {e.name: e for e in eqs if e.kind == "sub"}
+    names = [e.name for e in eqs if e.kind != "sub"]
+    if not diff:
+        raise NotImplementedError(f"{what}: no differential equation")
+    if len(names) > 8:
+        raise NotImplementedError(f"{what}: {len(names)} state variables (at most 8)")
+    for n in names + list(subs):
+        if n in RESERVED or n in _FUNCS:
+            raise NotImplementedError(f"{what}: variable name {n!r} is reserved")
+    state = {n: sp.Symbol(n, real=True) for n in names}
+    T, DT, XI, I = (sp.Symbol(n, real=True) for n in ("t", "dt", "xi", "i"))
+    xi_names = set()
+    sub_expr = {}
+
+    def sym(name, stack=()):
+        if name in state:
+            return state[name]
+        if name in subs:
+            if name in stack:
+                raise NotImplementedError(f"{what}: subexpression {name!r} refers to itself")
+            if name not in sub_expr:
+                sub_expr[name] = _parse(subs[name].expr, lambda n: sym(n, stack + (name,)),
+                                        f"{what}: {name} = {subs[name].expr}")
+            return sub_expr[name]
+        if name == "t":
+            return T
+        if name == "dt":
+            return DT
+        if name == "i":
+            return I
+        if name == "xi" or name.startswith("xi_"):
+            xi_names.add(name)
+            return XI
+        if name in ns:
+            return sp.Float(float(ns[name]))
+        if name in UNITS:
+            return sp.Float(UNITS[name])
+        raise NotImplementedError(f"{what}: no value for {name!r} (not a variable, subexpression, constant or unit)")
+
+    rhs = {e.name: _parse(e.expr, sym, f"{what}: d{e.name}/dt = {e.expr}") for e in diff}
+    if len(xi_names) > 1:
+        raise NotImplementedError(f"{what}: more than one noise variable ({sorted(xi_names)})")
+    noisy = bool(xi_names)
+    unless = {e.name for e in diff if "unless refractory" in e.flags}
+    if method is None:
+        method = "euler" if noisy else ("exact" if _linear(rhs, state, T) else "euler")
+    local = {s: n for n, s in state.items()}
+    local.update({T: "((R)t)", DT: "dt", XI: "xi", I: "((R)i)"})
+    pr = _Printer(local)
+    out = []
+    order = [e.name for e in diff]
+    if method == "euler":
+        for x in order:
+            f = rhs[x]
+            g = sp.diff(f, XI)
+            if g.has(XI) or any(g.has(y) for y in state.values()):
+                raise NotImplementedError(f"{what}: the noise of d{x}/dt is not additive (euler)")
+            det = f.subs(XI, 0)
+            new = state[x] + DT * det
+            if g != 0:
+                new = new + sp.sqrt(DT) * g * XI
+            out.append(f"const R _hs_{x} = {pr(new)};")
+    elif method in ("rk2", "rk4"):
+        if noisy:
+            raise NotImplementedError(f"{what}: method {method!r} takes no noise (use 'euler')")
+        def stage(k, shift, tshift):
+            vals = {state[x]: state[x] + shift * sp.Symbol(f"_hs_k{k - 1}_{x}", real=True) for x in order} \
+                if k > 1 else {}
+            for x in order:
+                ex = DT * rhs[x].subs({**vals, T: T + tshift * DT}, simultaneous=True)
+                out.append(f"const R _hs_k{k}_{x} = {pr(ex)};")
+
+        for x in order:
+            for k in (1, 2, 3, 4):
+                local[sp.Symbol(f"_hs_k{k}_{x}", real=True)] = f"_hs_k{k}_{x}"
+        if method == "rk2":
+            stage(1, 0, 0)
+            stage(2, sp.Rational(1, 2), sp.Rational(1, 2))
+            for x in order:
+                out.append(f"const R _hs_{x} = {x} + _hs_k2_{x};")
+        else:
+            stage(1, 0, 0)
+            stage(2, sp.Rational(1, 2), sp.Rational(1, 2))
+            stage(3, sp.Rational(1, 2), sp.Rational(1, 2))
+            stage(4, 1, 1)
+            for x in order:
+                out.append(f"const R _hs_{x} = {x} + (_hs_k1_{x} + ((R)2)*_hs_k2_{x} + ((R)2)*_hs_k3_{x} + "
+                           f"_hs_k4_{x})/((R)6);")
+    elif method == "exponential_euler":
+        if noisy:
+            raise NotImplementedError(f"{what}: method 'exponential_euler' takes no noise (use 'euler')")
+        for x in order:
+            f = rhs[x]
+            A = sp.diff(f, state[x])
+            if A.has(state[x]):
+                raise NotImplementedError(f"{what}: d{x}/dt is not linear in {x} (exponential_euler)")
+            if A == 0:
+                out.append(f"const R _hs_{x} = {pr(state[x] + DT * f)};")
+                continue
+            B = f.subs(state[x], 0)
+            out.append(f"const R _hs_BA_{x} = {pr(B / A)};")
+            out.append(f"const R _hs_{x} = -_hs_BA_{x} + (_hs_BA_{x} + {x})*exp({pr(A * DT)});")
+    elif method in ("exact", "linear"):
+        if dt is None:
+            raise NotImplementedError(f"{what}: 'exact' needs the time step")
+        if noisy or not _linear(rhs, state, T):
+            raise NotImplementedError(f"{what}: 'exact' needs a linear system with constant coefficients")
+        E, b = _exact_update(rhs, state, float(dt), set())
+        lines = _exact_lines(E, b, list(state), order)
+        if unless:
+            E2, b2 = _exact_update(rhs, state, float(dt), unless)
+            # variables not held keep the free system's coefficients where the two
+            # agree (their dynamics do not involve a held variable): the same
+            # numbers whether or not the neuron is refractory
+            for r in range(E.shape[0]):
+                if np.allclose(E2[r], E[r], rtol=1e-12, atol=0) and np.isclose(b2[r], b[r], rtol=1e-12, atol=0):
+                    E2[r], b2[r] = E[r], b[r]
+            lines2 = _exact_lines(E2, b2, list(state), order)
+            out.append("R " + ", ".join("_hs_" + x for x in order) + ";")
+            out.append("if (not_refractory) {")
+            out += ["    " + ln.replace("const R ", "") for ln in lines]
+            out.append("} else {")
+            out += ["    " + ln.replace("const R ", "") for ln in lines2]
+            out.append("}")
+            unless = set()  # handled by the two systems
+        else:
+            out += lines
+    else:
+        raise NotImplementedError(f"{what}: method {method!r} is not supported "
+                                  "(euler, rk2, rk4, exponential_euler, exact)")
+    for x in order:
+        if x in unless:
+            out.append(f"{x} = not_refractory ? _hs_{x} : {x};")
+        else:
+            out.append(f"{x} = _hs_{x};")
+    update = "\n".join(out)
+    thr = _condition(threshold, sym, pr, f"{what}: threshold") if threshold else ""
+    rst = _statements(reset, state, sym, pr, f"{what}: reset") if reset else ""
+    return CodeModel(names, update, thr, rst, float(refractory), noisy, int(stream)), names
+
+
+def _linear(rhs, state, T):
+    """X' = M X + c with M and c numbers: free of the state, of t, dt, i and
+    of noise (every constant is folded in by then)."""
+    xs = list(state.values())
+    for f in rhs.values():
+        if any(not sp.diff(f, x).is_Number for x in xs):
+            return False
+        if not f.subs({x: 0 for x in xs}).is_Number:
+            return False
+    return True
+
+
+def _exact_update(rhs, state, dt, frozen):
+    """(E, b) with X(t+dt) = E X(t) + b for X' = M X + c, X every state
+    variable (parameters have zero rows: they hold); rows of frozen variables
+    zero too (they hold while refractory).  The matrix exponential of the
+    augmented system [[M, c], [0, 0]] * dt gives both."""
+    n = len(state)
+    keys = list(state)
+    M = np.zeros((n + 1, n + 1))
+    for r, x in enumerate(keys):
+        if x not in rhs or x in frozen:
+            continue
+        f = rhs[x]
+        for c_, y in enumerate(keys):
+            M[r, c_] = float(sp.diff(f, state[y]))
+        M[r, n] = float(f.subs({s: 0 for s in state.values()}))
+    from scipy.linalg import expm
+    P = expm(M * dt)
+    return P[:n, :n], P[:n, n]
+
+
+def _exact_lines(E, b, keys, order):
+    lines = []
+    for x in order:
+        r = keys.index(x)
+        terms = [y if E[r, c] == 1.0 else f"{_lit(E[r, c])}*{y}" for c, y in enumerate(keys) if E[r, c] != 0.0]
+        if b[r] != 0.0:
+            terms.append(_lit(b[r]))
+        lines.append(f"const R _hs_{x} = {' + '.join(terms) if terms else '((R)0)'};")
+    return lines
+
+
+def _condition(text, sym, pr, what):
+    """A Brian2 boolean expression -> C."""
+    try:
+        tree = ast.parse(text.strip(), mode="eval").body
+    except SyntaxError as e:
+        raise NotImplementedError(f"{what}: cannot parse {text!r} ({e.msg})")
+
+    def c(n):
+        if isinstance(n, ast.BoolOp):
+            op = " && " if isinstance(n.op, ast.And) else " || "
+            return "(" + op.join(c(v) for v in n.values) + ")"
+        if isinstance(n, ast.UnaryOp) and isinstance(n.op, ast.Not):
+            return f"(!{c(n.operand)})"
+        if isinstance(n, ast.Compare):
+            ops = {ast.Gt: ">", ast.Lt: "<", ast.GtE: ">=", ast.LtE: "<=", ast.Eq: "==", ast.NotEq: "!="}
+            parts, left = [], n.left
+            for o, right in zip(n.ops, n.comparators):
+                if type(o) not in ops:
+                    raise NotImplementedError(f"{what}: comparison {type(o).__name__}")
+                parts.append(f"({pr(_Conv(sym, what).visit(left))} {ops[type(o)]} "
+                             f"{pr(_Conv(sym, what).visit(right))})")
+                left = right
+            return "(" + " && ".join(parts) + ")"
+        if isinstance(n, ast.Name) and n.id == "not_refractory":
+            return "not_refractory"
+        if isinstance(n, ast.Constant) and isinstance(n.value, bool):
+            return "true" if n.value else "false"
+        raise NotImplementedError(f"{what}: {ast.unparse(n)!r} is not a condition")
+    return c(tree)
+
+
+def _statements(text, state, sym, pr, what):
+    """Brian2 reset statements ('x = e', 'x += e', ...; newline- or ';'-separated) -> C."""
+    out = []
+    for st in re.split(r"[;\n]", text):
+        st = st.split("#")[0].strip()
+        if not st:
+            continue
+        m = re.match(r"^([A-Za-z_]\w*)\s*(=|\+=|-=|\*=|/=)\s*(.+)$", st)
+        if not m:
+            raise NotImplementedError(f"{what}: {st!r} is not an assignment")
+        x, op, e = m.groups()
+        if x not in state:
+            raise NotImplementedError(f"{what}: {x!r} is not a state variable")
+        out.append(f"{x} {op} {pr(_parse(e, sym, what))};")
+    return "\n".join(out)

@@ -1,0 +1,50 @@
+/*
+ * hs_codegen.h -- code models: neuron groups whose update, threshold and reset
+ * are given as code, compiled with hipRTC into the engine's own kernels when a
+ * network holding them is built (hs_codegen.hip).  Host side only.
+ */
+#ifndef HS_CODEGEN_H
+#define HS_CODEGEN_H
+
+#include <string>
+#include <vector>
+
+#include "hs_internal.h"
+
+/* one code model: variable v is names[v]; update and reset are C statements,
+   threshold a C expression (empty: never) */
+struct HsCgModel {
+    std::vector<std::string> names;
+    std::string update, threshold, reset;
+    int noisy = 0;
+};
+
+/* the engine kernels of one network form, compiled for its code models */
+struct HsCgKernels {
+    hipFunction_t update[2];   /* plain, fused */
+    hipFunction_t window;
+    hipFunction_t fused;
+    hipFunction_t finalize[2]; /* plain, fused */
+};
+
+/* checks a model's names and code; returns HS_OK or fails with a message */
+int hs_cg_check(const HsCgModel &m);
+/* the program compiled for a network's code models in precision prec */
+std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models);
+/* compiled kernels for these models (compiled on first request, then cached
+   for the process by precision and program text); fails with the compiler's log */
+int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const HsCgKernels **out);
+
+/* the engine launches of hs_engine.hip, through the compiled kernels (same
+   arguments and checks as hs_launch_update / _window / _fused / _finalize) */
+hipError_t hs_cg_launch_update(const HsCgKernels *K, const void *groups, int ng, int total_tiles, HsCtl *ctl, int koff,
+                               int w, int wmax, int fused, int block, int grid, int n_noise, hipStream_t s);
+hipError_t hs_cg_launch_window(const HsCgKernels *K, const void *groups, int ng, int total_tiles, const void *paths,
+                               int np, HsCtl *ctl, int koff, int w, int W, int block, int n_upd, int n_far,
+                               int n_noise, hipStream_t s);
+hipError_t hs_cg_launch_fused(const HsCgKernels *K, const void *groups, int ng, int total_tiles, const void *paths,
+                              int np, HsCtl *ctl, int koff, int block, int n_upd, int n_del, hipStream_t s);
+hipError_t hs_cg_launch_finalize(const HsCgKernels *K, const void *groups, int ng, int total_tiles, HsCtl *ctl,
+                                 int fused, int block, hipStream_t s);
+
+#endif /* HS_CODEGEN_H */

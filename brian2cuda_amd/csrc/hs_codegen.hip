@@ -1,0 +1,288 @@
+/*
+ * hs_codegen.hip -- kernels for networks with code models, compiled at run time.
+ *
+ * The reference turns every NeuronGroup's equations into generated CUDA
+ * (cuda_generator.py:200-808 translating the state updater's abstract code,
+ * stateupdate.cu / threshold.cu / reset.cu as the kernel templates) and
+ * compiles the whole project with nvcc.  Here the engine's own kernels are the
+ * templates: a network's code models become HsCg<R, c> specialisations (the
+ * update statements, the threshold expression and the reset statements over
+ * the model's named variables), and the engine source -- embedded in the
+ * library at build time -- is compiled once with hipRTC for the device with
+ * those specialisations in scope.  The update, window, fused and finalize
+ * kernels of the result then replace the library's built-in ones for that
+ * network; delivery, noise, plasticity and monitors are unchanged.
+ *
+ * The numerical contract of the built-in models holds: no contraction, IEEE
+ * division and square root, the library's deterministic exp and log, one
+ * Philox normal draw per neuron-step for noisy models.  So a code model gives
+ * the same bits as the same statements compiled for the host against the same
+ * math (the oracle's code models).
+ */
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/hs.h"
+#include "hs_codegen.h"
+
+/* hs_rtc_names[k], hs_rtc_texts[k] (k < hs_rtc_count): the engine's sources by
+   the names its #include lines use; written by the library's build script */
+#include "hs_rtc_sources.inc"
+
+static bool hs_cg_ident(const std::string &s)
+{
+    if (s.empty() || s.size() > 64) return false;
+    if (!(isalpha((unsigned char)s[0]) || s[0] == '_')) return false;
+    for (char c : s)
+        if (!(isalnum((unsigned char)c) || c == '_')) return false;
+    return true;
+}
+
+int hs_cg_check(const HsCgModel &m)
+{
+    static const char *const reserved[] = {"R", "dt", "t", "i", "xi", "not_refractory", "exp", "log", "log10",
+                                           "sqrt", "abs", "fabs", "clip", "floor", "ceil", "sign", "int"};
+    if (m.names.empty() || (int)m.names.size() > HS_MAX_VARS)
+        return hs_rt_fail(HS_ERR_ARG, "a code model has 1 to %d variables", HS_MAX_VARS);
+    for (size_t a = 0; a < m.names.size(); ++a) {
+        const std::string &n = m.names[a];
+        if (!hs_cg_ident(n) || n.compare(0, 3, "hs_") == 0 || n.compare(0, 2, "__") == 0)
+            return hs_rt_fail(HS_ERR_ARG, "code model variable name '%s' is not an identifier of its own",
+                              n.c_str());
+        for (const char *r : reserved)
+            if (n == r) return hs_rt_fail(HS_ERR_ARG, "code model variable name '%s' is reserved", n.c_str());
+        for (size_t b = 0; b < a; ++b)
+            if (m.names[b] == n) return hs_rt_fail(HS_ERR_ARG, "code model variable '%s' twice", n.c_str());
+    }
+    for (const std::string *c : {&m.update, &m.threshold, &m.reset}) {
+        if (c->find('#') != std::string::npos)
+            return hs_rt_fail(HS_ERR_ARG, "code model code holds a preprocessor directive");
+        int depth = 0;
+        for (char ch : *c) {
+            depth += ch == '{' ? 1 : (ch == '}' ? -1 : 0);
+            if (depth < 0) break;
+        }
+        if (depth != 0) return hs_rt_fail(HS_ERR_ARG, "code model code has unbalanced braces");
+    }
+    return HS_OK;
+}
+
+/* one precision's specialisation of model c */
+static void hs_cg_struct(std::string &o, const char *R, int c, const HsCgModel &m)
+{
+    const int nv = (int)m.names.size();
+    o += "template <> struct HsCg<" + std::string(R) + ", " + std::to_string(c) + "> {\n";
+    o += "    static constexpr int NV = " + std::to_string(nv) + ", NOISY = " + std::to_string(m.noisy ? 1 : 0) +
+         ";\n";
+    o += "    typedef " + std::string(R) + " R;\n";
+    auto load = [&]() {
+        for (int v = 0; v < nv; ++v) o += "        R " + m.names[v] + " = hs_s[" + std::to_string(v) + "];\n";
+    };
+    auto store = [&]() {
+        for (int v = 0; v < nv; ++v) o += "        hs_s[" + std::to_string(v) + "] = " + m.names[v] + ";\n";
+    };
+    o += "    static __device__ __forceinline__ bool step(R *hs_s, int not_refractory, R xi, R dt, double t, int i)\n";
+    o += "    {\n        (void)not_refractory; (void)xi; (void)dt; (void)t; (void)i;\n";
+    load();
+    o += "        {\n" + m.update + "\n        }\n";
+    store();
+    for (int v = 0; v < nv; ++v) o += "        (void)" + m.names[v] + ";\n";
+    o += "        return " + (m.threshold.empty() ? std::string("false") : "(bool)(" + m.threshold + ")") + ";\n";
+    o += "    }\n";
+    o += "    static __device__ __forceinline__ void reset(R *hs_s, R dt, double t, int i)\n";
+    o += "    {\n        (void)dt; (void)t; (void)i;\n";
+    load();
+    o += "        {\n" + m.reset + "\n        }\n";
+    store();
+    o += "    }\n};\n";
+}
+
+std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models)
+{
+    (void)prec;
+    std::string o;
+    o += "#define HS_CG_MODELS " + std::to_string(models.size()) + "\n";
+    o += "#include \"hs_engine.hip\"\n";
+    /* the functions code models may call: the library's deterministic exp and
+       log, IEEE sqrt; overloads keep the argument's precision */
+    o += R"HSCG(
+__device__ __forceinline__ double hs_cg_exp(double x) { return hs_exp(x); }
+__device__ __forceinline__ float hs_cg_exp(float x) { return hs_expf(x); }
+__device__ __forceinline__ double hs_cg_exp(int x) { return hs_exp((double)x); }
+__device__ __forceinline__ double hs_cg_log(double x) { return hs_log(x); }
+__device__ __forceinline__ float hs_cg_log(float x) { return hs_logf(x); }
+__device__ __forceinline__ double hs_cg_log(int x) { return hs_log((double)x); }
+__device__ __forceinline__ double hs_cg_log10(double x) { return hs_log(x) / 2.302585092994046; }
+__device__ __forceinline__ float hs_cg_log10(float x) { return hs_logf(x) / 2.3025851f; }
+__device__ __forceinline__ double hs_cg_sqrt(double x) { return hs_sqrt(x); }
+__device__ __forceinline__ float hs_cg_sqrt(float x) { return hs_sqrtf(x); }
+__device__ __forceinline__ double hs_cg_sqrt(int x) { return hs_sqrt((double)x); }
+__device__ __forceinline__ double hs_cg_abs(double x) { return x < 0.0 ? -x : x; }
+__device__ __forceinline__ float hs_cg_abs(float x) { return x < 0.0f ? -x : x; }
+__device__ __forceinline__ int hs_cg_abs(int x) { return x < 0 ? -x : x; }
+__device__ __forceinline__ double hs_cg_floor(double x) { return floor(x); }
+__device__ __forceinline__ float hs_cg_floor(float x) { return floorf(x); }
+__device__ __forceinline__ double hs_cg_ceil(double x) { return ceil(x); }
+__device__ __forceinline__ float hs_cg_ceil(float x) { return ceilf(x); }
+template <typename T> __device__ __forceinline__ T hs_cg_sign(T x) { return (T)((x > (T)0) - (x < (T)0)); }
+template <typename A, typename B, typename C>
+__device__ __forceinline__ auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo + hi)
+{
+    typedef decltype(x + lo + hi) T;
+    const T v = (T)x;
+    return v < (T)lo ? (T)lo : (v > (T)hi ? (T)hi : v);
+}
+#define exp hs_cg_exp
+#define log hs_cg_log
+#define log10 hs_cg_log10
+#define sqrt hs_cg_sqrt
+#define abs hs_cg_abs
+#define fabs hs_cg_abs
+#define floor hs_cg_floor
+#define ceil hs_cg_ceil
+#define sign hs_cg_sign
+#define clip hs_cg_clip
+)HSCG";
+    for (size_t c = 0; c < models.size(); ++c) {
+        hs_cg_struct(o, "double", (int)c, models[c]);
+        hs_cg_struct(o, "float", (int)c, models[c]);
+    }
+    return o;
+}
+
+struct HsCgEntry {
+    hipModule_t mod = nullptr;
+    HsCgKernels k;
+};
+
+static std::mutex g_cg_mu;
+static std::map<std::string, HsCgEntry> g_cg_cache; /* precision + program text -> loaded kernels (device of hs_init) */
+
+int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const HsCgKernels **out)
+{
+    if (models.empty() || (int)models.size() > HS_MAX_CODE_MODELS)
+        return hs_rt_fail(HS_ERR_UNSUPPORTED, "1 to %d code models per network", HS_MAX_CODE_MODELS);
+    const std::string src = hs_cg_source(prec, models);
+    const std::string key = std::to_string(prec) + "\n" + src; /* the kernels are instantiated for one precision */
+    std::lock_guard<std::mutex> lock(g_cg_mu);
+    auto it = g_cg_cache.find(key);
+    if (it != g_cg_cache.end()) {
+        *out = &it->second.k;
+        return HS_OK;
+    }
+    const char *R = prec == HS_F64 ? "double" : "float";
+    const std::string names[6] = {
+        std::string("hs_k_update<") + R + ", false, 0>", std::string("hs_k_update<") + R + ", true, 0>",
+        std::string("hs_k_window<") + R + ", 0>",        std::string("hs_k_fused<") + R + ", 0>",
+        std::string("hs_k_finalize<") + R + ", false>",  std::string("hs_k_finalize<") + R + ", true>"};
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "hs_code_models.hip", hs_rtc_count, hs_rtc_texts, hs_rtc_names) !=
+        HIPRTC_SUCCESS)
+        return hs_rt_fail(HS_ERR_HIP, "hipRTC: program creation failed");
+    for (const std::string &n : names) (void)hiprtcAddNameExpression(prog, n.c_str());
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        (void)hiprtcDestroyProgram(&prog);
+        return hs_rt_fail(HS_ERR_HIP, "hipRTC: no device properties");
+    }
+    std::string arch = prop.gcnArchName;
+    arch = "--offload-arch=" + arch.substr(0, arch.find(':'));
+    /* the flags the library itself is built with: the numerical contract
+       (no contraction, no fast math, no packed FP32) and hardware FP atomics */
+    const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+                          "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult cr = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    if (cr != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        std::string log;
+        if (hiprtcGetProgramLogSize(prog, &n) == HIPRTC_SUCCESS && n > 1) {
+            log.resize(n);
+            (void)hiprtcGetProgramLog(prog, &log[0]);
+        }
+        (void)hiprtcDestroyProgram(&prog);
+        if (log.size() > 3000) log = "..." + log.substr(log.size() - 3000);
+        return hs_rt_fail(HS_ERR_ARG, "code models do not compile: %s", log.c_str());
+    }
+    size_t code_n = 0;
+    std::vector<char> code;
+    if (hiprtcGetCodeSize(prog, &code_n) == HIPRTC_SUCCESS && code_n) {
+        code.resize(code_n);
+        (void)hiprtcGetCode(prog, code.data());
+    }
+    std::string lowered[6];
+    for (int k = 0; k < 6; ++k) {
+        const char *ln = nullptr;
+        if (hiprtcGetLoweredName(prog, names[k].c_str(), &ln) != HIPRTC_SUCCESS || !ln) {
+            (void)hiprtcDestroyProgram(&prog);
+            return hs_rt_fail(HS_ERR_HIP, "hipRTC: no symbol for %s", names[k].c_str());
+        }
+        lowered[k] = ln;
+    }
+    (void)hiprtcDestroyProgram(&prog);
+    if (code.empty()) return hs_rt_fail(HS_ERR_HIP, "hipRTC: empty code object");
+    HsCgEntry e;
+    hipError_t he = hipModuleLoadData(&e.mod, code.data());
+    if (he != hipSuccess) return hs_rt_fail(HS_ERR_HIP, "code model kernels: %s", hipGetErrorString(he));
+    hipFunction_t *slots[6] = {&e.k.update[0], &e.k.update[1], &e.k.window, &e.k.fused, &e.k.finalize[0],
+                               &e.k.finalize[1]};
+    for (int k = 0; k < 6; ++k) {
+        if ((he = hipModuleGetFunction(slots[k], e.mod, lowered[k].c_str())) != hipSuccess) {
+            (void)hipModuleUnload(e.mod);
+            return hs_rt_fail(HS_ERR_HIP, "code model kernel %s: %s", names[k].c_str(), hipGetErrorString(he));
+        }
+    }
+    auto ins = g_cg_cache.emplace(key, e);
+    *out = &ins.first->second.k;
+    return HS_OK;
+}
+
+static hipError_t hs_cg_go(hipFunction_t f, int grid, int block, hipStream_t s, void **args)
+{
+    return hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, 0, s, args, nullptr);
+}
+
+hipError_t hs_cg_launch_update(const HsCgKernels *K, const void *groups, int ng, int total_tiles, HsCtl *ctl, int koff,
+                               int w, int wmax, int fused, int block, int grid, int n_noise, hipStream_t s)
+{
+    if (w < 1 || w > HS_MAX_WINDOW || wmax < w || wmax > HS_MAX_WINDOW || block > 256 || grid < 1 || n_noise < 0)
+        return hipErrorInvalidValue;
+    void *args[] = {&groups, &ng, &total_tiles, &ctl, &koff, &w, &wmax, &grid};
+    return hs_cg_go(K->update[fused ? 1 : 0], grid + n_noise, block, s, args);
+}
+
+hipError_t hs_cg_launch_window(const HsCgKernels *K, const void *groups, int ng, int total_tiles, const void *paths,
+                               int np, HsCtl *ctl, int koff, int w, int W, int block, int n_upd, int n_far,
+                               int n_noise, hipStream_t s)
+{
+    if (w < 1 || w > W || W > HS_MAX_WINDOW || block > 256 || block < 64 || block % 64 || n_upd < 1 || n_far < 0 ||
+        n_noise < 0)
+        return hipErrorInvalidValue;
+    void *args[] = {&groups, &ng, &total_tiles, &paths, &np, &ctl, &koff, &w, &W, &n_upd, &n_far};
+    return hs_cg_go(K->window, n_upd + n_far + n_noise, block, s, args);
+}
+
+hipError_t hs_cg_launch_fused(const HsCgKernels *K, const void *groups, int ng, int total_tiles, const void *paths,
+                              int np, HsCtl *ctl, int koff, int block, int n_upd, int n_del, hipStream_t s)
+{
+    if (block > 256) return hipErrorInvalidValue;
+    void *args[] = {&groups, &ng, &total_tiles, &paths, &np, &ctl, &koff, &n_upd};
+    return hs_cg_go(K->fused, n_upd + n_del, block, s, args);
+}
+
+hipError_t hs_cg_launch_finalize(const HsCgKernels *K, const void *groups, int ng, int total_tiles, HsCtl *ctl,
+                                 int fused, int block, hipStream_t s)
+{
+    int grid = total_tiles < 4096 ? total_tiles : 4096;
+    if (grid < 1) grid = 1;
+    void *args[] = {&groups, &ng, &total_tiles, &ctl};
+    hipError_t e = hs_cg_go(K->finalize[fused ? 1 : 0], grid, block, s, args);
+    if (e != hipSuccess) return e;
+    return hs_launch_mark_reset(ctl, s);
+}

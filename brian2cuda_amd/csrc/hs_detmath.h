@@ -27,7 +27,9 @@
 #ifndef HS_DETMATH_H
 #define HS_DETMATH_H
 
+#ifndef __HIPCC_RTC__
 #include <math.h>
+#endif
 
 #ifdef __HIP__
 #define HS_HD __host__ __device__ static inline __attribute__((always_inline))

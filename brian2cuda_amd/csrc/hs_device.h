@@ -140,7 +140,29 @@ template <> struct HsOps<float> {
     static __device__ __forceinline__ void hh_fast(float *s, const float *k) { hs_hh_coba_update_f32fast(s, k); }
 };
 
+/*
+ * Code models (HS_MODEL_CODE + c): compiled only into the kernels that are
+ * built at run time for a network holding such groups (hs_codegen.hip), whose
+ * source defines HS_CG_MODELS (the count) and, after this header, one
+ * HsCg<R, c> per model and precision:
+ *   NV, NOISY                   variables, one normal draw per neuron-step
+ *   step(s, not_ref, xi, dt, t, i)  update, then the threshold condition
+ *   reset(s, dt, t, i)          the reset of a neuron that spiked at time t
+ */
+#ifdef HS_CG_MODELS
+template <typename R, int C> struct HsCg;
+template <int MODEL> struct HsNV {
+    static constexpr int value = HsCg<double, MODEL - HS_MODEL_CODE>::NV;
+};
+template <int MODEL> __device__ constexpr bool hs_cg_noisy()
+{
+    if constexpr (HS_IS_CODE_MODEL(MODEL)) return HsCg<double, MODEL - HS_MODEL_CODE>::NOISY != 0;
+    else return false;
+}
+#else
 template <int MODEL> struct HsNV;
+template <int MODEL> __device__ constexpr bool hs_cg_noisy() { return false; }
+#endif
 template <> struct HsNV<HS_MODEL_LIF_CUBA> { static constexpr int value = HS_LIF_CUBA_NVARS; };
 template <> struct HsNV<HS_MODEL_LIF_NOISE> { static constexpr int value = HS_LIF_NOISE_NVARS; };
 template <> struct HsNV<HS_MODEL_HH_COBA> { static constexpr int value = HS_HH_NVARS; };
@@ -169,7 +191,7 @@ __device__ __forceinline__ bool hs_spikegen_fires(const DGroup<R> &g, long long 
 template <typename R, int MODEL>
 __device__ __forceinline__ R hs_noise(unsigned long long seed, unsigned int stream, int i, long long t)
 {
-    if (MODEL == HS_MODEL_LIF_NOISE) {
+    if (MODEL == HS_MODEL_LIF_NOISE || hs_cg_noisy<MODEL>()) {
         if (sizeof(R) == sizeof(float)) return (R)hs_randn_f32(seed, stream, (uint32_t)i, (int64_t)t);
         return (R)hs_randn(seed, stream, (uint32_t)i, (int64_t)t);
     }

@@ -147,7 +147,14 @@ __device__ __forceinline__ bool hs_step_one(const DGroup<R> &g, R *s, const R *k
         if (fire) last = (int32_t)t;
         return fire;
     } else {
-        if (g.has_reset && (long long)last == t - 1 && t - 1 >= no_reset_before) s[0] = g.vr;
+        if (g.has_reset && (long long)last == t - 1 && t - 1 >= no_reset_before) {
+#ifdef HS_CG_MODELS
+            if constexpr (HS_IS_CODE_MODEL(MODEL))
+                HsCg<R, MODEL - HS_MODEL_CODE>::reset(s, (R)g.dt, (double)(t - 1) * g.dt, i);
+            else
+#endif
+                s[0] = g.vr;
+        }
         for (int m = 0; m < g.nst; ++m) {
             /* the row of step t (uniform over the launch), then this neuron's column */
             const long long dt0 = t - g.st_t0[m];
@@ -167,7 +174,13 @@ __device__ __forceinline__ bool hs_step_one(const DGroup<R> &g, R *s, const R *k
             }
         }
         const int not_ref = (t - (long long)last) >= (long long)g.ref_steps;
-        const bool spk = hs_neuron_step<R, MODEL>(s, k, not_ref, g.vth, xi);
+        bool spk;
+#ifdef HS_CG_MODELS
+        if constexpr (HS_IS_CODE_MODEL(MODEL))
+            spk = HsCg<R, MODEL - HS_MODEL_CODE>::step(s, not_ref, xi, (R)g.dt, (double)t * g.dt, i) && not_ref;
+        else
+#endif
+            spk = hs_neuron_step<R, MODEL>(s, k, not_ref, g.vth, xi);
         if (spk) last = (int32_t)t;
         return spk;
     }
@@ -203,7 +216,7 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
         /* the noise of the whole window does not depend on the state: its w
            draws are independent chains, evaluated together (instruction-level
            parallelism) and consumed one per step */
-        constexpr bool NOISY = MODEL == HS_MODEL_LIF_NOISE;
+        constexpr bool NOISY = MODEL == HS_MODEL_LIF_NOISE || hs_cg_noisy<MODEL>();
         R xi[NOISY ? HS_MAX_WINDOW : 1];
         if (NOISY) {
             if (g.xi) {
@@ -326,6 +339,31 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
     __syncthreads();
 }
 
+#ifdef HS_CG_MODELS
+/* a tile of a code-model group: the update specialised for its model */
+template <typename R, bool FUSED, bool PUSH>
+__device__ __forceinline__ void hs_cg_update_tile(const DGroup<R> &g, int gid, int tile, long long T, int w, int wmax,
+                                                  long long no_reset_before, HsCompactLdsW *L,
+                                                  unsigned long long *spk_total, const HsPush<R> &push)
+{
+    const int c = g.model - HS_MODEL_CODE;
+    if (c == 0)
+        hs_update_tile<R, HS_MODEL_CODE + 0, FUSED, PUSH>(g, gid, tile, T, w, wmax, no_reset_before, L, spk_total, push);
+#if HS_CG_MODELS > 1
+    else if (c == 1)
+        hs_update_tile<R, HS_MODEL_CODE + 1, FUSED, PUSH>(g, gid, tile, T, w, wmax, no_reset_before, L, spk_total, push);
+#endif
+#if HS_CG_MODELS > 2
+    else if (c == 2)
+        hs_update_tile<R, HS_MODEL_CODE + 2, FUSED, PUSH>(g, gid, tile, T, w, wmax, no_reset_before, L, spk_total, push);
+#endif
+#if HS_CG_MODELS > 3
+    else if (c == 3)
+        hs_update_tile<R, HS_MODEL_CODE + 3, FUSED, PUSH>(g, gid, tile, T, w, wmax, no_reset_before, L, spk_total, push);
+#endif
+}
+#endif
+
 /* MS: the model of every group (kernel specialised for it, so the register
    budget is that model's), or 0 for networks mixing models */
 template <typename R, bool FUSED, int MS, bool PUSH = false>
@@ -357,6 +395,10 @@ __device__ __forceinline__ void hs_update_phase(const DGroup<R> *groups, int ng,
         } else if (g.model == HS_MODEL_HH_FAST) {
             hs_update_tile<R, HS_MODEL_HH_FAST, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
                                                              push);
+#ifdef HS_CG_MODELS
+        } else if (HS_IS_CODE_MODEL(g.model)) {
+            hs_cg_update_tile<R, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total, push);
+#endif
         } else {
             hs_update_tile<R, HS_MODEL_HH_COBA, FUSED, PUSH>(g, gi, lt, T, w, wmax, no_reset_before, L, spk_total,
                                                              push);
@@ -550,6 +592,38 @@ __global__ __launch_bounds__(256) void hs_k_fused(const DGroup<R> *__restrict__ 
     }
 }
 
+#ifdef HS_CG_MODELS
+/* the reset of code-model neuron i that spiked at step ts: its variables through
+   the model's reset code */
+template <typename R, int C>
+__device__ __forceinline__ void hs_cg_reset_c(const DGroup<R> &g, int i, long long ts)
+{
+    constexpr int NV = HsCg<R, C>::NV;
+    R s[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s[v] = g.var[v][i];
+    HsCg<R, C>::reset(s, (R)g.dt, (double)ts * g.dt, i);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) g.var[v][i] = s[v];
+}
+
+template <typename R>
+__device__ __forceinline__ void hs_cg_reset_one(const DGroup<R> &g, int i, long long ts)
+{
+    const int c = g.model - HS_MODEL_CODE;
+    if (c == 0) hs_cg_reset_c<R, 0>(g, i, ts);
+#if HS_CG_MODELS > 1
+    else if (c == 1) hs_cg_reset_c<R, 1>(g, i, ts);
+#endif
+#if HS_CG_MODELS > 2
+    else if (c == 2) hs_cg_reset_c<R, 2>(g, i, ts);
+#endif
+#if HS_CG_MODELS > 3
+    else if (c == 3) hs_cg_reset_c<R, 3>(g, i, ts);
+#endif
+}
+#endif
+
 /* end of a run at step t_end = ctl->base: apply the last step's accumulated
    effects (fused form) and the resets of its spikers */
 template <typename R, bool FUSED>
@@ -572,13 +646,22 @@ __global__ __launch_bounds__(256) void hs_k_finalize(const DGroup<R> *__restrict
                 }
             }
         }
-        if (g.has_reset && (long long)g.last[i] == t - 1 && t - 1 >= ctl->no_reset_before) g.var[0][i] = g.vr;
+        if (g.has_reset && (long long)g.last[i] == t - 1 && t - 1 >= ctl->no_reset_before) {
+#ifdef HS_CG_MODELS
+            if (HS_IS_CODE_MODEL(g.model)) {
+                hs_cg_reset_one<R>(g, i, t - 1);
+                continue;
+            }
+#endif
+            g.var[0][i] = g.vr;
+        }
     }
 }
 
 __global__ void hs_k_tick(HsCtl *ctl, long long k) { ctl->base += k; }
 __global__ void hs_k_mark_reset(HsCtl *ctl) { ctl->no_reset_before = ctl->base; }
 
+#ifndef __HIPCC_RTC__
 /* ------------------------------------------------------------------------ */
 /* launchers                                                                 */
 /* ------------------------------------------------------------------------ */
@@ -675,6 +758,11 @@ hipError_t hs_launch_finalize(const DGroup<R> *groups, int ng, int total_tiles, 
         hipLaunchKernelGGL((hs_k_finalize<R, false>), dim3(grid), dim3(block), 0, s, groups, ng, total_tiles, ctl);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    return hs_launch_mark_reset(ctl, s);
+}
+
+hipError_t hs_launch_mark_reset(HsCtl *ctl, hipStream_t s)
+{
     hipLaunchKernelGGL(hs_k_mark_reset, dim3(1), dim3(1), 0, s, ctl);
     return hipGetLastError();
 }
@@ -728,3 +816,4 @@ hipError_t hs_launch_tick(HsCtl *ctl, long long k, hipStream_t s)
     template hipError_t hs_launch_pending<R>(const DPath<R> *, int, HsCtl *, hipStream_t);
 HS_INSTANTIATE(double)
 HS_INSTANTIATE(float)
+#endif /* !__HIPCC_RTC__ */

@@ -15,8 +15,19 @@
 #ifndef HS_INTERNAL_H
 #define HS_INTERNAL_H
 
+#ifdef __HIPCC_RTC__
+/* compiled at run time (generated neuron models, hs_codegen.hip): hipRTC
+   provides the HIP device API; the fixed-width integer types come from here */
+typedef signed char int8_t;
+typedef unsigned char uint8_t;
+typedef signed int int32_t;
+typedef unsigned int uint32_t;
+typedef signed long long int64_t;
+typedef unsigned long long uint64_t;
+#else
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 #include "../../include/hs_models.h"
 
 #define HS_MAX_STATE_MONS 4
@@ -79,6 +90,7 @@ struct DGroup {
     int32_t *hist_cnt;
     R k[16];
     R vth, vr;
+    double dt; /* the network's dt (seconds): dt and t of code models */
     unsigned long long seed;
     /* spike monitor */
     int32_t *smon_i;
@@ -192,6 +204,7 @@ struct HsConnSpec {
     int fixed_k;           /* > 0: fixed out-degree K (then p and jump are unused) */
 };
 
+#ifndef __HIPCC_RTC__ /* host launchers */
 /* count pass: cnt[i*D + slot] (int32, zeroed by the caller) for rows [pre_lo, pre_hi);
    *err becomes nonzero if a slot falls outside [0, D) */
 hipError_t hs_launch_conn_count(const HsConnSpec &c, int32_t *cnt, int *err, hipStream_t s);
@@ -285,9 +298,12 @@ hipError_t hs_launch_window(const DGroup<R> *groups, int ng, int total_tiles, co
 template <typename R>
 hipError_t hs_launch_pending(const DPath<R> *paths, int np, HsCtl *ctl, hipStream_t s);
 hipError_t hs_launch_tick(HsCtl *ctl, long long k, hipStream_t s);
+/* resets of spikes before ctl->base are applied (after a finalize) */
+hipError_t hs_launch_mark_reset(HsCtl *ctl, hipStream_t s);
 /* STDP pathways at step ctl->base + koff (plain form, after the delivery of the
    step): on_pre of the bundles due now, then on_post of the targets' spikes */
 template <typename R>
 hipError_t hs_launch_stdp(const DPath<R> *paths, int np, HsCtl *ctl, int koff, int grid, hipStream_t s);
+#endif /* !__HIPCC_RTC__ */
 
 #endif /* HS_INTERNAL_H */

@@ -18,7 +18,9 @@
 #ifndef HS_PHILOX_H
 #define HS_PHILOX_H
 
+#ifndef __HIPCC_RTC__
 #include <stdint.h>
+#endif
 #include "hs_detmath.h"
 
 #define HS_PHILOX_M0 0xD2511F53u

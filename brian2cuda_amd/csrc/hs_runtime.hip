@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/hs.h"
+#include "hs_codegen.h"
 #include "hs_internal.h"
 #include "hs_model_exprs.h"
 
@@ -432,6 +433,7 @@ struct HGroup {
     int32_t *gen_off = nullptr, *gen_idx = nullptr; /* SpikeGeneratorGroup pattern (see DGroup) */
     long long gen_steps = 0, gen_period = 0;
     int rmon = -1; /* PopulationRateMonitor */
+    int code = -1; /* code model: index into hs_net::cg_models */
 };
 
 struct HPath {
@@ -518,6 +520,8 @@ struct hs_net {
     uint64_t events = 0;
     uint64_t spikes = 0;
     std::vector<void *> allocs;
+    std::vector<HsCgModel> cg_models;  /* code models, ids HS_MODEL_CODE + index */
+    const HsCgKernels *cg = nullptr;   /* their compiled engine kernels (from the first run) */
 };
 
 static size_t hs_rsize(const hs_net *net) { return net->prec == HS_F64 ? sizeof(double) : sizeof(float); }
@@ -619,6 +623,48 @@ extern "C" int hs_net_add_group(hs_net *net, int model, int n, const double *par
     std::vector<int32_t> never((size_t)n, HS_NEVER_SPIKED);
     HS_HIP(hipMemcpyAsync(g.last, never.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, g_rt.stream));
     HS_HIP(hipStreamSynchronize(g_rt.stream));
+    net->g.push_back(g);
+    net->desc_dirty = true;
+    if (gid) *gid = (int)net->g.size() - 1;
+    return HS_OK;
+}
+
+extern "C" int hs_net_add_group_code(hs_net *net, int n, int nvars, const char *const *var_names, const char *update,
+                                     const char *threshold, const char *reset, double refractory, int noise,
+                                     uint32_t stream, int *gid)
+{
+    if (!net) return hs_fail(HS_ERR_ARG, "null net");
+    if (net->built) return hs_fail(HS_ERR_STATE, "groups must be added before the first run");
+    if (n <= 0 || nvars < 1 || nvars > HS_MAX_VARS || !var_names)
+        return hs_fail(HS_ERR_ARG, "hs_net_add_group_code: n > 0 and 1 to %d variables", HS_MAX_VARS);
+    if ((int)net->cg_models.size() >= HS_MAX_CODE_MODELS)
+        return hs_fail(HS_ERR_UNSUPPORTED, "at most %d code models per network", HS_MAX_CODE_MODELS);
+    if (!(refractory >= 0.0)) return hs_fail(HS_ERR_ARG, "refractory period %g < 0", refractory);
+    HsCgModel m;
+    for (int v = 0; v < nvars; ++v) {
+        if (!var_names[v]) return hs_fail(HS_ERR_ARG, "variable name %d is null", v);
+        m.names.push_back(var_names[v]);
+    }
+    m.update = update ? update : "";
+    m.threshold = threshold ? threshold : "";
+    m.reset = reset ? reset : "";
+    m.noisy = noise ? 1 : 0;
+    HS_TRY(hs_cg_check(m));
+    HGroup g;
+    g.model = HS_MODEL_CODE + (int)net->cg_models.size();
+    g.code = (int)net->cg_models.size();
+    g.n = n;
+    g.mi = {nvars, 0, m.reset.empty() ? 0 : 1, -1, -1, -1, -1};
+    memset(g.params, 0, sizeof(g.params));
+    memset(g.k, 0, sizeof(g.k));
+    g.ref_steps = hs_ref_steps(refractory, net->dt);
+    g.stream = stream;
+    for (int v = 0; v < nvars; ++v) HS_TRY(hs_dalloc(net, hs_rsize(net) * n, &g.var[v], true));
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * n, (void **)&g.last, false));
+    std::vector<int32_t> never((size_t)n, HS_NEVER_SPIKED);
+    HS_HIP(hipMemcpyAsync(g.last, never.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    net->cg_models.push_back(m);
     net->g.push_back(g);
     net->desc_dirty = true;
     if (gid) *gid = (int)net->g.size() - 1;
@@ -1595,6 +1641,8 @@ static int hs_build(hs_net *net)
                 HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)g.n * g.xi_slots, &g.xi, false));
                 net->noise_ahead = true;
             }
+    /* code models: the engine kernels compiled with them (cached per program) */
+    if (!net->cg_models.empty()) HS_TRY(hs_cg_kernels(net->prec, net->cg_models, &net->cg));
     HsCtl c;
     memset(&c, 0, sizeof(c));
     c.base = net->step;
@@ -1646,6 +1694,7 @@ static int hs_upload_desc_t(hs_net *net)
             d.rmon_stop = M.stop;
         }
         d.vr = g.mi.i_vr >= 0 ? (R)g.params[g.mi.i_vr] : (R)0;
+        d.dt = net->dt;
         d.seed = net->seed;
         if (g.smon >= 0) {
             const HMon &M = net->m[g.smon];
@@ -1870,6 +1919,7 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
     int ms = net->g.empty() ? 0 : hs_kernel_model(net, net->g[0].model); /* one model for all groups: specialised kernels */
     for (const HGroup &g : net->g)
         if (hs_kernel_model(net, g.model) != ms) ms = 0;
+    const HsCgKernels *cg = net->cg; /* code models: the kernels compiled for them (every model dispatched) */
     const int ggrid = std::max(1, std::min(net->gather_tiles, 1 << 20));
     /* where a window's noise is drawn: by extra blocks of its scatter-delivery
        launch (two-launch windows), else one window ahead by the update or window
@@ -1894,8 +1944,12 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
     };
     auto update = [&](int koff, int w, int wmax, int fu) -> int {
         return hs_kt_launch(net, 0, [&]() -> int {
-            HS_HIP(hs_launch_update<R>(dg, ng, net->total_tiles, net->ctl, koff, w, wmax, fu, ms, block,
-                                       hs_update_grid(net), n_noise_ahead, g_rt.stream));
+            if (cg)
+                HS_HIP(hs_cg_launch_update(cg, dg, ng, net->total_tiles, net->ctl, koff, w, wmax, fu, block,
+                                           hs_update_grid(net), n_noise_ahead, g_rt.stream));
+            else
+                HS_HIP(hs_launch_update<R>(dg, ng, net->total_tiles, net->ctl, koff, w, wmax, fu, ms, block,
+                                           hs_update_grid(net), n_noise_ahead, g_rt.stream));
             return HS_OK;
         });
     };
@@ -1905,9 +1959,14 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
             const int w = (int)std::min<int64_t>(net->window, k - j);
             if (net->mode == 4) {
                 HS_TRY(hs_kt_launch(net, 4, [&]() -> int {
-                    HS_HIP(hs_launch_window<R>(dg, ng, net->total_tiles, dp, np, net->ctl, (int)j, w, net->window, ms,
-                                               block, hs_update_grid(net), hs_has_far(net) ? hs_deliver_grid(net) : 0,
-                                               n_noise, g_rt.stream));
+                    const int n_far = hs_has_far(net) ? hs_deliver_grid(net) : 0;
+                    if (cg)
+                        HS_HIP(hs_cg_launch_window(cg, dg, ng, net->total_tiles, dp, np, net->ctl, (int)j, w,
+                                                   net->window, block, hs_update_grid(net), n_far, n_noise,
+                                                   g_rt.stream));
+                    else
+                        HS_HIP(hs_launch_window<R>(dg, ng, net->total_tiles, dp, np, net->ctl, (int)j, w, net->window,
+                                                   ms, block, hs_update_grid(net), n_far, n_noise, g_rt.stream));
                     return HS_OK;
                 }));
             } else {
@@ -1919,8 +1978,12 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
         for (int64_t j = 0; j < k; ++j) {
             if (net->mode == 2) {
                 HS_TRY(hs_kt_launch(net, 3, [&]() -> int {
-                    HS_HIP(hs_launch_fused<R>(dg, ng, net->total_tiles, dp, np, net->ctl, (int)j, ms, block,
-                                              hs_update_grid(net), hs_fused_deliver_blocks(net), g_rt.stream));
+                    if (cg)
+                        HS_HIP(hs_cg_launch_fused(cg, dg, ng, net->total_tiles, dp, np, net->ctl, (int)j, block,
+                                                  hs_update_grid(net), hs_fused_deliver_blocks(net), g_rt.stream));
+                    else
+                        HS_HIP(hs_launch_fused<R>(dg, ng, net->total_tiles, dp, np, net->ctl, (int)j, ms, block,
+                                                  hs_update_grid(net), hs_fused_deliver_blocks(net), g_rt.stream));
                     return HS_OK;
                 }));
             } else {
@@ -2067,7 +2130,10 @@ extern "C" int hs_net_run(hs_net *net, int64_t nsteps)
     HS_HIP(hipEventRecord(net->ev1, g_rt.stream));
     const int block = (int)net->opt[0];
     const bool fused = net->engine == HS_ENGINE_FUSED;
-    if (net->prec == HS_F64)
+    if (net->cg)
+        HS_HIP(hs_cg_launch_finalize(net->cg, net->dgroups, (int)net->g.size(), net->total_tiles, net->ctl, fused,
+                                     block, g_rt.stream));
+    else if (net->prec == HS_F64)
         HS_HIP(hs_launch_finalize<double>((const DGroup<double> *)net->dgroups, (int)net->g.size(), net->total_tiles,
                                           net->ctl, fused, block, g_rt.stream));
     else

@@ -29,6 +29,7 @@ class GroupSpec:
     params: list
     init: dict = field(default_factory=dict)  # var index -> array (or scalar)
     spikes: Optional[tuple] = None            # spike generator: (indices, steps)
+    code: object = None                       # a codegen.CodeModel: the group's model is code (model unused)
 
 
 @dataclass
@@ -172,6 +173,123 @@ def cuba(N=4000, seed=1234, p=0.02, delay_steps=1, dt=0.1 * ms, record=100, we=1
         spec.spike_monitors = [0]
         if record:
             spec.state_monitors = [(0, 0, 0, min(record, N))]
+    return spec
+
+
+CUBA_EQS = """
+dv/dt  = (ge+gi-(v-El))/taum : volt (unless refractory)
+dge/dt = -ge/taue : volt
+dgi/dt = -gi/taui : volt
+"""
+
+
+def code_cuba(N=4000, seed=1234, p=0.02, delay_steps=1, dt=0.1 * ms, method=None, we=1.62 * mV, wi=-9 * mV,
+              monitors=True):
+    """C1 with the model given as code: examples/cuba.py's equations, threshold
+    and reset through the code generator (brian2cuda_amd.codegen, method None:
+    'exact', as Brian2 picks for this linear system), same connectivity as cuba()."""
+    from .codegen import neuron_model
+    spec = cuba(N=N, seed=seed, p=p, delay_steps=delay_steps, dt=dt, record=0, we=we, wi=wi, monitors=monitors)
+    ns = dict(El=-49 * mV, taum=20 * ms, taue=5 * ms, taui=10 * ms, Vt=-50 * mV, Vr=-60 * mV)
+    model, names = neuron_model(CUBA_EQS, "v > Vt", "v = Vr", 5 * ms, method, ns, dt=dt)
+    g = spec.groups[0]
+    spec.groups[0] = GroupSpec(0, g.n, [], g.init, code=model)
+    spec.name = "code_cuba"
+    return spec
+
+
+ADEX_EQS = """
+dv/dt = (gL*(EL - v) + gL*DeltaT*exp((v - VT)/DeltaT) + (ge + gr)*(Ee - v) + gi*(Ei - v) - w + I)/C : volt
+dw/dt = (a*(v - EL) - w)/tauw : amp
+dge/dt = -ge/taue : siemens
+dgi/dt = -gi/taui : siemens
+dgr/dt = -gr/taue : siemens
+I : amp (constant)
+"""
+NOISY_LIF_EQS = """
+dV/dt = (-V + mu + sigma*sqrt(tau)*xi)/tau : volt
+dg/dt = -g/taug : volt
+"""
+
+
+def code_mix(N=3000, seed=7, dt=0.1 * ms, method="rk4", monitors=True):
+    """Three code models in one network with a built-in Poisson input: adaptive
+    exponential integrate-and-fire neurons (AdEx, conductance inputs, reset
+    'v = Vr; w += b'), noise-driven LIF neurons (Euler-Maruyama, '(unless
+    refractory)'-free, a second variable) and a CUBA LIF inhibitory population
+    (exact), with random delays of 1 to 30 steps between them.  Each target
+    variable has one incoming pathway of one weight, so the effects a neuron
+    receives in a step are equal and their sum does not depend on the order the
+    device adds them in."""
+    from .codegen import neuron_model
+    rng = np.random.default_rng(seed)
+    ne, nn, ni, npo = int(0.5 * N), int(0.25 * N), N - int(0.5 * N) - int(0.25 * N), 500
+    adex, _ = neuron_model(ADEX_EQS, "v > -40*mV", "v = Vr\nw += b", 2 * ms, method,
+                           dict(gL=30 * nS, EL=-70.6 * mV, DeltaT=2 * mV, VT=-50.4 * mV, C=281e-12, a=4 * nS,
+                                tauw=144 * ms, b=0.0805e-9, Vr=-70.6 * mV, Ee=0 * mV, Ei=-80 * mV, taue=5 * ms,
+                                taui=10 * ms), dt=dt)
+    noisy, _ = neuron_model(NOISY_LIF_EQS.replace("-V + mu", "-V + mu + g"), "V > theta", "V = Vr", 2 * ms, "euler",
+                            dict(mu=18 * mV, sigma=3 * mV, tau=20 * ms, taug=5 * ms, theta=20 * mV, Vr=10 * mV),
+                            dt=dt, stream=3)
+    inh, _ = neuron_model(CUBA_EQS, "v > Vt", "v = Vr", 1 * ms, None,
+                          dict(El=-49 * mV, taum=10 * ms, taue=5 * ms, taui=10 * ms, Vt=-50 * mV, Vr=-60 * mV), dt=dt)
+    groups = [
+        GroupSpec(0, ne, [], {0: -70.6 * mV + 10 * mV * rng.random(ne), 5: 0.4e-9 + 0.2e-9 * rng.random(ne)},
+                  code=adex),
+        GroupSpec(0, nn, [], {0: 10 * mV + 10 * mV * rng.random(nn)}, code=noisy),
+        GroupSpec(0, ni, [], {0: -60 * mV + 10 * mV * rng.random(ni)}, code=inh),
+        GroupSpec(HS_MODEL_POISSON, npo, [20.0, 5.0]),
+    ]
+
+    def rnd(src_n, tgt_n, p, src, tgt, var, w, dmax):
+        pre, post = bernoulli_connect(rng, src_n, tgt_n, p)
+        d = rng.integers(1, dmax + 1, pre.size).astype(np.int32)
+        return PathSpec(src, tgt, var, pre, post, delay_steps=d, wmode=HS_W_SCALAR, wscalar=w)
+
+    paths = [
+        rnd(npo, ne, 0.05, 3, 0, 2, 3 * nS, 5),       # Poisson -> AdEx ge
+        rnd(ne, ne, 0.02, 0, 0, 4, 1 * nS, 30),       # AdEx -> AdEx gr
+        rnd(ni, ne, 0.05, 2, 0, 3, 4 * nS, 10),       # inh -> AdEx gi
+        rnd(ne, nn, 0.02, 0, 1, 1, 0.5 * mV, 20),     # AdEx -> noisy g
+        rnd(nn, ni, 0.05, 1, 2, 1, 1.0 * mV, 15),     # noisy -> inh ge
+        rnd(ni, ni, 0.05, 2, 2, 2, -2.0 * mV, 8),     # inh -> inh gi
+    ]
+    spec = NetSpec("code_mix", dt, seed, groups, paths, info={"N": N})
+    if monitors:
+        spec.spike_monitors = [0, 1, 2]
+    return spec
+
+
+HH_EQS = """
+dv/dt = (gl*(El-v)+ge*(Ee-v)+gi*(Ei-v)-gNa*(m*m*m)*h*(v-ENa)-gK*(n*n*n*n)*(v-EK))/Cm : volt
+dm/dt = alpha_m*(1-m)-beta_m*m : 1
+dn/dt = alpha_n*(1-n)-beta_n*n : 1
+dh/dt = alpha_h*(1-h)-beta_h*h : 1
+dge/dt = -ge*(1./taue) : siemens
+dgi/dt = -gi*(1./taui) : siemens
+alpha_m = 0.32*(mV**-1)*(13.*mV-v+VT)/(exp((13.*mV-v+VT)/(4.*mV))-1.)/ms : Hz
+beta_m = 0.28*(mV**-1.)*(v-VT-40.*mV)/(exp((v-VT-40.*mV)/(5.*mV))-1.)/ms : Hz
+alpha_h = 0.128*exp((17.*mV-v+VT)/(18.*mV))/ms : Hz
+beta_h = 4./(1.+exp((40.*mV-v+VT)/(5.*mV)))/ms : Hz
+alpha_n = 0.032*(mV**-1.)*(15.*mV-v+VT)/(exp((15.*mV-v+VT)/(5.*mV))-1.)/ms : Hz
+beta_n = .5*exp((10.*mV-v+VT)/(40.*mV))/ms : Hz
+"""
+
+
+def code_cobahh(N=4000, seed=1234, method="exponential_euler", **kw):
+    """C2 with the model given as code: examples/cobahh.py's equations
+    (exponential Euler, as the example asks) through the code generator, same
+    connectivity and initial state as cobahh()."""
+    from .codegen import neuron_model
+    spec = cobahh(N=N, seed=seed, **kw)
+    p = cobahh_params()
+    ns = dict(Cm=p[0], gl=p[1], El=p[2], EK=p[3], ENa=p[4], gNa=p[5], gK=p[6], VT=p[7], Ee=p[8], Ei=p[9],
+              taue=p[10], taui=p[11])
+    model, names = neuron_model(HH_EQS, "v > -20*mV", None, p[13], method, ns, dt=spec.dt)
+    assert names == ["v", "m", "n", "h", "ge", "gi"]
+    g = spec.groups[0]
+    spec.groups[0] = GroupSpec(0, g.n, [], g.init, code=model)
+    spec.name = "code_cobahh"
     return spec
 
 
@@ -382,7 +500,7 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
     for k, v in (options or {}).items():  # before the pathways: the delivery mode is chosen per pathway
         net.set_option(k, v)
     for g in spec.groups:
-        gid = net.add_group(g.model, g.n, g.params)
+        gid = net.add_group_code(g.n, g.code) if g.code is not None else net.add_group(g.model, g.n, g.params)
         for var, val in g.init.items():
             net.set_state(gid, var, val)
         if g.spikes is not None:

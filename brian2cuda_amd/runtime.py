@@ -17,6 +17,7 @@ HS_ENGINE_PLAIN, HS_ENGINE_FUSED = 0, 1
 HS_W_SCALAR, HS_W_SYNAPSE, HS_W_PRE = 0, 1, 2
 
 HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_HH_COBA, HS_MODEL_SPIKEGEN, HS_MODEL_POISSON = 1, 2, 3, 4, 5
+HS_MODEL_CODE, HS_MAX_CODE_MODELS = 64, 4  # code models: ids HS_MODEL_CODE + k (hs_net_add_group_code)
 MODEL_NVARS = {HS_MODEL_LIF_CUBA: 3, HS_MODEL_LIF_NOISE: 1, HS_MODEL_HH_COBA: 6, HS_MODEL_SPIKEGEN: 1,
                HS_MODEL_POISSON: 1}
 MODEL_NPARAMS = {HS_MODEL_LIF_CUBA: 7, HS_MODEL_LIF_NOISE: 7, HS_MODEL_HH_COBA: 14, HS_MODEL_SPIKEGEN: 1,
@@ -202,6 +203,20 @@ class Net:
         gid = C.c_int()
         call("hs_net_add_group", self.h, int(model), int(n), _f64(p), p.size, C.byref(gid))
         self.groups.append((model, n))
+        return gid.value
+
+    def add_group_code(self, n, model):
+        """A group of n neurons whose model is code (a brian2cuda_amd.codegen.CodeModel:
+        variable names, update statements, threshold expression, reset
+        statements, refractory period, noise); compiled for the device with the
+        network's other code models on its first run.  Returns the group id."""
+        names = [s.encode() for s in model.names]
+        arr = (C.c_char_p * len(names))(*names)
+        gid = C.c_int()
+        call("hs_net_add_group_code", self.h, int(n), len(names), arr, model.update.encode(),
+             (model.threshold or "").encode(), (model.reset or "").encode(), float(model.refractory),
+             int(bool(model.noise)), int(model.stream), C.byref(gid))
+        self.groups.append((HS_MODEL_CODE + sum(1 for m, _ in self.groups if m >= HS_MODEL_CODE), n))
         return gid.value
 
     def set_state(self, gid, var, values):

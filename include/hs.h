@@ -162,6 +162,39 @@ void hs_net_destroy(hs_net *net);
 /* Add a NeuronGroup of n neurons with a model of hs_models.h. */
 int hs_net_add_group(hs_net *net, int model, int n, const double *params, int nparams, int *gid);
 
+/*
+ * Add a NeuronGroup of n neurons whose model is given as code: what the
+ * reference's code generator emits for any NeuronGroup's equations, threshold
+ * and reset (cuda_generator.py:200-808 into the stateupdate.cu,
+ * common_group.cu:108-160, threshold.cu:19-45 and reset.cu:3-27 templates).
+ * The group has nvars state variables named var_names[0..nvars-1] (variable v
+ * of hs_net_set_state / hs_net_get_state / monitors is var_names[v]).  Per
+ * neuron and step, in the network's precision R (double or float), with the
+ * variables as local R values of those names and these in scope:
+ *   dt (R, seconds), t (double, the step's time), i (int, the neuron index),
+ *   xi (R, a standard normal draw of (seed, stream, i, step) when noise != 0,
+ *   else 0), not_refractory (int: the neuron's last spike is at least
+ *   `refractory` seconds back, Brian2's timestep rule)
+ *   update     C statements advancing the variables by one step
+ *   threshold  C expression evaluated after update: the neuron spikes when it
+ *              holds and not_refractory (Brian2 adds 'and not_refractory'
+ *              when a refractory period is set); "" or NULL: never
+ *   reset      C statements run for a neuron that spiked (t: the spike's
+ *              time), after the effects of its spike step were applied, as
+ *              Brian2's resetter runs after the synapses; "" or NULL: none
+ * The code may call exp, log, log10, sqrt, abs/fabs, floor, ceil, sign and
+ * clip(x, lo, hi) (the library's deterministic exp and log, IEEE sqrt: the same
+ * numbers as the host oracle) and use the type R; no preprocessor lines.  The
+ * engine kernels are compiled with the network's code models (hipRTC) on its
+ * first run, once per distinct program in the process; a compile error fails
+ * that run with HS_ERR_ARG and the compiler's messages.  At most
+ * HS_MAX_CODE_MODELS code models per network; the group's model id is
+ * HS_MODEL_CODE + (its index among them).
+ */
+int hs_net_add_group_code(hs_net *net, int n, int nvars, const char *const *var_names, const char *update,
+                          const char *threshold, const char *reset, double refractory, int noise, uint32_t stream,
+                          int *gid);
+
 /* Set / get a state variable (host arrays of n doubles). */
 int hs_net_set_state(hs_net *net, int gid, int var, const double *values);
 int hs_net_get_state(hs_net *net, int gid, int var, double *values);

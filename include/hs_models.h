@@ -97,7 +97,20 @@
 #define HS_POISSON_NPARAMS_MAX 2
 #define HS_POISSON_RATES 0
 
-#define HS_MAX_VARS 6
+/*
+ * HS_MODEL_CODE -- a group whose model is given as code (hs_net_add_group_code):
+ * the statements of its state update, its threshold condition and its reset,
+ * as the reference's code generator emits them for any NeuronGroup
+ * (common_group.cu:108-160 via stateupdate.cu, threshold.cu:19-45,
+ * reset.cu:3-27), compiled for the device when the network is built.  A
+ * network's code models get the ids HS_MODEL_CODE + 0, 1, ... in the order
+ * they are added, at most HS_MAX_CODE_MODELS of them.
+ */
+#define HS_MODEL_CODE 64
+#define HS_MAX_CODE_MODELS 4
+#define HS_IS_CODE_MODEL(m) ((m) >= HS_MODEL_CODE && (m) < HS_MODEL_CODE + HS_MAX_CODE_MODELS)
+
+#define HS_MAX_VARS 8
 
 /*
  * Spike-timing-dependent plasticity on a pathway (hs_net_pathway_stdp), the

@@ -64,6 +64,7 @@ def lib():
                                       C.c_uint64, C.c_int64]),
         "ref_net_create": (V, [C.c_int, C.c_int, C.c_double, C.c_uint64]),
         "ref_net_add_group": (C.c_int, [V, C.c_int, C.c_int, DP, C.c_int]),
+        "ref_net_add_group_code": (C.c_int, [V, C.c_int, C.c_int, V, V, C.c_double, C.c_int, C.c_uint32]),
         "ref_net_set_state": (C.c_int, [V, C.c_int, C.c_int, DP]),
         "ref_net_get_state": (C.c_int, [V, C.c_int, C.c_int, DP]),
         "ref_net_get_lastspike": (C.c_int, [V, C.c_int, DP]),
@@ -179,6 +180,98 @@ def stateupdate(model, precision, arrays, last, params, dt, seed, step):
     return arrays
 
 
+# ---- code models ------------------------------------------------------------
+# A code model's statements (hs_net_add_group_code) compiled for the host: the
+# same wrapping (variables as locals of the network's precision, dt, t, i, xi,
+# not_refractory in scope; the threshold evaluated after the update) and the same
+# math (the deterministic exp and log of hs_detmath.h, IEEE sqrt), with g++ under
+# the oracle's IEEE flags, so the host and device results agree bit for bit.
+_CG_MATH = r"""
+#include <math.h>
+#include "hs_detmath.h"
+static inline double hs_cg_exp(double x) { return hs_exp(x); }
+static inline float hs_cg_exp(float x) { return hs_expf(x); }
+static inline double hs_cg_exp(int x) { return hs_exp((double)x); }
+static inline double hs_cg_log(double x) { return hs_log(x); }
+static inline float hs_cg_log(float x) { return hs_logf(x); }
+static inline double hs_cg_log(int x) { return hs_log((double)x); }
+static inline double hs_cg_log10(double x) { return hs_log(x) / 2.302585092994046; }
+static inline float hs_cg_log10(float x) { return hs_logf(x) / 2.3025851f; }
+static inline double hs_cg_sqrt(double x) { return hs_sqrt(x); }
+static inline float hs_cg_sqrt(float x) { return hs_sqrtf(x); }
+static inline double hs_cg_sqrt(int x) { return hs_sqrt((double)x); }
+static inline double hs_cg_abs(double x) { return x < 0.0 ? -x : x; }
+static inline float hs_cg_abs(float x) { return x < 0.0f ? -x : x; }
+static inline int hs_cg_abs(int x) { return x < 0 ? -x : x; }
+static inline double hs_cg_floor(double x) { return floor(x); }
+static inline float hs_cg_floor(float x) { return floorf(x); }
+static inline double hs_cg_ceil(double x) { return ceil(x); }
+static inline float hs_cg_ceil(float x) { return ceilf(x); }
+template <typename T> static inline T hs_cg_sign(T x) { return (T)((x > (T)0) - (x < (T)0)); }
+template <typename A, typename B, typename C>
+static inline auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo + hi)
+{
+    typedef decltype(x + lo + hi) T;
+    const T v = (T)x;
+    return v < (T)lo ? (T)lo : (v > (T)hi ? (T)hi : v);
+}
+#define exp hs_cg_exp
+#define log hs_cg_log
+#define log10 hs_cg_log10
+#define sqrt hs_cg_sqrt
+#define abs hs_cg_abs
+#define fabs hs_cg_abs
+#define floor hs_cg_floor
+#define ceil hs_cg_ceil
+#define sign hs_cg_sign
+#define clip hs_cg_clip
+"""
+_cg_libs = {}
+
+
+def code_model_source(model, precision):
+    """The host program of one code model: extern "C" hs_cg_step / hs_cg_reset."""
+    R = "double" if precision == 64 else "float"
+    load = "".join(f"    R {n} = hs_s[{k}];\n" for k, n in enumerate(model.names))
+    store = "".join(f"    hs_s[{k}] = {n};\n" for k, n in enumerate(model.names))
+    thr = f"({model.threshold}) ? 1 : 0" if model.threshold else "0"
+    return (_CG_MATH + f"typedef {R} R;\n"
+            f'extern "C" int hs_cg_step(R *hs_s, int not_refractory, R xi, R dt, double t, int i)\n{{\n'
+            f"    (void)not_refractory; (void)xi; (void)dt; (void)t; (void)i;\n{load}"
+            f"    {{\n{model.update}\n    }}\n{store}    return {thr};\n}}\n"
+            f'extern "C" void hs_cg_reset(R *hs_s, R dt, double t, int i)\n{{\n'
+            f"    (void)dt; (void)t; (void)i;\n{load}    {{\n{model.reset or ''}\n    }}\n{store}}}\n")
+
+
+def code_model_functions(model, precision):
+    """(step, reset) function addresses of a code model compiled for the host
+    (reset None when the model has no reset); compiled once per source."""
+    import hashlib
+    import tempfile
+    src = code_model_source(model, precision)
+    key = hashlib.sha1(src.encode()).hexdigest()[:20]
+    if key not in _cg_libs:
+        d = os.path.join(tempfile.gettempdir(), f"hs_oracle_cg_{os.getuid()}")
+        os.makedirs(d, exist_ok=True)
+        so = os.path.join(d, key + ".so")
+        if not os.path.exists(so):
+            cpp = os.path.join(d, key + ".cpp")
+            with open(cpp, "w") as f:
+                f.write(src)
+            tmp = so + f".{os.getpid()}.tmp"
+            r = subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+                                "-I", os.path.join(HERE, "..", "brian2cuda_amd", "csrc"), "-o", tmp, cpp],
+                               capture_output=True, text=True)
+            if r.returncode != 0:
+                raise ValueError("code model does not compile for the host:\n" + r.stderr[-3000:])
+            os.replace(tmp, so)
+        _cg_libs[key] = C.CDLL(so)
+    L = _cg_libs[key]
+    step = C.cast(L.hs_cg_step, C.c_void_p).value
+    reset = C.cast(L.hs_cg_reset, C.c_void_p).value if model.reset else None
+    return step, reset
+
+
 class RefNet:
     """Serial cpp_standalone-semantics network (SEQ) or the fused form's semantics (ACC)."""
 
@@ -187,6 +280,7 @@ class RefNet:
         self.h = self.L.ref_net_create(precision, semantics, dt, seed)
         if not self.h:
             raise ValueError("bad precision")
+        self.precision = precision
         self.sizes = []
 
     def close(self):
@@ -205,6 +299,17 @@ class RefNet:
         g = self.L.ref_net_add_group(self.h, model, n, p.ctypes.data_as(C.POINTER(C.c_double)), p.size)
         if g < 0:
             raise ValueError("bad group")
+        self.sizes.append(n)
+        return g
+
+    def add_group_code(self, n, model):
+        """A group whose model is code (brian2cuda_amd.codegen.CodeModel), its
+        statements compiled for the host (code_model_functions)."""
+        step, reset = code_model_functions(model, self.precision)
+        g = self.L.ref_net_add_group_code(self.h, n, len(model.names), step, reset, float(model.refractory),
+                                          int(bool(model.noise)), int(model.stream))
+        if g < 0:
+            raise ValueError("bad code group")
         self.sizes.append(n)
         return g
 
@@ -356,7 +461,8 @@ def build_ref(spec, precision=64, semantics=SEQ, threads=1):
     if threads > 1:
         net.set_threads(threads)
     for g in spec.groups:
-        gid = net.add_group(g.model, g.n, g.params)
+        gid = net.add_group_code(g.n, g.code) if getattr(g, "code", None) is not None else \
+            net.add_group(g.model, g.n, g.params)
         for var, val in g.init.items():
             net.set_state(gid, var, val)
         if getattr(g, "spikes", None) is not None:

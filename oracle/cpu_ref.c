@@ -292,6 +292,10 @@ typedef struct {
     int64_t *gen_off;
     int32_t *gen_idx;
     int64_t gen_steps, gen_period;
+    /* code model (model HS_MODEL_CODE + k): the step and reset functions of the
+       network's precision, compiled for the host from the model's code */
+    void *cg_step, *cg_reset;
+    int cg_noisy;
 } ref_group;
 
 typedef struct {
@@ -435,6 +439,39 @@ int ref_net_add_group(ref_net *net, int model, int n, const double *params, int 
             if (net->prec == HS_F64) ((double *)g->var[0])[i] = params[0];
             else ((float *)g->var[0])[i] = (float)params[0];
         }
+    g->lastspike = (double *)malloc(sizeof(double) * n);
+    for (int i = 0; i < n; ++i) g->lastspike[i] = -1e4; /* never spiked */
+    g->spk = (int32_t *)malloc(sizeof(int32_t) * n);
+    g->pend = (int32_t *)malloc(sizeof(int32_t) * n);
+    return net->ng++;
+}
+
+/*
+ * A group whose model is code (hs_net_add_group_code): step(s, not_ref, xi, dt,
+ * t, i) advances the variables in s and returns the threshold condition;
+ * reset(s, dt, t_spike, i).  The pointers take the network's precision
+ * (double* or float* state, R xi and dt).
+ */
+int ref_net_add_group_code(ref_net *net, int n, int nvars, void *step_fn, void *reset_fn, double refractory,
+                           int noise, uint32_t stream)
+{
+    if (net->ng >= REF_MAX_GROUPS || n <= 0 || nvars < 1 || nvars > HS_MAX_VARS || !step_fn) return -1;
+    int ncode = 0;
+    for (int q = 0; q < net->ng; ++q) ncode += HS_IS_CODE_MODEL(net->g[q].model);
+    if (ncode >= HS_MAX_CODE_MODELS || !(refractory >= 0.0)) return -1;
+    ref_group *g = &net->g[net->ng];
+    memset(g, 0, sizeof(*g));
+    g->model = HS_MODEL_CODE + ncode;
+    g->n = n;
+    g->nvars = nvars;
+    g->refractory = refractory;
+    g->ref_steps = hs_timestep(refractory, net->dt);
+    g->has_reset = reset_fn != NULL;
+    g->stream = stream;
+    g->cg_step = step_fn;
+    g->cg_reset = reset_fn;
+    g->cg_noisy = noise != 0;
+    for (int v = 0; v < nvars; ++v) g->var[v] = calloc((size_t)n, rsz(net));
     g->lastspike = (double *)malloc(sizeof(double) * n);
     for (int i = 0; i < n; ++i) g->lastspike[i] = -1e4; /* never spiked */
     g->spk = (int32_t *)malloc(sizeof(int32_t) * n);

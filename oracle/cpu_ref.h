@@ -42,6 +42,8 @@ ref_net *ref_net_create(int precision, int semantics, double dt, uint64_t seed);
    before the first step (-2: built without OpenMP) */
 int ref_net_set_threads(ref_net *net, int nthreads);
 int ref_net_add_group(ref_net *net, int model, int n, const double *params, int nparams);
+int ref_net_add_group_code(ref_net *net, int n, int nvars, void *step_fn, void *reset_fn, double refractory,
+                           int noise, uint32_t stream);
 int ref_net_set_state(ref_net *net, int gid, int var, const double *values);
 int ref_net_get_state(ref_net *net, int gid, int var, double *out);
 int ref_net_get_lastspike(ref_net *net, int gid, double *out);

@@ -1,0 +1,157 @@
+"""CPU tests of the code generator (brian2cuda_amd.codegen: Brian2 equations ->
+the statements of hs_net_add_group_code) run through the oracle's code models
+(the same statements compiled for the host), pinned against what the
+mathematics fixes:
+
+  - each integration method on dv/dt = (a - v)/tau is the recurrence
+    v <- a + (v - a) q with q = 1 - h (euler), 1 - h + h^2/2 (rk2),
+    1 - h + h^2/2 - h^3/6 + h^4/24 (rk4), exp(-h) (exponential_euler, exact);
+  - 'exact' on the coupled CUBA system equals repeated products with the
+    matrix exponential, and the built-in CUBA model (its own closed form);
+  - Euler-Maruyama equals a numpy recurrence over the oracle's normal draws;
+  - threshold, reset (two variables) and refractoriness with and without
+    '(unless refractory)' give the spike steps derived by hand;
+  - what the generator cannot express raises NotImplementedError.
+"""
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+import oracle
+from brian2cuda_amd import codegen as cg
+from brian2cuda_amd import networks as nw
+
+DT = 1e-4
+
+
+def _run(model, init, steps, prec=64, sem=oracle.SEQ, n=None, monitor=False):
+    n = n or len(next(iter(init.values())))
+    spec = nw.NetSpec("t", DT, 5, [nw.GroupSpec(0, n, [], init, code=model)], [])
+    if monitor:
+        spec.spike_monitors = [0]
+    ref, sm, _ = oracle.build_ref(spec, prec, sem)
+    ref.run(steps)
+    return ref, sm
+
+
+@pytest.mark.parametrize("method,q", [
+    ("euler", lambda h: 1 - h), ("rk2", lambda h: 1 - h + h * h / 2),
+    ("rk4", lambda h: 1 - h + h ** 2 / 2 - h ** 3 / 6 + h ** 4 / 24),
+    ("exponential_euler", lambda h: np.exp(-h)), ("exact", lambda h: np.exp(-h))])
+def test_methods_follow_their_recurrence(method, q):
+    a, tau = 0.7, 7e-3
+    m, names = cg.neuron_model("dv/dt = (a - v)/tau : 1", None, None, 0.0, method, dict(a=a, tau=tau), dt=DT)
+    v0 = np.array([-1.0, 0.0, 0.5, 2.0])
+    ref, _ = _run(m, {0: v0}, 500)
+    expect = a + (v0 - a) * q(DT / tau) ** 500
+    np.testing.assert_allclose(ref.get_state(0, 0), expect, rtol=1e-11, atol=1e-14)
+
+
+def test_exact_coupled_system_and_the_builtin_cuba():
+    ns = dict(El=-49e-3, taum=20e-3, taue=5e-3, taui=10e-3, Vt=1.0, Vr=-60e-3)
+    m, names = cg.neuron_model(nw.CUBA_EQS, "v > Vt", "v = Vr", 0.0, None, ns, dt=DT)
+    assert names == ["v", "ge", "gi"]
+    x0 = np.array([[-55e-3, -52e-3], [2e-3, 0.0], [-1e-3, -3e-3]])
+    ref, _ = _run(m, {0: x0[0], 1: x0[1], 2: x0[2]}, 400)
+    # dv/dt = (ge + gi - v + El)/taum, dge/dt = -ge/taue, dgi/dt = -gi/taui, augmented by the constant
+    M = np.array([[-1 / 20e-3, 1 / 20e-3, 1 / 20e-3, -49e-3 / 20e-3],
+                  [0, -1 / 5e-3, 0, 0], [0, 0, -1 / 10e-3, 0], [0, 0, 0, 0]])
+    P = np.linalg.matrix_power(expm(M * DT), 400)
+    X = P @ np.vstack([x0, np.ones((1, 2))])
+    for v in range(3):
+        np.testing.assert_allclose(ref.get_state(0, v), X[v], rtol=1e-10, atol=1e-15)
+    # the built-in CUBA model (a closed form of its own) on the same states
+    b = nw.cuba(N=2, seed=1, we=0.0, wi=0.0, record=0, monitors=False)
+    b.groups[0].params[1] = 1.0
+    b.groups[0].init = {0: x0[0], 1: x0[1], 2: x0[2]}
+    rb, _, _ = oracle.build_ref(b, 64, oracle.SEQ)
+    rb.run(400)
+    for v in range(3):
+        np.testing.assert_allclose(ref.get_state(0, v), rb.get_state(0, v), rtol=0, atol=1e-13)
+
+
+def test_euler_maruyama_matches_the_draws():
+    mu, sigma, tau = 0.02, 0.004, 0.02
+    m, _ = cg.neuron_model("dV/dt = (-V + mu + sigma*sqrt(tau)*xi)/tau : volt", None, None, 0.0, None,
+                           dict(mu=mu, sigma=sigma, tau=tau), stream=4)
+    assert m.noise and m.stream == 4
+    n, steps = 3, 200
+    ref, _ = _run(m, {0: np.zeros(n)}, steps)
+    v = np.zeros(n)
+    for s in range(steps):
+        xi = np.array([oracle.randn(5, 4, i, s) for i in range(n)])
+        v = v + DT * (mu - v) / tau + sigma * np.sqrt(tau) / tau * np.sqrt(DT) * xi
+    np.testing.assert_allclose(ref.get_state(0, 0), v, rtol=1e-10, atol=1e-15)
+
+
+@pytest.mark.parametrize("unless,period,first", [(False, 50, 40), (True, 90, 40)])
+def test_threshold_reset_and_refractoriness(unless, period, first):
+    """dv/dt = (I - v)/tau, euler, I = 3, h = dt/tau = 0.01: from v = 0 the
+    threshold 1 is crossed after 41 updates (first spike at step 40).  With a
+    5 ms (50-step) refractory period v keeps integrating and the next spike
+    waits for step 90 (period 50); held by '(unless refractory)' it restarts
+    from 0 at step 90 and spikes 41 updates later (period 90).  The reset also
+    counts spikes in a second variable."""
+    eqs = "dv/dt = (I - v)/tau : 1" + (" (unless refractory)" if unless else "") + "\nc : 1\nI : 1"
+    m, names = cg.neuron_model(eqs, "v > 1", "v = 0\nc += 1", 5e-3, "euler", dict(tau=10e-3))
+    assert names == ["v", "c", "I"]
+    ref, sm = _run(m, {0: np.zeros(1), 2: np.full(1, 3.0)}, 1000, monitor=True)
+    i, t = ref.spikes(sm[0])
+    expect = np.arange(first, 1000, period)
+    np.testing.assert_array_equal(t, expect)
+    assert ref.get_state(0, 1)[0] == expect.size
+
+
+@pytest.mark.parametrize("prec,sem", [(64, oracle.SEQ), (32, oracle.ACC)])
+def test_code_mix_runs_in_both_semantics(prec, sem):
+    spec = nw.code_mix(N=1200, seed=2)
+    ref, sm, _ = oracle.build_ref(spec, prec, sem)
+    ref.run(600)
+    assert all(len(ref.spikes(s)[0]) > 0 for s in sm)
+    assert np.all(np.isfinite(ref.get_state(0, 0)))
+
+
+@pytest.mark.parametrize("eqs,kw,msg", [
+    ("dv/dt = sin(v)/tau : 1", {}, "function"),
+    ("dv/dt = (xi_1 + xi_2)/tau : 1", {}, "more than one noise"),
+    ("dv/dt = v*xi/tau : 1", {"method": "euler"}, "additive"),
+    ("dv/dt = v*v/tau : 1", {"method": "exponential_euler"}, "not linear"),
+    ("dv/dt = -v/tau + xi/tau : 1", {"method": "rk4"}, "no noise"),
+    ("dv/dt = -v/foo : 1", {}, "no value"),
+    ("dv/dt = -v/tau : 1 (event-driven)", {}, "flag"),
+    ("dv/dt = -v/tau : 1", {"method": "heun"}, "not supported"),
+    ("dv/dt = v*t/tau : 1", {"method": "exact"}, "linear system"),
+])
+def test_unsupported_forms_are_refused(eqs, kw, msg):
+    with pytest.raises(NotImplementedError, match=msg):
+        cg.neuron_model(eqs, None, None, 0.0, kw.get("method"), dict(tau=0.01), dt=DT)
+
+
+def test_units_subexpressions_and_printing():
+    m, names = cg.neuron_model("""
+        dv/dt = (I_syn - v)/(10*ms) : volt
+        I_syn = g*(E - v) : volt
+        g : 1
+    """, "v > -40*mV and v < 1*volt", "v = -65*mV", 2 * 1e-3, "euler", dict(E=0.0))
+    assert names == ["v", "g"]
+    # I_syn substituted (E = 0 drops out), 10*ms folded: -v/tau - g*v/tau with 1/tau = 100
+    assert "I_syn" not in m.update and "(((R)-100.0)*g*v)" in m.update
+    assert "&&" in m.threshold and "((R)-0.04)" in m.threshold
+    assert m.reset == "v = ((R)-0.065);"
+
+
+def test_code_cobahh_matches_the_builtin_model():
+    """examples/cobahh.py's equations through the generator (exponential Euler)
+    against the built-in HH model on the same network: the same 3,949 spikes
+    over 300 steps and states within 1e-11 of each other (FP64)."""
+    a = nw.code_cobahh(N=4000, seed=2)
+    b = nw.cobahh(N=4000, seed=2)
+    ra, sa, _ = oracle.build_ref(a, 64, oracle.SEQ)
+    rb, sb, _ = oracle.build_ref(b, 64, oracle.SEQ)
+    ra.run(300)
+    rb.run(300)
+    (ia, ta), (ib, tb) = ra.spikes(sa[0]), rb.spikes(sb[0])
+    assert len(ia) > 3000 and np.array_equal(ia, ib) and np.array_equal(ta, tb)
+    assert ra.events == rb.events
+    for v in range(6):
+        np.testing.assert_allclose(ra.get_state(0, v), rb.get_state(0, v), rtol=0, atol=1e-11)

@@ -1,0 +1,109 @@
+"""GPU: neuron groups whose model is code (hs_net_add_group_code) -- Brian2
+equations through the code generator (brian2cuda_amd.codegen), compiled with
+hipRTC into the engine's kernels -- bit for bit against the oracle running the
+same statements compiled for the host, in every engine form."""
+import numpy as np
+import pytest
+
+from brian2cuda_amd import codegen as cg
+from brian2cuda_amd import networks as nw
+from brian2cuda_amd import runtime as rt
+from brian2cuda_amd._lib import HsError
+
+pytestmark = pytest.mark.gpu
+
+FORMS = [("plain", 0, {}), ("fused", 1, {}), ("fused", 1, {9: 1})]
+
+
+def _compare(spec, prec, engine, sem, opts, oracle_lib, steps):
+    net, gsm, gst = nw.build_hs(spec, prec, engine, options=opts)
+    ref, rsm, rst = oracle_lib.build_ref(spec, prec, sem)
+    for k in steps:
+        net.run(k)
+        ref.run(k)
+    for a, b in zip(gsm, rsm):
+        gi, gt = net.spikes(a)
+        ri, rt_ = ref.spikes(b)
+        assert len(gi) == len(ri) > 0
+        assert np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    for gid, g in enumerate(spec.groups):
+        nv = len(g.code.names) if g.code is not None else 1
+        for v in range(nv):
+            assert np.array_equal(net.get_state(gid, v), ref.get_state(gid, v), equal_nan=True), (gid, v)
+    assert net.stats()["events"] == ref.events
+    for a, b, m in zip(gst, rst, spec.state_monitors):
+        w = nw.state_monitor_width(m)
+        assert np.array_equal(net.state_monitor(a, w), ref.state_monitor(b, w))
+    net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine,sem,opts", FORMS)
+def test_code_cuba_vs_oracle(hs, oracle_lib, prec, engine, sem, opts):
+    spec = nw.code_cuba(N=4000, seed=3)
+    spec.state_monitors = [(0, 0, 0, 50), nw.StateMon(0, 1, np.array([7, 3999, 100]), every=3)]
+    _compare(spec, prec, engine, sem, opts, oracle_lib, (1, 333, 1166))
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine,sem,opts", FORMS)
+def test_code_mix_vs_oracle(hs, oracle_lib, prec, engine, sem, opts):
+    """AdEx (rk4, reset of two variables), noise-driven LIF (Euler-Maruyama) and
+    an exact CUBA population, plus a built-in Poisson input, delays 1..30 steps."""
+    spec = nw.code_mix(N=3000, seed=7)
+    spec.state_monitors = [nw.StateMon(0, 1, np.array([0, 5, 1499]), every=1)]
+    _compare(spec, prec, engine, sem, opts, oracle_lib, (250, 1250))
+
+
+@pytest.mark.parametrize("method", ["euler", "rk2"])
+def test_code_methods_vs_oracle(hs, oracle_lib, method):
+    spec = nw.code_mix(N=2000, seed=11, method=method)
+    _compare(spec, 64, "fused", 1, {9: 1}, oracle_lib, (700,))
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine,sem,opts", FORMS[:2])
+def test_code_cobahh_vs_oracle(hs, oracle_lib, prec, engine, sem, opts):
+    """examples/cobahh.py's equations, exponential Euler, through the code
+    generator (delay 0: update then delivery in each step)."""
+    spec = nw.code_cobahh(N=4000, seed=2)
+    _compare(spec, prec, engine, sem, opts, oracle_lib, (200, 300))
+
+
+def test_code_model_errors(hs):
+    net = rt.Net(64, "plain", 1e-4, 1)
+    bad = cg.CodeModel(["v"], "v = v + ;", "v > 1", "")
+    net.add_group_code(10, bad)
+    with pytest.raises(HsError, match="do not compile"):
+        net.run(1)
+    net.close()
+    net = rt.Net(64, "plain", 1e-4, 1)
+    for names, upd in [(["dt"], ""), (["v", "v"], ""), (["hs_x"], ""), (["v"], "#define X 1"), (["v"], "{"),
+                       ([f"x{k}" for k in range(9)], "")]:
+        with pytest.raises(HsError):
+            net.add_group_code(10, cg.CodeModel(names, upd))
+    for k in range(4):
+        net.add_group_code(10, cg.CodeModel(["v"], "v = v + dt;"))
+    with pytest.raises(HsError):
+        net.add_group_code(10, cg.CodeModel(["v"], "v = v + dt;"))
+    net.close()
+
+
+def test_code_cuba_matches_the_builtin_model(hs, oracle_lib):
+    """The code form of the CUBA model and the built-in one integrate the same
+    linear system exactly: over 300 subthreshold steps (no spikes) their states
+    agree to FP64 rounding."""
+    a = nw.code_cuba(N=400, seed=5, we=0.0, wi=0.0)
+    b = nw.cuba(N=400, seed=5, we=0.0, wi=0.0, record=0)
+    for s in (a, b):
+        s.groups[0].init[0] = -55e-3 + 4e-3 * np.random.default_rng(1).random(400)
+        s.groups[0].init[1] = 2e-3
+        s.groups[0].init[2] = -1e-3
+    na, _, _ = nw.build_hs(a, 64, "plain")
+    nb, _, _ = nw.build_hs(b, 64, "plain")
+    na.run(300)
+    nb.run(300)
+    for v in range(3):
+        np.testing.assert_allclose(na.get_state(0, v), nb.get_state(0, v), rtol=0, atol=1e-12)
+    na.close()
+    nb.close()
