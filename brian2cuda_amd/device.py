@@ -196,9 +196,27 @@ def _const_expr(s, ns, what):
     return float(m.group(1)) * (_UNITS[m.group(2)] if m.group(2) else 1.0)
 
 
-def group_spec(g: GroupDesc):
-    """(GroupSpec, variable names) of one group description."""
+def group_spec(g: GroupDesc, dt=None):
+    """(GroupSpec, variable names) of one group description: a built-in model
+    where the equations, threshold, reset and method are one of them, else a
+    code model through the code generator (dt: the network's, for 'exact')."""
     what = f"NeuronGroup {g.name!r}"
+    if g.kind in ("poisson", "spikegenerator"):
+        return _builtin_spec(g, what)
+    try:
+        built = _builtin_spec(g, what)
+    except NotImplementedError:
+        built = None  # a built-in model's equations with another threshold, reset or method
+    if built is not None:
+        return built
+    from .codegen import neuron_model
+    model, names = neuron_model(g.equations, g.threshold, g.reset, float(g.refractory), g.method, g.namespace,
+                                dt=dt, what=what)
+    return nw.GroupSpec(0, g.N, [], _init(g, names), code=model), names
+
+
+def _builtin_spec(g: GroupDesc, what):
+    """(GroupSpec, names) of a built-in model, None if the equations are none of them."""
     if g.kind == "poisson":
         rates = np.asarray(g.rates, dtype=np.float64)
         first = float(rates.ravel()[0]) if rates.size else 0.0
@@ -243,8 +261,7 @@ def group_spec(g: GroupDesc):
                   v["taue"], v["taui"], _threshold_value(g.threshold, "v", ns, what), float(g.refractory)]
         names = ["v", "m", "n", "h", "ge", "gi"]
         return nw.GroupSpec(HS_MODEL_HH_COBA, g.N, params, _init(g, names)), names
-    raise NotImplementedError(f"{what}: its equations are not one of the built-in models "
-                              "(CUBA LIF, noise-driven LIF, COBA Hodgkin-Huxley)")
+    return None
 
 
 def _init(g, names):
@@ -306,7 +323,7 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
     """(NetSpec, object names for the project writer) of a described network."""
     gspecs, gvars, gindex = [], [], {}
     for g in groups:
-        spec, names = group_spec(g)
+        spec, names = group_spec(g, dt)
         if g.kind == "spikegenerator":
             spec.spikes = (np.asarray(g.indices, np.int32), spikegen_steps(np.asarray(g.times, np.float64), dt))
         gindex[g.name] = len(gspecs)

@@ -68,6 +68,18 @@ def _c_double(x):
     return repr(float(x)) if np.isfinite(x) else ("INFINITY" if x > 0 else "-INFINITY")
 
 
+def _c_str(s):
+    """A C string literal of s (one literal per line of s, joined by adjacency)."""
+    lines = (s or "").split("\n")
+    esc = [ln.replace("\\", "\\\\").replace('"', '\\"') for ln in lines]
+    return " ".join(f'"{e}\\n"' if k + 1 < len(esc) else f'"{e}"' for k, e in enumerate(esc))
+
+
+def group_vars(g):
+    """State-variable names of a group spec, in slot order."""
+    return list(g.code.names) if getattr(g, "code", None) is not None else MODEL_VARS[g.model]
+
+
 class ProjectWriter:
     """Writes one network and one run as a standalone project.
 
@@ -133,7 +145,7 @@ class ProjectWriter:
         decl, settable = [], []
         for gi, g in enumerate(spec.groups):
             gname = self.gnames[gi]
-            vars_ = MODEL_VARS[g.model]
+            vars_ = group_vars(g)
             ginfo = {"name": gname, "model": g.model, "n": g.n, "vars": vars_, "init": {}}
             for var, val in sorted(g.init.items()):
                 cname = f"_init_{gname}_{vars_[var]}"
@@ -168,14 +180,23 @@ class ProjectWriter:
             L.append(f"    HS_CHECK(hs_net_set_option(net, {k}, {v}));")
         for gi, g in enumerate(spec.groups):
             gname = self.gnames[gi]
-            p = ", ".join(_c_double(x) for x in g.params)
             L.append(f"    int g_{gname};")
-            L.append(f"    {{ const double p[] = {{{p}}};")
-            L.append(f"      HS_CHECK(hs_net_add_group(net, {MODEL_MACROS[g.model]}, {g.n}, p, {len(g.params)}, "
-                     f"&g_{gname})); }}")
+            if g.code is not None:  # the model as code: compiled for the device on the first run
+                m = g.code
+                nm = ", ".join(f'"{n}"' for n in m.names)
+                L.append(f"    {{ const char *names[] = {{{nm}}};")
+                L.append(f"      HS_CHECK(hs_net_add_group_code(net, {g.n}, {len(m.names)}, names,")
+                L.append(f"          {_c_str(m.update)},")
+                L.append(f"          {_c_str(m.threshold)}, {_c_str(m.reset)}, {_c_double(m.refractory)}, "
+                         f"{int(bool(m.noise))}, {int(m.stream)}u, &g_{gname})); }}")
+            else:
+                p = ", ".join(_c_double(x) for x in g.params)
+                L.append(f"    {{ const double p[] = {{{p}}};")
+                L.append(f"      HS_CHECK(hs_net_add_group(net, {MODEL_MACROS[g.model]}, {g.n}, p, {len(g.params)}, "
+                         f"&g_{gname})); }}")
             for var in sorted(g.init):
                 L.append(f"    HS_CHECK(hs_net_set_state(net, g_{gname}, {var}, _init_{gname}_"
-                         f"{MODEL_VARS[g.model][var]}.data()));")
+                         f"{group_vars(g)[var]}.data()));")
             if g.spikes is not None:
                 idx, st = g.spikes
                 ri, n = self._static(directory, f"_array_{gname}__spikegen_indices", idx, np.int32)
@@ -257,7 +278,7 @@ class ProjectWriter:
                 gname = self.gnames[gid]
                 L.append(f"    HS_CHECK(hs_net_add_state_monitor(net, g_{gname}, {var}, {start}, {stop}, &m_{mname}));")
                 width, every = int(stop - start), 1
-            vname = MODEL_VARS[spec.groups[gid].model][var]
+            vname = group_vars(spec.groups[gid])[var]
             files = {"t": f"_dynamic_array_{mname}_t", vname: f"_dynamic_array_{mname}_{vname}",
                      "indices": f"_array_{mname}__indices", "N": f"_array_{mname}_N"}
             writes.append((mname, [files["t"], files[vname], files["indices"], files["N"]]))
@@ -281,7 +302,7 @@ class ProjectWriter:
             gname = self.gnames[gi]
             if g.model == HS_MODEL_SPIKEGEN:
                 continue
-            for var, vname in enumerate(MODEL_VARS[g.model]):
+            for var, vname in enumerate(group_vars(g)):
                 f = f"_array_{gname}_{vname}"
                 state_files[f"{gname}.{vname}"] = f
                 L.append(f"    {{ std::vector<double> x({g.n});")

@@ -154,17 +154,17 @@ def test_poisson_and_spike_generator_groups():
 
 
 @pytest.mark.parametrize("change,msg", [
-    (dict(equations="dv/dt = -v/tau : volt"), "not one of the built-in models"),
-    (dict(threshold="v > Vt and v < 0*mV"), "threshold"),
-    (dict(reset="v = Vr; w += 1"), "reset"),
-    (dict(method="euler"), "integrated 'exact'"),
+    (dict(equations="dv/dt = -v/tau : volt"), "no value for 'tau'"),
+    (dict(equations="dv/dt = sin(v)/taum : volt"), "function 'sin'"),
+    (dict(reset="v = Vr; w += 1"), "'w' is not a state variable"),
+    (dict(method="heun"), "method 'heun'"),
     (dict(namespace={k: v for k, v in CUBA_NS.items() if k != "taum"}), "taum"),
 ])
 def test_unsupported_groups_are_refused(change, msg):
     kw = dict(equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr", refractory=5 * ms, namespace=CUBA_NS)
     kw.update(change)
     with pytest.raises(NotImplementedError, match=msg):
-        dv.group_spec(dv.GroupDesc("g", 10, **kw))
+        dv.group_spec(dv.GroupDesc("g", 10, **kw), dt=0.1 * ms)
 
 
 @pytest.mark.parametrize("on_pre,kw,msg", [
@@ -206,3 +206,34 @@ def test_device_module_imports_with_and_without_brian2():
                             "print(hasattr(d, 'HipStandaloneDevice'))"], cwd=ROOT, capture_output=True, text=True,
                            timeout=120)
         assert r.returncode == 0 and r.stdout.strip() == "False", r.stderr
+
+
+@pytest.mark.parametrize("change", [dict(threshold="v > Vt and v < 0*mV"), dict(method="euler"),
+                                    dict(reset="v = Vr + 1*mV"),
+                                    dict(equations=CUBA_EQS.replace("-ge/taue", "-ge/taue + 0*v"))])
+def test_other_models_become_code_models(change):
+    """What is not a built-in model (another threshold, reset, method or equation
+    set) maps onto a code model through the generator."""
+    kw = dict(equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr", refractory=5 * ms, namespace=CUBA_NS)
+    kw.update(change)
+    spec, names = dv.group_spec(dv.GroupDesc("g", 10, init={"v": -60 * mV}, **kw), dt=0.1 * ms)
+    assert spec.code is not None and names == ["v", "ge", "gi"] and spec.init == {0: -60 * mV}
+    assert spec.code.refractory == 5 * ms and spec.code.reset
+
+
+def test_adex_example_maps_to_a_code_model():
+    """An adaptive exponential integrate-and-fire population (Brian2's AdEx form)
+    with a two-variable reset: a code model, rk4 as asked."""
+    eqs = """
+    dvm/dt = (gL*(EL - vm) + gL*DeltaT*exp((vm - VT)/DeltaT) + I - w)/C : volt
+    dw/dt = (a*(vm - EL) - w)/tauw : amp
+    I : amp
+    """
+    ns = dict(gL=30 * nS, EL=-70.6 * mV, DeltaT=2 * mV, VT=-50.4 * mV, C=281e-12, a=4 * nS, tauw=144 * ms,
+              b=0.0805e-9, Vr=-70.6 * mV)
+    g = dv.GroupDesc("adex", 100, equations=eqs, threshold="vm > -40*mV", reset="vm = Vr; w += b", method="rk4",
+                     namespace=ns, init={"I": np.linspace(0, 1e-9, 100)})
+    spec, _ = dv.network_spec([g], [], [dv.MonitorDesc("m", "spike", "adex")], dt=0.1 * ms)
+    c = spec.groups[0].code
+    assert c.names == ["vm", "w", "I"] and "_hs_k4_vm" in c.update and c.reset.count(";") == 2
+    np.testing.assert_array_equal(spec.groups[0].init[2], np.linspace(0, 1e-9, 100))

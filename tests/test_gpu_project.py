@@ -128,3 +128,23 @@ def test_described_cuba_example_through_the_device_layer(hs, oracle_lib, tmp_pat
     np.testing.assert_array_equal(res["spikemonitor"]["i"], ri)
     np.testing.assert_array_equal(res["spikemonitor"]["t"], rstep * spec.dt)
     np.testing.assert_array_equal(res["statemonitor"]["v"], ref.state_monitor(rst[0], 50))
+
+
+def test_code_models_through_the_project(hs, oracle_lib, tmp_path):
+    """Three code models and a Poisson input (networks.code_mix) through the
+    project: main.cpp passes the generated statements to hs_net_add_group_code;
+    spikes and final state equal the oracle's."""
+    spec = nw.code_mix(N=2000, seed=9)
+    project.write_project(spec, tmp_path, 800, precision=64, engine="fused")
+    project.build_project(tmp_path)
+    project.run_project(tmp_path, "results", timeout=180)
+    res = project.read_results(tmp_path)
+    ref, rsm, _ = _oracle(oracle_lib, spec, 64, oracle_lib.ACC, 800)
+    names = ["spikemonitor", "spikemonitor_1", "spikemonitor_2"]
+    for k, name in enumerate(names):
+        ri, rstep = ref.spikes(rsm[k])
+        assert len(ri) > 0
+        np.testing.assert_array_equal(res[name]["i"], ri)
+        np.testing.assert_array_equal(res[name]["t"], rstep * spec.dt)
+    np.testing.assert_array_equal(res["neurongroup.v"], ref.get_state(0, 0))
+    np.testing.assert_array_equal(res["neurongroup.w"], ref.get_state(0, 1))
