@@ -290,9 +290,7 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
     diff = [e for e in eqs if e.kind == "diff"]
     params = [e for e in eqs if e.kind == "param"]
     subs = {e.name: e for e in eqs if e.kind == "sub"}
-    names = [e.name for e in eqs if e.kind != "sub"]
-    if not diff:
-        raise NotImplementedError(f"{what}: no differential equation")
+    names = [e.name for e in eqs if e.kind != "sub"] or ["_placeholder"]  # model='': one unused variable
     if len(names) > 8:
         raise NotImplementedError(f"{what}: {len(names)} state variables (at most 8)")
     for n in names + list(subs):
@@ -333,14 +331,18 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
         raise NotImplementedError(f"{what}: more than one noise variable ({sorted(xi_names)})")
     noisy = bool(xi_names)
     unless = {e.name for e in diff if "unless refractory" in e.flags}
-    if method is None:
+    if not diff:
+        method = "none"  # nothing to integrate
+    elif method is None:
         method = "euler" if noisy else ("exact" if _linear(rhs, state, T) else "euler")
     local = {s: n for n, s in state.items()}
     local.update({T: "((R)t)", DT: "dt", XI: "xi", I: "((R)i)"})
     pr = _Printer(local)
     out = []
     order = [e.name for e in diff]
-    if method == "euler":
+    if method == "none":
+        pass
+    elif method == "euler":
         for x in order:
             f = rhs[x]
             g = sp.diff(f, XI)
@@ -519,3 +521,86 @@ def _statements(text, state, sym, pr, what):
             raise NotImplementedError(f"{what}: {x!r} is not a state variable")
         out.append(f"{x} {op} {pr(_parse(e, sym, what))};")
     return "\n".join(out)
+
+
+# ---- synaptic code ---------------------------------------------------------------------
+@dataclass
+class SynapseCode:
+    """The arguments of hs_net_pathway_code for one pathway."""
+    syn_names: List[str]
+    on_pre: str
+    post_names: List[str]          # target group variables the code adds to
+    pre_names: List[str]           # source group variables the code reads
+    init: Optional[Dict[str, object]] = None  # synapse variable -> scalar or array (synapse creation order)
+
+
+def synapse_variables(model):
+    """The variables of a Synapses model given as 'name : unit' lines (differential
+    equations of synapses are not supported here)."""
+    out = []
+    for e in parse_equations(model or ""):
+        if e.kind != "param":
+            raise NotImplementedError(f"Synapses model line {e.name!r}: only 'name : unit' variables are supported")
+        out.append(e.name)
+    return out
+
+
+def synapse_code(on_pre, syn_vars, post_vars, pre_vars, namespace=None, what="Synapses"):
+    """SynapseCode of a Synapses object's on_pre statements: 'x = e', 'x += e', ...
+    on synapse variables, and 'y += e' / 'y -= e' on the target's variables (y or
+    y_post).  Expressions read synapse variables, the source's variables
+    (y_pre), the namespace, units, t, dt, i and j; reading the target's
+    variables is not supported (other synapses of the step add to them)."""
+    ns = dict(namespace or {})
+    syn_vars, post_vars, pre_vars = list(syn_vars), list(post_vars), list(pre_vars)
+    used_post, used_pre = [], []
+    syms = {}
+
+    def S(name):
+        if name not in syms:
+            syms[name] = sp.Symbol(name, real=True)
+        return syms[name]
+
+    def sym(name):
+        if name in syn_vars:
+            return S(name)
+        if name.endswith("_pre") and name[:-4] in pre_vars:
+            if name[:-4] not in used_pre:
+                used_pre.append(name[:-4])
+            return S(name)
+        if name in ("t", "dt", "i", "j"):
+            return S("_hs_" + name)
+        if (name.endswith("_post") and name[:-5] in post_vars) or name in post_vars:
+            raise NotImplementedError(f"{what}: on_pre reads the target variable {name!r} (not supported)")
+        if name in ns:
+            return sp.Float(float(ns[name]))
+        if name in UNITS:
+            return sp.Float(UNITS[name])
+        raise NotImplementedError(f"{what}: no value for {name!r} in on_pre")
+
+    out = []
+    for st in re.split(r"[;\n]", on_pre or ""):
+        st = st.split("#")[0].strip()
+        if not st:
+            continue
+        m = re.match(r"^([A-Za-z_]\w*)\s*(=|\+=|-=|\*=|/=)\s*(.+)$", st)
+        if not m:
+            raise NotImplementedError(f"{what}: on_pre {st!r} is not an assignment")
+        x, op, e = m.groups()
+        expr = _parse(e, sym, f"{what}: on_pre {st!r}")
+        if x in syn_vars:
+            target = x
+        else:
+            base = x[:-5] if x.endswith("_post") else x
+            if base not in post_vars:
+                raise NotImplementedError(f"{what}: on_pre assigns to {x!r}, not a synapse or target variable")
+            if op not in ("+=", "-="):
+                raise NotImplementedError(f"{what}: on_pre {st!r}: target variables take additions only")
+            if base not in used_post:
+                used_post.append(base)
+            target = base + "_post"
+        local = {s: n for n, s in syms.items()}
+        local.update({syms[k]: v for k, v in (("_hs_t", "((R)t)"), ("_hs_dt", "dt"), ("_hs_i", "((R)i)"),
+                                                  ("_hs_j", "((R)j)")) if k in syms})
+        out.append(f"{target} {op} {_Printer(local)(expr)};")
+    return SynapseCode(syn_vars, "\n".join(out), used_post, used_pre)

@@ -19,21 +19,33 @@ struct HsCgModel {
     int noisy = 0;
 };
 
-/* the engine kernels of one network form, compiled for its code models */
+/* one pathway's synaptic code: the on_pre statements over the synapse
+   variables syn_names, the additions <post_names[k]>_post to target variables
+   and the source variables <pre_names[k]>_pre */
+struct HsSynCgModel {
+    std::vector<std::string> syn_names, post_names, pre_names;
+    std::string on_pre;
+};
+
+/* the engine kernels of one network, compiled for its code models (null when
+   the network has none) and its synaptic code (null when none) */
 struct HsCgKernels {
-    hipFunction_t update[2];   /* plain, fused */
-    hipFunction_t window;
-    hipFunction_t fused;
-    hipFunction_t finalize[2]; /* plain, fused */
+    hipFunction_t update[2] = {nullptr, nullptr};   /* plain, fused */
+    hipFunction_t window = nullptr;
+    hipFunction_t fused = nullptr;
+    hipFunction_t finalize[2] = {nullptr, nullptr}; /* plain, fused */
+    hipFunction_t syn_code = nullptr;
 };
 
 /* checks a model's names and code; returns HS_OK or fails with a message */
 int hs_cg_check(const HsCgModel &m);
-/* the program compiled for a network's code models in precision prec */
-std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models);
+int hs_cg_check_syn(const HsSynCgModel &m);
+/* the program compiled for a network's code models and synaptic code in precision prec */
+std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn);
 /* compiled kernels for these models (compiled on first request, then cached
    for the process by precision and program text); fails with the compiler's log */
-int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const HsCgKernels **out);
+int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn,
+                  const HsCgKernels **out);
 
 /* the engine launches of hs_engine.hip, through the compiled kernels (same
    arguments and checks as hs_launch_update / _window / _fused / _finalize) */
@@ -46,5 +58,8 @@ hipError_t hs_cg_launch_fused(const HsCgKernels *K, const void *groups, int ng, 
                               int np, HsCtl *ctl, int koff, int block, int n_upd, int n_del, hipStream_t s);
 hipError_t hs_cg_launch_finalize(const HsCgKernels *K, const void *groups, int ng, int total_tiles, HsCtl *ctl,
                                  int fused, int block, hipStream_t s);
+/* the synaptic code of step ctl->base + koff, after the step's update and delivery */
+hipError_t hs_cg_launch_syn_code(const HsCgKernels *K, const void *paths, int np, const void *groups, HsCtl *ctl,
+                                 int koff, int fused, int grid, hipStream_t s);
 
 #endif /* HS_CODEGEN_H */

@@ -73,6 +73,59 @@ int hs_cg_check(const HsCgModel &m)
     return HS_OK;
 }
 
+int hs_cg_check_syn(const HsSynCgModel &m)
+{
+    static const char *const reserved[] = {"R", "dt", "t", "i", "j", "exp", "log", "log10", "sqrt", "abs", "fabs",
+                                           "clip", "floor", "ceil", "sign", "int"};
+    if ((int)m.syn_names.size() > HS_MAX_SYN_VARS || (int)m.post_names.size() > HS_MAX_VARS ||
+        (int)m.pre_names.size() > HS_MAX_VARS)
+        return hs_rt_fail(HS_ERR_ARG, "synaptic code: at most %d synapse and %d target/source variables",
+                          HS_MAX_SYN_VARS, HS_MAX_VARS);
+    std::vector<std::string> all = m.syn_names;
+    for (const std::string &n : m.post_names) all.push_back(n + "_post");
+    for (const std::string &n : m.pre_names) all.push_back(n + "_pre");
+    for (size_t a = 0; a < all.size(); ++a) {
+        const std::string &n = all[a];
+        if (!hs_cg_ident(n) || n.compare(0, 3, "hs_") == 0 || n.compare(0, 2, "__") == 0)
+            return hs_rt_fail(HS_ERR_ARG, "synaptic code name '%s' is not an identifier of its own", n.c_str());
+        for (const char *r : reserved)
+            if (n == r) return hs_rt_fail(HS_ERR_ARG, "synaptic code name '%s' is reserved", n.c_str());
+        for (size_t b = 0; b < a; ++b)
+            if (all[b] == n) return hs_rt_fail(HS_ERR_ARG, "synaptic code name '%s' twice", n.c_str());
+    }
+    if (m.on_pre.find('#') != std::string::npos)
+        return hs_rt_fail(HS_ERR_ARG, "synaptic code holds a preprocessor directive");
+    int depth = 0;
+    for (char ch : m.on_pre) {
+        depth += ch == '{' ? 1 : (ch == '}' ? -1 : 0);
+        if (depth < 0) break;
+    }
+    if (depth != 0) return hs_rt_fail(HS_ERR_ARG, "synaptic code has unbalanced braces");
+    return HS_OK;
+}
+
+/* one precision's specialisation of synaptic code c */
+static void hs_cg_syn_struct(std::string &o, const char *R, int c, const HsSynCgModel &m)
+{
+    const int ns = (int)m.syn_names.size(), npost = (int)m.post_names.size(), npre = (int)m.pre_names.size();
+    o += "template <> struct HsSynCg<" + std::string(R) + ", " + std::to_string(c) + "> {\n";
+    o += "    static constexpr int NS = " + std::to_string(ns) + ", NPOST = " + std::to_string(npost) +
+         ", NPRE = " + std::to_string(npre) + ";\n";
+    o += "    typedef " + std::string(R) + " R;\n";
+    o += "    static __device__ __forceinline__ void on_pre(R *hs_sv, const R *hs_pre, R *hs_eff, R dt, double t, "
+         "int i, int j)\n    {\n        (void)hs_sv; (void)hs_pre; (void)hs_eff; (void)dt; (void)t; (void)i; "
+         "(void)j;\n";
+    for (int v = 0; v < ns; ++v) o += "        R " + m.syn_names[v] + " = hs_sv[" + std::to_string(v) + "];\n";
+    for (int v = 0; v < npre; ++v)
+        o += "        const R " + m.pre_names[v] + "_pre = hs_pre[" + std::to_string(v) + "];\n";
+    for (int v = 0; v < npost; ++v) o += "        R " + m.post_names[v] + "_post = (R)0;\n";
+    o += "        {\n" + m.on_pre + "\n        }\n";
+    for (int v = 0; v < ns; ++v) o += "        hs_sv[" + std::to_string(v) + "] = " + m.syn_names[v] + ";\n";
+    for (int v = 0; v < npost; ++v) o += "        hs_eff[" + std::to_string(v) + "] = " + m.post_names[v] + "_post;\n";
+    for (int v = 0; v < npre; ++v) o += "        (void)" + m.pre_names[v] + "_pre;\n";
+    o += "    }\n};\n";
+}
+
 /* one precision's specialisation of model c */
 static void hs_cg_struct(std::string &o, const char *R, int c, const HsCgModel &m)
 {
@@ -103,12 +156,14 @@ static void hs_cg_struct(std::string &o, const char *R, int c, const HsCgModel &
     o += "    }\n};\n";
 }
 
-std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models)
+std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn)
 {
     (void)prec;
     std::string o;
-    o += "#define HS_CG_MODELS " + std::to_string(models.size()) + "\n";
+    if (!models.empty()) o += "#define HS_CG_MODELS " + std::to_string(models.size()) + "\n";
+    if (!syn.empty()) o += "#define HS_SCG_PATHS " + std::to_string(syn.size()) + "\n";
     o += "#include \"hs_engine.hip\"\n";
+    o += "#include \"hs_syncode.h\"\n";
     /* the functions code models may call: the library's deterministic exp and
        log, IEEE sqrt; overloads keep the argument's precision */
     o += R"HSCG(
@@ -153,6 +208,10 @@ __device__ __forceinline__ auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo +
         hs_cg_struct(o, "double", (int)c, models[c]);
         hs_cg_struct(o, "float", (int)c, models[c]);
     }
+    for (size_t c = 0; c < syn.size(); ++c) {
+        hs_cg_syn_struct(o, "double", (int)c, syn[c]);
+        hs_cg_syn_struct(o, "float", (int)c, syn[c]);
+    }
     return o;
 }
 
@@ -164,11 +223,14 @@ struct HsCgEntry {
 static std::mutex g_cg_mu;
 static std::map<std::string, HsCgEntry> g_cg_cache; /* precision + program text -> loaded kernels (device of hs_init) */
 
-int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const HsCgKernels **out)
+int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn,
+                  const HsCgKernels **out)
 {
-    if (models.empty() || (int)models.size() > HS_MAX_CODE_MODELS)
-        return hs_rt_fail(HS_ERR_UNSUPPORTED, "1 to %d code models per network", HS_MAX_CODE_MODELS);
-    const std::string src = hs_cg_source(prec, models);
+    if ((int)models.size() > HS_MAX_CODE_MODELS || (int)syn.size() > HS_MAX_CODE_MODELS ||
+        (models.empty() && syn.empty()))
+        return hs_rt_fail(HS_ERR_UNSUPPORTED, "1 to %d code models and synaptic codes per network",
+                          HS_MAX_CODE_MODELS);
+    const std::string src = hs_cg_source(prec, models, syn);
     const std::string key = std::to_string(prec) + "\n" + src; /* the kernels are instantiated for one precision */
     std::lock_guard<std::mutex> lock(g_cg_mu);
     auto it = g_cg_cache.find(key);
@@ -177,10 +239,20 @@ int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const HsCgKern
         return HS_OK;
     }
     const char *R = prec == HS_F64 ? "double" : "float";
-    const std::string names[6] = {
-        std::string("hs_k_update<") + R + ", false, 0>", std::string("hs_k_update<") + R + ", true, 0>",
-        std::string("hs_k_window<") + R + ", 0>",        std::string("hs_k_fused<") + R + ", 0>",
-        std::string("hs_k_finalize<") + R + ", false>",  std::string("hs_k_finalize<") + R + ", true>"};
+    HsCgEntry e;
+    std::vector<std::string> names;
+    std::vector<hipFunction_t *> slots;
+    if (!models.empty()) {
+        names = {std::string("hs_k_update<") + R + ", false, 0>", std::string("hs_k_update<") + R + ", true, 0>",
+                 std::string("hs_k_window<") + R + ", 0>",        std::string("hs_k_fused<") + R + ", 0>",
+                 std::string("hs_k_finalize<") + R + ", false>",  std::string("hs_k_finalize<") + R + ", true>"};
+        slots = {&e.k.update[0], &e.k.update[1], &e.k.window, &e.k.fused, &e.k.finalize[0], &e.k.finalize[1]};
+    }
+    if (!syn.empty()) {
+        names.push_back(std::string("hs_k_syn_code<") + R + ">");
+        slots.push_back(&e.k.syn_code);
+    }
+    const int nk = (int)names.size();
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "hs_code_models.hip", hs_rtc_count, hs_rtc_texts, hs_rtc_names) !=
         HIPRTC_SUCCESS)
@@ -216,8 +288,8 @@ int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const HsCgKern
         code.resize(code_n);
         (void)hiprtcGetCode(prog, code.data());
     }
-    std::string lowered[6];
-    for (int k = 0; k < 6; ++k) {
+    std::vector<std::string> lowered(nk);
+    for (int k = 0; k < nk; ++k) {
         const char *ln = nullptr;
         if (hiprtcGetLoweredName(prog, names[k].c_str(), &ln) != HIPRTC_SUCCESS || !ln) {
             (void)hiprtcDestroyProgram(&prog);
@@ -227,12 +299,9 @@ int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const HsCgKern
     }
     (void)hiprtcDestroyProgram(&prog);
     if (code.empty()) return hs_rt_fail(HS_ERR_HIP, "hipRTC: empty code object");
-    HsCgEntry e;
     hipError_t he = hipModuleLoadData(&e.mod, code.data());
     if (he != hipSuccess) return hs_rt_fail(HS_ERR_HIP, "code model kernels: %s", hipGetErrorString(he));
-    hipFunction_t *slots[6] = {&e.k.update[0], &e.k.update[1], &e.k.window, &e.k.fused, &e.k.finalize[0],
-                               &e.k.finalize[1]};
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < nk; ++k) {
         if ((he = hipModuleGetFunction(slots[k], e.mod, lowered[k].c_str())) != hipSuccess) {
             (void)hipModuleUnload(e.mod);
             return hs_rt_fail(HS_ERR_HIP, "code model kernel %s: %s", names[k].c_str(), hipGetErrorString(he));
@@ -285,4 +354,12 @@ hipError_t hs_cg_launch_finalize(const HsCgKernels *K, const void *groups, int n
     hipError_t e = hs_cg_go(K->finalize[fused ? 1 : 0], grid, block, s, args);
     if (e != hipSuccess) return e;
     return hs_launch_mark_reset(ctl, s);
+}
+
+hipError_t hs_cg_launch_syn_code(const HsCgKernels *K, const void *paths, int np, const void *groups, HsCtl *ctl,
+                                 int koff, int fused, int grid, hipStream_t s)
+{
+    if (!K->syn_code || grid < 1) return hipErrorInvalidValue;
+    void *args[] = {&paths, &np, &groups, &ctl, &koff, &fused};
+    return hs_cg_go(K->syn_code, grid, 256, s, args);
 }

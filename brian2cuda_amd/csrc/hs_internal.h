@@ -42,6 +42,7 @@ typedef unsigned long long uint64_t;
 #define HS_DELIVER_SCATTER 1 /* per (pre, delay) bundles, global atomics on the targets */
 #define HS_DELIVER_GATHER 2  /* per (pre, target tile) segments, LDS accumulation per tile */
 #define HS_DELIVER_PLASTIC 3 /* scatter bundles with STDP (on_pre and on_post kernels, plain form) */
+#define HS_DELIVER_CODE 4    /* scatter bundles whose on_pre is code (hs_net_pathway_code) */
 
 /* The delivered-event count is split over HS_EV_SLOTS counters, one per 128-byte
    line: thousands of waves add to it per launch, and same-address atomics
@@ -161,6 +162,12 @@ struct DPath {
     R sk[5]; /* dApre, dApost, wmin, wmax, scale */
     const int32_t *thist, *thist_cnt;
     int tQ, tnsub, tsubcap;
+    /* synaptic code (dmode HS_DELIVER_CODE): code pathway cg of the network;
+       nsv synapse variables sv[k][q] in bundle order; the code adds to target
+       variables post_var[0..npost) and reads source variables pre_var[0..npre) */
+    int cg, nsv, npost, npre;
+    R *sv[HS_MAX_SYN_VARS];
+    int post_var[HS_MAX_VARS], pre_var[HS_MAX_VARS];
 };
 
 /*

@@ -453,6 +453,12 @@ struct HPath {
     void *apre = nullptr, *apost = nullptr;
     int32_t *lastup = nullptr, *qstart = nullptr, *qsyn = nullptr;
     double stdp[HS_STDP_NPARAMS] = {0};
+    /* synaptic code (dmode HS_DELIVER_CODE): index into hs_net::cg_syn, synapse
+       variables in bundle order (the weight array when one is named w), target
+       and source variables of the code */
+    int cg = -1, nsv = 0;
+    void *sv[HS_MAX_SYN_VARS] = {nullptr};
+    std::vector<int> post_vars, pre_vars;
 };
 
 struct HMon {
@@ -521,7 +527,8 @@ struct hs_net {
     uint64_t spikes = 0;
     std::vector<void *> allocs;
     std::vector<HsCgModel> cg_models;  /* code models, ids HS_MODEL_CODE + index */
-    const HsCgKernels *cg = nullptr;   /* their compiled engine kernels (from the first run) */
+    std::vector<HsSynCgModel> cg_syn;  /* synaptic code of the pathways with HS_DELIVER_CODE */
+    const HsCgKernels *cg = nullptr;   /* their compiled kernels (from the first run) */
 };
 
 static size_t hs_rsize(const hs_net *net) { return net->prec == HS_F64 ? sizeof(double) : sizeof(float); }
@@ -1281,6 +1288,79 @@ extern "C" int hs_net_pathway_stdp(hs_net *net, int pid, const double *params, i
     return HS_OK;
 }
 
+extern "C" int hs_net_pathway_code(hs_net *net, int pid, int nsvars, const char *const *syn_names,
+                                   const char *on_pre, int npost, const int *post_vars,
+                                   const char *const *post_names, int npre, const int *pre_vars,
+                                   const char *const *pre_names)
+{
+    if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");
+    if (net->built) return hs_fail(HS_ERR_STATE, "synaptic code must be set before the first run");
+    HPath &P = net->p[pid];
+    if (P.dmode != HS_DELIVER_SCATTER) return hs_fail(HS_ERR_UNSUPPORTED, "synaptic code needs a scatter-mode pathway");
+    if ((int)net->cg_syn.size() >= HS_MAX_CODE_MODELS)
+        return hs_fail(HS_ERR_UNSUPPORTED, "at most %d pathways with synaptic code per network", HS_MAX_CODE_MODELS);
+    if (nsvars < 0 || nsvars > HS_MAX_SYN_VARS || npost < 0 || npost > HS_MAX_VARS || npre < 0 ||
+        npre > HS_MAX_VARS || (nsvars && !syn_names) || (npost && (!post_vars || !post_names)) ||
+        (npre && (!pre_vars || !pre_names)))
+        return hs_fail(HS_ERR_ARG, "hs_net_pathway_code: bad variable lists");
+    const HGroup &gs = net->g[P.src], &gt = net->g[P.tgt];
+    HsSynCgModel m;
+    m.on_pre = on_pre ? on_pre : "";
+    for (int k = 0; k < nsvars; ++k) {
+        if (!syn_names[k]) return hs_fail(HS_ERR_ARG, "synapse variable name %d is null", k);
+        m.syn_names.push_back(syn_names[k]);
+    }
+    for (int k = 0; k < npost; ++k) {
+        if (!post_names[k] || post_vars[k] < 0 || post_vars[k] >= gt.mi.nvars)
+            return hs_fail(HS_ERR_ARG, "target variable %d of the synaptic code", k);
+        for (int q = 0; q < k; ++q)
+            if (post_vars[q] == post_vars[k]) return hs_fail(HS_ERR_ARG, "target variable %d twice", post_vars[k]);
+        m.post_names.push_back(post_names[k]);
+    }
+    for (int k = 0; k < npre; ++k) {
+        if (!pre_names[k] || pre_vars[k] < 0 || pre_vars[k] >= gs.mi.nvars)
+            return hs_fail(HS_ERR_ARG, "source variable %d of the synaptic code", k);
+        m.pre_names.push_back(pre_names[k]);
+    }
+    HS_TRY(hs_cg_check_syn(m));
+    const size_t ns = (size_t)(P.nsyn ? P.nsyn : 1);
+    for (int k = 0; k < nsvars; ++k) {
+        if (m.syn_names[k] == "w" && P.wmode == HS_W_SYNAPSE) P.sv[k] = P.w; /* the pathway's weights */
+        else HS_TRY(hs_dalloc(net, hs_rsize(net) * ns, &P.sv[k], true));
+    }
+    P.nsv = nsvars;
+    P.post_vars.assign(post_vars, post_vars + npost);
+    P.pre_vars.assign(pre_vars, pre_vars + npre);
+    P.cg = (int)net->cg_syn.size();
+    P.dmode = HS_DELIVER_CODE;
+    net->cg_syn.push_back(m);
+    net->n_scatter--;
+    net->desc_dirty = true;
+    return HS_OK;
+}
+
+/* a synapse variable of a code pathway (HS_SYN_CODE + k) from a host array in bundle order */
+extern "C" int hs_net_set_synapse_var(hs_net *net, int pid, int var, const double *values, int64_t n)
+{
+    if (!net || pid < 0 || pid >= (int)net->p.size() || !values) return hs_fail(HS_ERR_ARG, "bad pathway");
+    HPath &P = net->p[pid];
+    if (P.dmode != HS_DELIVER_CODE || var < HS_SYN_CODE || var >= HS_SYN_CODE + P.nsv)
+        return hs_fail(HS_ERR_ARG, "synapse variable %d of a pathway with synaptic code", var);
+    if (n != P.nsyn) return hs_fail(HS_ERR_ARG, "%lld values for %lld synapses", (long long)n, (long long)P.nsyn);
+    if (!n) return HS_OK;
+    void *dst = P.sv[var - HS_SYN_CODE];
+    if (net->prec == HS_F64) {
+        HS_HIP(hipMemcpyAsync(dst, values, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    } else {
+        std::vector<float> f((size_t)n);
+        for (int64_t s = 0; s < n; ++s) f[(size_t)s] = (float)values[s];
+        HS_HIP(hipMemcpyAsync(dst, f.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice, g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+    }
+    return HS_OK;
+}
+
 /* a synapse variable (HS_SYN_*) of a pathway in bundle order (hs_net_get_synapses' order) */
 extern "C" int hs_net_get_synapse_var(hs_net *net, int pid, int var, double *out, int64_t cap)
 {
@@ -1288,6 +1368,21 @@ extern "C" int hs_net_get_synapse_var(hs_net *net, int pid, int var, double *out
     const HPath &P = net->p[pid];
     if (P.dmode == HS_DELIVER_GATHER) return hs_fail(HS_ERR_UNSUPPORTED, "synapse readback of gather-mode pathways");
     if (cap < P.nsyn) return hs_fail(HS_ERR_ARG, "capacity %lld < %lld synapses", (long long)cap, (long long)P.nsyn);
+    if (P.dmode == HS_DELIVER_CODE && var >= HS_SYN_CODE && var < HS_SYN_CODE + P.nsv) {
+        if (P.nsyn == 0) return HS_OK;
+        const size_t n = (size_t)P.nsyn;
+        const void *src = P.sv[var - HS_SYN_CODE];
+        if (net->prec == HS_F64) {
+            HS_HIP(hipMemcpyAsync(out, src, sizeof(double) * n, hipMemcpyDeviceToHost, g_rt.stream));
+            HS_HIP(hipStreamSynchronize(g_rt.stream));
+        } else {
+            std::vector<float> v(n);
+            HS_HIP(hipMemcpyAsync(v.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, g_rt.stream));
+            HS_HIP(hipStreamSynchronize(g_rt.stream));
+            for (size_t s = 0; s < n; ++s) out[s] = (double)v[s];
+        }
+        return HS_OK;
+    }
     if (var < HS_SYN_W || var > HS_SYN_LASTUPDATE) return hs_fail(HS_ERR_ARG, "bad synapse variable %d", var);
     if (var != HS_SYN_W && P.dmode != HS_DELIVER_PLASTIC)
         return hs_fail(HS_ERR_ARG, "synapse variable %d exists on STDP pathways only", var);
@@ -1373,11 +1468,14 @@ extern "C" int hs_net_add_synapse_monitor(hs_net *net, int pid, int var, const i
     if (net->engine != HS_ENGINE_PLAIN && net->built && net->mode != 1)
         return hs_fail(HS_ERR_UNSUPPORTED, "synapse StateMonitors run in the per-step forms: add them before the "
                                            "first run of a fused network");
-    if (var < HS_SYN_W || var > HS_SYN_APOST) return hs_fail(HS_ERR_ARG, "bad synapse variable %d", var);
-    if (var == HS_SYN_W && P.wmode != HS_W_SYNAPSE)
-        return hs_fail(HS_ERR_ARG, "w is recorded on pathways with per-synapse weights");
-    if (var != HS_SYN_W && P.dmode != HS_DELIVER_PLASTIC)
-        return hs_fail(HS_ERR_ARG, "synapse variable %d exists on STDP pathways only", var);
+    const bool code_var = P.dmode == HS_DELIVER_CODE && var >= HS_SYN_CODE && var < HS_SYN_CODE + P.nsv;
+    if (!code_var) {
+        if (var < HS_SYN_W || var > HS_SYN_APOST) return hs_fail(HS_ERR_ARG, "bad synapse variable %d", var);
+        if (var == HS_SYN_W && P.wmode != HS_W_SYNAPSE)
+            return hs_fail(HS_ERR_ARG, "w is recorded on pathways with per-synapse weights");
+        if (var != HS_SYN_W && P.dmode != HS_DELIVER_PLASTIC)
+            return hs_fail(HS_ERR_ARG, "synapse variable %d exists on STDP pathways only", var);
+    }
     if (P.dmode == HS_DELIVER_GATHER) return hs_fail(HS_ERR_UNSUPPORTED, "gather-mode pathways");
     if (n <= 0 || !indices) return hs_fail(HS_ERR_ARG, "no synapses to record");
     HMon M;
@@ -1574,7 +1672,8 @@ static int hs_build(hs_net *net)
     net->window = 1;
     if (net->engine == HS_ENGINE_PLAIN) {
         net->mode = 0;
-    } else if (net->n_plastic || std::any_of(net->m.begin(), net->m.end(), [](const HMon &M) { return M.kind == 3; })) {
+    } else if (net->n_plastic || !net->cg_syn.empty() ||
+               std::any_of(net->m.begin(), net->m.end(), [](const HMon &M) { return M.kind == 3; })) {
         /* STDP: its pre and post code run after each step's update (the post
            code needs the step's spikes), effects into the step's accumulator;
            synapse StateMonitors record at the start of each step */
@@ -1627,8 +1726,9 @@ static int hs_build(hs_net *net)
             HGroup &gt = net->g[P.tgt];
             /* pipelined windows write up to 3W-2 steps ahead of the window they update */
             gt.nacc = net->mode == 4 ? 3 * W + 1 : W + 1;
-            if (!gt.acc[P.var])
-                HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)gt.n * gt.nacc, &gt.acc[P.var], true));
+            std::vector<int> vars = P.dmode == HS_DELIVER_CODE ? P.post_vars : std::vector<int>{P.var};
+            for (int v : vars)
+                if (!gt.acc[v]) HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)gt.n * gt.nacc, &gt.acc[v], true));
         }
     }
     /* windowed form: the noise of a window is drawn one window ahead, by extra
@@ -1642,7 +1742,8 @@ static int hs_build(hs_net *net)
                 net->noise_ahead = true;
             }
     /* code models: the engine kernels compiled with them (cached per program) */
-    if (!net->cg_models.empty()) HS_TRY(hs_cg_kernels(net->prec, net->cg_models, &net->cg));
+    if (!net->cg_models.empty() || !net->cg_syn.empty())
+        HS_TRY(hs_cg_kernels(net->prec, net->cg_models, net->cg_syn, &net->cg));
     HsCtl c;
     memset(&c, 0, sizeof(c));
     c.base = net->step;
@@ -1772,6 +1873,14 @@ static int hs_upload_desc_t(hs_net *net)
             d.tnsub = gt.nsub;
             d.tsubcap = gt.subcap;
         }
+        d.dt = net->dt;
+        d.cg = P.cg;
+        d.nsv = P.nsv;
+        for (int k = 0; k < P.nsv; ++k) d.sv[k] = (R *)P.sv[k];
+        d.npost = (int)P.post_vars.size();
+        for (int k = 0; k < d.npost; ++k) d.post_var[k] = P.post_vars[(size_t)k];
+        d.npre = (int)P.pre_vars.size();
+        for (int k = 0; k < d.npre; ++k) d.pre_var[k] = P.pre_vars[(size_t)k];
     }
     if (!net->dgroups) HS_TRY(hs_dalloc(net, sizeof(DGroup<R>) * std::max<size_t>(1, dg.size()), &net->dgroups, false));
     if (!net->dpaths) HS_TRY(hs_dalloc(net, sizeof(DPath<R>) * std::max<size_t>(1, dp.size()), &net->dpaths, false));
@@ -1919,7 +2028,8 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
     int ms = net->g.empty() ? 0 : hs_kernel_model(net, net->g[0].model); /* one model for all groups: specialised kernels */
     for (const HGroup &g : net->g)
         if (hs_kernel_model(net, g.model) != ms) ms = 0;
-    const HsCgKernels *cg = net->cg; /* code models: the kernels compiled for them (every model dispatched) */
+    /* code models: the kernels compiled for them (every model dispatched) */
+    const HsCgKernels *cg = net->cg_models.empty() ? nullptr : net->cg;
     const int ggrid = std::max(1, std::min(net->gather_tiles, 1 << 20));
     /* where a window's noise is drawn: by extra blocks of its scatter-delivery
        launch (two-launch windows), else one window ahead by the update or window
@@ -1990,7 +2100,8 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
                 for (const HMon &M : net->m) /* synapse StateMonitors, when='start' */
                     if (M.kind == 3) {
                         const HPath &P = net->p[M.pid];
-                        const void *src = M.var == HS_SYN_W ? P.w : (M.var == HS_SYN_APRE ? P.apre : P.apost);
+                        const void *src = M.var >= HS_SYN_CODE ? P.sv[M.var - HS_SYN_CODE]
+                                          : (M.var == HS_SYN_W ? P.w : (M.var == HS_SYN_APRE ? P.apre : P.apost));
                         HS_HIP(hs_launch_synmon(net->prec, src, M.sidx, (int)M.hsel.size(), M.buf, M.rows, M.t0,
                                                 M.every, net->ctl, (int)j, g_rt.stream));
                     }
@@ -1998,6 +2109,9 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
                 if (np > 0) HS_TRY(deliver((int)j, 1));
                 if (net->n_plastic)
                     HS_HIP(hs_launch_stdp<R>(dp, np, net->ctl, (int)j, std::max(1, hs_rt_cus()), g_rt.stream));
+                if (!net->cg_syn.empty())
+                    HS_HIP(hs_cg_launch_syn_code(net->cg, dp, np, dg, net->ctl, (int)j, fused,
+                                                 std::max(1, hs_rt_cus()), g_rt.stream));
             }
         }
     }
@@ -2130,7 +2244,7 @@ extern "C" int hs_net_run(hs_net *net, int64_t nsteps)
     HS_HIP(hipEventRecord(net->ev1, g_rt.stream));
     const int block = (int)net->opt[0];
     const bool fused = net->engine == HS_ENGINE_FUSED;
-    if (net->cg)
+    if (net->cg && !net->cg_models.empty())
         HS_HIP(hs_cg_launch_finalize(net->cg, net->dgroups, (int)net->g.size(), net->total_tiles, net->ctl, fused,
                                      block, g_rt.stream));
     else if (net->prec == HS_F64)

@@ -16,8 +16,8 @@ from typing import List, Optional
 
 import numpy as np
 
-from .runtime import (HS_MODEL_HH_COBA, HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_POISSON, HS_MODEL_SPIKEGEN, HS_W_PRE,
-                      HS_W_SCALAR, HS_W_SYNAPSE, spikegen_steps)
+from .runtime import (HS_MODEL_HH_COBA, HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_POISSON, HS_MODEL_SPIKEGEN,
+                      HS_SYN_CODE, HS_W_PRE, HS_W_SCALAR, HS_W_SYNAPSE, spikegen_steps)
 
 mV, ms, nS = 1e-3, 1e-3, 1e-9
 
@@ -66,6 +66,7 @@ class PathSpec:
     wscalar: float = 0.0
     random: Optional[RandomConnect] = None
     stdp: Optional[tuple] = None  # taupre, taupost, dApre, dApost, wmin, wmax, scale
+    code: object = None           # a codegen.SynapseCode: the pathway's on_pre is code (var unused)
 
     @property
     def nsyn(self):
@@ -120,6 +121,41 @@ def quantise_delay(d, dt):
     """Delay in steps as the reference computes it: (int)(d/dt + 0.5)
     (brian2cuda/templates/synapses_push_spikes.cu:375)."""
     return (np.asarray(d, dtype=np.float64) / dt + 0.5).astype(np.int32)
+
+
+#: state-variable names of the built-in models, in slot order (include/hs_models.h)
+MODEL_VAR_NAMES = {
+    HS_MODEL_LIF_CUBA: ["v", "ge", "gi"],
+    HS_MODEL_LIF_NOISE: ["V"],
+    HS_MODEL_HH_COBA: ["v", "m", "n", "h", "ge", "gi"],
+    HS_MODEL_SPIKEGEN: ["_placeholder"],
+    HS_MODEL_POISSON: ["rates"],
+}
+
+
+def group_var_names(g):
+    """State-variable names of a group spec, in slot order."""
+    return list(g.code.names) if g.code is not None else MODEL_VAR_NAMES[g.model]
+
+
+def code_var_indices(spec, p):
+    """(target, source) variable indices of a code pathway's post_names / pre_names."""
+    tn, sn = group_var_names(spec.groups[p.tgt]), group_var_names(spec.groups[p.src])
+    for n in p.code.post_names:
+        if n not in tn:
+            raise ValueError(f"the target group has no variable {n!r}")
+    for n in p.code.pre_names:
+        if n not in sn:
+            raise ValueError(f"the source group has no variable {n!r}")
+    return [tn.index(n) for n in p.code.post_names], [sn.index(n) for n in p.code.pre_names]
+
+
+def bundle_order(p):
+    """Creation-order indices of an explicit pathway's synapses in bundle order
+    (stable by presynaptic neuron, then delay): the order the device keeps
+    synapse variables in."""
+    d = np.zeros(len(p.pre), np.int64) if p.delay_steps is None else np.asarray(p.delay_steps, np.int64)
+    return np.lexsort((d, np.asarray(p.pre, np.int64)))
 
 
 def bernoulli_connect(rng, n_pre, n_post, p, pre_offset=0, post_offset=0):
@@ -520,6 +556,18 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
                       p.wscalar)
         if p.stdp is not None:
             net.pathway_stdp(pid, *p.stdp)
+        if p.code is not None:
+            post_i, pre_i = code_var_indices(spec, p)
+            net.pathway_code(pid, p.code, post_i, pre_i)
+            for name, val in (p.code.init or {}).items():
+                if name == "w" and p.wmode == HS_W_SYNAPSE:
+                    continue  # the pathway's weights
+                a = np.asarray(val, np.float64)
+                if a.size > 1:
+                    if p.random is not None:
+                        raise ValueError("per-synapse initial values need explicit synapses")
+                    a = a[bundle_order(p)]
+                net.set_synapse_var(pid, HS_SYN_CODE + p.code.syn_names.index(name), a)
     smons = [net.add_spike_monitor(g, spike_capacity) for g in spec.spike_monitors]
     stmons = [net.add_state_monitor_indices(m.gid, m.var, m.indices, m.every) if isinstance(m, StateMon)
               else net.add_state_monitor(*m) for m in spec.state_monitors]

@@ -43,13 +43,7 @@ REPO = os.path.dirname(HERE)
 INCLUDE = os.path.join(REPO, "include")
 
 #: state-variable names of each model, in slot order (include/hs_models.h)
-MODEL_VARS = {
-    HS_MODEL_LIF_CUBA: ["v", "ge", "gi"],
-    HS_MODEL_LIF_NOISE: ["V"],
-    HS_MODEL_HH_COBA: ["v", "m", "n", "h", "ge", "gi"],
-    HS_MODEL_SPIKEGEN: ["_placeholder"],
-    HS_MODEL_POISSON: ["rates"],
-}
+MODEL_VARS = nw.MODEL_VAR_NAMES
 MODEL_MACROS = {
     HS_MODEL_LIF_CUBA: "HS_MODEL_LIF_CUBA",
     HS_MODEL_LIF_NOISE: "HS_MODEL_LIF_NOISE",

@@ -18,6 +18,7 @@ HS_W_SCALAR, HS_W_SYNAPSE, HS_W_PRE = 0, 1, 2
 
 HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE, HS_MODEL_HH_COBA, HS_MODEL_SPIKEGEN, HS_MODEL_POISSON = 1, 2, 3, 4, 5
 HS_MODEL_CODE, HS_MAX_CODE_MODELS = 64, 4  # code models: ids HS_MODEL_CODE + k (hs_net_add_group_code)
+HS_SYN_CODE = 16  # synapse variable k of a pathway with on_pre code (hs_net_pathway_code)
 MODEL_NVARS = {HS_MODEL_LIF_CUBA: 3, HS_MODEL_LIF_NOISE: 1, HS_MODEL_HH_COBA: 6, HS_MODEL_SPIKEGEN: 1,
                HS_MODEL_POISSON: 1}
 MODEL_NPARAMS = {HS_MODEL_LIF_CUBA: 7, HS_MODEL_LIF_NOISE: 7, HS_MODEL_HH_COBA: 14, HS_MODEL_SPIKEGEN: 1,
@@ -218,6 +219,22 @@ class Net:
              int(bool(model.noise)), int(model.stream), C.byref(gid))
         self.groups.append((HS_MODEL_CODE + sum(1 for m, _ in self.groups if m >= HS_MODEL_CODE), n))
         return gid.value
+
+    def pathway_code(self, pid, code, post_vars, pre_vars):
+        """on_pre code (a brian2cuda_amd.codegen.SynapseCode) on pathway pid: the
+        code adds to target variables post_vars and reads source variables
+        pre_vars (indices, in the order of code.post_names / code.pre_names)."""
+        enc = lambda names: (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+        pv = (C.c_int * max(1, len(post_vars)))(*post_vars)
+        qv = (C.c_int * max(1, len(pre_vars)))(*pre_vars)
+        call("hs_net_pathway_code", self.h, pid, len(code.syn_names), enc(code.syn_names), code.on_pre.encode(),
+             len(post_vars), pv, enc(code.post_names), len(pre_vars), qv, enc(code.pre_names))
+
+    def set_synapse_var(self, pid, var, values):
+        """Synapse variable var (HS_SYN_CODE + k) of a code pathway, values in bundle order."""
+        n = self.pathway_info(pid)["nsyn"]
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), (n,)))
+        call("hs_net_set_synapse_var", self.h, pid, int(var), _f64(v), n)
 
     def set_state(self, gid, var, values):
         n = self.groups[gid][1]

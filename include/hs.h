@@ -296,8 +296,37 @@ int hs_net_set_spikegen(hs_net *net, int gid, int64_t n, const int32_t *indices,
  * such synapses.
  */
 int hs_net_pathway_stdp(hs_net *net, int pid, const double *params, int nparams);
-/* a synapse variable (HS_SYN_W, _APRE, _APOST, _LASTUPDATE) in bundle order
-   (the order of hs_net_get_synapses); cap >= nsyn */
+/*
+ * Make a pathway's on_pre code: what the reference's code generator emits for a
+ * Synapses object's on_pre statements (cuda_generator.py:393-527 into the
+ * synapses.cu:12-164 effect kernel, atomics for additive target writes).  For
+ * every synapse whose spike arrives at step t, in the network's precision R,
+ * with these names in scope:
+ *   syn_names[k]        the synapse's variable k (read and write; a variable named
+ *                       w on a pathway with per-synapse weights is its weights,
+ *                       others start at 0: hs_net_set_synapse_var); variable k is
+ *                       HS_SYN_CODE + k for hs_net_get_synapse_var, monitors
+ *   <post_names[k]>_post  starts at 0; what the code leaves in it is added to
+ *                       the target neuron's variable post_vars[k] (Brian2's
+ *                       'x_post += ...': additions only, as several synapses of a
+ *                       step may share a target)
+ *   <pre_names[k]>_pre  the source neuron's variable pre_vars[k] (read)
+ *   dt (R), t (double, the step's time), i (source index), j (target index)
+ * and the functions of hs_net_add_group_code.  The pathway's own target
+ * variable and weight effect are replaced by the code.  Needs a scatter
+ * pathway; before the first run; compiled with the network's code models on
+ * its first run.  In the fused form the network then advances one step per
+ * update launch and the additions go to the step's accumulator, as STDP's do.
+ */
+int hs_net_pathway_code(hs_net *net, int pid, int nsvars, const char *const *syn_names, const char *on_pre, int npost,
+                        const int *post_vars, const char *const *post_names, int npre, const int *pre_vars,
+                        const char *const *pre_names);
+/* Set synapse variable var (HS_SYN_CODE + k) of a pathway with on_pre code from
+   n = its synapse count values in bundle order (hs_net_get_synapses' order). */
+int hs_net_set_synapse_var(hs_net *net, int pid, int var, const double *values, int64_t n);
+/* a synapse variable (HS_SYN_W, _APRE, _APOST, _LASTUPDATE; HS_SYN_CODE + k of
+   a pathway with on_pre code) in bundle order (the order of
+   hs_net_get_synapses); cap >= nsyn */
 int hs_net_get_synapse_var(hs_net *net, int pid, int var, double *out, int64_t cap);
 
 /* SpikeMonitor on a group: capacity = device records kept (i, step) at first;

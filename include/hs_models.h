@@ -127,6 +127,10 @@
 #define HS_SYN_APRE 1
 #define HS_SYN_APOST 2
 #define HS_SYN_LASTUPDATE 3
+/* synapse variables of a pathway with synaptic code (hs_net_pathway_code):
+   variable k is HS_SYN_CODE + k, at most HS_MAX_SYN_VARS of them */
+#define HS_SYN_CODE 16
+#define HS_MAX_SYN_VARS 8
 
 /* "never spiked" marker for the integer last-spike step */
 #define HS_NEVER_SPIKED (-1073741824)

@@ -65,6 +65,9 @@ def lib():
         "ref_net_create": (V, [C.c_int, C.c_int, C.c_double, C.c_uint64]),
         "ref_net_add_group": (C.c_int, [V, C.c_int, C.c_int, DP, C.c_int]),
         "ref_net_add_group_code": (C.c_int, [V, C.c_int, C.c_int, V, V, C.c_double, C.c_int, C.c_uint32]),
+        "ref_net_pathway_code": (C.c_int, [V, C.c_int, V, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int,
+                                           C.POINTER(C.c_int)]),
+        "ref_net_set_synapse_var": (C.c_int, [V, C.c_int, C.c_int, DP]),
         "ref_net_set_state": (C.c_int, [V, C.c_int, C.c_int, DP]),
         "ref_net_get_state": (C.c_int, [V, C.c_int, C.c_int, DP]),
         "ref_net_get_lastspike": (C.c_int, [V, C.c_int, DP]),
@@ -243,12 +246,26 @@ def code_model_source(model, precision):
             f"    (void)dt; (void)t; (void)i;\n{load}    {{\n{model.reset or ''}\n    }}\n{store}}}\n")
 
 
-def code_model_functions(model, precision):
-    """(step, reset) function addresses of a code model compiled for the host
-    (reset None when the model has no reset); compiled once per source."""
+def synapse_code_source(code, precision):
+    """The host program of one pathway's on_pre code: extern "C" hs_scg_on_pre."""
+    R = "double" if precision == 64 else "float"
+    lines = [f"    R {n} = hs_sv[{k}];" for k, n in enumerate(code.syn_names)]
+    lines += [f"    const R {n}_pre = hs_pre[{k}];" for k, n in enumerate(code.pre_names)]
+    lines += [f"    R {n}_post = (R)0;" for n in code.post_names]
+    lines += ["    {", code.on_pre, "    }"]
+    lines += [f"    hs_sv[{k}] = {n};" for k, n in enumerate(code.syn_names)]
+    lines += [f"    hs_eff[{k}] = {n}_post;" for k, n in enumerate(code.post_names)]
+    lines += [f"    (void){n}_pre;" for n in code.pre_names]
+    return (_CG_MATH + f"typedef {R} R;\n"
+            'extern "C" void hs_scg_on_pre(R *hs_sv, const R *hs_pre, R *hs_eff, R dt, double t, int i, int j)\n{\n'
+            "    (void)hs_sv; (void)hs_pre; (void)hs_eff; (void)dt; (void)t; (void)i; (void)j;\n"
+            + "\n".join(lines) + "\n}\n")
+
+
+def _host_lib(src):
+    """ctypes handle of src compiled for the host (g++, the oracle's IEEE flags), cached by text."""
     import hashlib
     import tempfile
-    src = code_model_source(model, precision)
     key = hashlib.sha1(src.encode()).hexdigest()[:20]
     if key not in _cg_libs:
         d = os.path.join(tempfile.gettempdir(), f"hs_oracle_cg_{os.getuid()}")
@@ -263,10 +280,21 @@ def code_model_functions(model, precision):
                                 "-I", os.path.join(HERE, "..", "brian2cuda_amd", "csrc"), "-o", tmp, cpp],
                                capture_output=True, text=True)
             if r.returncode != 0:
-                raise ValueError("code model does not compile for the host:\n" + r.stderr[-3000:])
+                raise ValueError("code does not compile for the host:\n" + r.stderr[-3000:])
             os.replace(tmp, so)
         _cg_libs[key] = C.CDLL(so)
-    L = _cg_libs[key]
+    return _cg_libs[key]
+
+
+def synapse_code_function(code, precision):
+    """The address of a pathway's on_pre code compiled for the host."""
+    return C.cast(_host_lib(synapse_code_source(code, precision)).hs_scg_on_pre, C.c_void_p).value
+
+
+def code_model_functions(model, precision):
+    """(step, reset) function addresses of a code model compiled for the host
+    (reset None when the model has no reset); compiled once per source."""
+    L = _host_lib(code_model_source(model, precision))
     step = C.cast(L.hs_cg_step, C.c_void_p).value
     reset = C.cast(L.hs_cg_reset, C.c_void_p).value if model.reset else None
     return step, reset
@@ -281,6 +309,7 @@ class RefNet:
         if not self.h:
             raise ValueError("bad precision")
         self.precision = precision
+        self._wmode = {}
         self.sizes = []
 
     def close(self):
@@ -343,6 +372,7 @@ class RefNet:
                                        float(wscalar))
         if r < 0:
             raise ValueError(f"bad pathway ({r})")
+        self._wmode[r] = int(wmode)
         return r
 
     def set_spikegen(self, gid, indices, steps):
@@ -352,6 +382,24 @@ class RefNet:
                                         s.ctypes.data_as(C.POINTER(C.c_int64)))
         if r < 0:
             raise ValueError(f"bad spike generator pattern ({r})")
+
+    def pathway_code(self, pid, code, post_vars, pre_vars):
+        """on_pre code (brian2cuda_amd.codegen.SynapseCode) on pathway pid: the
+        code adds to target variables post_vars and reads source variables
+        pre_vars (indices, in the order of code.post_names / code.pre_names)."""
+        fn = synapse_code_function(code, self.precision)
+        w_sv = code.syn_names.index("w") if "w" in code.syn_names and self._wmode[pid] == 1 else -1
+        pv = (C.c_int * max(1, len(post_vars)))(*post_vars)
+        qv = (C.c_int * max(1, len(pre_vars)))(*pre_vars)
+        r = self.L.ref_net_pathway_code(self.h, pid, fn, len(code.syn_names), w_sv, len(post_vars), pv,
+                                        len(pre_vars), qv)
+        if r < 0:
+            raise ValueError(f"bad synaptic code ({r})")
+
+    def set_synapse_var(self, pid, var, values, nsyn):
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), (nsyn,)))
+        if self.L.ref_net_set_synapse_var(self.h, pid, var, v.ctypes.data_as(C.POINTER(C.c_double))) < 0:
+            raise ValueError("bad synapse variable")
 
     def pathway_stdp(self, pid, params):
         p = np.ascontiguousarray(params, dtype=np.float64)
@@ -454,6 +502,16 @@ def random_pathway(spec, path, rows=None):
     return connect_random(pre, r.post, r.p, skip_self, r.seed, r.stream, r.delay, spec.dt, r.w)
 
 
+HS_SYN_CODE = 16  # synapse variable k of a pathway with on_pre code
+_BUILTIN_VARS = {1: ["v", "ge", "gi"], 2: ["V"], 3: ["v", "m", "n", "h", "ge", "gi"], 4: ["_placeholder"],
+                 5: ["rates"]}
+
+
+def _var_names(g):
+    """State-variable names of a group spec (code models: their own)."""
+    return list(g.code.names) if getattr(g, "code", None) is not None else _BUILTIN_VARS[g.model]
+
+
 def build_ref(spec, precision=64, semantics=SEQ, threads=1):
     """Instantiate a brian2cuda_amd.networks.NetSpec on the oracle (its step
     loop over `threads` OpenMP threads)."""
@@ -477,6 +535,16 @@ def build_ref(spec, precision=64, semantics=SEQ, threads=1):
                                   p.weights, p.wscalar)
         if getattr(p, "stdp", None) is not None:
             net.pathway_stdp(pid, p.stdp)
+        if getattr(p, "code", None) is not None:
+            tn = _var_names(spec.groups[p.tgt])
+            sn = _var_names(spec.groups[p.src])
+            net.pathway_code(pid, p.code, [tn.index(n) for n in p.code.post_names],
+                             [sn.index(n) for n in p.code.pre_names])
+            nsyn = len(syn["pre"]) if p.random is not None else len(p.pre)
+            for name, val in (p.code.init or {}).items():
+                if name == "w" and (p.random.wmode if p.random is not None else p.wmode) == 1:
+                    continue  # the pathway's weights
+                net.set_synapse_var(pid, HS_SYN_CODE + p.code.syn_names.index(name), val, nsyn)
     smons = [net.add_spike_monitor(g) for g in spec.spike_monitors]
     stmons = [net.add_state_monitor_indices(m.gid, m.var, m.indices, m.every) if hasattr(m, "indices")
               else net.add_state_monitor(*m) for m in spec.state_monitors]

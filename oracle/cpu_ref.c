@@ -323,6 +323,14 @@ typedef struct {
     void *apre, *apost;
     int64_t *lastup;
     int64_t *post_start, *by_post;
+    /* synaptic code (cg_fn != NULL): on_pre(sv, pre, eff, dt, t, i, j) compiled for
+       the host; nsv synapse variables in synapse order (sv[w_sv] is the weight
+       array); additions to target variables post_var[], reads of source
+       variables pre_var[] */
+    void *cg_fn;
+    int nsv, npost, npre;
+    void *sv[HS_MAX_SYN_VARS];
+    int post_var[HS_MAX_VARS], pre_var[HS_MAX_VARS];
 } ref_path;
 
 typedef struct {
@@ -619,11 +627,63 @@ int ref_net_pathway_stdp(ref_net *net, int pid, const double *params, int nparam
     return 0;
 }
 
-/* synapse variable in synapse (creation) order: 0 w, 1 Apre, 2 Apost, 3 lastupdate (step) */
+/*
+ * A pathway whose on_pre is code (hs_net_pathway_code): fn(sv, pre, eff, dt, t,
+ * i, j) of the network's precision; nsv synapse variables (w_sv >= 0: that one
+ * is the pathway's per-synapse weight array), the code adds to target variables
+ * post_vars[0..npost) and reads source variables pre_vars[0..npre).
+ */
+int ref_net_pathway_code(ref_net *net, int pid, void *fn, int nsv, int w_sv, int npost, const int *post_vars, int npre,
+                         const int *pre_vars)
+{
+    if (pid < 0 || pid >= net->np || !fn || nsv < 0 || nsv > HS_MAX_SYN_VARS || npost < 0 || npost > HS_MAX_VARS ||
+        npre < 0 || npre > HS_MAX_VARS)
+        return -1;
+    ref_path *p = &net->pw[pid];
+    ref_group *gt = &net->g[p->tgt];
+    const ref_group *gs = &net->g[p->src];
+    if (p->plastic || p->cg_fn || (w_sv >= 0 && (w_sv >= nsv || p->wmode != HS_W_SYNAPSE))) return -2;
+    for (int k = 0; k < npost; ++k)
+        if (post_vars[k] < 0 || post_vars[k] >= gt->nvars) return -3;
+    for (int k = 0; k < npre; ++k)
+        if (pre_vars[k] < 0 || pre_vars[k] >= gs->nvars) return -3;
+    const int64_t n = p->nsyn ? p->nsyn : 1;
+    for (int k = 0; k < nsv; ++k) p->sv[k] = k == w_sv ? p->w : calloc((size_t)n, rsz(net));
+    p->nsv = nsv;
+    p->npost = npost;
+    p->npre = npre;
+    for (int k = 0; k < npost; ++k) {
+        p->post_var[k] = post_vars[k];
+        if (net->sem == 1 && !gt->acc[post_vars[k]]) gt->acc[post_vars[k]] = calloc((size_t)gt->n, rsz(net));
+    }
+    for (int k = 0; k < npre; ++k) p->pre_var[k] = pre_vars[k];
+    p->cg_fn = fn;
+    return 0;
+}
+
+/* synapse variable k (HS_SYN_CODE + k) of a code pathway, in synapse order */
+int ref_net_set_synapse_var(ref_net *net, int pid, int var, const double *values)
+{
+    if (pid < 0 || pid >= net->np) return -1;
+    ref_path *p = &net->pw[pid];
+    if (!p->cg_fn || var < HS_SYN_CODE || var >= HS_SYN_CODE + p->nsv) return -2;
+    for (int64_t s = 0; s < p->nsyn; ++s) {
+        if (net->prec == HS_F64) ((double *)p->sv[var - HS_SYN_CODE])[s] = values[s];
+        else ((float *)p->sv[var - HS_SYN_CODE])[s] = (float)values[s];
+    }
+    return 0;
+}
+
+/* synapse variable in synapse (creation) order: 0 w, 1 Apre, 2 Apost, 3 lastupdate (step),
+   HS_SYN_CODE + k for code pathways */
 int ref_net_get_synapse_var(ref_net *net, int pid, int var, double *out)
 {
     if (pid < 0 || pid >= net->np) return -1;
     ref_path *p = &net->pw[pid];
+    if (p->cg_fn && var >= HS_SYN_CODE && var < HS_SYN_CODE + p->nsv) {
+        for (int64_t s = 0; s < p->nsyn; ++s) out[s] = rget(net, p->sv[var - HS_SYN_CODE], s);
+        return 0;
+    }
     for (int64_t s = 0; s < p->nsyn; ++s) {
         if (var == HS_SYN_W) out[s] = p->wmode == HS_W_SYNAPSE ? rget(net, p->w, s) : p->wscalar;
         else if (var == HS_SYN_APRE && p->plastic) out[s] = rget(net, p->apre, s);
@@ -690,10 +750,14 @@ int ref_net_add_state_monitor_indices(ref_net *net, int gid, int var, const int3
    when='start' (statemonitor.cu:9-100 on a Synapses source) */
 int ref_net_add_synapse_monitor(ref_net *net, int pid, int var, const int32_t *sel, int n)
 {
-    if (net->nm >= REF_MAX_MONS || pid < 0 || pid >= net->np || var < 0 || var > 2 || n <= 0) return -1;
+    if (net->nm >= REF_MAX_MONS || pid < 0 || pid >= net->np || n <= 0) return -1;
     const ref_path *p = &net->pw[pid];
-    if (var == 0 && p->wmode != HS_W_SYNAPSE) return -1;
-    if (var > 0 && !p->plastic) return -1;
+    const int code_var = p->cg_fn && var >= HS_SYN_CODE && var < HS_SYN_CODE + p->nsv;
+    if (!code_var) {
+        if (var < 0 || var > 2) return -1;
+        if (var == 0 && p->wmode != HS_W_SYNAPSE) return -1;
+        if (var > 0 && !p->plastic) return -1;
+    }
     for (int k = 0; k < n; ++k)
         if (sel[k] < 0 || sel[k] >= p->nsyn) return -1;
     ref_mon *m = &net->mon[net->nm];
@@ -710,7 +774,8 @@ static void mon_syn(ref_net *net, ref_mon *m)
 {
     if (m->every > 1 && net->step % m->every != 0) return;
     const ref_path *p = &net->pw[m->pid];
-    const void *src = m->var == 0 ? p->w : (m->var == 1 ? p->apre : p->apost);
+    const void *src = m->var >= HS_SYN_CODE ? p->sv[m->var - HS_SYN_CODE]
+                                            : (m->var == 0 ? p->w : (m->var == 1 ? p->apre : p->apost));
     int64_t w = m->nsel;
     if ((m->n + 1) * w > m->cap) {
         int64_t c = m->cap ? m->cap : 1024 * w;

@@ -155,3 +155,54 @@ def test_code_cobahh_matches_the_builtin_model():
     assert ra.events == rb.events
     for v in range(6):
         np.testing.assert_allclose(ra.get_state(0, v), rb.get_state(0, v), rtol=0, atol=1e-11)
+
+
+# ---- synaptic code ---------------------------------------------------------------------
+def sorted_indices_spec(n=5, prec_dt=1e-4):
+    """The network of the reference's test_sorted_indices_statemonitor
+    (test_cpp_cuda_consistency.py:177-205): NeuronGroup(n, model='',
+    threshold='True'), Synapses(P, P, 'w : 1', on_pre='w += 1'),
+    connect(j='i'), delay of synapse k = k*dt."""
+    m, names = cg.neuron_model("", "True", None, 0.0, None, {})
+    code = cg.synapse_code("w += 1", cg.synapse_variables("w : 1"), names, names)
+    i = np.arange(n, dtype=np.int32)
+    return nw.NetSpec("sorted_indices", prec_dt, 1, [nw.GroupSpec(0, n, [], {}, code=m)],
+                      [nw.PathSpec(0, 0, 0, i, i, delay_steps=np.arange(n, dtype=np.int32), code=code)])
+
+
+@pytest.mark.parametrize("prec,sem", [(64, oracle.SEQ), (32, oracle.ACC)])
+def test_sorted_indices_known_answer(prec, sem):
+    """Every neuron fires every step; synapse k's spikes arrive k steps later,
+    so at the start of step s its w is max(0, s - k)."""
+    n = 5
+    ref, _, _ = oracle.build_ref(sorted_indices_spec(n), prec, sem)
+    mid = ref.add_synapse_monitor(0, nw.HS_SYN_CODE, np.arange(n))
+    ref.run(n + 1)
+    s, k = np.meshgrid(np.arange(n + 1), np.arange(n), indexing="ij")
+    np.testing.assert_array_equal(ref.state_monitor(mid, n), np.maximum(0, s - k))
+
+
+def test_additive_code_equals_the_builtin_effect():
+    """'ge += we' as synaptic code on C1 gives the built-in pathway's bits."""
+    a = nw.cuba(N=1000, seed=4, record=0)
+    b = nw.cuba(N=1000, seed=4, record=0)
+    for p, var in zip(b.paths, ("ge", "gi")):
+        p.code = cg.synapse_code(f"{var} += {p.wscalar!r}", [], ["v", "ge", "gi"], ["v", "ge", "gi"])
+    for prec, sem in ((64, oracle.SEQ), (32, oracle.ACC)):
+        ra, sa, _ = oracle.build_ref(a, prec, sem)
+        rb, sb, _ = oracle.build_ref(b, prec, sem)
+        ra.run(500)
+        rb.run(500)
+        assert np.array_equal(ra.spikes(sa[0])[0], rb.spikes(sb[0])[0]) and ra.events == rb.events
+        for v in range(3):
+            assert np.array_equal(ra.get_state(0, v), rb.get_state(0, v))
+
+
+@pytest.mark.parametrize("code,msg", [
+    ("w += v_post", "reads the target"), ("v = w", "additions only"), ("q += 1", "not a synapse or target"),
+    ("w += foo", "no value"), ("w + 1", "not an assignment")])
+def test_synaptic_code_refusals(code, msg):
+    with pytest.raises(NotImplementedError, match=msg):
+        cg.synapse_code(code, ["w"], ["v"], ["v"])
+    with pytest.raises(NotImplementedError, match="only 'name : unit'"):
+        cg.synapse_variables("dw/dt = -w/tau : 1")

@@ -107,3 +107,110 @@ def test_code_cuba_matches_the_builtin_model(hs, oracle_lib):
         np.testing.assert_allclose(na.get_state(0, v), nb.get_state(0, v), rtol=0, atol=1e-12)
     na.close()
     nb.close()
+
+
+# ---- synaptic code --------------------------------------------------------------------------
+def _sorted_indices_spec(n=5):
+    m, names = cg.neuron_model("", "True", None, 0.0, None, {})
+    code = cg.synapse_code("w += 1", cg.synapse_variables("w : 1"), names, names)
+    i = np.arange(n, dtype=np.int32)
+    return nw.NetSpec("sorted_indices", 1e-4, 1, [nw.GroupSpec(0, n, [], {}, code=m)],
+                      [nw.PathSpec(0, 0, 0, i, i, delay_steps=np.arange(n, dtype=np.int32), code=code)])
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine", ["plain", "fused"])
+def test_sorted_indices_statemonitor(hs, prec, engine):
+    """The reference's test_sorted_indices_statemonitor (test_cpp_cuda_consistency.py:177-205):
+    NeuronGroup(5, model='', threshold='True'), on_pre 'w += 1', connect(j='i'),
+    delay k*dt; StateMonitor(S, 'w') at the start of step s holds max(0, s - k)."""
+    n = 5
+    net, _, _ = nw.build_hs(_sorted_indices_spec(n), prec, engine)
+    mid = net.add_synapse_monitor(0, rt.HS_SYN_CODE, np.arange(n))
+    net.run(n + 1)
+    s, k = np.meshgrid(np.arange(n + 1), np.arange(n), indexing="ij")
+    np.testing.assert_array_equal(net.state_monitor(mid, n), np.maximum(0, s - k))
+    np.testing.assert_array_equal(net.synapse_var(0, rt.HS_SYN_CODE), n + 1 - np.arange(n))
+    net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine", ["plain", "fused"])
+def test_additive_code_equals_the_builtin_effect(hs, prec, engine):
+    """C1 with 'ge += we' / 'gi += wi' as synaptic code: the built-in pathways' bits."""
+    a = nw.cuba(N=4000, seed=4, record=0)
+    b = nw.cuba(N=4000, seed=4, record=0)
+    for p, var in zip(b.paths, ("ge", "gi")):
+        p.code = cg.synapse_code(f"{var} += {p.wscalar!r}", [], ["v", "ge", "gi"], ["v", "ge", "gi"])
+    na, sa, _ = nw.build_hs(a, prec, engine)
+    nb, sb, _ = nw.build_hs(b, prec, engine)
+    na.run(800)
+    nb.run(800)
+    assert np.array_equal(na.spikes(sa[0])[0], nb.spikes(sb[0])[0])
+    assert na.stats()["events"] == nb.stats()["events"] > 0
+    for v in range(3):
+        assert np.array_equal(na.get_state(0, v), nb.get_state(0, v))
+    na.close()
+    nb.close()
+
+
+def _plastic_code_spec(seed=3):
+    """Poisson inputs onto code-model LIF neurons through synaptic code that adds
+    a constant to ge, counts arrivals and moves w with the source's rate."""
+    rng = np.random.default_rng(seed)
+    lif, names = cg.neuron_model("dv/dt = (ge - v)/(10*ms) : volt\ndge/dt = -ge/(5*ms) : volt",
+                                 "v > 5*mV", "v = 0*volt", 2e-3, "euler", {})
+    npo, nl = 400, 60
+    pre, post = nw.bernoulli_connect(rng, npo, nl, 0.2)
+    d = rng.integers(0, 12, pre.size).astype(np.int32)
+    code = cg.synapse_code("ge_post += 1.5*mV\nhits += 1\nw = clip(w + 1e-3*rates_pre*w - 0.002, 0, 1)",
+                           ["w", "hits"], names, ["rates"])
+    code.init = {"w": rng.random(pre.size), "hits": 0.0}
+    rates = 5.0 + 30.0 * rng.random(npo)
+    groups = [nw.GroupSpec(rt.HS_MODEL_POISSON, npo, [20.0, 2.0], {0: rates}), nw.GroupSpec(0, nl, [], {}, code=lif)]
+    spec = nw.NetSpec("plastic_code", 1e-4, seed, groups, [nw.PathSpec(0, 1, 0, pre, post, delay_steps=d, code=code)])
+    spec.spike_monitors = [1]
+    return spec
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine,sem", [("plain", 0), ("fused", 1)])
+def test_synaptic_code_vs_oracle(hs, oracle_lib, prec, engine, sem):
+    spec = _plastic_code_spec()
+    p = spec.paths[0]
+    net, gsm, _ = nw.build_hs(spec, prec, engine)
+    ref, rsm, _ = oracle_lib.build_ref(spec, prec, sem)
+    mid = net.add_synapse_monitor(0, rt.HS_SYN_CODE, np.arange(20))
+    order = nw.bundle_order(p)
+    rmid = ref.add_synapse_monitor(0, rt.HS_SYN_CODE, order[:20])
+    for k in (300, 900):
+        net.run(k)
+        ref.run(k)
+    gi, gt = net.spikes(gsm[0])
+    ri, rt_ = ref.spikes(rsm[0])
+    assert len(gi) == len(ri) > 0 and np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    assert net.stats()["events"] == ref.events
+    for k in range(2):
+        g = net.synapse_var(0, rt.HS_SYN_CODE + k)
+        r = ref.synapse_var(0, rt.HS_SYN_CODE + k, len(p.pre))[order]
+        assert np.array_equal(g, r), k
+    assert np.array_equal(net.state_monitor(mid, 20), ref.state_monitor(rmid, 20))
+    for v in range(2):
+        assert np.array_equal(net.get_state(1, v), ref.get_state(1, v))
+    net.close()
+
+
+def test_synaptic_code_errors(hs):
+    net = rt.Net(64, "plain", 1e-4, 1)
+    g = net.add_group(rt.HS_MODEL_LIF_CUBA, 10, nw.cuba_params())
+    pid = net.add_pathway(g, g, 1, np.arange(10), np.arange(10)[::-1], scalar_delay=1)
+    for args in [(["dt"], "", [], []), (["w", "w"], "", [], []), (["w"], "{", [], []),
+                 (["w"], "", [(7, "v")], []), (["w"], "", [], [(-1, "v")])]:
+        names, code, post, pre = args
+        c = cg.SynapseCode(names, code, [n for _, n in post], [n for _, n in pre])
+        with pytest.raises(HsError):
+            net.pathway_code(pid, c, [k for k, _ in post], [k for k, _ in pre])
+    net.pathway_code(pid, cg.SynapseCode(["w"], "w += ;", [], []), [], [])
+    with pytest.raises(HsError, match="do not compile"):
+        net.run(1)
+    net.close()
