@@ -65,6 +65,7 @@ class SynapsesDesc:
     source: str
     target: str
     on_pre: str
+    model: str = ""                       # synapse variables, 'name : unit' lines (any on_pre then becomes code)
     source_range: Optional[tuple] = None  # a Subgroup source G[a:b]
     target_range: Optional[tuple] = None
     namespace: Dict[str, float] = field(default_factory=dict)
@@ -337,7 +338,23 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
         n_src, n_tgt = gspecs[src].n, gspecs[tgt].n
         sr = s.source_range or (0, n_src)
         tr = s.target_range or (0, n_tgt)
-        var, wkind, wval, neg = _on_pre(s.on_pre, gvars[tgt], s)
+        code = None
+        try:
+            if s.model.strip():
+                raise NotImplementedError("declared synapse variables: the code form")
+            var, wkind, wval, neg = _on_pre(s.on_pre, gvars[tgt], s)
+        except NotImplementedError:
+            # any other on_pre: synaptic code over the declared synapse variables
+            # (and those given values), additions to the target's variables
+            from .codegen import synapse_code, synapse_variables
+            syn_vars = synapse_variables(s.model)
+            syn_vars += [k for k in s.values if k not in syn_vars]
+            code = synapse_code(s.on_pre, syn_vars, gvars[tgt], gvars[src], s.namespace,
+                                what=f"Synapses {s.name!r}")
+            code.init = {k: v for k, v in s.values.items() if k in syn_vars}
+            var, neg = 0, False
+            wkind, wval = ("synapse", "w") if "w" in syn_vars and ("w" in s.values or s.weight_range) \
+                else ("scalar", 0.0)
         if s.p is not None or s.k is not None:
             if s.p is not None and s.k is not None:
                 raise ValueError(f"Synapses {s.name!r}: connect() takes p or a sample size K, not both")
@@ -353,12 +370,15 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
                 w = (-hi, -lo) if neg else (lo, hi)
                 rc = nw.RandomConnect(float(s.p or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
                                       autapses=autapses, delay=delay, wmode=HS_W_SYNAPSE, w=w, k=int(s.k or 0))
-                paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SYNAPSE, random=rc))
+                paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SYNAPSE, random=rc, code=code))
             else:
                 rc = nw.RandomConnect(float(s.p or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
                                       autapses=autapses, delay=delay, wmode=HS_W_SCALAR, w=(wval, wval),
                                       k=int(s.k or 0))
-                paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SCALAR, wscalar=wval, random=rc))
+                paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SCALAR, wscalar=wval, random=rc,
+                                         code=code))
+            if code is not None:
+                code.init = {k: v for k, v in code.init.items() if np.asarray(v).size == 1}
             continue
         if s.i is None or s.j is None:
             raise NotImplementedError(f"Synapses {s.name!r}: connect() with i/j arrays, p= or a sample size only")
@@ -376,10 +396,10 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
             w = np.asarray(s.values[wval], np.float64)
             w = np.broadcast_to(w, pre.shape).astype(np.float64)
             paths.append(nw.PathSpec(src, tgt, var, pre.astype(np.int32), post.astype(np.int32), wmode=HS_W_SYNAPSE,
-                                     weights=-w if neg else w, **kw))
+                                     weights=-w if neg else w, code=code, **kw))
         else:
             paths.append(nw.PathSpec(src, tgt, var, pre.astype(np.int32), post.astype(np.int32), wmode=HS_W_SCALAR,
-                                     wscalar=wval, **kw))
+                                     wscalar=wval, code=code, **kw))
     spec = nw.NetSpec(name, float(dt), int(seed), gspecs, paths)
     names = {"groups": [g.name for g in groups], "paths": [s.name for s in synapses], "spike_monitors": [],
              "state_monitors": [], "rate_monitors": []}

@@ -36,7 +36,7 @@ import numpy as np
 
 from . import networks as nw
 from .runtime import (HS_ENGINE_FUSED, HS_ENGINE_PLAIN, HS_MODEL_HH_COBA, HS_MODEL_LIF_CUBA, HS_MODEL_LIF_NOISE,
-                      HS_MODEL_POISSON, HS_MODEL_SPIKEGEN, HS_W_SCALAR)
+                      HS_MODEL_POISSON, HS_MODEL_SPIKEGEN, HS_W_SCALAR, HS_W_SYNAPSE)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
@@ -245,6 +245,46 @@ class ProjectWriter:
                 L.append(f"    {{ const double sp[] = {{{sp}}};")
                 L.append(f"      HS_CHECK(hs_net_pathway_stdp(net, p_{pname}, sp, {len(p.stdp)})); }}")
                 pinfo["stdp"] = list(map(float, p.stdp))
+            if p.code is not None:  # on_pre as code: compiled for the device on the first run
+                c = p.code
+                post_i, pre_i = nw.code_var_indices(spec, p)
+                def arr(xs, empty):
+                    return "{" + (", ".join(xs) if xs else empty) + "}"
+
+                def quoted(names):
+                    return ['"' + n + '"' for n in names]
+
+                sn = arr(quoted(c.syn_names), "nullptr")
+                pv = arr([str(k) for k in post_i], "0")
+                pn = arr(quoted(c.post_names), "nullptr")
+                qv = arr([str(k) for k in pre_i], "0")
+                qn = arr(quoted(c.pre_names), "nullptr")
+                L.append(f"    {{ const char *sn[] = {sn};")
+                L.append(f"      const int pv[] = {pv};")
+                L.append(f"      const char *pn[] = {pn};")
+                L.append(f"      const int qv[] = {qv};")
+                L.append(f"      const char *qn[] = {qn};")
+                L.append(f"      HS_CHECK(hs_net_pathway_code(net, p_{pname}, {len(c.syn_names)}, sn,")
+                L.append(f"          {_c_str(c.on_pre)},")
+                L.append(f"          {len(post_i)}, pv, pn, {len(pre_i)}, qv, qn)); }}")
+                for name, val in (c.init or {}).items():
+                    if name == "w" and p.wmode == HS_W_SYNAPSE:
+                        continue  # the pathway's weights
+                    k = c.syn_names.index(name)
+                    a = np.asarray(val, np.float64)
+                    L.append("    { int64_t ns = 0; int dm, dd, mo;")
+                    L.append(f"      HS_CHECK(hs_net_pathway_info(net, p_{pname}, &ns, &dm, &dd, &mo));")
+                    if a.size > 1:
+                        if p.random is not None:
+                            raise ValueError("per-synapse initial values need explicit synapses")
+                        rv, nv = self._static(directory, f"_array_{pname}_{name}", a[nw.bundle_order(p)],
+                                              np.float64)
+                        L.append(f"      std::vector<double> v({max(nv, 1)});")
+                        L.append(f"      HS_IO(read_array(\"{rv}\", v.data(), sizeof(double) * {nv}));")
+                    else:
+                        L.append(f"      std::vector<double> v((size_t)(ns ? ns : 1), {_c_double(float(a.ravel()[0]))});")
+                    L.append(f"      HS_CHECK(hs_net_set_synapse_var(net, p_{pname}, HS_SYN_CODE + {k}, v.data(), ns)); }}")
+                pinfo["code"] = {"syn_vars": list(c.syn_names), "on_pre": c.on_pre}
             info["paths"].append(pinfo)
         writes = []
         for mi, gid in enumerate(spec.spike_monitors):

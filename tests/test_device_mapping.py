@@ -168,7 +168,8 @@ def test_unsupported_groups_are_refused(change, msg):
 
 
 @pytest.mark.parametrize("on_pre,kw,msg", [
-    ("w += 1", {}, "not 'var \\+= w'"),
+    ("w += 1", {}, "not a synapse or target variable"),
+    ("w += v_post", dict(model="w : 1"), "reads the target variable"),
     ("ge += we", dict(p=0.1, condition="i % 2 == 0"), "condition"),
     ("ge += w", dict(p=0.1, values={"w": 1.0}), "'lo \\+ \\(hi-lo\\)\\*rand\\(\\)'"),
 ])
@@ -237,3 +238,24 @@ def test_adex_example_maps_to_a_code_model():
     c = spec.groups[0].code
     assert c.names == ["vm", "w", "I"] and "_hs_k4_vm" in c.update and c.reset.count(";") == 2
     np.testing.assert_array_equal(spec.groups[0].init[2], np.linspace(0, 1e-9, 100))
+
+
+def test_synaptic_code_from_the_description():
+    """The reference's sorted-indices network (test_cpp_cuda_consistency.py:177-205)
+    described: NeuronGroup(5, '', threshold='True'), Synapses(P, P, 'w : 1',
+    on_pre='w += 1'), connect(j='i'), delays k*dt -> a code model and a code
+    pathway; an on_pre that reads a source variable and adds to a target one."""
+    g = dv.GroupDesc("neurongroup", 5, equations="", threshold="True")
+    s = dv.SynapsesDesc("synapses", "neurongroup", "neurongroup", "w += 1", model="w : 1", i=np.arange(5),
+                        j=np.arange(5), delay=np.arange(5) * 0.1 * ms)
+    spec, _ = dv.network_spec([g], [s], [], dt=0.1 * ms)
+    p = spec.paths[0]
+    assert spec.groups[0].code.threshold == "true" and p.code.on_pre == "w += ((R)1);"
+    np.testing.assert_array_equal(p.delay_steps, np.arange(5))
+    lif = dv.GroupDesc("lif", 10, equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr", namespace=CUBA_NS)
+    s2 = dv.SynapsesDesc("s2", "lif", "lif", "ge_post += w*(1 + 0*v_pre)\nc += 1", model="w : volt\nc : 1",
+                         i=[0, 1], j=[2, 3], values={"w": np.array([1e-3, 2e-3])})
+    spec2, _ = dv.network_spec([lif], [s2], [], dt=0.1 * ms)
+    q = spec2.paths[0]
+    assert q.code.syn_names == ["w", "c"] and q.code.post_names == ["ge"] and q.code.pre_names == ["v"]
+    assert q.wmode == 1 and list(q.weights) == [1e-3, 2e-3]

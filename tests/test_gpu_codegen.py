@@ -154,29 +154,10 @@ def test_additive_code_equals_the_builtin_effect(hs, prec, engine):
     nb.close()
 
 
-def _plastic_code_spec(seed=3):
-    """Poisson inputs onto code-model LIF neurons through synaptic code that adds
-    a constant to ge, counts arrivals and moves w with the source's rate."""
-    rng = np.random.default_rng(seed)
-    lif, names = cg.neuron_model("dv/dt = (ge - v)/(10*ms) : volt\ndge/dt = -ge/(5*ms) : volt",
-                                 "v > 5*mV", "v = 0*volt", 2e-3, "euler", {})
-    npo, nl = 400, 60
-    pre, post = nw.bernoulli_connect(rng, npo, nl, 0.2)
-    d = rng.integers(0, 12, pre.size).astype(np.int32)
-    code = cg.synapse_code("ge_post += 1.5*mV\nhits += 1\nw = clip(w + 1e-3*rates_pre*w - 0.002, 0, 1)",
-                           ["w", "hits"], names, ["rates"])
-    code.init = {"w": rng.random(pre.size), "hits": 0.0}
-    rates = 5.0 + 30.0 * rng.random(npo)
-    groups = [nw.GroupSpec(rt.HS_MODEL_POISSON, npo, [20.0, 2.0], {0: rates}), nw.GroupSpec(0, nl, [], {}, code=lif)]
-    spec = nw.NetSpec("plastic_code", 1e-4, seed, groups, [nw.PathSpec(0, 1, 0, pre, post, delay_steps=d, code=code)])
-    spec.spike_monitors = [1]
-    return spec
-
-
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("engine,sem", [("plain", 0), ("fused", 1)])
 def test_synaptic_code_vs_oracle(hs, oracle_lib, prec, engine, sem):
-    spec = _plastic_code_spec()
+    spec = nw.code_synapses()
     p = spec.paths[0]
     net, gsm, _ = nw.build_hs(spec, prec, engine)
     ref, rsm, _ = oracle_lib.build_ref(spec, prec, sem)

@@ -148,3 +148,20 @@ def test_code_models_through_the_project(hs, oracle_lib, tmp_path):
         np.testing.assert_array_equal(res[name]["t"], rstep * spec.dt)
     np.testing.assert_array_equal(res["neurongroup.v"], ref.get_state(0, 0))
     np.testing.assert_array_equal(res["neurongroup.w"], ref.get_state(0, 1))
+
+
+def test_synaptic_code_through_the_project(hs, oracle_lib, tmp_path):
+    """Synaptic code with per-synapse initial values (networks.code_synapses)
+    through the project: spikes and the target state equal the oracle's."""
+    spec = nw.code_synapses()
+    project.write_project(spec, tmp_path, 1000, precision=64, engine="plain")
+    project.build_project(tmp_path)
+    project.run_project(tmp_path, "results", timeout=180)
+    res = project.read_results(tmp_path)
+    ref, rsm, _ = _oracle(oracle_lib, spec, 64, oracle_lib.SEQ, 1000)
+    ri, rstep = ref.spikes(rsm[0])
+    assert len(ri) > 0
+    np.testing.assert_array_equal(res["spikemonitor"]["i"], ri)
+    np.testing.assert_array_equal(res["spikemonitor"]["t"], rstep * spec.dt)
+    np.testing.assert_array_equal(res["neurongroup_1.v"], ref.get_state(1, 0))
+    np.testing.assert_array_equal(res["neurongroup_1.ge"], ref.get_state(1, 1))
