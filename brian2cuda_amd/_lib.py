@@ -76,6 +76,8 @@ SIGNATURES = {
                                       C.POINTER(C.c_int), C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_char_p)]),
     "hs_net_set_synapse_var": (C.c_int, [P, C.c_int, C.c_int, DP, C.c_int64]),
+    "hs_net_pathway_code_post": (C.c_int, [P, C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_char_p)]),
     "hs_net_add_group_code": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_char_p), C.c_char_p, C.c_char_p,
                                         C.c_char_p, C.c_double, C.c_int, C.c_uint32, C.POINTER(C.c_int)]),
     "hs_net_set_state": (C.c_int, [P, C.c_int, C.c_int, DP]),

@@ -31,7 +31,7 @@ the network's precision R, so an FP32 network computes in FP32 throughout.
 """
 import ast
 import re
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Dict, List, Optional
 
 import numpy as np
@@ -132,7 +132,7 @@ def parse_equations(text):
         else:
             raise NotImplementedError(f"equation {st!r} is not 'dx/dt = ...', 'x = ...' or 'x : unit'")
         for f in flags:
-            if f not in ("unless refractory", "constant"):
+            if f not in ("unless refractory", "constant", "event-driven"):
                 raise NotImplementedError(f"equation {st!r}: flag {f!r} is not supported")
     names = [e.name for e in eqs]
     dup = {n for n in names if names.count(n) > 1}
@@ -287,6 +287,9 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
     is needed by 'exact' only."""
     ns = dict(namespace or {})
     eqs = parse_equations(equations)
+    for e in eqs:
+        if "event-driven" in e.flags:
+            raise NotImplementedError(f"{what}: '(event-driven)' belongs to Synapses equations")
     diff = [e for e in eqs if e.kind == "diff"]
     params = [e for e in eqs if e.kind == "param"]
     subs = {e.name: e for e in eqs if e.kind == "sub"}
@@ -526,81 +529,149 @@ def _statements(text, state, sym, pr, what):
 # ---- synaptic code ---------------------------------------------------------------------
 @dataclass
 class SynapseCode:
-    """The arguments of hs_net_pathway_code for one pathway."""
+    """The arguments of hs_net_pathway_code (and hs_net_pathway_code_post) for one pathway."""
     syn_names: List[str]
     on_pre: str
-    post_names: List[str]          # target group variables the code adds to
-    pre_names: List[str]           # source group variables the code reads
+    post_names: List[str]          # target group variables on_pre adds to
+    pre_names: List[str]           # source group variables on_pre and on_post read
     init: Optional[Dict[str, object]] = None  # synapse variable -> scalar or array (synapse creation order)
+    on_post: str = ""              # statements run for each synapse of a spiking target neuron
+    post_read_names: List[str] = field(default_factory=list)  # target variables on_post reads
+
+
+def synapse_model(model):
+    """(variable names, event-driven equations [(name, rhs)]) of a Synapses model:
+    'name : unit' variables and 'dx/dt = f : unit (event-driven)' traces."""
+    names, ed = [], []
+    for e in parse_equations(model or ""):
+        if e.kind == "param":
+            names.append(e.name)
+        elif e.kind == "diff" and "event-driven" in e.flags:
+            names.append(e.name)
+            ed.append((e.name, e.expr))
+        else:
+            raise NotImplementedError(f"Synapses model line {e.name!r}: only 'name : unit' variables and "
+                                      "'(event-driven)' equations are supported")
+    return names, ed
 
 
 def synapse_variables(model):
-    """The variables of a Synapses model given as 'name : unit' lines (differential
-    equations of synapses are not supported here)."""
-    out = []
-    for e in parse_equations(model or ""):
-        if e.kind != "param":
-            raise NotImplementedError(f"Synapses model line {e.name!r}: only 'name : unit' variables are supported")
-        out.append(e.name)
+    """The variables of a Synapses model ('name : unit' lines and event-driven traces)."""
+    return synapse_model(model)[0]
+
+
+def _advance_code(ed, ns, what):
+    """Statements that bring the event-driven variables from lastupdate to t, each
+    solved exactly: dx/dt = a x + b (a, b constants) over the elapsed time."""
+    if not ed:
+        return []
+    out = ["const R _hs_el = (R)(t - lastupdate);"]
+    EL = sp.Symbol("_hs_el", real=True)
+    for name, rhs in ed:
+        x = sp.Symbol(name, real=True)
+
+        def sym(n):
+            if n == name:
+                return x
+            if n in ns:
+                return sp.Float(float(ns[n]))
+            if n in UNITS:
+                return sp.Float(UNITS[n])
+            raise NotImplementedError(f"{what}: event-driven d{name}/dt may use only {name} and constants ({n!r})")
+
+        f = _parse(rhs, sym, f"{what}: d{name}/dt = {rhs}")
+        a, b = sp.diff(f, x), f.subs(x, 0)
+        if not (a.is_Number and b.is_Number):
+            raise NotImplementedError(f"{what}: event-driven d{name}/dt is not linear with constant coefficients")
+        pr = _Printer({x: name, EL: "_hs_el"})
+        if a == 0:
+            out.append(f"{name} = {pr(x + b * EL)};")
+        elif b == 0:
+            out.append(f"{name} = {name}*exp({pr(a * EL)});")
+        else:
+            out.append(f"{name} = {pr(-b / a)} + ({name} + {pr(b / a)})*exp({pr(a * EL)});")
     return out
 
 
-def synapse_code(on_pre, syn_vars, post_vars, pre_vars, namespace=None, what="Synapses"):
-    """SynapseCode of a Synapses object's on_pre statements: 'x = e', 'x += e', ...
-    on synapse variables, and 'y += e' / 'y -= e' on the target's variables (y or
-    y_post).  Expressions read synapse variables, the source's variables
-    (y_pre), the namespace, units, t, dt, i and j; reading the target's
-    variables is not supported (other synapses of the step add to them)."""
+def synapse_code(on_pre, syn_vars, post_vars, pre_vars, namespace=None, what="Synapses", on_post=None, model=None):
+    """SynapseCode of a Synapses object: on_pre statements ('x = e', 'x += e', ...
+    on synapse variables; 'y += e' / 'y -= e' on the target's variables, y or
+    y_post) and on_post statements (synapse variables only).  Expressions read
+    synapse variables, the source's variables (y_pre), the namespace, units, t,
+    dt, i, j and lastupdate; on_post may also read the target's variables
+    (y_post); on_pre may not (other synapses of the step add to them).
+    model: the Synapses model text; its event-driven equations advance their
+    variables from lastupdate to t before either statement list, exactly."""
     ns = dict(namespace or {})
     syn_vars, post_vars, pre_vars = list(syn_vars), list(post_vars), list(pre_vars)
-    used_post, used_pre = [], []
-    syms = {}
+    ed = []
+    if model:
+        mnames, ed = synapse_model(model)
+        syn_vars += [n for n in mnames if n not in syn_vars]
+    advance = _advance_code(ed, ns, what)
+    used_post, used_pre, read_post = [], [], []
 
-    def S(name):
-        if name not in syms:
-            syms[name] = sp.Symbol(name, real=True)
-        return syms[name]
+    def translate(text, post_reads):
+        syms = {}
 
-    def sym(name):
-        if name in syn_vars:
-            return S(name)
-        if name.endswith("_pre") and name[:-4] in pre_vars:
-            if name[:-4] not in used_pre:
-                used_pre.append(name[:-4])
-            return S(name)
-        if name in ("t", "dt", "i", "j"):
-            return S("_hs_" + name)
-        if (name.endswith("_post") and name[:-5] in post_vars) or name in post_vars:
-            raise NotImplementedError(f"{what}: on_pre reads the target variable {name!r} (not supported)")
-        if name in ns:
-            return sp.Float(float(ns[name]))
-        if name in UNITS:
-            return sp.Float(UNITS[name])
-        raise NotImplementedError(f"{what}: no value for {name!r} in on_pre")
+        def S(name):
+            if name not in syms:
+                syms[name] = sp.Symbol(name, real=True)
+            return syms[name]
 
-    out = []
-    for st in re.split(r"[;\n]", on_pre or ""):
-        st = st.split("#")[0].strip()
-        if not st:
-            continue
-        m = re.match(r"^([A-Za-z_]\w*)\s*(=|\+=|-=|\*=|/=)\s*(.+)$", st)
-        if not m:
-            raise NotImplementedError(f"{what}: on_pre {st!r} is not an assignment")
-        x, op, e = m.groups()
-        expr = _parse(e, sym, f"{what}: on_pre {st!r}")
-        if x in syn_vars:
-            target = x
-        else:
-            base = x[:-5] if x.endswith("_post") else x
-            if base not in post_vars:
-                raise NotImplementedError(f"{what}: on_pre assigns to {x!r}, not a synapse or target variable")
-            if op not in ("+=", "-="):
-                raise NotImplementedError(f"{what}: on_pre {st!r}: target variables take additions only")
-            if base not in used_post:
-                used_post.append(base)
-            target = base + "_post"
-        local = {s: n for n, s in syms.items()}
-        local.update({syms[k]: v for k, v in (("_hs_t", "((R)t)"), ("_hs_dt", "dt"), ("_hs_i", "((R)i)"),
-                                                  ("_hs_j", "((R)j)")) if k in syms})
-        out.append(f"{target} {op} {_Printer(local)(expr)};")
-    return SynapseCode(syn_vars, "\n".join(out), used_post, used_pre)
+        def sym(name):
+            if name in syn_vars:
+                return S(name)
+            if name.endswith("_pre") and name[:-4] in pre_vars:
+                if name[:-4] not in used_pre:
+                    used_pre.append(name[:-4])
+                return S(name)
+            if name in ("t", "dt", "i", "j", "lastupdate"):
+                return S("_hs_" + name)
+            if (name.endswith("_post") and name[:-5] in post_vars) or name in post_vars:
+                if not post_reads:
+                    raise NotImplementedError(f"{what}: on_pre reads the target variable {name!r} (not supported)")
+                base = name[:-5] if name.endswith("_post") else name
+                if base not in read_post:
+                    read_post.append(base)
+                return S(base + "_post")
+            if name in ns:
+                return sp.Float(float(ns[name]))
+            if name in UNITS:
+                return sp.Float(UNITS[name])
+            raise NotImplementedError(f"{what}: no value for {name!r}")
+
+        out = []
+        for st in re.split(r"[;\n]", text or ""):
+            st = st.split("#")[0].strip()
+            if not st:
+                continue
+            m = re.match(r"^([A-Za-z_]\w*)\s*(=|\+=|-=|\*=|/=)\s*(.+)$", st)
+            if not m:
+                raise NotImplementedError(f"{what}: {st!r} is not an assignment")
+            x, op, e = m.groups()
+            expr = _parse(e, sym, f"{what}: {st!r}")
+            if x in syn_vars:
+                target = x
+            elif post_reads:
+                raise NotImplementedError(f"{what}: on_post assigns to {x!r}; it may assign synapse variables only")
+            else:
+                base = x[:-5] if x.endswith("_post") else x
+                if base not in post_vars:
+                    raise NotImplementedError(f"{what}: on_pre assigns to {x!r}, not a synapse or target variable")
+                if op not in ("+=", "-="):
+                    raise NotImplementedError(f"{what}: on_pre {st!r}: target variables take additions only")
+                if base not in used_post:
+                    used_post.append(base)
+                target = base + "_post"
+            local = {s_: n for n, s_ in syms.items()}
+            local.update({syms[k]: v for k, v in (("_hs_t", "((R)t)"), ("_hs_dt", "dt"), ("_hs_i", "((R)i)"),
+                                                      ("_hs_j", "((R)j)"), ("_hs_lastupdate", "((R)lastupdate)"))
+                          if k in syms})
+            out.append(f"{target} {op} {_Printer(local)(expr)};")
+        return out
+
+    pre_code = translate(on_pre, False)
+    post_code = translate(on_post, True) if on_post else []
+    return SynapseCode(syn_vars, "\n".join(advance + pre_code), used_post, used_pre, None,
+                       "\n".join(advance + post_code) if on_post else "", read_post)

@@ -25,6 +25,10 @@ struct HsCgModel {
 struct HsSynCgModel {
     std::vector<std::string> syn_names, post_names, pre_names;
     std::string on_pre;
+    /* on_post (empty: none): statements for each synapse of a spiking target
+       neuron, reading the target variables <post_read_names[k]>_post */
+    std::string on_post;
+    std::vector<std::string> post_read_names;
 };
 
 /* the engine kernels of one network, compiled for its code models (null when
@@ -35,6 +39,7 @@ struct HsCgKernels {
     hipFunction_t fused = nullptr;
     hipFunction_t finalize[2] = {nullptr, nullptr}; /* plain, fused */
     hipFunction_t syn_code = nullptr;
+    hipFunction_t syn_post = nullptr;
 };
 
 /* checks a model's names and code; returns HS_OK or fails with a message */
@@ -58,8 +63,11 @@ hipError_t hs_cg_launch_fused(const HsCgKernels *K, const void *groups, int ng, 
                               int np, HsCtl *ctl, int koff, int block, int n_upd, int n_del, hipStream_t s);
 hipError_t hs_cg_launch_finalize(const HsCgKernels *K, const void *groups, int ng, int total_tiles, HsCtl *ctl,
                                  int fused, int block, hipStream_t s);
-/* the synaptic code of step ctl->base + koff, after the step's update and delivery */
+/* the synaptic code of step ctl->base + koff, after the step's update and delivery:
+   on_pre of the bundles due, then on_post of the targets' spikes */
 hipError_t hs_cg_launch_syn_code(const HsCgKernels *K, const void *paths, int np, const void *groups, HsCtl *ctl,
                                  int koff, int fused, int grid, hipStream_t s);
+hipError_t hs_cg_launch_syn_post(const HsCgKernels *K, const void *paths, int np, const void *groups, HsCtl *ctl,
+                                 int koff, int grid, hipStream_t s);
 
 #endif /* HS_CODEGEN_H */

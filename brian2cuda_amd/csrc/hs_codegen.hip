@@ -31,6 +31,12 @@
 #include "../../include/hs.h"
 #include "hs_codegen.h"
 
+#define HS_CG_TRY(expr)               \
+    do {                              \
+        const int _r = (expr);        \
+        if (_r != HS_OK) return _r;   \
+    } while (0)
+
 /* hs_rtc_names[k], hs_rtc_texts[k] (k < hs_rtc_count): the engine's sources by
    the names its #include lines use; written by the library's build script */
 #include "hs_rtc_sources.inc"
@@ -73,17 +79,11 @@ int hs_cg_check(const HsCgModel &m)
     return HS_OK;
 }
 
-int hs_cg_check_syn(const HsSynCgModel &m)
+/* the names in scope of one synaptic code function, and its statements */
+static int hs_cg_check_scope(const std::vector<std::string> &all, const std::string &code)
 {
-    static const char *const reserved[] = {"R", "dt", "t", "i", "j", "exp", "log", "log10", "sqrt", "abs", "fabs",
-                                           "clip", "floor", "ceil", "sign", "int"};
-    if ((int)m.syn_names.size() > HS_MAX_SYN_VARS || (int)m.post_names.size() > HS_MAX_VARS ||
-        (int)m.pre_names.size() > HS_MAX_VARS)
-        return hs_rt_fail(HS_ERR_ARG, "synaptic code: at most %d synapse and %d target/source variables",
-                          HS_MAX_SYN_VARS, HS_MAX_VARS);
-    std::vector<std::string> all = m.syn_names;
-    for (const std::string &n : m.post_names) all.push_back(n + "_post");
-    for (const std::string &n : m.pre_names) all.push_back(n + "_pre");
+    static const char *const reserved[] = {"R", "dt", "t", "i", "j", "lastupdate", "exp", "log", "log10", "sqrt",
+                                           "abs", "fabs", "clip", "floor", "ceil", "sign", "int"};
     for (size_t a = 0; a < all.size(); ++a) {
         const std::string &n = all[a];
         if (!hs_cg_ident(n) || n.compare(0, 3, "hs_") == 0 || n.compare(0, 2, "__") == 0)
@@ -93,10 +93,10 @@ int hs_cg_check_syn(const HsSynCgModel &m)
         for (size_t b = 0; b < a; ++b)
             if (all[b] == n) return hs_rt_fail(HS_ERR_ARG, "synaptic code name '%s' twice", n.c_str());
     }
-    if (m.on_pre.find('#') != std::string::npos)
+    if (code.find('#') != std::string::npos)
         return hs_rt_fail(HS_ERR_ARG, "synaptic code holds a preprocessor directive");
     int depth = 0;
-    for (char ch : m.on_pre) {
+    for (char ch : code) {
         depth += ch == '{' ? 1 : (ch == '}' ? -1 : 0);
         if (depth < 0) break;
     }
@@ -104,25 +104,61 @@ int hs_cg_check_syn(const HsSynCgModel &m)
     return HS_OK;
 }
 
-/* one precision's specialisation of synaptic code c */
+int hs_cg_check_syn(const HsSynCgModel &m)
+{
+    if ((int)m.syn_names.size() > HS_MAX_SYN_VARS || (int)m.post_names.size() > HS_MAX_VARS ||
+        (int)m.pre_names.size() > HS_MAX_VARS || (int)m.post_read_names.size() > HS_MAX_VARS)
+        return hs_rt_fail(HS_ERR_ARG, "synaptic code: at most %d synapse and %d target/source variables",
+                          HS_MAX_SYN_VARS, HS_MAX_VARS);
+    std::vector<std::string> pre_scope = m.syn_names, post_scope = m.syn_names;
+    for (const std::string &n : m.pre_names) {
+        pre_scope.push_back(n + "_pre");
+        post_scope.push_back(n + "_pre");
+    }
+    for (const std::string &n : m.post_names) pre_scope.push_back(n + "_post");
+    for (const std::string &n : m.post_read_names) post_scope.push_back(n + "_post");
+    HS_CG_TRY(hs_cg_check_scope(pre_scope, m.on_pre));
+    return hs_cg_check_scope(post_scope, m.on_post);
+}
+
+/* one precision's specialisation of synaptic code c: on_pre(sv, pre, eff, ...)
+   and on_post(sv, pre, post, ...) over named locals */
 static void hs_cg_syn_struct(std::string &o, const char *R, int c, const HsSynCgModel &m)
 {
-    const int ns = (int)m.syn_names.size(), npost = (int)m.post_names.size(), npre = (int)m.pre_names.size();
+    const int ns = (int)m.syn_names.size(), npost = (int)m.post_names.size(), npre = (int)m.pre_names.size(),
+              npr = (int)m.post_read_names.size();
     o += "template <> struct HsSynCg<" + std::string(R) + ", " + std::to_string(c) + "> {\n";
     o += "    static constexpr int NS = " + std::to_string(ns) + ", NPOST = " + std::to_string(npost) +
-         ", NPRE = " + std::to_string(npre) + ";\n";
+         ", NPRE = " + std::to_string(npre) + ", NPR = " + std::to_string(npr) +
+         ", HAS_POST = " + std::string(m.on_post.empty() ? "0" : "1") + ";\n";
     o += "    typedef " + std::string(R) + " R;\n";
+    auto head = [&]() {
+        for (int v = 0; v < ns; ++v) o += "        R " + m.syn_names[v] + " = hs_sv[" + std::to_string(v) + "];\n";
+        for (int v = 0; v < npre; ++v)
+            o += "        const R " + m.pre_names[v] + "_pre = hs_pre[" + std::to_string(v) + "];\n";
+    };
+    auto tail = [&]() {
+        for (int v = 0; v < ns; ++v) o += "        hs_sv[" + std::to_string(v) + "] = " + m.syn_names[v] + ";\n";
+        for (int v = 0; v < npre; ++v) o += "        (void)" + m.pre_names[v] + "_pre;\n";
+    };
     o += "    static __device__ __forceinline__ void on_pre(R *hs_sv, const R *hs_pre, R *hs_eff, R dt, double t, "
-         "int i, int j)\n    {\n        (void)hs_sv; (void)hs_pre; (void)hs_eff; (void)dt; (void)t; (void)i; "
-         "(void)j;\n";
-    for (int v = 0; v < ns; ++v) o += "        R " + m.syn_names[v] + " = hs_sv[" + std::to_string(v) + "];\n";
-    for (int v = 0; v < npre; ++v)
-        o += "        const R " + m.pre_names[v] + "_pre = hs_pre[" + std::to_string(v) + "];\n";
+         "int i, int j, double lastupdate)\n    {\n        (void)hs_sv; (void)hs_pre; (void)hs_eff; (void)dt; "
+         "(void)t; (void)i; (void)j; (void)lastupdate;\n";
+    head();
     for (int v = 0; v < npost; ++v) o += "        R " + m.post_names[v] + "_post = (R)0;\n";
     o += "        {\n" + m.on_pre + "\n        }\n";
-    for (int v = 0; v < ns; ++v) o += "        hs_sv[" + std::to_string(v) + "] = " + m.syn_names[v] + ";\n";
+    tail();
     for (int v = 0; v < npost; ++v) o += "        hs_eff[" + std::to_string(v) + "] = " + m.post_names[v] + "_post;\n";
-    for (int v = 0; v < npre; ++v) o += "        (void)" + m.pre_names[v] + "_pre;\n";
+    o += "    }\n";
+    o += "    static __device__ __forceinline__ void on_post(R *hs_sv, const R *hs_pre, const R *hs_post, R dt, "
+         "double t, int i, int j, double lastupdate)\n    {\n        (void)hs_sv; (void)hs_pre; (void)hs_post; "
+         "(void)dt; (void)t; (void)i; (void)j; (void)lastupdate;\n";
+    head();
+    for (int v = 0; v < npr; ++v)
+        o += "        const R " + m.post_read_names[v] + "_post = hs_post[" + std::to_string(v) + "];\n";
+    o += "        {\n" + m.on_post + "\n        }\n";
+    tail();
+    for (int v = 0; v < npr; ++v) o += "        (void)" + m.post_read_names[v] + "_post;\n";
     o += "    }\n};\n";
 }
 
@@ -251,6 +287,8 @@ int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const std::vec
     if (!syn.empty()) {
         names.push_back(std::string("hs_k_syn_code<") + R + ">");
         slots.push_back(&e.k.syn_code);
+        names.push_back(std::string("hs_k_syn_post<") + R + ">");
+        slots.push_back(&e.k.syn_post);
     }
     const int nk = (int)names.size();
     hiprtcProgram prog;
@@ -362,4 +400,12 @@ hipError_t hs_cg_launch_syn_code(const HsCgKernels *K, const void *paths, int np
     if (!K->syn_code || grid < 1) return hipErrorInvalidValue;
     void *args[] = {&paths, &np, &groups, &ctl, &koff, &fused};
     return hs_cg_go(K->syn_code, grid, 256, s, args);
+}
+
+hipError_t hs_cg_launch_syn_post(const HsCgKernels *K, const void *paths, int np, const void *groups, HsCtl *ctl,
+                                 int koff, int grid, hipStream_t s)
+{
+    if (!K->syn_post || grid < 1) return hipErrorInvalidValue;
+    void *args[] = {&paths, &np, &groups, &ctl, &koff};
+    return hs_cg_go(K->syn_post, grid, 256, s, args);
 }

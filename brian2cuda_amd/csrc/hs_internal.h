@@ -168,6 +168,12 @@ struct DPath {
     int cg, nsv, npost, npre;
     R *sv[HS_MAX_SYN_VARS];
     int post_var[HS_MAX_VARS], pre_var[HS_MAX_VARS];
+    /* on_post: reads target variables post_read_var[0..npr); synpre[q] is the
+       source neuron of synapse q (bundle order); lastup (above) the step of the
+       synapse's last on_pre or on_post */
+    int npr;
+    int post_read_var[HS_MAX_VARS];
+    const int32_t *synpre;
 };
 
 /*

@@ -458,7 +458,8 @@ struct HPath {
        and source variables of the code */
     int cg = -1, nsv = 0;
     void *sv[HS_MAX_SYN_VARS] = {nullptr};
-    std::vector<int> post_vars, pre_vars;
+    std::vector<int> post_vars, pre_vars, post_read_vars;
+    int32_t *synpre = nullptr; /* on_post: source neuron of each synapse (bundle order) */
 };
 
 struct HMon {
@@ -1251,16 +1252,10 @@ extern "C" int hs_net_set_spikegen(hs_net *net, int gid, int64_t n, const int32_
 
 /* STDP on a pathway: traces and last-update steps allocated (zero: Brian2's
    initial values), the target-indexed synapse table built from the bundles */
-extern "C" int hs_net_pathway_stdp(hs_net *net, int pid, const double *params, int nparams)
+/* the synapses of each target neuron j: qsyn[qstart[j] .. qstart[j+1]) in bundle
+   order (the post pathway's table, built once from the bundles) */
+static int hs_post_index(hs_net *net, HPath &P)
 {
-    if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");
-    if (net->built) return hs_fail(HS_ERR_STATE, "plasticity must be set before the first run");
-    if (!params || nparams != HS_STDP_NPARAMS) return hs_fail(HS_ERR_ARG, "STDP takes %d parameters", HS_STDP_NPARAMS);
-    HPath &P = net->p[pid];
-    if (P.dmode != HS_DELIVER_SCATTER) return hs_fail(HS_ERR_UNSUPPORTED, "STDP needs a scatter-mode pathway");
-    if (P.wmode != HS_W_SYNAPSE) return hs_fail(HS_ERR_ARG, "STDP needs per-synapse weights (HS_W_SYNAPSE)");
-    if (!(params[0] > 0.0) || !(params[1] > 0.0)) return hs_fail(HS_ERR_ARG, "trace time constants must be > 0");
-    if (!(params[4] <= params[5])) return hs_fail(HS_ERR_ARG, "wmin > wmax");
     const HGroup &gt = net->g[P.tgt];
     const size_t ns = (size_t)(P.nsyn ? P.nsyn : 1);
     std::vector<int32_t> po(ns), qs((size_t)gt.n + 1, 0), qy(ns);
@@ -1271,15 +1266,30 @@ extern "C" int hs_net_pathway_stdp(hs_net *net, int pid, const double *params, i
     for (int j = 0; j < gt.n; ++j) qs[(size_t)j + 1] += qs[(size_t)j];
     std::vector<int32_t> fill(qs.begin(), qs.end() - 1);
     for (int64_t s = 0; s < P.nsyn; ++s) qy[(size_t)fill[(size_t)po[(size_t)s]]++] = (int32_t)s;
-    const size_t rs = hs_rsize(net);
-    HS_TRY(hs_dalloc(net, rs * ns, &P.apre, true));
-    HS_TRY(hs_dalloc(net, rs * ns, &P.apost, true));
-    HS_TRY(hs_dalloc(net, sizeof(int32_t) * ns, (void **)&P.lastup, true));
     HS_TRY(hs_dalloc(net, sizeof(int32_t) * qs.size(), (void **)&P.qstart, false));
     HS_TRY(hs_dalloc(net, sizeof(int32_t) * ns, (void **)&P.qsyn, false));
     HS_HIP(hipMemcpyAsync(P.qstart, qs.data(), sizeof(int32_t) * qs.size(), hipMemcpyHostToDevice, g_rt.stream));
     HS_HIP(hipMemcpyAsync(P.qsyn, qy.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, g_rt.stream));
-    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream)); /* host vectors die here */
+    return HS_OK;
+}
+
+extern "C" int hs_net_pathway_stdp(hs_net *net, int pid, const double *params, int nparams)
+{
+    if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");
+    if (net->built) return hs_fail(HS_ERR_STATE, "plasticity must be set before the first run");
+    if (!params || nparams != HS_STDP_NPARAMS) return hs_fail(HS_ERR_ARG, "STDP takes %d parameters", HS_STDP_NPARAMS);
+    HPath &P = net->p[pid];
+    if (P.dmode != HS_DELIVER_SCATTER) return hs_fail(HS_ERR_UNSUPPORTED, "STDP needs a scatter-mode pathway");
+    if (P.wmode != HS_W_SYNAPSE) return hs_fail(HS_ERR_ARG, "STDP needs per-synapse weights (HS_W_SYNAPSE)");
+    if (!(params[0] > 0.0) || !(params[1] > 0.0)) return hs_fail(HS_ERR_ARG, "trace time constants must be > 0");
+    if (!(params[4] <= params[5])) return hs_fail(HS_ERR_ARG, "wmin > wmax");
+    const size_t ns = (size_t)(P.nsyn ? P.nsyn : 1);
+    const size_t rs = hs_rsize(net);
+    HS_TRY(hs_dalloc(net, rs * ns, &P.apre, true));
+    HS_TRY(hs_dalloc(net, rs * ns, &P.apost, true));
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * ns, (void **)&P.lastup, true));
+    HS_TRY(hs_post_index(net, P));
     for (int j = 0; j < HS_STDP_NPARAMS; ++j) P.stdp[j] = params[j];
     P.dmode = HS_DELIVER_PLASTIC;
     net->n_scatter--;
@@ -1328,6 +1338,7 @@ extern "C" int hs_net_pathway_code(hs_net *net, int pid, int nsvars, const char 
         if (m.syn_names[k] == "w" && P.wmode == HS_W_SYNAPSE) P.sv[k] = P.w; /* the pathway's weights */
         else HS_TRY(hs_dalloc(net, hs_rsize(net) * ns, &P.sv[k], true));
     }
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * ns, (void **)&P.lastup, true)); /* lastupdate = 0 s */
     P.nsv = nsvars;
     P.post_vars.assign(post_vars, post_vars + npost);
     P.pre_vars.assign(pre_vars, pre_vars + npre);
@@ -1335,6 +1346,45 @@ extern "C" int hs_net_pathway_code(hs_net *net, int pid, int nsvars, const char 
     P.dmode = HS_DELIVER_CODE;
     net->cg_syn.push_back(m);
     net->n_scatter--;
+    net->desc_dirty = true;
+    return HS_OK;
+}
+
+extern "C" int hs_net_pathway_code_post(hs_net *net, int pid, const char *on_post, int npost,
+                                        const int *post_vars, const char *const *post_names)
+{
+    if (!net || pid < 0 || pid >= (int)net->p.size()) return hs_fail(HS_ERR_ARG, "bad pathway");
+    if (net->built) return hs_fail(HS_ERR_STATE, "synaptic code must be set before the first run");
+    HPath &P = net->p[pid];
+    if (P.dmode != HS_DELIVER_CODE) return hs_fail(HS_ERR_ARG, "on_post needs a pathway with on_pre code");
+    if (P.qstart) return hs_fail(HS_ERR_STATE, "the pathway has on_post code already");
+    if (!on_post || !*on_post) return hs_fail(HS_ERR_ARG, "empty on_post code");
+    if (npost < 0 || npost > HS_MAX_VARS || (npost && (!post_vars || !post_names)))
+        return hs_fail(HS_ERR_ARG, "hs_net_pathway_code_post: bad variable list");
+    const HGroup &gt = net->g[P.tgt];
+    HsSynCgModel m = net->cg_syn[(size_t)P.cg];
+    m.on_post = on_post;
+    m.post_read_names.clear();
+    for (int k = 0; k < npost; ++k) {
+        if (!post_names[k] || post_vars[k] < 0 || post_vars[k] >= gt.mi.nvars)
+            return hs_fail(HS_ERR_ARG, "target variable %d of the on_post code", k);
+        m.post_read_names.push_back(post_names[k]);
+    }
+    HS_TRY(hs_cg_check_syn(m));
+    /* the source neuron of each synapse, from the bundle starts */
+    const HGroup &gs = net->g[P.src];
+    const size_t nb = (size_t)gs.n * (size_t)P.D + 1, ns = (size_t)(P.nsyn ? P.nsyn : 1);
+    std::vector<int32_t> bs(nb), sp(ns, 0);
+    HS_HIP(hipMemcpyAsync(bs.data(), P.bstart, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    for (size_t r = 0; r + 1 < nb; ++r)
+        for (int32_t q = bs[r]; q < bs[r + 1]; ++q) sp[(size_t)q] = (int32_t)(r / (size_t)P.D);
+    HS_TRY(hs_dalloc(net, sizeof(int32_t) * ns, (void **)&P.synpre, false));
+    HS_HIP(hipMemcpyAsync(P.synpre, sp.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice, g_rt.stream));
+    HS_HIP(hipStreamSynchronize(g_rt.stream));
+    HS_TRY(hs_post_index(net, P));
+    P.post_read_vars.assign(post_vars, post_vars + npost);
+    net->cg_syn[(size_t)P.cg] = m;
     net->desc_dirty = true;
     return HS_OK;
 }
@@ -1368,6 +1418,15 @@ extern "C" int hs_net_get_synapse_var(hs_net *net, int pid, int var, double *out
     const HPath &P = net->p[pid];
     if (P.dmode == HS_DELIVER_GATHER) return hs_fail(HS_ERR_UNSUPPORTED, "synapse readback of gather-mode pathways");
     if (cap < P.nsyn) return hs_fail(HS_ERR_ARG, "capacity %lld < %lld synapses", (long long)cap, (long long)P.nsyn);
+    if (P.dmode == HS_DELIVER_CODE && var == HS_SYN_LASTUPDATE) {
+        std::vector<int32_t> v((size_t)(P.nsyn ? P.nsyn : 1));
+        if (P.nsyn)
+            HS_HIP(hipMemcpyAsync(v.data(), P.lastup, sizeof(int32_t) * (size_t)P.nsyn, hipMemcpyDeviceToHost,
+                                  g_rt.stream));
+        HS_HIP(hipStreamSynchronize(g_rt.stream));
+        for (int64_t q = 0; q < P.nsyn; ++q) out[q] = (double)v[(size_t)q];
+        return HS_OK;
+    }
     if (P.dmode == HS_DELIVER_CODE && var >= HS_SYN_CODE && var < HS_SYN_CODE + P.nsv) {
         if (P.nsyn == 0) return HS_OK;
         const size_t n = (size_t)P.nsyn;
@@ -1881,6 +1940,21 @@ static int hs_upload_desc_t(hs_net *net)
         for (int k = 0; k < d.npost; ++k) d.post_var[k] = P.post_vars[(size_t)k];
         d.npre = (int)P.pre_vars.size();
         for (int k = 0; k < d.npre; ++k) d.pre_var[k] = P.pre_vars[(size_t)k];
+        if (P.dmode == HS_DELIVER_CODE) {
+            d.lastup = P.lastup;
+            d.synpre = P.synpre;
+            d.npr = (int)P.post_read_vars.size();
+            for (int k = 0; k < d.npr; ++k) d.post_read_var[k] = P.post_read_vars[(size_t)k];
+            if (P.qstart) {
+                d.qstart = P.qstart;
+                d.qsyn = P.qsyn;
+                d.thist = gt.hist;
+                d.thist_cnt = gt.hist_cnt;
+                d.tQ = gt.Q;
+                d.tnsub = gt.nsub;
+                d.tsubcap = gt.subcap;
+            }
+        }
     }
     if (!net->dgroups) HS_TRY(hs_dalloc(net, sizeof(DGroup<R>) * std::max<size_t>(1, dg.size()), &net->dgroups, false));
     if (!net->dpaths) HS_TRY(hs_dalloc(net, sizeof(DPath<R>) * std::max<size_t>(1, dp.size()), &net->dpaths, false));
@@ -2109,9 +2183,14 @@ static int hs_enqueue_steps_t(hs_net *net, int64_t k)
                 if (np > 0) HS_TRY(deliver((int)j, 1));
                 if (net->n_plastic)
                     HS_HIP(hs_launch_stdp<R>(dp, np, net->ctl, (int)j, std::max(1, hs_rt_cus()), g_rt.stream));
-                if (!net->cg_syn.empty())
+                if (!net->cg_syn.empty()) {
                     HS_HIP(hs_cg_launch_syn_code(net->cg, dp, np, dg, net->ctl, (int)j, fused,
                                                  std::max(1, hs_rt_cus()), g_rt.stream));
+                    if (std::any_of(net->cg_syn.begin(), net->cg_syn.end(),
+                                    [](const HsSynCgModel &c) { return !c.on_post.empty(); }))
+                        HS_HIP(hs_cg_launch_syn_post(net->cg, dp, np, dg, net->ctl, (int)j,
+                                                     std::max(1, hs_rt_cus()), g_rt.stream));
+                }
             }
         }
     }

@@ -139,15 +139,17 @@ def group_var_names(g):
 
 
 def code_var_indices(spec, p):
-    """(target, source) variable indices of a code pathway's post_names / pre_names."""
+    """(target, source, target-read) variable indices of a code pathway's
+    post_names / pre_names / post_read_names."""
     tn, sn = group_var_names(spec.groups[p.tgt]), group_var_names(spec.groups[p.src])
-    for n in p.code.post_names:
+    for n in list(p.code.post_names) + list(p.code.post_read_names):
         if n not in tn:
             raise ValueError(f"the target group has no variable {n!r}")
     for n in p.code.pre_names:
         if n not in sn:
             raise ValueError(f"the source group has no variable {n!r}")
-    return [tn.index(n) for n in p.code.post_names], [sn.index(n) for n in p.code.pre_names]
+    return ([tn.index(n) for n in p.code.post_names], [sn.index(n) for n in p.code.pre_names],
+            [tn.index(n) for n in p.code.post_read_names])
 
 
 def bundle_order(p):
@@ -348,6 +350,42 @@ def code_synapses(seed=3, npo=400, nl=60, p=0.2):
     spec = NetSpec("code_synapses", 0.1 * ms, seed, groups, [PathSpec(0, 1, 0, pre, post, delay_steps=d, code=code)])
     spec.spike_monitors = [1]
     return spec
+
+
+STDP_SYNAPSE_MODEL = """
+w : 1
+dApre/dt = -Apre / taupre : 1 (event-driven)
+dApost/dt = -Apost / taupost : 1 (event-driven)
+"""
+
+
+def stdp_feedforward(delay_steps=0, n_in=100, n_out=60, seed=5, code=False):
+    """A spike generator (at most one spike per step) onto LIF neurons, all to
+    all, with STDP (test_cpp_cuda_consistency.py:45-61's synapse model):
+    built in (hs_net_pathway_stdp), or with code=True the same synapse model
+    written as Brian2 code -- event-driven traces, on_pre 'ge += w*scale;
+    Apre += dApre; w = clip(w + Apost, wmin, wmax)', on_post 'Apost += dApost;
+    w = clip(w + Apre, wmin, wmax)' -- through the code generator."""
+    from .codegen import synapse_code
+    rs = np.random.RandomState(seed)
+    times = rs.choice(np.arange(3000), 1500, replace=False)
+    src = rs.randint(0, n_in, times.size)
+    G = GroupSpec(HS_MODEL_SPIKEGEN, n_in, [0.0], spikes=(src.astype(np.int32), times.astype(np.int64)))
+    P = GroupSpec(HS_MODEL_LIF_CUBA, n_out, [-49e-3, -50e-3, -60e-3, 20e-3, 5e-3, 5e-3, 5e-3, 1.0],
+                  {0: -60e-3 + 10e-3 * rs.rand(n_out), 1: 0.0})
+    ii, jj = np.meshgrid(np.arange(n_in), np.arange(n_out), indexing="ij")
+    w = 1.5 + 2.0 * rs.rand(n_in * n_out)
+    params = dict(taupre=20e-3, taupost=20e-3, dApre=0.05, dApost=-0.055, wmin=0.0, wmax=4.0, scale=1e-3)
+    S = PathSpec(0, 1, 1, ii.ravel().astype(np.int32), jj.ravel().astype(np.int32), scalar_delay=delay_steps,
+                 wmode=HS_W_SYNAPSE, weights=w)
+    if code:
+        S.code = synapse_code("ge += w*scale\nApre += dApre\nw = clip(w + Apost, wmin, wmax)", [], ["v", "ge", "gi"],
+                              ["_placeholder"], params, on_post="Apost += dApost\nw = clip(w + Apre, wmin, wmax)",
+                              model=STDP_SYNAPSE_MODEL)
+    else:
+        S.stdp = tuple(params[k] for k in ("taupre", "taupost", "dApre", "dApost", "wmin", "wmax", "scale"))
+    return NetSpec("stdp_ff" + ("_code" if code else ""), 0.1 * ms, seed, [G, P], [S], spike_monitors=[1],
+                   state_monitors=[(1, 0, 0, 20)])
 
 
 # ---- C2: COBAHH ------------------------------------------------------------------
@@ -578,8 +616,8 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
         if p.stdp is not None:
             net.pathway_stdp(pid, *p.stdp)
         if p.code is not None:
-            post_i, pre_i = code_var_indices(spec, p)
-            net.pathway_code(pid, p.code, post_i, pre_i)
+            post_i, pre_i, read_i = code_var_indices(spec, p)
+            net.pathway_code(pid, p.code, post_i, pre_i, read_i)
             for name, val in (p.code.init or {}).items():
                 if name == "w" and p.wmode == HS_W_SYNAPSE:
                     continue  # the pathway's weights

@@ -247,7 +247,7 @@ class ProjectWriter:
                 pinfo["stdp"] = list(map(float, p.stdp))
             if p.code is not None:  # on_pre as code: compiled for the device on the first run
                 c = p.code
-                post_i, pre_i = nw.code_var_indices(spec, p)
+                post_i, pre_i, read_i = nw.code_var_indices(spec, p)
                 def arr(xs, empty):
                     return "{" + (", ".join(xs) if xs else empty) + "}"
 
@@ -266,7 +266,16 @@ class ProjectWriter:
                 L.append(f"      const char *qn[] = {qn};")
                 L.append(f"      HS_CHECK(hs_net_pathway_code(net, p_{pname}, {len(c.syn_names)}, sn,")
                 L.append(f"          {_c_str(c.on_pre)},")
-                L.append(f"          {len(post_i)}, pv, pn, {len(pre_i)}, qv, qn)); }}")
+                L.append(f"          {len(post_i)}, pv, pn, {len(pre_i)}, qv, qn));")
+                if c.on_post:
+                    rv = arr([str(k) for k in read_i], "0")
+                    rn = arr(quoted(c.post_read_names), "nullptr")
+                    L.append(f"      const int rv[] = {rv};")
+                    L.append(f"      const char *rn[] = {rn};")
+                    L.append(f"      HS_CHECK(hs_net_pathway_code_post(net, p_{pname},")
+                    L.append(f"          {_c_str(c.on_post)},")
+                    L.append(f"          {len(read_i)}, rv, rn));")
+                L.append("    }")
                 for name, val in (c.init or {}).items():
                     if name == "w" and p.wmode == HS_W_SYNAPSE:
                         continue  # the pathway's weights

@@ -220,15 +220,20 @@ class Net:
         self.groups.append((HS_MODEL_CODE + sum(1 for m, _ in self.groups if m >= HS_MODEL_CODE), n))
         return gid.value
 
-    def pathway_code(self, pid, code, post_vars, pre_vars):
-        """on_pre code (a brian2cuda_amd.codegen.SynapseCode) on pathway pid: the
-        code adds to target variables post_vars and reads source variables
-        pre_vars (indices, in the order of code.post_names / code.pre_names)."""
+    def pathway_code(self, pid, code, post_vars, pre_vars, post_read_vars=()):
+        """Synaptic code (a brian2cuda_amd.codegen.SynapseCode) on pathway pid:
+        on_pre adds to target variables post_vars, both codes read source
+        variables pre_vars, on_post reads target variables post_read_vars
+        (indices, in the order of code.post_names / pre_names / post_read_names)."""
         enc = lambda names: (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
         pv = (C.c_int * max(1, len(post_vars)))(*post_vars)
         qv = (C.c_int * max(1, len(pre_vars)))(*pre_vars)
         call("hs_net_pathway_code", self.h, pid, len(code.syn_names), enc(code.syn_names), code.on_pre.encode(),
              len(post_vars), pv, enc(code.post_names), len(pre_vars), qv, enc(code.pre_names))
+        if code.on_post:
+            rv = (C.c_int * max(1, len(post_read_vars)))(*post_read_vars)
+            call("hs_net_pathway_code_post", self.h, pid, code.on_post.encode(), len(post_read_vars), rv,
+                 enc(code.post_read_names))
 
     def set_synapse_var(self, pid, var, values):
         """Synapse variable var (HS_SYN_CODE + k) of a code pathway, values in bundle order."""

@@ -311,7 +311,9 @@ int hs_net_pathway_stdp(hs_net *net, int pid, const double *params, int nparams)
  *                       'x_post += ...': additions only, as several synapses of a
  *                       step may share a target)
  *   <pre_names[k]>_pre  the source neuron's variable pre_vars[k] (read)
- *   dt (R), t (double, the step's time), i (source index), j (target index)
+ *   dt (R), t (double, the step's time), i (source index), j (target index),
+ *   lastupdate (double: the time of the synapse's last on_pre or on_post
+ *   event, 0 at first; HS_SYN_LASTUPDATE reads its step)
  * and the functions of hs_net_add_group_code.  The pathway's own target
  * variable and weight effect are replaced by the code.  Needs a scatter
  * pathway; before the first run; compiled with the network's code models on
@@ -321,6 +323,20 @@ int hs_net_pathway_stdp(hs_net *net, int pid, const double *params, int nparams)
 int hs_net_pathway_code(hs_net *net, int pid, int nsvars, const char *const *syn_names, const char *on_pre, int npost,
                         const int *post_vars, const char *const *post_names, int npre, const int *pre_vars,
                         const char *const *pre_names);
+/*
+ * Give a pathway with on_pre code (hs_net_pathway_code) on_post code as well:
+ * statements run at step t for every synapse of each target neuron that
+ * spikes at t, after the step's on_pre code (Brian2 orders the pre pathway
+ * before the post pathway).  In scope: the synapse variables (read and write),
+ * the source variables <pre_names>_pre of hs_net_pathway_code, the target
+ * variables <post_names[k]>_post (read; variable post_vars[k] of the target),
+ * dt, t, i, j and lastupdate (double: the time of the synapse's last on_pre or
+ * on_post; every such event sets it to t, starting from 0).  Writes go to
+ * synapse variables only.  Replaces the post pathway's effect kernel
+ * (synapses.cu:201-322) for such synapses; before the first run.
+ */
+int hs_net_pathway_code_post(hs_net *net, int pid, const char *on_post, int npost, const int *post_vars,
+                             const char *const *post_names);
 /* Set synapse variable var (HS_SYN_CODE + k) of a pathway with on_pre code from
    n = its synapse count values in bundle order (hs_net_get_synapses' order). */
 int hs_net_set_synapse_var(hs_net *net, int pid, int var, const double *values, int64_t n);

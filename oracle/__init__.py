@@ -68,6 +68,7 @@ def lib():
         "ref_net_pathway_code": (C.c_int, [V, C.c_int, V, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int,
                                            C.POINTER(C.c_int)]),
         "ref_net_set_synapse_var": (C.c_int, [V, C.c_int, C.c_int, DP]),
+        "ref_net_pathway_code_post": (C.c_int, [V, C.c_int, V, C.c_int, C.POINTER(C.c_int)]),
         "ref_net_set_state": (C.c_int, [V, C.c_int, C.c_int, DP]),
         "ref_net_get_state": (C.c_int, [V, C.c_int, C.c_int, DP]),
         "ref_net_get_lastspike": (C.c_int, [V, C.c_int, DP]),
@@ -247,19 +248,25 @@ def code_model_source(model, precision):
 
 
 def synapse_code_source(code, precision):
-    """The host program of one pathway's on_pre code: extern "C" hs_scg_on_pre."""
+    """The host program of one pathway's synaptic code: extern "C" hs_scg_on_pre
+    and hs_scg_on_post (a no-op without on_post code)."""
     R = "double" if precision == 64 else "float"
-    lines = [f"    R {n} = hs_sv[{k}];" for k, n in enumerate(code.syn_names)]
-    lines += [f"    const R {n}_pre = hs_pre[{k}];" for k, n in enumerate(code.pre_names)]
-    lines += [f"    R {n}_post = (R)0;" for n in code.post_names]
-    lines += ["    {", code.on_pre, "    }"]
-    lines += [f"    hs_sv[{k}] = {n};" for k, n in enumerate(code.syn_names)]
-    lines += [f"    hs_eff[{k}] = {n}_post;" for k, n in enumerate(code.post_names)]
-    lines += [f"    (void){n}_pre;" for n in code.pre_names]
+    head = [f"    R {n} = hs_sv[{k}];" for k, n in enumerate(code.syn_names)]
+    head += [f"    const R {n}_pre = hs_pre[{k}];" for k, n in enumerate(code.pre_names)]
+    tail = [f"    hs_sv[{k}] = {n};" for k, n in enumerate(code.syn_names)]
+    tail += [f"    (void){n}_pre;" for n in code.pre_names]
+    pre = head + [f"    R {n}_post = (R)0;" for n in code.post_names] + ["    {", code.on_pre, "    }"] + tail
+    pre += [f"    hs_eff[{k}] = {n}_post;" for k, n in enumerate(code.post_names)]
+    post = head + [f"    const R {n}_post = hs_post[{k}];" for k, n in enumerate(code.post_read_names)]
+    post += ["    {", code.on_post or "", "    }"] + tail
+    post += [f"    (void){n}_post;" for n in code.post_read_names]
+    args = "R dt, double t, int i, int j, double lastupdate"
+    unused = "(void)dt; (void)t; (void)i; (void)j; (void)lastupdate;"
     return (_CG_MATH + f"typedef {R} R;\n"
-            'extern "C" void hs_scg_on_pre(R *hs_sv, const R *hs_pre, R *hs_eff, R dt, double t, int i, int j)\n{\n'
-            "    (void)hs_sv; (void)hs_pre; (void)hs_eff; (void)dt; (void)t; (void)i; (void)j;\n"
-            + "\n".join(lines) + "\n}\n")
+            f'extern "C" void hs_scg_on_pre(R *hs_sv, const R *hs_pre, R *hs_eff, {args})\n{{\n'
+            f"    (void)hs_sv; (void)hs_pre; (void)hs_eff; {unused}\n" + "\n".join(pre) + "\n}\n"
+            f'extern "C" void hs_scg_on_post(R *hs_sv, const R *hs_pre, const R *hs_post, {args})\n{{\n'
+            f"    (void)hs_sv; (void)hs_pre; (void)hs_post; {unused}\n" + "\n".join(post) + "\n}\n")
 
 
 def _host_lib(src):
@@ -286,9 +293,12 @@ def _host_lib(src):
     return _cg_libs[key]
 
 
-def synapse_code_function(code, precision):
-    """The address of a pathway's on_pre code compiled for the host."""
-    return C.cast(_host_lib(synapse_code_source(code, precision)).hs_scg_on_pre, C.c_void_p).value
+def synapse_code_functions(code, precision):
+    """(on_pre, on_post) addresses of a pathway's synaptic code compiled for the
+    host (on_post None without on_post code)."""
+    L = _host_lib(synapse_code_source(code, precision))
+    post = C.cast(L.hs_scg_on_post, C.c_void_p).value if code.on_post else None
+    return C.cast(L.hs_scg_on_pre, C.c_void_p).value, post
 
 
 def code_model_functions(model, precision):
@@ -383,11 +393,12 @@ class RefNet:
         if r < 0:
             raise ValueError(f"bad spike generator pattern ({r})")
 
-    def pathway_code(self, pid, code, post_vars, pre_vars):
-        """on_pre code (brian2cuda_amd.codegen.SynapseCode) on pathway pid: the
-        code adds to target variables post_vars and reads source variables
-        pre_vars (indices, in the order of code.post_names / code.pre_names)."""
-        fn = synapse_code_function(code, self.precision)
+    def pathway_code(self, pid, code, post_vars, pre_vars, post_read_vars=()):
+        """Synaptic code (brian2cuda_amd.codegen.SynapseCode) on pathway pid: on_pre
+        adds to target variables post_vars, both read source variables pre_vars,
+        on_post reads target variables post_read_vars (indices, in the order of
+        code.post_names / pre_names / post_read_names)."""
+        fn, fn_post = synapse_code_functions(code, self.precision)
         w_sv = code.syn_names.index("w") if "w" in code.syn_names and self._wmode[pid] == 1 else -1
         pv = (C.c_int * max(1, len(post_vars)))(*post_vars)
         qv = (C.c_int * max(1, len(pre_vars)))(*pre_vars)
@@ -395,6 +406,11 @@ class RefNet:
                                         len(pre_vars), qv)
         if r < 0:
             raise ValueError(f"bad synaptic code ({r})")
+        if fn_post is not None:
+            rv = (C.c_int * max(1, len(post_read_vars)))(*post_read_vars)
+            r = self.L.ref_net_pathway_code_post(self.h, pid, fn_post, len(post_read_vars), rv)
+            if r < 0:
+                raise ValueError(f"bad on_post code ({r})")
 
     def set_synapse_var(self, pid, var, values, nsyn):
         v = np.ascontiguousarray(np.broadcast_to(np.asarray(values, dtype=np.float64), (nsyn,)))
@@ -539,7 +555,8 @@ def build_ref(spec, precision=64, semantics=SEQ, threads=1):
             tn = _var_names(spec.groups[p.tgt])
             sn = _var_names(spec.groups[p.src])
             net.pathway_code(pid, p.code, [tn.index(n) for n in p.code.post_names],
-                             [sn.index(n) for n in p.code.pre_names])
+                             [sn.index(n) for n in p.code.pre_names],
+                             [tn.index(n) for n in getattr(p.code, "post_read_names", [])])
             nsyn = len(syn["pre"]) if p.random is not None else len(p.pre)
             for name, val in (p.code.init or {}).items():
                 if name == "w" and (p.random.wmode if p.random is not None else p.wmode) == 1:

@@ -331,6 +331,10 @@ typedef struct {
     int nsv, npost, npre;
     void *sv[HS_MAX_SYN_VARS];
     int post_var[HS_MAX_VARS], pre_var[HS_MAX_VARS];
+    /* on_post code: fn(sv, pre, post, dt, t, i, j, lastupdate), reading target
+       variables post_read_var[0..npr) (post_start/by_post and lastup above) */
+    void *cg_post_fn;
+    int npr, post_read_var[HS_MAX_VARS];
 } ref_path;
 
 typedef struct {
@@ -657,7 +661,34 @@ int ref_net_pathway_code(ref_net *net, int pid, void *fn, int nsv, int w_sv, int
         if (net->sem == 1 && !gt->acc[post_vars[k]]) gt->acc[post_vars[k]] = calloc((size_t)gt->n, rsz(net));
     }
     for (int k = 0; k < npre; ++k) p->pre_var[k] = pre_vars[k];
+    p->lastup = (int64_t *)calloc((size_t)n, sizeof(int64_t)); /* lastupdate = 0 s */
     p->cg_fn = fn;
+    return 0;
+}
+
+/* on_post code fn(sv, pre, post, dt, t, i, j, lastupdate) of a code pathway,
+   reading target variables post_vars[0..npost) */
+int ref_net_pathway_code_post(ref_net *net, int pid, void *fn, int npost, const int *post_vars)
+{
+    if (pid < 0 || pid >= net->np || !fn || npost < 0 || npost > HS_MAX_VARS) return -1;
+    ref_path *p = &net->pw[pid];
+    const ref_group *gt = &net->g[p->tgt];
+    if (!p->cg_fn || p->cg_post_fn) return -2;
+    for (int k = 0; k < npost; ++k) {
+        if (post_vars[k] < 0 || post_vars[k] >= gt->nvars) return -3;
+        p->post_read_var[k] = post_vars[k];
+    }
+    p->npr = npost;
+    const int64_t n = p->nsyn ? p->nsyn : 1;
+    p->post_start = (int64_t *)calloc((size_t)gt->n + 1, sizeof(int64_t));
+    for (int64_t s = 0; s < p->nsyn; ++s) p->post_start[p->post[s] + 1]++;
+    for (int i = 0; i < gt->n; ++i) p->post_start[i + 1] += p->post_start[i];
+    p->by_post = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)gt->n);
+    memcpy(fill, p->post_start, sizeof(int64_t) * (size_t)gt->n);
+    for (int64_t s = 0; s < p->nsyn; ++s) p->by_post[fill[p->post[s]]++] = s;
+    free(fill);
+    p->cg_post_fn = fn;
     return 0;
 }
 
@@ -682,6 +713,10 @@ int ref_net_get_synapse_var(ref_net *net, int pid, int var, double *out)
     ref_path *p = &net->pw[pid];
     if (p->cg_fn && var >= HS_SYN_CODE && var < HS_SYN_CODE + p->nsv) {
         for (int64_t s = 0; s < p->nsyn; ++s) out[s] = rget(net, p->sv[var - HS_SYN_CODE], s);
+        return 0;
+    }
+    if (p->cg_fn && var == HS_SYN_LASTUPDATE) {
+        for (int64_t s = 0; s < p->nsyn; ++s) out[s] = (double)p->lastup[s];
         return 0;
     }
     for (int64_t s = 0; s < p->nsyn; ++s) {
@@ -952,6 +987,8 @@ void ref_net_destroy(ref_net *net)
             for (int64_t q = 0; q < (int64_t)net->nthr * pw->Q; ++q) free(pw->tq[q]);
         free(pw->tq); free(pw->tqlen); free(pw->tqcap);
         free(pw->apre); free(pw->apost); free(pw->lastup); free(pw->post_start); free(pw->by_post);
+        for (int k = 0; k < pw->nsv; ++k)
+            if (pw->sv[k] != pw->w) free(pw->sv[k]); /* the weight array is freed above */
     }
     for (int m = 0; m < net->nm; ++m) {
         free(net->mon[m].i); free(net->mon[m].step); free(net->mon[m].data); free(net->mon[m].sel);

@@ -54,6 +54,7 @@ int ref_net_set_spikegen(ref_net *net, int gid, int64_t n, const int32_t *idx, c
 int ref_net_pathway_stdp(ref_net *net, int pid, const double *params, int nparams);
 int ref_net_pathway_code(ref_net *net, int pid, void *fn, int nsv, int w_sv, int npost, const int *post_vars, int npre,
                          const int *pre_vars);
+int ref_net_pathway_code_post(ref_net *net, int pid, void *fn, int npost, const int *post_vars);
 int ref_net_set_synapse_var(ref_net *net, int pid, int var, const double *values);
 int ref_net_get_synapse_var(ref_net *net, int pid, int var, double *out);
 int ref_net_add_rate_monitor(ref_net *net, int gid, int start, int stop);

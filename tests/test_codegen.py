@@ -118,7 +118,8 @@ def test_code_mix_runs_in_both_semantics(prec, sem):
     ("dv/dt = v*v/tau : 1", {"method": "exponential_euler"}, "not linear"),
     ("dv/dt = -v/tau + xi/tau : 1", {"method": "rk4"}, "no noise"),
     ("dv/dt = -v/foo : 1", {}, "no value"),
-    ("dv/dt = -v/tau : 1 (event-driven)", {}, "flag"),
+    ("dv/dt = -v/tau : 1 (event-driven)", {}, "belongs to Synapses"),
+    ("dv/dt = -v/tau : 1 (clock-driven)", {}, "flag"),
     ("dv/dt = -v/tau : 1", {"method": "heun"}, "not supported"),
     ("dv/dt = v*t/tau : 1", {"method": "exact"}, "linear system"),
 ])
@@ -206,3 +207,49 @@ def test_synaptic_code_refusals(code, msg):
         cg.synapse_code(code, ["w"], ["v"], ["v"])
     with pytest.raises(NotImplementedError, match="only 'name : unit'"):
         cg.synapse_variables("dw/dt = -w/tau : 1")
+    with pytest.raises(NotImplementedError, match="constant coefficients|only"):
+        cg.synapse_code("w += 1", ["w"], ["v"], ["v"], {"tau": 1.0}, model="dA/dt = -A*w/tau : 1 (event-driven)")
+
+
+@pytest.mark.parametrize("delay", [0, 3])
+def test_stdp_as_code_matches_the_builtin_rule(delay):
+    """The STDP synapse model written as Brian2 code (event-driven traces,
+    on_pre and on_post) against the built-in rule on the same feed-forward
+    network: identical spikes, weights and traces to FP64 rounding (the two
+    compute the decay exponent differently), equal last-update steps."""
+    a = nw.stdp_feedforward(delay)
+    b = nw.stdp_feedforward(delay, code=True)
+    assert b.paths[0].code.syn_names == ["w", "Apre", "Apost"] and b.paths[0].code.on_post
+    ra, sa, _ = oracle.build_ref(a, 64, oracle.SEQ)
+    rb, sb, _ = oracle.build_ref(b, 64, oracle.SEQ)
+    ra.run(3000)
+    rb.run(3000)
+    (ia, ta), (ib, tb) = ra.spikes(sa[0]), rb.spikes(sb[0])
+    assert len(ia) > 50 and np.array_equal(ia, ib) and np.array_equal(ta, tb)
+    n = len(a.paths[0].pre)
+    for k, var in enumerate((0, 1, 2)):
+        np.testing.assert_allclose(rb.synapse_var(0, nw.HS_SYN_CODE + k, n), ra.synapse_var(0, var, n),
+                                   rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(rb.synapse_var(0, 3, n), ra.synapse_var(0, 3, n))
+
+
+def test_event_driven_trace_known_answer():
+    """One synapse, source spikes at steps 10 and 60, target at 30 (generators):
+    Apre after the second pre spike is dApre*exp(-50 dt/tau) + dApre; Apost,
+    updated at 60 from its value set at 30, is dApost*exp(-30 dt/tau)."""
+    tau, dA = 20e-3, 0.5
+    G = nw.GroupSpec(nw.HS_MODEL_SPIKEGEN, 1, [0.0], spikes=(np.array([0, 0], np.int32), np.array([10, 60])))
+    H = nw.GroupSpec(nw.HS_MODEL_SPIKEGEN, 1, [0.0], spikes=(np.array([0], np.int32), np.array([30])))
+    code = cg.synapse_code("Apre += dA\nn_pre += 1", ["n_pre"], ["_placeholder"], [], dict(tau=tau, dA=dA),
+                           on_post="Apost += dA", model="dApre/dt = -Apre/tau : 1 (event-driven)\n"
+                                                        "dApost/dt = -Apost/tau : 1 (event-driven)")
+    S = nw.PathSpec(0, 1, 0, np.array([0], np.int32), np.array([0], np.int32), code=code)
+    spec = nw.NetSpec("ed", DT, 1, [G, H], [S])
+    ref, _, _ = oracle.build_ref(spec, 64, oracle.SEQ)
+    ref.run(100)
+    names = code.syn_names
+    val = {n: ref.synapse_var(0, nw.HS_SYN_CODE + k, 1)[0] for k, n in enumerate(names)}
+    assert val["n_pre"] == 2
+    np.testing.assert_allclose(val["Apre"], dA * np.exp(-50 * DT / tau) + dA, rtol=1e-12)
+    np.testing.assert_allclose(val["Apost"], dA * np.exp(-30 * DT / tau), rtol=1e-12)  # decayed at 60
+    assert ref.synapse_var(0, 3, 1)[0] == 60

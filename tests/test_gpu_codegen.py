@@ -195,3 +195,44 @@ def test_synaptic_code_errors(hs):
     with pytest.raises(HsError, match="do not compile"):
         net.run(1)
     net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine,sem", [("plain", 0), ("fused", 1)])
+@pytest.mark.parametrize("delay", [0, 3])
+def test_stdp_as_code_vs_oracle(hs, oracle_lib, prec, engine, sem, delay):
+    """The STDP synapse model as Brian2 code (event-driven traces, on_pre and
+    on_post through the generator) on the feed-forward network: spikes,
+    weights, traces, last updates, voltages and events bit for bit."""
+    spec = nw.stdp_feedforward(delay, code=True)
+    S = spec.paths[0]
+    perm = nw.bundle_order(S)
+    net, sm, st = nw.build_hs(spec, prec, engine)
+    ref, rsm, rst = oracle_lib.build_ref(spec, prec, sem)
+    for k in (700, 1100, 1300):
+        net.run(k)
+        ref.run(k)
+        for var in [rt.HS_SYN_CODE + k for k in range(3)] + [rt.HS_SYN_LASTUPDATE]:
+            assert np.array_equal(net.synapse_var(0, var), ref.synapse_var(0, var, len(S.pre))[perm]), var
+    gi, gt = net.spikes(sm[0])
+    ri, rt_ = ref.spikes(rsm[0])
+    assert len(gi) > 50 and np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    assert np.array_equal(net.state_monitor(st[0], 20), ref.state_monitor(rst[0], 20))
+    assert net.stats()["events"] == ref.events
+    net.close()
+
+
+def test_stdp_as_code_matches_the_builtin_rule_gpu(hs):
+    """On the GPU, the code form and the built-in STDP agree: same spikes,
+    weights and traces to FP64 rounding."""
+    a, b = nw.stdp_feedforward(3), nw.stdp_feedforward(3, code=True)
+    na, sa, _ = nw.build_hs(a, 64, "plain")
+    nb, sb, _ = nw.build_hs(b, 64, "plain")
+    na.run(3000)
+    nb.run(3000)
+    assert np.array_equal(na.spikes(sa[0])[0], nb.spikes(sb[0])[0])
+    for k, var in enumerate((rt.HS_SYN_W, rt.HS_SYN_APRE, rt.HS_SYN_APOST)):
+        np.testing.assert_allclose(nb.synapse_var(0, rt.HS_SYN_CODE + k), na.synapse_var(0, var), rtol=1e-12,
+                                   atol=1e-15)
+    na.close()
+    nb.close()
