@@ -65,7 +65,8 @@ class SynapsesDesc:
     source: str
     target: str
     on_pre: str
-    model: str = ""                       # synapse variables, 'name : unit' lines (any on_pre then becomes code)
+    model: str = ""                       # synapse variables: 'name : unit' lines and event-driven traces
+    on_post: str = ""                     # statements per synapse of a spiking target neuron (code form)
     source_range: Optional[tuple] = None  # a Subgroup source G[a:b]
     target_range: Optional[tuple] = None
     namespace: Dict[str, float] = field(default_factory=dict)
@@ -340,17 +341,18 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
         tr = s.target_range or (0, n_tgt)
         code = None
         try:
-            if s.model.strip():
-                raise NotImplementedError("declared synapse variables: the code form")
+            if s.model.strip() or s.on_post.strip():
+                raise NotImplementedError("declared synapse variables or on_post: the code form")
             var, wkind, wval, neg = _on_pre(s.on_pre, gvars[tgt], s)
         except NotImplementedError:
-            # any other on_pre: synaptic code over the declared synapse variables
-            # (and those given values), additions to the target's variables
+            # any other on_pre (or an on_post): synaptic code over the declared
+            # synapse variables and event-driven traces (and those given values),
+            # additions to the target's variables
             from .codegen import synapse_code, synapse_variables
             syn_vars = synapse_variables(s.model)
             syn_vars += [k for k in s.values if k not in syn_vars]
             code = synapse_code(s.on_pre, syn_vars, gvars[tgt], gvars[src], s.namespace,
-                                what=f"Synapses {s.name!r}")
+                                what=f"Synapses {s.name!r}", on_post=s.on_post or None, model=s.model)
             code.init = {k: v for k, v in s.values.items() if k in syn_vars}
             var, neg = 0, False
             wkind, wval = ("synapse", "w") if "w" in syn_vars and ("w" in s.values or s.weight_range) \

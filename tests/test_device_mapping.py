@@ -259,3 +259,19 @@ def test_synaptic_code_from_the_description():
     q = spec2.paths[0]
     assert q.code.syn_names == ["w", "c"] and q.code.post_names == ["ge"] and q.code.pre_names == ["v"]
     assert q.wmode == 1 and list(q.weights) == [1e-3, 2e-3]
+
+
+def test_stdp_example_synapses_map_to_synaptic_code():
+    """examples/stdp.py's Synapses (event-driven Apre/Apost, on_pre and on_post)
+    described: a code pathway whose statements advance the traces first."""
+    g = dv.GroupDesc("neurons", 10, equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr", refractory=5 * ms,
+                     namespace=CUBA_NS)
+    ns = dict(taupre=20 * ms, taupost=20 * ms, dApre=0.01, dApost=-0.0105, gmax=0.01)
+    s = dv.SynapsesDesc("synapses", "neurons", "neurons", "ge += w\nApre += dApre\nw = clip(w + Apost, 0, gmax)",
+                        model=nw.STDP_SYNAPSE_MODEL, on_post="Apost += dApost\nw = clip(w + Apre, 0, gmax)",
+                        namespace=ns, i=np.arange(5), j=np.arange(5)[::-1], values={"w": np.full(5, 0.005)})
+    spec, _ = dv.network_spec([g], [s], [], dt=0.1 * ms)
+    c = spec.paths[0].code
+    assert c.syn_names == ["w", "Apre", "Apost"] and c.post_names == ["ge"]
+    assert c.on_pre.startswith("const R _hs_el = (R)(t - lastupdate);") and "exp(" in c.on_post
+    assert spec.paths[0].wmode == 1

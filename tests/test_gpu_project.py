@@ -165,3 +165,19 @@ def test_synaptic_code_through_the_project(hs, oracle_lib, tmp_path):
     np.testing.assert_array_equal(res["spikemonitor"]["t"], rstep * spec.dt)
     np.testing.assert_array_equal(res["neurongroup_1.v"], ref.get_state(1, 0))
     np.testing.assert_array_equal(res["neurongroup_1.ge"], ref.get_state(1, 1))
+
+
+def test_stdp_as_code_through_the_project(hs, oracle_lib, tmp_path):
+    """STDP written as synaptic code (event-driven traces, on_pre, on_post)
+    through the project: spikes and voltages equal the oracle's."""
+    spec = nw.stdp_feedforward(3, code=True)
+    project.write_project(spec, tmp_path, 2000, precision=64, engine="fused")
+    project.build_project(tmp_path)
+    project.run_project(tmp_path, "results", timeout=180)
+    res = project.read_results(tmp_path)
+    ref, rsm, _ = _oracle(oracle_lib, spec, 64, oracle_lib.ACC, 2000)
+    ri, rstep = ref.spikes(rsm[0])
+    assert len(ri) > 0
+    np.testing.assert_array_equal(res["spikemonitor"]["i"], ri)
+    np.testing.assert_array_equal(res["spikemonitor"]["t"], rstep * spec.dt)
+    np.testing.assert_array_equal(res["neurongroup_1.v"], ref.get_state(1, 0))
