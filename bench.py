@@ -583,9 +583,45 @@ def other_configs(args):
         bstep = spec.n_neurons * (r_s + w_s) + f["spikes_per_step"] * 12 + f["events_per_step"] * 4
         row["fp32_step_bytes_model"] = int(bstep)
         row["fp32_step_hbm_frac"] = round(bstep / (f["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
+        if name in ("cuba_4k", "cobahh_512k") and not args.no_code:
+            # the same network with its model as generated code (compiled at run
+            # time into the engine kernels), in the FP32 form chosen above
+            cspec = code_spec(name, args.seed)
+            c = rnd(run_form(cspec, 32, "fused", args.other_steps, args.other_settle, options=forms[exact]))
+            row["fp32_code_model"] = c
+            row["code_model_vs_builtin_ms_per_step"] = round(c["ms_per_step"] / f["ms_per_step"], 4)
         out[name] = row
         log(name, row)
+    if not args.no_code:
+        # C3 with its model as generated code, in the headline's FP32 pipelined form
+        cspec = code_spec("brunel_100k", args.seed)
+        opts = {9: 1, 0: 128, 1: 512}
+        c = rnd(run_form(cspec, 32, "fused", args.other_steps, args.other_settle, options=opts))
+        b = rnd(run_form(other_spec_brunel(args.seed), 32, "fused", args.other_steps, args.other_settle,
+                         options=opts))
+        out["brunel_100k_code_model"] = {"options": {str(k): v for k, v in opts.items()}, "fp32_code_model": c,
+                                         "fp32_builtin": b,
+                                         "code_model_vs_builtin_ms_per_step": round(c["ms_per_step"] /
+                                                                                    b["ms_per_step"], 4)}
+        log("brunel_100k_code_model", out["brunel_100k_code_model"])
     return out
+
+
+def code_spec(name, seed):
+    """the code-model form of a configuration (brian2cuda_amd.networks.code_*)"""
+    from brian2cuda_amd import networks as nw
+    if name == "cuba_4k":
+        return nw.code_cuba(N=4000, seed=seed, monitors=False)
+    if name == "cobahh_512k":
+        return nw.code_cobahh(N=512000, seed=seed, monitors=False)
+    if name == "brunel_100k":
+        return nw.code_brunel(N=100_000, K=1000, seed=seed)
+    raise ValueError(name)
+
+
+def other_spec_brunel(seed):
+    from brian2cuda_amd import networks as nw
+    return nw.brunel(N=100_000, K=1000, seed=seed)
 
 
 CONFIG_STEPS = 10000  # every configuration simulates 1 s at dt = 0.1 ms
@@ -615,6 +651,8 @@ def main():
     ap.add_argument("--kt-steps", type=int, default=2000, help="steps of the kernel-timed run (0: skip)")
     ap.add_argument("--drift-steps", type=int, default=CONFIG_STEPS, help="steps of the drift comparisons (0: skip)")
     ap.add_argument("--hines-trees", type=int, default=2048, help="trees of the multicompartment batch")
+    ap.add_argument("--no-code", action="store_true",
+                    help="skip the code-model forms of the other configurations")
     ap.add_argument("--hines-steps", type=int, default=2000, help="steps of the multicompartment run (0: skip)")
     ap.add_argument("--other-steps", type=int, default=3000, help="timed steps of the other configurations (0: skip)")
     ap.add_argument("--other-settle", type=int, default=500, help="untimed steps of the other configurations")

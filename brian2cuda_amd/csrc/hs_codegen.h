@@ -48,8 +48,11 @@ int hs_cg_check_syn(const HsSynCgModel &m);
 /* the program compiled for a network's code models and synaptic code in precision prec */
 std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn);
 /* compiled kernels for these models (compiled on first request, then cached
-   for the process by precision and program text); fails with the compiler's log */
-int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn,
+   for the process by precision, specialisation and program text); ms: the
+   model every group of the network has (HS_MODEL_CODE + c: kernels
+   specialised for it) or 0 (every model dispatched per tile); fails with the
+   compiler's log */
+int hs_cg_kernels(int prec, int ms, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn,
                   const HsCgKernels **out);
 
 /* the engine launches of hs_engine.hip, through the compiled kernels (same

@@ -259,15 +259,18 @@ struct HsCgEntry {
 static std::mutex g_cg_mu;
 static std::map<std::string, HsCgEntry> g_cg_cache; /* precision + program text -> loaded kernels (device of hs_init) */
 
-int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn,
+int hs_cg_kernels(int prec, int ms, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn,
                   const HsCgKernels **out)
 {
+    if (ms != 0 && !(HS_IS_CODE_MODEL(ms) && ms - HS_MODEL_CODE < (int)models.size()))
+        return hs_rt_fail(HS_ERR_ARG, "bad kernel specialisation %d", ms);
     if ((int)models.size() > HS_MAX_CODE_MODELS || (int)syn.size() > HS_MAX_CODE_MODELS ||
         (models.empty() && syn.empty()))
         return hs_rt_fail(HS_ERR_UNSUPPORTED, "1 to %d code models and synaptic codes per network",
                           HS_MAX_CODE_MODELS);
     const std::string src = hs_cg_source(prec, models, syn);
-    const std::string key = std::to_string(prec) + "\n" + src; /* the kernels are instantiated for one precision */
+    /* the kernels are instantiated for one precision and one specialisation */
+    const std::string key = std::to_string(prec) + " " + std::to_string(ms) + "\n" + src;
     std::lock_guard<std::mutex> lock(g_cg_mu);
     auto it = g_cg_cache.find(key);
     if (it != g_cg_cache.end()) {
@@ -279,9 +282,13 @@ int hs_cg_kernels(int prec, const std::vector<HsCgModel> &models, const std::vec
     std::vector<std::string> names;
     std::vector<hipFunction_t *> slots;
     if (!models.empty()) {
-        names = {std::string("hs_k_update<") + R + ", false, 0>", std::string("hs_k_update<") + R + ", true, 0>",
-                 std::string("hs_k_window<") + R + ", 0>",        std::string("hs_k_fused<") + R + ", 0>",
-                 std::string("hs_k_finalize<") + R + ", false>",  std::string("hs_k_finalize<") + R + ", true>"};
+        const std::string m = std::to_string(ms);
+        names = {std::string("hs_k_update<") + R + ", false, " + m + ">",
+                 std::string("hs_k_update<") + R + ", true, " + m + ">",
+                 std::string("hs_k_window<") + R + ", " + m + ">",
+                 std::string("hs_k_fused<") + R + ", " + m + ">",
+                 std::string("hs_k_finalize<") + R + ", false>",
+                 std::string("hs_k_finalize<") + R + ", true>"};
         slots = {&e.k.update[0], &e.k.update[1], &e.k.window, &e.k.fused, &e.k.finalize[0], &e.k.finalize[1]};
     }
     if (!syn.empty()) {

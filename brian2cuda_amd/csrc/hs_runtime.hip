@@ -1795,14 +1795,20 @@ static int hs_build(hs_net *net)
        of the chip idle), into a ring of 2W steps that the update reads */
     if (net->mode == 3 || net->mode == 4)
         for (HGroup &g : net->g)
-            if (g.model == HS_MODEL_LIF_NOISE) {
+            if (g.model == HS_MODEL_LIF_NOISE || (g.code >= 0 && net->cg_models[(size_t)g.code].noisy)) {
                 g.xi_slots = 2 * W;
                 HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)g.n * g.xi_slots, &g.xi, false));
                 net->noise_ahead = true;
             }
     /* code models: the engine kernels compiled with them (cached per program) */
-    if (!net->cg_models.empty() || !net->cg_syn.empty())
-        HS_TRY(hs_cg_kernels(net->prec, net->cg_models, net->cg_syn, &net->cg));
+    if (!net->cg_models.empty() || !net->cg_syn.empty()) {
+        /* one code model in every group: kernels specialised for it */
+        int ms = net->cg_models.empty() ? 0 : net->g[0].model;
+        for (const HGroup &g : net->g)
+            if (g.model != ms) ms = 0;
+        if (!HS_IS_CODE_MODEL(ms)) ms = 0;
+        HS_TRY(hs_cg_kernels(net->prec, ms, net->cg_models, net->cg_syn, &net->cg));
+    }
     HsCtl c;
     memset(&c, 0, sizeof(c));
     c.base = net->step;

@@ -462,6 +462,22 @@ def brunel(N=100_000, K=1000, seed=1234, dt=0.1 * ms, J=-0.1 * mV, dlo=0.5 * ms,
     return spec
 
 
+def code_brunel(N=100_000, K=1000, seed=1234, **kw):
+    """C3 with the model given as code: examples/brunelhakim.py's equation
+    'dV/dt = (-V+muext + sigmaext*sqrt(tau)*xi)/tau' (Euler-Maruyama), threshold,
+    reset and refractory period through the code generator; same connectivity,
+    delays and initial state as brunel()."""
+    from .codegen import neuron_model
+    spec = brunel(N=N, K=K, seed=seed, **kw)
+    g = spec.groups[0]
+    tau, mu, sigma, theta, vr, refr, stream = g.params
+    model, _ = neuron_model("dV/dt = (-V + muext + sigmaext*sqrt(tau)*xi)/tau : volt", "V > theta", "V = Vr", refr,
+                            "euler", dict(tau=tau, muext=mu, sigmaext=sigma, theta=theta, Vr=vr), stream=int(stream))
+    spec.groups[0] = GroupSpec(0, g.n, [], g.init, code=model)
+    spec.name = "code_brunel"
+    return spec
+
+
 # ---- C4: batched parameter sweep ----------------------------------------------------
 
 def batch_cuba(n_nets=256, seed=1234, nmin=500, nmax=16000, p=0.02, dt=0.1 * ms, wi=-9 * mV, delay_steps=1,

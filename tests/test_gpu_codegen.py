@@ -236,3 +236,24 @@ def test_stdp_as_code_matches_the_builtin_rule_gpu(hs):
                                    atol=1e-15)
     na.close()
     nb.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("opts", [{}, {9: 1}, {4: 3}])
+def test_code_brunel_vs_oracle(hs, oracle_lib, prec, opts):
+    """C3's noise-driven LIF as code (Euler-Maruyama through the generator), one
+    code model in every group (kernels specialised for it), in the fused,
+    pipelined and windowed forms (the windowed forms draw the noise one window
+    ahead into a ring, as the built-in model does)."""
+    spec = nw.code_brunel(N=4000, K=400, seed=8, monitors=True)
+    net, gsm, _ = nw.build_hs(spec, prec, "fused", options=opts)
+    ref, rsm, _ = oracle_lib.build_ref(spec, prec, 1)
+    for k in (600, 900):
+        net.run(k)
+        ref.run(k)
+    gi, gt = net.spikes(gsm[0])
+    ri, rt_ = ref.spikes(rsm[0])
+    assert len(gi) == len(ri) > 0 and np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    assert np.array_equal(net.get_state(0, 0), ref.get_state(0, 0))
+    assert net.stats()["events"] == ref.events
+    net.close()
