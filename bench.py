@@ -83,6 +83,8 @@ def make_spec(workload, args):
     from brian2cuda_amd import networks as nw
     if workload == "brunel":
         return nw.brunel(N=args.N, K=args.K, seed=args.seed, monitors=False)
+    if workload == "brunel_code":  # the same network with its model as generated code
+        return nw.code_brunel(N=args.N, K=args.K, seed=args.seed, monitors=False)
     if workload == "cuba":
         return nw.cuba(N=args.N, seed=args.seed, monitors=False)
     if workload == "batch":
@@ -95,7 +97,7 @@ def make_spec(workload, args):
 def state_bytes(workload, prec):
     """(r_s, w_s): bytes of per-neuron state read and written per step (Brian2's arrays)."""
     f = 8 if prec == 64 else 4
-    if workload == "brunel":       # V, lastspike, not_refractory
+    if workload in ("brunel", "brunel_code"):  # V, lastspike, not_refractory
         return 2 * f + 1, f + 1
     if workload in ("cuba", "batch"):  # v, ge, gi, lastspike, not_refractory
         return 4 * f + 1, 3 * f + 1
@@ -636,7 +638,7 @@ def main():
                          f"{CONFIG_STEPS} (1 s) time the {CONFIG_STEPS} anyway")
     ap.add_argument("--warmup", type=int, default=SETTLE_MIN,
                     help=f"untimed steps before timing requested; fewer than {SETTLE_MIN} settle {SETTLE_MIN}")
-    ap.add_argument("--workload", default="brunel", choices=["brunel", "cuba", "batch", "cobahh"])
+    ap.add_argument("--workload", default="brunel", choices=["brunel", "brunel_code", "cuba", "batch", "cobahh"])
     ap.add_argument("--N", type=int, default=None)
     ap.add_argument("--K", type=int, default=1000)
     ap.add_argument("--nets", type=int, default=256)
@@ -682,7 +684,7 @@ def main():
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.N is None:
-        args.N = {"brunel": 100_000, "cuba": 4000, "cobahh": 4000, "batch": 0}[args.workload]
+        args.N = {"brunel": 100_000, "brunel_code": 100_000, "cuba": 4000, "cobahh": 4000, "batch": 0}[args.workload]
     steps = args.steps if args.exact_steps else max(args.steps, CONFIG_STEPS)
     settle = args.warmup if args.exact_steps else max(args.warmup, SETTLE_MIN)
     log(f"timing {steps} steps after {settle} settle steps (asked: --steps {args.steps} --warmup {args.warmup})")
@@ -834,7 +836,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded generator; inputs resident in HBM before timing)",
         "config": {"workload": {"brunel": "brunel_hakim_heterog_delays", "cuba": "cuba", "batch": "batch_cuba",
-                                "cobahh": "cobahh"}[args.workload],
+                                "cobahh": "cobahh",
+                                "brunel_code": "brunel_hakim_heterog_delays_code_model"}[args.workload],
                    "neurons": N, "synapses": spec.n_synapses, "dt_ms": spec.dt * 1e3,
                    "sim_s": steps * spec.dt, "delay_slots": int(D), "seed": args.seed},
         "sim_s_per_wall_s": round(fp32["sim_s_per_wall_s"], 4),

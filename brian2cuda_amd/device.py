@@ -520,10 +520,13 @@ if _CPPBase is not None:  # pragma: no cover
                         init={v: self._hs_values[(obj.name, v)] for (o, v) in self._hs_values if o == obj.name}))
                 elif isinstance(obj, Synapses):
                     code = obj.pre.code if hasattr(obj, "pre") else ""
+                    post_code = obj.post.code if hasattr(obj, "post") else ""
+                    model = str(obj.equations) if getattr(obj, "equations", None) is not None else ""
                     src = obj.source
                     syns.append(SynapsesDesc(
                         obj.name, getattr(src, "source", src).name, getattr(obj.target, "source", obj.target).name,
-                        on_pre=str(code), namespace=ns_of(obj, str(code)),
+                        on_pre=str(code), model=model, on_post=str(post_code),
+                        namespace=ns_of(obj, " ".join((str(code), str(post_code), model))),
                         i=self._hs_values.get((obj.name, "_synaptic_pre")),
                         j=self._hs_values.get((obj.name, "_synaptic_post")),
                         values={k[1]: v for k, v in self._hs_values.items() if k[0] == obj.name},

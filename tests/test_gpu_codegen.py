@@ -257,3 +257,23 @@ def test_code_brunel_vs_oracle(hs, oracle_lib, prec, opts):
     assert np.array_equal(net.get_state(0, 0), ref.get_state(0, 0))
     assert net.stats()["events"] == ref.events
     net.close()
+
+
+def test_code_models_with_gather_delivery_and_builtin_stdp(hs, oracle_lib):
+    """Code models beside the other engine parts: C1 as code with gather-mode
+    delivery (plain form), and a code-model target under the built-in STDP rule."""
+    spec = nw.code_cuba(N=4000, seed=12)
+    _compare(spec, 64, "plain", 0, {5: 2}, oracle_lib, (700,))
+    ff = nw.stdp_feedforward(2)
+    lif, names = cg.neuron_model(nw.CUBA_EQS.replace(" (unless refractory)", ""), "v > Vt", "v = Vr", 5e-3, None,
+                                 dict(El=-49e-3, taum=20e-3, taue=5e-3, taui=5e-3, Vt=-50e-3, Vr=-60e-3), dt=1e-4)
+    g = ff.groups[1]
+    ff.groups[1] = nw.GroupSpec(0, g.n, [], g.init, code=lif)
+    net, sm, _ = nw.build_hs(ff, 64, "plain")
+    ref, rsm, _ = oracle_lib.build_ref(ff, 64, 0)
+    net.run(2500)
+    ref.run(2500)
+    assert np.array_equal(net.spikes(sm[0])[0], ref.spikes(rsm[0])[0]) and len(ref.spikes(rsm[0])[0]) > 20
+    perm = nw.bundle_order(ff.paths[0])
+    assert np.array_equal(net.synapse_var(0, rt.HS_SYN_W), ref.synapse_var(0, rt.HS_SYN_W, len(perm))[perm])
+    net.close()
