@@ -277,3 +277,25 @@ def test_code_models_with_gather_delivery_and_builtin_stdp(hs, oracle_lib):
     perm = nw.bundle_order(ff.paths[0])
     assert np.array_equal(net.synapse_var(0, rt.HS_SYN_W), ref.synapse_var(0, rt.HS_SYN_W, len(perm))[perm])
     net.close()
+
+
+def test_synaptic_code_on_device_connectivity(hs, oracle_lib):
+    """Synaptic code on a pathway generated on the device (connect(p) with drawn
+    per-synapse weights and delays): 'V_post += w', with w the pathway's weight
+    array, gives the built-in pathway's bits; a second variable counts arrivals."""
+    a = nw.brunel(N=4000, K=400, seed=5, monitors=True, device_connect=True)
+    b = nw.brunel(N=4000, K=400, seed=5, monitors=True, device_connect=True)
+    b.paths[0].code = cg.synapse_code("V_post += w\nn += 1", ["w", "n"], ["V"], ["V"])
+    for prec, engine, sem in ((64, "plain", 0), (32, "fused", 1)):
+        na, sa, _ = nw.build_hs(a, prec, engine)
+        nb, sb, _ = nw.build_hs(b, prec, engine)
+        rb, rsb, _ = oracle_lib.build_ref(b, prec, sem)
+        for net in (na, nb, rb):
+            net.run(1200)
+        assert np.array_equal(na.spikes(sa[0])[0], nb.spikes(sb[0])[0])
+        assert np.array_equal(nb.spikes(sb[0])[0], rb.spikes(rsb[0])[0])
+        assert np.array_equal(na.get_state(0, 0), nb.get_state(0, 0))
+        n = nb.synapse_var(0, rt.HS_SYN_CODE + 1)
+        assert n.sum() == nb.stats()["events"] == rb.events
+        na.close()
+        nb.close()
