@@ -465,15 +465,16 @@ __global__ __launch_bounds__(256) void hs_k_update(const DGroup<R> *__restrict__
  * goes to slots >= T+W (nacc = 3W+1 keeps them apart).  After the launch every
  * effect arriving before T+w+W has been applied or accumulated.
  */
+union HsWindowLds {
+    HsCompactLdsW c;
+    HsDeliverLds d;
+};
+
 template <typename R, int MS>
-__global__ __launch_bounds__(256) void hs_k_window(const DGroup<R> *__restrict__ groups, int ng, int total_tiles,
-                                                    const DPath<R> *__restrict__ paths, int np, HsCtl *ctl, int koff, int w, int W,
-                                                    int n_upd, int n_far)
+__device__ __forceinline__ void hs_window_body(const DGroup<R> *__restrict__ groups, int ng, int total_tiles,
+                                               const DPath<R> *__restrict__ paths, int np, HsCtl *ctl, int koff, int w,
+                                               int W, int n_upd, int n_far, HsWindowLds &L)
 {
-    __shared__ union {
-        HsCompactLdsW c;
-        HsDeliverLds d;
-    } L;
     const long long T = ctl->base + koff;
     const int b = blockIdx.x;
     if (b < n_upd) {
@@ -491,6 +492,28 @@ __global__ __launch_bounds__(256) void hs_k_window(const DGroup<R> *__restrict__
     } else {
         hs_noise_ahead<R>(groups, ng, T + w, W, b - n_upd - n_far, gridDim.x - n_upd - n_far);
     }
+}
+
+template <typename R, int MS>
+__global__ __launch_bounds__(256) void hs_k_window(const DGroup<R> *__restrict__ groups, int ng, int total_tiles,
+                                                    const DPath<R> *__restrict__ paths, int np, HsCtl *ctl, int koff,
+                                                    int w, int W, int n_upd, int n_far)
+{
+    __shared__ HsWindowLds L;
+    hs_window_body<R, MS>(groups, ng, total_tiles, paths, np, ctl, koff, w, W, n_upd, n_far, L);
+}
+
+/* the same with 8 waves per SIMD (at most 64 VGPRs): the FP32 kernels of one
+   model fit without spilling, and a 512,000-neuron update (8,000 waves) then
+   takes one round of the chip's 8,192 wave slots instead of 7,168 + a tail;
+   launched only where the update has more waves than 7 per SIMD */
+template <typename R, int MS>
+__global__ __launch_bounds__(256, 8) void hs_k_window_o8(const DGroup<R> *__restrict__ groups, int ng,
+                                                         int total_tiles, const DPath<R> *__restrict__ paths, int np,
+                                                         HsCtl *ctl, int koff, int w, int W, int n_upd, int n_far)
+{
+    __shared__ HsWindowLds L;
+    hs_window_body<R, MS>(groups, ng, total_tiles, paths, np, ctl, koff, w, W, n_upd, n_far, L);
 }
 
 /* events still in flight at step E = ctl->base (arrival >= E) of spikes emitted
@@ -776,9 +799,20 @@ hipError_t hs_launch_window(const DGroup<R> *groups, int ng, int total_tiles, co
         n_noise < 0)
         return hipErrorInvalidValue;
     const dim3 grid(n_upd + n_far + n_noise);
+    /* 8 waves per SIMD where the update needs more waves than 7 per SIMD hold */
+    const bool o8 = (long long)n_upd * (block / HS_WAVE) > 7LL * 4 * hs_rt_cus();
 #define HS_WIN(M)                                                                                                    \
-    hipLaunchKernelGGL((hs_k_window<R, M>), grid, dim3(block), 0, s, groups, ng, total_tiles, paths, np, ctl, koff, w, \
-                       W, n_upd, n_far)
+    do {                                                                                                             \
+        if constexpr (sizeof(R) == sizeof(float) && M != 0)                                                          \
+            if (o8)                                                                                                  \
+            {                                                                                                        \
+                hipLaunchKernelGGL((hs_k_window_o8<R, M>), grid, dim3(block), 0, s, groups, ng, total_tiles, paths,  \
+                                   np, ctl, koff, w, W, n_upd, n_far);                                               \
+                break;                                                                                               \
+            }                                                                                                        \
+        hipLaunchKernelGGL((hs_k_window<R, M>), grid, dim3(block), 0, s, groups, ng, total_tiles, paths, np, ctl,     \
+                           koff, w, W, n_upd, n_far);                                                                \
+    } while (0)
     if (ms == HS_MODEL_LIF_CUBA) HS_WIN(HS_MODEL_LIF_CUBA);
     else if (ms == HS_MODEL_LIF_NOISE) HS_WIN(HS_MODEL_LIF_NOISE);
     else if (ms == HS_MODEL_HH_COBA) HS_WIN(HS_MODEL_HH_COBA);
