@@ -36,6 +36,7 @@ struct HsSynCgModel {
 struct HsCgKernels {
     hipFunction_t update[2] = {nullptr, nullptr};   /* plain, fused */
     hipFunction_t window = nullptr;
+    hipFunction_t window_o8 = nullptr;              /* 8 waves per SIMD (FP32, one model) */
     hipFunction_t fused = nullptr;
     hipFunction_t finalize[2] = {nullptr, nullptr}; /* plain, fused */
     hipFunction_t syn_code = nullptr;

@@ -290,6 +290,10 @@ int hs_cg_kernels(int prec, int ms, const std::vector<HsCgModel> &models, const 
                  std::string("hs_k_finalize<") + R + ", false>",
                  std::string("hs_k_finalize<") + R + ", true>"};
         slots = {&e.k.update[0], &e.k.update[1], &e.k.window, &e.k.fused, &e.k.finalize[0], &e.k.finalize[1]};
+        if (prec != HS_F64 && ms != 0) {
+            names.push_back(std::string("hs_k_window_o8<") + R + ", " + m + ">");
+            slots.push_back(&e.k.window_o8);
+        }
     }
     if (!syn.empty()) {
         names.push_back(std::string("hs_k_syn_code<") + R + ">");
@@ -379,7 +383,9 @@ hipError_t hs_cg_launch_window(const HsCgKernels *K, const void *groups, int ng,
         n_noise < 0)
         return hipErrorInvalidValue;
     void *args[] = {&groups, &ng, &total_tiles, &paths, &np, &ctl, &koff, &w, &W, &n_upd, &n_far};
-    return hs_cg_go(K->window, n_upd + n_far + n_noise, block, s, args);
+    /* as hs_launch_window: the 8-wave kernel when the update waves (64 lanes) exceed 7 per SIMD */
+    const bool o8 = K->window_o8 && (long long)n_upd * (block / 64) > 7LL * 4 * hs_rt_cus();
+    return hs_cg_go(o8 ? K->window_o8 : K->window, n_upd + n_far + n_noise, block, s, args);
 }
 
 hipError_t hs_cg_launch_fused(const HsCgKernels *K, const void *groups, int ng, int total_tiles, const void *paths,

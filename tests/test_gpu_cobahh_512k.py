@@ -1,8 +1,9 @@
 """COBAHH at its largest configured size: N = 512,000 Hodgkin-Huxley neurons,
 80/20 exc/inh, p = 80/N (4.1e7 synapses), seed 1234, exponential Euler, delay 0.
 
-  parity     : both timed forms (FP32 batched, FP64 plain) and the other two
-               (FP64 fused, FP32 plain) bit for bit against the oracle in their
+  parity     : both timed forms (FP32 batched, FP64 plain), the other two
+               (FP64 fused, FP32 plain) and the FP32 pipelined form (built in
+               and as code) bit for bit against the oracle in their
                own precision and semantics over 60 steps in two chunks: every
                spike, the final state of all six variables and the event count.
                NaN counts as equal to NaN: the reference's alpha_n is 0/0 where v
@@ -41,6 +42,17 @@ def c2_512k():
     return spec, out_degree
 
 
+def _parity(net, smon, ref, rsmon, nvars=6):
+    gi, gt = net.spikes(smon[0])
+    ri, rt = ref.spikes(rsmon[0])
+    assert len(ri) > 1000, "the network must fire within the compared steps"
+    assert len(gi) == len(ri), f"{len(gi)} spikes on the GPU vs {len(ri)} in the oracle"
+    assert np.array_equal(gt, rt) and np.array_equal(gi, ri)
+    for v in range(nvars):
+        assert np.array_equal(net.get_state(0, v), ref.get_state(0, v), equal_nan=True), f"var {v}"
+    assert net.stats()["events"] == ref.events
+
+
 @pytest.mark.parametrize("prec,engine,sem", [(32, "fused", 1), (64, "plain", 0), (64, "fused", 1), (32, "plain", 0)])
 def test_cobahh_512k_bit_exact(hs, oracle_lib, c2_512k, prec, engine, sem):
     spec = c2_512k[0]
@@ -50,16 +62,27 @@ def test_cobahh_512k_bit_exact(hs, oracle_lib, c2_512k, prec, engine, sem):
     for k in (35, 25):
         net.run(k)
         ref.run(k)
-    gi, gt = net.spikes(smon[0])
-    ri, rt = ref.spikes(rsmon[0])
-    assert len(ri) > 1000, "the network must fire within the compared steps"
-    assert len(gi) == len(ri), f"{len(gi)} spikes on the GPU vs {len(ri)} in the oracle"
-    assert np.array_equal(gt, rt) and np.array_equal(gi, ri)
-    for v in range(6):
-        assert np.array_equal(net.get_state(0, v), ref.get_state(0, v), equal_nan=True), f"var {v}"
-    assert net.stats()["events"] == ref.events
+    _parity(net, smon, ref, rsmon)
     if prec == 64:
         assert not np.isnan(ref.get_state(0, 0)).any()
+    net.close()
+    ref.close()
+
+
+@pytest.mark.parametrize("code", [False, True])
+def test_cobahh_512k_pipelined_bit_exact(hs, oracle_lib, c2_512k, code):
+    """The timed FP32 form, one pipelined launch per step (option 9): its
+    8,000 update waves take the 8-wave-per-SIMD window kernel, built in and,
+    with the model given as examples/cobahh.py's equations, compiled at run
+    time."""
+    spec = nw.code_cobahh(N=N, seed=1234) if code else c2_512k[0]
+    print(f"cobahh 512k pipelined parity: code {code}", flush=True)
+    net, smon, _ = nw.build_hs(spec, 32, "fused", spike_capacity=1 << 24, options={9: 1})
+    ref, rsmon, _ = oracle_lib.build_ref(spec, 32, 1, threads=_threads())
+    for k in (35, 25):
+        net.run(k)
+        ref.run(k)
+    _parity(net, smon, ref, rsmon)
     net.close()
     ref.close()
 
