@@ -76,8 +76,13 @@ __device__ __forceinline__ int hs_wave_excl_scan(int x, int *total)
     return inc - x;
 }
 
+/* a mod m in [0, m) for m > 0.  The engine sizes every ring (spike lists,
+   accumulators, noise) to a power of two, so a step's slot is a mask instead
+   of a signed 64-bit remainder (some 60 instructions per call); the standalone
+   operators take rings of any size */
 __device__ __forceinline__ long long hs_mod(long long a, long long m)
 {
+    if ((m & (m - 1)) == 0) return a & (m - 1);
     long long r = a % m;
     return r < 0 ? r + m : r;
 }

@@ -201,7 +201,7 @@ __device__ __forceinline__ void hs_update_tile(const DGroup<R> &g, int gid, int 
             hs_st(g.hist_cnt + ((long long)hs_wrap(tq + w + q / g.nsub, g.Q) * g.nsub + q % g.nsub) * HS_CNT_STRIDE,
                   0);
     /* this tile's sub-list (a tile lies inside one 256-neuron chunk) */
-    const int sub = (int)(((long long)tile * blockDim.x >> 8) % g.nsub);
+    const int sub = (int)(((long long)tile * blockDim.x >> 8) & (g.nsub - 1)); /* nsub: a power of two */
     const long long rowlen = (long long)g.nsub * g.subcap;
     R s[NV];
     int32_t last = HS_NEVER_SPIKED;

@@ -109,7 +109,7 @@ struct DGroup {
     int tile_begin; /* first tile index of this group in the concatenated tile space */
     R *xi;          /* windowed form: noise ring [xi_slots][n], draw of step t in slot t mod xi_slots (or nullptr) */
     int xi_slots;
-    /* the spike list of a ring slot is split into nsub sub-lists of subcap
+    /* the spike list of a ring slot is split into nsub (a power of two) sub-lists of subcap
        entries: neuron i goes to sub-list (i / 256) mod nsub, so that the blocks
        of a large group reserve list space on nsub counters, not one.  List of
        (slot, s): hist[slot*nsub*subcap + s*subcap ..], count at

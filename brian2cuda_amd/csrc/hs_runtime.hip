@@ -534,6 +534,15 @@ struct hs_net {
 
 static size_t hs_rsize(const hs_net *net) { return net->prec == HS_F64 ? sizeof(double) : sizeof(float); }
 
+/* ring sizes are rounded up to a power of two: the kernels then take a step's
+   slot as a mask (hs_mod); a ring only has to hold at least n slots */
+static int hs_ring_size(int n)
+{
+    int m = 1;
+    while (m < n) m <<= 1;
+    return m;
+}
+
 static int hs_dalloc(hs_net *net, size_t bytes, void **p, bool zero)
 {
     if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
@@ -1762,7 +1771,7 @@ static int hs_build(hs_net *net)
         int dmax = 0;
         for (const HPath &P : net->p)
             if (P.src == gid) dmax = std::max(dmax, P.dmax);
-        g.Q = dmax + 2 * W; /* see the ring invariants in hs_engine.hip */
+        g.Q = hs_ring_size(dmax + 2 * W); /* see the ring invariants in hs_engine.hip */
         /* large groups split each step's spike list into sub-lists so that
            thousands of blocks do not reserve list space on one counter; the
            delivery prefix tables bound the (step, delay, sub-list) items */
@@ -1784,7 +1793,7 @@ static int hs_build(hs_net *net)
         for (const HPath &P : net->p) {
             HGroup &gt = net->g[P.tgt];
             /* pipelined windows write up to 3W-2 steps ahead of the window they update */
-            gt.nacc = net->mode == 4 ? 3 * W + 1 : W + 1;
+            gt.nacc = hs_ring_size(net->mode == 4 ? 3 * W + 1 : W + 1);
             std::vector<int> vars = P.dmode == HS_DELIVER_CODE ? P.post_vars : std::vector<int>{P.var};
             for (int v : vars)
                 if (!gt.acc[v]) HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)gt.n * gt.nacc, &gt.acc[v], true));
@@ -1796,7 +1805,7 @@ static int hs_build(hs_net *net)
     if (net->mode == 3 || net->mode == 4)
         for (HGroup &g : net->g)
             if (g.model == HS_MODEL_LIF_NOISE || (g.code >= 0 && net->cg_models[(size_t)g.code].noisy)) {
-                g.xi_slots = 2 * W;
+                g.xi_slots = hs_ring_size(2 * W);
                 HS_TRY(hs_dalloc(net, hs_rsize(net) * (size_t)g.n * g.xi_slots, &g.xi, false));
                 net->noise_ahead = true;
             }
