@@ -85,11 +85,47 @@ __device__ __forceinline__ void hs_push_near(const HsPush<R> &ps, int gid, int i
         int tot;
         const int off = hs_wave_excl_scan(cnt * len, &tot);
         const int ta = (int)hs_mod(T, p.nacc);
+        if (w == 1 && p.nnear == 1) {
+            /* one step and one near delay slot (delay-0 networks): a lane holds
+               at most one spike, its items are synapses s0 + k, and they all
+               arrive in the one slot of step T + dmin */
+            R *acc = p.acc + (long long)hs_wrap(ta + p.dmin, p.nacc) * p.n_tgt;
+            for (int c0 = 0; c0 < tot; c0 += UNR * HS_WAVE) {
+                int tg[UNR];
+                R wv[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    tg[u] = -1;
+                    if (c0 + u * HS_WAVE >= tot) continue; /* uniform: no items left */
+                    const int idx = c0 + u * HS_WAVE + hs_lane();
+                    int lo = 0;
+#pragma unroll
+                    for (int step = 32; step > 0; step >>= 1) {
+                        const int cand = lo + step;
+                        if (__shfl(off, cand, HS_WAVE) <= idx) lo = cand;
+                    }
+                    const int sy = __shfl(s0, lo, HS_WAVE) + (idx - __shfl(off, lo, HS_WAVE));
+                    const int pb = __shfl(i, lo, HS_WAVE);
+                    if (idx < tot) {
+                        if (p.wmode == HS_W_SCALAR) wv[u] = p.wscalar;
+                        else if (p.wmode == HS_W_SYNAPSE) wv[u] = hs_ld(p.w + sy);
+                        else wv[u] = hs_ld(p.w + pb);
+                        tg[u] = hs_ld(p.post + sy);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UNR; ++u)
+                    if (tg[u] >= 0) hs_atomic_add(acc + tg[u], wv[u]);
+            }
+            continue;
+        }
         for (int c0 = 0; c0 < tot; c0 += UNR * HS_WAVE) {
             int tg[UNR], slot[UNR];
             R wv[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
+                tg[u] = -1;
+                if (c0 + u * HS_WAVE >= tot) continue; /* uniform: no items left */
                 const int idx = c0 + u * HS_WAVE + hs_lane();
                 int lo = 0; /* owner: the last lane whose items start at or before idx */
 #pragma unroll
@@ -104,7 +140,6 @@ __device__ __forceinline__ void hs_push_near(const HsPush<R> &ps, int gid, int i
                 int ob_q[HS_MAX_WINDOW + 1];
 #pragma unroll
                 for (int q = 0; q <= HS_MAX_WINDOW; ++q) ob_q[q] = __shfl(o[q], lo, HS_WAVE);
-                tg[u] = -1;
                 if (idx < tot) {
                     const int k = idx - ob;
                     const int r = k / lb; /* the owner's r-th spike of the window, synapse k - r*lb */
