@@ -58,6 +58,9 @@ __device__ __forceinline__ void hs_push_near(const HsPush<R> &ps, int gid, int i
     const int cnt = __popc(bits); /* this lane's spikes in the window (same neuron, same synapses) */
     if (!__ballot(cnt != 0)) return;
     constexpr int UNR = 4; /* passes whose loads are issued together */
+    /* every lane at most one spike in the window (refractoriness as long as the
+       window, or one-step windows): an item's spike is the lane's only one */
+    const bool one = !__ballot(cnt > 1);
     unsigned long long emitted = 0;
     for (int pi = 0; pi < ps.np; ++pi) {
         const DPath<R> &p = ps.paths[pi];
@@ -137,12 +140,16 @@ __device__ __forceinline__ void hs_push_near(const HsPush<R> &ps, int gid, int i
                 const int lb = __shfl(len, lo, HS_WAVE);
                 const unsigned int bb = (unsigned int)__shfl((int)bits, lo, HS_WAVE);
                 const int pb = __shfl(i, lo, HS_WAVE);
-                int ob_q[HS_MAX_WINDOW + 1];
+                int ob_q[HS_MAX_WINDOW]; /* the owner's near bundle starts (items read [0, nnear)) */
 #pragma unroll
-                for (int q = 0; q <= HS_MAX_WINDOW; ++q) ob_q[q] = __shfl(o[q], lo, HS_WAVE);
+                for (int q = 0; q < HS_MAX_WINDOW; ++q) {
+                    ob_q[q] = 0;
+                    if (q < p.nnear) ob_q[q] = __shfl(o[q], lo, HS_WAVE);
+                }
                 if (idx < tot) {
                     const int k = idx - ob;
-                    const int r = k / lb; /* the owner's r-th spike of the window, synapse k - r*lb */
+                    int r = 0; /* the owner's r-th spike of the window, synapse k - r*lb */
+                    if (!one) r = k / lb;
                     const int s = ob_q[0] + (k - r * lb);
                     unsigned int bm = bb;
                     for (int q = 0; q < r; ++q) bm &= bm - 1;
