@@ -675,3 +675,177 @@ def synapse_code(on_pre, syn_vars, post_vars, pre_vars, namespace=None, what="Sy
     post_code = translate(on_post, True) if on_post else []
     return SynapseCode(syn_vars, "\n".join(advance + pre_code), used_post, used_pre, None,
                        "\n".join(advance + post_code) if on_post else "", read_post)
+
+
+# ---- connect conditions and probabilities --------------------------------------------
+_CONN_FUNCS = {"exp": "float", "log": "float", "log10": "float", "sqrt": "float", "abs": None, "floor": "float",
+               "ceil": "float", "sign": None, "clip": "float", "int": "int", "float": "float"}
+
+
+def connect_expression(expr, namespace=None, pre_names=(), post_names=(), what="connect"):
+    """A Brian2 connect condition or probability -> (C expression, constant).
+
+    The expression is Brian2's (Python's) over i and j (indices within the
+    source and target of the Synapses), N_pre, N_post, source / target
+    variables as name_pre / name_post (pre_names / post_names: the groups'
+    variables in the runtime's order), the namespace (SI values) and Brian2's
+    units; operators + - * / // % **, comparisons (chained), and / or / not,
+    and the functions exp, log, log10, sqrt, abs, floor, ceil, sign, clip, int,
+    float.  Python's numbers are kept: integers are long long in C, / is true
+    division, // and % floor and take the divisor's sign (hs_cg_floordiv,
+    hs_cg_pymod), a non-literal power is hs_cg_pow.  The C text is that of
+    hs_net_connect_code (variables as hs_pre(k) / hs_post(k)).  constant is the
+    expression's value when it reads neither an index nor a variable (else
+    None), so a constant probability can take the geometric skips.  rand(),
+    randn() and anything else raise NotImplementedError naming it.
+    """
+    ns = dict(namespace or {})
+    pre_idx = {n: k for k, n in enumerate(pre_names)}
+    post_idx = {n: k for k, n in enumerate(post_names)}
+    try:
+        tree = ast.parse(str(expr).strip(), mode="eval").body
+    except SyntaxError as e:
+        raise NotImplementedError(f"{what}: cannot parse {expr!r} ({e.msg})")
+
+    def lit(v):
+        if isinstance(v, (bool, np.bool_)):
+            return ("true" if v else "false"), "bool", bool(v)
+        if isinstance(v, (int, np.integer)):
+            return f"{int(v)}LL", "int", int(v)
+        v = float(v)
+        if not np.isfinite(v):
+            raise NotImplementedError(f"{what}: non-finite constant {v}")
+        return f"({v!r})", "float", v
+
+    def as_num(t):  # bool operands of arithmetic are 0 / 1
+        c, k, v = t
+        return (f"((long long)({c}))", "int", None if v is None else int(v)) if k == "bool" else t
+
+    def num(kind_a, kind_b):
+        return "float" if "float" in (kind_a, kind_b) else "int"
+
+    def fold(v, f):
+        return None if v is None else f()
+
+    def c(n):  # subtrees that read no index and no variable become literals
+        t = c0(n)
+        return lit(t[2]) if t[2] is not None else t
+
+    def c0(n):
+        if isinstance(n, ast.Constant):
+            if isinstance(n.value, (bool, int, float)):
+                return lit(n.value)
+            raise NotImplementedError(f"{what}: constant {n.value!r}")
+        if isinstance(n, ast.Name):
+            name = n.id
+            if name in ("i", "j", "N_pre", "N_post"):
+                return name, "int", None
+            if name.endswith("_pre") and name[:-4] in pre_idx:
+                return f"hs_pre({pre_idx[name[:-4]]})", "float", None
+            if name.endswith("_post") and name[:-5] in post_idx:
+                return f"hs_post({post_idx[name[:-5]]})", "float", None
+            if name in ns:
+                return lit(ns[name])
+            if name in UNITS:
+                return lit(UNITS[name])
+            if name in ("True", "False"):
+                return lit(name == "True")
+            raise NotImplementedError(f"{what}: unknown name {name!r} in {expr!r}")
+        if isinstance(n, ast.UnaryOp):
+            a = c(n.operand)
+            if isinstance(n.op, ast.Not):
+                return f"(!{a[0]})", "bool", fold(a[2], lambda: not a[2])
+            a = as_num(a)
+            if isinstance(n.op, ast.USub):
+                return f"(-{a[0]})", a[1], fold(a[2], lambda: -a[2])
+            if isinstance(n.op, ast.UAdd):
+                return a
+            raise NotImplementedError(f"{what}: operator {type(n.op).__name__}")
+        if isinstance(n, ast.BinOp):
+            a, b = as_num(c(n.left)), as_num(c(n.right))
+            both = None if a[2] is None or b[2] is None else (a[2], b[2])
+            k = num(a[1], b[1])
+            if isinstance(n.op, (ast.Div, ast.FloorDiv, ast.Mod)) and b[2] is not None and b[2] == 0:
+                raise NotImplementedError(f"{what}: division by zero in {expr!r}")
+            try:
+                if isinstance(n.op, (ast.Add, ast.Sub, ast.Mult)):
+                    sym = {ast.Add: "+", ast.Sub: "-", ast.Mult: "*"}[type(n.op)]
+                    f = {ast.Add: lambda x, y: x + y, ast.Sub: lambda x, y: x - y,
+                         ast.Mult: lambda x, y: x * y}[type(n.op)]
+                    return f"({a[0]} {sym} {b[0]})", k, fold(both, lambda: f(*both))
+                if isinstance(n.op, ast.Div):
+                    return (f"((double)({a[0]}) / (double)({b[0]}))", "float",
+                            fold(both, lambda: both[0] / both[1]))
+                if isinstance(n.op, ast.FloorDiv):
+                    return f"hs_cg_floordiv({a[0]}, {b[0]})", k, fold(both, lambda: both[0] // both[1])
+                if isinstance(n.op, ast.Mod):
+                    return f"hs_cg_pymod({a[0]}, {b[0]})", k, fold(both, lambda: both[0] % both[1])
+                if isinstance(n.op, ast.Pow):
+                    value = fold(both, lambda: both[0] ** both[1])
+                    e = b[2]
+                    if b[1] == "int" and e is not None and -8 <= e <= 8:
+                        if e == 0:
+                            return ("1LL" if a[1] == "int" else "(1.0)"), a[1], value
+                        prod = "(" + " * ".join([a[0]] * abs(e)) + ")"
+                        if e > 0:
+                            return prod, a[1], value
+                        return f"(1.0 / (double){prod})", "float", value
+                    return f"hs_cg_pow((double)({a[0]}), (double)({b[0]}))", "float", value
+            except ZeroDivisionError:
+                raise NotImplementedError(f"{what}: division by zero in {expr!r}")
+            raise NotImplementedError(f"{what}: operator {type(n.op).__name__}")
+        if isinstance(n, ast.Compare):
+            ops = {ast.Gt: ">", ast.Lt: "<", ast.GtE: ">=", ast.LtE: "<=", ast.Eq: "==", ast.NotEq: "!="}
+            pyops = {ast.Gt: lambda x, y: x > y, ast.Lt: lambda x, y: x < y, ast.GtE: lambda x, y: x >= y,
+                     ast.LtE: lambda x, y: x <= y, ast.Eq: lambda x, y: x == y, ast.NotEq: lambda x, y: x != y}
+            parts, vals, left = [], [], as_num(c(n.left))
+            for o, rn in zip(n.ops, n.comparators):
+                if type(o) not in ops:
+                    raise NotImplementedError(f"{what}: comparison {type(o).__name__}")
+                right = as_num(c(rn))
+                parts.append(f"({left[0]} {ops[type(o)]} {right[0]})")
+                vals.append(None if left[2] is None or right[2] is None else pyops[type(o)](left[2], right[2]))
+                left = right
+            value = None if any(v is None for v in vals) else all(vals)
+            return "(" + " && ".join(parts) + ")", "bool", value
+        if isinstance(n, ast.BoolOp):
+            items = [c(v) for v in n.values]
+            op = " && " if isinstance(n.op, ast.And) else " || "
+            vals = [t[2] for t in items]
+            value = None
+            if all(v is not None for v in vals):
+                value = all(vals) if isinstance(n.op, ast.And) else any(vals)
+            return "(" + op.join(f"(bool)({t[0]})" for t in items) + ")", "bool", value
+        if isinstance(n, ast.Call):
+            if not isinstance(n.func, ast.Name) or n.keywords:
+                raise NotImplementedError(f"{what}: call {ast.unparse(n)!r} is not supported")
+            fn = n.func.id
+            if fn in ("rand", "randn"):
+                raise NotImplementedError(f"{what}: {fn}() in a connect expression is not supported (use p=)")
+            if fn not in _CONN_FUNCS:
+                raise NotImplementedError(f"{what}: function {fn!r} is not supported")
+            args = [as_num(c(x)) for x in n.args]
+            nargs = 3 if fn == "clip" else 1
+            if len(args) != nargs:
+                raise NotImplementedError(f"{what}: {fn}() takes {nargs} argument(s)")
+            vals = [t[2] for t in args]
+            const = None if any(v is None for v in vals) else vals
+            if fn == "int":
+                return f"((long long)({args[0][0]}))", "int", fold(const, lambda: int(const[0]))
+            if fn == "float":
+                return f"((double)({args[0][0]}))", "float", fold(const, lambda: float(const[0]))
+            if fn == "clip":
+                k = "float" if any(t[1] == "float" for t in args) else "int"
+                return ("clip(" + ", ".join(t[0] for t in args) + ")", k,
+                        fold(const, lambda: min(max(const[0], const[1]), const[2])))
+            kind = _CONN_FUNCS[fn] or args[0][1]
+            arg = args[0][0] if kind == "int" or fn in ("abs", "sign") else f"(double)({args[0][0]})"
+            py = {"exp": np.exp, "log": np.log, "log10": np.log10, "sqrt": np.sqrt, "abs": abs,
+                  "floor": np.floor, "ceil": np.ceil, "sign": np.sign}[fn]
+            return f"{fn}({arg})", kind, fold(const, lambda: py(const[0]))
+        raise NotImplementedError(f"{what}: {ast.unparse(n)!r} is not supported in a connect expression")
+
+    text, kind, value = c(tree)
+    if value is not None and not isinstance(value, bool):
+        value = float(value)
+    return text, value

@@ -43,6 +43,11 @@ struct HsCgKernels {
     hipFunction_t syn_post = nullptr;
 };
 
+/* the two connect passes compiled with a condition and a per-pair probability */
+struct HsCgConnKernels {
+    hipFunction_t count = nullptr, write = nullptr;
+};
+
 /* checks a model's names and code; returns HS_OK or fails with a message */
 int hs_cg_check(const HsCgModel &m);
 int hs_cg_check_syn(const HsSynCgModel &m);
@@ -73,5 +78,16 @@ hipError_t hs_cg_launch_syn_code(const HsCgKernels *K, const void *paths, int np
                                  int koff, int fused, int grid, hipStream_t s);
 hipError_t hs_cg_launch_syn_post(const HsCgKernels *K, const void *paths, int np, const void *groups, HsCtl *ctl,
                                  int koff, int grid, hipStream_t s);
+
+/* connect(condition=..., p=...): cond a C expression (empty: none) over i, j
+   (indices within the source and target ranges), N_pre, N_post, hs_pre(k) and
+   hs_post(k) (variable k of the source / target neuron, as double); p a C
+   expression of the pair's probability (empty: the constant of the spec) */
+int hs_cg_conn_check(const std::string &cond, const std::string &p);
+int hs_cg_conn_kernels(int prec, const std::string &cond, const std::string &p, const HsCgConnKernels **out);
+hipError_t hs_cg_launch_conn_count(const HsCgConnKernels *K, const HsConnSpec &c, const HsConnVars &a, int32_t *cnt,
+                                   int *err, hipStream_t s);
+hipError_t hs_cg_launch_conn_write(const HsCgConnKernels *K, const HsConnSpec &c, const HsConnVars &a,
+                                   const int32_t *bstart, int32_t *post, void *w, int *err, hipStream_t s);
 
 #endif /* HS_CODEGEN_H */

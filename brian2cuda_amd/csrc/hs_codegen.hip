@@ -192,17 +192,10 @@ static void hs_cg_struct(std::string &o, const char *R, int c, const HsCgModel &
     o += "    }\n};\n";
 }
 
-std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn)
-{
-    (void)prec;
-    std::string o;
-    if (!models.empty()) o += "#define HS_CG_MODELS " + std::to_string(models.size()) + "\n";
-    if (!syn.empty()) o += "#define HS_SCG_PATHS " + std::to_string(syn.size()) + "\n";
-    o += "#include \"hs_engine.hip\"\n";
-    o += "#include \"hs_syncode.h\"\n";
-    /* the functions code models may call: the library's deterministic exp and
-       log, IEEE sqrt; overloads keep the argument's precision */
-    o += R"HSCG(
+/* the functions generated code may call: the library's deterministic exp and
+   log, IEEE sqrt; overloads keep the argument's precision.  Included after
+   the engine sources, whose own calls are not renamed by the macros */
+static const char *const hs_cg_math = R"HSCG(
 __device__ __forceinline__ double hs_cg_exp(double x) { return hs_exp(x); }
 __device__ __forceinline__ float hs_cg_exp(float x) { return hs_expf(x); }
 __device__ __forceinline__ double hs_cg_exp(int x) { return hs_exp((double)x); }
@@ -217,6 +210,12 @@ __device__ __forceinline__ double hs_cg_sqrt(int x) { return hs_sqrt((double)x);
 __device__ __forceinline__ double hs_cg_abs(double x) { return x < 0.0 ? -x : x; }
 __device__ __forceinline__ float hs_cg_abs(float x) { return x < 0.0f ? -x : x; }
 __device__ __forceinline__ int hs_cg_abs(int x) { return x < 0 ? -x : x; }
+__device__ __forceinline__ long long hs_cg_abs(long long x) { return x < 0 ? -x : x; }
+__device__ __forceinline__ double hs_cg_exp(long long x) { return hs_exp((double)x); }
+__device__ __forceinline__ double hs_cg_log(long long x) { return hs_log((double)x); }
+__device__ __forceinline__ double hs_cg_sqrt(long long x) { return hs_sqrt((double)x); }
+__device__ __forceinline__ double hs_cg_floor(long long x) { return (double)x; }
+__device__ __forceinline__ double hs_cg_ceil(long long x) { return (double)x; }
 __device__ __forceinline__ double hs_cg_floor(double x) { return floor(x); }
 __device__ __forceinline__ float hs_cg_floor(float x) { return floorf(x); }
 __device__ __forceinline__ double hs_cg_ceil(double x) { return ceil(x); }
@@ -240,6 +239,16 @@ __device__ __forceinline__ auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo +
 #define sign hs_cg_sign
 #define clip hs_cg_clip
 )HSCG";
+
+std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn)
+{
+    (void)prec;
+    std::string o;
+    if (!models.empty()) o += "#define HS_CG_MODELS " + std::to_string(models.size()) + "\n";
+    if (!syn.empty()) o += "#define HS_SCG_PATHS " + std::to_string(syn.size()) + "\n";
+    o += "#include \"hs_engine.hip\"\n";
+    o += "#include \"hs_syncode.h\"\n";
+    o += hs_cg_math;
     for (size_t c = 0; c < models.size(); ++c) {
         hs_cg_struct(o, "double", (int)c, models[c]);
         hs_cg_struct(o, "float", (int)c, models[c]);
@@ -258,6 +267,72 @@ struct HsCgEntry {
 
 static std::mutex g_cg_mu;
 static std::map<std::string, HsCgEntry> g_cg_cache; /* precision + program text -> loaded kernels (device of hs_init) */
+
+/* compiles src (with the engine sources as its headers) for the current device
+   and loads the kernels names[k] into *slots[k]; what names the program in
+   errors */
+static int hs_rtc_build(const std::string &src, const char *prog_name, const char *what,
+                        const std::vector<std::string> &names, const std::vector<hipFunction_t *> &slots,
+                        hipModule_t *mod)
+{
+    const int nk = (int)names.size();
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), prog_name, hs_rtc_count, hs_rtc_texts, hs_rtc_names) !=
+        HIPRTC_SUCCESS)
+        return hs_rt_fail(HS_ERR_HIP, "hipRTC: program creation failed");
+    for (const std::string &n : names) (void)hiprtcAddNameExpression(prog, n.c_str());
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        (void)hiprtcDestroyProgram(&prog);
+        return hs_rt_fail(HS_ERR_HIP, "hipRTC: no device properties");
+    }
+    std::string arch = prop.gcnArchName;
+    arch = "--offload-arch=" + arch.substr(0, arch.find(':'));
+    /* the flags the library itself is built with: the numerical contract
+       (no contraction, no fast math, no packed FP32) and hardware FP atomics */
+    const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+                          "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult cr = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    if (cr != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        std::string log;
+        if (hiprtcGetProgramLogSize(prog, &n) == HIPRTC_SUCCESS && n > 1) {
+            log.resize(n);
+            (void)hiprtcGetProgramLog(prog, &log[0]);
+        }
+        (void)hiprtcDestroyProgram(&prog);
+        if (log.size() > 3000) log = "..." + log.substr(log.size() - 3000);
+        return hs_rt_fail(HS_ERR_ARG, "%s do not compile: %s", what, log.c_str());
+    }
+    size_t code_n = 0;
+    std::vector<char> code;
+    if (hiprtcGetCodeSize(prog, &code_n) == HIPRTC_SUCCESS && code_n) {
+        code.resize(code_n);
+        (void)hiprtcGetCode(prog, code.data());
+    }
+    std::vector<std::string> lowered(nk);
+    for (int k = 0; k < nk; ++k) {
+        const char *ln = nullptr;
+        if (hiprtcGetLoweredName(prog, names[k].c_str(), &ln) != HIPRTC_SUCCESS || !ln) {
+            (void)hiprtcDestroyProgram(&prog);
+            return hs_rt_fail(HS_ERR_HIP, "hipRTC: no symbol for %s", names[k].c_str());
+        }
+        lowered[k] = ln;
+    }
+    (void)hiprtcDestroyProgram(&prog);
+    if (code.empty()) return hs_rt_fail(HS_ERR_HIP, "hipRTC: empty code object");
+    hipError_t he = hipModuleLoadData(mod, code.data());
+    if (he != hipSuccess) return hs_rt_fail(HS_ERR_HIP, "%s kernels: %s", what, hipGetErrorString(he));
+    for (int k = 0; k < nk; ++k) {
+        if ((he = hipModuleGetFunction(slots[k], *mod, lowered[k].c_str())) != hipSuccess) {
+            (void)hipModuleUnload(*mod);
+            *mod = nullptr;
+            return hs_rt_fail(HS_ERR_HIP, "%s kernel %s: %s", what, names[k].c_str(), hipGetErrorString(he));
+        }
+    }
+    return HS_OK;
+}
 
 int hs_cg_kernels(int prec, int ms, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn,
                   const HsCgKernels **out)
@@ -301,64 +376,123 @@ int hs_cg_kernels(int prec, int ms, const std::vector<HsCgModel> &models, const 
         names.push_back(std::string("hs_k_syn_post<") + R + ">");
         slots.push_back(&e.k.syn_post);
     }
-    const int nk = (int)names.size();
-    hiprtcProgram prog;
-    if (hiprtcCreateProgram(&prog, src.c_str(), "hs_code_models.hip", hs_rtc_count, hs_rtc_texts, hs_rtc_names) !=
-        HIPRTC_SUCCESS)
-        return hs_rt_fail(HS_ERR_HIP, "hipRTC: program creation failed");
-    for (const std::string &n : names) (void)hiprtcAddNameExpression(prog, n.c_str());
-    int dev = 0;
-    hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
-        (void)hiprtcDestroyProgram(&prog);
-        return hs_rt_fail(HS_ERR_HIP, "hipRTC: no device properties");
-    }
-    std::string arch = prop.gcnArchName;
-    arch = "--offload-arch=" + arch.substr(0, arch.find(':'));
-    /* the flags the library itself is built with: the numerical contract
-       (no contraction, no fast math, no packed FP32) and hardware FP atomics */
-    const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
-                          "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-    const hiprtcResult cr = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
-    if (cr != HIPRTC_SUCCESS) {
-        size_t n = 0;
-        std::string log;
-        if (hiprtcGetProgramLogSize(prog, &n) == HIPRTC_SUCCESS && n > 1) {
-            log.resize(n);
-            (void)hiprtcGetProgramLog(prog, &log[0]);
-        }
-        (void)hiprtcDestroyProgram(&prog);
-        if (log.size() > 3000) log = "..." + log.substr(log.size() - 3000);
-        return hs_rt_fail(HS_ERR_ARG, "code models do not compile: %s", log.c_str());
-    }
-    size_t code_n = 0;
-    std::vector<char> code;
-    if (hiprtcGetCodeSize(prog, &code_n) == HIPRTC_SUCCESS && code_n) {
-        code.resize(code_n);
-        (void)hiprtcGetCode(prog, code.data());
-    }
-    std::vector<std::string> lowered(nk);
-    for (int k = 0; k < nk; ++k) {
-        const char *ln = nullptr;
-        if (hiprtcGetLoweredName(prog, names[k].c_str(), &ln) != HIPRTC_SUCCESS || !ln) {
-            (void)hiprtcDestroyProgram(&prog);
-            return hs_rt_fail(HS_ERR_HIP, "hipRTC: no symbol for %s", names[k].c_str());
-        }
-        lowered[k] = ln;
-    }
-    (void)hiprtcDestroyProgram(&prog);
-    if (code.empty()) return hs_rt_fail(HS_ERR_HIP, "hipRTC: empty code object");
-    hipError_t he = hipModuleLoadData(&e.mod, code.data());
-    if (he != hipSuccess) return hs_rt_fail(HS_ERR_HIP, "code model kernels: %s", hipGetErrorString(he));
-    for (int k = 0; k < nk; ++k) {
-        if ((he = hipModuleGetFunction(slots[k], e.mod, lowered[k].c_str())) != hipSuccess) {
-            (void)hipModuleUnload(e.mod);
-            return hs_rt_fail(HS_ERR_HIP, "code model kernel %s: %s", names[k].c_str(), hipGetErrorString(he));
-        }
-    }
+    HS_CG_TRY(hs_rtc_build(src, "hs_code_models.hip", "code models", names, slots, &e.mod));
     auto ins = g_cg_cache.emplace(key, e);
     *out = &ins.first->second.k;
     return HS_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* connect conditions and probabilities                                      */
+/* ------------------------------------------------------------------------ */
+int hs_cg_conn_check(const std::string &cond, const std::string &p)
+{
+    for (const std::string *e : {&cond, &p}) {
+        if (e->size() > 16384) return hs_rt_fail(HS_ERR_ARG, "connect expression longer than 16 kB");
+        if (e->find_first_of("#;{}") != std::string::npos)
+            return hs_rt_fail(HS_ERR_ARG, "a connect expression holds '#', ';' or a brace: expressions only");
+        int depth = 0;
+        for (char ch : *e) {
+            depth += ch == '(' ? 1 : (ch == ')' ? -1 : 0);
+            if (depth < 0) break;
+        }
+        if (depth != 0) return hs_rt_fail(HS_ERR_ARG, "a connect expression has unbalanced parentheses");
+    }
+    return HS_OK;
+}
+
+/* the program of one condition / probability pair: the filter policy HsConnCg
+   of hs_conndev.h and the two passes instantiated with it */
+static std::string hs_cg_conn_source(const std::string &cond, const std::string &p)
+{
+    std::string o = "#include \"hs_conndev.h\"\n#include \"hs_cgconn.h\"\n";
+    o += hs_cg_math;
+    o += "#define hs_pre(k) ((double)((const R *)hs_a.pre[k])[hs_ai])\n";
+    o += "#define hs_post(k) ((double)((const R *)hs_a.post[k])[hs_aj])\n";
+    o += "struct HsConnCg {\n";
+    o += "    static constexpr bool COND = " + std::string(cond.empty() ? "false" : "true") +
+         ", PEXPR = " + std::string(p.empty() ? "false" : "true") + ";\n";
+    const char *head =
+        "(const HsConnSpec &hs_c, const HsConnVars &hs_a, long long hs_ai, long long hs_aj)\n    {\n"
+        "        const long long i = hs_ai - hs_c.pre_lo, j = hs_aj - hs_c.post_lo;\n"
+        "        const long long N_pre = (long long)hs_c.pre_hi - hs_c.pre_lo, N_post = (long long)hs_c.post_hi - "
+        "hs_c.post_lo;\n"
+        "        (void)hs_a; (void)i; (void)j; (void)N_pre; (void)N_post;\n";
+    o += "    template <typename R>\n    static __device__ __forceinline__ bool cond" + std::string(head);
+    o += "        return (bool)(" + (cond.empty() ? std::string("true") : cond) + ");\n    }\n";
+    o += "    template <typename R>\n    static __device__ __forceinline__ double p" + std::string(head);
+    o += "        return (double)(" + (p.empty() ? std::string("0.0") : p) + ");\n    }\n};\n";
+    o += R"HSCG(
+template <typename R>
+__global__ void __launch_bounds__(HS_CONN_WAVES * HS_WAVE)
+    hs_k_conn_count_cg(HsConnSpec c, HsConnVars a, int32_t *__restrict__ cnt, int *err)
+{
+    extern __shared__ int32_t tab[];
+    hs_conn_count_body<HsConnCg, R>(c, a, cnt, err, tab);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(HS_CONN_WAVES * HS_WAVE)
+    hs_k_conn_write_cg(HsConnSpec c, HsConnVars a, const int32_t *__restrict__ bstart, int32_t *__restrict__ post,
+                       R *__restrict__ w, int *err)
+{
+    extern __shared__ int32_t cur[];
+    hs_conn_write_body<HsConnCg, R>(c, a, bstart, post, w, err, cur);
+}
+)HSCG";
+    return o;
+}
+
+struct HsCgConnEntry {
+    hipModule_t mod = nullptr;
+    HsCgConnKernels k;
+};
+static std::map<std::string, HsCgConnEntry> g_cg_conn_cache; /* precision + program text -> kernels */
+
+int hs_cg_conn_kernels(int prec, const std::string &cond, const std::string &p, const HsCgConnKernels **out)
+{
+    HS_CG_TRY(hs_cg_conn_check(cond, p));
+    const std::string src = hs_cg_conn_source(cond, p);
+    const std::string key = std::to_string(prec) + "\n" + src;
+    std::lock_guard<std::mutex> lock(g_cg_mu);
+    auto it = g_cg_conn_cache.find(key);
+    if (it != g_cg_conn_cache.end()) {
+        *out = &it->second.k;
+        return HS_OK;
+    }
+    const char *R = prec == HS_F64 ? "double" : "float";
+    HsCgConnEntry e;
+    const std::vector<std::string> names = {std::string("hs_k_conn_count_cg<") + R + ">",
+                                            std::string("hs_k_conn_write_cg<") + R + ">"};
+    const std::vector<hipFunction_t *> slots = {&e.k.count, &e.k.write};
+    HS_CG_TRY(hs_rtc_build(src, "hs_connect_code.hip", "connect expressions", names, slots, &e.mod));
+    auto ins = g_cg_conn_cache.emplace(key, e);
+    *out = &ins.first->second.k;
+    return HS_OK;
+}
+
+hipError_t hs_cg_launch_conn_count(const HsCgConnKernels *K, const HsConnSpec &c, const HsConnVars &a, int32_t *cnt,
+                                   int *err, hipStream_t s)
+{
+    if (c.pre_hi <= c.pre_lo) return hipSuccess;
+    if (c.D < 1 || c.D > HS_MAX_DELAY_SLOTS || c.fixed_k < 0 || c.fixed_k > HS_CONN_KMAX) return hipErrorInvalidValue;
+    HsConnSpec cc = c;
+    HsConnVars aa = a;
+    void *args[] = {&cc, &aa, &cnt, &err};
+    return hipModuleLaunchKernel(K->count, (unsigned)hs_conn_grid(c), 1, 1, (unsigned)(hs_conn_waves(c) * 64),
+                                 1, 1, (unsigned)hs_conn_lds(c), s, args, nullptr);
+}
+
+hipError_t hs_cg_launch_conn_write(const HsCgConnKernels *K, const HsConnSpec &c, const HsConnVars &a,
+                                   const int32_t *bstart, int32_t *post, void *w, int *err, hipStream_t s)
+{
+    if (c.pre_hi <= c.pre_lo) return hipSuccess;
+    if (c.D < 1 || c.D > HS_MAX_DELAY_SLOTS || c.fixed_k < 0 || c.fixed_k > HS_CONN_KMAX) return hipErrorInvalidValue;
+    HsConnSpec cc = c;
+    HsConnVars aa = a;
+    void *args[] = {&cc, &aa, &bstart, &post, &w, &err};
+    return hipModuleLaunchKernel(K->write, (unsigned)hs_conn_grid(c), 1, 1, (unsigned)(hs_conn_waves(c) * 64),
+                                 1, 1, (unsigned)hs_conn_lds(c), s, args, nullptr);
 }
 
 static hipError_t hs_cg_go(hipFunction_t f, int grid, int block, hipStream_t s, void **args)

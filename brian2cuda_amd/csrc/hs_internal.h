@@ -217,6 +217,13 @@ struct HsConnSpec {
     int fixed_k;           /* > 0: fixed out-degree K (then p and jump are unused) */
 };
 
+/* the state arrays (precision of the network) of the source and target groups,
+   for connect conditions and probabilities that read them */
+struct HsConnVars {
+    const void *pre[HS_MAX_VARS];
+    const void *post[HS_MAX_VARS];
+};
+
 #ifndef __HIPCC_RTC__ /* host launchers */
 /* count pass: cnt[i*D + slot] (int32, zeroed by the caller) for rows [pre_lo, pre_hi);
    *err becomes nonzero if a slot falls outside [0, D) */
@@ -225,6 +232,10 @@ hipError_t hs_launch_conn_count(const HsConnSpec &c, int32_t *cnt, int *err, hip
    bstart[i*D+slot] + the synapse's rank in its bundle */
 hipError_t hs_launch_conn_write(int prec, const HsConnSpec &c, const int32_t *bstart, int32_t *post, void *w,
                                 int *err, hipStream_t s);
+/* launch shape of the two passes: waves (rows in flight) per block, dynamic LDS bytes, grid */
+int hs_conn_waves(const HsConnSpec &c);
+size_t hs_conn_lds(const HsConnSpec &c);
+int hs_conn_grid(const HsConnSpec &c);
 /* exclusive prefix sum of n int32 counts: out[0..n] (out[n] = total, also at
    bsum[hs_scan_blocks(n)] as int64); bsum holds hs_scan_blocks(n) + 1 entries */
 int hs_scan_blocks(long long n);

@@ -892,9 +892,30 @@ static int hs_connect_q(double d, double dt, int *q)
 }
 
 /* fixed_k > 0: K distinct targets per source neuron (p unused); else connect(p) */
+/* every hs_pre(k) / hs_post(k) of a connect expression names a variable the
+   source / target group has (a literal k below its count): the kernels read
+   the group's array k */
+static int hs_conn_expr_vars(const std::string &e, const char *fn, int nvars)
+{
+    const std::string pat = std::string(fn) + "(";
+    for (size_t at = e.find(pat); at != std::string::npos; at = e.find(pat, at + 1)) {
+        if (at > 0 && (isalnum((unsigned char)e[at - 1]) || e[at - 1] == '_')) continue;
+        size_t q = at + pat.size();
+        int k = 0, digits = 0;
+        while (q < e.size() && isdigit((unsigned char)e[q]) && digits < 3) k = 10 * k + (e[q++] - '0'), ++digits;
+        if (!digits || q >= e.size() || e[q] != ')' || k >= nvars)
+            return hs_fail(HS_ERR_ARG, "connect expression: %s(...) must name one of the group's %d variables", fn,
+                           nvars);
+    }
+    return HS_OK;
+}
+
+/* cond / pexpr: a connect condition and a per-pair probability as C expressions
+   (empty: none; hs_cg_conn_kernels), compiled into the two passes */
 static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo, int post_hi,
                              double p, int fixed_k, int autapses, uint64_t seed, uint32_t stream, double delay_lo,
-                             double delay_hi, int wmode, double w_lo, double w_hi, int *pid)
+                             double delay_hi, int wmode, double w_lo, double w_hi, int *pid,
+                             const std::string &cond = std::string(), const std::string &pexpr = std::string())
 {
     if (!net) return hs_fail(HS_ERR_ARG, "null net");
     if (net->built) return hs_fail(HS_ERR_STATE, "pathways must be added before the first run");
@@ -906,8 +927,18 @@ static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo,
     if (pre_lo < 0 || pre_lo > pre_hi || pre_hi > gs.n) return hs_fail(HS_ERR_ARG, "bad source range");
     if (post_lo < 0 || post_lo > post_hi || post_hi > gt.n) return hs_fail(HS_ERR_ARG, "bad target range");
     if (fixed_k < 0) return hs_fail(HS_ERR_ARG, "out-degree %d < 0", fixed_k);
+    if (!pexpr.empty()) {
+        if (fixed_k) return hs_fail(HS_ERR_ARG, "a probability expression with a fixed out-degree");
+        p = 1.0; /* rows are not skipped; every pair takes its own draw against p(i, j) */
+    }
     if (!fixed_k && (!(p >= 0.0) || !(p <= 1.0)))
         return hs_fail(HS_ERR_ARG, "connection probability %g outside [0, 1]", p);
+    const HsCgConnKernels *K = nullptr;
+    for (const std::string *x : {&cond, &pexpr}) {
+        HS_TRY(hs_conn_expr_vars(*x, "hs_pre", gs.mi.nvars));
+        HS_TRY(hs_conn_expr_vars(*x, "hs_post", gt.mi.nvars));
+    }
+    if (!cond.empty() || !pexpr.empty()) HS_TRY(hs_cg_conn_kernels(net->prec, cond, pexpr, &K));
     if (!(delay_lo >= 0.0) || !(delay_hi >= delay_lo) || delay_hi > 1e300)
         return hs_fail(HS_ERR_ARG, "bad delay range [%g, %g]", delay_lo, delay_hi);
     if (wmode != HS_W_SCALAR && wmode != HS_W_SYNAPSE)
@@ -930,7 +961,7 @@ static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo,
     }
     c.fixed_k = fixed_k;
     c.p = fixed_k ? 0.0 : p;
-    c.jump = !fixed_k && p < 0.25 ? 1 : 0;
+    c.jump = !fixed_k && pexpr.empty() && p < 0.25 ? 1 : 0;
     c.pconst = c.jump && p > 0.0 ? 1.0 / hs_log(1.0 - p) : 0.0;
     c.seed = seed;
     c.stream = stream;
@@ -985,7 +1016,12 @@ static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo,
     if ((e = hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)(nb ? nb : 1), g_rt.stream)) != hipSuccess ||
         (e = hipMemsetAsync(err, 0, sizeof(int), g_rt.stream)) != hipSuccess)
         return fail_hip(e, "clear");
-    if ((e = hs_launch_conn_count(c, cnt, err, g_rt.stream)) != hipSuccess) return fail_hip(e, "count pass");
+    HsConnVars cv = {};
+    for (int v = 0; v < gs.mi.nvars && v < HS_MAX_VARS; ++v) cv.pre[v] = gs.var[v];
+    for (int v = 0; v < gt.mi.nvars && v < HS_MAX_VARS; ++v) cv.post[v] = gt.var[v];
+    if ((e = K ? hs_cg_launch_conn_count(K, c, cv, cnt, err, g_rt.stream) : hs_launch_conn_count(c, cnt, err, g_rt.stream)) !=
+        hipSuccess)
+        return fail_hip(e, "count pass");
     if ((e = hs_launch_scan(cnt, nb, bsum, P.bstart, g_rt.stream)) != hipSuccess) return fail_hip(e, "prefix sum");
     long long total = 0;
     int herr = 0;
@@ -1014,8 +1050,10 @@ static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo,
         release();
         return r;
     }
-    if ((e = hs_launch_conn_write(net->prec == HS_F64 ? 64 : 32, c, P.bstart, P.post,
-                                  wmode == HS_W_SYNAPSE ? P.w : nullptr, err, g_rt.stream)) != hipSuccess)
+    if ((e = K ? hs_cg_launch_conn_write(K, c, cv, P.bstart, P.post, wmode == HS_W_SYNAPSE ? P.w : nullptr, err,
+                                         g_rt.stream)
+               : hs_launch_conn_write(net->prec == HS_F64 ? 64 : 32, c, P.bstart, P.post,
+                                      wmode == HS_W_SYNAPSE ? P.w : nullptr, err, g_rt.stream)) != hipSuccess)
         return fail_hip(e, "write pass");
     if ((e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, g_rt.stream)) != hipSuccess ||
         (e = hipStreamSynchronize(g_rt.stream)) != hipSuccess)
@@ -1045,6 +1083,17 @@ extern "C" int hs_net_connect_fixed(hs_net *net, int src, int tgt, int var, int 
     /* K = 0 creates no synapse: the connect(p = 0) form */
     return hs_connect_device(net, src, tgt, var, pre_lo, pre_hi, post_lo, post_hi, k ? 1.0 : 0.0, k, autapses, seed,
                              stream, delay_lo, delay_hi, wmode, w_lo, w_hi, pid);
+}
+
+extern "C" int hs_net_connect_code(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo,
+                                   int post_hi, const char *condition, const char *p_expr, double p, int k, int autapses,
+                                   uint64_t seed, uint32_t stream, double delay_lo, double delay_hi, int wmode,
+                                   double w_lo, double w_hi, int *pid)
+{
+    if (!net) return hs_fail(HS_ERR_ARG, "null net");
+    const std::string cond = condition ? condition : "", pe = p_expr ? p_expr : "";
+    return hs_connect_device(net, src, tgt, var, pre_lo, pre_hi, post_lo, post_hi, k ? 1.0 : p, k, autapses, seed,
+                             stream, delay_lo, delay_hi, wmode, w_lo, w_hi, pid, cond, pe);
 }
 
 /*

@@ -50,6 +50,10 @@ class RandomConnect:
     wmode: int = HS_W_SCALAR
     w: tuple = (0.0, 0.0)
     k: int = 0  # > 0: fixed out-degree K instead of p (sample(N_post, size=K))
+    # connect(condition=..., p='expression'): C expressions of
+    # codegen.connect_expression ("" for none), compiled where the pathway is built
+    cond: str = ""
+    p_expr: str = ""
 
 
 @dataclass
@@ -619,7 +623,10 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
     for p in spec.paths:
         if p.random is not None:
             r = p.random
-            if r.k:
+            if r.cond or r.p_expr:
+                pid = net.connect_code(p.src, p.tgt, p.var, r.cond, r.p_expr, r.p, r.k, r.pre, r.post, r.autapses,
+                                       r.seed, r.stream, r.delay, r.wmode, r.w)
+            elif r.k:
                 pid = net.connect_fixed(p.src, p.tgt, p.var, r.k, r.pre, r.post, r.autapses, r.seed, r.stream,
                                         r.delay, r.wmode, r.w)
             else:

@@ -311,6 +311,28 @@ class Net:
              float(dhi), int(wmode), float(wlo), float(whi), C.byref(pid))
         return pid.value
 
+    def connect_code(self, src, tgt, var, cond="", p_expr="", p=1.0, K=0, pre=None, post=None, autapses=True,
+                     seed=None, stream=0, delay=0.0, wmode=HS_W_SCALAR, w=0.0):
+        """connect(condition=..., p=...) generated on the device with the
+        condition and the per-pair probability compiled from C expressions
+        (brian2cuda_amd.codegen.connect_expression; "" for none): with p_expr
+        each pair is kept where cond holds and its draw is below p_expr, else
+        the candidates of connect(p) (K > 0: of the fixed out-degree sample)
+        are kept where cond holds.  The expressions read the groups' state as
+        it is when this is called.  Delays and weights as connect_random.
+        Returns the pathway id."""
+        n_src, n_tgt = self.groups[src][1], self.groups[tgt][1]
+        pre = (0, n_src) if pre is None else pre
+        post = (0, n_tgt) if post is None else post
+        dlo, dhi = (delay, delay) if np.isscalar(delay) else delay
+        wlo, whi = (w, w) if np.isscalar(w) else w
+        pid = C.c_int()
+        call("hs_net_connect_code", self.h, src, tgt, var, int(pre[0]), int(pre[1]), int(post[0]), int(post[1]),
+             (cond or "").encode(), (p_expr or "").encode(), float(p), int(K), int(bool(autapses)),
+             int(self.seed if seed is None else seed), int(stream), float(dlo), float(dhi), int(wmode), float(wlo),
+             float(whi), C.byref(pid))
+        return pid.value
+
     def set_spikegen(self, gid, indices, steps):
         """SpikeGeneratorGroup pattern: neuron indices[k] fires at step steps[k]."""
         i = np.ascontiguousarray(indices, dtype=np.int32)

@@ -250,6 +250,39 @@ int hs_net_connect_fixed(hs_net *net, int src, int tgt, int var, int pre_lo, int
                          int k, int autapses, uint64_t seed, uint32_t stream, double delay_lo, double delay_hi,
                          int wmode, double w_lo, double w_hi, int *pid);
 /*
+ * Connectivity with a condition and a per-pair probability given as code
+ * (Synapses.connect(condition='...', p=...), connect(condition='...',
+ * p='<expression of i, j, x_pre, x_post>'), and the generator form
+ * connect(j='k for k in sample(N_post, size=K) if <condition>')), generated on
+ * the device by the passes of hs_net_connect_random / _fixed compiled at this
+ * call (hipRTC) with:
+ *   condition  a C expression (NULL or "": none) over i, j (long long, the
+ *              indices within [pre_lo, pre_hi) and [post_lo, post_hi), as
+ *              Brian2's i and j of a Synapses on those subgroups), N_pre,
+ *              N_post (the range sizes), hs_pre(k) / hs_post(k) (variable k of
+ *              the source / target neuron as double, read when this is
+ *              called: set the state first) and the functions exp, log,
+ *              log10, sqrt, abs, floor, ceil, sign, clip, hs_cg_floordiv,
+ *              hs_cg_pymod, hs_cg_pow (Python's //, % and **);
+ *   p_expr     the pair's probability as such an expression (NULL or "": the
+ *              constant p);
+ *   k          > 0: fixed out-degree K (then p and p_expr are unused).
+ * Selection is that of the two entry points: with a constant p, candidates
+ * are drawn by geometric skips (p < 0.25) or one draw per pair and then kept
+ * where the condition holds ('k for k in sample(N_post, p=p) if condition',
+ * synapses_create_generator.cu:213-333); with p_expr each pair (i, j) is kept
+ * where the condition holds and its draw is below p_expr; with K the K
+ * sampled targets are filtered by the condition.  Delays, weights and the
+ * bundle table as hs_net_connect_random.  An expression that does not compile
+ * fails with HS_ERR_ARG and the compiler's log (hs_last_error).
+ * Replaces the generated host loop of synapses_create_generator.cu:164-465 for
+ * conditions and probabilities that are expressions.
+ */
+int hs_net_connect_code(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo, int post_hi,
+                        const char *condition, const char *p_expr, double p, int k, int autapses, uint64_t seed,
+                        uint32_t stream, double delay_lo, double delay_hi, int wmode, double w_lo, double w_hi,
+                        int *pid);
+/*
  * hs_net_add_pathway with the bundle table built on the device: pre, post,
  * delay_steps (or NULL: scalar_delay) and weights are DEVICE pointers (hs_alloc
  * or any allocation on the device) holding int32 / double arrays of the same

@@ -47,6 +47,10 @@ def lib():
         "ref_connect_fixed": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                           C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                           I32P, I32P, I32P, DP, C.c_int64]),
+        "ref_connect_code": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                         C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         I32P, I32P, I32P, DP, C.c_int64]),
         "ref_timestep": (C.c_longlong, [C.c_double, C.c_double]),
         "ref_exp": (C.c_double, [C.c_double]),
         "ref_expf": (C.c_float, [C.c_float]),
@@ -207,6 +211,12 @@ static inline double hs_cg_sqrt(int x) { return hs_sqrt((double)x); }
 static inline double hs_cg_abs(double x) { return x < 0.0 ? -x : x; }
 static inline float hs_cg_abs(float x) { return x < 0.0f ? -x : x; }
 static inline int hs_cg_abs(int x) { return x < 0 ? -x : x; }
+static inline long long hs_cg_abs(long long x) { return x < 0 ? -x : x; }
+static inline double hs_cg_exp(long long x) { return hs_exp((double)x); }
+static inline double hs_cg_log(long long x) { return hs_log((double)x); }
+static inline double hs_cg_sqrt(long long x) { return hs_sqrt((double)x); }
+static inline double hs_cg_floor(long long x) { return (double)x; }
+static inline double hs_cg_ceil(long long x) { return (double)x; }
 static inline double hs_cg_floor(double x) { return floor(x); }
 static inline float hs_cg_floor(float x) { return floorf(x); }
 static inline double hs_cg_ceil(double x) { return ceil(x); }
@@ -291,6 +301,55 @@ def _host_lib(src):
             os.replace(tmp, so)
         _cg_libs[key] = C.CDLL(so)
     return _cg_libs[key]
+
+
+def connect_code_source(cond, p_expr, precision):
+    """The host program of a connect condition / probability: extern "C"
+    hs_cn_cond and hs_cn_p over the absolute indices (ai, aj), the ranges and
+    the groups' state arrays, the expressions of the device program."""
+    R = "double" if precision == 64 else "float"
+    args = ("(const void *const *hs_pre_v, const void *const *hs_post_v, long long hs_ai, long long hs_aj, "
+            "long long pre_lo, long long pre_hi, long long post_lo, long long post_hi)")
+    head = ("{\n    const long long i = hs_ai - pre_lo, j = hs_aj - post_lo, N_pre = pre_hi - pre_lo, "
+            "N_post = post_hi - post_lo;\n    (void)hs_pre_v; (void)hs_post_v; (void)i; (void)j; (void)N_pre; "
+            "(void)N_post;\n")
+    return (_CG_MATH + '#include "hs_cgconn.h"\n' + f"typedef {R} R;\n"
+            "#define hs_pre(k) ((double)((const R *)hs_pre_v[k])[hs_ai])\n"
+            "#define hs_post(k) ((double)((const R *)hs_post_v[k])[hs_aj])\n"
+            f'extern "C" int hs_cn_cond{args}\n{head}    return ({cond or "true"}) ? 1 : 0;\n}}\n'
+            f'extern "C" double hs_cn_p{args}\n{head}    return (double)({p_expr or "0.0"});\n}}\n')
+
+
+def connect_code(pre_range, post_range, cond, p_expr, p, K, skip_self, seed, stream, delay, dt, weight,
+                 pre_vars=(), post_vars=(), precision=64):
+    """Synapses of connect(condition=cond, p=p or p_expr) or of a fixed
+    out-degree K filtered by cond, the C expressions of the device program
+    (brian2cuda_amd.codegen.connect_expression), compiled for the host;
+    pre_vars / post_vars: the source / target groups' state arrays (read in
+    precision).  Creation order; dict of pre, post, delay_steps, w."""
+    L = lib()
+    fns = _host_lib(connect_code_source(cond, p_expr, precision))
+    dt_np = np.float64 if precision == 64 else np.float32
+    keep = [np.ascontiguousarray(v, dtype=dt_np) for v in list(pre_vars) + list(post_vars)]
+    npre = len(pre_vars)
+    arr_pre = (C.c_void_p * max(1, npre))(*[a.ctypes.data for a in keep[:npre]])
+    arr_post = (C.c_void_p * max(1, len(keep) - npre))(*[a.ctypes.data for a in keep[npre:]])
+    cf = C.cast(fns.hs_cn_cond, C.c_void_p).value if cond else None
+    pf = C.cast(fns.hs_cn_p, C.c_void_p).value if p_expr else None
+    args = (int(pre_range[0]), int(pre_range[1]), int(post_range[0]), int(post_range[1]), float(p), int(K),
+            int(bool(skip_self)), int(seed), int(stream), float(delay[0]), float(delay[1]), float(dt),
+            float(weight[0]), float(weight[1]), cf, pf, C.cast(arr_pre, C.c_void_p), C.cast(arr_post, C.c_void_p))
+    if K > 0 and K > int(post_range[1]) - int(post_range[0]) and int(pre_range[1]) > int(pre_range[0]):
+        raise ValueError("sample size K bigger than the population size")
+    n = L.ref_connect_code(*args, None, None, None, None, 0)
+    out = {"pre": np.empty(n, np.int32), "post": np.empty(n, np.int32), "delay_steps": np.empty(n, np.int32),
+           "w": np.empty(n, np.float64)}
+    I32P, DP = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    m = L.ref_connect_code(*args, out["pre"].ctypes.data_as(I32P), out["post"].ctypes.data_as(I32P),
+                           out["delay_steps"].ctypes.data_as(I32P), out["w"].ctypes.data_as(DP), n)
+    assert m == n
+    del keep
+    return out
 
 
 def synapse_code_functions(code, precision):
@@ -507,12 +566,26 @@ class RefNet:
         return out[: rows * width].reshape(rows, width)
 
 
-def random_pathway(spec, path, rows=None):
+def initial_state(g):
+    """A group spec's state arrays before the first step: zeros, then its init values."""
+    out = [np.zeros(g.n) for _ in _var_names(g)]
+    for var, val in g.init.items():
+        out[var][:] = val
+    return out
+
+
+def random_pathway(spec, path, rows=None, precision=64):
     """The synapses of a RandomConnect pathway (creation order); rows: a
-    sub-range of source rows (rows are generated independently of each other)."""
+    sub-range of source rows (rows are generated independently of each other).
+    A condition or probability given as code reads the groups' initial state
+    in precision."""
     r = path.random
     pre = r.pre if rows is None else rows
     skip_self = (not r.autapses) and path.src == path.tgt
+    if getattr(r, "cond", "") or getattr(r, "p_expr", ""):
+        return connect_code(pre, r.post, r.cond, r.p_expr, r.p, r.k, skip_self, r.seed, r.stream, r.delay, spec.dt,
+                            r.w, initial_state(spec.groups[path.src]), initial_state(spec.groups[path.tgt]),
+                            precision)
     if getattr(r, "k", 0):
         return connect_fixed(pre, r.post, r.k, skip_self, r.seed, r.stream, r.delay, spec.dt, r.w)
     return connect_random(pre, r.post, r.p, skip_self, r.seed, r.stream, r.delay, spec.dt, r.w)
@@ -543,7 +616,7 @@ def build_ref(spec, precision=64, semantics=SEQ, threads=1):
             net.set_spikegen(gid, *g.spikes)
     for p in spec.paths:
         if p.random is not None:
-            syn = random_pathway(spec, p)
+            syn = random_pathway(spec, p, precision=precision)
             pid = net.add_pathway(p.src, p.tgt, p.var, syn["pre"], syn["post"], syn["delay_steps"], 0,
                                   p.random.wmode, syn["w"], p.random.w[0])
         else:

@@ -131,13 +131,36 @@ static void ref_conn_words(uint64_t seed, uint32_t stream, uint32_t i, uint32_t 
     ref_philox4x32_10(c, key, w);
 }
 
-int64_t ref_connect_random(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int skip_self, uint64_t seed,
-                           uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo, double w_hi,
-                           int32_t *pre, int32_t *post, int32_t *delay_steps, double *w, int64_t cap)
+/* a condition / per-pair probability given as code (connect(condition=...,
+   p='...')): compiled for the host from the same C expressions as the device
+   program, called with the absolute indices (ai, aj) and the ranges, and the
+   source / target state arrays */
+typedef int (*ref_conn_cond_fn)(const void *const *pre_vars, const void *const *post_vars, long long ai, long long aj,
+                                long long pre_lo, long long pre_hi, long long post_lo, long long post_hi);
+typedef double (*ref_conn_p_fn)(const void *const *pre_vars, const void *const *post_vars, long long ai, long long aj,
+                                long long pre_lo, long long pre_hi, long long post_lo, long long post_hi);
+struct ref_conn_code {
+    ref_conn_cond_fn cond; /* NULL: none */
+    ref_conn_p_fn p;       /* NULL: the constant p */
+    const void *const *pre_vars, *const *post_vars;
+    int pre_lo, pre_hi, post_lo, post_hi;
+};
+
+static int ref_conn_keep(const struct ref_conn_code *cc, int i, long long j)
+{
+    return !cc || !cc->cond ||
+           cc->cond(cc->pre_vars, cc->post_vars, i, j, cc->pre_lo, cc->pre_hi, cc->post_lo, cc->post_hi) != 0;
+}
+
+static int64_t ref_connect_random_c(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int skip_self,
+                                    uint64_t seed, uint32_t stream, double delay_lo, double delay_hi, double dt,
+                                    double w_lo, double w_hi, const struct ref_conn_code *cc, int32_t *pre,
+                                    int32_t *post, int32_t *delay_steps, double *w, int64_t cap)
 {
     int64_t n = 0;
-    if (p == 0.0) return 0;
-    const int jump = p < 0.25;
+    const int pexpr = cc && cc->p;
+    if (p == 0.0 && !pexpr) return 0;
+    const int jump = !pexpr && p < 0.25;
     const double pconst = jump ? 1.0 / ref_log(1.0 - p) : 0.0;
     const double dspan = delay_hi - delay_lo, wspan = w_hi - w_lo;
     for (int i = pre_lo; i < pre_hi; ++i) {
@@ -157,12 +180,15 @@ int64_t ref_connect_random(int pre_lo, int pre_hi, int post_lo, int post_hi, dou
                 const uint32_t m = (uint32_t)(j - post_lo);
                 if ((m & 1u) == 0) ref_conn_words(seed, stream, (uint32_t)i, m / 2, 2, words);
                 const double r = (m & 1u) ? ref_u53(words[2], words[3]) : ref_u53(words[0], words[1]);
-                if (r >= p) {
+                const double pp =
+                    pexpr ? cc->p(cc->pre_vars, cc->post_vars, i, j, cc->pre_lo, cc->pre_hi, cc->post_lo, cc->post_hi)
+                          : p;
+                if (!(r < pp)) {
                     ++j;
                     continue;
                 }
             }
-            if (!(skip_self && j == i)) {
+            if (!(skip_self && j == i) && ref_conn_keep(cc, i, j)) {
                 if (n < cap) {
                     uint32_t sw[4];
                     ref_conn_words(seed, stream, (uint32_t)i, (uint32_t)j, 1, sw);
@@ -178,6 +204,14 @@ int64_t ref_connect_random(int pre_lo, int pre_hi, int post_lo, int post_hi, dou
         }
     }
     return n;
+}
+
+int64_t ref_connect_random(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int skip_self, uint64_t seed,
+                           uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo, double w_hi,
+                           int32_t *pre, int32_t *post, int32_t *delay_steps, double *w, int64_t cap)
+{
+    return ref_connect_random_c(pre_lo, pre_hi, post_lo, post_hi, p, skip_self, seed, stream, delay_lo, delay_hi, dt,
+                                w_lo, w_hi, NULL, pre, post, delay_steps, w, cap);
 }
 
 static int ref_cmp_i32(const void *a, const void *b)
@@ -200,9 +234,10 @@ static int ref_cmp_i32(const void *a, const void *b)
  * connect(p).  Returns the synapse count, -1 if K > n (the reference's error
  * "sample size bigger than the population size").
  */
-int64_t ref_connect_fixed(int pre_lo, int pre_hi, int post_lo, int post_hi, int K, int skip_self, uint64_t seed,
-                          uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo, double w_hi,
-                          int32_t *pre, int32_t *post, int32_t *delay_steps, double *w, int64_t cap)
+static int64_t ref_connect_fixed_c(int pre_lo, int pre_hi, int post_lo, int post_hi, int K, int skip_self,
+                                   uint64_t seed, uint32_t stream, double delay_lo, double delay_hi, double dt,
+                                   double w_lo, double w_hi, const struct ref_conn_code *cc, int32_t *pre,
+                                   int32_t *post, int32_t *delay_steps, double *w, int64_t cap)
 {
     int64_t n = 0;
     const int cand = post_hi - post_lo;
@@ -227,7 +262,7 @@ int64_t ref_connect_fixed(int pre_lo, int pre_hi, int post_lo, int post_hi, int 
         }
         for (int k = 0; k < K; ++k) {
             const int32_t j = post_lo + x[k];
-            if (skip_self && j == i) continue;
+            if ((skip_self && j == i) || !ref_conn_keep(cc, i, j)) continue;
             if (n < cap) {
                 uint32_t sw[4];
                 ref_conn_words(seed, stream, (uint32_t)i, (uint32_t)j, 1, sw);
@@ -242,6 +277,47 @@ int64_t ref_connect_fixed(int pre_lo, int pre_hi, int post_lo, int post_hi, int 
     }
     free(x);
     return n;
+}
+
+int64_t ref_connect_fixed(int pre_lo, int pre_hi, int post_lo, int post_hi, int K, int skip_self, uint64_t seed,
+                          uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo, double w_hi,
+                          int32_t *pre, int32_t *post, int32_t *delay_steps, double *w, int64_t cap)
+{
+    return ref_connect_fixed_c(pre_lo, pre_hi, post_lo, post_hi, K, skip_self, seed, stream, delay_lo, delay_hi, dt,
+                               w_lo, w_hi, NULL, pre, post, delay_steps, w, cap);
+}
+
+/*
+ * connect(condition=..., p=...) with the condition and, optionally, the
+ * per-pair probability compiled from code (cond_fn, p_fn: NULL for none):
+ * K > 0 filters the fixed out-degree sample by the condition; otherwise the
+ * candidates of connect(p) (geometric skips or one draw per pair, p_fn taking
+ * the place of p in the per-pair test) are kept where the condition holds --
+ * Brian2's 'k for k in sample(N_post, p=p) if condition'.  The draws are those
+ * of ref_connect_random / ref_connect_fixed, so a condition that always holds
+ * gives their synapses.  pre_vars / post_vars: the groups' state arrays the
+ * code reads.
+ */
+int64_t ref_connect_code(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int K, int skip_self,
+                         uint64_t seed, uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo,
+                         double w_hi, void *cond_fn, void *p_fn, const void *const *pre_vars,
+                         const void *const *post_vars, int32_t *pre, int32_t *post, int32_t *delay_steps, double *w,
+                         int64_t cap)
+{
+    struct ref_conn_code cc;
+    cc.cond = (ref_conn_cond_fn)cond_fn;
+    cc.p = (ref_conn_p_fn)p_fn;
+    cc.pre_vars = pre_vars;
+    cc.post_vars = post_vars;
+    cc.pre_lo = pre_lo;
+    cc.pre_hi = pre_hi;
+    cc.post_lo = post_lo;
+    cc.post_hi = post_hi;
+    if (K > 0)
+        return ref_connect_fixed_c(pre_lo, pre_hi, post_lo, post_hi, K, skip_self, seed, stream, delay_lo, delay_hi,
+                                   dt, w_lo, w_hi, &cc, pre, post, delay_steps, w, cap);
+    return ref_connect_random_c(pre_lo, pre_hi, post_lo, post_hi, p_fn ? 1.0 : p, skip_self, seed, stream, delay_lo,
+                                delay_hi, dt, w_lo, w_hi, &cc, pre, post, delay_steps, w, cap);
 }
 
 long long ref_timestep(double t, double dt) { return hs_timestep(t, dt); }

@@ -1,0 +1,134 @@
+"""GPU parity of connect(condition=..., p=...) given as code
+(hs_net_connect_code: the connect passes compiled at run time with the
+condition and the per-pair probability): the bundle table read back through
+the C ABI equals the oracle's synapses (oracle.connect_code, the same
+expressions compiled for the host) stably sorted by (pre, delay), in both
+precisions; a network connected this way simulates to the oracle's bits; bad
+expressions are refused without a launch."""
+import numpy as np
+import pytest
+
+from brian2cuda_amd import codegen as cg
+from brian2cuda_amd import networks as nw
+from brian2cuda_amd import runtime as rt
+from brian2cuda_amd._lib import HsError
+
+pytestmark = pytest.mark.gpu
+
+DT = 1e-4
+NAMES = ["v", "ge", "gi"]  # the CUBA model's variables, in the runtime's order
+
+
+def _bundle_order(s):
+    order = np.lexsort((s["delay_steps"], s["pre"]))
+    return {k: v[order] for k, v in s.items()}
+
+
+CASES = [
+    # n_src, n_tgt, pre, post, condition, p (number or expression), K, autapses, delay (s), wmode, w
+    (1500, 1500, None, None, "abs(i - j) < 20", 1.0, 0, False, (0.1e-3, 0.1e-3), rt.HS_W_SCALAR, (0.5, 0.5)),
+    (3000, 2000, (200, 2900), (5, 1990), "(i + j) % 3 != 0", 0.05, 0, True, (0.5e-3, 4e-3), rt.HS_W_SYNAPSE,
+     (-1.0, 2.0)),
+    (1200, 1200, None, None, "j < 2 * i and i - j != 7", 0.4, 0, True, (0.0, 2e-3), rt.HS_W_SYNAPSE, (0.0, 1.0)),
+    (2000, 2000, None, None, "i != j", "exp(-(v_pre - v_post)**2 / (2 * 0.02**2))", 0, True, (0.2e-3, 1e-3),
+     rt.HS_W_SYNAPSE, (1.0, 3.0)),
+    (1000, 1500, None, None, "j % 2 == 0 and v_post > 0.25", 1.0, 60, True, (0.3e-3, 0.9e-3), rt.HS_W_SYNAPSE,
+     (1.0, 2.0)),
+    (900, 900, None, None, "", "0.3 * ((i // 7) % 2) + 0.1 * (j > N_post / 2)", 0, True, (0.0, 0.0),
+     rt.HS_W_SCALAR, (1.0, 1.0)),
+    (800, 800, None, None, "i > N_pre", 0.5, 0, True, (0.0, 0.0), rt.HS_W_SCALAR, (1.0, 1.0)),  # empty
+]
+
+
+def _setup(prec, n_src, n_tgt):
+    net = rt.Net(prec, "plain", DT, 1)
+    sizes = [n_src] if n_src == n_tgt else [n_src, n_tgt]
+    pos = []
+    for g, n in enumerate(sizes):
+        net.add_group(rt.HS_MODEL_LIF_CUBA, n, nw.cuba_params())
+        x = np.random.default_rng(100 + g).random(n)  # positions in v, read by the expressions
+        net.set_state(g, 0, x)
+        pos.append(x)
+    return net, pos
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_connect_code_matches_oracle(hs, oracle_lib, prec, case):
+    n_src, n_tgt, pre, post, cond, p, K, autapses, delay, wmode, w = CASES[case]
+    print(f"connect code case {case} prec {prec}: {cond!r} p={p!r} K={K}", flush=True)
+    net, pos = _setup(prec, n_src, n_tgt)
+    tgt = 0 if n_src == n_tgt else 1
+    c, cconst = cg.connect_expression(cond, {}, NAMES, NAMES) if cond else ("", None)
+    if isinstance(p, str):
+        pe, pconst = cg.connect_expression(p, {}, NAMES, NAMES)
+        assert pconst is None
+        pnum = 1.0
+    else:
+        pe, pnum = "", p
+    seed, stream = 777 + case, 5
+    pid = net.connect_code(0, tgt, 1, c, pe, pnum, K, pre, post, autapses, seed, stream, delay, wmode, w)
+    ref = oracle_lib.connect_code(pre or (0, n_src), post or (0, n_tgt), c, pe, pnum, K,
+                                  (not autapses) and tgt == 0, seed, stream, delay, DT, w, [pos[0], 0 * pos[0],
+                                                                                            0 * pos[0]],
+                                  [pos[tgt], 0 * pos[tgt], 0 * pos[tgt]], prec)
+    ref = _bundle_order(ref)
+    info = net.pathway_info(pid)
+    assert info["nsyn"] == ref["pre"].size
+    if case != len(CASES) - 1:
+        assert ref["pre"].size > 500
+    if info["nsyn"]:
+        gp, gq, gd, gw = net.synapses(pid)
+        np.testing.assert_array_equal(gp, ref["pre"])
+        np.testing.assert_array_equal(gq, ref["post"])
+        np.testing.assert_array_equal(gd, ref["delay_steps"])
+        rw = ref["w"] if wmode == rt.HS_W_SYNAPSE else np.full(ref["pre"].size, w[0])
+        if prec == 32 and wmode == rt.HS_W_SYNAPSE:
+            rw = rw.astype(np.float32).astype(np.float64)
+        np.testing.assert_array_equal(gw, rw)
+    net.close()
+
+
+@pytest.mark.parametrize("engine,sem", [("plain", 0), ("fused", 1)])
+def test_condition_connected_cuba_simulates_to_oracle_bits(hs, oracle_lib, engine, sem):
+    """C1's network with its two pathways connected by conditions compiled on
+    the device (about 80 excitatory and 70 inhibitory targets per source, one
+    step of delay), simulated in both engine forms: bits equal the oracle's."""
+    spec = nw.cuba(N=4000, seed=9, monitors=True)
+    ne = 3200
+    ce, _ = cg.connect_expression("(i + 3 * j) % 50 == 0")
+    ci, _ = cg.connect_expression("i != j and abs(i - j) % 41 == 0")
+    we, wi = spec.paths[0].wscalar, spec.paths[1].wscalar
+    spec.paths = [
+        nw.PathSpec(0, 0, 1, None, None, random=nw.RandomConnect(1.0, (0, ne), (0, 4000), 31, 0, True, (1e-4, 1e-4),
+                                                                 rt.HS_W_SCALAR, (we, we), cond=ce)),
+        nw.PathSpec(0, 0, 2, None, None, random=nw.RandomConnect(0.7, (ne, 4000), (0, 4000), 32, 1, True,
+                                                                 (1e-4, 1e-4), rt.HS_W_SCALAR, (wi, wi), cond=ci)),
+    ]
+    net, gsm, _ = nw.build_hs(spec, 64, engine)
+    ref, rsm, _ = oracle_lib.build_ref(spec, 64, sem)
+    for k in (100, 400):
+        net.run(k)
+        ref.run(k)
+    gi, gt = net.spikes(gsm[0])
+    ri, rt_ = ref.spikes(rsm[0])
+    assert len(ri) > 100 and np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    assert np.array_equal(net.get_state(0, 0), ref.get_state(0, 0))
+    assert net.stats()["events"] == ref.events
+    net.close()
+
+
+def test_connect_code_refusals(hs):
+    net, _ = _setup(64, 100, 100)
+    with pytest.raises(HsError, match="variables"):  # a variable the group does not have
+        net.connect_code(0, 0, 1, "hs_pre(3) > 0.5", "", 1.0)
+    with pytest.raises(HsError, match="expressions only"):
+        net.connect_code(0, 0, 1, "i > 0; int x = 1", "", 1.0)
+    with pytest.raises(HsError, match="do not compile"):
+        net.connect_code(0, 0, 1, "i > undefined_name", "", 1.0)
+    with pytest.raises(HsError, match="fixed out-degree"):
+        net.connect_code(0, 0, 1, "", "0.5", 1.0, K=5)
+    # the network still works after the refusals
+    pid = net.connect_code(0, 0, 1, "i == j", "", 1.0)
+    assert net.pathway_info(pid)["nsyn"] == 100
+    net.close()
