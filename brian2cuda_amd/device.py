@@ -72,7 +72,8 @@ class SynapsesDesc:
     namespace: Dict[str, float] = field(default_factory=dict)
     i: object = None                      # connect(i=..., j=...) (within the (sub)groups)
     j: object = None
-    p: Optional[float] = None             # connect(p=...), optionally with a condition
+    p: object = None                      # connect(p=...): a number or an expression (str) of i, j,
+                                          # x_pre, x_post; optionally with a condition
     k: Optional[int] = None               # connect(j='k for k in sample(N_post, size=K)'), optionally
                                           # with condition 'i != j' (its 'if k != i') and i/j ranges
     condition: Optional[str] = None
@@ -357,10 +358,33 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
             var, neg = 0, False
             wkind, wval = ("synapse", "w") if "w" in syn_vars and ("w" in s.values or s.weight_range) \
                 else ("scalar", 0.0)
-        if s.p is not None or s.k is not None:
+        every_pair = s.p is None and s.k is None and s.i is None and s.j is None  # connect(condition=...), connect()
+        if s.p is not None or s.k is not None or every_pair:
             if s.p is not None and s.k is not None:
                 raise ValueError(f"Synapses {s.name!r}: connect() takes p or a sample size K, not both")
-            pre, post, autapses = _condition_ranges(s.condition, sr[1] - sr[0], tr[1] - tr[0], s)
+            what = f"Synapses {s.name!r}"
+            cond_c, p_c, pnum = "", "", 1.0 if every_pair else s.p
+            try:  # index ranges and i != j: the built-in generator
+                pre, post, autapses = _condition_ranges(s.condition, sr[1] - sr[0], tr[1] - tr[0], s)
+                if not autapses and (src != tgt or sr[0] != tr[0]):
+                    # the generator's flag drops equal neuron indices of one group; Brian2's i
+                    # and j count within the Synapses' source and target, so anything else is code
+                    raise NotImplementedError("i != j across groups or subgroup offsets")
+            except NotImplementedError:  # any other condition: compiled into the generator
+                from .codegen import connect_expression
+                pre, post, autapses = [0, sr[1] - sr[0]], [0, tr[1] - tr[0]], True
+                cond_c, const = connect_expression(s.condition, s.namespace, gvars[src], gvars[tgt], what)
+                if const is not None:
+                    cond_c = ""
+                    if not const:
+                        pnum = 0.0  # a condition that never holds
+            if isinstance(pnum, str):
+                from .codegen import connect_expression
+                p_c, const = connect_expression(pnum, s.namespace, gvars[src], gvars[tgt], what)
+                if const is not None:
+                    p_c, pnum = "", min(1.0, max(0.0, float(const)))
+                else:
+                    pnum = 1.0
             pre = (sr[0] + pre[0], sr[0] + pre[1])
             post = (tr[0] + post[0], tr[0] + post[1])
             delay = s.delay if isinstance(s.delay, tuple) else (float(s.delay), float(s.delay))
@@ -370,13 +394,14 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
                                               "constant or 'lo + (hi-lo)*rand()'")
                 lo, hi = s.weight_range
                 w = (-hi, -lo) if neg else (lo, hi)
-                rc = nw.RandomConnect(float(s.p or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
-                                      autapses=autapses, delay=delay, wmode=HS_W_SYNAPSE, w=w, k=int(s.k or 0))
+                rc = nw.RandomConnect(float(pnum or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
+                                      autapses=autapses, delay=delay, wmode=HS_W_SYNAPSE, w=w, k=int(s.k or 0),
+                                      cond=cond_c, p_expr=p_c)
                 paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SYNAPSE, random=rc, code=code))
             else:
-                rc = nw.RandomConnect(float(s.p or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
+                rc = nw.RandomConnect(float(pnum or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
                                       autapses=autapses, delay=delay, wmode=HS_W_SCALAR, w=(wval, wval),
-                                      k=int(s.k or 0))
+                                      k=int(s.k or 0), cond=cond_c, p_expr=p_c)
                 paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SCALAR, wscalar=wval, random=rc,
                                          code=code))
             if code is not None:

@@ -225,6 +225,29 @@ dgi/dt = -gi/taui : volt
 """
 
 
+def cuba_conditional(N=4000, seed=9, dt=0.1 * ms, we=1.62 * mV, wi=-9 * mV, monitors=True):
+    """C1's neurons with connectivity given as code, generated where the network
+    is built: excitatory sources (the first 80%) connect with the probability
+    expression p = '0.5*exp(-abs(i*N_post/N_pre - j)/50)' (about 50 targets
+    around each source's place), inhibitory ones where the condition
+    'i != j and abs(i - j) % 41 == 0' holds, with p = 0.7 (about 68 targets);
+    one step of delay."""
+    from .codegen import connect_expression
+    spec = cuba(N=N, seed=seed, dt=dt, we=we, wi=wi, monitors=monitors)
+    ne = int(0.8 * N)
+    pe, _ = connect_expression("0.5*exp(-abs(i*N_post/N_pre - j)/50)")
+    ci, _ = connect_expression("i != j and abs(i - j) % 41 == 0")
+    d = (dt, dt)
+    spec.paths = [
+        PathSpec(0, 0, 1, None, None, wmode=HS_W_SCALAR, wscalar=we,
+                 random=RandomConnect(1.0, (0, ne), (0, N), seed + 1, 0, True, d, HS_W_SCALAR, (we, we), p_expr=pe)),
+        PathSpec(0, 0, 2, None, None, wmode=HS_W_SCALAR, wscalar=wi,
+                 random=RandomConnect(0.7, (ne, N), (0, N), seed + 2, 1, True, d, HS_W_SCALAR, (wi, wi), cond=ci)),
+    ]
+    spec.name = "cuba_conditional"
+    return spec
+
+
 def code_cuba(N=4000, seed=1234, p=0.02, delay_steps=1, dt=0.1 * ms, method=None, we=1.62 * mV, wi=-9 * mV,
               monitors=True):
     """C1 with the model given as code: examples/cuba.py's equations, threshold

@@ -207,12 +207,19 @@ class ProjectWriter:
             if p.random is not None:
                 r = p.random
                 k = int(getattr(r, "k", 0) or 0)
-                fn, arg = ("hs_net_connect_fixed", str(k)) if k else ("hs_net_connect_random", _c_double(r.p))
+                if getattr(r, "cond", "") or getattr(r, "p_expr", ""):
+                    # condition / probability as code: compiled for the device by this call,
+                    # after the groups' state (command-line values included) is set
+                    fn = "hs_net_connect_code"
+                    arg = f"{_c_str(r.cond)}, {_c_str(r.p_expr)}, {_c_double(r.p)}, {k}"
+                    pinfo["connect"] = "code"
+                else:
+                    fn, arg = ("hs_net_connect_fixed", str(k)) if k else ("hs_net_connect_random", _c_double(r.p))
+                    pinfo["connect"] = "fixed" if k else "random"
                 L.append(f"    HS_CHECK({fn}(net, g_{src}, g_{tgt}, {p.var}, {r.pre[0]}, {r.pre[1]}, "
                          f"{r.post[0]}, {r.post[1]}, {arg}, {int(bool(r.autapses))}, {int(r.seed)}ULL, "
                          f"{int(r.stream)}u, {_c_double(r.delay[0])}, {_c_double(r.delay[1])}, {int(r.wmode)}, "
                          f"{_c_double(r.w[0])}, {_c_double(r.w[1])}, &p_{pname}));")
-                pinfo["connect"] = "fixed" if k else "random"
             else:
                 rp, n = self._static(directory, f"_dynamic_array_{pname}__synaptic_pre", p.pre, np.int32)
                 rq, _ = self._static(directory, f"_dynamic_array_{pname}__synaptic_post", p.post, np.int32)

@@ -170,7 +170,8 @@ def test_unsupported_groups_are_refused(change, msg):
 @pytest.mark.parametrize("on_pre,kw,msg", [
     ("w += 1", {}, "not a synapse or target variable"),
     ("w += v_post", dict(model="w : 1"), "reads the target variable"),
-    ("ge += we", dict(p=0.1, condition="i % 2 == 0"), "condition"),
+    ("ge += we", dict(p=0.1, condition="rand() < 0.5"), "rand"),
+    ("ge += we", dict(p="0.1 * unknown_name"), "unknown_name"),
     ("ge += w", dict(p=0.1, values={"w": 1.0}), "'lo \\+ \\(hi-lo\\)\\*rand\\(\\)'"),
 ])
 def test_unsupported_synapses_are_refused(on_pre, kw, msg):
@@ -178,6 +179,30 @@ def test_unsupported_synapses_are_refused(on_pre, kw, msg):
     s = dv.SynapsesDesc("s", "g", "g", on_pre, namespace=CUBA_NS, **kw)
     with pytest.raises(NotImplementedError, match=msg):
         dv.network_spec([g], [s], [], dt=0.1 * ms)
+
+
+def test_connect_conditions_as_code():
+    """connect(condition=..., p=...) beyond index ranges and i != j: the
+    condition and a probability that reads indices or variables become the C
+    expressions of the device generator (codegen.connect_expression), i and j
+    within the Synapses' subgroups; constant probabilities fold to numbers;
+    connect(condition=...) without p connects every pair where it holds."""
+    g = dv.GroupDesc("g", 100, equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr", namespace=CUBA_NS)
+    syns = [dv.SynapsesDesc("s0", "g", "g", "ge += we", namespace=CUBA_NS, p=0.1, condition="abs(i - j) < 5"),
+            dv.SynapsesDesc("s1", "g", "g", "gi += wi", namespace=dict(CUBA_NS, sigma=20.0),
+                            source_range=(80, 100), target_range=(10, 60),
+                            p="exp(-(i - j)**2 / (2 * sigma**2))", condition="i != j"),
+            dv.SynapsesDesc("s2", "g", "g", "ge += we", namespace=CUBA_NS, condition="v_pre > v_post"),
+            dv.SynapsesDesc("s3", "g", "g", "ge += we", namespace=CUBA_NS, p="2 * 0.05", condition="i < 50"),
+            dv.SynapsesDesc("s4", "g", "g", "ge += we", namespace=CUBA_NS, k=10, condition="j % 3 == 0")]
+    spec, _ = dv.network_spec([g], syns, [], dt=0.1 * ms)
+    r = [p.random for p in spec.paths]
+    assert r[0].cond == "((abs((i - j)) < 5LL))" and r[0].p == 0.1 and r[0].p_expr == ""
+    assert r[1].pre == (80, 100) and r[1].post == (10, 60) and r[1].cond == "((i != j))" and r[1].autapses
+    assert r[1].p_expr.startswith("exp(") and "(800.0)" in r[1].p_expr and r[1].p == 1.0
+    assert r[2].cond == "((hs_pre(0) > hs_post(0)))" and r[2].p == 1.0
+    assert r[3].cond == "" and r[3].pre == (0, 50) and r[3].p == pytest.approx(0.1)  # ranges stay ranges
+    assert r[4].k == 10 and r[4].cond == "((hs_cg_pymod(j, 3LL) == 0LL))"
 
 
 def test_state_monitor_indices_and_dt():

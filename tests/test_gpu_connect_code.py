@@ -89,24 +89,18 @@ def test_connect_code_matches_oracle(hs, oracle_lib, prec, case):
     net.close()
 
 
+@pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("engine,sem", [("plain", 0), ("fused", 1)])
-def test_condition_connected_cuba_simulates_to_oracle_bits(hs, oracle_lib, engine, sem):
-    """C1's network with its two pathways connected by conditions compiled on
-    the device (about 80 excitatory and 70 inhibitory targets per source, one
-    step of delay), simulated in both engine forms: bits equal the oracle's."""
-    spec = nw.cuba(N=4000, seed=9, monitors=True)
-    ne = 3200
-    ce, _ = cg.connect_expression("(i + 3 * j) % 50 == 0")
-    ci, _ = cg.connect_expression("i != j and abs(i - j) % 41 == 0")
-    we, wi = spec.paths[0].wscalar, spec.paths[1].wscalar
-    spec.paths = [
-        nw.PathSpec(0, 0, 1, None, None, random=nw.RandomConnect(1.0, (0, ne), (0, 4000), 31, 0, True, (1e-4, 1e-4),
-                                                                 rt.HS_W_SCALAR, (we, we), cond=ce)),
-        nw.PathSpec(0, 0, 2, None, None, random=nw.RandomConnect(0.7, (ne, 4000), (0, 4000), 32, 1, True,
-                                                                 (1e-4, 1e-4), rt.HS_W_SCALAR, (wi, wi), cond=ci)),
-    ]
-    net, gsm, _ = nw.build_hs(spec, 64, engine)
-    ref, rsm, _ = oracle_lib.build_ref(spec, 64, sem)
+def test_conditional_cuba_simulates_to_oracle_bits(hs, oracle_lib, prec, engine, sem):
+    """C1's neurons connected by a probability expression and by a condition,
+    both compiled for the device (networks.cuba_conditional), simulated in both
+    engine forms and precisions: bits equal the oracle's."""
+    spec = nw.cuba_conditional(N=4000, seed=9)
+    print(f"cuba_conditional prec {prec} engine {engine}", flush=True)
+    net, gsm, _ = nw.build_hs(spec, prec, engine)
+    ref, rsm, _ = oracle_lib.build_ref(spec, prec, sem)
+    for p in range(2):
+        assert net.pathway_info(p)["nsyn"] == oracle_lib.random_pathway(spec, spec.paths[p], precision=prec)["pre"].size
     for k in (100, 400):
         net.run(k)
         ref.run(k)

@@ -97,6 +97,23 @@ def test_fixed_outdegree_project(hs, oracle_lib, tmp_path):
     np.testing.assert_array_equal(res["neurongroup.V"], ref.get_state(0, 0).astype(np.float32).astype(np.float64))
 
 
+def test_conditional_connect_project(hs, oracle_lib, tmp_path):
+    """connect with a probability expression and a condition through the
+    project (hs_net_connect_code in main.cpp: the binary compiles them for the
+    device when it runs): spikes and state equal the oracle's."""
+    spec = nw.cuba_conditional(N=4000, seed=13)
+    project.write_project(spec, tmp_path, 500, precision=64, engine="plain")
+    project.build_project(tmp_path)
+    project.run_project(tmp_path, "results", timeout=120)
+    res = project.read_results(tmp_path)
+    ref, rsm, _ = _oracle(oracle_lib, spec, 64, oracle_lib.SEQ, 500)
+    ri, rstep = ref.spikes(rsm[0])
+    assert len(ri) > 100
+    np.testing.assert_array_equal(res["spikemonitor"]["i"], ri)
+    np.testing.assert_array_equal(res["spikemonitor"]["t"], rstep * spec.dt)
+    np.testing.assert_array_equal(res["neurongroup.v"], ref.get_state(0, 0))
+
+
 def test_described_cuba_example_through_the_device_layer(hs, oracle_lib, tmp_path):
     """examples/cuba.py described object by object (equation text, threshold,
     reset, namespace, connect('i<Ne', p=0.02), on_pre 'ge += we') goes through

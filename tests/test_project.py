@@ -82,15 +82,19 @@ def test_every_model_and_pathway_kind_compiles(tmp_path):
     specs = [nw.cuba(N=100, seed=1, record=5), nw.cobahh(N=100, seed=1), nw.brunel(N=200, K=20, seed=1),
              nw.batch_cuba(n_nets=3, seed=1, nmin=50, nmax=80), nw.stdp_poisson_example(N=200, K=20),
              nw.brunel(N=200, K=20, seed=1, device_connect=True, fixed_outdegree=True),
-             nw.code_synapses(npo=50, nl=10), nw.code_mix(N=300, seed=1), nw.stdp_feedforward(2, 20, 10, code=True)]
+             nw.code_synapses(npo=50, nl=10), nw.code_mix(N=300, seed=1), nw.stdp_feedforward(2, 20, 10, code=True),
+             nw.cuba_conditional(N=200, seed=1)]
     specs[0].rate_monitors = [(0, 10, 60)]
     for k, spec in enumerate(specs):
         d = tmp_path / f"p{k}"
         project.write_project(spec, d, 5)
         assert os.path.exists(project.build_project(d))
     assert "hs_net_connect_fixed(net, g_neurongroup, g_neurongroup, 0, 0, 200, 0, 200, 20," in \
-        (tmp_path / f"p{len(specs) - 4}" / "main.cpp").read_text()
-    assert "hs_net_pathway_code(net, p_synapses, 2, sn," in (tmp_path / f"p{len(specs) - 3}" / "main.cpp").read_text()
-    main = (tmp_path / f"p{len(specs) - 2}" / "main.cpp").read_text()
+        (tmp_path / f"p{len(specs) - 5}" / "main.cpp").read_text()
+    assert "hs_net_pathway_code(net, p_synapses, 2, sn," in (tmp_path / f"p{len(specs) - 4}" / "main.cpp").read_text()
+    main = (tmp_path / f"p{len(specs) - 3}" / "main.cpp").read_text()
     assert main.count("hs_net_add_group_code(") == 3 and '"gr"' in main
-    assert "hs_net_pathway_code_post(net, p_synapses," in (tmp_path / f"p{len(specs) - 1}" / "main.cpp").read_text()
+    assert "hs_net_pathway_code_post(net, p_synapses," in (tmp_path / f"p{len(specs) - 2}" / "main.cpp").read_text()
+    main = (tmp_path / f"p{len(specs) - 1}" / "main.cpp").read_text()
+    assert main.count("hs_net_connect_code(") == 2
+    assert '"((bool)(((i != j))) && (bool)(((hs_cg_pymod(abs((i - j)), 41LL) == 0LL))))", "", 0.7, 0,' in main
