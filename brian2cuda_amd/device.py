@@ -27,6 +27,7 @@ Two layers:
   and tested: this glue is written against Brian2's documented object
   attributes and is not exercised against a real Brian2.
 """
+import ast
 import re
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
@@ -293,6 +294,17 @@ def _on_pre(code, tgt_vars, syn):
     return var, "scalar", _const_expr(body, syn.namespace, f"Synapses {syn.name!r}") * (-1.0 if neg else 1.0), False
 
 
+def _outer_parens(s):
+    """s is one parenthesised expression: its first '(' closes at its end."""
+    depth = 0
+    for k, ch in enumerate(s):
+        depth += ch == "("
+        depth -= ch == ")"
+        if depth == 0:
+            return k == len(s) - 1
+    return False
+
+
 def _condition_ranges(cond, n_src, n_tgt, syn):
     """connect(condition, p=...) conditions the device generator takes:
     'i<X', 'i>=X', 'i!=j' (and their conjunctions with 'and')."""
@@ -301,6 +313,8 @@ def _condition_ranges(cond, n_src, n_tgt, syn):
         return pre, post, autapses
     for part in re.split(r"\s+and\s+", cond.strip()):
         s = re.sub(r"\s+", "", part)
+        while s.startswith("(") and s.endswith(")") and _outer_parens(s):
+            s = s[1:-1]
         m = re.match(r"^(i|j)(<|>=|<=|>)(.+)$", s)
         if s in ("i!=j", "j!=i"):
             autapses = False
@@ -319,6 +333,105 @@ def _condition_ranges(cond, n_src, n_tgt, syn):
         else:
             raise NotImplementedError(f"Synapses {syn.name!r}: connect condition {part!r} is not supported")
     return pre, post, autapses
+
+
+def _generator(jtext, syn, n_src, n_tgt):
+    """connect(j='...') in Brian2's generator syntax -> (condition, p, K): the
+    condition is a Brian2 expression over i and j that holds exactly for the
+    generated pairs (None: every pair), p the sampling probability (a number
+    or an expression of i; None: 1) and K (None, or (size, (lo, hi)): a fixed
+    sample size over the target range [lo, hi), whose condition then only
+    filters the sample).
+      'k for k in range(stop | start, stop[, step]) [if cond]'
+            -> start <= j < stop (or stop < j <= start for a negative step),
+               (j - start) % step == 0, and cond with k read as j
+      'k for k in sample(N | start, stop[, step], p=P | size=K) [if cond]'
+            -> the same range condition, and p = P or K
+      '<expression of i>' (no generator, e.g. 'i' one-to-one) -> j == expression
+    Targets outside the target group are skipped, as with skip_if_invalid=True.
+    The range and sample bounds and steps may be expressions of i; the
+    element must be the loop variable."""
+    what = f"Synapses {syn.name!r}: connect(j={jtext!r})"
+    try:
+        tree = ast.parse("(" + jtext.strip() + ")", mode="eval").body  # a bare generator needs parentheses
+    except SyntaxError as e:
+        raise NotImplementedError(f"{what}: cannot parse ({e.msg})")
+    if not isinstance(tree, ast.GeneratorExp):
+        return f"j == ({ast.unparse(tree)})", None, None
+    if len(tree.generators) != 1:
+        raise NotImplementedError(f"{what}: one 'for' clause only")
+    gen = tree.generators[0]
+    if not isinstance(gen.target, ast.Name) or not isinstance(tree.elt, ast.Name) or tree.elt.id != gen.target.id:
+        raise NotImplementedError(f"{what}: the element must be the loop variable ('k for k in ...')")
+    k = gen.target.id
+
+    class _KtoJ(ast.NodeTransformer):  # the loop variable is the target index j (+ off), N_post fixed if given
+        def __init__(self, off=0, n_post=None):
+            self.off, self.n_post = off, n_post
+
+        def visit_Name(self, n):
+            if n.id == k:
+                e = ast.Name("j", ast.Load())
+                if self.off:
+                    e = ast.BinOp(e, ast.Add(), ast.Constant(self.off))
+                return ast.copy_location(e, n)
+            if n.id == "N_post" and self.n_post is not None:
+                return ast.copy_location(ast.Constant(self.n_post), n)
+            return n
+
+    call = gen.iter
+    if not (isinstance(call, ast.Call) and isinstance(call.func, ast.Name) and call.func.id in ("range", "sample")):
+        raise NotImplementedError(f"{what}: the iterator must be range(...) or sample(...)")
+    args = [ast.unparse(a) for a in call.args]
+    kw = {x.arg: x.value for x in call.keywords}
+    p, K = None, None
+    if call.func.id == "sample":
+        if set(kw) - {"p", "size"} or len(kw) != 1:
+            raise NotImplementedError(f"{what}: sample() takes p= or size=")
+        if "p" in kw:
+            p = ast.unparse(kw["p"])
+            try:
+                p = float(ast.literal_eval(kw["p"]))
+            except (ValueError, TypeError, SyntaxError):
+                pass
+        else:
+            try:
+                K = int(ast.literal_eval(kw["size"]))
+            except (ValueError, TypeError, SyntaxError):
+                raise NotImplementedError(f"{what}: sample(size=...) takes a constant size")
+    if not 1 <= len(args) <= 3:
+        raise NotImplementedError(f"{what}: {call.func.id}() takes 1 to 3 positional arguments")
+    start, stop = ("0", args[0]) if len(args) == 1 else (args[0], args[1])
+    step = args[2] if len(args) == 3 else "1"
+    try:
+        step_v = int(ast.literal_eval(step))
+    except (ValueError, TypeError, SyntaxError):
+        raise NotImplementedError(f"{what}: a constant step")
+    if step_v == 0:
+        raise ValueError(f"{what}: step 0")
+    if K is not None:
+        # K distinct targets of the range itself: the range narrows the target
+        # range (constant bounds, step 1), whose j then counts from lo; the if
+        # clauses filter the sample, in the Synapses' own j and N_post
+        if step_v != 1:
+            raise NotImplementedError(f"{what}: sample(size=...) over a range with a step")
+        ns = dict(syn.namespace, N_pre=n_src, N_post=n_tgt)
+        lo, hi = (_const_expr(x, ns, what) for x in (start, stop))
+        if lo != int(lo) or hi != int(hi):
+            raise NotImplementedError(f"{what}: sample(size=...) takes integer bounds")
+        lo, hi = int(lo), int(hi)
+        ifs = ["(" + ast.unparse(_KtoJ(lo, n_tgt).visit(cond)) + ")" for cond in gen.ifs]
+        return (" and ".join(ifs) or None), None, (int(K), (lo, hi))
+    ifs = ["(" + ast.unparse(_KtoJ().visit(cond)) + ")" for cond in gen.ifs]
+    if call.func.id == "sample" and len(args) == 1 and stop == "N_post":
+        parts = []  # the whole target range
+    elif step_v > 0:
+        parts = [f"j >= ({start})", f"j < ({stop})"]
+    else:
+        parts = [f"j <= ({start})", f"j > ({stop})"]
+    if abs(step_v) != 1:
+        parts.append(f"(j - ({start})) % {abs(step_v)} == 0")
+    return (" and ".join(parts + ifs) or None), p, None
 
 
 def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors: List[MonitorDesc], dt, seed=0,
@@ -358,14 +471,28 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
             var, neg = 0, False
             wkind, wval = ("synapse", "w") if "w" in syn_vars and ("w" in s.values or s.weight_range) \
                 else ("scalar", 0.0)
-        every_pair = s.p is None and s.k is None and s.i is None and s.j is None  # connect(condition=...), connect()
-        if s.p is not None or s.k is not None or every_pair:
-            if s.p is not None and s.k is not None:
+        condition, p_val, k_val = s.condition, s.p, s.k
+        if isinstance(s.j, str):  # generator syntax: a condition, p or K over the target range
+            if s.i is not None or s.p is not None or s.k is not None:
+                raise ValueError(f"Synapses {s.name!r}: connect(j='...') takes no i, p or K besides")
+            gcond, p_val, k_val = _generator(s.j, s, sr[1] - sr[0], tr[1] - tr[0])
+            if k_val is not None:
+                k_val, (lo, hi) = k_val
+                if not 0 <= lo <= hi <= tr[1] - tr[0]:
+                    raise ValueError(f"Synapses {s.name!r}: sample range [{lo}, {hi}) outside the target")
+                tr = (tr[0] + lo, tr[0] + hi)
+            if gcond:
+                condition = f"({condition}) and ({gcond})" if condition else gcond
+            if p_val is None and k_val is None:
+                p_val = 1.0
+        every_pair = p_val is None and k_val is None and s.i is None and s.j is None  # connect(condition=...)
+        if p_val is not None or k_val is not None or every_pair:
+            if p_val is not None and k_val is not None:
                 raise ValueError(f"Synapses {s.name!r}: connect() takes p or a sample size K, not both")
             what = f"Synapses {s.name!r}"
-            cond_c, p_c, pnum = "", "", 1.0 if every_pair else s.p
+            cond_c, p_c, pnum = "", "", 1.0 if every_pair else p_val
             try:  # index ranges and i != j: the built-in generator
-                pre, post, autapses = _condition_ranges(s.condition, sr[1] - sr[0], tr[1] - tr[0], s)
+                pre, post, autapses = _condition_ranges(condition, sr[1] - sr[0], tr[1] - tr[0], s)
                 if not autapses and (src != tgt or sr[0] != tr[0]):
                     # the generator's flag drops equal neuron indices of one group; Brian2's i
                     # and j count within the Synapses' source and target, so anything else is code
@@ -373,7 +500,7 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
             except NotImplementedError:  # any other condition: compiled into the generator
                 from .codegen import connect_expression
                 pre, post, autapses = [0, sr[1] - sr[0]], [0, tr[1] - tr[0]], True
-                cond_c, const = connect_expression(s.condition, s.namespace, gvars[src], gvars[tgt], what)
+                cond_c, const = connect_expression(condition, s.namespace, gvars[src], gvars[tgt], what)
                 if const is not None:
                     cond_c = ""
                     if not const:
@@ -395,13 +522,13 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
                 lo, hi = s.weight_range
                 w = (-hi, -lo) if neg else (lo, hi)
                 rc = nw.RandomConnect(float(pnum or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
-                                      autapses=autapses, delay=delay, wmode=HS_W_SYNAPSE, w=w, k=int(s.k or 0),
+                                      autapses=autapses, delay=delay, wmode=HS_W_SYNAPSE, w=w, k=int(k_val or 0),
                                       cond=cond_c, p_expr=p_c)
                 paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SYNAPSE, random=rc, code=code))
             else:
                 rc = nw.RandomConnect(float(pnum or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
                                       autapses=autapses, delay=delay, wmode=HS_W_SCALAR, w=(wval, wval),
-                                      k=int(s.k or 0), cond=cond_c, p_expr=p_c)
+                                      k=int(k_val or 0), cond=cond_c, p_expr=p_c)
                 paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SCALAR, wscalar=wval, random=rc,
                                          code=code))
             if code is not None:

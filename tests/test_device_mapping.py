@@ -205,6 +205,40 @@ def test_connect_conditions_as_code():
     assert r[4].k == 10 and r[4].cond == "((hs_cg_pymod(j, 3LL) == 0LL))"
 
 
+def test_generator_syntax():
+    """connect(j='...') in Brian2's generator syntax: one-to-one and other
+    expressions of i, range() with bounds of i and a step, sample(p=...) with
+    if clauses, sample(size=K) over a sub-range (the loop variable counts in the
+    Synapses' own j); the synapses the oracle generates for each are the pairs
+    Python enumerates from the same generator."""
+    import oracle
+    n = 60
+    g = dv.GroupDesc("g", n, equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr", namespace=CUBA_NS)
+    forms = ["i", "(i + 7) % N_post", "k for k in range(i - 3, i + 4) if k != i",
+             "k for k in range(N_post - 1, 0, -3) if k > i", "k for k in sample(N_post, p=1.0) if k % 4 == i % 4",
+             "k for k in sample(10, 50, size=8) if k != i"]
+    syns = [dv.SynapsesDesc(f"s{q}", "g", "g", "ge += we", namespace=CUBA_NS, j=f, seed=3) for q, f in enumerate(forms)]
+    spec, _ = dv.network_spec([g], syns, [], dt=0.1 * ms)
+    got = [set(zip(*(lambda s: (s["pre"].tolist(), s["post"].tolist()))(oracle.random_pathway(spec, p))))
+           for p in spec.paths]
+    assert got[0] == {(i, i) for i in range(n)}
+    assert got[1] == {(i, (i + 7) % n) for i in range(n)}
+    assert got[2] == {(i, k) for i in range(n) for k in range(i - 3, i + 4) if k != i and 0 <= k < n}
+    assert got[3] == {(i, k) for i in range(n) for k in range(n - 1, 0, -3) if k > i}
+    assert got[4] == {(i, k) for i in range(n) for k in range(n) if k % 4 == i % 4}
+    r5 = spec.paths[5].random
+    assert r5.k == 8 and r5.post == (10, 50) and r5.cond
+    rows = {}
+    for i, j in got[5]:
+        rows.setdefault(i, []).append(j)
+    assert set(rows) == set(range(n)) and all(10 <= j < 50 and j != i for i, js in rows.items() for j in js)
+    # 8 sampled per row; the if clause drops j == i, which only a source inside the range can lose
+    assert all(len(js) == 8 or (len(js) == 7 and 10 <= i < 50) for i, js in rows.items())
+    with pytest.raises(NotImplementedError, match="element"):
+        dv.network_spec([g], [dv.SynapsesDesc("s", "g", "g", "ge += we", namespace=CUBA_NS,
+                                              j="k + 1 for k in range(3)")], [], dt=0.1 * ms)
+
+
 def test_state_monitor_indices_and_dt():
     """StateMonitor(S, 'v', record=[...], dt=...) as the reference's consistency
     tests use it (test_cpp_cuda_consistency.py:72): arbitrary indices and a

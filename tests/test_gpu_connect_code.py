@@ -126,3 +126,44 @@ def test_connect_code_refusals(hs):
     pid = net.connect_code(0, 0, 1, "i == j", "", 1.0)
     assert net.pathway_info(pid)["nsyn"] == 100
     net.close()
+
+
+CUBA_EQS = """
+dv/dt  = (ge+gi-(v-El))/taum : volt (unless refractory)
+dge/dt = -ge/taue : volt
+dgi/dt = -gi/taui : volt
+"""
+CUBA_NS = {"taum": 20e-3, "taue": 5e-3, "taui": 10e-3, "Vt": -50e-3, "Vr": -60e-3, "El": -49e-3, "we": 1.62e-3,
+           "wi": -9e-3}
+
+
+@pytest.mark.parametrize("prec,engine,sem", [(64, "plain", 0), (32, "fused", 1)])
+def test_generator_syntax_network(hs, oracle_lib, prec, engine, sem):
+    """A CUBA network described with Brian2's generator syntax (device layer):
+    excitatory 'k for k in range(i-60, i+61) if k != i and (k + i) % 3 == 0'
+    and inhibitory sources from a subgroup 'k for k in sample(N_post, size=40)
+    if k % 2 == 1', both generated on the device; spikes and state equal the
+    oracle's."""
+    from brian2cuda_amd import device as dv
+    n = 4000
+    v0 = -0.060 + 0.010 * np.random.default_rng(4).random(n)
+    g = dv.GroupDesc("g", n, equations=CUBA_EQS, threshold="v>Vt", reset="v = Vr", refractory=5e-3,
+                     namespace=CUBA_NS, init={"v": v0})
+    syns = [dv.SynapsesDesc("se", "g", "g", "ge += we", namespace=CUBA_NS, source_range=(0, 3200),
+                            j="k for k in range(i - 60, i + 61) if k != i and (k + i) % 3 == 0", delay=1e-4, seed=5),
+            dv.SynapsesDesc("si", "g", "g", "gi += wi", namespace=CUBA_NS, source_range=(3200, n),
+                            j="k for k in sample(N_post, size=40) if k % 2 == 1", delay=1e-4, seed=6)]
+    mons = [dv.MonitorDesc("spikes", "spike", "g")]
+    spec, _ = dv.network_spec([g], syns, mons, dt=1e-4, seed=8)
+    print(f"generator-syntax network prec {prec} engine {engine}", flush=True)
+    net, gsm, _ = nw.build_hs(spec, prec, engine)
+    ref, rsm, _ = oracle_lib.build_ref(spec, prec, sem)
+    for k in (200, 300):
+        net.run(k)
+        ref.run(k)
+    gi, gt = net.spikes(gsm[0])
+    ri, rt_ = ref.spikes(rsm[0])
+    assert len(ri) > 100 and np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    assert np.array_equal(net.get_state(0, 0), ref.get_state(0, 0))
+    assert net.stats()["events"] == ref.events
+    net.close()
