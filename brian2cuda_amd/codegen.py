@@ -22,7 +22,10 @@ Equations are Brian2's text form: 'dx/dt = expr : unit [(flags)]',
 'name : unit [(constant)]' (per-neuron parameters, state variables the update
 leaves alone).  Names resolve to the group's variables, its subexpressions,
 the namespace (SI values), Brian2's units, and t, dt, xi, i.  Functions: exp,
-log, log10, sqrt, abs, clip, floor, ceil, sign.  '(unless refractory)' holds
+log, log10, sqrt, abs, clip, floor, ceil, sign, sin, cos, tan, sinh, cosh,
+tanh, arcsin, arccos, arctan, expm1, log1p, exprel (evaluated by the library's
+deterministic implementations on the device and in the oracle alike).
+'(unless refractory)' holds
 a variable while the neuron is refractory.  Anything else raises
 NotImplementedError naming it.
 
@@ -82,8 +85,26 @@ class log10(sp.Function):  # noqa: N801
     nargs = 1
 
 
+class expm1(sp.Function):  # noqa: N801 -- Brian2's expm1, log1p, exprel, printed by name
+    nargs = 1
+
+
+class log1p(sp.Function):  # noqa: N801
+    nargs = 1
+
+
+class exprel(sp.Function):  # noqa: N801
+    nargs = 1
+
+
 _FUNCS = {"exp": sp.exp, "log": sp.log, "sqrt": sp.sqrt, "abs": sp.Abs, "floor": sp.floor, "ceil": sp.ceiling,
-          "sign": sp.sign, "clip": clip, "log10": log10}
+          "sign": sp.sign, "clip": clip, "log10": log10, "sin": sp.sin, "cos": sp.cos, "tan": sp.tan,
+          "sinh": sp.sinh, "cosh": sp.cosh, "tanh": sp.tanh, "arcsin": sp.asin, "arccos": sp.acos,
+          "arctan": sp.atan, "expm1": expm1, "log1p": log1p, "exprel": exprel}
+# sympy function class -> the name the generated code calls
+_FUNC_NAMES = {sp.sin: "sin", sp.cos: "cos", sp.tan: "tan", sp.sinh: "sinh", sp.cosh: "cosh", sp.tanh: "tanh",
+               sp.asin: "arcsin", sp.acos: "arccos", sp.atan: "arctan", expm1: "expm1", log1p: "log1p",
+               exprel: "exprel"}
 
 
 # ---- parsing ------------------------------------------------------------------------
@@ -260,6 +281,8 @@ class _Printer:
             return f"sign({self.p(e.args[0])})"
         if isinstance(e, clip):
             return "clip(" + ", ".join(self.p(a) for a in e.args) + ")"
+        if type(e) in _FUNC_NAMES:
+            return f"{_FUNC_NAMES[type(e)]}({self.p(e.args[0])})"
         raise NotImplementedError(f"cannot print {e}")
 
     def pow(self, b, x):
@@ -679,7 +702,13 @@ def synapse_code(on_pre, syn_vars, post_vars, pre_vars, namespace=None, what="Sy
 
 # ---- connect conditions and probabilities --------------------------------------------
 _CONN_FUNCS = {"exp": "float", "log": "float", "log10": "float", "sqrt": "float", "abs": None, "floor": "float",
-               "ceil": "float", "sign": None, "clip": "float", "int": "int", "float": "float"}
+               "ceil": "float", "sign": None, "clip": "float", "int": "int", "float": "float", "sin": "float",
+               "cos": "float", "tan": "float", "sinh": "float", "cosh": "float", "tanh": "float", "arcsin": "float",
+               "arccos": "float", "arctan": "float", "expm1": "float", "log1p": "float", "exprel": "float"}
+_CONN_PY = {"exp": np.exp, "log": np.log, "log10": np.log10, "sqrt": np.sqrt, "abs": abs, "floor": np.floor,
+            "ceil": np.ceil, "sign": np.sign, "sin": np.sin, "cos": np.cos, "tan": np.tan, "sinh": np.sinh,
+            "cosh": np.cosh, "tanh": np.tanh, "arcsin": np.arcsin, "arccos": np.arccos, "arctan": np.arctan,
+            "expm1": np.expm1, "log1p": np.log1p, "exprel": lambda x: np.expm1(x) / x if x != 0 else 1.0}
 
 
 def connect_expression(expr, namespace=None, pre_names=(), post_names=(), what="connect"):
@@ -840,9 +869,7 @@ def connect_expression(expr, namespace=None, pre_names=(), post_names=(), what="
                         fold(const, lambda: min(max(const[0], const[1]), const[2])))
             kind = _CONN_FUNCS[fn] or args[0][1]
             arg = args[0][0] if kind == "int" or fn in ("abs", "sign") else f"(double)({args[0][0]})"
-            py = {"exp": np.exp, "log": np.log, "log10": np.log10, "sqrt": np.sqrt, "abs": abs,
-                  "floor": np.floor, "ceil": np.ceil, "sign": np.sign}[fn]
-            return f"{fn}({arg})", kind, fold(const, lambda: py(const[0]))
+            return f"{fn}({arg})", kind, fold(const, lambda: _CONN_PY[fn](const[0]))
         raise NotImplementedError(f"{what}: {ast.unparse(n)!r} is not supported in a connect expression")
 
     text, kind, value = c(tree)

@@ -1,0 +1,36 @@
+/*
+ * hs_cgfuncs.h -- the trigonometric, hyperbolic and related functions that
+ * generated code (code models, synaptic code, connect expressions) may call,
+ * under the names the generated text uses (hs_cg_sin ... hs_cg_exprel, mapped
+ * from Brian2's sin ... exprel by macros after the engine sources).  Each
+ * evaluates hs_detmath.h's double function: the device and the host oracle
+ * compute the same bits, and an FP32 argument gets the double result rounded
+ * once to float.
+ */
+#ifndef HS_CGFUNCS_H
+#define HS_CGFUNCS_H
+
+#include "hs_detmath.h"
+
+#define HS_CG_FUNC1(name, fn)                                                 \
+    HS_HD double name(double x) { return fn(x); }                             \
+    HS_HD float name(float x) { return (float)fn((double)x); }                \
+    HS_HD double name(int x) { return fn((double)x); }                        \
+    HS_HD double name(long long x) { return fn((double)x); }
+
+HS_CG_FUNC1(hs_cg_sin, hs_sin)
+HS_CG_FUNC1(hs_cg_cos, hs_cos)
+HS_CG_FUNC1(hs_cg_tan, hs_tan)
+HS_CG_FUNC1(hs_cg_sinh, hs_sinh)
+HS_CG_FUNC1(hs_cg_cosh, hs_cosh)
+HS_CG_FUNC1(hs_cg_tanh, hs_tanh)
+HS_CG_FUNC1(hs_cg_arcsin, hs_asin)
+HS_CG_FUNC1(hs_cg_arccos, hs_acos)
+HS_CG_FUNC1(hs_cg_arctan, hs_atan)
+HS_CG_FUNC1(hs_cg_expm1, hs_expm1)
+HS_CG_FUNC1(hs_cg_log1p, hs_log1p)
+HS_CG_FUNC1(hs_cg_exprel, hs_exprel)
+
+#undef HS_CG_FUNC1
+
+#endif /* HS_CGFUNCS_H */

@@ -53,7 +53,9 @@ static bool hs_cg_ident(const std::string &s)
 int hs_cg_check(const HsCgModel &m)
 {
     static const char *const reserved[] = {"R", "dt", "t", "i", "xi", "not_refractory", "exp", "log", "log10",
-                                           "sqrt", "abs", "fabs", "clip", "floor", "ceil", "sign", "int"};
+                                           "sqrt", "abs", "fabs", "clip", "floor", "ceil", "sign", "int", "sin",
+                                           "cos", "tan", "sinh", "cosh", "tanh", "arcsin", "arccos", "arctan",
+                                           "expm1", "log1p", "exprel"};
     if (m.names.empty() || (int)m.names.size() > HS_MAX_VARS)
         return hs_rt_fail(HS_ERR_ARG, "a code model has 1 to %d variables", HS_MAX_VARS);
     for (size_t a = 0; a < m.names.size(); ++a) {
@@ -83,7 +85,9 @@ int hs_cg_check(const HsCgModel &m)
 static int hs_cg_check_scope(const std::vector<std::string> &all, const std::string &code)
 {
     static const char *const reserved[] = {"R", "dt", "t", "i", "j", "lastupdate", "exp", "log", "log10", "sqrt",
-                                           "abs", "fabs", "clip", "floor", "ceil", "sign", "int"};
+                                           "abs", "fabs", "clip", "floor", "ceil", "sign", "int", "sin", "cos",
+                                           "tan", "sinh", "cosh", "tanh", "arcsin", "arccos", "arctan", "expm1",
+                                           "log1p", "exprel"};
     for (size_t a = 0; a < all.size(); ++a) {
         const std::string &n = all[a];
         if (!hs_cg_ident(n) || n.compare(0, 3, "hs_") == 0 || n.compare(0, 2, "__") == 0)
@@ -196,6 +200,7 @@ static void hs_cg_struct(std::string &o, const char *R, int c, const HsCgModel &
    log, IEEE sqrt; overloads keep the argument's precision.  Included after
    the engine sources, whose own calls are not renamed by the macros */
 static const char *const hs_cg_math = R"HSCG(
+#include "hs_cgfuncs.h"
 __device__ __forceinline__ double hs_cg_exp(double x) { return hs_exp(x); }
 __device__ __forceinline__ float hs_cg_exp(float x) { return hs_expf(x); }
 __device__ __forceinline__ double hs_cg_exp(int x) { return hs_exp((double)x); }
@@ -238,6 +243,18 @@ __device__ __forceinline__ auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo +
 #define ceil hs_cg_ceil
 #define sign hs_cg_sign
 #define clip hs_cg_clip
+#define sin hs_cg_sin
+#define cos hs_cg_cos
+#define tan hs_cg_tan
+#define sinh hs_cg_sinh
+#define cosh hs_cg_cosh
+#define tanh hs_cg_tanh
+#define arcsin hs_cg_arcsin
+#define arccos hs_cg_arccos
+#define arctan hs_cg_arctan
+#define expm1 hs_cg_expm1
+#define log1p hs_cg_log1p
+#define exprel hs_cg_exprel
 )HSCG";
 
 std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn)

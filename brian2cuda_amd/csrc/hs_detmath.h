@@ -183,6 +183,197 @@ HS_HD double hs_box_muller(double u1, double u2)
     return hs_sqrt(-2.0 * hs_log(u1)) * hs_cos2pi(u2);
 }
 
+/* ---- elementary functions of generated code (double) ---------------------
+ * sin, cos, tan: x = k pi/2 + r, k the integer nearest to x 2/pi and r from a
+ * three-part split of pi/2 (33 + 33 + 53 bits: k*hi and k*mid are exact for
+ * |k| < 2^20, |x| < 1.6e6; beyond that r loses about log2(|k| / 2^20) bits),
+ * then the series of hs_sin_red / hs_cos_red on |r| <= pi/4 and the quadrant
+ * k mod 4.  Infinite or NaN x gives NaN.
+ * sinh, tanh, expm1, exprel: Taylor series for |x| < 0.5 (no cancellation near
+ * 0; truncation below 2^-60 relative), exp beyond; cosh from exp.
+ * log1p: log(u) * x / (u - 1) with u = 1 + x (exact where u - 1 is).
+ * arctan: odd; x > 1 -> pi/2 - arctan(1/x); x > tan(pi/12) ->
+ * pi/6 + arctan((sqrt3 x - 1) / (x + sqrt3)); then the series to t^27 on
+ * |t| <= 0.268 (truncation below 2^-59).  arcsin x = arctan(x / sqrt((1-x)(1+x))),
+ * arccos x = 2 arctan(sqrt((1-x) / (1+x))); |x| > 1 gives NaN.
+ */
+#define HS_PIO2_HI 1.5707963267341256     /* pi/2 to 33 significant bits */
+#define HS_PIO2_MID 6.077100506303966e-11 /* the next 33 bits */
+#define HS_PIO2_LO 2.0222662487959506e-21 /* the rest */
+#define HS_2_PI 0.6366197723675814
+#define HS_PI 3.141592653589793
+#define HS_PI_6 0.5235987755982989
+#define HS_SQRT3 1.7320508075688772
+#define HS_TAN_PI_12 0.2679491924311227
+
+HS_HD void hs_sincos(double x, double *s, double *c)
+{
+    if (!(x - x == 0.0)) { /* inf or NaN */
+        *s = x - x;
+        *c = x - x;
+        return;
+    }
+    const double kd = floor(x * HS_2_PI + 0.5);
+    const double r = ((x - kd * HS_PIO2_HI) - kd * HS_PIO2_MID) - kd * HS_PIO2_LO;
+    const double sr = hs_sin_red(r), cr = hs_cos_red(r);
+    const int q = (int)(kd - 4.0 * floor(kd * 0.25)); /* k mod 4 */
+    if (q == 0) {
+        *s = sr;
+        *c = cr;
+    } else if (q == 1) {
+        *s = cr;
+        *c = -sr;
+    } else if (q == 2) {
+        *s = -sr;
+        *c = -cr;
+    } else {
+        *s = -cr;
+        *c = sr;
+    }
+}
+
+HS_HD double hs_sin(double x)
+{
+    double s, c;
+    hs_sincos(x, &s, &c);
+    return s;
+}
+
+HS_HD double hs_cos(double x)
+{
+    double s, c;
+    hs_sincos(x, &s, &c);
+    return c;
+}
+
+HS_HD double hs_tan(double x)
+{
+    double s, c;
+    hs_sincos(x, &s, &c);
+    return s / c;
+}
+
+/* (e^x - 1) / x for |x| < 0.5: 1 + x/2! + x^2/3! + ... + x^16/17! */
+HS_HD double hs_exprel_series(double x)
+{
+    double q = 2.8114572543455206e-15;
+    q = 4.779477332387385e-14 + x * q;
+    q = 7.647163731819816e-13 + x * q;
+    q = 1.1470745597729725e-11 + x * q;
+    q = 1.6059043836821613e-10 + x * q;
+    q = 2.08767569878681e-09 + x * q;
+    q = 2.505210838544172e-08 + x * q;
+    q = 2.755731922398589e-07 + x * q;
+    q = 2.7557319223985893e-06 + x * q;
+    q = 2.48015873015873e-05 + x * q;
+    q = 0.0001984126984126984 + x * q;
+    q = 0.001388888888888889 + x * q;
+    q = 0.008333333333333333 + x * q;
+    q = 0.041666666666666664 + x * q;
+    q = 0.16666666666666666 + x * q;
+    q = 0.5 + x * q;
+    return 1.0 + x * q;
+}
+
+HS_HD double hs_expm1(double x)
+{
+    if (x > -0.5 && x < 0.5) return x * hs_exprel_series(x);
+    return hs_exp(x) - 1.0;
+}
+
+HS_HD double hs_exprel(double x)
+{
+    if (x > -0.5 && x < 0.5) return hs_exprel_series(x);
+    return hs_expm1(x) / x;
+}
+
+HS_HD double hs_log1p(double x)
+{
+    const double u = 1.0 + x;
+    if (u == 1.0) return x;
+    return hs_log(u) * (x / (u - 1.0));
+}
+
+HS_HD double hs_sinh(double x)
+{
+    if (x > -0.5 && x < 0.5) { /* x (1 + x^2/3! + ... + x^16/17!) */
+        const double x2 = x * x;
+        double q = 2.8114572543455206e-15;
+        q = 7.647163731819816e-13 + x2 * q;
+        q = 1.6059043836821613e-10 + x2 * q;
+        q = 2.505210838544172e-08 + x2 * q;
+        q = 2.7557319223985893e-06 + x2 * q;
+        q = 0.0001984126984126984 + x2 * q;
+        q = 0.008333333333333333 + x2 * q;
+        q = 0.16666666666666666 + x2 * q;
+        return x + x * (x2 * q);
+    }
+    const double e = hs_exp(x < 0.0 ? -x : x);
+    const double r = 0.5 * (e - 1.0 / e);
+    return x < 0.0 ? -r : r;
+}
+
+HS_HD double hs_cosh(double x)
+{
+    const double e = hs_exp(x < 0.0 ? -x : x);
+    return 0.5 * (e + 1.0 / e);
+}
+
+HS_HD double hs_tanh(double x)
+{
+    if (x > -0.5 && x < 0.5) {
+        const double em = hs_expm1(2.0 * x);
+        return em / (em + 2.0);
+    }
+    const double e = hs_exp(x < 0.0 ? -2.0 * x : 2.0 * x);
+    const double r = 1.0 - 2.0 / (e + 1.0);
+    return x < 0.0 ? -r : r;
+}
+
+HS_HD double hs_atan(double x)
+{
+    if (x != x) return x;
+    double t = x < 0.0 ? -x : x, base = 0.0;
+    const int inv = t > 1.0;
+    if (inv) t = 1.0 / t;
+    if (t > HS_TAN_PI_12) {
+        t = (HS_SQRT3 * t - 1.0) / (t + HS_SQRT3);
+        base = HS_PI_6;
+    }
+    const double t2 = t * t;
+    double p = -0.037037037037037035;
+    p = 0.04 + t2 * p;
+    p = -0.043478260869565216 + t2 * p;
+    p = 0.047619047619047616 + t2 * p;
+    p = -0.05263157894736842 + t2 * p;
+    p = 0.058823529411764705 + t2 * p;
+    p = -0.06666666666666667 + t2 * p;
+    p = 0.07692307692307693 + t2 * p;
+    p = -0.09090909090909091 + t2 * p;
+    p = 0.1111111111111111 + t2 * p;
+    p = -0.14285714285714285 + t2 * p;
+    p = 0.2 + t2 * p;
+    p = -0.3333333333333333 + t2 * p;
+    double r = base + (t + t * (t2 * p));
+    if (inv) r = HS_PI_2 - r;
+    return x < 0.0 ? -r : r;
+}
+
+HS_HD double hs_asin(double x)
+{
+    if (!(x >= -1.0 && x <= 1.0)) return (x - x) / (x - x); /* NaN */
+    if (x == 1.0) return HS_PI_2;
+    if (x == -1.0) return -HS_PI_2;
+    return hs_atan(x / hs_sqrt((1.0 - x) * (1.0 + x)));
+}
+
+HS_HD double hs_acos(double x)
+{
+    if (!(x >= -1.0 && x <= 1.0)) return (x - x) / (x - x); /* NaN */
+    if (x == -1.0) return HS_PI;
+    return 2.0 * hs_atan(hs_sqrt((1.0 - x) / (1.0 + x)));
+}
+
 /* ---- single precision (the noise of the FP32 form) ------------------------
  * Same methods with float arithmetic throughout: the series stop where the
  * truncation falls below 2^-26 relative.  IEEE float + - * / sqrt are correctly

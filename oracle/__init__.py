@@ -197,6 +197,7 @@ def stateupdate(model, precision, arrays, last, params, dt, seed, step):
 _CG_MATH = r"""
 #include <math.h>
 #include "hs_detmath.h"
+#include "hs_cgfuncs.h"
 static inline double hs_cg_exp(double x) { return hs_exp(x); }
 static inline float hs_cg_exp(float x) { return hs_expf(x); }
 static inline double hs_cg_exp(int x) { return hs_exp((double)x); }
@@ -239,6 +240,18 @@ static inline auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo + hi)
 #define ceil hs_cg_ceil
 #define sign hs_cg_sign
 #define clip hs_cg_clip
+#define sin hs_cg_sin
+#define cos hs_cg_cos
+#define tan hs_cg_tan
+#define sinh hs_cg_sinh
+#define cosh hs_cg_cosh
+#define tanh hs_cg_tanh
+#define arcsin hs_cg_arcsin
+#define arccos hs_cg_arccos
+#define arctan hs_cg_arctan
+#define expm1 hs_cg_expm1
+#define log1p hs_cg_log1p
+#define exprel hs_cg_exprel
 """
 _cg_libs = {}
 
@@ -313,7 +326,7 @@ def connect_code_source(cond, p_expr, precision):
     head = ("{\n    const long long i = hs_ai - pre_lo, j = hs_aj - post_lo, N_pre = pre_hi - pre_lo, "
             "N_post = post_hi - post_lo;\n    (void)hs_pre_v; (void)hs_post_v; (void)i; (void)j; (void)N_pre; "
             "(void)N_post;\n")
-    return (_CG_MATH + '#include "hs_cgconn.h"\n' + f"typedef {R} R;\n"
+    return ('#include "hs_cgconn.h"\n' + _CG_MATH + f"typedef {R} R;\n"
             "#define hs_pre(k) ((double)((const R *)hs_pre_v[k])[hs_ai])\n"
             "#define hs_post(k) ((double)((const R *)hs_post_v[k])[hs_aj])\n"
             f'extern "C" int hs_cn_cond{args}\n{head}    return ({cond or "true"}) ? 1 : 0;\n}}\n'

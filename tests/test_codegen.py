@@ -112,7 +112,7 @@ def test_code_mix_runs_in_both_semantics(prec, sem):
 
 
 @pytest.mark.parametrize("eqs,kw,msg", [
-    ("dv/dt = sin(v)/tau : 1", {}, "function"),
+    ("dv/dt = erf(v)/tau : 1", {}, "function"),
     ("dv/dt = (xi_1 + xi_2)/tau : 1", {}, "more than one noise"),
     ("dv/dt = v*xi/tau : 1", {"method": "euler"}, "additive"),
     ("dv/dt = v*v/tau : 1", {"method": "exponential_euler"}, "not linear"),

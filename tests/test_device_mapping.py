@@ -155,7 +155,7 @@ def test_poisson_and_spike_generator_groups():
 
 @pytest.mark.parametrize("change,msg", [
     (dict(equations="dv/dt = -v/tau : volt"), "no value for 'tau'"),
-    (dict(equations="dv/dt = sin(v)/taum : volt"), "function 'sin'"),
+    (dict(equations="dv/dt = erf(v)/taum : volt"), "function 'erf'"),
     (dict(reset="v = Vr; w += 1"), "'w' is not a state variable"),
     (dict(method="heun"), "method 'heun'"),
     (dict(namespace={k: v for k, v in CUBA_NS.items() if k != "taum"}), "taum"),

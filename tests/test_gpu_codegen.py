@@ -299,3 +299,28 @@ def test_synaptic_code_on_device_connectivity(hs, oracle_lib):
         assert n.sum() == nb.stats()["events"] == rb.events
         na.close()
         nb.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine,sem,opts", [("plain", 0, {}), ("fused", 1, {9: 1})])
+def test_trig_and_hyperbolic_functions_vs_oracle(hs, oracle_lib, prec, engine, sem, opts):
+    """A code model calling every function added to exp / log / sqrt (sin, cos,
+    tan, sinh, cosh, tanh, arcsin, arccos, arctan, expm1, log1p, exprel):
+    the device and the oracle evaluate them with the same operations, so
+    spikes and state agree bit for bit."""
+    eqs = """
+    dv/dt = (I*(1 + sin(2*pi*f*t + phase)) + 3*mV*tanh(v/(10*mV)) + mV*arctan(v/mV) - v)/tau : volt
+    dw/dt = (0.1*(cosh(0.5*tanh(w)) - 1) + 0.1*expm1(-w*w) + 0.05*log1p(w*w) + 0.1*(exprel(-w) - 1)
+             + 0.2*arcsin(0.5*sin(w)) + 0.1*(arccos(0.5*cos(w)) - 1) + 0.1*tan(0.1*tanh(w))
+             + 0.1*sinh(0.2*tanh(w)) - w)/tau_w : 1
+    phase : 1 (constant)
+    """
+    ns = {"I": 12e-3, "f": 40.0, "tau": 10e-3, "tau_w": 30e-3, "pi": np.pi}
+    model, names = cg.neuron_model(eqs, "v > 10*mV", "v = 0*mV; w = w + 0.1", 2e-3, "euler", ns, dt=1e-4)
+    n = 2000
+    rng = np.random.default_rng(5)
+    init = {names.index("v"): 10e-3 * rng.random(n), names.index("w"): rng.random(n),
+            names.index("phase"): 2 * np.pi * rng.random(n)}
+    spec = nw.NetSpec("trig", 1e-4, 3, [nw.GroupSpec(0, n, [], init, code=model)], [])
+    spec.spike_monitors = [0]
+    _compare(spec, prec, engine, sem, opts, oracle_lib, (150, 350))

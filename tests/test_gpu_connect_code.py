@@ -167,3 +167,28 @@ def test_generator_syntax_network(hs, oracle_lib, prec, engine, sem):
     assert np.array_equal(net.get_state(0, 0), ref.get_state(0, 0))
     assert net.stats()["events"] == ref.events
     net.close()
+
+
+# brian2cuda's test_default_function_implementations (tests/test_cuda_generator.py:20-90):
+# (function, zero point, offset)
+DEFAULT_FUNCS = [("sin", 0, 0), ("cos", 1, 0), ("tan", 0, 0), ("sinh", 0, 0), ("cosh", 1, 0), ("tanh", 0, 0),
+                 ("exp", 1, 0), ("sqrt", 0, 0), ("ceil", 0, 0), ("floor", 0, 0), ("arcsin", 0, 0),
+                 ("arctan", 0, 0), ("int", 0, 0), ("arccos", 0, 1), ("log", 0, 1), ("log10", 0, 1),
+                 ("abs", 1, -1), ("sign", 1, 1)]
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_default_function_implementations(hs, prec):
+    """Each default function in a connect condition on a one-neuron group,
+    'f(i + offset) == zero_point' with an integer and with a real argument
+    (the reference's int and double overloads), and clip: exactly one synapse
+    each, generated on the device."""
+    conds = [c for f, z, o in DEFAULT_FUNCS for c in (f"{f}(i + {o}) == {z}", f"{f}(i * 1.0 + {o}) == {z}")]
+    conds += ["clip(i+1,-1,0)==0", "clip(i+1,-1.0,0.0)==0"]
+    net, _ = _setup(prec, 1, 1)
+    for cond in conds:
+        c, const = cg.connect_expression(cond)
+        assert const is None
+        pid = net.connect_code(0, 0, 1, c, "", 1.0)
+        assert net.pathway_info(pid)["nsyn"] == 1, cond
+    net.close()
