@@ -74,6 +74,8 @@ def _unit_table():
 
 
 UNITS = _unit_table()
+# Brian2's default constants, where the namespace does not name them
+CONSTANTS = {"pi": float(np.pi), "e": float(np.e), "inf": float("inf")}
 RESERVED = {"t", "dt", "xi", "i", "not_refractory"}
 
 
@@ -350,6 +352,8 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
             return sp.Float(float(ns[name]))
         if name in UNITS:
             return sp.Float(UNITS[name])
+        if name in CONSTANTS and np.isfinite(CONSTANTS[name]):
+            return sp.Float(CONSTANTS[name])
         raise NotImplementedError(f"{what}: no value for {name!r} (not a variable, subexpression, constant or unit)")
 
     rhs = {e.name: _parse(e.expr, sym, f"{what}: d{e.name}/dt = {e.expr}") for e in diff}
@@ -419,7 +423,7 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
             B = f.subs(state[x], 0)
             out.append(f"const R _hs_BA_{x} = {pr(B / A)};")
             out.append(f"const R _hs_{x} = -_hs_BA_{x} + (_hs_BA_{x} + {x})*exp({pr(A * DT)});")
-    elif method in ("exact", "linear"):
+    elif method in ("exact", "linear", "independent"):  # Brian2's older names of the linear solution
         if dt is None:
             raise NotImplementedError(f"{what}: 'exact' needs the time step")
         if noisy or not _linear(rhs, state, T):
@@ -528,6 +532,9 @@ def _condition(text, sym, pr, what):
             return "not_refractory"
         if isinstance(n, ast.Constant) and isinstance(n.value, bool):
             return "true" if n.value else "false"
+        if isinstance(n, (ast.Name, ast.Call, ast.BinOp, ast.UnaryOp, ast.Constant)):
+            # a number as a condition (a boolean variable 'x : boolean' holds 0 or 1)
+            return f"({pr(_Conv(sym, what).visit(n))} != ((R)0))"
         raise NotImplementedError(f"{what}: {ast.unparse(n)!r} is not a condition")
     return c(tree)
 
@@ -777,6 +784,8 @@ def connect_expression(expr, namespace=None, pre_names=(), post_names=(), what="
                 return lit(ns[name])
             if name in UNITS:
                 return lit(UNITS[name])
+            if name in CONSTANTS and np.isfinite(CONSTANTS[name]):
+                return lit(CONSTANTS[name])
             if name in ("True", "False"):
                 return lit(name == "True")
             raise NotImplementedError(f"{what}: unknown name {name!r} in {expr!r}")
