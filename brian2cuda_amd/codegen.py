@@ -24,7 +24,9 @@ leaves alone).  Names resolve to the group's variables, its subexpressions,
 the namespace (SI values), Brian2's units, and t, dt, xi, i.  Functions: exp,
 log, log10, sqrt, abs, clip, floor, ceil, sign, sin, cos, tan, sinh, cosh,
 tanh, arcsin, arccos, arctan, expm1, log1p, exprel (evaluated by the library's
-deterministic implementations on the device and in the oracle alike).
+deterministic implementations on the device and in the oracle alike), and
+rand() / randn() in euler updates, thresholds and resets (each call site its
+own Philox draw per neuron and step, hs_rand_u(k) / hs_rand_n(k)).
 '(unless refractory)' holds
 a variable while the neuron is refractory.  Anything else raises
 NotImplementedError naming it.
@@ -203,6 +205,8 @@ class _Conv(ast.NodeVisitor):
         return f()
 
     def visit_Call(self, n):
+        if isinstance(n.func, ast.Name) and n.func.id in ("rand", "randn") and not n.args and not n.keywords:
+            return self.sym(n.func.id + "()")  # a draw: the caller's name resolution decides
         if not isinstance(n.func, ast.Name) or n.func.id not in _FUNCS or n.keywords:
             raise NotImplementedError(f"{self.what}: function {ast.unparse(n.func)!r} is not supported")
         return _FUNCS[n.func.id](*[self.visit(a) for a in n.args])
@@ -328,8 +332,19 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
     T, DT, XI, I = (sp.Symbol(n, real=True) for n in ("t", "dt", "xi", "i"))
     xi_names = set()
     sub_expr = {}
+    draws = []  # rand() / randn() call sites in order: their symbols and kinds ("u", "n")
 
     def sym(name, stack=()):
+        if name in ("rand()", "randn()"):
+            if stack:
+                raise NotImplementedError(f"{what}: {name} inside a subexpression (its uses would share one draw)")
+            kind = "u" if name == "rand()" else "n"
+            k = sum(1 for _, q in draws if q == kind)
+            if k >= 16:
+                raise NotImplementedError(f"{what}: more than 16 {name} calls")
+            d = sp.Symbol(f"_hs_rnd_{kind}{k}", real=True)
+            draws.append((d, kind))
+            return d
         if name in state:
             return state[name]
         if name in subs:
@@ -360,13 +375,33 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
     if len(xi_names) > 1:
         raise NotImplementedError(f"{what}: more than one noise variable ({sorted(xi_names)})")
     noisy = bool(xi_names)
+    ndraw_update = len(draws)
     unless = {e.name for e in diff if "unless refractory" in e.flags}
     if not diff:
         method = "none"  # nothing to integrate
     elif method is None:
         method = "euler" if noisy else ("exact" if _linear(rhs, state, T) else "euler")
+    if ndraw_update and method != "euler" and method != "none":
+        raise NotImplementedError(f"{what}: rand() / randn() in the equations need method 'euler' "
+                                  f"(not {method!r})")
     local = {s: n for n, s in state.items()}
     local.update({T: "((R)t)", DT: "dt", XI: "xi", I: "((R)i)"})
+
+    class _DrawLocal(dict):  # draw symbols print as the draw of their call site
+        def __missing__(self, key):
+            raise KeyError(key)
+
+        def __contains__(self, key):
+            return dict.__contains__(self, key) or (isinstance(key, sp.Symbol) and
+                                                    key.name.startswith("_hs_rnd_"))
+
+        def __getitem__(self, key):
+            if dict.__contains__(self, key):
+                return dict.__getitem__(self, key)
+            kind, k = key.name[len("_hs_rnd_")], key.name[len("_hs_rnd_") + 1:]
+            return f"((R)hs_rand_{kind}({k}))"
+
+    local = _DrawLocal(local)
     pr = _Printer(local)
     out = []
     order = [e.name for e in diff]

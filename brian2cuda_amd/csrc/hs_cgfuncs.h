@@ -11,6 +11,7 @@
 #define HS_CGFUNCS_H
 
 #include "hs_detmath.h"
+#include "hs_philox.h"
 
 #define HS_CG_FUNC1(name, fn)                                                 \
     HS_HD double name(double x) { return fn(x); }                             \
@@ -32,5 +33,14 @@ HS_CG_FUNC1(hs_cg_log1p, hs_log1p)
 HS_CG_FUNC1(hs_cg_exprel, hs_exprel)
 
 #undef HS_CG_FUNC1
+
+/* the stream of rand() (normal = 0) or randn() (normal = 1) call site k < 16
+   of a code model whose group draws from stream (below 2^24): the group's
+   stream in the low 24 bits, 1 + k + 16 normal above, apart from its xi
+   (stream itself) and from every other call site */
+HS_HD unsigned int hs_cg_rnd_stream(unsigned int stream, int k, int normal)
+{
+    return (stream & 0x00FFFFFFu) | ((unsigned int)(1 + k + 16 * normal) << 24);
+}
 
 #endif /* HS_CGFUNCS_H */

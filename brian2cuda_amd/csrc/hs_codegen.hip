@@ -180,16 +180,19 @@ static void hs_cg_struct(std::string &o, const char *R, int c, const HsCgModel &
     auto store = [&]() {
         for (int v = 0; v < nv; ++v) o += "        hs_s[" + std::to_string(v) + "] = " + m.names[v] + ";\n";
     };
-    o += "    static __device__ __forceinline__ bool step(R *hs_s, int not_refractory, R xi, R dt, double t, int i)\n";
-    o += "    {\n        (void)not_refractory; (void)xi; (void)dt; (void)t; (void)i;\n";
+    o += "    static __device__ __forceinline__ bool step(R *hs_s, int not_refractory, R xi, R dt, double t, int i, "
+         "unsigned long long hs_seed, unsigned int hs_stream, long long hs_step)\n";
+    o += "    {\n        (void)not_refractory; (void)xi; (void)dt; (void)t; (void)i; (void)hs_seed; (void)hs_stream; "
+         "(void)hs_step;\n";
     load();
     o += "        {\n" + m.update + "\n        }\n";
     store();
     for (int v = 0; v < nv; ++v) o += "        (void)" + m.names[v] + ";\n";
     o += "        return " + (m.threshold.empty() ? std::string("false") : "(bool)(" + m.threshold + ")") + ";\n";
     o += "    }\n";
-    o += "    static __device__ __forceinline__ void reset(R *hs_s, R dt, double t, int i)\n";
-    o += "    {\n        (void)dt; (void)t; (void)i;\n";
+    o += "    static __device__ __forceinline__ void reset(R *hs_s, R dt, double t, int i, unsigned long long hs_seed, "
+         "unsigned int hs_stream, long long hs_step)\n";
+    o += "    {\n        (void)dt; (void)t; (void)i; (void)hs_seed; (void)hs_stream; (void)hs_step;\n";
     load();
     o += "        {\n" + m.reset + "\n        }\n";
     store();
@@ -255,6 +258,10 @@ __device__ __forceinline__ auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo +
 #define expm1 hs_cg_expm1
 #define log1p hs_cg_log1p
 #define exprel hs_cg_exprel
+/* rand() / randn() call site k of a code model: a pure function of the
+   network's seed, the group's stream, the neuron and the step */
+#define hs_rand_u(k) hs_rand(hs_seed, hs_cg_rnd_stream(hs_stream, k, 0), (uint32_t)i, hs_step)
+#define hs_rand_n(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 1), (uint32_t)i, hs_step)
 )HSCG";
 
 std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn)

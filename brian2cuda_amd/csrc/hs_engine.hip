@@ -192,7 +192,8 @@ __device__ __forceinline__ bool hs_step_one(const DGroup<R> &g, R *s, const R *k
         if (g.has_reset && (long long)last == t - 1 && t - 1 >= no_reset_before) {
 #ifdef HS_CG_MODELS
             if constexpr (HS_IS_CODE_MODEL(MODEL))
-                HsCg<R, MODEL - HS_MODEL_CODE>::reset(s, (R)g.dt, (double)(t - 1) * g.dt, i);
+                HsCg<R, MODEL - HS_MODEL_CODE>::reset(s, (R)g.dt, (double)(t - 1) * g.dt, i, g.seed, g.stream,
+                                                      t - 1);
             else
 #endif
                 s[0] = g.vr;
@@ -219,7 +220,9 @@ __device__ __forceinline__ bool hs_step_one(const DGroup<R> &g, R *s, const R *k
         bool spk;
 #ifdef HS_CG_MODELS
         if constexpr (HS_IS_CODE_MODEL(MODEL))
-            spk = HsCg<R, MODEL - HS_MODEL_CODE>::step(s, not_ref, xi, (R)g.dt, (double)t * g.dt, i) && not_ref;
+            spk = HsCg<R, MODEL - HS_MODEL_CODE>::step(s, not_ref, xi, (R)g.dt, (double)t * g.dt, i, g.seed, g.stream,
+                                                       t) &&
+                  not_ref;
         else
 #endif
             spk = hs_neuron_step<R, MODEL>(s, k, not_ref, g.vth, xi);
@@ -667,7 +670,7 @@ __device__ __forceinline__ void hs_cg_reset_c(const DGroup<R> &g, int i, long lo
     R s[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) s[v] = g.var[v][i];
-    HsCg<R, C>::reset(s, (R)g.dt, (double)ts * g.dt, i);
+    HsCg<R, C>::reset(s, (R)g.dt, (double)ts * g.dt, i, g.seed, g.stream, ts);
 #pragma unroll
     for (int v = 0; v < NV; ++v) g.var[v][i] = s[v];
 }

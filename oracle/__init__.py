@@ -252,6 +252,8 @@ static inline auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo + hi)
 #define expm1 hs_cg_expm1
 #define log1p hs_cg_log1p
 #define exprel hs_cg_exprel
+#define hs_rand_u(k) hs_rand(hs_seed, hs_cg_rnd_stream(hs_stream, k, 0), (uint32_t)i, hs_step)
+#define hs_rand_n(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 1), (uint32_t)i, hs_step)
 """
 _cg_libs = {}
 
@@ -262,12 +264,14 @@ def code_model_source(model, precision):
     load = "".join(f"    R {n} = hs_s[{k}];\n" for k, n in enumerate(model.names))
     store = "".join(f"    hs_s[{k}] = {n};\n" for k, n in enumerate(model.names))
     thr = f"({model.threshold}) ? 1 : 0" if model.threshold else "0"
+    rnd = "unsigned long long hs_seed, unsigned int hs_stream, long long hs_step"
+    unused = "(void)hs_seed; (void)hs_stream; (void)hs_step;"
     return (_CG_MATH + f"typedef {R} R;\n"
-            f'extern "C" int hs_cg_step(R *hs_s, int not_refractory, R xi, R dt, double t, int i)\n{{\n'
-            f"    (void)not_refractory; (void)xi; (void)dt; (void)t; (void)i;\n{load}"
+            f'extern "C" int hs_cg_step(R *hs_s, int not_refractory, R xi, R dt, double t, int i, {rnd})\n{{\n'
+            f"    (void)not_refractory; (void)xi; (void)dt; (void)t; (void)i; {unused}\n{load}"
             f"    {{\n{model.update}\n    }}\n{store}    return {thr};\n}}\n"
-            f'extern "C" void hs_cg_reset(R *hs_s, R dt, double t, int i)\n{{\n'
-            f"    (void)dt; (void)t; (void)i;\n{load}    {{\n{model.reset or ''}\n    }}\n{store}}}\n")
+            f'extern "C" void hs_cg_reset(R *hs_s, R dt, double t, int i, {rnd})\n{{\n'
+            f"    (void)dt; (void)t; (void)i; {unused}\n{load}    {{\n{model.reset or ''}\n    }}\n{store}}}\n")
 
 
 def synapse_code_source(code, precision):

@@ -253,3 +253,83 @@ def test_event_driven_trace_known_answer():
     np.testing.assert_allclose(val["Apre"], dA * np.exp(-50 * DT / tau) + dA, rtol=1e-12)
     np.testing.assert_allclose(val["Apost"], dA * np.exp(-30 * DT / tau), rtol=1e-12)  # decayed at 60
     assert ref.synapse_var(0, 3, 1)[0] == 60
+
+
+# ---- rand() / randn() in code models ------------------------------------------------
+def _rnd_stream(stream, k, normal):
+    """the stream of call site k (hs_cg_rnd_stream in hs_cgfuncs.h)"""
+    return (stream & 0xFFFFFF) | ((1 + k + 16 * normal) << 24)
+
+
+def test_rand_draws_known_answer():
+    """dv/dt = rand()/ms (euler): after 3 steps v is the sum of the draws of
+    call site 0, each (1000 u) dt in the update's own order; the reset
+    'v = 0.5*randn()' of a spike at step s holds the normal draw of call site
+    0 at (neuron, s).  The draws are the oracle's Philox words under the
+    documented keys (network seed, the group's stream with the call site's
+    index above bit 24, neuron, step)."""
+    dt = 1e-4
+    m, names = cg.neuron_model("dv/dt = rand()/ms : 1", None, None, 0.0, "euler", {}, dt=dt, stream=5)
+    spec = nw.NetSpec("rand", dt, 77, [nw.GroupSpec(0, 20, [], {}, code=m)], [])
+    ref, _, _ = oracle.build_ref(spec, 64, oracle.SEQ)
+    ref.run(3)
+    L = oracle.lib()
+    want = np.zeros(20)
+    for t in range(3):
+        for i in range(20):
+            want[i] = want[i] + 1000.0 * L.ref_rand(77, _rnd_stream(5, 0, 0), i, t) * dt
+    assert np.array_equal(ref.get_state(0, 0), want)
+    # a threshold that always fires and a reset that draws
+    m, names = cg.neuron_model("v : 1", "True", "v = 0.5*randn()", 0.0, None, {}, dt=dt, stream=9)
+    spec = nw.NetSpec("randn_reset", dt, 77, [nw.GroupSpec(0, 10, [], {}, code=m)], [])
+    ref, _, _ = oracle.build_ref(spec, 64, oracle.SEQ)
+    ref.run(4)  # the spike of step 3 is reset at the run's end
+    want = np.array([0.5 * oracle.randn(77, _rnd_stream(9, 0, 1), i, 3) for i in range(10)])
+    assert np.array_equal(ref.get_state(0, 0), want)
+
+
+def test_rand_threshold_equals_builtin_poisson():
+    """threshold='rand() < rates*dt' on a group of rates is the PoissonGroup
+    (poissongroup template: rand() < rates*dt) drawing from the call site's
+    stream: identical spikes in FP64."""
+    dt = 1e-4
+    rates = 5.0 + 80.0 * np.random.default_rng(2).random(400)
+    m, names = cg.neuron_model("rates : Hz", "rand() < rates*dt", None, 0.0, None, {}, dt=dt, stream=3)
+    code = nw.NetSpec("poisson_code", dt, 11, [nw.GroupSpec(0, 400, [], {0: rates}, code=m)], [])
+    code.spike_monitors = [0]
+    builtin = nw.NetSpec("poisson", dt, 11, [nw.GroupSpec(nw.HS_MODEL_POISSON, 400, [0.0, _rnd_stream(3, 0, 0)],
+                                                          {0: rates})], [])
+    builtin.spike_monitors = [0]
+    a, sa, _ = oracle.build_ref(code, 64, oracle.SEQ)
+    b, sb, _ = oracle.build_ref(builtin, 64, oracle.SEQ)
+    a.run(2000)
+    b.run(2000)
+    ia, ta = a.spikes(sa[0])
+    ib, tb = b.spikes(sb[0])
+    assert len(ia) > 1000 and np.array_equal(ia, ib) and np.array_equal(ta, tb)
+
+
+def test_rand_reference_test_rand():
+    """brian2cuda's test_rand (tests/test_random_number_generation.py:194-204):
+    dv/dt = rand() : second over 3 steps; the last values are not all 0 and
+    differ from the step before."""
+    m, _ = cg.neuron_model("dv/dt = rand() : second", None, None, 0.0, None, {}, dt=1e-4)
+    spec = nw.NetSpec("test_rand", 1e-4, 1, [nw.GroupSpec(0, 1000, [], {}, code=m)], [])
+    spec.state_monitors = [(0, 0, 0, 1000)]
+    ref, _, st = oracle.build_ref(spec, 64, oracle.SEQ)
+    ref.run(3)
+    v = ref.state_monitor(st[0], 1000)
+    assert not np.all(v[-1] == 0) and not np.array_equal(v[-2], v[-1])
+    # the second run of test_random_number_generation_with_multiple_runs continues the draws
+    ref.run(2)
+    v2 = ref.state_monitor(st[0], 1000)
+    assert v2.shape == (5, 1000) and not np.array_equal(v2[-2], v2[-1])
+
+
+@pytest.mark.parametrize("eqs,kw,msg", [
+    ("dv/dt = rand()/ms : 1", {"method": "rk4"}, "need method 'euler'"),
+    ("dv/dt = I/ms : 1\nI = rand() : 1", {"method": "euler"}, "subexpression"),
+])
+def test_rand_refusals(eqs, kw, msg):
+    with pytest.raises(NotImplementedError, match=msg):
+        cg.neuron_model(eqs, None, None, 0.0, kw["method"], {}, dt=1e-4)

@@ -324,3 +324,59 @@ def test_trig_and_hyperbolic_functions_vs_oracle(hs, oracle_lib, prec, engine, s
     spec = nw.NetSpec("trig", 1e-4, 3, [nw.GroupSpec(0, n, [], init, code=model)], [])
     spec.spike_monitors = [0]
     _compare(spec, prec, engine, sem, opts, oracle_lib, (150, 350))
+
+
+def _rnd_stream(stream, k, normal):
+    return (stream & 0xFFFFFF) | ((1 + k + 16 * normal) << 24)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine,sem,opts", FORMS)
+def test_rand_randn_code_vs_oracle(hs, oracle_lib, prec, engine, sem, opts):
+    """rand() and randn() in a code model's update (euler), threshold and reset:
+    each call site a Philox draw keyed by seed, stream, call site, neuron and
+    step, the same on the device and in the oracle: spikes and state bit for bit."""
+    eqs = "dv/dt = (I + 2*mV*randn() - v)/tau : volt"
+    model, names = cg.neuron_model(eqs, "v > 10*mV and rand() < 0.8", "v = rand()*2*mV", 2e-3, "euler",
+                                   {"I": 12e-3, "tau": 10e-3}, dt=1e-4, stream=4)
+    n = 3000
+    init = {0: 10e-3 * np.random.default_rng(8).random(n)}
+    spec = nw.NetSpec("rand_code", 1e-4, 21, [nw.GroupSpec(0, n, [], init, code=model)], [])
+    spec.spike_monitors = [0]
+    spec.state_monitors = [(0, 0, 0, 50)]
+    _compare(spec, prec, engine, sem, opts, oracle_lib, (200, 400))
+
+
+@pytest.mark.parametrize("engine", ["plain", "fused"])
+def test_rand_threshold_equals_builtin_poisson_gpu(hs, engine):
+    """threshold='rand() < rates*dt' equals the built-in PoissonGroup drawing
+    from the call site's stream: identical spikes on the device (FP64)."""
+    dt = 1e-4
+    rates = 5.0 + 80.0 * np.random.default_rng(2).random(400)
+    m, _ = cg.neuron_model("rates : Hz", "rand() < rates*dt", None, 0.0, None, {}, dt=dt, stream=3)
+    code = nw.NetSpec("poisson_code", dt, 11, [nw.GroupSpec(0, 400, [], {0: rates}, code=m)], [])
+    builtin = nw.NetSpec("poisson", dt, 11, [nw.GroupSpec(nw.HS_MODEL_POISSON, 400, [0.0, _rnd_stream(3, 0, 0)],
+                                                          {0: rates})], [])
+    out = []
+    for spec in (code, builtin):
+        spec.spike_monitors = [0]
+        net, sm, _ = nw.build_hs(spec, 64, engine)
+        net.run(2000)
+        out.append(net.spikes(sm[0]))
+        net.close()
+    assert len(out[0][0]) > 1000
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_rand_reference_test_rand_gpu(hs):
+    """brian2cuda's test_rand and test_random_number_generation_with_multiple_runs
+    (tests/test_random_number_generation.py:194-224) on the device."""
+    m, _ = cg.neuron_model("dv/dt = rand() : second", None, None, 0.0, None, {}, dt=1e-4)
+    spec = nw.NetSpec("test_rand", 1e-4, 1, [nw.GroupSpec(0, 1000, [], {}, code=m)], [])
+    spec.state_monitors = [(0, 0, 0, 1000)]
+    net, _, st = nw.build_hs(spec, 64, "plain")
+    net.run(1)
+    net.run(2)
+    v = net.state_monitor(st[0], 1000)
+    assert v.shape == (3, 1000) and not np.all(v[-1] == 0) and not np.array_equal(v[-2], v[-1])
+    net.close()
