@@ -50,6 +50,29 @@ static bool hs_cg_ident(const std::string &s)
     return true;
 }
 
+/* code may name no identifier of the generated program's own (hs_..., __...)
+   other than the allowed[0..n) it is documented to call */
+static int hs_cg_reserved_ids(const std::string &e, const char *const *allowed, int n, const char *what)
+{
+    for (size_t k = 0; k < e.size();) {
+        if (isalpha((unsigned char)e[k]) || e[k] == '_') {
+            size_t q = k;
+            while (q < e.size() && (isalnum((unsigned char)e[q]) || e[q] == '_')) ++q;
+            const std::string id = e.substr(k, q - k);
+            const bool suffix = k > 0 && (isalnum((unsigned char)e[k - 1]) || e[k - 1] == '.'); /* 1e5, 2LL */
+            if (!suffix && (id.compare(0, 3, "hs_") == 0 || id.compare(0, 2, "__") == 0)) {
+                bool ok = false;
+                for (int a = 0; a < n; ++a) ok = ok || id == allowed[a];
+                if (!ok) return hs_rt_fail(HS_ERR_ARG, "%s: name '%s' is reserved", what, id.c_str());
+            }
+            k = q;
+        } else {
+            ++k;
+        }
+    }
+    return HS_OK;
+}
+
 int hs_cg_check(const HsCgModel &m)
 {
     static const char *const reserved[] = {"R", "dt", "t", "i", "xi", "not_refractory", "exp", "log", "log10",
@@ -77,6 +100,8 @@ int hs_cg_check(const HsCgModel &m)
             if (depth < 0) break;
         }
         if (depth != 0) return hs_rt_fail(HS_ERR_ARG, "code model code has unbalanced braces");
+        static const char *const allowed[] = {"hs_rand_u", "hs_rand_n"};
+        HS_CG_TRY(hs_cg_reserved_ids(*c, allowed, 2, "code model code"));
     }
     return HS_OK;
 }
@@ -105,7 +130,7 @@ static int hs_cg_check_scope(const std::vector<std::string> &all, const std::str
         if (depth < 0) break;
     }
     if (depth != 0) return hs_rt_fail(HS_ERR_ARG, "synaptic code has unbalanced braces");
-    return HS_OK;
+    return hs_cg_reserved_ids(code, nullptr, 0, "synaptic code");
 }
 
 int hs_cg_check_syn(const HsSynCgModel &m)
@@ -421,6 +446,10 @@ int hs_cg_conn_check(const std::string &cond, const std::string &p)
             if (depth < 0) break;
         }
         if (depth != 0) return hs_rt_fail(HS_ERR_ARG, "a connect expression has unbalanced parentheses");
+        /* the generated function's own names (hs_a, hs_c, hs_ai, ...) are not the
+           expression's: of the hs_ names only the documented functions */
+        static const char *const allowed[] = {"hs_pre", "hs_post", "hs_cg_floordiv", "hs_cg_pymod", "hs_cg_pow"};
+        HS_CG_TRY(hs_cg_reserved_ids(*e, allowed, 5, "connect expression"));
     }
     return HS_OK;
 }

@@ -79,7 +79,7 @@ def test_code_model_errors(hs):
     net.close()
     net = rt.Net(64, "plain", 1e-4, 1)
     for names, upd in [(["dt"], ""), (["v", "v"], ""), (["hs_x"], ""), (["v"], "#define X 1"), (["v"], "{"),
-                       ([f"x{k}" for k in range(9)], "")]:
+                       ([f"x{k}" for k in range(9)], ""), (["v"], "v = hs_s[7];"), (["v"], "v = __builtin_inf();")]:
         with pytest.raises(HsError):
             net.add_group_code(10, cg.CodeModel(names, upd))
     for k in range(4):

@@ -120,6 +120,8 @@ def test_connect_code_refusals(hs):
         net.connect_code(0, 0, 1, "i > 0; int x = 1", "", 1.0)
     with pytest.raises(HsError, match="do not compile"):
         net.connect_code(0, 0, 1, "i > undefined_name", "", 1.0)
+    with pytest.raises(HsError, match="reserved"):  # the generated function's own names
+        net.connect_code(0, 0, 1, "((const double *)hs_a.pre[5])[0] > 0.5", "", 1.0)
     with pytest.raises(HsError, match="fixed out-degree"):
         net.connect_code(0, 0, 1, "", "0.5", 1.0, K=5)
     # the network still works after the refusals
