@@ -9,7 +9,12 @@ translates the resulting abstract code statement by statement into CUDA
 stateupdate.cu, threshold.cu and reset.cu).  This module does both steps for
 the methods Brian2 offers for point neurons:
 
-  euler              x += dt*f(x)  (+ sqrt(dt)*g*xi for additive noise: Euler-Maruyama)
+  euler              x += dt*f(x)  (+ sqrt(dt)*sum_k g_k*xi_k for additive noise: Euler-Maruyama)
+  heun               the stochastic Heun method (Stratonovich): x_s = x + f dt + sum_k g_k dW_k,
+                     x += f dt + sum_k dW_k (g_k(x) + g_k(x_s, t+dt))/2; multiplicative noise
+  milstein           Kloeden and Platen's derivative-free Milstein scheme (Ito, one noise
+                     variable): x_s = x + f dt + g sqrt(dt),
+                     x += f dt + g dW + (g(x_s) - g(x))(dW^2 - dt)/(2 sqrt(dt))
   rk2                midpoint: k = dt*f(x, t); x += dt*f(x + k/2, t + dt/2)
   rk4                the classical four-stage Runge-Kutta
   exponential_euler  dx/dt = A x + B with A, B free of x: x = -B/A + (x + B/A) exp(A dt)
@@ -331,6 +336,7 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
     state = {n: sp.Symbol(n, real=True) for n in names}
     T, DT, XI, I = (sp.Symbol(n, real=True) for n in ("t", "dt", "xi", "i"))
     xi_names = set()
+    xi_syms = {}  # each noise variable (xi, xi_1, ...) its own symbol and draw
     sub_expr = {}
     draws = []  # rand() / randn() call sites in order: their symbols and kinds ("u", "n")
 
@@ -362,7 +368,9 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
             return I
         if name == "xi" or name.startswith("xi_"):
             xi_names.add(name)
-            return XI
+            if name not in xi_syms:
+                xi_syms[name] = sp.Symbol(f"_hs_noise_{name}", real=True)
+            return xi_syms[name]
         if name in ns:
             return sp.Float(float(ns[name]))
         if name in UNITS:
@@ -372,9 +380,12 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
         raise NotImplementedError(f"{what}: no value for {name!r} (not a variable, subexpression, constant or unit)")
 
     rhs = {e.name: _parse(e.expr, sym, f"{what}: d{e.name}/dt = {e.expr}") for e in diff}
-    if len(xi_names) > 1:
-        raise NotImplementedError(f"{what}: more than one noise variable ({sorted(xi_names)})")
-    noisy = bool(xi_names)
+    if len(xi_names) > 16:
+        raise NotImplementedError(f"{what}: more than 16 noise variables")
+    # the first noise variable (by name) is the engine's per-neuron draw xi; the
+    # others are normal draws of their own (hs_rand_xi(k), k = 1, 2, ...)
+    noise = [xi_syms[n] for n in sorted(xi_names)]
+    noisy = bool(noise)
     ndraw_update = len(draws)
     unless = {e.name for e in diff if "unless refractory" in e.flags}
     if not diff:
@@ -384,8 +395,12 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
     if ndraw_update and method != "euler" and method != "none":
         raise NotImplementedError(f"{what}: rand() / randn() in the equations need method 'euler' "
                                   f"(not {method!r})")
+    if noisy and method not in ("euler", "heun", "milstein"):
+        raise NotImplementedError(f"{what}: method {method!r} takes no noise (use 'euler', 'heun' or 'milstein')")
     local = {s: n for n, s in state.items()}
     local.update({T: "((R)t)", DT: "dt", XI: "xi", I: "((R)i)"})
+    for k, z in enumerate(noise):
+        local[z] = "xi" if k == 0 else f"((R)hs_rand_xi({k}))"
 
     class _DrawLocal(dict):  # draw symbols print as the draw of their call site
         def __missing__(self, key):
@@ -407,17 +422,58 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
     order = [e.name for e in diff]
     if method == "none":
         pass
-    elif method == "euler":
+    elif method in ("euler", "heun", "milstein"):
+        # dx = f dt + sum_k g_k dW_k, dW_k = sqrt(dt) xi_k: f and the g_k from the
+        # equations (the noise must enter linearly)
+        drift, diffusion = {}, {}
         for x in order:
             f = rhs[x]
-            g = sp.diff(f, XI)
-            if g.has(XI) or any(g.has(y) for y in state.values()):
-                raise NotImplementedError(f"{what}: the noise of d{x}/dt is not additive (euler)")
-            det = f.subs(XI, 0)
-            new = state[x] + DT * det
-            if g != 0:
-                new = new + sp.sqrt(DT) * g * XI
-            out.append(f"const R _hs_{x} = {pr(new)};")
+            gs = [sp.diff(f, z) for z in noise]
+            if any(g.has(z) for g in gs for z in noise):
+                raise NotImplementedError(f"{what}: the noise of d{x}/dt is not linear in the noise variables")
+            drift[x] = f.subs({z: 0 for z in noise}, simultaneous=True)
+            diffusion[x] = gs
+        additive = all(not g.has(y) for x in order for g in diffusion[x] for y in state.values())
+        SQ = sp.sqrt(DT)
+        if method == "euler" or (method == "milstein" and additive):
+            # Euler-Maruyama (milstein's correction vanishes for additive noise)
+            if not additive:
+                raise NotImplementedError(f"{what}: the noise is not additive (euler); use 'heun' (Stratonovich) "
+                                          "or 'milstein' (Ito)")
+            for x in order:
+                new = state[x] + DT * drift[x] + SQ * sum(g * z for g, z in zip(diffusion[x], noise))
+                out.append(f"const R _hs_{x} = {pr(new)};")
+        elif method == "heun":
+            # the stochastic Heun method (Stratonovich):
+            #   x_s = x + f(x,t) dt + sum_k g_k(x,t) dW_k
+            #   x'  = x + f(x,t) dt + sum_k dW_k (g_k(x,t) + g_k(x_s, t+dt)) / 2
+            sup = {state[x]: sp.Symbol(f"_hs_s_{x}", real=True) for x in order}
+            for x in order:
+                local[sup[state[x]]] = f"_hs_s_{x}"
+                new = state[x] + DT * drift[x] + SQ * sum(g * z for g, z in zip(diffusion[x], noise))
+                out.append(f"const R _hs_s_{x} = {pr(new)};")
+            for x in order:
+                corr = sum(z * (g + g.subs({**sup, T: T + DT}, simultaneous=True))
+                           for g, z in zip(diffusion[x], noise))
+                new = state[x] + DT * drift[x] + SQ * corr / 2
+                out.append(f"const R _hs_{x} = {pr(new)};")
+        else:
+            # milstein with multiplicative noise: Kloeden & Platen's derivative-free
+            # Milstein scheme for Ito equations (one noise variable):
+            #   x_s = x + f dt + g sqrt(dt)
+            #   x'  = x + f dt + g dW + (g(x_s) - g(x)) (dW^2 - dt) / (2 sqrt(dt))
+            if len(noise) != 1:
+                raise NotImplementedError(f"{what}: 'milstein' takes one noise variable (use 'heun')")
+            z = noise[0]
+            sup = {state[x]: sp.Symbol(f"_hs_s_{x}", real=True) for x in order}
+            for x in order:
+                local[sup[state[x]]] = f"_hs_s_{x}"
+                out.append(f"const R _hs_s_{x} = {pr(state[x] + DT * drift[x] + SQ * diffusion[x][0])};")
+            for x in order:
+                g = diffusion[x][0]
+                gsup = g.subs(sup, simultaneous=True)
+                new = (state[x] + DT * drift[x] + g * SQ * z + (gsup - g) * (DT * z * z - DT) / (2 * SQ))
+                out.append(f"const R _hs_{x} = {pr(new)};")
     elif method in ("rk2", "rk4"):
         if noisy:
             raise NotImplementedError(f"{what}: method {method!r} takes no noise (use 'euler')")
@@ -485,7 +541,7 @@ def neuron_model(equations, threshold=None, reset=None, refractory=0.0, method=N
             out += lines
     else:
         raise NotImplementedError(f"{what}: method {method!r} is not supported "
-                                  "(euler, rk2, rk4, exponential_euler, exact)")
+                                  "(euler, heun, milstein, rk2, rk4, exponential_euler, exact)")
     for x in order:
         if x in unless:
             out.append(f"{x} = not_refractory ? _hs_{x} : {x};")

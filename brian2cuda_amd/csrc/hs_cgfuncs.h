@@ -34,10 +34,11 @@ HS_CG_FUNC1(hs_cg_exprel, hs_exprel)
 
 #undef HS_CG_FUNC1
 
-/* the stream of rand() (normal = 0) or randn() (normal = 1) call site k < 16
-   of a code model whose group draws from stream (below 2^24): the group's
-   stream in the low 24 bits, 1 + k + 16 normal above, apart from its xi
-   (stream itself) and from every other call site */
+/* the stream of rand() (normal = 0) or randn() (normal = 1) call site k < 16,
+   or of noise variable k (normal = 2, k >= 1), of a code model whose group
+   draws from stream (below 2^24): the group's stream in the low 24 bits,
+   1 + k + 16 normal above, apart from its xi (stream itself) and from every
+   other call site */
 HS_HD unsigned int hs_cg_rnd_stream(unsigned int stream, int k, int normal)
 {
     return (stream & 0x00FFFFFFu) | ((unsigned int)(1 + k + 16 * normal) << 24);

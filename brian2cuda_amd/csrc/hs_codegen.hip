@@ -100,8 +100,8 @@ int hs_cg_check(const HsCgModel &m)
             if (depth < 0) break;
         }
         if (depth != 0) return hs_rt_fail(HS_ERR_ARG, "code model code has unbalanced braces");
-        static const char *const allowed[] = {"hs_rand_u", "hs_rand_n"};
-        HS_CG_TRY(hs_cg_reserved_ids(*c, allowed, 2, "code model code"));
+        static const char *const allowed[] = {"hs_rand_u", "hs_rand_n", "hs_rand_xi"};
+        HS_CG_TRY(hs_cg_reserved_ids(*c, allowed, 3, "code model code"));
     }
     return HS_OK;
 }
@@ -287,6 +287,8 @@ __device__ __forceinline__ auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo +
    network's seed, the group's stream, the neuron and the step */
 #define hs_rand_u(k) hs_rand(hs_seed, hs_cg_rnd_stream(hs_stream, k, 0), (uint32_t)i, hs_step)
 #define hs_rand_n(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 1), (uint32_t)i, hs_step)
+/* noise variable k >= 1 of an equation with several (xi_1, xi_2, ...; k = 0 is xi) */
+#define hs_rand_xi(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 2), (uint32_t)i, hs_step)
 )HSCG";
 
 std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn)

@@ -254,6 +254,7 @@ static inline auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo + hi)
 #define exprel hs_cg_exprel
 #define hs_rand_u(k) hs_rand(hs_seed, hs_cg_rnd_stream(hs_stream, k, 0), (uint32_t)i, hs_step)
 #define hs_rand_n(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 1), (uint32_t)i, hs_step)
+#define hs_rand_xi(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 2), (uint32_t)i, hs_step)
 """
 _cg_libs = {}
 

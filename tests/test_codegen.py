@@ -113,14 +113,14 @@ def test_code_mix_runs_in_both_semantics(prec, sem):
 
 @pytest.mark.parametrize("eqs,kw,msg", [
     ("dv/dt = erf(v)/tau : 1", {}, "function"),
-    ("dv/dt = (xi_1 + xi_2)/tau : 1", {}, "more than one noise"),
+    ("dv/dt = v*(xi_1 + xi_2)/tau : 1", {"method": "milstein"}, "one noise variable"),
     ("dv/dt = v*xi/tau : 1", {"method": "euler"}, "additive"),
     ("dv/dt = v*v/tau : 1", {"method": "exponential_euler"}, "not linear"),
     ("dv/dt = -v/tau + xi/tau : 1", {"method": "rk4"}, "no noise"),
     ("dv/dt = -v/foo : 1", {}, "no value"),
     ("dv/dt = -v/tau : 1 (event-driven)", {}, "belongs to Synapses"),
     ("dv/dt = -v/tau : 1 (clock-driven)", {}, "flag"),
-    ("dv/dt = -v/tau : 1", {"method": "heun"}, "not supported"),
+    ("dv/dt = -v/tau : 1", {"method": "gsl"}, "not supported"),
     ("dv/dt = v*t/tau : 1", {"method": "exact"}, "linear system"),
 ])
 def test_unsupported_forms_are_refused(eqs, kw, msg):
@@ -333,3 +333,102 @@ def test_rand_reference_test_rand():
 def test_rand_refusals(eqs, kw, msg):
     with pytest.raises(NotImplementedError, match=msg):
         cg.neuron_model(eqs, None, None, 0.0, kw["method"], {}, dt=1e-4)
+
+
+# ---- stochastic methods and several noise variables ------------------------------------
+def test_multiple_noise_variables_extended():
+    """brian2cuda's test_multiple_noise_variables_extended (tests/test_stateupdaters.py:16-63):
+    an oscillator with two noise variables scaled by noise_factor = 0, in both
+    of the reference's forms, integrates exactly like the noise-free equations
+    with euler and with heun."""
+    dt = 1e-4
+    base = "dx/dt = y : 1\ndy/dt = - 1*ms**-1*y - 40*ms**-2*x : Hz"
+    forms = ["dx/dt = y : 1\ndy/dt = noise_factor*ms**-1.5*xi_1 + noise_factor*ms**-1.5*xi_2 "
+             "- 1*ms**-1*y - 40*ms**-2*x : Hz",
+             "dx/dt = y + noise_factor*ms**-0.5*xi_1: 1\ndy/dt = noise_factor*ms**-1.5*xi_2 "
+             "- 1*ms**-1*y - 40*ms**-2*x : Hz"]
+    init = {0: np.array([0.5, 1.0]), 1: np.array([0.0, 0.5])}
+
+    def run(eqs, method):
+        m, _ = cg.neuron_model(eqs, None, None, 0.0, method, {"noise_factor": 0.0}, dt=dt)
+        spec = nw.NetSpec("noise", dt, 3, [nw.GroupSpec(0, 2, [], dict(init), code=m)], [])
+        spec.state_monitors = [(0, 0, 0, 2), (0, 1, 0, 2)]
+        ref, _, st = oracle.build_ref(spec, 64, oracle.SEQ)
+        ref.run(100)
+        return ref.state_monitor(st[0], 2), ref.state_monitor(st[1], 2)
+
+    x0, y0 = run(base, "euler")
+    assert np.abs(x0).max() > 0.1
+    for eqs in forms:
+        for method in ("euler", "heun"):
+            x, y = run(eqs, method)
+            assert np.array_equal(x, x0) and np.array_equal(y, y0), method
+
+
+def _gbm(method, n=50, steps=200, dt=1e-4, seed=17, stream=6):
+    """x' = mu x + s x xi / sqrt(ms) (geometric Brownian motion) on the oracle:
+    (states, the primary noise draws of every step)."""
+    mu, sg = 10.0, 0.3
+    m, _ = cg.neuron_model("dx/dt = mu*x + s*x*xi/sqrt(ms) : 1", None, None, 0.0, method, {"mu": mu, "s": sg},
+                           dt=dt, stream=stream)
+    spec = nw.NetSpec("gbm", dt, seed, [nw.GroupSpec(0, n, [], {0: 1.0}, code=m)], [])
+    spec.state_monitors = [(0, 0, 0, n)]
+    ref, _, st = oracle.build_ref(spec, 64, oracle.SEQ)
+    ref.run(steps)
+    z = np.array([[oracle.randn(seed, stream, i, t) for i in range(n)] for t in range(steps)])
+    return ref.state_monitor(st[0], n), z, mu, sg * np.sqrt(1e3), dt
+
+
+def test_heun_multiplicative_noise():
+    """method='heun' (Stratonovich): the recurrence x_s = x + f dt + g dW,
+    x' = x + f dt + dW (g(x) + g(x_s))/2 restated in numpy over the same draws;
+    and strong agreement with the exact Stratonovich solution
+    x0 exp(mu t + s W_t) of the same Brownian path."""
+    rec, z, mu, s, dt = _gbm("heun")
+    x = np.ones(z.shape[1])
+    W = np.zeros_like(x)
+    for t in range(z.shape[0]):
+        # the monitor records each step's state before its update
+        np.testing.assert_allclose(rec[t], x, rtol=1e-13)
+        dW = np.sqrt(dt) * z[t]
+        xs = x + mu * x * dt + s * x * dW
+        x = x + mu * x * dt + dW * (s * x + s * xs) / 2
+        W = W + dW
+    exact = np.exp(mu * dt * z.shape[0] + s * W)
+    assert np.max(np.abs(x / exact - 1)) < 5e-2  # strong order 1; s^2 t = 1.8 here
+
+
+def test_milstein_multiplicative_noise():
+    """method='milstein' (Ito, derivative-free, Kloeden and Platen 11.1.7):
+    x_s = x + f dt + g sqrt(dt), x' = x + f dt + g dW + (g(x_s) - g(x))(dW^2 - dt)/(2 sqrt(dt))
+    restated in numpy over the same draws; strong agreement with the exact Ito
+    solution x0 exp((mu - s^2/2) t + s W_t)."""
+    rec, z, mu, s, dt = _gbm("milstein")
+    x = np.ones(z.shape[1])
+    W = np.zeros_like(x)
+    for t in range(z.shape[0]):
+        np.testing.assert_allclose(rec[t], x, rtol=1e-13)
+        dW = np.sqrt(dt) * z[t]
+        xs = x + mu * x * dt + s * x * np.sqrt(dt)
+        x = x + mu * x * dt + s * x * dW + (s * xs - s * x) * (dW * dW - dt) / (2 * np.sqrt(dt))
+        W = W + dW
+    exact = np.exp((mu - s * s / 2) * dt * z.shape[0] + s * W)
+    assert np.max(np.abs(x / exact - 1)) < 5e-2  # strong order 1; s^2 t = 1.8 here
+
+
+def test_second_noise_variable_draws():
+    """dx/dt = xi_1/sqrt(ms), dy/dt = xi_2/sqrt(ms) (euler): x follows the group's
+    own draws, y the draws of noise variable 1 (the stream's noise-variable
+    class, hs_cg_rnd_stream(stream, 1, 2))."""
+    dt, seed, stream = 1e-4, 23, 12
+    m, names = cg.neuron_model("dx/dt = xi_1/sqrt(ms) : 1\ndy/dt = xi_2/sqrt(ms) : 1", None, None, 0.0, "euler", {},
+                               dt=dt, stream=stream)
+    spec = nw.NetSpec("xi2", dt, seed, [nw.GroupSpec(0, 8, [], {}, code=m)], [])
+    ref, _, _ = oracle.build_ref(spec, 64, oracle.SEQ)
+    ref.run(3)
+    sq = np.sqrt(1e3)
+    for var, st in ((0, stream), (1, _rnd_stream(stream, 1, 2))):
+        want = np.zeros(8)
+        for t in range(3):
+            want = want + np.array([oracle.randn(seed, st, i, t) for i in range(8)]) * sq * np.sqrt(dt)
+        np.testing.assert_allclose(ref.get_state(0, var), want, rtol=1e-13)

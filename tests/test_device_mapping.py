@@ -157,7 +157,7 @@ def test_poisson_and_spike_generator_groups():
     (dict(equations="dv/dt = -v/tau : volt"), "no value for 'tau'"),
     (dict(equations="dv/dt = erf(v)/taum : volt"), "function 'erf'"),
     (dict(reset="v = Vr; w += 1"), "'w' is not a state variable"),
-    (dict(method="heun"), "method 'heun'"),
+    (dict(method="gsl"), "method 'gsl'"),
     (dict(namespace={k: v for k, v in CUBA_NS.items() if k != "taum"}), "taum"),
 ])
 def test_unsupported_groups_are_refused(change, msg):

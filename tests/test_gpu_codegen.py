@@ -380,3 +380,26 @@ def test_rand_reference_test_rand_gpu(hs):
     v = net.state_monitor(st[0], 1000)
     assert v.shape == (3, 1000) and not np.all(v[-1] == 0) and not np.array_equal(v[-2], v[-1])
     net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine,sem,opts", FORMS)
+@pytest.mark.parametrize("method", ["heun", "milstein"])
+def test_stochastic_methods_vs_oracle(hs, oracle_lib, prec, engine, sem, opts, method):
+    """heun (two noise variables, one multiplicative) and milstein
+    (multiplicative noise) on a spiking LIF: the engine's xi, the second
+    variable's own draw, the support values and the corrections give the
+    oracle's bits in every form and precision."""
+    if method == "heun":
+        eqs = ("dv/dt = (I - v)/tau + s1*(1 + v/(20*mV))*xi_1*mV/sqrt(ms) + s2*xi_2*mV/sqrt(ms) : volt")
+    else:
+        eqs = "dv/dt = (I - v)/tau + s1*(1 + v/(20*mV))*xi*mV/sqrt(ms) : volt"
+    model, names = cg.neuron_model(eqs, "v > 10*mV", "v = 0*mV", 2e-3, method,
+                                   {"I": 11e-3, "tau": 10e-3, "s1": 0.5, "s2": 0.3}, dt=1e-4, stream=6)
+    assert model.noise
+    n = 2500
+    init = {0: 10e-3 * np.random.default_rng(12).random(n)}
+    spec = nw.NetSpec(f"sde_{method}", 1e-4, 31, [nw.GroupSpec(0, n, [], init, code=model)], [])
+    spec.spike_monitors = [0]
+    spec.state_monitors = [(0, 0, 0, 40)]
+    _compare(spec, prec, engine, sem, opts, oracle_lib, (150, 350))
