@@ -31,7 +31,9 @@ log, log10, sqrt, abs, clip, floor, ceil, sign, sin, cos, tan, sinh, cosh,
 tanh, arcsin, arccos, arctan, expm1, log1p, exprel (evaluated by the library's
 deterministic implementations on the device and in the oracle alike), and
 rand() / randn() in euler updates, thresholds and resets (each call site its
-own Philox draw per neuron and step, hs_rand_u(k) / hs_rand_n(k)).
+own Philox draw per neuron and step, hs_rand_u(k) / hs_rand_n(k)) and in
+synaptic code (per synapse pair (i, j) and step, hs_srand_u(k) / hs_srand_n(k)).
+A comparison inside arithmetic is the number 1 or 0 (w*(rand() < p)).
 '(unless refractory)' holds
 a variable while the neuron is refractory.  Anything else raises
 NotImplementedError naming it.
@@ -104,6 +106,13 @@ class log1p(sp.Function):  # noqa: N801
 
 class exprel(sp.Function):  # noqa: N801
     nargs = 1
+
+
+class cmp_(sp.Function):  # noqa: N801 -- a comparison as a number (1 or 0): (a, b, operator index)
+    nargs = 3
+
+
+_CMP_OPS = [(ast.Lt, "<"), (ast.LtE, "<="), (ast.Gt, ">"), (ast.GtE, ">="), (ast.Eq, "=="), (ast.NotEq, "!=")]
 
 
 _FUNCS = {"exp": sp.exp, "log": sp.log, "sqrt": sp.sqrt, "abs": sp.Abs, "floor": sp.floor, "ceil": sp.ceiling,
@@ -209,6 +218,18 @@ class _Conv(ast.NodeVisitor):
             raise NotImplementedError(f"{self.what}: operator {type(n.op).__name__}")
         return f()
 
+    def visit_Compare(self, n):
+        """a < b as a number, 1 or 0 (Brian2's boolean in arithmetic: w*(rand() < p));
+        a chain a < b < c is the product of its links"""
+        out, left = sp.Integer(1), self.visit(n.left)
+        for o, right in zip(n.ops, n.comparators):
+            k = next((k for k, (t, _) in enumerate(_CMP_OPS) if isinstance(o, t)), None)
+            if k is None:
+                raise NotImplementedError(f"{self.what}: comparison {type(o).__name__}")
+            r = self.visit(right)
+            out, left = out * cmp_(left, r, sp.Integer(k)), r
+        return out
+
     def visit_Call(self, n):
         if isinstance(n.func, ast.Name) and n.func.id in ("rand", "randn") and not n.args and not n.keywords:
             return self.sym(n.func.id + "()")  # a draw: the caller's name resolution decides
@@ -292,6 +313,8 @@ class _Printer:
             return f"sign({self.p(e.args[0])})"
         if isinstance(e, clip):
             return "clip(" + ", ".join(self.p(a) for a in e.args) + ")"
+        if isinstance(e, cmp_):
+            return f"((R)({self.p(e.args[0])} {_CMP_OPS[int(e.args[2])][1]} {self.p(e.args[1])}))"
         if type(e) in _FUNC_NAMES:
             return f"{_FUNC_NAMES[type(e)]}({self.p(e.args[0])})"
         raise NotImplementedError(f"cannot print {e}")
@@ -722,7 +745,9 @@ def synapse_code(on_pre, syn_vars, post_vars, pre_vars, namespace=None, what="Sy
     dt, i, j and lastupdate; on_post may also read the target's variables
     (y_post); on_pre may not (other synapses of the step add to them).
     model: the Synapses model text; its event-driven equations advance their
-    variables from lastupdate to t before either statement list, exactly."""
+    variables from lastupdate to t before either statement list, exactly.
+    rand() / randn(): each call site (numbered through on_pre, then on_post) a
+    Philox draw keyed by the network seed, the pathway, i, j and the step."""
     ns = dict(namespace or {})
     syn_vars, post_vars, pre_vars = list(syn_vars), list(post_vars), list(pre_vars)
     ed = []
@@ -731,6 +756,7 @@ def synapse_code(on_pre, syn_vars, post_vars, pre_vars, namespace=None, what="Sy
         syn_vars += [n for n in mnames if n not in syn_vars]
     advance = _advance_code(ed, ns, what)
     used_post, used_pre, read_post = [], [], []
+    ndraw = {"u": 0, "n": 0}  # rand() / randn() call sites so far (on_pre, then on_post)
 
     def translate(text, post_reads):
         syms = {}
@@ -741,6 +767,12 @@ def synapse_code(on_pre, syn_vars, post_vars, pre_vars, namespace=None, what="Sy
             return syms[name]
 
         def sym(name):
+            if name in ("rand()", "randn()"):
+                kind = "u" if name == "rand()" else "n"
+                if ndraw[kind] >= 16:
+                    raise NotImplementedError(f"{what}: more than 16 {name} calls")
+                ndraw[kind] += 1
+                return S(f"_hs_rnd_{kind}{ndraw[kind] - 1}")
             if name in syn_vars:
                 return S(name)
             if name.endswith("_pre") and name[:-4] in pre_vars:
@@ -789,6 +821,7 @@ def synapse_code(on_pre, syn_vars, post_vars, pre_vars, namespace=None, what="Sy
             local.update({syms[k]: v for k, v in (("_hs_t", "((R)t)"), ("_hs_dt", "dt"), ("_hs_i", "((R)i)"),
                                                       ("_hs_j", "((R)j)"), ("_hs_lastupdate", "((R)lastupdate)"))
                           if k in syms})
+            local.update({s_: f"((R)hs_srand_{n[8]}({n[9:]}))" for n, s_ in syms.items() if n.startswith("_hs_rnd_")})
             out.append(f"{target} {op} {_Printer(local)(expr)};")
         return out
 

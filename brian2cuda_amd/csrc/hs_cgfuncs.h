@@ -44,4 +44,20 @@ HS_HD unsigned int hs_cg_rnd_stream(unsigned int stream, int k, int normal)
     return (stream & 0x00FFFFFFu) | ((unsigned int)(1 + k + 16 * normal) << 24);
 }
 
+/* the stream of rand() (normal = 0) or randn() (normal = 1) call site k < 16 of
+   pathway pid's synaptic code: the pathway in the low 24 bits, 64 + k + 16 normal
+   above (apart from every code-model stream, whose top byte is below 64) */
+HS_HD unsigned int hs_cg_syn_stream(unsigned int pid, int k, int normal)
+{
+    return (pid & 0x00FFFFFFu) | ((unsigned int)(64 + k + 16 * normal) << 24);
+}
+
+/* the step word of a synapse draw: target j in the high 32 bits, the step's low
+   32 bits below (so the Philox counter is (i, step_lo, j, stream); runs shorter
+   than 2^32 steps never repeat a counter) */
+HS_HD long long hs_cg_syn_ctr(long long step, int j)
+{
+    return (long long)(((unsigned long long)(unsigned int)j << 32) | (unsigned long long)(unsigned int)step);
+}
+
 #endif /* HS_CGFUNCS_H */

@@ -130,7 +130,8 @@ static int hs_cg_check_scope(const std::vector<std::string> &all, const std::str
         if (depth < 0) break;
     }
     if (depth != 0) return hs_rt_fail(HS_ERR_ARG, "synaptic code has unbalanced braces");
-    return hs_cg_reserved_ids(code, nullptr, 0, "synaptic code");
+    static const char *const allowed[] = {"hs_srand_u", "hs_srand_n"};
+    return hs_cg_reserved_ids(code, allowed, 2, "synaptic code");
 }
 
 int hs_cg_check_syn(const HsSynCgModel &m)
@@ -171,8 +172,9 @@ static void hs_cg_syn_struct(std::string &o, const char *R, int c, const HsSynCg
         for (int v = 0; v < npre; ++v) o += "        (void)" + m.pre_names[v] + "_pre;\n";
     };
     o += "    static __device__ __forceinline__ void on_pre(R *hs_sv, const R *hs_pre, R *hs_eff, R dt, double t, "
-         "int i, int j, double lastupdate)\n    {\n        (void)hs_sv; (void)hs_pre; (void)hs_eff; (void)dt; "
-         "(void)t; (void)i; (void)j; (void)lastupdate;\n";
+         "int i, int j, double lastupdate, unsigned long long hs_seed, unsigned int hs_path, long long hs_step)\n"
+         "    {\n        (void)hs_sv; (void)hs_pre; (void)hs_eff; (void)dt; (void)t; (void)i; (void)j; "
+         "(void)lastupdate; (void)hs_seed; (void)hs_path; (void)hs_step;\n";
     head();
     for (int v = 0; v < npost; ++v) o += "        R " + m.post_names[v] + "_post = (R)0;\n";
     o += "        {\n" + m.on_pre + "\n        }\n";
@@ -180,8 +182,9 @@ static void hs_cg_syn_struct(std::string &o, const char *R, int c, const HsSynCg
     for (int v = 0; v < npost; ++v) o += "        hs_eff[" + std::to_string(v) + "] = " + m.post_names[v] + "_post;\n";
     o += "    }\n";
     o += "    static __device__ __forceinline__ void on_post(R *hs_sv, const R *hs_pre, const R *hs_post, R dt, "
-         "double t, int i, int j, double lastupdate)\n    {\n        (void)hs_sv; (void)hs_pre; (void)hs_post; "
-         "(void)dt; (void)t; (void)i; (void)j; (void)lastupdate;\n";
+         "double t, int i, int j, double lastupdate, unsigned long long hs_seed, unsigned int hs_path, "
+         "long long hs_step)\n    {\n        (void)hs_sv; (void)hs_pre; (void)hs_post; (void)dt; (void)t; "
+         "(void)i; (void)j; (void)lastupdate; (void)hs_seed; (void)hs_path; (void)hs_step;\n";
     head();
     for (int v = 0; v < npr; ++v)
         o += "        const R " + m.post_read_names[v] + "_post = hs_post[" + std::to_string(v) + "];\n";
@@ -289,6 +292,10 @@ __device__ __forceinline__ auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo +
 #define hs_rand_n(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 1), (uint32_t)i, hs_step)
 /* noise variable k >= 1 of an equation with several (xi_1, xi_2, ...; k = 0 is xi) */
 #define hs_rand_xi(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 2), (uint32_t)i, hs_step)
+/* rand() / randn() call site k of synaptic code: a pure function of the
+   network's seed, the pathway, the synapse's (i, j) and the step */
+#define hs_srand_u(k) hs_rand(hs_seed, hs_cg_syn_stream(hs_path, k, 0), (uint32_t)i, hs_cg_syn_ctr(hs_step, j))
+#define hs_srand_n(k) hs_randn(hs_seed, hs_cg_syn_stream(hs_path, k, 1), (uint32_t)i, hs_cg_syn_ctr(hs_step, j))
 )HSCG";
 
 std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn)

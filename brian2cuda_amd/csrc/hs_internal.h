@@ -158,6 +158,9 @@ struct DPath {
     R *wm, *apre, *apost;
     int32_t *lastup;
     const int32_t *qstart, *qsyn;
+    /* synaptic code's rand() / randn(): the network's seed and the pathway's index */
+    unsigned long long seed;
+    unsigned int pid;
     double taupre, taupost, dt;
     R sk[5]; /* dApre, dApost, wmin, wmax, scale */
     const int32_t *thist, *thist_cnt;

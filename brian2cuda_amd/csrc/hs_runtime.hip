@@ -1955,6 +1955,8 @@ static int hs_upload_desc_t(hs_net *net)
         d.src = P.src;
         d.tgt = P.tgt;
         d.var = P.var;
+        d.pid = (unsigned int)pi;
+        d.seed = net->seed;
         d.n_src = gs.n;
         d.dmin = P.dmin;
         d.D = P.D;

@@ -66,7 +66,8 @@ __device__ __forceinline__ void hs_syn_code_path(const DPath<R> &p, const DGroup
                     for (int v = 0; v < NS; ++v) sv[v] = hs_ld(p.sv[v] + q);
 #pragma unroll
                     for (int v = 0; v < NPOST; ++v) eff[v] = (R)0;
-                    HsSynCg<R, C>::on_pre(sv, pre, eff, (R)p.dt, t, i, j, (double)hs_ld(p.lastup + q) * p.dt);
+                    HsSynCg<R, C>::on_pre(sv, pre, eff, (R)p.dt, t, i, j, (double)hs_ld(p.lastup + q) * p.dt, p.seed,
+                                          p.pid, T);
 #pragma unroll
                     for (int v = 0; v < NS; ++v) hs_st(p.sv[v] + q, sv[v]);
                     hs_st(p.lastup + q, (int32_t)T);
@@ -134,7 +135,8 @@ __device__ __forceinline__ void hs_syn_post_path(const DPath<R> &p, const DGroup
                     for (int v = 0; v < NS; ++v) sv[v] = hs_ld(p.sv[v] + q);
 #pragma unroll
                     for (int v = 0; v < NPRE; ++v) pre[v] = hs_ld(gs.var[p.pre_var[v]] + i);
-                    HsSynCg<R, C>::on_post(sv, pre, post, (R)p.dt, t, i, j, (double)hs_ld(p.lastup + q) * p.dt);
+                    HsSynCg<R, C>::on_post(sv, pre, post, (R)p.dt, t, i, j, (double)hs_ld(p.lastup + q) * p.dt,
+                                           p.seed, p.pid, T);
 #pragma unroll
                     for (int v = 0; v < NS; ++v) hs_st(p.sv[v] + q, sv[v]);
                     hs_st(p.lastup + q, (int32_t)T);

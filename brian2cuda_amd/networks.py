@@ -358,20 +358,31 @@ def code_cobahh(N=4000, seed=1234, method="exponential_euler", **kw):
     return spec
 
 
-def code_synapses(seed=3, npo=400, nl=60, p=0.2):
+def code_synapses(seed=3, npo=400, nl=60, p=0.2, release=False):
     """Poisson inputs onto code-model LIF neurons through synaptic code: each
     arrival adds 1.5 mV to ge, counts itself in 'hits' and moves w with the
     source's rate ('w = clip(w + 1e-3*rates_pre*w - 0.002, 0, 1)'); per-synapse
-    delays of 0 to 11 steps, w initially uniform."""
+    delays of 0 to 11 steps, w initially uniform.
+    release=True: stochastic release instead -- an arrival is released with
+    probability 0.5 ('released = rand() < 0.5'), adds 1.5 mV only then and counts
+    in 'nrel'; w takes a multiplicative normal step ('w = clip(w + 0.01*randn()*w,
+    0, 1)'), and each spike of the target lowers w by up to 0.002 (on_post
+    'w = clip(w - 0.002*rand(), 0, 1)')."""
     from .codegen import neuron_model, synapse_code
     rng = np.random.default_rng(seed)
     lif, names = neuron_model("dv/dt = (ge - v)/(10*ms) : volt\ndge/dt = -ge/(5*ms) : volt",
                               "v > 5*mV", "v = 0*volt", 2e-3, "euler", {})
     pre, post = bernoulli_connect(rng, npo, nl, p)
     d = rng.integers(0, 12, pre.size).astype(np.int32)
-    code = synapse_code("ge_post += 1.5*mV\nhits += 1\nw = clip(w + 1e-3*rates_pre*w - 0.002, 0, 1)",
-                        ["w", "hits"], names, ["rates"])
-    code.init = {"w": rng.random(pre.size), "hits": 0.0}
+    if release:
+        code = synapse_code("released = rand() < 0.5\nge_post += 1.5*mV*released\nhits += 1\nnrel += released\n"
+                            "w = clip(w + 0.01*randn()*w, 0, 1)", ["w", "hits", "released", "nrel"], names, [],
+                            on_post="w = clip(w - 0.002*rand(), 0, 1)")
+        code.init = {"w": rng.random(pre.size), "hits": 0.0, "released": 0.0, "nrel": 0.0}
+    else:
+        code = synapse_code("ge_post += 1.5*mV\nhits += 1\nw = clip(w + 1e-3*rates_pre*w - 0.002, 0, 1)",
+                            ["w", "hits"], names, ["rates"])
+        code.init = {"w": rng.random(pre.size), "hits": 0.0}
     rates = 5.0 + 30.0 * rng.random(npo)
     groups = [GroupSpec(HS_MODEL_POISSON, npo, [20.0, 2.0], {0: rates}), GroupSpec(0, nl, [], {}, code=lif)]
     spec = NetSpec("code_synapses", 0.1 * ms, seed, groups, [PathSpec(0, 1, 0, pre, post, delay_steps=d, code=code)])

@@ -156,6 +156,11 @@ def connect_fixed(pre_range, post_range, K, skip_self, seed, stream, delay, dt, 
     return out
 
 
+def rand(seed, stream, index, step):
+    """One uniform draw in [0, 1) (the same in both precisions)."""
+    return lib().ref_rand(seed, stream, index, step)
+
+
 def randn(seed, stream, index, step, precision=64):
     """One draw; precision 32 gives the FP32 form's draw (24-bit uniforms, float math)."""
     if precision == 32:
@@ -255,6 +260,8 @@ static inline auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo + hi)
 #define hs_rand_u(k) hs_rand(hs_seed, hs_cg_rnd_stream(hs_stream, k, 0), (uint32_t)i, hs_step)
 #define hs_rand_n(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 1), (uint32_t)i, hs_step)
 #define hs_rand_xi(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 2), (uint32_t)i, hs_step)
+#define hs_srand_u(k) hs_rand(hs_seed, hs_cg_syn_stream(hs_path, k, 0), (uint32_t)i, hs_cg_syn_ctr(hs_step, j))
+#define hs_srand_n(k) hs_randn(hs_seed, hs_cg_syn_stream(hs_path, k, 1), (uint32_t)i, hs_cg_syn_ctr(hs_step, j))
 """
 _cg_libs = {}
 
@@ -288,8 +295,9 @@ def synapse_code_source(code, precision):
     post = head + [f"    const R {n}_post = hs_post[{k}];" for k, n in enumerate(code.post_read_names)]
     post += ["    {", code.on_post or "", "    }"] + tail
     post += [f"    (void){n}_post;" for n in code.post_read_names]
-    args = "R dt, double t, int i, int j, double lastupdate"
-    unused = "(void)dt; (void)t; (void)i; (void)j; (void)lastupdate;"
+    args = ("R dt, double t, int i, int j, double lastupdate, unsigned long long hs_seed, unsigned int hs_path, "
+            "long long hs_step")
+    unused = "(void)dt; (void)t; (void)i; (void)j; (void)lastupdate; (void)hs_seed; (void)hs_path; (void)hs_step;"
     return (_CG_MATH + f"typedef {R} R;\n"
             f'extern "C" void hs_scg_on_pre(R *hs_sv, const R *hs_pre, R *hs_eff, {args})\n{{\n'
             f"    (void)hs_sv; (void)hs_pre; (void)hs_eff; {unused}\n" + "\n".join(pre) + "\n}\n"

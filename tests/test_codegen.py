@@ -432,3 +432,85 @@ def test_second_noise_variable_draws():
         for t in range(3):
             want = want + np.array([oracle.randn(seed, st, i, t) for i in range(8)]) * sq * np.sqrt(dt)
         np.testing.assert_allclose(ref.get_state(0, var), want, rtol=1e-13)
+
+
+# ---- rand() / randn() in synaptic code ----------------------------------------------
+def _syn_stream(pid, k, normal):
+    """the stream of synaptic call site k (hs_cg_syn_stream in hs_cgfuncs.h)"""
+    return (pid & 0xFFFFFF) | ((64 + k + 16 * normal) << 24)
+
+
+def test_synaptic_rand_known_answer():
+    """Three generator sources (spikes at steps 5, 5 and 40) onto four generator
+    targets (spikes at step 20), all to all: on_pre 'x = rand(); y = randn()',
+    on_post 'z = rand()' (the second rand() call site, k = 1).  Each value is the
+    draw of its call site for (seed, pathway, i, j, step): Philox counter
+    (i, step, j, stream) -- the same keys the device uses."""
+    G = nw.GroupSpec(nw.HS_MODEL_SPIKEGEN, 3, [0.0], spikes=(np.array([0, 1, 2], np.int32), np.array([5, 5, 40])))
+    H = nw.GroupSpec(nw.HS_MODEL_SPIKEGEN, 4, [0.0], spikes=(np.arange(4, dtype=np.int32), np.full(4, 20)))
+    code = cg.synapse_code("x = rand()\ny = randn()", ["x", "y", "z"], ["_placeholder"], [], on_post="z = rand()")
+    assert "hs_srand_u(0)" in code.on_pre and "hs_srand_n(0)" in code.on_pre and "hs_srand_u(1)" in code.on_post
+    pre, post = np.repeat(np.arange(3), 4).astype(np.int32), np.tile(np.arange(4), 3).astype(np.int32)
+    seed = 77
+    spec = nw.NetSpec("syn_rand", DT, seed, [G, H], [nw.PathSpec(0, 1, 0, pre, post, code=code)])
+    for prec in (64, 32):
+        ref, _, _ = oracle.build_ref(spec, prec, oracle.SEQ)
+        ref.run(60)
+        got = {n: ref.synapse_var(0, nw.HS_SYN_CODE + k, 12) for k, n in enumerate(code.syn_names)}
+        f = np.float64 if prec == 64 else np.float32
+        for s, (i, j) in enumerate(zip(pre, post)):
+            step = 5 if i < 2 else 40
+            ctr = (int(j) << 32) | step
+            assert got["x"][s] == f(oracle.rand(seed, _syn_stream(0, 0, 0), int(i), ctr))
+            assert got["y"][s] == f(oracle.randn(seed, _syn_stream(0, 0, 1), int(i), ctr))
+            assert got["z"][s] == f(oracle.rand(seed, _syn_stream(0, 1, 0), int(i), (int(j) << 32) | 20))
+
+
+def test_stochastic_release_statistics():
+    """code_synapses(release=True): 'released = rand() < 0.5' releases half the
+    arrivals (within 4 sigma of the binomial count), every released arrival and
+    only those adds to ge (the target's summed input equals 1.5 mV per release),
+    the multiplicative randn() step keeps w in [0, 1], and two call sites of one
+    statement list draw independently (released differs from a fresh draw)."""
+    spec = nw.code_synapses(release=True)
+    p = spec.paths[0]
+    assert "hs_srand_u(0)" in p.code.on_pre and "hs_srand_n(0)" in p.code.on_pre
+    ref, sm, _ = oracle.build_ref(spec, 64, oracle.SEQ)
+    ref.run(2000)
+    n = len(p.pre)
+    hits = ref.synapse_var(0, nw.HS_SYN_CODE + 1, n)
+    nrel = ref.synapse_var(0, nw.HS_SYN_CODE + 3, n)
+    w = ref.synapse_var(0, nw.HS_SYN_CODE + 0, n)
+    H, K = hits.sum(), nrel.sum()
+    assert H == ref.events > 10000
+    assert abs(K - 0.5 * H) < 4 * np.sqrt(0.25 * H)
+    assert np.all(nrel <= hits) and np.all((w >= 0) & (w <= 1))
+    assert len(ref.spikes(sm[0])[0]) > 20  # the released input drives the targets
+    # the same network with every arrival released moves w the same way but
+    # drives the targets harder
+    code = cg.synapse_code("ge_post += 1.5*mV\nhits += 1", ["w", "hits"], ["v", "ge"], [])
+    code.init = {"w": 0.5, "hits": 0.0}
+    spec2 = nw.code_synapses(release=True)
+    spec2.paths[0].code = code
+    r2, sm2, _ = oracle.build_ref(spec2, 64, oracle.SEQ)
+    r2.run(2000)
+    assert len(r2.spikes(sm2[0])[0]) > len(ref.spikes(sm[0])[0])
+
+
+def test_comparisons_as_numbers():
+    """'a < b' in arithmetic is 1 or 0 (Brian2's booleans in expressions), a
+    chain is the product of its links; in neuron models too."""
+    code = cg.synapse_code("w += (i < j) + 2*(1 < j <= 3) + (i == j)", ["w"], ["v"], [])
+    G = nw.GroupSpec(nw.HS_MODEL_SPIKEGEN, 5, [0.0], spikes=(np.arange(5, dtype=np.int32), np.full(5, 2)))
+    H = nw.GroupSpec(nw.HS_MODEL_SPIKEGEN, 5, [0.0])
+    pre, post = np.repeat(np.arange(5), 5).astype(np.int32), np.tile(np.arange(5), 5).astype(np.int32)
+    code.post_names = []
+    spec = nw.NetSpec("cmp", DT, 1, [G, H], [nw.PathSpec(0, 1, 0, pre, post, code=code)])
+    ref, _, _ = oracle.build_ref(spec, 64, oracle.SEQ)
+    ref.run(5)
+    want = [(i < j) + 2 * (1 < j <= 3) + (i == j) for i, j in zip(pre, post)]
+    assert np.array_equal(ref.synapse_var(0, nw.HS_SYN_CODE, 25), np.array(want, float))
+    m, _ = cg.neuron_model("dv/dt = (v < 0.5)/(10*ms) : 1", None, None, 0.0, "euler", {}, dt=DT)
+    assert "<" in m.update
+    with pytest.raises(NotImplementedError, match="comparison"):
+        cg.synapse_code("w += (i in j)", ["w"], ["v"], [])

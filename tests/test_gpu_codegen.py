@@ -403,3 +403,33 @@ def test_stochastic_methods_vs_oracle(hs, oracle_lib, prec, engine, sem, opts, m
     spec.spike_monitors = [0]
     spec.state_monitors = [(0, 0, 0, 40)]
     _compare(spec, prec, engine, sem, opts, oracle_lib, (150, 350))
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("engine,sem", [("plain", 0), ("fused", 1)])
+def test_stochastic_release_vs_oracle(hs, oracle_lib, prec, engine, sem):
+    """rand() and randn() in synaptic code (networks.code_synapses(release=True)):
+    'released = rand() < 0.5' gates each arrival's 1.5 mV, w takes a randn() step,
+    on_post draws too; each call site a Philox draw keyed by seed, pathway, i, j
+    and step, the same on the device and in the oracle: spikes, every synapse
+    variable and the target state bit for bit."""
+    spec = nw.code_synapses(release=True)
+    p = spec.paths[0]
+    print(f"stochastic release prec {prec} engine {engine}", flush=True)
+    net, gsm, _ = nw.build_hs(spec, prec, engine)
+    ref, rsm, _ = oracle_lib.build_ref(spec, prec, sem)
+    for k in (300, 900):
+        net.run(k)
+        ref.run(k)
+    gi, gt = net.spikes(gsm[0])
+    ri, rt_ = ref.spikes(rsm[0])
+    assert len(gi) == len(ri) > 0 and np.array_equal(gi, ri) and np.array_equal(gt, rt_)
+    assert net.stats()["events"] == ref.events
+    order = nw.bundle_order(p)
+    for k in range(4):
+        g = net.synapse_var(0, rt.HS_SYN_CODE + k)
+        r = ref.synapse_var(0, rt.HS_SYN_CODE + k, len(p.pre))[order]
+        assert np.array_equal(g, r), k
+    for v in range(2):
+        assert np.array_equal(net.get_state(1, v), ref.get_state(1, v))
+    net.close()
