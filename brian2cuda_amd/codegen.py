@@ -842,7 +842,7 @@ _CONN_PY = {"exp": np.exp, "log": np.log, "log10": np.log10, "sqrt": np.sqrt, "a
             "expm1": np.expm1, "log1p": np.log1p, "exprel": lambda x: np.expm1(x) / x if x != 0 else 1.0}
 
 
-def connect_expression(expr, namespace=None, pre_names=(), post_names=(), what="connect"):
+def connect_expression(expr, namespace=None, pre_names=(), post_names=(), what="connect", first_draw=0):
     """A Brian2 connect condition or probability -> (C expression, constant).
 
     The expression is Brian2's (Python's) over i and j (indices within the
@@ -856,10 +856,14 @@ def connect_expression(expr, namespace=None, pre_names=(), post_names=(), what="
     hs_cg_pymod), a non-literal power is hs_cg_pow.  The C text is that of
     hs_net_connect_code (variables as hs_pre(k) / hs_post(k)).  constant is the
     expression's value when it reads neither an index nor a variable (else
-    None), so a constant probability can take the geometric skips.  rand(),
-    randn() and anything else raise NotImplementedError naming it.
+    None), so a constant probability can take the geometric skips.  rand() and
+    randn() are draws keyed by the connect call's seed and stream and the pair
+    (hs_crand_u(k) / hs_crand_n(k)), call sites numbered from first_draw (the
+    draws of a condition, when the probability expression of the same call
+    also draws); anything else raises NotImplementedError naming it.
     """
     ns = dict(namespace or {})
+    ndraw = [int(first_draw)]
     pre_idx = {n: k for k, n in enumerate(pre_names)}
     post_idx = {n: k for k, n in enumerate(post_names)}
     try:
@@ -983,7 +987,12 @@ def connect_expression(expr, namespace=None, pre_names=(), post_names=(), what="
                 raise NotImplementedError(f"{what}: call {ast.unparse(n)!r} is not supported")
             fn = n.func.id
             if fn in ("rand", "randn"):
-                raise NotImplementedError(f"{what}: {fn}() in a connect expression is not supported (use p=)")
+                if n.args:
+                    raise NotImplementedError(f"{what}: {fn}() takes no arguments")
+                if ndraw[0] >= 16:
+                    raise NotImplementedError(f"{what}: more than 16 rand() / randn() calls")
+                ndraw[0] += 1
+                return f"((double)hs_crand_{'u' if fn == 'rand' else 'n'}({ndraw[0] - 1}))", "float", None
             if fn not in _CONN_FUNCS:
                 raise NotImplementedError(f"{what}: function {fn!r} is not supported")
             args = [as_num(c(x)) for x in n.args]

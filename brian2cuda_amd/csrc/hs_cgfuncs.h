@@ -60,4 +60,19 @@ HS_HD long long hs_cg_syn_ctr(long long step, int j)
     return (long long)(((unsigned long long)(unsigned int)j << 32) | (unsigned long long)(unsigned int)step);
 }
 
+/* the stream of rand() (normal = 0) or randn() (normal = 1) call site k < 16 of
+   a connect condition or probability: the connect call's stream in the low 24
+   bits, 96 + k + 16 normal above */
+HS_HD unsigned int hs_cg_conn_stream(unsigned int stream, int k, int normal)
+{
+    return (stream & 0x00FFFFFFu) | ((unsigned int)(96 + k + 16 * normal) << 24);
+}
+
+/* the step word of a connect-code draw: j in the high 32 bits over the tag
+   0x43524E44 ("CRND") */
+HS_HD long long hs_cg_conn_ctr(long long j)
+{
+    return (long long)(((unsigned long long)(unsigned int)j << 32) | 0x43524E44ull);
+}
+
 #endif /* HS_CGFUNCS_H */

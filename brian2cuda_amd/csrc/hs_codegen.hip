@@ -296,6 +296,11 @@ __device__ __forceinline__ auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo +
    network's seed, the pathway, the synapse's (i, j) and the step */
 #define hs_srand_u(k) hs_rand(hs_seed, hs_cg_syn_stream(hs_path, k, 0), (uint32_t)i, hs_cg_syn_ctr(hs_step, j))
 #define hs_srand_n(k) hs_randn(hs_seed, hs_cg_syn_stream(hs_path, k, 1), (uint32_t)i, hs_cg_syn_ctr(hs_step, j))
+/* rand() / randn() call site k of a connect condition or probability: a pure
+   function of the connect call's seed and stream and the pair (i, j) relative to
+   the ranges, so both connect passes see the same draw */
+#define hs_crand_u(k) hs_rand(hs_seed, hs_cg_conn_stream(hs_cstream, k, 0), (uint32_t)i, hs_cg_conn_ctr(j))
+#define hs_crand_n(k) hs_randn(hs_seed, hs_cg_conn_stream(hs_cstream, k, 1), (uint32_t)i, hs_cg_conn_ctr(j))
 )HSCG";
 
 std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn)
@@ -457,8 +462,9 @@ int hs_cg_conn_check(const std::string &cond, const std::string &p)
         if (depth != 0) return hs_rt_fail(HS_ERR_ARG, "a connect expression has unbalanced parentheses");
         /* the generated function's own names (hs_a, hs_c, hs_ai, ...) are not the
            expression's: of the hs_ names only the documented functions */
-        static const char *const allowed[] = {"hs_pre", "hs_post", "hs_cg_floordiv", "hs_cg_pymod", "hs_cg_pow"};
-        HS_CG_TRY(hs_cg_reserved_ids(*e, allowed, 5, "connect expression"));
+        static const char *const allowed[] = {"hs_pre",      "hs_post",   "hs_cg_floordiv", "hs_cg_pymod",
+                                              "hs_cg_pow",   "hs_crand_u", "hs_crand_n"};
+        HS_CG_TRY(hs_cg_reserved_ids(*e, allowed, 7, "connect expression"));
     }
     return HS_OK;
 }
@@ -479,7 +485,9 @@ static std::string hs_cg_conn_source(const std::string &cond, const std::string 
         "        const long long i = hs_ai - hs_c.pre_lo, j = hs_aj - hs_c.post_lo;\n"
         "        const long long N_pre = (long long)hs_c.pre_hi - hs_c.pre_lo, N_post = (long long)hs_c.post_hi - "
         "hs_c.post_lo;\n"
-        "        (void)hs_a; (void)i; (void)j; (void)N_pre; (void)N_post;\n";
+        "        const unsigned long long hs_seed = hs_c.seed;\n"
+        "        const unsigned int hs_cstream = hs_c.stream;\n"
+        "        (void)hs_a; (void)i; (void)j; (void)N_pre; (void)N_post; (void)hs_seed; (void)hs_cstream;\n";
     o += "    template <typename R>\n    static __device__ __forceinline__ bool cond" + std::string(head);
     o += "        return (bool)(" + (cond.empty() ? std::string("true") : cond) + ");\n    }\n";
     o += "    template <typename R>\n    static __device__ __forceinline__ double p" + std::string(head);

@@ -507,7 +507,8 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
                         pnum = 0.0  # a condition that never holds
             if isinstance(pnum, str):
                 from .codegen import connect_expression
-                p_c, const = connect_expression(pnum, s.namespace, gvars[src], gvars[tgt], what)
+                p_c, const = connect_expression(pnum, s.namespace, gvars[src], gvars[tgt], what,
+                                                first_draw=cond_c.count("hs_crand_") if cond_c else 0)
                 if const is not None:
                     p_c, pnum = "", min(1.0, max(0.0, float(const)))
                 else:

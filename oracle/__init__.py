@@ -262,6 +262,8 @@ static inline auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo + hi)
 #define hs_rand_xi(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 2), (uint32_t)i, hs_step)
 #define hs_srand_u(k) hs_rand(hs_seed, hs_cg_syn_stream(hs_path, k, 0), (uint32_t)i, hs_cg_syn_ctr(hs_step, j))
 #define hs_srand_n(k) hs_randn(hs_seed, hs_cg_syn_stream(hs_path, k, 1), (uint32_t)i, hs_cg_syn_ctr(hs_step, j))
+#define hs_crand_u(k) hs_rand(hs_seed, hs_cg_conn_stream(hs_cstream, k, 0), (uint32_t)i, hs_cg_conn_ctr(j))
+#define hs_crand_n(k) hs_randn(hs_seed, hs_cg_conn_stream(hs_cstream, k, 1), (uint32_t)i, hs_cg_conn_ctr(j))
 """
 _cg_libs = {}
 
@@ -335,10 +337,11 @@ def connect_code_source(cond, p_expr, precision):
     the groups' state arrays, the expressions of the device program."""
     R = "double" if precision == 64 else "float"
     args = ("(const void *const *hs_pre_v, const void *const *hs_post_v, long long hs_ai, long long hs_aj, "
-            "long long pre_lo, long long pre_hi, long long post_lo, long long post_hi)")
+            "long long pre_lo, long long pre_hi, long long post_lo, long long post_hi, unsigned long long hs_seed, "
+            "unsigned int hs_cstream)")
     head = ("{\n    const long long i = hs_ai - pre_lo, j = hs_aj - post_lo, N_pre = pre_hi - pre_lo, "
             "N_post = post_hi - post_lo;\n    (void)hs_pre_v; (void)hs_post_v; (void)i; (void)j; (void)N_pre; "
-            "(void)N_post;\n")
+            "(void)N_post; (void)hs_seed; (void)hs_cstream;\n")
     return ('#include "hs_cgconn.h"\n' + _CG_MATH + f"typedef {R} R;\n"
             "#define hs_pre(k) ((double)((const R *)hs_pre_v[k])[hs_ai])\n"
             "#define hs_post(k) ((double)((const R *)hs_post_v[k])[hs_aj])\n"

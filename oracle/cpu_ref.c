@@ -136,20 +136,25 @@ static void ref_conn_words(uint64_t seed, uint32_t stream, uint32_t i, uint32_t 
    program, called with the absolute indices (ai, aj) and the ranges, and the
    source / target state arrays */
 typedef int (*ref_conn_cond_fn)(const void *const *pre_vars, const void *const *post_vars, long long ai, long long aj,
-                                long long pre_lo, long long pre_hi, long long post_lo, long long post_hi);
+                                long long pre_lo, long long pre_hi, long long post_lo, long long post_hi,
+                                uint64_t seed, uint32_t stream);
 typedef double (*ref_conn_p_fn)(const void *const *pre_vars, const void *const *post_vars, long long ai, long long aj,
-                                long long pre_lo, long long pre_hi, long long post_lo, long long post_hi);
+                                long long pre_lo, long long pre_hi, long long post_lo, long long post_hi,
+                                uint64_t seed, uint32_t stream);
 struct ref_conn_code {
     ref_conn_cond_fn cond; /* NULL: none */
     ref_conn_p_fn p;       /* NULL: the constant p */
     const void *const *pre_vars, *const *post_vars;
     int pre_lo, pre_hi, post_lo, post_hi;
+    uint64_t seed; /* the connect call's: keys of rand() / randn() in the code */
+    uint32_t stream;
 };
 
 static int ref_conn_keep(const struct ref_conn_code *cc, int i, long long j)
 {
     return !cc || !cc->cond ||
-           cc->cond(cc->pre_vars, cc->post_vars, i, j, cc->pre_lo, cc->pre_hi, cc->post_lo, cc->post_hi) != 0;
+           cc->cond(cc->pre_vars, cc->post_vars, i, j, cc->pre_lo, cc->pre_hi, cc->post_lo, cc->post_hi, cc->seed,
+                    cc->stream) != 0;
 }
 
 static int64_t ref_connect_random_c(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int skip_self,
@@ -181,7 +186,8 @@ static int64_t ref_connect_random_c(int pre_lo, int pre_hi, int post_lo, int pos
                 if ((m & 1u) == 0) ref_conn_words(seed, stream, (uint32_t)i, m / 2, 2, words);
                 const double r = (m & 1u) ? ref_u53(words[2], words[3]) : ref_u53(words[0], words[1]);
                 const double pp =
-                    pexpr ? cc->p(cc->pre_vars, cc->post_vars, i, j, cc->pre_lo, cc->pre_hi, cc->post_lo, cc->post_hi)
+                    pexpr ? cc->p(cc->pre_vars, cc->post_vars, i, j, cc->pre_lo, cc->pre_hi, cc->post_lo, cc->post_hi,
+                                  cc->seed, cc->stream)
                           : p;
                 if (!(r < pp)) {
                     ++j;
@@ -313,6 +319,8 @@ int64_t ref_connect_code(int pre_lo, int pre_hi, int post_lo, int post_hi, doubl
     cc.pre_hi = pre_hi;
     cc.post_lo = post_lo;
     cc.post_hi = post_hi;
+    cc.seed = seed;
+    cc.stream = stream;
     if (K > 0)
         return ref_connect_fixed_c(pre_lo, pre_hi, post_lo, post_hi, K, skip_self, seed, stream, delay_lo, delay_hi,
                                    dt, w_lo, w_hi, &cc, pre, post, delay_steps, w, cap);

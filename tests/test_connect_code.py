@@ -13,6 +13,11 @@ Pins:
     0.25, one draw per pair above); a fixed out-degree likewise filters
     connect_fixed; a probability expression that is constant per pair draws
     what connect_random draws for that constant;
+  * rand() / randn() in a condition or probability are the draws of their call
+    site for the pair (seed, stream, i, j): a known answer against the oracle's
+    Philox words, and brian2cuda's
+    test_random_values_init_synapses_fixed_and_random_seed
+    (brian2cuda/tests/test_random_number_generation.py:563-600);
   * constants fold (a probability that reads no index is passed as a number),
     and what the translator does not know is refused with its name.
 """
@@ -147,9 +152,50 @@ def test_constant_folding_and_skip_self():
     assert _pairs(out) == {(i, j) for i in range(20) for j in range(20) if i != j}
 
 
-@pytest.mark.parametrize("expr,msg", [("rand() < 0.1", "rand"), ("foo(i) > 1", "foo"), ("i < unknown", "unknown"),
+@pytest.mark.parametrize("expr,msg", [("rand(1) < 0.1", "no arguments"), ("foo(i) > 1", "foo"), ("i < unknown", "unknown"),
                                       ("i in [1, 2]", "comparison"), ("[i for i in x]", "not supported"),
                                       ("i / (3 - 3) > 1", "division by zero")])
 def test_refusals(expr, msg):
     with pytest.raises(NotImplementedError, match=msg):
         cg.connect_expression(expr)
+
+
+def _conn_stream(stream, k, normal):
+    """the stream of connect call site k (hs_cg_conn_stream in hs_cgfuncs.h)"""
+    return (stream & 0xFFFFFF) | ((96 + k + 16 * normal) << 24)
+
+
+def test_rand_in_condition_known_answer():
+    """'rand() < 0.3 and randn() > -1' at p = 1: the pairs whose call-site
+    draws, Philox counter (i, tag, j, stream), satisfy it -- about
+    0.3 * 0.84 of the pairs -- and a probability expression drawing after the
+    condition (first_draw) takes the next call site."""
+    c, const = cg.connect_expression("rand() < 0.3 and randn() > -1")
+    assert const is None and "hs_crand_u(0)" in c and "hs_crand_n(1)" in c
+    seed, stream, tag = 11, 0, 0x43524E44
+    out = _code(c)
+    want = {(i, j) for i in range(40) for j in range(30)
+            if oracle.rand(seed, _conn_stream(stream, 0, 0), i, (j << 32) | tag) < 0.3
+            and oracle.randn(seed, _conn_stream(stream, 1, 1), i, (j << 32) | tag) > -1}
+    assert _pairs(out) == want and 200 < len(want) < 400
+    pe, _ = cg.connect_expression("0.5 * (rand() < 0.5) + 0.5", first_draw=2)
+    assert "hs_crand_u(2)" in pe
+    out = _code(c, pe)
+    assert _pairs(out) <= want and len(_pairs(out)) > 0.5 * len(want)
+
+
+def test_random_values_init_synapses_fixed_and_random_seed():
+    """brian2cuda's test (test_random_number_generation.py:563-600): four
+    connect('randn() + 0.5 < rand()') over 10 x 10, two with fresh seeds and two
+    with seed 12345678: the two seeded calls give equal synapses, every other
+    pair of calls different ones."""
+    c, _ = cg.connect_expression("randn() + 0.5 < rand()")
+    fresh = np.random.SeedSequence().generate_state(2, np.uint64)
+    seeds = [int(fresh[0]), 12345678, int(fresh[1]), 12345678]
+    idx = []
+    for s in seeds:
+        out = _code(c, pre=(0, 10), post=(0, 10), seed=s)
+        idx.append(np.hstack([out["pre"], out["post"]]))
+    same = lambda a, b: a.shape == b.shape and np.array_equal(a, b)  # noqa: E731
+    assert same(idx[1], idx[3])
+    assert not same(idx[0], idx[1]) and not same(idx[0], idx[2]) and not same(idx[1], idx[2])

@@ -170,7 +170,7 @@ def test_unsupported_groups_are_refused(change, msg):
 @pytest.mark.parametrize("on_pre,kw,msg", [
     ("w += 1", {}, "not a synapse or target variable"),
     ("w += v_post", dict(model="w : 1"), "reads the target variable"),
-    ("ge += we", dict(p=0.1, condition="rand() < 0.5"), "rand"),
+    ("ge += we", dict(p=0.1, condition="rand(3) < 0.5"), "no arguments"),
     ("ge += we", dict(p="0.1 * unknown_name"), "unknown_name"),
     ("ge += w", dict(p=0.1, values={"w": 1.0}), "'lo \\+ \\(hi-lo\\)\\*rand\\(\\)'"),
 ])

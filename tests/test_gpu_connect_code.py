@@ -36,6 +36,8 @@ CASES = [
      (1.0, 2.0)),
     (900, 900, None, None, "", "0.3 * ((i // 7) % 2) + 0.1 * (j > N_post / 2)", 0, True, (0.0, 0.0),
      rt.HS_W_SCALAR, (1.0, 1.0)),
+    (1600, 1400, None, (100, 1300), "rand() < 0.4 and randn() > -0.5", "0.5 + 0.5 * (rand() < 0.6)", 0, True,
+     (0.1e-3, 2e-3), rt.HS_W_SYNAPSE, (0.0, 1.0)),  # draws in both expressions
     (800, 800, None, None, "i > N_pre", 0.5, 0, True, (0.0, 0.0), rt.HS_W_SCALAR, (1.0, 1.0)),  # empty
 ]
 
@@ -61,7 +63,7 @@ def test_connect_code_matches_oracle(hs, oracle_lib, prec, case):
     tgt = 0 if n_src == n_tgt else 1
     c, cconst = cg.connect_expression(cond, {}, NAMES, NAMES) if cond else ("", None)
     if isinstance(p, str):
-        pe, pconst = cg.connect_expression(p, {}, NAMES, NAMES)
+        pe, pconst = cg.connect_expression(p, {}, NAMES, NAMES, first_draw=c.count("hs_crand_"))
         assert pconst is None
         pnum = 1.0
     else:
@@ -193,4 +195,24 @@ def test_default_function_implementations(hs, prec):
         assert const is None
         pid = net.connect_code(0, 0, 1, c, "", 1.0)
         assert net.pathway_info(pid)["nsyn"] == 1, cond
+    net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_random_values_init_synapses_fixed_and_random_seed(hs, prec):
+    """brian2cuda's test (tests/test_random_number_generation.py:563-600) on the
+    device: four connect('randn() + 0.5 < rand()') over a 10-neuron group, two
+    with fresh seeds and two with seed 12345678: the seeded pair gives equal
+    synapses, every other pair different ones."""
+    c, _ = cg.connect_expression("randn() + 0.5 < rand()")
+    fresh = np.random.SeedSequence().generate_state(2, np.uint64)
+    net, _ = _setup(prec, 10, 10)
+    idx = []
+    for s in (int(fresh[0]), 12345678, int(fresh[1]), 12345678):
+        pid = net.connect_code(0, 0, 1, c, "", 1.0, 0, None, None, True, s, 0)
+        pre, post, _, _ = net.synapses(pid) if net.pathway_info(pid)["nsyn"] else (np.zeros(0),) * 4
+        idx.append(np.hstack([pre, post]))
+    same = lambda a, b: a.shape == b.shape and np.array_equal(a, b)  # noqa: E731
+    assert same(idx[1], idx[3]) and idx[1].size > 0
+    assert not same(idx[0], idx[1]) and not same(idx[0], idx[2]) and not same(idx[1], idx[2])
     net.close()
