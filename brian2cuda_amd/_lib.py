@@ -104,7 +104,8 @@ SIGNATURES = {
                                         C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_int,
                                         C.c_double, C.c_double, C.POINTER(C.c_int)]),
     "hs_net_connect_code": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                       C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int, C.c_uint64, C.c_uint32,
+                                       C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int, C.c_int, C.c_uint64,
+                                       C.c_uint32,
                                        C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int)]),
     "hs_net_add_pathway_device": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int64, P, P, P, C.c_int, C.c_int, P,
                                             C.c_double, C.POINTER(C.c_int)]),

@@ -222,10 +222,11 @@ __device__ __forceinline__ void hs_conn_count_body(const HsConnSpec &c, const Hs
                 hs_conn_batch<F, R>(c, a, i, jb, k0, done, sel, j);
             }
             if constexpr (F::COND) sel = sel && F::template cond<R>(c, a, i, j);
-            if (sel) {
+            const int ncp = c.ncopy > 1 ? c.ncopy : 1;
+            for (int cp = 0; sel && cp < ncp; ++cp) {
                 double ud = 0.0;
                 if (c.dspan != 0.0) {
-                    const hs_u32x4 x = hs_conn_words(c, (uint32_t)i, (uint32_t)j, HS_CONN_TAG_SYN);
+                    const hs_u32x4 x = hs_conn_words(c, (uint32_t)i, (uint32_t)j, HS_CONN_TAG_COPY(cp));
                     ud = hs_u53(x.v[0], x.v[1]);
                 }
                 const int s = hs_conn_slot(c, ud);
@@ -234,6 +235,73 @@ __device__ __forceinline__ void hs_conn_count_body(const HsConnSpec &c, const Hs
             }
         }
         for (int d = lane; d < c.D; d += HS_WAVE) cnt[row + d] = t[d];
+    }
+}
+
+/* write of one batch's selected pairs with c.ncopy > 1 synapses each (their
+   delays and weights from tags HS_CONN_TAG_COPY(cp)): positions within each
+   delay slot follow creation order, lane (pair) first, then copy -- per slot, a
+   lane's copies in that slot are counted and the counts prefix-summed over the
+   wave */
+template <typename R>
+__device__ __forceinline__ void hs_conn_write_copies(const HsConnSpec &c, int i, long long j, bool sel, bool draw,
+                                                     size_t row, const int32_t *__restrict__ bstart,
+                                                     int32_t *__restrict__ post, R *__restrict__ w, int *err,
+                                                     int32_t *q)
+{
+    const int lane = hs_lane();
+    int sl[HS_CONN_NMAX];
+    double uwv[HS_CONN_NMAX];
+#pragma unroll
+    for (int cp = 0; cp < HS_CONN_NMAX; ++cp) {
+        sl[cp] = -1;
+        uwv[cp] = 0.0;
+        if (sel && cp < c.ncopy) {
+            double ud = 0.0;
+            if (draw) {
+                const hs_u32x4 x = hs_conn_words(c, (uint32_t)i, (uint32_t)j, HS_CONN_TAG_COPY(cp));
+                ud = hs_u53(x.v[0], x.v[1]);
+                uwv[cp] = hs_u53(x.v[2], x.v[3]);
+            }
+            const int s = hs_conn_slot(c, ud);
+            if ((unsigned)s >= (unsigned)c.D) *err = 1;
+            else sl[cp] = s;
+        }
+    }
+    for (;;) {
+        int mine = -1;
+#pragma unroll
+        for (int cp = 0; cp < HS_CONN_NMAX; ++cp)
+            if (mine < 0 && sl[cp] >= 0) mine = sl[cp];
+        const unsigned long long act = __ballot(mine >= 0);
+        if (!act) break;
+        const int leader = __ffsll((long long)act) - 1;
+        const int s0 = __shfl(mine, leader, HS_WAVE);
+        int cnt = 0;
+#pragma unroll
+        for (int cp = 0; cp < HS_CONN_NMAX; ++cp) cnt += sl[cp] == s0 ? 1 : 0;
+        int incl = cnt;
+        for (int o = 1; o < HS_WAVE; o <<= 1) {
+            const int y = __shfl_up(incl, o, HS_WAVE);
+            if (lane >= o) incl += y;
+        }
+        const int total = __shfl(incl, HS_WAVE - 1, HS_WAVE);
+        int pos = q[s0] + incl - cnt;
+        const int end = bstart[row + s0 + 1];
+#pragma unroll
+        for (int cp = 0; cp < HS_CONN_NMAX; ++cp) {
+            if (sl[cp] == s0) {
+                if (pos >= end) {
+                    *err = 2;
+                } else {
+                    post[pos] = (int32_t)j;
+                    if (w) w[pos] = (R)(c.wlo + uwv[cp] * c.wspan);
+                }
+                ++pos;
+                sl[cp] = -1;
+            }
+        }
+        if (lane == leader) q[s0] += total;
     }
 }
 
@@ -275,6 +343,10 @@ __device__ __forceinline__ void hs_conn_write_body(const HsConnSpec &c, const Hs
                 hs_conn_batch<F, R>(c, a, i, jb, k0, done, sel, j);
             }
             if constexpr (F::COND) sel = sel && F::template cond<R>(c, a, i, j);
+            if (c.ncopy > 1) {
+                hs_conn_write_copies(c, i, j, sel, draw, row, bstart, post, w, err, q);
+                continue;
+            }
             double ud = 0.0, uw = 0.0;
             int s = -1;
             if (sel) {

@@ -196,6 +196,10 @@ struct DPath {
 #define HS_CONN_TAG_SYN 1u
 #define HS_CONN_TAG_PAIR 2u
 #define HS_CONN_TAG_FIX 3u
+/* connect(n=ncopy): copy c of a pair draws its delay and weight from tag
+   HS_CONN_TAG_SYN | c << 4 (copy 0 as a single synapse), c < HS_CONN_NMAX */
+#define HS_CONN_NMAX 16
+#define HS_CONN_TAG_COPY(c) (HS_CONN_TAG_SYN | ((uint32_t)(c) << 4))
 /* fixed out-degree (connect(j='k for k in sample(N_post, size=K)')): row i's draw
    c (c = 0, 1, ...) is the 53-bit uniform u of words 0-1 of counter
    (i, c, HS_CONN_CTR | HS_CONN_TAG_FIX, stream), value min(floor(u * n), n - 1),
@@ -218,6 +222,7 @@ struct HsConnSpec {
     int dmin, D;           /* delay slot = delay - dmin, in [0, D) */
     double wlo, wspan;     /* weight = wlo + u*wspan */
     int fixed_k;           /* > 0: fixed out-degree K (then p and jump are unused) */
+    int ncopy;             /* synapses per selected pair (connect(n=...)); 0 or 1: one */
 };
 
 /* the state arrays (precision of the network) of the source and target groups,

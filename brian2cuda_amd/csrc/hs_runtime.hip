@@ -915,7 +915,8 @@ static int hs_conn_expr_vars(const std::string &e, const char *fn, int nvars)
 static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo, int post_hi,
                              double p, int fixed_k, int autapses, uint64_t seed, uint32_t stream, double delay_lo,
                              double delay_hi, int wmode, double w_lo, double w_hi, int *pid,
-                             const std::string &cond = std::string(), const std::string &pexpr = std::string())
+                             const std::string &cond = std::string(), const std::string &pexpr = std::string(),
+                             int ncopy = 1)
 {
     if (!net) return hs_fail(HS_ERR_ARG, "null net");
     if (net->built) return hs_fail(HS_ERR_STATE, "pathways must be added before the first run");
@@ -927,6 +928,8 @@ static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo,
     if (pre_lo < 0 || pre_lo > pre_hi || pre_hi > gs.n) return hs_fail(HS_ERR_ARG, "bad source range");
     if (post_lo < 0 || post_lo > post_hi || post_hi > gt.n) return hs_fail(HS_ERR_ARG, "bad target range");
     if (fixed_k < 0) return hs_fail(HS_ERR_ARG, "out-degree %d < 0", fixed_k);
+    if (ncopy < 1 || ncopy > HS_CONN_NMAX)
+        return hs_fail(HS_ERR_UNSUPPORTED, "%d synapses per pair (1 to %d)", ncopy, HS_CONN_NMAX);
     if (!pexpr.empty()) {
         if (fixed_k) return hs_fail(HS_ERR_ARG, "a probability expression with a fixed out-degree");
         p = 1.0; /* rows are not skipped; every pair takes its own draw against p(i, j) */
@@ -960,6 +963,7 @@ static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo,
             return hs_fail(HS_ERR_UNSUPPORTED, "out-degree %d > %d", fixed_k, HS_CONN_KMAX);
     }
     c.fixed_k = fixed_k;
+    c.ncopy = ncopy;
     c.p = fixed_k ? 0.0 : p;
     c.jump = !fixed_k && pexpr.empty() && p < 0.25 ? 1 : 0;
     c.pconst = c.jump && p > 0.0 ? 1.0 / hs_log(1.0 - p) : 0.0;
@@ -1086,14 +1090,14 @@ extern "C" int hs_net_connect_fixed(hs_net *net, int src, int tgt, int var, int 
 }
 
 extern "C" int hs_net_connect_code(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo,
-                                   int post_hi, const char *condition, const char *p_expr, double p, int k, int autapses,
-                                   uint64_t seed, uint32_t stream, double delay_lo, double delay_hi, int wmode,
-                                   double w_lo, double w_hi, int *pid)
+                                   int post_hi, const char *condition, const char *p_expr, double p, int k, int n,
+                                   int autapses, uint64_t seed, uint32_t stream, double delay_lo, double delay_hi,
+                                   int wmode, double w_lo, double w_hi, int *pid)
 {
     if (!net) return hs_fail(HS_ERR_ARG, "null net");
     const std::string cond = condition ? condition : "", pe = p_expr ? p_expr : "";
     return hs_connect_device(net, src, tgt, var, pre_lo, pre_hi, post_lo, post_hi, k ? 1.0 : p, k, autapses, seed,
-                             stream, delay_lo, delay_hi, wmode, w_lo, w_hi, pid, cond, pe);
+                             stream, delay_lo, delay_hi, wmode, w_lo, w_hi, pid, cond, pe, n);
 }
 
 /*

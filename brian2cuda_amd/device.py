@@ -78,6 +78,7 @@ class SynapsesDesc:
     k: Optional[int] = None               # connect(j='k for k in sample(N_post, size=K)'), optionally
                                           # with condition 'i != j' (its 'if k != i') and i/j ranges
     condition: Optional[str] = None
+    n: int = 1                            # connect(n=...): synapses per selected pair (p= or a sample size)
     values: Dict[str, object] = field(default_factory=dict)  # per-synapse variables (w ...)
     weight_range: Optional[tuple] = None  # w = 'lo + (hi-lo)*rand()' with connect(p)
     delay: object = 0.0                   # seconds: scalar, array, or (lo, hi) uniform with connect(p)
@@ -524,12 +525,12 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
                 w = (-hi, -lo) if neg else (lo, hi)
                 rc = nw.RandomConnect(float(pnum or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
                                       autapses=autapses, delay=delay, wmode=HS_W_SYNAPSE, w=w, k=int(k_val or 0),
-                                      cond=cond_c, p_expr=p_c)
+                                      cond=cond_c, p_expr=p_c, n=int(s.n))
                 paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SYNAPSE, random=rc, code=code))
             else:
                 rc = nw.RandomConnect(float(pnum or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
                                       autapses=autapses, delay=delay, wmode=HS_W_SCALAR, w=(wval, wval),
-                                      k=int(k_val or 0), cond=cond_c, p_expr=p_c)
+                                      k=int(k_val or 0), cond=cond_c, p_expr=p_c, n=int(s.n))
                 paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SCALAR, wscalar=wval, random=rc,
                                          code=code))
             if code is not None:
@@ -537,6 +538,8 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
             continue
         if s.i is None or s.j is None:
             raise NotImplementedError(f"Synapses {s.name!r}: connect() with i/j arrays, p= or a sample size only")
+        if int(s.n) != 1:
+            raise NotImplementedError(f"Synapses {s.name!r}: n={s.n} with i/j arrays (repeat the arrays instead)")
         pre = np.asarray(s.i, np.int64) + sr[0]
         post = np.asarray(s.j, np.int64) + tr[0]
         if pre.size != post.size:

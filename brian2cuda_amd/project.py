@@ -207,11 +207,11 @@ class ProjectWriter:
             if p.random is not None:
                 r = p.random
                 k = int(getattr(r, "k", 0) or 0)
-                if getattr(r, "cond", "") or getattr(r, "p_expr", ""):
+                if getattr(r, "cond", "") or getattr(r, "p_expr", "") or getattr(r, "n", 1) > 1:
                     # condition / probability as code: compiled for the device by this call,
                     # after the groups' state (command-line values included) is set
                     fn = "hs_net_connect_code"
-                    arg = f"{_c_str(r.cond)}, {_c_str(r.p_expr)}, {_c_double(r.p)}, {k}"
+                    arg = f"{_c_str(r.cond)}, {_c_str(r.p_expr)}, {_c_double(r.p)}, {k}, {int(getattr(r, 'n', 1))}"
                     pinfo["connect"] = "code"
                 else:
                     fn, arg = ("hs_net_connect_fixed", str(k)) if k else ("hs_net_connect_random", _c_double(r.p))

@@ -47,7 +47,7 @@ def lib():
         "ref_connect_fixed": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                           C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                           I32P, I32P, I32P, DP, C.c_int64]),
-        "ref_connect_code": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+        "ref_connect_code": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                          C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          I32P, I32P, I32P, DP, C.c_int64]),
@@ -350,12 +350,13 @@ def connect_code_source(cond, p_expr, precision):
 
 
 def connect_code(pre_range, post_range, cond, p_expr, p, K, skip_self, seed, stream, delay, dt, weight,
-                 pre_vars=(), post_vars=(), precision=64):
+                 pre_vars=(), post_vars=(), precision=64, n=1):
     """Synapses of connect(condition=cond, p=p or p_expr) or of a fixed
     out-degree K filtered by cond, the C expressions of the device program
     (brian2cuda_amd.codegen.connect_expression), compiled for the host;
     pre_vars / post_vars: the source / target groups' state arrays (read in
-    precision).  Creation order; dict of pre, post, delay_steps, w."""
+    precision); n: synapses per selected pair (consecutive in creation order).
+    Creation order; dict of pre, post, delay_steps, w."""
     L = lib()
     fns = _host_lib(connect_code_source(cond, p_expr, precision))
     dt_np = np.float64 if precision == 64 else np.float32
@@ -365,7 +366,7 @@ def connect_code(pre_range, post_range, cond, p_expr, p, K, skip_self, seed, str
     arr_post = (C.c_void_p * max(1, len(keep) - npre))(*[a.ctypes.data for a in keep[npre:]])
     cf = C.cast(fns.hs_cn_cond, C.c_void_p).value if cond else None
     pf = C.cast(fns.hs_cn_p, C.c_void_p).value if p_expr else None
-    args = (int(pre_range[0]), int(pre_range[1]), int(post_range[0]), int(post_range[1]), float(p), int(K),
+    args = (int(pre_range[0]), int(pre_range[1]), int(post_range[0]), int(post_range[1]), float(p), int(K), int(n),
             int(bool(skip_self)), int(seed), int(stream), float(delay[0]), float(delay[1]), float(dt),
             float(weight[0]), float(weight[1]), cf, pf, C.cast(arr_pre, C.c_void_p), C.cast(arr_post, C.c_void_p))
     if K > 0 and K > int(post_range[1]) - int(post_range[0]) and int(pre_range[1]) > int(pre_range[0]):
@@ -611,10 +612,10 @@ def random_pathway(spec, path, rows=None, precision=64):
     r = path.random
     pre = r.pre if rows is None else rows
     skip_self = (not r.autapses) and path.src == path.tgt
-    if getattr(r, "cond", "") or getattr(r, "p_expr", ""):
+    if getattr(r, "cond", "") or getattr(r, "p_expr", "") or getattr(r, "n", 1) > 1:
         return connect_code(pre, r.post, r.cond, r.p_expr, r.p, r.k, skip_self, r.seed, r.stream, r.delay, spec.dt,
                             r.w, initial_state(spec.groups[path.src]), initial_state(spec.groups[path.tgt]),
-                            precision)
+                            precision, getattr(r, "n", 1))
     if getattr(r, "k", 0):
         return connect_fixed(pre, r.post, r.k, skip_self, r.seed, r.stream, r.delay, spec.dt, r.w)
     return connect_random(pre, r.post, r.p, skip_self, r.seed, r.stream, r.delay, spec.dt, r.w)

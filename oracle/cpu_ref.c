@@ -148,6 +148,7 @@ struct ref_conn_code {
     int pre_lo, pre_hi, post_lo, post_hi;
     uint64_t seed; /* the connect call's: keys of rand() / randn() in the code */
     uint32_t stream;
+    int ncopy;     /* synapses per selected pair (connect(n=...)) */
 };
 
 static int ref_conn_keep(const struct ref_conn_code *cc, int i, long long j)
@@ -195,16 +196,19 @@ static int64_t ref_connect_random_c(int pre_lo, int pre_hi, int post_lo, int pos
                 }
             }
             if (!(skip_self && j == i) && ref_conn_keep(cc, i, j)) {
-                if (n < cap) {
-                    uint32_t sw[4];
-                    ref_conn_words(seed, stream, (uint32_t)i, (uint32_t)j, 1, sw);
-                    const double ud = ref_u53(sw[0], sw[1]), uw = ref_u53(sw[2], sw[3]);
-                    if (pre) pre[n] = i;
-                    if (post) post[n] = (int32_t)j;
-                    if (delay_steps) delay_steps[n] = ref_quantize_delay(delay_lo + ud * dspan, dt);
-                    if (w) w[n] = w_lo + uw * wspan;
+                /* copy c of the pair: delay and weight from tag 1 | c << 4 */
+                for (int c = 0; c < (cc ? cc->ncopy : 1); ++c) {
+                    if (n < cap) {
+                        uint32_t sw[4];
+                        ref_conn_words(seed, stream, (uint32_t)i, (uint32_t)j, 1u | ((uint32_t)c << 4), sw);
+                        const double ud = ref_u53(sw[0], sw[1]), uw = ref_u53(sw[2], sw[3]);
+                        if (pre) pre[n] = i;
+                        if (post) post[n] = (int32_t)j;
+                        if (delay_steps) delay_steps[n] = ref_quantize_delay(delay_lo + ud * dspan, dt);
+                        if (w) w[n] = w_lo + uw * wspan;
+                    }
+                    n++;
                 }
-                n++;
             }
             ++j;
         }
@@ -269,16 +273,18 @@ static int64_t ref_connect_fixed_c(int pre_lo, int pre_hi, int post_lo, int post
         for (int k = 0; k < K; ++k) {
             const int32_t j = post_lo + x[k];
             if ((skip_self && j == i) || !ref_conn_keep(cc, i, j)) continue;
-            if (n < cap) {
-                uint32_t sw[4];
-                ref_conn_words(seed, stream, (uint32_t)i, (uint32_t)j, 1, sw);
-                const double ud = ref_u53(sw[0], sw[1]), uw = ref_u53(sw[2], sw[3]);
-                if (pre) pre[n] = i;
-                if (post) post[n] = j;
-                if (delay_steps) delay_steps[n] = ref_quantize_delay(delay_lo + ud * dspan, dt);
-                if (w) w[n] = w_lo + uw * wspan;
+            for (int c = 0; c < (cc ? cc->ncopy : 1); ++c) {
+                if (n < cap) {
+                    uint32_t sw[4];
+                    ref_conn_words(seed, stream, (uint32_t)i, (uint32_t)j, 1u | ((uint32_t)c << 4), sw);
+                    const double ud = ref_u53(sw[0], sw[1]), uw = ref_u53(sw[2], sw[3]);
+                    if (pre) pre[n] = i;
+                    if (post) post[n] = j;
+                    if (delay_steps) delay_steps[n] = ref_quantize_delay(delay_lo + ud * dspan, dt);
+                    if (w) w[n] = w_lo + uw * wspan;
+                }
+                n++;
             }
-            n++;
         }
     }
     free(x);
@@ -304,7 +310,7 @@ int64_t ref_connect_fixed(int pre_lo, int pre_hi, int post_lo, int post_hi, int 
  * gives their synapses.  pre_vars / post_vars: the groups' state arrays the
  * code reads.
  */
-int64_t ref_connect_code(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int K, int skip_self,
+int64_t ref_connect_code(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int K, int ncopy, int skip_self,
                          uint64_t seed, uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo,
                          double w_hi, void *cond_fn, void *p_fn, const void *const *pre_vars,
                          const void *const *post_vars, int32_t *pre, int32_t *post, int32_t *delay_steps, double *w,
@@ -321,6 +327,7 @@ int64_t ref_connect_code(int pre_lo, int pre_hi, int post_lo, int post_hi, doubl
     cc.post_hi = post_hi;
     cc.seed = seed;
     cc.stream = stream;
+    cc.ncopy = ncopy > 1 ? ncopy : 1;
     if (K > 0)
         return ref_connect_fixed_c(pre_lo, pre_hi, post_lo, post_hi, K, skip_self, seed, stream, delay_lo, delay_hi,
                                    dt, w_lo, w_hi, &cc, pre, post, delay_steps, w, cap);

@@ -25,7 +25,7 @@ int64_t ref_connect_fixed(int pre_lo, int pre_hi, int post_lo, int post_hi, int 
                           int32_t *pre, int32_t *post, int32_t *delay_steps, double *w, int64_t cap);
 /* the same with a condition and a per-pair probability compiled from code
    (function pointers, NULL: none) and the fixed out-degree K (> 0) */
-int64_t ref_connect_code(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int K, int skip_self,
+int64_t ref_connect_code(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int K, int ncopy, int skip_self,
                          uint64_t seed, uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo,
                          double w_hi, void *cond_fn, void *p_fn, const void *const *pre_vars,
                          const void *const *post_vars, int32_t *pre, int32_t *post, int32_t *delay_steps, double *w,

@@ -199,3 +199,23 @@ def test_random_values_init_synapses_fixed_and_random_seed():
     same = lambda a, b: a.shape == b.shape and np.array_equal(a, b)  # noqa: E731
     assert same(idx[1], idx[3])
     assert not same(idx[0], idx[1]) and not same(idx[0], idx[2]) and not same(idx[1], idx[2])
+
+
+@pytest.mark.parametrize("K", [0, 9])
+def test_several_synapses_per_pair(K):
+    """connect(n=3): each selected pair three times, consecutively in creation
+    order; copy 0 carries the single synapse's delay and weight, the other
+    copies their own draws (tag 1 | c << 4), so copies of one pair differ."""
+    kw = dict(pre=(0, 60), post=(0, 50), delay=(1e-4, 5e-3), w=(0.5, 2.0), K=K)
+    c, _ = cg.connect_expression("(i + j) % 4 != 1")
+    one = _code(c, p=0.3 if not K else 1.0, **kw)
+    three = oracle.connect_code(kw["pre"], kw["post"], c, "", 0.3 if not K else 1.0, K, False, 11, 0, kw["delay"], DT,
+                                kw["w"], (), (), 64, 3)
+    m = one["pre"].size
+    assert m > 100 and three["pre"].size == 3 * m
+    for k in ("pre", "post"):
+        assert np.array_equal(three[k], np.repeat(one[k], 3)), k
+    for k in ("delay_steps", "w"):
+        assert np.array_equal(three[k][0::3], one[k]), k
+    assert np.mean(three["w"][1::3] != three["w"][0::3]) > 0.99
+    assert np.mean(three["delay_steps"][2::3] != three["delay_steps"][1::3]) > 0.8

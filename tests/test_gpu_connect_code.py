@@ -216,3 +216,30 @@ def test_random_values_init_synapses_fixed_and_random_seed(hs, prec):
     assert same(idx[1], idx[3]) and idx[1].size > 0
     assert not same(idx[0], idx[1]) and not same(idx[0], idx[2]) and not same(idx[1], idx[2])
     net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("cond,p,K,n", [("", 0.1, 0, 2), ("i % 3 != 0", 0.5, 0, 3), ("", 1.0, 7, 16),
+                                        ("j < i", "0.2", 0, 4)])
+def test_several_synapses_per_pair_matches_oracle(hs, oracle_lib, prec, cond, p, K, n):
+    """connect(n=...): the bundle table equals the oracle's synapses (each
+    selected pair n times, each copy its own delay and weight) stably sorted
+    by (pre, delay), so copies of one pair keep their order within a bundle."""
+    net, pos = _setup(prec, 1200, 900)
+    c = cg.connect_expression(cond)[0] if cond else ""
+    pe, pnum = (cg.connect_expression(p)[0], 1.0) if isinstance(p, str) else ("", p)
+    print(f"connect n={n} prec {prec}: {cond!r} p={p!r} K={K}", flush=True)
+    pid = net.connect_code(0, 1, 1, c, pe, pnum, K, None, None, True, 31, 2, (0.1e-3, 3e-3), rt.HS_W_SYNAPSE,
+                           (0.0, 1.0), n)
+    ref = _bundle_order(oracle_lib.connect_code((0, 1200), (0, 900), c, pe, pnum, K, False, 31, 2, (0.1e-3, 3e-3), DT,
+                                                (0.0, 1.0), (), (), prec, n))
+    assert net.pathway_info(pid)["nsyn"] == ref["pre"].size > 5000
+    gp, gq, gd, gw = net.synapses(pid)
+    np.testing.assert_array_equal(gp, ref["pre"])
+    np.testing.assert_array_equal(gq, ref["post"])
+    np.testing.assert_array_equal(gd, ref["delay_steps"])
+    rw = ref["w"].astype(np.float32).astype(np.float64) if prec == 32 else ref["w"]
+    np.testing.assert_array_equal(gw, rw)
+    with pytest.raises(HsError, match="per pair"):
+        net.connect_code(0, 1, 1, "", "", 0.1, 0, None, None, True, 31, 2, 0.0, rt.HS_W_SCALAR, 1.0, 17)
+    net.close()
