@@ -106,7 +106,8 @@ SIGNATURES = {
     "hs_net_connect_code": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int, C.c_int, C.c_uint64,
                                        C.c_uint32,
-                                       C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int)]),
+                                       C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.c_char_p,
+                                       C.POINTER(C.c_int)]),
     "hs_net_add_pathway_device": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int64, P, P, P, C.c_int, C.c_int, P,
                                             C.c_double, C.POINTER(C.c_int)]),
     "hs_net_pathway_info": (C.c_int, [P, C.c_int, I64P, C.POINTER(C.c_int), C.POINTER(C.c_int),

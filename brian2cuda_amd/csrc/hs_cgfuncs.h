@@ -69,10 +69,12 @@ HS_HD unsigned int hs_cg_conn_stream(unsigned int stream, int k, int normal)
 }
 
 /* the step word of a connect-code draw: j in the high 32 bits over the tag
-   0x43524E44 ("CRND") */
-HS_HD long long hs_cg_conn_ctr(long long j)
+   0x43524E44 ("CRND") plus 256 x the synapse copy (connect(n=...); 0 in
+   conditions and probabilities, which see the pair once) */
+HS_HD long long hs_cg_conn_ctr(long long j, int copy)
 {
-    return (long long)(((unsigned long long)(unsigned int)j << 32) | 0x43524E44ull);
+    return (long long)(((unsigned long long)(unsigned int)j << 32) |
+                       (0x43524E44ull + ((unsigned long long)(unsigned int)copy << 8)));
 }
 
 #endif /* HS_CGFUNCS_H */

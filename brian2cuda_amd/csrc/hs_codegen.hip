@@ -299,8 +299,8 @@ __device__ __forceinline__ auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo +
 /* rand() / randn() call site k of a connect condition or probability: a pure
    function of the connect call's seed and stream and the pair (i, j) relative to
    the ranges, so both connect passes see the same draw */
-#define hs_crand_u(k) hs_rand(hs_seed, hs_cg_conn_stream(hs_cstream, k, 0), (uint32_t)i, hs_cg_conn_ctr(j))
-#define hs_crand_n(k) hs_randn(hs_seed, hs_cg_conn_stream(hs_cstream, k, 1), (uint32_t)i, hs_cg_conn_ctr(j))
+#define hs_crand_u(k) hs_rand(hs_seed, hs_cg_conn_stream(hs_cstream, k, 0), (uint32_t)i, hs_cg_conn_ctr(j, hs_ccopy))
+#define hs_crand_n(k) hs_randn(hs_seed, hs_cg_conn_stream(hs_cstream, k, 1), (uint32_t)i, hs_cg_conn_ctr(j, hs_ccopy))
 )HSCG";
 
 std::string hs_cg_source(int prec, const std::vector<HsCgModel> &models, const std::vector<HsSynCgModel> &syn)
@@ -448,9 +448,9 @@ int hs_cg_kernels(int prec, int ms, const std::vector<HsCgModel> &models, const 
 /* ------------------------------------------------------------------------ */
 /* connect conditions and probabilities                                      */
 /* ------------------------------------------------------------------------ */
-int hs_cg_conn_check(const std::string &cond, const std::string &p)
+int hs_cg_conn_check(const std::string &cond, const std::string &p, const std::string &w)
 {
-    for (const std::string *e : {&cond, &p}) {
+    for (const std::string *e : {&cond, &p, &w}) {
         if (e->size() > 16384) return hs_rt_fail(HS_ERR_ARG, "connect expression longer than 16 kB");
         if (e->find_first_of("#;{}") != std::string::npos)
             return hs_rt_fail(HS_ERR_ARG, "a connect expression holds '#', ';' or a brace: expressions only");
@@ -471,7 +471,7 @@ int hs_cg_conn_check(const std::string &cond, const std::string &p)
 
 /* the program of one condition / probability pair: the filter policy HsConnCg
    of hs_conndev.h and the two passes instantiated with it */
-static std::string hs_cg_conn_source(const std::string &cond, const std::string &p)
+static std::string hs_cg_conn_source(const std::string &cond, const std::string &p, const std::string &w)
 {
     std::string o = "#include \"hs_conndev.h\"\n#include \"hs_cgconn.h\"\n";
     o += hs_cg_math;
@@ -479,19 +479,27 @@ static std::string hs_cg_conn_source(const std::string &cond, const std::string 
     o += "#define hs_post(k) ((double)((const R *)hs_a.post[k])[hs_aj])\n";
     o += "struct HsConnCg {\n";
     o += "    static constexpr bool COND = " + std::string(cond.empty() ? "false" : "true") +
-         ", PEXPR = " + std::string(p.empty() ? "false" : "true") + ";\n";
+         ", PEXPR = " + std::string(p.empty() ? "false" : "true") + ", WEXPR = " +
+         std::string(w.empty() ? "false" : "true") + ";\n";
+    /* the functions' common head (after their signatures h0 / h1) */
     const char *head =
-        "(const HsConnSpec &hs_c, const HsConnVars &hs_a, long long hs_ai, long long hs_aj)\n    {\n"
         "        const long long i = hs_ai - hs_c.pre_lo, j = hs_aj - hs_c.post_lo;\n"
         "        const long long N_pre = (long long)hs_c.pre_hi - hs_c.pre_lo, N_post = (long long)hs_c.post_hi - "
         "hs_c.post_lo;\n"
         "        const unsigned long long hs_seed = hs_c.seed;\n"
         "        const unsigned int hs_cstream = hs_c.stream;\n"
-        "        (void)hs_a; (void)i; (void)j; (void)N_pre; (void)N_post; (void)hs_seed; (void)hs_cstream;\n";
-    o += "    template <typename R>\n    static __device__ __forceinline__ bool cond" + std::string(head);
+        "        (void)hs_a; (void)i; (void)j; (void)N_pre; (void)N_post; (void)hs_seed; (void)hs_cstream; "
+        "(void)hs_ccopy;\n";
+    const std::string h0 = "(const HsConnSpec &hs_c, const HsConnVars &hs_a, long long hs_ai, long long hs_aj)\n"
+                           "    {\n        const int hs_ccopy = 0;\n";
+    const std::string h1 = "(const HsConnSpec &hs_c, const HsConnVars &hs_a, long long hs_ai, long long hs_aj, "
+                           "int hs_ccopy)\n    {\n";
+    o += "    template <typename R>\n    static __device__ __forceinline__ bool cond" + h0 + std::string(head);
     o += "        return (bool)(" + (cond.empty() ? std::string("true") : cond) + ");\n    }\n";
-    o += "    template <typename R>\n    static __device__ __forceinline__ double p" + std::string(head);
-    o += "        return (double)(" + (p.empty() ? std::string("0.0") : p) + ");\n    }\n};\n";
+    o += "    template <typename R>\n    static __device__ __forceinline__ double p" + h0 + std::string(head);
+    o += "        return (double)(" + (p.empty() ? std::string("0.0") : p) + ");\n    }\n";
+    o += "    template <typename R>\n    static __device__ __forceinline__ double w" + h1 + std::string(head);
+    o += "        return (double)(" + (w.empty() ? std::string("0.0") : w) + ");\n    }\n};\n";
     o += R"HSCG(
 template <typename R>
 __global__ void __launch_bounds__(HS_CONN_WAVES * HS_WAVE)
@@ -519,10 +527,11 @@ struct HsCgConnEntry {
 };
 static std::map<std::string, HsCgConnEntry> g_cg_conn_cache; /* precision + program text -> kernels */
 
-int hs_cg_conn_kernels(int prec, const std::string &cond, const std::string &p, const HsCgConnKernels **out)
+int hs_cg_conn_kernels(int prec, const std::string &cond, const std::string &p, const std::string &w,
+                       const HsCgConnKernels **out)
 {
-    HS_CG_TRY(hs_cg_conn_check(cond, p));
-    const std::string src = hs_cg_conn_source(cond, p);
+    HS_CG_TRY(hs_cg_conn_check(cond, p, w));
+    const std::string src = hs_cg_conn_source(cond, p, w);
     const std::string key = std::to_string(prec) + "\n" + src;
     std::lock_guard<std::mutex> lock(g_cg_mu);
     auto it = g_cg_conn_cache.find(key);

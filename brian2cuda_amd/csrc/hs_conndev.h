@@ -22,7 +22,7 @@
 
 /* no condition, constant probability */
 struct HsConnPlain {
-    static constexpr bool COND = false, PEXPR = false;
+    static constexpr bool COND = false, PEXPR = false, WEXPR = false;
     template <typename R>
     static __device__ __forceinline__ bool cond(const HsConnSpec &, const HsConnVars &, long long, long long)
     {
@@ -30,6 +30,12 @@ struct HsConnPlain {
     }
     template <typename R>
     static __device__ __forceinline__ double p(const HsConnSpec &, const HsConnVars &, long long, long long)
+    {
+        return 0.0;
+    }
+    /* the weight of copy cp of pair (i, j), when WEXPR */
+    template <typename R>
+    static __device__ __forceinline__ double w(const HsConnSpec &, const HsConnVars &, long long, long long, int)
     {
         return 0.0;
     }
@@ -243,8 +249,9 @@ __device__ __forceinline__ void hs_conn_count_body(const HsConnSpec &c, const Hs
    delay slot follow creation order, lane (pair) first, then copy -- per slot, a
    lane's copies in that slot are counted and the counts prefix-summed over the
    wave */
-template <typename R>
-__device__ __forceinline__ void hs_conn_write_copies(const HsConnSpec &c, int i, long long j, bool sel, bool draw,
+template <typename F, typename R>
+__device__ __forceinline__ void hs_conn_write_copies(const HsConnSpec &c, const HsConnVars &a, int i, long long j,
+                                                     bool sel, bool draw,
                                                      size_t row, const int32_t *__restrict__ bstart,
                                                      int32_t *__restrict__ post, R *__restrict__ w, int *err,
                                                      int32_t *q)
@@ -295,7 +302,11 @@ __device__ __forceinline__ void hs_conn_write_copies(const HsConnSpec &c, int i,
                     *err = 2;
                 } else {
                     post[pos] = (int32_t)j;
-                    if (w) w[pos] = (R)(c.wlo + uwv[cp] * c.wspan);
+                    if constexpr (F::WEXPR) {
+                        if (w) w[pos] = (R)F::template w<R>(c, a, i, j, cp);
+                    } else {
+                        if (w) w[pos] = (R)(c.wlo + uwv[cp] * c.wspan);
+                    }
                 }
                 ++pos;
                 sl[cp] = -1;
@@ -344,7 +355,7 @@ __device__ __forceinline__ void hs_conn_write_body(const HsConnSpec &c, const Hs
             }
             if constexpr (F::COND) sel = sel && F::template cond<R>(c, a, i, j);
             if (c.ncopy > 1) {
-                hs_conn_write_copies(c, i, j, sel, draw, row, bstart, post, w, err, q);
+                hs_conn_write_copies<F, R>(c, a, i, j, sel, draw, row, bstart, post, w, err, q);
                 continue;
             }
             double ud = 0.0, uw = 0.0;
@@ -379,7 +390,11 @@ __device__ __forceinline__ void hs_conn_write_body(const HsConnSpec &c, const Hs
                     *err = 2;
                 } else {
                     post[pos] = (int32_t)j;
-                    if (w) w[pos] = (R)(c.wlo + uw * c.wspan);
+                    if constexpr (F::WEXPR) {
+                        if (w) w[pos] = (R)F::template w<R>(c, a, i, j, 0);
+                    } else {
+                        if (w) w[pos] = (R)(c.wlo + uw * c.wspan);
+                    }
                 }
             }
         }

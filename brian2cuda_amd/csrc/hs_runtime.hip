@@ -916,7 +916,7 @@ static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo,
                              double p, int fixed_k, int autapses, uint64_t seed, uint32_t stream, double delay_lo,
                              double delay_hi, int wmode, double w_lo, double w_hi, int *pid,
                              const std::string &cond = std::string(), const std::string &pexpr = std::string(),
-                             int ncopy = 1)
+                             int ncopy = 1, const std::string &wexpr = std::string())
 {
     if (!net) return hs_fail(HS_ERR_ARG, "null net");
     if (net->built) return hs_fail(HS_ERR_STATE, "pathways must be added before the first run");
@@ -937,11 +937,14 @@ static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo,
     if (!fixed_k && (!(p >= 0.0) || !(p <= 1.0)))
         return hs_fail(HS_ERR_ARG, "connection probability %g outside [0, 1]", p);
     const HsCgConnKernels *K = nullptr;
-    for (const std::string *x : {&cond, &pexpr}) {
+    if (!wexpr.empty() && wmode != HS_W_SYNAPSE)
+        return hs_fail(HS_ERR_ARG, "a weight expression needs per-synapse weights");
+    for (const std::string *x : {&cond, &pexpr, &wexpr}) {
         HS_TRY(hs_conn_expr_vars(*x, "hs_pre", gs.mi.nvars));
         HS_TRY(hs_conn_expr_vars(*x, "hs_post", gt.mi.nvars));
     }
-    if (!cond.empty() || !pexpr.empty()) HS_TRY(hs_cg_conn_kernels(net->prec, cond, pexpr, &K));
+    if (!cond.empty() || !pexpr.empty() || !wexpr.empty())
+        HS_TRY(hs_cg_conn_kernels(net->prec, cond, pexpr, wexpr, &K));
     if (!(delay_lo >= 0.0) || !(delay_hi >= delay_lo) || delay_hi > 1e300)
         return hs_fail(HS_ERR_ARG, "bad delay range [%g, %g]", delay_lo, delay_hi);
     if (wmode != HS_W_SCALAR && wmode != HS_W_SYNAPSE)
@@ -1092,12 +1095,12 @@ extern "C" int hs_net_connect_fixed(hs_net *net, int src, int tgt, int var, int 
 extern "C" int hs_net_connect_code(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo,
                                    int post_hi, const char *condition, const char *p_expr, double p, int k, int n,
                                    int autapses, uint64_t seed, uint32_t stream, double delay_lo, double delay_hi,
-                                   int wmode, double w_lo, double w_hi, int *pid)
+                                   int wmode, double w_lo, double w_hi, const char *w_expr, int *pid)
 {
     if (!net) return hs_fail(HS_ERR_ARG, "null net");
-    const std::string cond = condition ? condition : "", pe = p_expr ? p_expr : "";
+    const std::string cond = condition ? condition : "", pe = p_expr ? p_expr : "", we = w_expr ? w_expr : "";
     return hs_connect_device(net, src, tgt, var, pre_lo, pre_hi, post_lo, post_hi, k ? 1.0 : p, k, autapses, seed,
-                             stream, delay_lo, delay_hi, wmode, w_lo, w_hi, pid, cond, pe, n);
+                             stream, delay_lo, delay_hi, wmode, w_lo, w_hi, pid, cond, pe, n, we);
 }
 
 /*

@@ -518,14 +518,25 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
             post = (tr[0] + post[0], tr[0] + post[1])
             delay = s.delay if isinstance(s.delay, tuple) else (float(s.delay), float(s.delay))
             if wkind == "synapse":
-                if s.weight_range is None:
+                w_c = ""
+                if isinstance(s.values.get(wval), str):
+                    # S.w = '<expression>': evaluated per synapse by the generator (its
+                    # draws numbered after the condition's and the probability's)
+                    from .codegen import connect_expression
+                    w_c, wconst = connect_expression(s.values[wval], s.namespace, gvars[src], gvars[tgt], what,
+                                                     first_draw=(cond_c + p_c).count("hs_crand_"))
+                    if wconst is not None:
+                        w_c, s.weight_range = "", (float(wconst), float(wconst))
+                    elif neg:
+                        w_c = f"(-({w_c}))"
+                if s.weight_range is None and not w_c:
                     raise NotImplementedError(f"Synapses {s.name!r}: with connect(p) or a sample size the weight is a "
-                                              "constant or 'lo + (hi-lo)*rand()'")
-                lo, hi = s.weight_range
+                                              "constant, 'lo + (hi-lo)*rand()' or an expression (values={'w': '...'})")
+                lo, hi = s.weight_range or (0.0, 0.0)
                 w = (-hi, -lo) if neg else (lo, hi)
                 rc = nw.RandomConnect(float(pnum or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
                                       autapses=autapses, delay=delay, wmode=HS_W_SYNAPSE, w=w, k=int(k_val or 0),
-                                      cond=cond_c, p_expr=p_c, n=int(s.n))
+                                      cond=cond_c, p_expr=p_c, n=int(s.n), w_expr=w_c)
                 paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SYNAPSE, random=rc, code=code))
             else:
                 rc = nw.RandomConnect(float(pnum or 0.0), pre, post, int(s.seed), stream=len(paths) + 1,
@@ -534,7 +545,7 @@ def network_spec(groups: List[GroupDesc], synapses: List[SynapsesDesc], monitors
                 paths.append(nw.PathSpec(src, tgt, var, None, None, wmode=HS_W_SCALAR, wscalar=wval, random=rc,
                                          code=code))
             if code is not None:
-                code.init = {k: v for k, v in code.init.items() if np.asarray(v).size == 1}
+                code.init = {k: v for k, v in code.init.items() if np.asarray(v).size == 1 and not isinstance(v, str)}
             continue
         if s.i is None or s.j is None:
             raise NotImplementedError(f"Synapses {s.name!r}: connect() with i/j arrays, p= or a sample size only")

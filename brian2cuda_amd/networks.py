@@ -55,6 +55,7 @@ class RandomConnect:
     cond: str = ""
     p_expr: str = ""
     n: int = 1  # synapses per selected pair (connect(n=...))
+    w_expr: str = ""  # S.w = '<expression>': each synapse's weight as code (wmode SYNAPSE)
 
 
 @dataclass
@@ -658,9 +659,9 @@ def build_hs(spec, precision=64, engine="plain", spike_capacity=1 << 20, options
     for p in spec.paths:
         if p.random is not None:
             r = p.random
-            if r.cond or r.p_expr or r.n > 1:
+            if r.cond or r.p_expr or r.n > 1 or r.w_expr:
                 pid = net.connect_code(p.src, p.tgt, p.var, r.cond, r.p_expr, r.p, r.k, r.pre, r.post, r.autapses,
-                                       r.seed, r.stream, r.delay, r.wmode, r.w, r.n)
+                                       r.seed, r.stream, r.delay, r.wmode, r.w, r.n, r.w_expr)
             elif r.k:
                 pid = net.connect_fixed(p.src, p.tgt, p.var, r.k, r.pre, r.post, r.autapses, r.seed, r.stream,
                                         r.delay, r.wmode, r.w)

@@ -207,7 +207,8 @@ class ProjectWriter:
             if p.random is not None:
                 r = p.random
                 k = int(getattr(r, "k", 0) or 0)
-                if getattr(r, "cond", "") or getattr(r, "p_expr", "") or getattr(r, "n", 1) > 1:
+                w_expr = getattr(r, "w_expr", "")
+                if getattr(r, "cond", "") or getattr(r, "p_expr", "") or getattr(r, "n", 1) > 1 or w_expr:
                     # condition / probability as code: compiled for the device by this call,
                     # after the groups' state (command-line values included) is set
                     fn = "hs_net_connect_code"
@@ -219,7 +220,8 @@ class ProjectWriter:
                 L.append(f"    HS_CHECK({fn}(net, g_{src}, g_{tgt}, {p.var}, {r.pre[0]}, {r.pre[1]}, "
                          f"{r.post[0]}, {r.post[1]}, {arg}, {int(bool(r.autapses))}, {int(r.seed)}ULL, "
                          f"{int(r.stream)}u, {_c_double(r.delay[0])}, {_c_double(r.delay[1])}, {int(r.wmode)}, "
-                         f"{_c_double(r.w[0])}, {_c_double(r.w[1])}, &p_{pname}));")
+                         f"{_c_double(r.w[0])}, {_c_double(r.w[1])}, "
+                         f"{(_c_str(w_expr) + ', ') if fn == 'hs_net_connect_code' else ''}&p_{pname}));")
             else:
                 rp, n = self._static(directory, f"_dynamic_array_{pname}__synaptic_pre", p.pre, np.int32)
                 rq, _ = self._static(directory, f"_dynamic_array_{pname}__synaptic_post", p.post, np.int32)

@@ -312,7 +312,7 @@ class Net:
         return pid.value
 
     def connect_code(self, src, tgt, var, cond="", p_expr="", p=1.0, K=0, pre=None, post=None, autapses=True,
-                     seed=None, stream=0, delay=0.0, wmode=HS_W_SCALAR, w=0.0, n=1):
+                     seed=None, stream=0, delay=0.0, wmode=HS_W_SCALAR, w=0.0, n=1, w_expr=""):
         """connect(condition=..., p=..., n=...) generated on the device with the
         condition and the per-pair probability compiled from C expressions
         (brian2cuda_amd.codegen.connect_expression; "" for none): with p_expr
@@ -320,7 +320,8 @@ class Net:
         the candidates of connect(p) (K > 0: of the fixed out-degree sample)
         are kept where cond holds.  The expressions read the groups' state as
         it is when this is called.  Delays and weights as connect_random; n > 1:
-        n synapses per selected pair, each with its own delay and weight.
+        n synapses per selected pair, each with its own delay and weight;
+        w_expr: each synapse's weight as such an expression (wmode HS_W_SYNAPSE).
         Returns the pathway id."""
         n_src, n_tgt = self.groups[src][1], self.groups[tgt][1]
         pre = (0, n_src) if pre is None else pre
@@ -331,7 +332,7 @@ class Net:
         call("hs_net_connect_code", self.h, src, tgt, var, int(pre[0]), int(pre[1]), int(post[0]), int(post[1]),
              (cond or "").encode(), (p_expr or "").encode(), float(p), int(K), int(n), int(bool(autapses)),
              int(self.seed if seed is None else seed), int(stream), float(dlo), float(dhi), int(wmode), float(wlo),
-             float(whi), C.byref(pid))
+             float(whi), (w_expr or "").encode(), C.byref(pid))
         return pid.value
 
     def set_spikegen(self, gid, indices, steps):

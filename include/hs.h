@@ -270,7 +270,11 @@ int hs_net_connect_fixed(hs_net *net, int src, int tgt, int var, int pre_lo, int
  *   n          synapses per selected pair, 1 to 16 (connect(n=...)): copy c of
  *              pair (i, j) draws its own delay and weight (counter tag
  *              1 | c << 4; copy 0 as a single synapse), the copies consecutive
- *              in creation order.
+ *              in creation order;
+ *   w_expr     NULL or "": weights uniform in [w_lo, w_hi] (wmode
+ *              HS_W_SYNAPSE); else each synapse's weight as an expression of
+ *              the same kind (S.w = '...' after connect), evaluated once per
+ *              synapse; its rand() / randn() draws also key on the copy.
  * Selection is that of the two entry points: with a constant p, candidates
  * are drawn by geometric skips (p < 0.25) or one draw per pair and then kept
  * where the condition holds ('k for k in sample(N_post, p=p) if condition',
@@ -285,7 +289,7 @@ int hs_net_connect_fixed(hs_net *net, int src, int tgt, int var, int pre_lo, int
 int hs_net_connect_code(hs_net *net, int src, int tgt, int var, int pre_lo, int pre_hi, int post_lo, int post_hi,
                         const char *condition, const char *p_expr, double p, int k, int n, int autapses,
                         uint64_t seed, uint32_t stream, double delay_lo, double delay_hi, int wmode, double w_lo,
-                        double w_hi, int *pid);
+                        double w_hi, const char *w_expr, int *pid);
 /*
  * hs_net_add_pathway with the bundle table built on the device: pre, post,
  * delay_steps (or NULL: scalar_delay) and weights are DEVICE pointers (hs_alloc

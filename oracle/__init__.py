@@ -49,7 +49,7 @@ def lib():
                                           I32P, I32P, I32P, DP, C.c_int64]),
         "ref_connect_code": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                          C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
-                                         C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          I32P, I32P, I32P, DP, C.c_int64]),
         "ref_timestep": (C.c_longlong, [C.c_double, C.c_double]),
         "ref_exp": (C.c_double, [C.c_double]),
@@ -262,8 +262,8 @@ static inline auto hs_cg_clip(A x, B lo, C hi) -> decltype(x + lo + hi)
 #define hs_rand_xi(k) hs_randn(hs_seed, hs_cg_rnd_stream(hs_stream, k, 2), (uint32_t)i, hs_step)
 #define hs_srand_u(k) hs_rand(hs_seed, hs_cg_syn_stream(hs_path, k, 0), (uint32_t)i, hs_cg_syn_ctr(hs_step, j))
 #define hs_srand_n(k) hs_randn(hs_seed, hs_cg_syn_stream(hs_path, k, 1), (uint32_t)i, hs_cg_syn_ctr(hs_step, j))
-#define hs_crand_u(k) hs_rand(hs_seed, hs_cg_conn_stream(hs_cstream, k, 0), (uint32_t)i, hs_cg_conn_ctr(j))
-#define hs_crand_n(k) hs_randn(hs_seed, hs_cg_conn_stream(hs_cstream, k, 1), (uint32_t)i, hs_cg_conn_ctr(j))
+#define hs_crand_u(k) hs_rand(hs_seed, hs_cg_conn_stream(hs_cstream, k, 0), (uint32_t)i, hs_cg_conn_ctr(j, hs_ccopy))
+#define hs_crand_n(k) hs_randn(hs_seed, hs_cg_conn_stream(hs_cstream, k, 1), (uint32_t)i, hs_cg_conn_ctr(j, hs_ccopy))
 """
 _cg_libs = {}
 
@@ -331,7 +331,7 @@ def _host_lib(src):
     return _cg_libs[key]
 
 
-def connect_code_source(cond, p_expr, precision):
+def connect_code_source(cond, p_expr, precision, w_expr=""):
     """The host program of a connect condition / probability: extern "C"
     hs_cn_cond and hs_cn_p over the absolute indices (ai, aj), the ranges and
     the groups' state arrays, the expressions of the device program."""
@@ -339,26 +339,30 @@ def connect_code_source(cond, p_expr, precision):
     args = ("(const void *const *hs_pre_v, const void *const *hs_post_v, long long hs_ai, long long hs_aj, "
             "long long pre_lo, long long pre_hi, long long post_lo, long long post_hi, unsigned long long hs_seed, "
             "unsigned int hs_cstream)")
-    head = ("{\n    const long long i = hs_ai - pre_lo, j = hs_aj - post_lo, N_pre = pre_hi - pre_lo, "
+    head = ("{\n    const int hs_ccopy = 0;\n    const long long i = hs_ai - pre_lo, j = hs_aj - post_lo, N_pre = pre_hi - pre_lo, "
             "N_post = post_hi - post_lo;\n    (void)hs_pre_v; (void)hs_post_v; (void)i; (void)j; (void)N_pre; "
             "(void)N_post; (void)hs_seed; (void)hs_cstream;\n")
     return ('#include "hs_cgconn.h"\n' + _CG_MATH + f"typedef {R} R;\n"
             "#define hs_pre(k) ((double)((const R *)hs_pre_v[k])[hs_ai])\n"
             "#define hs_post(k) ((double)((const R *)hs_post_v[k])[hs_aj])\n"
             f'extern "C" int hs_cn_cond{args}\n{head}    return ({cond or "true"}) ? 1 : 0;\n}}\n'
-            f'extern "C" double hs_cn_p{args}\n{head}    return (double)({p_expr or "0.0"});\n}}\n')
+            f'extern "C" double hs_cn_p{args}\n{head}    return (double)({p_expr or "0.0"});\n}}\n'
+            f'extern "C" double hs_cn_w{args[:-1]}, int hs_ccopy)\n'
+            f'{head.replace("    const int hs_ccopy = 0;", "    (void)hs_ccopy;")}'
+            f'    return (double)({w_expr or "0.0"});\n}}\n')
 
 
 def connect_code(pre_range, post_range, cond, p_expr, p, K, skip_self, seed, stream, delay, dt, weight,
-                 pre_vars=(), post_vars=(), precision=64, n=1):
+                 pre_vars=(), post_vars=(), precision=64, n=1, w_expr=""):
     """Synapses of connect(condition=cond, p=p or p_expr) or of a fixed
     out-degree K filtered by cond, the C expressions of the device program
     (brian2cuda_amd.codegen.connect_expression), compiled for the host;
     pre_vars / post_vars: the source / target groups' state arrays (read in
-    precision); n: synapses per selected pair (consecutive in creation order).
+    precision); n: synapses per selected pair (consecutive in creation order);
+    w_expr: each synapse's weight as such an expression ("": uniform in weight).
     Creation order; dict of pre, post, delay_steps, w."""
     L = lib()
-    fns = _host_lib(connect_code_source(cond, p_expr, precision))
+    fns = _host_lib(connect_code_source(cond, p_expr, precision, w_expr))
     dt_np = np.float64 if precision == 64 else np.float32
     keep = [np.ascontiguousarray(v, dtype=dt_np) for v in list(pre_vars) + list(post_vars)]
     npre = len(pre_vars)
@@ -366,9 +370,10 @@ def connect_code(pre_range, post_range, cond, p_expr, p, K, skip_self, seed, str
     arr_post = (C.c_void_p * max(1, len(keep) - npre))(*[a.ctypes.data for a in keep[npre:]])
     cf = C.cast(fns.hs_cn_cond, C.c_void_p).value if cond else None
     pf = C.cast(fns.hs_cn_p, C.c_void_p).value if p_expr else None
+    wf = C.cast(fns.hs_cn_w, C.c_void_p).value if w_expr else None
     args = (int(pre_range[0]), int(pre_range[1]), int(post_range[0]), int(post_range[1]), float(p), int(K), int(n),
             int(bool(skip_self)), int(seed), int(stream), float(delay[0]), float(delay[1]), float(dt),
-            float(weight[0]), float(weight[1]), cf, pf, C.cast(arr_pre, C.c_void_p), C.cast(arr_post, C.c_void_p))
+            float(weight[0]), float(weight[1]), cf, pf, wf, C.cast(arr_pre, C.c_void_p), C.cast(arr_post, C.c_void_p))
     if K > 0 and K > int(post_range[1]) - int(post_range[0]) and int(pre_range[1]) > int(pre_range[0]):
         raise ValueError("sample size K bigger than the population size")
     n = L.ref_connect_code(*args, None, None, None, None, 0)
@@ -612,10 +617,10 @@ def random_pathway(spec, path, rows=None, precision=64):
     r = path.random
     pre = r.pre if rows is None else rows
     skip_self = (not r.autapses) and path.src == path.tgt
-    if getattr(r, "cond", "") or getattr(r, "p_expr", "") or getattr(r, "n", 1) > 1:
+    if getattr(r, "cond", "") or getattr(r, "p_expr", "") or getattr(r, "n", 1) > 1 or getattr(r, "w_expr", ""):
         return connect_code(pre, r.post, r.cond, r.p_expr, r.p, r.k, skip_self, r.seed, r.stream, r.delay, spec.dt,
                             r.w, initial_state(spec.groups[path.src]), initial_state(spec.groups[path.tgt]),
-                            precision, getattr(r, "n", 1))
+                            precision, getattr(r, "n", 1), getattr(r, "w_expr", ""))
     if getattr(r, "k", 0):
         return connect_fixed(pre, r.post, r.k, skip_self, r.seed, r.stream, r.delay, spec.dt, r.w)
     return connect_random(pre, r.post, r.p, skip_self, r.seed, r.stream, r.delay, spec.dt, r.w)

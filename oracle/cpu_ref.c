@@ -141,15 +141,29 @@ typedef int (*ref_conn_cond_fn)(const void *const *pre_vars, const void *const *
 typedef double (*ref_conn_p_fn)(const void *const *pre_vars, const void *const *post_vars, long long ai, long long aj,
                                 long long pre_lo, long long pre_hi, long long post_lo, long long post_hi,
                                 uint64_t seed, uint32_t stream);
+/* a synapse's weight given as code (S.w = '...'), copy: its index among the
+   pair's synapses (connect(n=...)) */
+typedef double (*ref_conn_w_fn)(const void *const *pre_vars, const void *const *post_vars, long long ai, long long aj,
+                                long long pre_lo, long long pre_hi, long long post_lo, long long post_hi,
+                                uint64_t seed, uint32_t stream, int copy);
 struct ref_conn_code {
     ref_conn_cond_fn cond; /* NULL: none */
     ref_conn_p_fn p;       /* NULL: the constant p */
+    ref_conn_w_fn w;       /* NULL: weights uniform in [w_lo, w_hi] */
     const void *const *pre_vars, *const *post_vars;
     int pre_lo, pre_hi, post_lo, post_hi;
     uint64_t seed; /* the connect call's: keys of rand() / randn() in the code */
     uint32_t stream;
     int ncopy;     /* synapses per selected pair (connect(n=...)) */
 };
+
+/* weight of copy c of pair (i, j): the weight expression when given, else the uniform draw u */
+static double ref_conn_weight(const struct ref_conn_code *cc, int i, long long j, int c, double u)
+{
+    if (!cc || !cc->w) return u;
+    return cc->w(cc->pre_vars, cc->post_vars, i, j, cc->pre_lo, cc->pre_hi, cc->post_lo, cc->post_hi, cc->seed,
+                 cc->stream, c);
+}
 
 static int ref_conn_keep(const struct ref_conn_code *cc, int i, long long j)
 {
@@ -205,7 +219,7 @@ static int64_t ref_connect_random_c(int pre_lo, int pre_hi, int post_lo, int pos
                         if (pre) pre[n] = i;
                         if (post) post[n] = (int32_t)j;
                         if (delay_steps) delay_steps[n] = ref_quantize_delay(delay_lo + ud * dspan, dt);
-                        if (w) w[n] = w_lo + uw * wspan;
+                        if (w) w[n] = ref_conn_weight(cc, i, j, c, w_lo + uw * wspan);
                     }
                     n++;
                 }
@@ -281,7 +295,7 @@ static int64_t ref_connect_fixed_c(int pre_lo, int pre_hi, int post_lo, int post
                     if (pre) pre[n] = i;
                     if (post) post[n] = j;
                     if (delay_steps) delay_steps[n] = ref_quantize_delay(delay_lo + ud * dspan, dt);
-                    if (w) w[n] = w_lo + uw * wspan;
+                    if (w) w[n] = ref_conn_weight(cc, i, j, c, w_lo + uw * wspan);
                 }
                 n++;
             }
@@ -312,13 +326,14 @@ int64_t ref_connect_fixed(int pre_lo, int pre_hi, int post_lo, int post_hi, int 
  */
 int64_t ref_connect_code(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int K, int ncopy, int skip_self,
                          uint64_t seed, uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo,
-                         double w_hi, void *cond_fn, void *p_fn, const void *const *pre_vars,
+                         double w_hi, void *cond_fn, void *p_fn, void *w_fn, const void *const *pre_vars,
                          const void *const *post_vars, int32_t *pre, int32_t *post, int32_t *delay_steps, double *w,
                          int64_t cap)
 {
     struct ref_conn_code cc;
     cc.cond = (ref_conn_cond_fn)cond_fn;
     cc.p = (ref_conn_p_fn)p_fn;
+    cc.w = (ref_conn_w_fn)w_fn;
     cc.pre_vars = pre_vars;
     cc.post_vars = post_vars;
     cc.pre_lo = pre_lo;

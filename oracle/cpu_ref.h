@@ -27,7 +27,7 @@ int64_t ref_connect_fixed(int pre_lo, int pre_hi, int post_lo, int post_hi, int 
    (function pointers, NULL: none) and the fixed out-degree K (> 0) */
 int64_t ref_connect_code(int pre_lo, int pre_hi, int post_lo, int post_hi, double p, int K, int ncopy, int skip_self,
                          uint64_t seed, uint32_t stream, double delay_lo, double delay_hi, double dt, double w_lo,
-                         double w_hi, void *cond_fn, void *p_fn, const void *const *pre_vars,
+                         double w_hi, void *cond_fn, void *p_fn, void *w_fn, const void *const *pre_vars,
                          const void *const *post_vars, int32_t *pre, int32_t *post, int32_t *delay_steps, double *w,
                          int64_t cap);
 long long ref_timestep(double t, double dt);

@@ -219,3 +219,22 @@ def test_several_synapses_per_pair(K):
         assert np.array_equal(three[k][0::3], one[k]), k
     assert np.mean(three["w"][1::3] != three["w"][0::3]) > 0.99
     assert np.mean(three["delay_steps"][2::3] != three["delay_steps"][1::3]) > 0.8
+
+
+def test_weight_expression():
+    """S.w = '<expression>' after connect: each synapse's weight is the
+    expression at its pair (a closed form of i, j and a source variable),
+    draws in it are keyed by the pair and the copy (n=2 copies differ), and a
+    uniform 'rand()' weight spreads over [0, 1)."""
+    x = np.linspace(0.0, 1.0, 40)
+    we, _ = cg.connect_expression("0.01*i + j + 2*x_pre", {}, ["x"], [])
+    out = oracle.connect_code((0, 40), (0, 30), "", "", 0.3, 0, False, 11, 0, (0.0, 0.0), DT, (0.0, 0.0), [x], [],
+                              64, 1, we)
+    ref = oracle.connect_random((0, 40), (0, 30), 0.3, False, 11, 0, (0.0, 0.0), DT, (0.0, 0.0))
+    assert np.array_equal(out["pre"], ref["pre"]) and np.array_equal(out["post"], ref["post"])
+    assert np.array_equal(out["w"], 0.01 * out["pre"] + out["post"] + 2 * x[out["pre"]])
+    wr, _ = cg.connect_expression("rand()")
+    two = oracle.connect_code((0, 200), (0, 200), "", "", 0.2, 0, False, 11, 0, (0.0, 0.0), DT, (0.0, 0.0), (), (),
+                              64, 2, wr)
+    w = two["w"]
+    assert np.all((w >= 0) & (w < 1)) and abs(w.mean() - 0.5) < 0.02 and np.all(w[0::2] != w[1::2])

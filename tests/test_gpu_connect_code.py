@@ -243,3 +243,35 @@ def test_several_synapses_per_pair_matches_oracle(hs, oracle_lib, prec, cond, p,
     with pytest.raises(HsError, match="per pair"):
         net.connect_code(0, 1, 1, "", "", 0.1, 0, None, None, True, 31, 2, 0.0, rt.HS_W_SCALAR, 1.0, 17)
     net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("wexpr,cond,p,n", [("0.01*i + j + 2*v_pre - v_post", "", 0.1, 1),
+                                            ("rand()*0.5 + 0.1*randn()", "j % 2 == 0", 0.3, 2),
+                                            ("exp(-abs(i - j)/50.0)", "", "0.4*(rand() < 0.5)", 3)])
+def test_weight_expression_matches_oracle(hs, oracle_lib, prec, wexpr, cond, p, n):
+    """S.w = '<expression>' after connect, evaluated by the device generator per
+    synapse (draws keyed by pair and copy): the bundle table's weights equal the
+    oracle's (the same expression compiled for the host), in the network's
+    precision."""
+    net, pos = _setup(prec, 1500, 1100)
+    c = cg.connect_expression(cond)[0] if cond else ""
+    pe, pnum = (cg.connect_expression(p)[0], 1.0) if isinstance(p, str) else ("", p)
+    we, wconst = cg.connect_expression(wexpr, {}, NAMES, NAMES, first_draw=(c + pe).count("hs_crand_"))
+    assert wconst is None
+    print(f"weight expression {wexpr!r} prec {prec} n={n}", flush=True)
+    pid = net.connect_code(0, 1, 1, c, pe, pnum, 0, None, None, True, 41, 3, (0.1e-3, 2e-3), rt.HS_W_SYNAPSE,
+                           (0.0, 0.0), n, we)
+    z = [0 * pos[0], 0 * pos[0]]
+    ref = _bundle_order(oracle_lib.connect_code((0, 1500), (0, 1100), c, pe, pnum, 0, False, 41, 3, (0.1e-3, 2e-3),
+                                                DT, (0.0, 0.0), [pos[0]] + z, [pos[1], 0 * pos[1], 0 * pos[1]], prec,
+                                                n, we))
+    assert net.pathway_info(pid)["nsyn"] == ref["pre"].size > 5000
+    gp, gq, gd, gw = net.synapses(pid)
+    np.testing.assert_array_equal(gp, ref["pre"])
+    np.testing.assert_array_equal(gd, ref["delay_steps"])
+    rw = ref["w"].astype(np.float32).astype(np.float64) if prec == 32 else ref["w"]
+    np.testing.assert_array_equal(gw, rw)
+    with pytest.raises(HsError, match="per-synapse weights"):
+        net.connect_code(0, 1, 1, "", "", 0.1, 0, None, None, True, 41, 3, 0.0, rt.HS_W_SCALAR, 1.0, 1, we)
+    net.close()
