@@ -238,3 +238,18 @@ def test_weight_expression():
                               64, 2, wr)
     w = two["w"]
     assert np.all((w >= 0) & (w < 1)) and abs(w.mean() - 0.5) < 0.02 and np.all(w[0::2] != w[1::2])
+
+
+def test_random_values_set_synapses_fixed_and_random_seed():
+    """brian2cuda's test_random_values_set_synapses_random_seed / _fixed_seed
+    (test_random_number_generation.py:480-511), with the synapse values set by
+    a weight expression 'rand() + randn()' over connect() on a 10-neuron group
+    and the connect call's seed as seed(): values vary; fresh seeds give
+    different values, equal seeds equal ones."""
+    we, _ = cg.connect_expression("rand() + randn()")
+    fresh = np.random.SeedSequence().generate_state(2, np.uint64)
+    v = [oracle.connect_code((0, 10), (0, 10), "", "", 1.0, 0, False, int(s), 0, (0.0, 0.0), DT, (0.0, 0.0), (), (),
+                             64, 1, we)["w"] for s in (fresh[0], fresh[1], 12345678, 12345678)]
+    assert all(np.var(x) > 0 for x in v) and v[0].size == 100
+    assert np.var(v[0] - v[1]) > 0
+    assert np.array_equal(v[2], v[3])

@@ -275,3 +275,24 @@ def test_weight_expression_matches_oracle(hs, oracle_lib, prec, wexpr, cond, p, 
     with pytest.raises(HsError, match="per-synapse weights"):
         net.connect_code(0, 1, 1, "", "", 0.1, 0, None, None, True, 41, 3, 0.0, rt.HS_W_SCALAR, 1.0, 1, we)
     net.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_random_values_set_synapses_fixed_and_random_seed(hs, prec):
+    """brian2cuda's test_random_values_set_synapses_random_seed / _fixed_seed
+    (tests/test_random_number_generation.py:480-511) on the device: the values
+    'rand() + randn()' set per synapse by the generator over connect() on a
+    10-neuron group, the connect call's seed as seed(): values vary, fresh seeds
+    differ, equal seeds agree."""
+    we, _ = cg.connect_expression("rand() + randn()")
+    fresh = np.random.SeedSequence().generate_state(2, np.uint64)
+    net, _ = _setup(prec, 10, 10)
+    v = []
+    for s in (int(fresh[0]), int(fresh[1]), 12345678, 12345678):
+        pid = net.connect_code(0, 0, 1, "", "", 1.0, 0, None, None, True, s, 0, 0.0, rt.HS_W_SYNAPSE, (0.0, 0.0), 1,
+                               we)
+        v.append(net.synapses(pid)[3])
+    assert all(np.var(x) > 0 for x in v) and v[0].size == 100
+    assert np.var(v[0] - v[1]) > 0
+    assert np.array_equal(v[2], v[3])
+    net.close()
