@@ -84,7 +84,7 @@ hipError_t hs_cg_launch_syn_post(const HsCgKernels *K, const void *paths, int np
    hs_post(k) (variable k of the source / target neuron, as double); p a C
    expression of the pair's probability (empty: the constant of the spec) */
 int hs_cg_conn_check(const std::string &cond, const std::string &p);
-int hs_cg_conn_kernels(int prec, const std::string &cond, const std::string &p, const std::string &w,
+int hs_cg_conn_kernels(int prec, const std::string &cond, const std::string &p, const std::string &w, bool copies,
                        const HsCgConnKernels **out);
 hipError_t hs_cg_launch_conn_count(const HsCgConnKernels *K, const HsConnSpec &c, const HsConnVars &a, int32_t *cnt,
                                    int *err, hipStream_t s);

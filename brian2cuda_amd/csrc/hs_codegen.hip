@@ -471,7 +471,8 @@ int hs_cg_conn_check(const std::string &cond, const std::string &p, const std::s
 
 /* the program of one condition / probability pair: the filter policy HsConnCg
    of hs_conndev.h and the two passes instantiated with it */
-static std::string hs_cg_conn_source(const std::string &cond, const std::string &p, const std::string &w)
+static std::string hs_cg_conn_source(const std::string &cond, const std::string &p, const std::string &w,
+                                     bool copies)
 {
     std::string o = "#include \"hs_conndev.h\"\n#include \"hs_cgconn.h\"\n";
     o += hs_cg_math;
@@ -480,7 +481,7 @@ static std::string hs_cg_conn_source(const std::string &cond, const std::string 
     o += "struct HsConnCg {\n";
     o += "    static constexpr bool COND = " + std::string(cond.empty() ? "false" : "true") +
          ", PEXPR = " + std::string(p.empty() ? "false" : "true") + ", WEXPR = " +
-         std::string(w.empty() ? "false" : "true") + ";\n";
+         std::string(w.empty() ? "false" : "true") + ", COPIES = " + std::string(copies ? "true" : "false") + ";\n";
     /* the functions' common head (after their signatures h0 / h1) */
     const char *head =
         "        const long long i = hs_ai - hs_c.pre_lo, j = hs_aj - hs_c.post_lo;\n"
@@ -527,11 +528,11 @@ struct HsCgConnEntry {
 };
 static std::map<std::string, HsCgConnEntry> g_cg_conn_cache; /* precision + program text -> kernels */
 
-int hs_cg_conn_kernels(int prec, const std::string &cond, const std::string &p, const std::string &w,
+int hs_cg_conn_kernels(int prec, const std::string &cond, const std::string &p, const std::string &w, bool copies,
                        const HsCgConnKernels **out)
 {
     HS_CG_TRY(hs_cg_conn_check(cond, p, w));
-    const std::string src = hs_cg_conn_source(cond, p, w);
+    const std::string src = hs_cg_conn_source(cond, p, w, copies);
     const std::string key = std::to_string(prec) + "\n" + src;
     std::lock_guard<std::mutex> lock(g_cg_mu);
     auto it = g_cg_conn_cache.find(key);

@@ -22,7 +22,8 @@
 
 /* no condition, constant probability */
 struct HsConnPlain {
-    static constexpr bool COND = false, PEXPR = false, WEXPR = false;
+    /* COPIES: the write pass of connect(n > 1) (only generated programs carry it) */
+    static constexpr bool COND = false, PEXPR = false, WEXPR = false, COPIES = false;
     template <typename R>
     static __device__ __forceinline__ bool cond(const HsConnSpec &, const HsConnVars &, long long, long long)
     {
@@ -354,7 +355,7 @@ __device__ __forceinline__ void hs_conn_write_body(const HsConnSpec &c, const Hs
                 hs_conn_batch<F, R>(c, a, i, jb, k0, done, sel, j);
             }
             if constexpr (F::COND) sel = sel && F::template cond<R>(c, a, i, j);
-            if (c.ncopy > 1) {
+            if constexpr (F::COPIES) {
                 hs_conn_write_copies<F, R>(c, a, i, j, sel, draw, row, bstart, post, w, err, q);
                 continue;
             }

@@ -943,8 +943,8 @@ static int hs_connect_device(hs_net *net, int src, int tgt, int var, int pre_lo,
         HS_TRY(hs_conn_expr_vars(*x, "hs_pre", gs.mi.nvars));
         HS_TRY(hs_conn_expr_vars(*x, "hs_post", gt.mi.nvars));
     }
-    if (!cond.empty() || !pexpr.empty() || !wexpr.empty())
-        HS_TRY(hs_cg_conn_kernels(net->prec, cond, pexpr, wexpr, &K));
+    if (!cond.empty() || !pexpr.empty() || !wexpr.empty() || ncopy > 1)
+        HS_TRY(hs_cg_conn_kernels(net->prec, cond, pexpr, wexpr, ncopy > 1, &K));
     if (!(delay_lo >= 0.0) || !(delay_hi >= delay_lo) || delay_hi > 1e300)
         return hs_fail(HS_ERR_ARG, "bad delay range [%g, %g]", delay_lo, delay_hi);
     if (wmode != HS_W_SCALAR && wmode != HS_W_SYNAPSE)
